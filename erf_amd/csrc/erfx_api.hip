@@ -1,0 +1,508 @@
+// erfx_api.hip -- C ABI (include/erfx.h): configuration, hook contexts and the
+// whole-step driver (ERF::Advance -> advance_dycore -> MRISplitIntegrator::advance).
+#include "erfx_common.h"
+#include "erfx_internal.h"
+
+#include <cstring>
+#include <vector>
+
+namespace erfx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return ERFX_ERR_HIP;
+}
+
+// ERF::init_bcs (Source/Initialization/ERF_init_bcs.cpp:300-500): physical face
+// types -> logical kinds per variable; plus derived constants.
+void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
+    P.cfg = c;
+    P.L = Lay{lay.nx, lay.ny, lay.nz, lay.ng, lay.sy, lay.sz, lay.nelem};
+    P.ng_cons = lay.ng;
+    P.ng_vel = lay.ng - 1;
+    P.rdOcp = R_d / c.c_p;
+    P.gravity = c.use_gravity ? CONST_GRAV : 0.0;
+    P.dxInv[0] = 1.0 / c.dx; P.dxInv[1] = 1.0 / c.dy; P.dxInv[2] = 1.0 / c.dz;
+    for (int d = 0; d < 3; ++d) P.periodic[d] = (c.bc[d] == ERFX_BC_PERIODIC);
+    for (int ori = 0; ori < 6; ++ori) {
+        int dir = ori % 3, t = c.bc[ori];
+        for (int v = 0; v < 3; ++v) {
+            P.vel_extdir[v][ori] = 0.0;
+            int k = INT_DIR;
+            switch (t) {
+                case ERFX_BC_PERIODIC: k = INT_DIR; break;
+                case ERFX_BC_SYMMETRY: k = (v == dir) ? REFLECT_ODD : REFLECT_EVEN; break;
+                case ERFX_BC_OUTFLOW: k = (v == dir) ? NEUMANN_INT : FOEXTRAP; break;
+                case ERFX_BC_NO_SLIP_WALL: k = EXT_DIR; break;
+                case ERFX_BC_SLIP_WALL: k = (v == dir) ? EXT_DIR : FOEXTRAP; break;
+            }
+            P.bck[v][ori] = k;
+        }
+        P.bck[3][ori] = (t == ERFX_BC_PERIODIC) ? INT_DIR : (t == ERFX_BC_SYMMETRY ? REFLECT_EVEN : FOEXTRAP);
+    }
+    P.consA = c.molec_diff_type == ERFX_MOLEC_CONSTANT_ALPHA;
+    P.use_turb = c.les_type == ERFX_LES_SMAGORINSKY;
+    P.use_diff = c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE;
+}
+
+static int validate(const erfx_config& c) {
+    if (c.nx < 4 || c.ny < 4 || c.nz < 4) { set_error("box must be at least 4 cells in each direction"); return ERFX_ERR_ARG; }
+    if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) { set_error("cell sizes must be positive"); return ERFX_ERR_ARG; }
+    if (c.ncons != ERFX_NCONS_DRY) { set_error("only the dry state (ncons = 5) is supported"); return ERFX_ERR_UNSUPPORTED; }
+    for (int o = 0; o < 6; ++o) {
+        if (c.bc[o] < ERFX_BC_PERIODIC || c.bc[o] > ERFX_BC_OUTFLOW) { set_error("bad bc type"); return ERFX_ERR_ARG; }
+        if ((c.bc[o] == ERFX_BC_PERIODIC) != (c.bc[o % 3 == o ? o + 3 : o - 3] == ERFX_BC_PERIODIC)) {
+            set_error("periodic faces must come in pairs"); return ERFX_ERR_ARG;
+        }
+    }
+    if (c.bc[2] == ERFX_BC_PERIODIC) { set_error("z cannot be periodic"); return ERFX_ERR_UNSUPPORTED; }
+    if (c.bc[2] == ERFX_BC_SYMMETRY || c.bc[2] == ERFX_BC_OUTFLOW || c.bc[5] == ERFX_BC_SYMMETRY) {
+        set_error("vertical faces must be slip or no-slip walls (or an outflow top)"); return ERFX_ERR_UNSUPPORTED;
+    }
+    int advs[4] = {c.dycore_horiz_adv, c.dycore_vert_adv, c.dryscal_horiz_adv, c.dryscal_vert_adv};
+    for (int a : advs)
+        if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_CENTERED6) { set_error("unsupported advection scheme"); return ERFX_ERR_UNSUPPORTED; }
+    if (c.molec_diff_type < 0 || c.molec_diff_type > 2 || c.les_type < 0 || c.les_type > 1) {
+        set_error("bad diffusion choice"); return ERFX_ERR_ARG;
+    }
+    if (c.les_type == ERFX_LES_SMAGORINSKY && !(c.Cs > 0)) { set_error("Smagorinsky needs Cs > 0"); return ERFX_ERR_ARG; }
+    if (!(c.mri_ratio > 1 && c.mri_ratio % 2 == 0)) { set_error("mri_ratio must be even and > 1 (ERF_MRI.H:205-207)"); return ERFX_ERR_ARG; }
+    if (!c.force_stage1_single_substep && c.mri_ratio % 3 != 0) {
+        set_error("without the single first-stage substep the ratio must be a multiple of 6"); return ERFX_ERR_ARG;
+    }
+    return ERFX_OK;
+}
+
+}  // namespace erfx
+
+using namespace erfx;
+
+// ---- hook context ------------------------------------------------------------
+struct erfx_ctx {
+    Params P;
+    hipStream_t stream = nullptr;
+    const double* r0 = nullptr;
+    const double* p0 = nullptr;
+    const double* pi0 = nullptr;
+    FastWork fw{};
+    SlowWork sw{};
+    std::vector<void*> allocs;
+};
+
+// ---- whole-step driver ---------------------------------------------------------
+struct erfx_sim {
+    erfx_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* cons[2] = {nullptr, nullptr};     // vars_old / vars_new, swapped each step
+    double* vel[2][3] = {{nullptr}};
+    int cur = 0;                              // index of the current (new) state
+    double *r0 = nullptr, *p0 = nullptr, *pi0 = nullptr;
+    erfx_state S[5];                          // S_old, S_new, S_sum, S_scratch, F_slow
+    double *prim = nullptr, *pi_stage = nullptr, *fast_coeffs = nullptr, *eddy = nullptr;
+    std::vector<void*> allocs;
+};
+
+extern "C" {
+
+const char* erfx_last_error(void) { return g_last_error.c_str(); }
+
+void erfx_config_defaults(erfx_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = c->nz = 16;
+    c->dx = c->dy = c->dz = 100.0;
+    for (int o = 0; o < 6; ++o) c->bc[o] = ERFX_BC_PERIODIC;
+    c->bc[2] = c->bc[5] = ERFX_BC_SLIP_WALL;
+    c->ncons = ERFX_NCONS_DRY;
+    c->dycore_horiz_adv = c->dycore_vert_adv = ERFX_ADV_UPWIND3;
+    c->dryscal_horiz_adv = c->dryscal_vert_adv = ERFX_ADV_UPWIND3;
+    c->dycore_horiz_upw_frac = c->dycore_vert_upw_frac = 1.0;
+    c->dryscal_horiz_upw_frac = c->dryscal_vert_upw_frac = 1.0;
+    c->molec_diff_type = ERFX_MOLEC_NONE;
+    c->rho0_trans = 1.0;
+    c->les_type = ERFX_LES_NONE;
+    c->Pr_t = 1.0 / 3.0;
+    c->Sc_t = 1.0;
+    c->use_gravity = 1;
+    c->c_p = Cp_d;
+    c->coriolis_factor = 2.0 * 2.0 * PI / 86400.0;
+    c->sinphi = 1.0;
+    c->mri_ratio = 6;
+    c->force_stage1_single_substep = 1;
+    c->implicit_substepping = 1;
+}
+
+int erfx_ghost_cells(const erfx_config* c) {
+    if (!c) return -1;
+    auto wide = [](int t) { return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6; };
+    int g = (wide(c->dycore_horiz_adv) || wide(c->dycore_vert_adv) || wide(c->dryscal_horiz_adv) ||
+             wide(c->dryscal_vert_adv)) ? 3 : 2;
+    return g + 1;
+}
+
+int erfx_layout_for(const erfx_config* c, erfx_layout* out) {
+    if (!c || !out) { set_error("null argument"); return ERFX_ERR_ARG; }
+    int ng = erfx_ghost_cells(c);
+    out->nx = c->nx; out->ny = c->ny; out->nz = c->nz; out->ng = ng;
+    long row = c->nx + 1 + 2 * ng;
+    out->sy = (row + 15) / 16 * 16;
+    out->sz = out->sy * (c->ny + 1 + 2 * ng);
+    out->nelem = out->sz * (c->nz + 1 + 2 * ng);
+    return ERFX_OK;
+}
+
+static int dalloc(std::vector<void*>& list, double** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    list.push_back(q);
+    // zero once so never-written ghost layers hold defined values
+    e = hipMemset(q, 0, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMemset");
+    *p = static_cast<double*>(q);
+    return ERFX_OK;
+}
+
+int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
+    if (!cfg || !out) { set_error("null argument"); return ERFX_ERR_ARG; }
+    int rc = validate(*cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return ERFX_ERR_NODEVICE; }
+    auto* c = new erfx_ctx();
+    erfx_layout lay;
+    erfx_layout_for(cfg, &lay);
+    make_params(*cfg, lay, c->P);
+    c->stream = static_cast<hipStream_t>(stream);
+    const size_t ne = (size_t)lay.nelem;
+    double** bufs[] = {&c->fw.extrap, &c->fw.drw, &c->fw.soln, &c->sw.pp,
+                       &c->sw.tau[0], &c->sw.tau[1], &c->sw.tau[2], &c->sw.tau[3], &c->sw.tau[4], &c->sw.tau[5]};
+    for (double** b : bufs)
+        if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
+    if ((rc = dalloc(c->allocs, &c->fw.trhs, 2 * ne))) { erfx_ctx_destroy(c); return rc; }
+    c->sw.xsrc = c->sw.ysrc = c->sw.zsrc = nullptr;
+    *out = c;
+    return ERFX_OK;
+}
+
+int erfx_ctx_destroy(erfx_ctx* c) {
+    if (!c) return ERFX_OK;
+    for (void* p : c->allocs) (void)hipFree(p);
+    delete c;
+    return ERFX_OK;
+}
+
+int erfx_ctx_layout(const erfx_ctx* c, erfx_layout* out) {
+    if (!c || !out) { set_error("null argument"); return ERFX_ERR_ARG; }
+    out->nx = c->P.L.nx; out->ny = c->P.L.ny; out->nz = c->P.L.nz; out->ng = c->P.L.ng;
+    out->sy = c->P.L.sy; out->sz = c->P.L.sz; out->nelem = c->P.L.ne;
+    return ERFX_OK;
+}
+
+int erfx_set_base_state(erfx_ctx* c, const double* r0, const double* p0, const double* pi0) {
+    if (!c || !r0 || !p0 || !pi0) { set_error("null argument"); return ERFX_ERR_ARG; }
+    c->r0 = r0; c->p0 = p0; c->pi0 = pi0;
+    return ERFX_OK;
+}
+
+static int launch_status() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return ERFX_OK;
+}
+
+static bool bad_state(const erfx_state* s) { return !s || !s->cons || !s->xmom || !s->ymom || !s->zmom; }
+
+int erfx_cons_to_prim(erfx_ctx* c, const double* cons, double* prim, double* pi_stage, int ng) {
+    if (!c || !cons || !prim || !pi_stage) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (ng < 1 || ng > c->P.ng_cons) { set_error("ng out of range"); return ERFX_ERR_ARG; }
+    cons_to_prim(c->P, cons, prim, pi_stage, ng, c->stream);
+    return launch_status();
+}
+
+int erfx_apply_bcs(erfx_ctx* c, erfx_state* S, double* xv, double* yv, double* zv, int ng_cons, int ng_vel,
+                   int fast_only, int synced) {
+    if (!c || bad_state(S) || !xv || !yv || !zv) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (ng_cons < 1 || ng_cons > c->P.ng_cons || ng_vel < 0 || ng_vel > c->P.ng_vel) {
+        set_error("ghost widths exceed the layout"); return ERFX_ERR_ARG;
+    }
+    apply_bcs(c->P, *S, xv, yv, zv, ng_cons, ng_vel, fast_only != 0, synced != 0, c->stream);
+    return launch_status();
+}
+
+int erfx_make_fast_coeffs(erfx_ctx* c, double* fc, const erfx_state* stage, const double* prim,
+                          const double* pi_stage, double dtau, double beta_s) {
+    if (!c || !fc || bad_state(stage) || !prim || !pi_stage) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!c->r0) { set_error("base state not set"); return ERFX_ERR_STATE; }
+    make_fast_coeffs(c->P, fc, *stage, prim, pi_stage, c->r0, c->pi0, dtau, beta_s, c->stream);
+    return launch_status();
+}
+
+int erfx_fast_rhs_N(erfx_ctx* c, int step, int nrk, const erfx_state* slow_rhs, const erfx_state* prev,
+                    const erfx_state* stage, const double* prim, const double* pi_stage, const double* fc,
+                    erfx_state* data, erfx_state* scratch, double dtau, double beta_s, double facinv) {
+    if (!c || bad_state(slow_rhs) || bad_state(prev) || bad_state(stage) || bad_state(data) || bad_state(scratch) ||
+        !prim || !pi_stage || !fc) {
+        set_error("null argument"); return ERFX_ERR_ARG;
+    }
+    if (!(nrk > 0 || step == 0)) { set_error("the first RK stage takes a single substep (ERF_fast_rhs_N.cpp:106)"); return ERFX_ERR_ARG; }
+    fast_rhs_N(c->P, c->fw, step, nrk, *slow_rhs, *prev, *stage, prim, pi_stage, fc, *data, *scratch, dtau, beta_s,
+               facinv, c->stream);
+    return launch_status();
+}
+
+int erfx_turbulent_viscosity(erfx_ctx* c, const double* cons, const double* xv, const double* yv,
+                             const double* zv, double* eddy, double* hfx3) {
+    (void)hfx3;
+    if (!c || !cons || !xv || !yv || !zv || !eddy) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!c->P.use_turb) { set_error("no LES model configured"); return ERFX_ERR_ARG; }
+    turbulent_viscosity(c->P, c->sw, cons, xv, yv, zv, eddy, c->stream);
+    return launch_status();
+}
+
+int erfx_slow_rhs_pre(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, const erfx_state* data,
+                      const double* prim, erfx_state* scratch, const double* xv, const double* yv,
+                      const double* zv, const double* eddy, double* hfx3) {
+    (void)hfx3;
+    if (!c || bad_state(rhs) || bad_state(data) || bad_state(scratch) || !prim || !xv || !yv || !zv) {
+        set_error("null argument"); return ERFX_ERR_ARG;
+    }
+    if (c->P.use_turb && !eddy) { set_error("LES model needs the eddy viscosity"); return ERFX_ERR_ARG; }
+    if (!c->r0) { set_error("base state not set"); return ERFX_ERR_STATE; }
+    slow_rhs_pre(c->P, c->sw, nrk, slow_dt, *rhs, *data, prim, *scratch, xv, yv, zv, eddy, c->r0, c->p0, c->stream);
+    return launch_status();
+}
+
+int erfx_slow_rhs_post(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, const erfx_state* old,
+                       erfx_state* snew, erfx_state* data, const double* prim, const erfx_state* scratch,
+                       const double* xv, const double* yv, const double* eddy) {
+    (void)xv; (void)yv;
+    if (!c || bad_state(rhs) || bad_state(old) || bad_state(snew) || bad_state(data) || bad_state(scratch) || !prim) {
+        set_error("null argument"); return ERFX_ERR_ARG;
+    }
+    if (c->P.use_turb && !eddy) { set_error("LES model needs the eddy viscosity"); return ERFX_ERR_ARG; }
+    slow_rhs_post(c->P, nrk, slow_dt, *rhs, *old, *snew, *data, prim, *scratch, eddy, c->stream);
+    return launch_status();
+}
+
+// ---- sim --------------------------------------------------------------------
+int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
+    if (!cfg || !out) { set_error("null argument"); return ERFX_ERR_ARG; }
+    auto* s = new erfx_sim();
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete s; return hip_fail(e, "hipStreamCreate"); }
+    s->own_stream = true;
+    int rc = erfx_ctx_create(cfg, s->stream, &s->ctx);
+    if (rc) { erfx_sim_destroy(s); return rc; }
+    const Params& P = s->ctx->P;
+    const size_t ne = (size_t)P.L.ne;
+    const int nc = cfg->ncons;
+    auto A = [&](double** p, size_t n) { return dalloc(s->allocs, p, n); };
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = A(&s->cons[b], nc * ne);
+        for (int d = 0; d < 3 && !rc; ++d) rc = A(&s->vel[b][d], ne);
+    }
+    if (!rc) rc = A(&s->r0, ne);
+    if (!rc) rc = A(&s->p0, ne);
+    if (!rc) rc = A(&s->pi0, ne);
+    for (int st = 0; st < 5 && !rc; ++st) {
+        if (st != 0 && st != 1) rc = A(&s->S[st].cons, nc * ne);
+        if (!rc) rc = A(&s->S[st].xmom, ne);
+        if (!rc) rc = A(&s->S[st].ymom, ne);
+        if (!rc) rc = A(&s->S[st].zmom, ne);
+    }
+    if (!rc) rc = A(&s->prim, (nc - 1) * ne);
+    if (!rc) rc = A(&s->pi_stage, ne);
+    if (!rc) rc = A(&s->fast_coeffs, 5 * ne);
+    if (!rc && P.use_turb) rc = A(&s->eddy, 13 * ne);
+    if (rc) { erfx_sim_destroy(s); return rc; }
+    // S_old and S_new cons alias vars_old / vars_new (ERF_Advance.cpp:127-160)
+    s->S[0].cons = s->cons[1 - s->cur];
+    s->S[1].cons = s->cons[s->cur];
+    erfx_set_base_state(s->ctx, s->r0, s->p0, s->pi0);
+    *out = s;
+    return ERFX_OK;
+}
+
+int erfx_sim_destroy(erfx_sim* s) {
+    if (!s) return ERFX_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->ctx) erfx_ctx_destroy(s->ctx);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return ERFX_OK;
+}
+
+int erfx_sim_layout(const erfx_sim* s, erfx_layout* out) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    return erfx_ctx_layout(s->ctx, out);
+}
+
+erfx_ctx* erfx_sim_ctx(erfx_sim* s) { return s ? s->ctx : nullptr; }
+void* erfx_sim_stream(erfx_sim* s) { return s ? (void*)s->stream : nullptr; }
+
+static double* field_ptr(erfx_sim* s, int id, int* ncomp) {
+    const int nc = s->ctx->P.cfg.ncons;
+    int n = 1;
+    double* p = nullptr;
+    switch (id) {
+        case ERFX_F_CONS: p = s->cons[s->cur]; n = nc; break;
+        case ERFX_F_XVEL: p = s->vel[s->cur][0]; break;
+        case ERFX_F_YVEL: p = s->vel[s->cur][1]; break;
+        case ERFX_F_ZVEL: p = s->vel[s->cur][2]; break;
+        case ERFX_F_R0: p = s->r0; break;
+        case ERFX_F_P0: p = s->p0; break;
+        case ERFX_F_PI0: p = s->pi0; break;
+        case ERFX_D_PRIM: p = s->prim; n = nc - 1; break;
+        case ERFX_D_PI_STAGE: p = s->pi_stage; break;
+        case ERFX_D_FAST_COEFFS: p = s->fast_coeffs; n = 5; break;
+        case ERFX_D_EDDY: p = s->eddy; n = 13; break;
+        case ERFX_D_XVEL_OLD: p = s->vel[1 - s->cur][0]; break;
+        case ERFX_D_YVEL_OLD: p = s->vel[1 - s->cur][1]; break;
+        case ERFX_D_ZVEL_OLD: p = s->vel[1 - s->cur][2]; break;
+        default:
+            if (id >= ERFX_D_STATE_BASE && id < ERFX_D_STATE_BASE + 20) {
+                int st = (id - ERFX_D_STATE_BASE) / 4, c = (id - ERFX_D_STATE_BASE) % 4;
+                erfx_state& S = s->S[st];
+                double* f[4] = {S.cons, S.xmom, S.ymom, S.zmom};
+                p = f[c];
+                n = c == 0 ? nc : 1;
+            }
+    }
+    if (ncomp) *ncomp = n;
+    return p;
+}
+
+double* erfx_sim_field_ptr(erfx_sim* s, int id) { return s ? field_ptr(s, id, nullptr) : nullptr; }
+
+long erfx_sim_field_size(const erfx_sim* s, int id) {
+    if (!s) return -1;
+    int n = 0;
+    double* p = field_ptr(const_cast<erfx_sim*>(s), id, &n);
+    return p ? (long)n * s->ctx->P.L.ne : -1;
+}
+
+int erfx_sim_upload(erfx_sim* s, int id, const double* host) {
+    if (!s || !host) { set_error("null argument"); return ERFX_ERR_ARG; }
+    int n = 0;
+    double* p = field_ptr(s, id, &n);
+    if (!p) { set_error("unknown field id"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipMemcpyAsync(p, host, (size_t)n * s->ctx->P.L.ne * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    return ERFX_OK;
+}
+
+int erfx_sim_download(erfx_sim* s, int id, double* host) {
+    if (!s || !host) { set_error("null argument"); return ERFX_ERR_ARG; }
+    int n = 0;
+    double* p = field_ptr(s, id, &n);
+    if (!p) { set_error("unknown field id"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipMemcpyAsync(host, p, (size_t)n * s->ctx->P.L.ne * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    return ERFX_OK;
+}
+
+// One step: ERF::Advance (ERF_Advance.cpp:20-233), advance_dycore
+// (ERF_advance_dycore.cpp:37-305) and MRISplitIntegrator::advance (ERF_MRI.H:187-334).
+static int sim_one_step(erfx_sim* s, double time, double dt) {
+    erfx_ctx* c = s->ctx;
+    const Params& P = c->P;
+    hipStream_t st = s->stream;
+    const size_t ne = (size_t)P.L.ne;
+    const int nc = P.cfg.ncons;
+    // swap vars_old / vars_new (ERF_Advance.cpp:25)
+    const int old = s->cur, nw = 1 - s->cur;
+    double* cons_old = s->cons[old];
+    double* uo = s->vel[old][0]; double* vo = s->vel[old][1]; double* wo = s->vel[old][2];
+    double* un = s->vel[nw][0]; double* vn = s->vel[nw][1]; double* wn = s->vel[nw][2];
+    erfx_state& So = s->S[0];
+    erfx_state& Sn = s->S[1];
+    erfx_state& Ss = s->S[2];
+    erfx_state& Sc = s->S[3];
+    erfx_state& Fs = s->S[4];
+    So.cons = cons_old;
+    Sn.cons = s->cons[nw];
+    // FillPatch of the old state and V2M on the full velocity ghost width (:98-108)
+    fill_patch(P, cons_old, uo, vo, wo, st);
+    velocity_to_momentum(P, uo, vo, wo, P.ng_vel, P.ng_vel, cons_old, So.xmom, So.ymom, So.zmom, st);
+    // advance_dycore
+    if (P.use_turb) turbulent_viscosity(P, c->sw, cons_old, uo, vo, wo, s->eddy, st);
+    ERFX_HIP(hipMemcpyAsync(un, uo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ERFX_HIP(hipMemcpyAsync(vn, vo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ERFX_HIP(hipMemcpyAsync(wn, wo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    apply_bcs(P, So, un, vn, wn, P.ng_cons, P.ng_vel, false, true, st);
+    cons_to_prim(P, So.cons, s->prim, s->pi_stage, P.ng_cons, st);
+    // MRI: S_new <- S_old (ERF_MRI.H:225-230)
+    ERFX_HIP(hipMemcpyAsync(Sn.cons, So.cons, nc * ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ERFX_HIP(hipMemcpyAsync(Sn.xmom, So.xmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ERFX_HIP(hipMemcpyAsync(Sn.ymom, So.ymom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ERFX_HIP(hipMemcpyAsync(Sn.zmom, So.zmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const int ratio = P.cfg.mri_ratio;
+    const double sub_timestep = dt / ratio;
+    const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
+    double time_stage = time;
+    for (int nrk = 0; nrk < 3; ++nrk) {
+        int nsub = 1;
+        double dtau = 0;
+        if (nrk == 0) {
+            if (P.cfg.force_stage1_single_substep) { nsub = 1; dtau = dt / 3.0; }
+            else { nsub = ratio / 3; dtau = sub_timestep; }
+            time_stage = time + dt / 3.0;
+        } else if (nrk == 1) {
+            nsub = ratio / 2; dtau = sub_timestep; time_stage = time + dt / 2.0;
+        } else {
+            nsub = ratio; dtau = sub_timestep; time_stage = time + dt;
+        }
+        if (nrk > 0) cons_to_prim(P, Sn.cons, s->prim, s->pi_stage, P.ng_cons, st);
+        const double slow_dt = time_stage - time;
+        slow_rhs_pre(P, c->sw, nrk, slow_dt, Fs, Sn, s->prim, Sc, un, vn, wn, s->eddy, s->r0, s->p0, st);
+        const double inv_fac = 1.0 / static_cast<double>(nsub);
+        for (int ks = 0; ks < nsub; ++ks) {
+            if (ks == 0) {
+                make_fast_coeffs(P, s->fast_coeffs, Sn, s->prim, s->pi_stage, s->r0, s->pi0, dtau, beta_s, st);
+                fast_rhs_N(P, c->fw, ks, nrk, Fs, So, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc, dtau, beta_s,
+                           inv_fac, st);
+            } else {
+                fast_rhs_N(P, c->fw, ks, nrk, Fs, Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc, dtau, beta_s,
+                           inv_fac, st);
+            }
+            apply_bcs(P, Ss, un, vn, wn, 1, 1, true, false, st);
+        }
+        slow_rhs_post(P, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);
+        apply_bcs(P, Sn, un, vn, wn, P.ng_cons, P.ng_vel, false, false, st);
+    }
+    s->cur = nw;
+    s->S[0].cons = s->cons[1 - s->cur];
+    s->S[1].cons = s->cons[s->cur];
+    return launch_status();
+}
+
+int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = sim_one_step(s, t0 + n * dt, dt);
+        if (rc) return rc;
+    }
+    return ERFX_OK;
+}
+
+int erfx_sim_sync(erfx_sim* s) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    return ERFX_OK;
+}
+
+int erfx_sim_step(erfx_sim* s, double t0, double dt, int nsteps) {
+    int rc = erfx_sim_step_async(s, t0, dt, nsteps);
+    if (rc) return rc;
+    return erfx_sim_sync(s);
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// erfx_common.h -- shared device/host definitions of the HIP dycore library.
+//
+// All fields of a box share one layout (include/erfx.h): i fastest, ng ghost
+// cells on every side, one extra face in each direction.  Kernels address every
+// field with the same linear index, which lets a fused kernel stream several
+// fields with one address computation and keeps every access x-coalesced.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "../../include/erfx.h"
+
+namespace erfx {
+
+// ---- physical constants (Source/ERF_Constants.H:7-20) ---------------------
+constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double R_d = 287.0;
+constexpr double R_v = 461.505;
+constexpr double Cp_d = 1004.5;
+constexpr double p_0 = 1.0e5;
+constexpr double Gamma = 1.4;
+constexpr double CONST_GRAV = 9.81;
+constexpr double ip_0 = 1. / p_0;
+
+// ---- EOS (Source/Utils/ERF_EOS.H:89-93, 130-135), dry ---------------------
+__device__ __forceinline__ double eos_p_of_rt(double rt) {
+    return p_0 * pow(R_d * rt * (1.0 + (R_v / R_d) * 0.0) * ip_0, Gamma);
+}
+__device__ __forceinline__ double eos_exner_of_rt(double rt, double rdOcp) {
+    return pow(R_d * rt * (1.0 + R_v / R_d * 0.0) * ip_0, Gamma * rdOcp);
+}
+
+// ---- index space ------------------------------------------------------------
+struct Lay {
+    int nx, ny, nz, ng;
+    long sy, sz, ne;
+    __host__ __device__ __forceinline__ long ix(int i, int j, int k) const {
+        return (long)(i + ng) + (long)(j + ng) * sy + (long)(k + ng) * sz;
+    }
+};
+
+struct Box {
+    int lo[3], hi[3];
+    __host__ __device__ Box() : lo{0, 0, 0}, hi{-1, -1, -1} {}
+    __host__ __device__ Box(int l0, int l1, int l2, int h0, int h1, int h2) : lo{l0, l1, l2}, hi{h0, h1, h2} {}
+    __host__ __device__ bool ok() const { return lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]; }
+    __host__ __device__ long npts() const {
+        return ok() ? (long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) : 0;
+    }
+};
+
+// Generic box launch: x-lanes along i (64-wide waves), 4 rows of j per block,
+// one z-slab per grid.z.  Every lane checks its point against the box, so any
+// grid covering the box is safe.
+template <class F>
+__global__ void __launch_bounds__(256) box_kernel(Box b, F f) {
+    int i = b.lo[0] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int j = b.lo[1] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    int k = b.lo[2] + (int)blockIdx.z;
+    if (i > b.hi[0] || j > b.hi[1] || k > b.hi[2]) return;
+    f(i, j, k);
+}
+
+template <class F>
+inline void launch_box(const Box& b, hipStream_t s, F f) {
+    if (!b.ok()) return;
+    dim3 blk(64, 4, 1);
+    dim3 grd((unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64), (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4),
+             (unsigned)(b.hi[2] - b.lo[2] + 1));
+    hipLaunchKernelGGL(box_kernel<F>, grd, blk, 0, s, b, f);
+}
+
+// Column launch over a 2-D (i, j) range; the functor loops over k itself.
+template <class F>
+__global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int jhi, F f) {
+    int i = ilo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int j = jlo + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (i > ihi || j > jhi) return;
+    f(i, j);
+}
+
+template <class F>
+inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
+    if (ilo > ihi || jlo > jhi) return;
+    dim3 blk(64, 4, 1);
+    dim3 grd((unsigned)((ihi - ilo + 1 + 63) / 64), (unsigned)((jhi - jlo + 1 + 3) / 4), 1);
+    hipLaunchKernelGGL(col_kernel<F>, grd, blk, 0, s, ilo, ihi, jlo, jhi, f);
+}
+
+// ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
+enum BCK { INT_DIR = 0, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_ODD, NEUMANN_INT };
+
+// Per-run constants derived from erfx_config.
+struct Params {
+    erfx_config cfg;
+    Lay L;
+    int ng_cons, ng_vel;
+    double rdOcp, gravity;
+    double dxInv[3];
+    int bck[4][6];            // [x/y/z velocity, cons][ori]
+    double vel_extdir[3][6];
+    bool periodic[3];
+    bool use_diff, use_turb, consA;
+};
+
+void make_params(const erfx_config& cfg, const erfx_layout& lay, Params& P);
+
+// error state
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+}  // namespace erfx
+
+#define ERFX_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return erfx::hip_fail(e_, #call);                \
+    } while (0)

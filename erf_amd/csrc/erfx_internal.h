@@ -1,0 +1,56 @@
+// erfx_internal.h -- launchers shared between the library's translation units.
+#pragma once
+
+#include "erfx_common.h"
+
+namespace erfx {
+
+// erfx_bc.hip
+void fill_boundary_periodic(const Params& P, double* a, int ncomp, int ng, int nodal_dir, hipStream_t s);
+void fill_cons(const Params& P, double* c, int icomp, int ncomp, int ng, hipStream_t s);
+void fill_xvel(const Params& P, double* a, int ng, hipStream_t s);
+void fill_yvel(const Params& P, double* a, int ng, hipStream_t s);
+void fill_zvel(const Params& P, double* a, int ng, hipStream_t s);
+void momentum_to_velocity(const Params& P, double* xv, double* yv, double* zv, const double* dens,
+                          const double* xm, const double* ym, const double* zm, hipStream_t s);
+void velocity_to_momentum(const Params& P, const double* xv, const double* yv, const double* zv, int gx,
+                          int gz_w, const double* dens, double* xm, double* ym, double* zm, hipStream_t s);
+void fill_intermediate_patch(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv,
+                             int ngc, int ngv, bool cons_only, int icomp, int ncomp, hipStream_t s);
+void apply_bcs(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv, int ngc, int ngv,
+               bool fast_only, bool synced, hipStream_t s);
+void fill_patch(const Params& P, double* c, double* u, double* v, double* w, hipStream_t s);
+void cons_to_prim(const Params& P, const double* c, double* prim, double* pi_stage, int ng, hipStream_t s);
+
+// erfx_fast.hip
+struct FastWork {
+    double* extrap;   // theta extrapolated forward, cells grown by 1
+    double* drw;      // delta rho*w at the previous substep, z-faces grown by 1 laterally
+    double* trhs;     // 2 comps: flux divergence of rho and rho*theta increments, cells
+    double* soln;     // tridiagonal solution on z-faces
+};
+void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, const double* prim,
+                      const double* pi_stage, const double* r0, const double* pi0, double dtau, double beta_s,
+                      hipStream_t s);
+void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erfx_state& slow_rhs,
+                const erfx_state& prev, const erfx_state& stage, const double* prim, const double* pi_stage,
+                const double* fc, const erfx_state& data, const erfx_state& scratch, double dtau,
+                double beta_s, double facinv, hipStream_t s);
+
+// erfx_slow.hip
+struct SlowWork {
+    double* pp;                   // perturbational pressure, cells grown by 1
+    double* tau[6];               // 11, 22, 33, 12, 13, 23
+    double* xsrc; double* ysrc; double* zsrc;
+};
+void turbulent_viscosity(const Params& P, const SlowWork& W, const double* cons, const double* u,
+                         const double* v, const double* w, double* eddy, hipStream_t s);
+void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, const erfx_state& rhs,
+                  const erfx_state& data, const double* prim, const erfx_state& scratch, const double* u,
+                  const double* v, const double* w, const double* eddy, const double* r0, const double* p0,
+                  hipStream_t s);
+void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& rhs, const erfx_state& old,
+                   const erfx_state& snew, const erfx_state& data, const double* prim,
+                   const erfx_state& scratch, const double* eddy, hipStream_t s);
+
+}  // namespace erfx

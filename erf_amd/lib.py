@@ -1,0 +1,146 @@
+"""ctypes binding of the HIP dycore library (erf_amd/liberfx.so, include/erfx.h).
+
+The library is the product: every call runs hand-written gfx950 kernels.  There
+is no CPU fallback; loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import config as C
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liberfx.so")
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+
+class State(ctypes.Structure):
+    """erfx_state: device pointers {cons, xmom, ymom, zmom}."""
+    _fields_ = [("cons", _P), ("xmom", _P), ("ymom", _P), ("zmom", _P)]
+
+
+# every symbol include/erfx.h declares, with its ctypes signature
+SIGNATURES = {
+    "erfx_last_error": (ctypes.c_char_p, []),
+    "erfx_config_defaults": (None, [ctypes.POINTER(C.Config)]),
+    "erfx_ghost_cells": (_I, [ctypes.POINTER(C.Config)]),
+    "erfx_layout_for": (_I, [ctypes.POINTER(C.Config), ctypes.POINTER(C.Layout)]),
+    "erfx_ctx_create": (_I, [ctypes.POINTER(C.Config), _P, ctypes.POINTER(_P)]),
+    "erfx_ctx_destroy": (_I, [_P]),
+    "erfx_ctx_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
+    "erfx_set_base_state": (_I, [_P, _P, _P, _P]),
+    "erfx_cons_to_prim": (_I, [_P, _P, _P, _P, _I]),
+    "erfx_apply_bcs": (_I, [_P, ctypes.POINTER(State), _P, _P, _P, _I, _I, _I, _I]),
+    "erfx_make_fast_coeffs": (_I, [_P, _P, ctypes.POINTER(State), _P, _P, _D, _D]),
+    "erfx_fast_rhs_N": (_I, [_P, _I, _I, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
+                             _P, _P, _P, ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
+    "erfx_turbulent_viscosity": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "erfx_slow_rhs_pre": (_I, [_P, _I, _D, ctypes.POINTER(State), ctypes.POINTER(State), _P,
+                               ctypes.POINTER(State), _P, _P, _P, _P, _P]),
+    "erfx_slow_rhs_post": (_I, [_P, _I, _D, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
+                                ctypes.POINTER(State), _P, ctypes.POINTER(State), _P, _P, _P]),
+    "erfx_sim_create": (_I, [ctypes.POINTER(C.Config), ctypes.POINTER(_P)]),
+    "erfx_sim_destroy": (_I, [_P]),
+    "erfx_sim_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
+    "erfx_sim_field_size": (ctypes.c_long, [_P, _I]),
+    "erfx_sim_field_ptr": (_P, [_P, _I]),
+    "erfx_sim_ctx": (_P, [_P]),
+    "erfx_sim_upload": (_I, [_P, _I, _P]),
+    "erfx_sim_download": (_I, [_P, _I, _P]),
+    "erfx_sim_step": (_I, [_P, _D, _D, _I]),
+    "erfx_sim_step_async": (_I, [_P, _D, _D, _I]),
+    "erfx_sim_sync": (_I, [_P]),
+    "erfx_sim_stream": (_P, [_P]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("erf_amd/liberfx.so is not built: run __graft_entry__.build() "
+                               "(make -C erf_amd/csrc)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+class ErfxError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise ErfxError("erfx error %d: %s" % (rc, lib().erfx_last_error().decode()))
+
+
+class Sim:
+    """Whole-step driver owning every buffer in HBM (erfx_sim)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = _P()
+        check(lib().erfx_sim_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.L = C.Layout()
+        check(lib().erfx_sim_layout(self.h, ctypes.byref(self.L)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erfx_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        n = lib().erfx_sim_field_size(self.h, fid)
+        if a.size != n:
+            raise ValueError("field %d: %d values given, %d expected" % (fid, a.size, n))
+        check(lib().erfx_sim_upload(self.h, fid, a.ctypes.data))
+
+    def download(self, fid):
+        n = lib().erfx_sim_field_size(self.h, fid)
+        if n < 0:
+            raise ValueError("unknown field %d" % fid)
+        a = np.empty((n // self.L.nelem,) + self.L.shape3())
+        check(lib().erfx_sim_download(self.h, fid, a.ctypes.data))
+        return a
+
+    def load(self, fields):
+        for fid, a in fields.items():
+            self.upload(fid, a)
+
+    def step(self, t0, dt, n):
+        check(lib().erfx_sim_step(self.h, t0, dt, n))
+
+    def step_async(self, t0, dt, n):
+        check(lib().erfx_sim_step_async(self.h, t0, dt, n))
+
+    def sync(self):
+        check(lib().erfx_sim_sync(self.h))
+
+    def ptr(self, fid):
+        return lib().erfx_sim_field_ptr(self.h, fid)
+
+    def state(self, s):
+        """erfx_state of integrator buffer s (config.S_OLD ... config.F_SLOW)."""
+        return State(*(self.ptr(C.state_id(s, c)) for c in range(4)))
+
+    @property
+    def ctx(self):
+        return lib().erfx_sim_ctx(self.h)
+
+    @property
+    def stream(self):
+        return lib().erfx_sim_stream(self.h)

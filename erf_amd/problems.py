@@ -1,0 +1,232 @@
+"""Synthetic initial states for the benchmark and test cases.
+
+Each setup returns (cfg, fields) where `fields` maps a field id (F_CONS, F_XVEL,
+..., F_PI0) to a float64 array shaped (ncomp, nz_tot, ny_tot, sy) in the layout
+of include/erfx.h (index [n, k+ng, j+ng, i+ng]).  Inputs are generated with a
+seeded numpy Generator; ERF's own random numbers cannot be reproduced.
+
+The base state follows ERF's discrete hydrostatic balance:
+  * density by Newton iteration per level (HSEutils::init_isentropic_hse,
+    Source/Utils/ERF_HSE_utils.H:88-160; generalised to a theta(z) profile),
+  * pressure, Exner function and ghost layers by ERF::erf_enforce_hse
+    (Source/Initialization/ERF_init1d.cpp:153-313).
+"""
+import math
+import numpy as np
+
+from . import config as C
+
+
+def _p_of_rt(rt):
+    """getPgivenRTh (ERF_EOS.H:89-93), dry."""
+    return C.p_0 * np.power(C.R_d * rt * 1.0 * (1.0 / C.p_0), C.Gamma)
+
+
+def _rt_of_p(p):
+    """getRhoThetagivenP (ERF_EOS.H:144-149), dry."""
+    return np.power(p * math.pow(C.p_0, C.Gamma - 1), 1.0 / C.Gamma) * (1.0 / C.R_d) / 1.0
+
+
+def _exner_of_p(p, rdOcp):
+    """getExnergivenP (ERF_EOS.H:124-128)."""
+    return np.power(p * (1.0 / C.p_0), rdOcp)
+
+
+def hse_density(nz, dz, theta_cc, g=C.CONST_GRAV):
+    """Column density in discrete HSE for cell-centre potential temperatures
+    theta_cc[0..nz-1] (Newton iteration as in init_isentropic_hse with the
+    surface pressure p_0 at z = 0).  Returns r[0..nz] (r[nz] = r[nz-1])."""
+    max_iter, tol = 10, 1.e-8
+    r = np.zeros(nz + 1)
+    p = np.zeros(nz + 1)
+    half_dz = 0.5 * dz
+    th = theta_cc[0]
+    r[0] = C.p_0 / (C.R_d * th)
+    for _ in range(max_iter):
+        p_hse = C.p_0 - half_dz * r[0] * g
+        p_eos = float(_p_of_rt(r[0] * th))
+        A = p_hse - p_eos
+        dpdr = C.Gamma * C.p_0 * math.pow(C.R_d * th * (1.0 / C.p_0), C.Gamma) * math.pow(r[0], C.Gamma - 1.0)
+        drho = A / (dpdr + half_dz * g)
+        r[0] = r[0] + drho
+        p[0] = float(_p_of_rt(r[0] * th))
+        if abs(drho) < tol:
+            break
+    for k in range(1, nz):
+        th = theta_cc[k]
+        r[k] = r[k - 1]
+        for _ in range(max_iter):
+            r_avg = 0.5 * (r[k - 1] + r[k])
+            p_hse = p[k - 1] - dz * r_avg * g
+            p_eos = float(_p_of_rt(r[k] * th))
+            A = p_hse - p_eos
+            dpdr = C.Gamma * C.p_0 * math.pow(C.R_d * th * (1.0 / C.p_0), C.Gamma) * math.pow(r[k], C.Gamma - 1.0)
+            drho = A / (dpdr + dz * g)
+            r[k] = r[k] + drho
+            p[k] = float(_p_of_rt(r[k] * th))
+            if abs(drho) < tol * r[k - 1]:
+                break
+    r[nz] = r[nz - 1]
+    return r
+
+
+def enforce_hse(r, dz, rdOcp, g=C.CONST_GRAV):
+    """erf_enforce_hse on a flat column: p, pi for k = -1..nz from r[-1..nz]
+    (index offset 1: arrays have length nz+2)."""
+    nz = len(r) - 2
+    p = np.zeros(nz + 2)
+    hz = 0.5 * dz
+    p[1] = C.p_0 - hz * r[1] * g
+    p[0] = C.p_0 + hz * r[1] * g
+    for k in range(2, nz + 2):
+        dens_interp = 0.5 * (r[k] + r[k - 1])
+        p[k] = p[k - 1] - dz * dens_interp * g
+    pi = _exner_of_p(p, rdOcp)
+    return p, pi
+
+
+def empty_fields(cfg):
+    L = C.layout_for(cfg)
+    shp = L.shape3()
+    f = {}
+    for fid in (C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0, C.F_PI0):
+        f[fid] = np.zeros((C.ncomp_of(cfg, fid),) + shp)
+    return L, f
+
+
+def _base_state(cfg, L, f, theta_cc):
+    """Fill r0, p0, pi0 (cells, one ghost layer all round, then copied into every
+    lateral ghost column as FillBoundary would on a periodic box)."""
+    g = C.CONST_GRAV if cfg.use_gravity else 0.0
+    rdOcp = C.R_d / cfg.c_p
+    nz, ng = cfg.nz, L.ng
+    r = hse_density(nz, cfg.dz, theta_cc, g)
+    rr = np.concatenate([[r[0]], r])               # k = -1 (Neumann) .. nz
+    p, pi = enforce_hse(rr, cfg.dz, rdOcp, g)
+    for fid, col in ((C.F_R0, rr), (C.F_P0, p), (C.F_PI0, pi)):
+        a = f[fid][0]
+        a[:] = 0.0
+        a[ng - 1:ng + nz + 1, :, :] = col[:, None, None]
+    return rr, p, pi
+
+
+def _cell_centres(cfg, L):
+    ng = L.ng
+    kz = (np.arange(L.shape3()[0]) - ng + 0.5) * cfg.dz + cfg.prob_lo[2]
+    jy = (np.arange(L.shape3()[1]) - ng + 0.5) * cfg.dy + cfg.prob_lo[1]
+    ix = (np.arange(L.shape3()[2]) - ng + 0.5) * cfg.dx + cfg.prob_lo[0]
+    return ix, jy, kz
+
+
+def _fill_cons_from_theta(cfg, L, f, theta3):
+    """rho*theta from the base-state pressure; rho = rho*theta / theta (pressure
+    is unchanged by a theta perturbation: p depends on rho*theta only)."""
+    ng, nz = L.ng, cfg.nz
+    p0 = f[C.F_P0][0]
+    rt = _rt_of_p(p0)
+    cons = f[C.F_CONS]
+    sl = (slice(ng, ng + nz),)
+    cons[C.RHOTHETA][sl] = rt[sl]
+    cons[C.RHO][sl] = rt[sl] / theta3[sl]
+
+
+def rising_thermal(nx=64, ny=64, nz=64, seed=0, adv=C.ADV_CENTERED2):
+    """Dry rising thermal: 20x20x10 km, periodic x/y, slip walls, theta = 300 K
+    isentropic, +2 K cos^2 bubble of radius 2 km at (10, 10, 2) km, uniform
+    noise of +-0.01 m/s on u, v, w; centred 2nd-order advection, no diffusion,
+    dt = 1 s, MRI ratio 6."""
+    cfg = C.default_config(nx, ny, nz, 20000.0 / nx, 20000.0 / ny, 10000.0 / nz)
+    cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    cfg.mri_ratio = 6
+    L, f = empty_fields(cfg)
+    _base_state(cfg, L, f, np.full(nz, 300.0))
+    ix, jy, kz = _cell_centres(cfg, L)
+    Lr = np.sqrt(((ix[None, None, :] - 10000.0) / 2000.0) ** 2 +
+                 ((jy[None, :, None] - 10000.0) / 2000.0) ** 2 +
+                 ((kz[:, None, None] - 2000.0) / 2000.0) ** 2)
+    theta = 300.0 + np.where(Lr <= 1.0, 2.0 * (np.cos(math.pi * Lr) + 1.0) / 2.0, 0.0)
+    _fill_cons_from_theta(cfg, L, f, theta)
+    rng = np.random.default_rng(seed)
+    _noise_vel(cfg, L, f, rng, 0.01, (0.0, 0.0, 0.0))
+    return cfg, f
+
+
+def _noise_vel(cfg, L, f, rng, amp, mean):
+    ng, nx, ny, nz = L.ng, cfg.nx, cfg.ny, cfg.nz
+    for fid, (ex, ey, ez), m in ((C.F_XVEL, (1, 0, 0), mean[0]), (C.F_YVEL, (0, 1, 0), mean[1]),
+                                 (C.F_ZVEL, (0, 0, 1), mean[2])):
+        a = f[fid][0]
+        shape = (nz + ez, ny + ey, nx + ex)
+        a[ng:ng + nz + ez, ng:ng + ny + ey, ng:ng + nx + ex] = m + rng.uniform(-amp, amp, shape)
+    # periodic faces must agree on both sides (ERF OverrideSync, ERF.cpp:1350)
+    if cfg.bc[0] == C.BC_PERIODIC:
+        f[C.F_XVEL][0][:, :, ng + nx] = f[C.F_XVEL][0][:, :, ng]
+    if cfg.bc[1] == C.BC_PERIODIC:
+        f[C.F_YVEL][0][:, ng + ny, :] = f[C.F_YVEL][0][:, ng, :]
+    # no-penetration walls in z
+    f[C.F_ZVEL][0][ng, :, :] = 0.0
+    f[C.F_ZVEL][0][ng + nz, :, :] = 0.0
+
+
+def cbl(nx=256, ny=256, nz=128, seed=0):
+    """Dry convective boundary layer LES: dx = dy = 20 m, dz = 10 m, periodic
+    laterally, no-slip bottom with a surface heat flux of 0.1 K m/s, slip top;
+    theta = 300 K below 500 m then +0.003 K/m; uniform theta noise of +-0.25 K
+    in the lowest 200 m; geostrophic wind (10, 0) m/s with f = 1e-4 s^-1;
+    5th-order upwind advection, Smagorinsky Cs = 0.1, dt = 0.5 s.
+
+    The MRI ratio is 16 (1 + 8 + 16 acoustic substeps per step), not 4: at
+    dx = dy = 20 m a ratio of 4 gives dtau = 0.125 s and a horizontal acoustic
+    Courant number c*dtau*sqrt(dx^-2 + dy^-2) ~ 3, and the forward-backward
+    substep of erf_fast_rhs_N diverges within one step (in the oracle as on
+    the GPU); 16 is the smallest even ratio that stays stable."""
+    cfg = C.default_config(nx, ny, nz, 20.0, 20.0, 10.0)
+    cfg.bc[2] = C.BC_NO_SLIP_WALL
+    cfg.bc[5] = C.BC_SLIP_WALL
+    cfg.dycore_horiz_adv = cfg.dycore_vert_adv = C.ADV_UPWIND5
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = C.ADV_UPWIND5
+    cfg.les_type = C.LES_SMAGORINSKY
+    cfg.Cs = 0.1
+    cfg.mri_ratio = 16
+    cfg.use_coriolis = 1
+    # rotational_time_period = 4 pi / f, latitude 90 deg (ERF_DataStruct.H:475-516)
+    rot_period = 4.0 * math.pi / 1.0e-4
+    cfg.coriolis_factor = 2.0 * 2.0 * math.pi / rot_period
+    lat = 90.0 * (math.pi / 180.)
+    cfg.sinphi = math.sin(lat)
+    cfg.cosphi = math.cos(lat)
+    ug = (10.0, 0.0, 0.0)
+    cfg.abl_geo_forcing[0] = -cfg.coriolis_factor * (ug[1] * cfg.sinphi - ug[2] * cfg.cosphi)
+    cfg.abl_geo_forcing[1] = cfg.coriolis_factor * ug[0] * cfg.sinphi
+    cfg.abl_geo_forcing[2] = -cfg.coriolis_factor * ug[0] * cfg.cosphi
+    L, f = empty_fields(cfg)
+    zc = (np.arange(nz) + 0.5) * cfg.dz
+    th_col = np.where(zc < 500.0, 300.0, 300.0 + 0.003 * (zc - 500.0))
+    rr, _, _ = _base_state(cfg, L, f, th_col)
+    # surface flux of rho*theta [kg m^-2 s^-1 K] = rho_sfc * 0.1 K m/s
+    cfg.use_surface_heat_flux = 1
+    cfg.surface_heat_flux = rr[1] * 0.1
+    ix, jy, kz = _cell_centres(cfg, L)
+    rng = np.random.default_rng(seed)
+    theta = np.zeros(L.shape3())
+    ng = L.ng
+    theta[ng:ng + nz] = th_col[:, None, None]
+    noise = rng.uniform(-0.25, 0.25, (nz, L.shape3()[1], L.shape3()[2]))
+    mask = (zc < 200.0)[:, None, None]
+    theta[ng:ng + nz] += np.where(mask, noise, 0.0)
+    theta[:ng] = 300.0
+    theta[ng + nz:] = 300.0
+    _fill_cons_from_theta(cfg, L, f, theta)
+    _noise_vel(cfg, L, f, rng, 0.0, ug)
+    f[C.F_XVEL][0][ng:ng + nz, ng:ng + ny, ng:ng + nx + 1] = ug[0]
+    return cfg, f
+
+
+def small_case(kind, nx, ny, nz, seed=0, **kw):
+    """Reduced-size versions of the cases for tests (same physics)."""
+    if kind == "thermal":
+        return rising_thermal(nx, ny, nz, seed)
+    if kind == "cbl":
+        return cbl(nx, ny, nz, seed)
+    raise ValueError(kind)

@@ -1,0 +1,256 @@
+/*
+ * erfx.h -- C ABI of the MI355X-native compressible dycore step.
+ *
+ * This is the drop-in boundary for ERF's multirate (MRI) RK3 dycore step.
+ * ERF wires its step through std::function hooks on
+ * MRISplitIntegrator<Vector<MultiFab>> (Source/TimeIntegration/ERF_MRI.H:137-180);
+ * each hook below replaces one of those callees with plain pointers and sizes.
+ *
+ * Data model
+ * ----------
+ * One box per context.  Every field of the box (cell-centred or face-centred)
+ * is stored with the SAME layout `erfx_layout`: i fastest, then j, then k, then
+ * component, with `ng` ghost cells on every side and room for the extra face
+ * in each direction:
+ *
+ *     elem(i,j,k,n) = p[(i+ng) + (j+ng)*sy + (k+ng)*sz + n*nelem]
+ *     valid cells  i in [0,nx-1]  (x-faces: [0,nx]), likewise j and k.
+ *
+ * This is AMReX's Array4 order (ERF_IndexDefines.H, component-major) with one
+ * common stride set so fused kernels can address every field with one index.
+ * A "state" is ERF's T = Vector<MultiFab>{cons, xmom, ymom, zmom}.
+ *
+ * Conventions
+ * -----------
+ *  - All pointers passed to hook functions are device pointers, laid out as
+ *    `erfx_layout` reports.  The caller owns them.
+ *  - Every call is stream-ordered on the context's stream and returns an int
+ *    status (0 = ok, < 0 = error); erfx_last_error() gives the message.  No
+ *    C++ exception crosses the ABI.  A context is not re-entrant.
+ *  - Arithmetic is IEEE FP64 with no contraction, matching the reference's
+ *    operation order (ERF computes in amrex::Real = double).
+ */
+#ifndef ERFX_H
+#define ERFX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    ERFX_OK              = 0,
+    ERFX_ERR_ARG         = -1,   /* bad argument (null pointer, bad enum, bad size) */
+    ERFX_ERR_LAYOUT      = -2,   /* layout does not match what the kernels assume */
+    ERFX_ERR_HIP         = -3,   /* a HIP runtime call failed                     */
+    ERFX_ERR_UNSUPPORTED = -4,   /* configuration not implemented                 */
+    ERFX_ERR_STATE       = -5,   /* physically invalid state (e.g. rho*theta <= 0) */
+    ERFX_ERR_NODEVICE    = -6    /* no usable GPU                                  */
+};
+
+/* ---- boundary types per domain face (ERF_BC, Source/ERF_IndexDefines.H) -- */
+enum {
+    ERFX_BC_PERIODIC     = 0,
+    ERFX_BC_SLIP_WALL    = 1,
+    ERFX_BC_NO_SLIP_WALL = 2,
+    ERFX_BC_SYMMETRY     = 3,
+    ERFX_BC_OUTFLOW      = 4
+};
+
+/* ---- advection schemes (AdvType, Source/DataStructs/ERF_AdvStruct.H) ---- */
+enum {
+    ERFX_ADV_CENTERED2 = 0,
+    ERFX_ADV_UPWIND3   = 1,
+    ERFX_ADV_CENTERED4 = 2,
+    ERFX_ADV_UPWIND5   = 3,
+    ERFX_ADV_CENTERED6 = 4
+};
+
+/* ---- diffusion (MolecDiffType / LESType, ERF_DiffStruct.H, ERF_TurbStruct.H) */
+enum { ERFX_MOLEC_NONE = 0, ERFX_MOLEC_CONSTANT = 1, ERFX_MOLEC_CONSTANT_ALPHA = 2 };
+enum { ERFX_LES_NONE = 0, ERFX_LES_SMAGORINSKY = 1 };
+
+/* ---- conserved-variable components (ERF_IndexDefines.H:13-50) ---------- */
+enum {
+    ERFX_RHO = 0, ERFX_RHOTHETA = 1, ERFX_RHOKE = 2, ERFX_RHOQKE = 3,
+    ERFX_RHOSCALAR = 4, ERFX_NCONS_DRY = 5
+};
+
+/* ---- run configuration: the subset of SolverChoice the step reads ----- */
+typedef struct erfx_config {
+    int    nx, ny, nz;            /* valid cells                                     */
+    double dx, dy, dz;            /* uniform spacing [m]                             */
+    double prob_lo[3];            /* domain origin [m]                               */
+    int    bc[6];                 /* xlo, ylo, zlo, xhi, yhi, zhi (AMReX Orientation) */
+    int    ncons;                 /* 5 (dry)                                         */
+
+    /* advection (ERF_AdvStruct.H) */
+    int    dycore_horiz_adv, dycore_vert_adv;
+    int    dryscal_horiz_adv, dryscal_vert_adv;
+    double dycore_horiz_upw_frac, dycore_vert_upw_frac;
+    double dryscal_horiz_upw_frac, dryscal_vert_upw_frac;
+
+    /* diffusion (ERF_DiffStruct.H / ERF_TurbStruct.H) */
+    int    molec_diff_type;
+    double dynamic_viscosity, rho0_trans, alpha_T, alpha_C;
+    int    les_type;
+    double Cs, Pr_t, Sc_t;
+    /* prescribed surface flux of rho*theta at the bottom face, used in place of
+     * the diffusive z-flux at k = 0 (the explicit-MOST branch of
+     * DiffusionSrcForState_N, ERF_DiffusionSrcForState_N.cpp:300-301) */
+    int    use_surface_heat_flux;
+    double surface_heat_flux;     /* kg m^-2 s^-1 K */
+
+    /* gravity / thermodynamics */
+    int    use_gravity;
+    double c_p;                   /* rdOcp = R_d / c_p                               */
+
+    /* forcing (ERF_make_mom_sources.cpp) */
+    int    use_coriolis;
+    double coriolis_factor, sinphi, cosphi;
+    double abl_geo_forcing[3];
+
+    /* time integration (ERF_MRI.H, ERF_TI_fast_rhs_fun.H) */
+    int    mri_ratio;             /* fixed_mri_dt_ratio, even, > 1                   */
+    int    force_stage1_single_substep;
+    int    implicit_substepping;  /* 1: beta_s = 0.1, 0: beta_s = -1 (explicit)      */
+} erfx_config;
+
+/* ---- layout shared by every field of a context ------------------------- */
+typedef struct erfx_layout {
+    int  nx, ny, nz, ng;
+    long sy, sz;                  /* strides in elements                             */
+    long nelem;                   /* elements per component (>= (nz+1+2ng)*sz)       */
+} erfx_layout;
+
+/* ---- a state T = {cons, xmom, ymom, zmom} (device pointers) ------------ */
+typedef struct erfx_state {
+    double* cons;                 /* ncons components, nelem apart                  */
+    double* xmom;
+    double* ymom;
+    double* zmom;
+} erfx_state;
+
+typedef struct erfx_ctx erfx_ctx;
+
+/* Thread-local text of the last error on this thread. */
+const char* erfx_last_error(void);
+
+/* Fill `cfg` with ERF's defaults for the fields the step reads
+ * (ERF_DataStruct.H, ERF_AdvStruct.H:90-99,281-286, ERF_DiffStruct.H). */
+void erfx_config_defaults(erfx_config* cfg);
+
+/* Ghost width ERF would use for this configuration, plus one for cons
+ * (ERF::ComputeGhostCells, ERF.H:1081-1125; ERF_make_new_arrays.cpp:112-113). */
+int erfx_ghost_cells(const erfx_config* cfg);
+
+/* Create a context: validates cfg, picks the layout, allocates the workspace
+ * on the current device.  `stream` is a hipStream_t or NULL (default stream). */
+int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out);
+int erfx_ctx_destroy(erfx_ctx* ctx);
+int erfx_ctx_layout(const erfx_ctx* ctx, erfx_layout* out);
+
+/* Base state {r0, p0, pi0} on cells with ghosts (BaseState, ERF_IndexDefines.H:63-71).
+ * The pointers are kept and read by later calls. */
+int erfx_set_base_state(erfx_ctx* ctx, const double* r0, const double* p0, const double* pi0);
+
+/* cons_to_prim (ERF_TI_utils.H:4-45): prim(n) = cons(n+1)/rho on the box grown
+ * by `ng`; pi_stage = Exner(rho*theta) on the box grown by 1. */
+int erfx_cons_to_prim(erfx_ctx* ctx, const double* cons, double* prim, double* pi_stage, int ng);
+
+/* apply_bcs (ERF_TI_utils.H:53-150): FillIntermediatePatch at level 0 on a state:
+ * momentum -> velocity on the valid region, physical/periodic ghost fills of cons
+ * and velocity, velocity -> momentum on valid + 1 ghost (ERF_FillIntermediatePatch.cpp:27-261). */
+int erfx_apply_bcs(erfx_ctx* ctx, erfx_state* S, double* xvel, double* yvel, double* zvel,
+                   int ng_cons, int ng_vel, int fast_only, int vel_and_mom_synced);
+
+/* make_fast_coeffs (ERF_make_fast_coeffs.cpp:27-294, flat branch): writes the
+ * five z-face coefficient fields A, 1/B, C, P, Q into `fast_coeffs` (5 comps). */
+int erfx_make_fast_coeffs(erfx_ctx* ctx, double* fast_coeffs, const erfx_state* stage,
+                          const double* prim, const double* pi_stage, double dtau, double beta_s);
+
+/* erf_fast_rhs_N (ERF_fast_rhs_N.cpp:36-614; ERF_TI_fast_headers.H:22-41):
+ * one acoustic substep on a flat grid.  For step > 0 the reference passes
+ * S_prev == S_data (same memory); pass the same pointers here. */
+int erfx_fast_rhs_N(erfx_ctx* ctx, int step, int nrk,
+                    const erfx_state* slow_rhs, const erfx_state* prev,
+                    const erfx_state* stage, const double* prim, const double* pi_stage,
+                    const double* fast_coeffs, erfx_state* data, erfx_state* scratch,
+                    double dtau, double beta_s, double facinv);
+
+/* Strain + Smagorinsky eddy viscosity, once per step (ERF_advance_dycore.cpp:114-223;
+ * ComputeStrain_N, ComputeTurbulentViscosityLES).  `eddy` holds 13 components
+ * (EddyDiff::NumDiffs, ERF_IndexDefines.H:152-169), `hfx3` the vertical SGS heat flux. */
+int erfx_turbulent_viscosity(erfx_ctx* ctx, const double* cons, const double* xvel,
+                             const double* yvel, const double* zvel, double* eddy, double* hfx3);
+
+/* erf_slow_rhs_pre (ERF_slow_rhs_pre.cpp:67-774) preceded by make_mom_sources
+ * (ERF_make_mom_sources.cpp:40-450) as in ERF_TI_slow_rhs_fun.H:216-241:
+ * slow RHS of rho, rho*theta and the momenta; seeds the averaged momenta in
+ * scratch.  `eddy`/`hfx3` may be NULL when no LES model is configured. */
+int erfx_slow_rhs_pre(erfx_ctx* ctx, int nrk, double slow_dt,
+                      erfx_state* rhs, const erfx_state* data, const double* prim,
+                      erfx_state* scratch, const double* xvel, const double* yvel,
+                      const double* zvel, const double* eddy, double* hfx3);
+
+/* erf_slow_rhs_post (ERF_slow_rhs_post.cpp:49-559): slow scalars with the
+ * time-averaged momenta, then S_new <- S_data. */
+int erfx_slow_rhs_post(erfx_ctx* ctx, int nrk, double slow_dt,
+                       erfx_state* rhs, const erfx_state* old, erfx_state* snew,
+                       erfx_state* data, const double* prim, const erfx_state* scratch,
+                       const double* xvel, const double* yvel, const double* eddy);
+
+/* ---- whole-step driver ------------------------------------------------- *
+ * ERF::Advance + advance_dycore + MRISplitIntegrator::advance on one box
+ * (ERF_Advance.cpp:20-233, ERF_advance_dycore.cpp:37-305, ERF_MRI.H:187-369).
+ * The sim owns every buffer in HBM; the persistent state is {cons, xvel, yvel,
+ * zvel} like ERF's vars_new. */
+typedef struct erfx_sim erfx_sim;
+
+enum {
+    ERFX_F_CONS = 0, ERFX_F_XVEL = 1, ERFX_F_YVEL = 2, ERFX_F_ZVEL = 3,
+    ERFX_F_R0 = 4, ERFX_F_P0 = 5, ERFX_F_PI0 = 6,
+    ERFX_F_NUM = 7
+};
+
+/* Diagnostic ids: the integrator's internal buffers, for hook-level testing.
+ * State buffers are ERFX_D_STATE_BASE + 4*s + c with s = 0 S_old, 1 S_new,
+ * 2 S_sum, 3 S_scratch, 4 F_slow (ERF_MRI.H:67-83) and c = 0 cons, 1 xmom,
+ * 2 ymom, 3 zmom. */
+enum {
+    ERFX_D_PRIM = 16, ERFX_D_PI_STAGE = 17, ERFX_D_FAST_COEFFS = 18, ERFX_D_EDDY = 19,
+    ERFX_D_XVEL_OLD = 20, ERFX_D_YVEL_OLD = 21, ERFX_D_ZVEL_OLD = 22,
+    ERFX_D_XMOM_SRC = 23, ERFX_D_YMOM_SRC = 24, ERFX_D_ZMOM_SRC = 25,
+    ERFX_D_STATE_BASE = 32
+};
+
+/* Layout every context built from `cfg` uses (no device needed):
+ * ng = erfx_ghost_cells(cfg), sy = roundup(nx+1+2ng, 16), sz = sy*(ny+1+2ng),
+ * nelem = sz*(nz+1+2ng). */
+int  erfx_layout_for(const erfx_config* cfg, erfx_layout* out);
+
+int  erfx_sim_create(const erfx_config* cfg, erfx_sim** out);
+/* device pointer of a sim buffer (ERFX_F_* or ERFX_D_*), NULL if unknown */
+double* erfx_sim_field_ptr(erfx_sim* sim, int id);
+/* the hook context the sim drives (same stream, same layout) */
+erfx_ctx* erfx_sim_ctx(erfx_sim* sim);
+int  erfx_sim_destroy(erfx_sim* sim);
+int  erfx_sim_layout(const erfx_sim* sim, erfx_layout* out);
+/* number of doubles a host buffer for field `id` must hold (ncomp * nelem) */
+long erfx_sim_field_size(const erfx_sim* sim, int id);
+int  erfx_sim_upload(erfx_sim* sim, int id, const double* host);
+int  erfx_sim_download(erfx_sim* sim, int id, double* host);
+/* Advance `nsteps` steps of size dt from time t0 (synchronous at return). */
+int  erfx_sim_step(erfx_sim* sim, double t0, double dt, int nsteps);
+/* Same, asynchronous on the sim's stream (for timing with events). */
+int  erfx_sim_step_async(erfx_sim* sim, double t0, double dt, int nsteps);
+int  erfx_sim_sync(erfx_sim* sim);
+/* The sim's stream (a hipStream_t) for event timing. */
+void* erfx_sim_stream(erfx_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ERFX_H */

@@ -1,0 +1,137 @@
+// api.cpp -- TEST INFRASTRUCTURE ONLY (oracle).  C entry points for ctypes.
+// Field ids follow include/erfx.h (ERFX_F_*) plus the diagnostic ids
+// ERFX_D_* for the integrator's internal buffers.
+#include "oracle.hpp"
+#include <cstdio>
+
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+
+Field* field_by_id(Level& L, int id) {
+    switch (id) {
+        case ERFX_F_CONS: return &L.cons;
+        case ERFX_F_XVEL: return &L.xvel;
+        case ERFX_F_YVEL: return &L.yvel;
+        case ERFX_F_ZVEL: return &L.zvel;
+        case ERFX_F_R0: return &L.r0;
+        case ERFX_F_P0: return &L.p0;
+        case ERFX_F_PI0: return &L.pi0;
+        case ERFX_D_PRIM: return &L.prim;
+        case ERFX_D_PI_STAGE: return &L.pi_stage;
+        case ERFX_D_FAST_COEFFS: return &L.fast_coeffs;
+        case ERFX_D_EDDY: return &L.eddy;
+        case ERFX_D_XVEL_OLD: return &L.xvel_old;
+        case ERFX_D_YVEL_OLD: return &L.yvel_old;
+        case ERFX_D_ZVEL_OLD: return &L.zvel_old;
+        case ERFX_D_XMOM_SRC: return &L.xmom_src;
+        case ERFX_D_YMOM_SRC: return &L.ymom_src;
+        case ERFX_D_ZMOM_SRC: return &L.zmom_src;
+        default: break;
+    }
+    if (id >= ERFX_D_STATE_BASE && id < ERFX_D_STATE_BASE + 20) {
+        int s = (id - ERFX_D_STATE_BASE) / 4, c = (id - ERFX_D_STATE_BASE) % 4;
+        StateF* st[5] = {&L.S_old, &L.S_new, &L.S_sum, &L.S_scratch, &L.F_slow};
+        Field* f[4] = {&st[s]->cons, &st[s]->xmom, &st[s]->ymom, &st[s]->zmom};
+        return f[c];
+    }
+    return nullptr;
+}
+
+template <class F> int guard(F&& f) {
+    try { f(); return 0; }
+    catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
+    auto* L = new Level();
+    try { L->init(*cfg, *lay); }
+    catch (const std::exception& e) { g_err = e.what(); delete L; return nullptr; }
+    return L;
+}
+void orc_destroy(void* h) { delete static_cast<Level*>(h); }
+
+long orc_field_size(void* h, int id) {
+    Level& L = *static_cast<Level*>(h);
+    Field* f = field_by_id(L, id);
+    return f ? (long)f->v.size() : -1;
+}
+int orc_set(void* h, int id, const double* src) {
+    Level& L = *static_cast<Level*>(h);
+    Field* f = field_by_id(L, id);
+    if (!f) { g_err = "bad field id"; return -1; }
+    std::memcpy(f->v.data(), src, f->v.size() * sizeof(double));
+    return 0;
+}
+int orc_get(void* h, int id, double* dst) {
+    Level& L = *static_cast<Level*>(h);
+    Field* f = field_by_id(L, id);
+    if (!f) { g_err = "bad field id"; return -1; }
+    std::memcpy(dst, f->v.data(), f->v.size() * sizeof(double));
+    return 0;
+}
+
+int orc_step(void* h, double t0, double dt, int nsteps) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { for (int s = 0; s < nsteps; ++s) L.advance(t0 + s * dt, dt); });
+}
+
+// ---- hook-level entry points on the level's own buffers ------------------
+int orc_cons_to_prim(void* h, int cons_id, int ng) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.cons_to_prim(L.A(*field_by_id(L, cons_id)), ng); });
+}
+// state_index: 0 S_old, 1 S_new, 2 S_sum
+int orc_apply_bcs(void* h, int state_index, int ng_cons, int ng_vel, int fast_only, int synced) {
+    Level& L = *static_cast<Level*>(h);
+    StateF* st[3] = {&L.S_old, &L.S_new, &L.S_sum};
+    return guard([&] {
+        L.apply_bcs(L.A(*st[state_index]), L.A(L.xvel), L.A(L.yvel), L.A(L.zvel), ng_cons, ng_vel,
+                    fast_only != 0, synced != 0);
+    });
+}
+int orc_make_fast_coeffs(void* h, double dtau, double beta_s) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        L.make_fast_coeffs(L.A(L.fast_coeffs), L.A(L.S_new), L.A(L.prim), L.A(L.pi_stage), dtau, beta_s);
+    });
+}
+int orc_fast_rhs_N(void* h, int step, int nrk, double dtau, double beta_s, double facinv) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        StateA prev = step == 0 ? L.A(L.S_old) : L.A(L.S_sum);
+        L.fast_rhs_N(step, nrk, L.A(L.F_slow), prev, L.A(L.S_new), L.A(L.prim), L.A(L.pi_stage),
+                     L.A(L.fast_coeffs), L.A(L.S_sum), L.A(L.S_scratch), dtau, beta_s, facinv);
+    });
+}
+int orc_turbulent_viscosity(void* h) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        L.turbulent_viscosity(L.A(L.S_old.cons), L.A(L.xvel_old), L.A(L.yvel_old), L.A(L.zvel_old));
+    });
+}
+int orc_slow_rhs_pre(void* h, int nrk, double slow_dt) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        std::fill(L.cc_src.v.begin(), L.cc_src.v.end(), 0.0);
+        L.make_mom_sources(L.A(L.S_new));
+        L.slow_rhs_pre(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_new), L.A(L.prim), L.A(L.S_scratch),
+                       L.A(L.xvel), L.A(L.yvel), L.A(L.zvel));
+    });
+}
+int orc_slow_rhs_post(void* h, int nrk, double slow_dt) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        std::fill(L.cc_src.v.begin(), L.cc_src.v.end(), 0.0);
+        L.slow_rhs_post(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_old), L.A(L.S_new), L.A(L.S_sum),
+                        L.A(L.prim), L.A(L.S_scratch), L.A(L.xvel), L.A(L.yvel));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// mri.cpp -- TEST INFRASTRUCTURE ONLY (oracle).  One ERF time step on a
+// single level: ERF::Advance -> advance_dycore -> MRISplitIntegrator::advance.
+#include "oracle.hpp"
+
+namespace orc {
+
+static void copy_field(Field& dst, const Field& src) { dst.v = src.v; dst.ncomp = src.ncomp; }
+
+// ERF::advance_dycore (Source/TimeIntegration/ERF_advance_dycore.cpp:37-305)
+// with the MRI stage loop of MRISplitIntegrator::advance (ERF_MRI.H:187-334)
+// and the hook bodies of ERF_TI_slow_rhs_fun.H / ERF_TI_fast_rhs_fun.H inlined.
+void Level::advance_dycore(double old_time, double dt) {
+    const bool use_diff = cfg.molec_diff_type != ERFX_MOLEC_NONE || cfg.les_type != ERFX_LES_NONE;
+    const bool use_kturb = cfg.les_type != ERFX_LES_NONE;
+    Arr u_old = A(xvel_old), v_old = A(yvel_old), w_old = A(zvel_old);
+    StateA So = A(S_old), Sn = A(S_new), Ss = A(S_sum), Sc = A(S_scratch), Fs = A(F_slow);
+    Arr xv = A(xvel), yv = A(yvel), zv = A(zvel);
+
+    // once-per-step strain + eddy viscosity (:114-223)
+    if (use_diff && use_kturb) turbulent_viscosity(So.cons, u_old, v_old, w_old);
+    else if (use_diff) {
+        // the strain is still computed into Tau (consumed by make_tau_terms' overwrite)
+    }
+    // V2M with one ghost (:256-264)
+    int g1[3] = {1, 1, 1}, g0[3] = {1, 1, 0};
+    velocity_to_momentum(u_old, g1, v_old, g1, w_old, g0, So.cons, So.xmom, So.ymom, So.zmom);
+    copy_field(xvel, xvel_old); copy_field(yvel, yvel_old); copy_field(zvel, zvel_old);
+    apply_bcs(So, xv, yv, zv, ng_cons, ng_vel, false, true);
+    cons_to_prim(So.cons, ng_cons);
+
+    // MRISplitIntegrator::advance
+    const int ratio = cfg.mri_ratio;
+    if (!(ratio > 1 && ratio % 2 == 0)) throw std::runtime_error("mri ratio must be even and > 1");
+    copy_field(S_new.cons, S_old.cons); copy_field(S_new.xmom, S_old.xmom);
+    copy_field(S_new.ymom, S_old.ymom); copy_field(S_new.zmom, S_old.zmom);
+    const double timestep = dt;
+    const double sub_timestep = timestep / ratio;
+    double time_stage = old_time, old_time_stage;
+    const double beta_s = cfg.implicit_substepping ? 0.1 : -1.0;
+    for (int nrk = 0; nrk < 3; nrk++) {
+        old_time_stage = time_stage;
+        int nsubsteps = 1; double dtau = 0;
+        if (nrk == 0) {
+            if (cfg.force_stage1_single_substep) { nsubsteps = 1; dtau = timestep / 3.0; }
+            else { nsubsteps = ratio / 3; dtau = sub_timestep; }
+            time_stage = old_time + timestep / 3.0;
+        }
+        if (nrk == 1) { nsubsteps = ratio / 2; dtau = sub_timestep; time_stage = old_time + timestep / 2.0; }
+        if (nrk == 2) { nsubsteps = ratio; dtau = sub_timestep; time_stage = old_time + timestep; }
+        if (nrk > 0) cons_to_prim(Sn.cons, ng_cons);   // pre_update
+        // slow_rhs_fun_pre: make_sources (cc_src = 0 for the supported forcing),
+        // make_mom_sources, erf_slow_rhs_pre
+        double slow_dt = time_stage - old_time;
+        std::fill(cc_src.v.begin(), cc_src.v.end(), 0.0);
+        make_mom_sources(Sn);
+        slow_rhs_pre(nrk, slow_dt, Fs, Sn, A(prim), Sc, xv, yv, zv);
+        double inv_fac = 1.0 / static_cast<double>(nsubsteps);
+        for (int ks = 0; ks < nsubsteps; ++ks) {
+            if (ks == 0) {
+                make_fast_coeffs(A(fast_coeffs), Sn, A(prim), A(pi_stage), dtau, beta_s);
+                fast_rhs_N(ks, nrk, Fs, So, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
+            } else {
+                fast_rhs_N(ks, nrk, Fs, Ss, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
+            }
+            apply_bcs(Ss, xv, yv, zv, 1, 1, true, false);
+        }
+        slow_rhs_post(nrk, slow_dt, Fs, So, Sn, Ss, A(prim), Sc, xv, yv);
+        apply_bcs(Sn, xv, yv, zv, ng_cons, ng_vel, false, false);   // post_update
+        (void)old_time_stage;
+    }
+    (void)use_diff;
+}
+
+// ERF::Advance, level 0, dycore only (Source/TimeIntegration/ERF_Advance.cpp:20-233):
+// the old state is the previous new state; FillPatch + V2M of the old
+// velocities, then the dycore step writes the new cons and velocities.
+void Level::advance(double time, double dt) {
+    copy_field(cons_old_vars, cons);
+    copy_field(xvel_old, xvel); copy_field(yvel_old, yvel); copy_field(zvel_old, zvel);
+    Arr c = A(cons_old_vars), u = A(xvel_old), v = A(yvel_old), w = A(zvel_old);
+    fill_patch(c, u, v, w);
+    StateA So = A(S_old);
+    int gv[3] = {ng_vel, ng_vel, ng_vel};
+    velocity_to_momentum(u, gv, v, gv, w, gv, c, So.xmom, So.ymom, So.zmom);
+    copy_field(S_old.cons, cons_old_vars);   // cons_mf copy incl. ghosts (:127-128)
+    advance_dycore(time, dt);
+    copy_field(cons, S_new.cons);             // state_new cons aliases vars_new
+}
+
+}  // namespace orc

@@ -1,0 +1,187 @@
+// oracle.hpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of ERF's compressible dycore step (flat grid, dry), used as
+// the correctness oracle and as the CPU baseline.  Only tests/, the package's
+// smoke check and bench.py's cpu_baseline leg may load it; the product path
+// never calls it.
+//
+// The restatement follows the reference kernel by kernel and keeps its
+// floating-point operation order (build with -ffp-contract=off), so that it
+// reproduces ERF's results to rounding.  Each function cites the reference
+// file:line it restates.  Arrays use the layout of include/erfx.h.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+
+#include "../include/erfx.h"
+
+namespace orc {
+
+// ---- constants (Source/ERF_Constants.H:7-20) -------------------------------
+constexpr double PI       = 3.14159265358979323846264338327950288;
+constexpr double PIoTwo   = PI / 2.0;
+constexpr double R_d      = 287.0;
+constexpr double R_v      = 461.505;
+constexpr double Cp_d     = 1004.5;
+constexpr double p_0      = 1.0e5;
+constexpr double Gamma    = 1.4;
+constexpr double CONST_GRAV = 9.81;
+constexpr double ip_0     = 1. / p_0;
+constexpr double iR_d     = 1. / R_d;
+constexpr double iGamma   = 1. / Gamma;
+
+// ---- EOS (Source/Utils/ERF_EOS.H) -------------------------------------------
+inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_EOS.H:89-93
+    return p_0 * std::pow(R_d * rhotheta * (1.0 + (R_v / R_d) * qv) * ip_0, Gamma);
+}
+inline double getExnergivenRTh(double rhotheta, double rdOcp, double qv = 0.0) { // ERF_EOS.H:130-135
+    return std::pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma * rdOcp);
+}
+inline double getRhoThetagivenP(double p, double qv = 0.0) {              // ERF_EOS.H:144-149
+    return std::pow(p * std::pow(p_0, Gamma - 1), iGamma) * iR_d / (1.0 + R_v / R_d * qv);
+}
+inline double getRhogivenThetaPress(double theta, double p, double rdOcp, double qv = 0.0) {
+    return std::pow(p_0, rdOcp) * std::pow(p, iGamma) / (R_d * theta * (1.0 + R_v / R_d * qv));
+}
+inline double getdPdRgivenConstantTheta(double rho, double theta, double qv = 0.0) {
+    return Gamma * p_0 * std::pow((R_d * theta * (1.0 + R_v / R_d * qv) * ip_0), Gamma) *
+           std::pow(rho, Gamma - 1.0);
+}
+
+// ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
+enum BCK { INT_DIR = 0, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_ODD, NEUMANN_INT };
+
+// ---- array view -------------------------------------------------------------
+struct Arr {
+    double* p = nullptr;
+    long sy = 0, sz = 0, ne = 0;
+    int ng = 0;
+    inline double& operator()(int i, int j, int k, int n = 0) const {
+        return p[(long)(i + ng) + (long)(j + ng) * sy + (long)(k + ng) * sz + (long)n * ne];
+    }
+    explicit operator bool() const { return p != nullptr; }
+};
+
+struct Field {
+    std::vector<double> v;
+    int ncomp = 0;
+    void alloc(const erfx_layout& L, int nc) {
+        ncomp = nc;
+        v.assign((size_t)nc * (size_t)L.nelem, std::nan(""));
+    }
+    Arr a(const erfx_layout& L) {
+        Arr r; r.p = v.data(); r.sy = L.sy; r.sz = L.sz; r.ne = L.nelem; r.ng = L.ng; return r;
+    }
+};
+
+struct Box {
+    int lo[3], hi[3];
+    Box() { lo[0]=lo[1]=lo[2]=0; hi[0]=hi[1]=hi[2]=-1; }
+    Box(int l0,int l1,int l2,int h0,int h1,int h2){lo[0]=l0;lo[1]=l1;lo[2]=l2;hi[0]=h0;hi[1]=h1;hi[2]=h2;}
+    bool ok() const { return lo[0]<=hi[0] && lo[1]<=hi[1] && lo[2]<=hi[2]; }
+};
+
+template <class F> inline void loop(const Box& b, F&& f) {
+    if (!b.ok()) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i) f(i, j, k);
+}
+// serial variant for loops that must keep the reference's sequential order
+template <class F> inline void loop_serial(const Box& b, F&& f) {
+    if (!b.ok()) return;
+    for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i) f(i, j, k);
+}
+
+struct StateA { Arr cons, xmom, ymom, zmom; };
+struct StateF {
+    Field cons, xmom, ymom, zmom;
+    void alloc(const erfx_layout& L, int nc) { cons.alloc(L, nc); xmom.alloc(L,1); ymom.alloc(L,1); zmom.alloc(L,1); }
+    StateA a(const erfx_layout& L) { return {cons.a(L), xmom.a(L), ymom.a(L), zmom.a(L)}; }
+};
+
+// EddyDiff component indices (Source/ERF_IndexDefines.H:152-169)
+namespace EddyDiff {
+enum { Mom_h = 0, Theta_h, KE_h, QKE_h, Scalar_h, Q_h, Mom_v, Theta_v, KE_v, QKE_v, Scalar_v, Q_v, PBL_lengthscale, NumDiffs };
+}
+
+// ---- the level: configuration, BCs and every buffer of the step ----------
+struct Level {
+    erfx_config cfg;
+    erfx_layout L;
+    int ng_cons = 0, ng_vel = 0;
+    int ncons = 5;
+    double rdOcp = 0, gravity = 0;
+    double dxInv[3];
+    // BC kinds: [var][ori], var 0..2 = x/y/z velocity, 3 = cons; ori 0..5 = xlo,ylo,zlo,xhi,yhi,zhi
+    int bck[4][6];
+    double vel_extdir[3][6];
+    bool periodic[3];
+
+    // persistent state (vars_new) and base state
+    Field cons, xvel, yvel, zvel;
+    Field r0, p0, pi0;
+    // advance scratch (vars_old aliases and momenta)
+    Field cons_old_vars, xvel_old, yvel_old, zvel_old;
+    // MRI buffers
+    StateF S_old, S_new, S_sum, S_scratch, F_slow;
+    Field prim, pi_stage, fast_coeffs;
+    Field omega, eddy, hfx3;
+    Field tau11, tau22, tau33, tau12, tau13, tau23;
+    Field xmom_src, ymom_src, zmom_src, cc_src;
+
+    void init(const erfx_config& c, const erfx_layout& lay);
+    Arr A(Field& f) { return f.a(L); }
+    StateA A(StateF& s) { return s.a(L); }
+
+    // ---- hooks (restatements) --------------------------------------------
+    void cons_to_prim(Arr cons_in, int ng);
+    void fill_cons(Arr c, int icomp, int ncomp, int ng);
+    void fill_xvel(Arr u, int ng);
+    void fill_yvel(Arr v, int ng);
+    void fill_zvel(Arr w, int ng);
+    void fill_boundary_periodic(Arr a, int ncomp, int ng, int nodal_dir);
+    void momentum_to_velocity(Arr xv, Arr yv, Arr zv, Arr dens, Arr xm, Arr ym, Arr zm);
+    void velocity_to_momentum(Arr xv, int ngu[3], Arr yv, int ngv[3], Arr zv, int ngw[3], Arr dens,
+                              Arr xm, Arr ym, Arr zm);
+    void fill_intermediate_patch(StateA S, Arr xv, Arr yv, Arr zv, int ng_cons_, int ng_vel_,
+                                 bool cons_only, int icomp, int ncomp);
+    void apply_bcs(StateA S, Arr xv, Arr yv, Arr zv, int ng_cons_, int ng_vel_, bool fast_only,
+                   bool vel_and_mom_synced);
+    void fill_patch(Arr c, Arr u, Arr v, Arr w);
+
+    void make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double dtau, double beta_s);
+    void fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA stage, Arr prim_,
+                    Arr pi_st, Arr fc, StateA data, StateA scratch, double dtau, double beta_s,
+                    double facinv);
+
+    void compute_strain_N(const Box& bxcc, Box tbxxy, Box tbxxz, Box tbxyz, Arr u, Arr v, Arr w,
+                          Arr t11, Arr t22, Arr t33, Arr t12, Arr t13, Arr t23);
+    void turbulent_viscosity(Arr cons_in, Arr u, Arr v, Arr w);
+    void make_tau_terms(StateA data, Arr u, Arr v, Arr w);
+    void make_mom_sources(StateA data);
+    void slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_, StateA scratch,
+                      Arr u, Arr v, Arr w);
+    void slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA snew, StateA data,
+                       Arr prim_, StateA scratch, Arr u, Arr v);
+    void advection_src_scalars(const Box& bx, int icomp, int ncomp, Arr ax, Arr ay, Arr az,
+                               Arr cprim, Arr rhs, int hadv, int vadv, double hfrac, double vfrac);
+    void diffusion_src_state(const Box& bx, int start_comp, int num_comp, Arr cell_data,
+                             Arr cell_prim, Arr rhs);
+    void advection_src_mom(const Box& tbx, const Box& tby, const Box& tbz, Arr ru_rhs, Arr rv_rhs,
+                           Arr rw_rhs, Arr u, Arr v, Arr w, Arr rho_u, Arr rho_v, Arr Omega);
+
+    void advance_dycore(double old_time, double dt);
+    void advance(double time, double dt);   // ERF::Advance (lev 0, dycore only)
+};
+
+}  // namespace orc

@@ -1,0 +1,209 @@
+// slow.cpp -- TEST INFRASTRUCTURE ONLY (oracle).  Slow right-hand side of one
+// RK stage on a flat, dry grid: momentum sources, erf_slow_rhs_pre and
+// erf_slow_rhs_post.
+#include "oracle.hpp"
+#include <cstdio>
+
+namespace orc {
+
+// make_mom_sources (Source/SourceTerms/ERF_make_mom_sources.cpp:40-450) with
+// make_buoyancy type 1, dry (Source/SourceTerms/ERF_make_buoyancy.cpp:95-107):
+// buoyancy, Coriolis and geostrophic forcing.
+void Level::make_mom_sources(StateA data) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    Arr xs = A(xmom_src), ys = A(ymom_src), zs = A(zmom_src);
+    std::fill(xmom_src.v.begin(), xmom_src.v.end(), 0.0);
+    std::fill(ymom_src.v.begin(), ymom_src.v.end(), 0.0);
+    std::fill(zmom_src.v.begin(), zmom_src.v.end(), 0.0);
+    Arr c = data.cons, r0a = A(r0);
+    const double grav_z = -gravity;
+    loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
+        zs(i, j, k) = grav_z * 0.5 * ((c(i, j, k) - r0a(i, j, k)) + (c(i, j, k - 1) - r0a(i, j, k - 1)));
+    });
+    Box tbx(0, 0, 0, nx, ny - 1, nz - 1), tby(0, 0, 0, nx - 1, ny, nz - 1), tbz(0, 0, 0, nx - 1, ny - 1, nz - 1);
+    Arr ru = data.xmom, rv = data.ymom, rw = data.zmom;
+    if (cfg.use_coriolis) {
+        const double cf = cfg.coriolis_factor, sinphi = cfg.sinphi, cosphi = cfg.cosphi;
+        loop(tbx, [&](int i, int j, int k) {
+            double rho_v_loc = 0.25 * (rv(i, j + 1, k) + rv(i, j, k) + rv(i - 1, j + 1, k) + rv(i - 1, j, k));
+            double rho_w_loc = 0.25 * (rw(i, j, k + 1) + rw(i, j, k) + rw(i, j - 1, k + 1) + rw(i, j - 1, k));
+            xs(i, j, k) += cf * (rho_v_loc * sinphi - rho_w_loc * cosphi);
+        });
+        loop(tby, [&](int i, int j, int k) {
+            double rho_u_loc = 0.25 * (ru(i + 1, j, k) + ru(i, j, k) + ru(i + 1, j - 1, k) + ru(i, j - 1, k));
+            ys(i, j, k) += -cf * rho_u_loc * sinphi;
+        });
+        loop(tbz, [&](int i, int j, int k) {
+            double rho_u_loc = 0.25 * (ru(i + 1, j, k) + ru(i, j, k) + ru(i + 1, j, k - 1) + ru(i, j, k - 1));
+            zs(i, j, k) += cf * rho_u_loc * cosphi;
+        });
+    }
+    const double* g = cfg.abl_geo_forcing;
+    loop(tbx, [&](int i, int j, int k) {
+        double rho_on_u_face = 0.5 * (c(i, j, k) + c(i - 1, j, k));
+        xs(i, j, k) += rho_on_u_face * g[0];
+    });
+    loop(tby, [&](int i, int j, int k) {
+        double rho_on_v_face = 0.5 * (c(i, j, k) + c(i, j - 1, k));
+        ys(i, j, k) += rho_on_v_face * g[1];
+    });
+    loop(tbz, [&](int i, int j, int k) {
+        double rho_on_w_face = 0.5 * (c(i, j, k) + c(i, j, k - 1));
+        zs(i, j, k) += rho_on_w_face * g[2];
+    });
+}
+
+// erf_slow_rhs_pre (Source/TimeIntegration/ERF_slow_rhs_pre.cpp:67-774), flat,
+// dry, no terrain: p', Omega, AdvectionSrcForRho (ERF_AdvectionSrcForState.cpp:29-93),
+// rho*theta advection and diffusion, momentum advection and diffusion,
+// pressure gradient plus sources.
+void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_, StateA scratch,
+                         Arr u, Arr v, Arr w) {
+    (void)nrk;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const bool use_diff = cfg.molec_diff_type != ERFX_MOLEC_NONE || cfg.les_type != ERFX_LES_NONE;
+    if (use_diff) make_tau_terms(data, u, v, w);
+    Arr rwr = rhs.zmom;
+    // S_rhs[zmom].setVal(0.0) (:232)
+    {
+        double* p = rwr.p;
+        std::fill(p, p + L.nelem, 0.0);
+    }
+    Arr c = data.cons;
+    Box bx(0, 0, 0, nx - 1, ny - 1, nz - 1);
+    // p' on the box grown by one, clipped at k >= 0 (:337-358)
+    Field fpp; fpp.alloc(L, 1);
+    Arr pp = fpp.a(L), p0a = A(p0);
+    int bad_theta = 0;
+    loop(Box(-1, -1, 0, nx, ny, nz), [&](int i, int j, int k) {
+        if (c(i, j, k, 1) <= 0.) {
+#pragma omp critical
+            { if (!bad_theta) { bad_theta = 1; std::fprintf(stderr, "BAD THETA AT %d %d %d %e\n", i, j, k, c(i, j, k, 1)); } }
+        }
+        double qv_for_p = 0.0;
+        pp(i, j, k) = getPgivenRTh(c(i, j, k, 1), qv_for_p) - p0a(i, j, k);
+    });
+    if (bad_theta) throw std::runtime_error("Bad theta in slow_rhs_pre");
+    // Omega = rho*w on z-faces grown by one (:410-414)
+    Arr om = A(omega);
+    loop(Box(-1, -1, -1, nx, ny, nz + 1), [&](int i, int j, int k) { om(i, j, k) = data.zmom(i, j, k); });
+    // AdvectionSrcForRho
+    Arr ax = scratch.xmom, ay = scratch.ymom, az = scratch.zmom;
+    const double dxi = dxInv[0], dyi = dxInv[1], dzi = dxInv[2];
+    Field ffx, ffy, ffz; ffx.alloc(L, 1); ffy.alloc(L, 1); ffz.alloc(L, 1);
+    Arr fx = ffx.a(L), fy = ffy.a(L), fz = ffz.a(L);
+    loop(Box(0, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) {
+        fx(i, j, k) = 1.0 * data.xmom(i, j, k) / 1.0;
+        ax(i, j, k) = fx(i, j, k);
+    });
+    loop(Box(0, 0, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) {
+        fy(i, j, k) = 1.0 * data.ymom(i, j, k) / 1.0;
+        ay(i, j, k) = fy(i, j, k);
+    });
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz), [&](int i, int j, int k) {
+        double mfsq = 1.0 * 1.0;
+        fz(i, j, k) = 1.0 * om(i, j, k) / mfsq;
+        az(i, j, k) = fz(i, j, k);
+    });
+    Arr cr = rhs.cons;
+    loop(bx, [&](int i, int j, int k) {
+        double mfsq = 1.0 * 1.0;
+        cr(i, j, k, 0) = -mfsq / 1.0 * ((fx(i + 1, j, k) - fx(i, j, k)) * dxi +
+                                        (fy(i, j + 1, k) - fy(i, j, k)) * dyi +
+                                        (fz(i, j, k + 1) - fz(i, j, k)) * dzi);
+    });
+    advection_src_scalars(bx, 1, 1, ax, ay, az, prim_, cr, cfg.dycore_horiz_adv, cfg.dycore_vert_adv,
+                          cfg.dycore_horiz_upw_frac, cfg.dycore_vert_upw_frac);
+    if (use_diff) diffusion_src_state(bx, 1, 1, c, prim_, cr);
+    Arr cs = A(cc_src);
+    loop(bx, [&](int i, int j, int k) {
+        cr(i, j, k, 0) += cs(i, j, k, 0);
+        cr(i, j, k, 1) += cs(i, j, k, 1);
+    });
+    // momentum
+    Box tbx(0, 0, 0, nx, ny - 1, nz - 1), tby(0, 0, 0, nx - 1, ny, nz - 1), tbz(0, 0, 1, nx - 1, ny - 1, nz - 1);
+    Arr rur = rhs.xmom, rvr = rhs.ymom;
+    advection_src_mom(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
+    if (use_diff) {
+        Arr t11 = A(tau11), t22 = A(tau22), t33 = A(tau33), t12 = A(tau12), t13 = A(tau13), t23 = A(tau23);
+        loop(tbx, [&](int i, int j, int k) {
+            double mf = 1.0;
+            rur(i, j, k) -= ((t11(i, j, k) - t11(i - 1, j, k)) * dxi * mf
+                           + (t12(i, j + 1, k) - t12(i, j, k)) * dyi * mf
+                           + (t13(i, j, k + 1) - t13(i, j, k)) * dzi);
+        });
+        loop(tby, [&](int i, int j, int k) {
+            double mf = 1.0;
+            rvr(i, j, k) -= ((t12(i + 1, j, k) - t12(i, j, k)) * dxi * mf
+                           + (t22(i, j, k) - t22(i, j - 1, k)) * dyi * mf
+                           + (t23(i, j, k + 1) - t23(i, j, k)) * dzi);
+        });
+        loop(tbz, [&](int i, int j, int k) {
+            double mf = 1.0;
+            rwr(i, j, k) -= ((t13(i + 1, j, k) - t13(i, j, k)) * dxi * mf
+                           + (t23(i, j + 1, k) - t23(i, j, k)) * dyi * mf
+                           + (t33(i, j, k) - t33(i, j, k - 1)) * dzi);
+        });
+    }
+    Arr xs = A(xmom_src), ys = A(ymom_src), zs = A(zmom_src);
+    const double apg0 = 0.0, apg1 = 0.0, apg2 = 0.0;
+    loop(tbx, [&](int i, int j, int k) {
+        double gp_xi = dxi * (pp(i, j, k) - pp(i - 1, j, k));
+        double gpx = gp_xi;
+        gpx *= 1.0;
+        double q = 0.0;
+        rur(i, j, k) += (-gpx - apg0) / (1.0 + q) + xs(i, j, k);
+    });
+    loop(tby, [&](int i, int j, int k) {
+        double gp_eta = dyi * (pp(i, j, k) - pp(i, j - 1, k));
+        double gpy = gp_eta;
+        gpy *= 1.0;
+        double q = 0.0;
+        rvr(i, j, k) += (-gpy - apg1) / (1.0 + q) + ys(i, j, k);
+    });
+    // zero the rhs on Dirichlet lateral faces (:697-726)
+    if (bck[0][0] == EXT_DIR) loop(Box(0, 0, 0, 0, ny - 1, nz - 1), [&](int i, int j, int k) { rur(i, j, k) = 0.; });
+    if (bck[0][3] == EXT_DIR) loop(Box(nx, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) { rur(i, j, k) = 0.; });
+    if (bck[1][1] == EXT_DIR) loop(Box(0, 0, 0, nx - 1, 0, nz - 1), [&](int i, int j, int k) { rvr(i, j, k) = 0.; });
+    if (bck[1][4] == EXT_DIR) loop(Box(0, ny, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) { rvr(i, j, k) = 0.; });
+    loop(tbz, [&](int i, int j, int k) {
+        double met_h_zeta = 1;
+        double gpz = dzi * (pp(i, j, k) - pp(i, j, k - 1)) / met_h_zeta;
+        double q = 0.0;
+        rwr(i, j, k) += (zs(i, j, k) - gpz - apg2) / (1.0 + q);
+    });
+    (void)dt;
+}
+
+// erf_slow_rhs_post (Source/TimeIntegration/ERF_slow_rhs_post.cpp:49-559),
+// dry: RhoScalar is the only slow variable advanced (:161-166).
+void Level::slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA snew, StateA data,
+                          Arr prim_, StateA scratch, Arr u, Arr v) {
+    (void)nrk; (void)u; (void)v;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const bool use_diff = cfg.molec_diff_type != ERFX_MOLEC_NONE || cfg.les_type != ERFX_LES_NONE;
+    Box tbx(0, 0, 0, nx - 1, ny - 1, nz - 1);
+    Arr cur = data.cons, ncons_ = snew.cons, oc = old.cons, cr = rhs.cons;
+    int nsv = ncons - 2;
+    loop(tbx, [&](int i, int j, int k) {
+        for (int nn = 0; nn < nsv; ++nn) cur(i, j, k, 2 + nn) = ncons_(i, j, k, 2 + nn);
+    });
+    const int ivar = ERFX_RHOSCALAR;
+    advection_src_scalars(tbx, ivar, 1, scratch.xmom, scratch.ymom, scratch.zmom, prim_, cr,
+                          cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv, cfg.dryscal_horiz_upw_frac,
+                          cfg.dryscal_vert_upw_frac);
+    if (use_diff) diffusion_src_state(tbx, ivar, 1, ncons_, prim_, cr);
+    Arr cs = A(cc_src);
+    loop(tbx, [&](int i, int j, int k) {
+        cr(i, j, k, ivar) += cs(i, j, k, ivar);
+        cur(i, j, k, ivar) = oc(i, j, k, ivar) + dt * cr(i, j, k, ivar);
+    });
+    loop(tbx, [&](int i, int j, int k) {
+        for (int n = 0; n < ncons; ++n) ncons_(i, j, k, n) = cur(i, j, k, n);
+    });
+    loop(Box(0, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) { snew.xmom(i, j, k) = data.xmom(i, j, k); });
+    loop(Box(0, 0, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) { snew.ymom(i, j, k) = data.ymom(i, j, k); });
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz), [&](int i, int j, int k) { snew.zmom(i, j, k) = data.zmom(i, j, k); });
+}
+
+}  // namespace orc

@@ -1,0 +1,77 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so) -- test infrastructure."""
+import ctypes
+import os
+import numpy as np
+
+from erf_amd import config as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        l = ctypes.CDLL(LIB_PATH)
+        l.orc_create.restype = ctypes.c_void_p
+        l.orc_create.argtypes = [ctypes.POINTER(C.Config), ctypes.POINTER(C.Layout)]
+        l.orc_destroy.argtypes = [ctypes.c_void_p]
+        l.orc_field_size.restype = ctypes.c_long
+        l.orc_field_size.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for n in ("orc_set", "orc_get"):
+            getattr(l, n).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        l.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        l.orc_last_error.restype = ctypes.c_char_p
+        l.orc_cons_to_prim.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        l.orc_apply_bcs.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+        l.orc_make_fast_coeffs.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        l.orc_fast_rhs_N.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        l.orc_turbulent_viscosity.argtypes = [ctypes.c_void_p]
+        l.orc_slow_rhs_pre.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        l.orc_slow_rhs_post.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        _lib = l
+    return _lib
+
+
+class Oracle:
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = C.layout_for(cfg)
+        self.h = lib().orc_create(ctypes.byref(cfg), ctypes.byref(self.L))
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def set(self, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        n = lib().orc_field_size(self.h, fid)
+        assert a.size == n, (fid, a.size, n)
+        self._chk(lib().orc_set(self.h, fid, a.ctypes.data))
+
+    def get(self, fid):
+        nc = C.ncomp_of(self.cfg, fid)
+        a = np.empty((nc,) + self.L.shape3())
+        self._chk(lib().orc_get(self.h, fid, a.ctypes.data))
+        return a
+
+    def load(self, fields):
+        for fid, a in fields.items():
+            self.set(fid, a)
+
+    def step(self, t0, dt, n):
+        self._chk(lib().orc_step(self.h, t0, dt, n))
+
+    def call(self, name, *args):
+        self._chk(getattr(lib(), name)(self.h, *args))
