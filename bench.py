@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X dycore step (one GPU).
+
+Workload: the dry convective-boundary-layer LES, 256 x 256 x 128 cells
+(dx = dy = 20 m, dz = 10 m), 5th-order upwind advection, Smagorinsky Cs = 0.1,
+Coriolis + geostrophic forcing, no-slip bottom with surface heat flux, FP64,
+dt = 0.5 s, MRI ratio 16 (25 acoustic substeps per RK3 step).  Synthetic
+seeded initial state.  A step is one full RK3 step (slow RHS, acoustic
+substeps, boundary fills) with the state resident in HBM.
+
+Prints one JSON line: cell-updates/s per RK3 step, the roofline of the
+dominant kernel (HBM bytes per launch / average launch time, HIP events on the
+library's stream) and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(nx, ny, nz, budget_s):
+    """Time the CPU oracle (test infrastructure, OpenMP) on a bounded sample of
+    the same workload: a 64 x 64 x nz column block of the CBL case, as many
+    steps as fit in ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from erf_amd import problems as P
+    from oracle_lib import Oracle
+    sx = min(nx, 64)
+    sy = min(ny, 64)
+    cfg, f = P.cbl(sx, sy, nz, seed=0)
+    orc = Oracle(cfg)
+    orc.load(f)
+    orc.step(0.0, 0.5, 1)                     # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.step(0.5 * (n + 1), 0.5, 1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": sx * sy * nz * n / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d steps of the CBL case on a %dx%dx%d block (OpenMP C++ oracle, -O2)" % (n, sx, sy, nz)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the x-y domain decomposition is a design note, not code)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from erf_amd import config as C, problems as P
+    from erf_amd.lib import Sim, lib, check
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    cfg, f = P.cbl(a.nx, a.ny, a.nz, seed=0)
+    sim = Sim(cfg)
+    sim.load(f)
+    dt = 0.5
+    stream = torch.cuda.ExternalStream(sim.stream)
+
+    sim.step(0.0, dt, a.warmup)
+    t = a.warmup * dt
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    check(lib().erfx_sim_timer_reset(sim.h))
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    wall0 = time.perf_counter()
+    sim.step_async(t, dt, a.steps)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    sim.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = e0.elapsed_time(e1)
+
+    cells = cfg.nx * cfg.ny * cfg.nz
+    nsub = 1 + cfg.mri_ratio // 2 + cfg.mri_ratio
+    value = cells * a.steps / (ms * 1e-3)
+
+    # dominant kernel: per-launch timing collected by the library with HIP events
+    kname = ctypes.create_string_buffer(128)
+    kms = ctypes.c_double()
+    kcnt = ctypes.c_long()
+    kbytes = ctypes.c_double()
+    check(lib().erfx_sim_timer_read(sim.h, kname, 128, ctypes.byref(kms), ctypes.byref(kcnt), ctypes.byref(kbytes)))
+    avg_ms = kms.value / max(kcnt.value, 1)
+    achieved = kbytes.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    # sanity: the state stayed physical
+    rho = sim.download(C.F_CONS)[0]
+    ng = sim.L.ng
+    rv = rho[ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
+    assert np.all(np.isfinite(rv)) and rv.min() > 0.5, "state blew up"
+    sim.close()
+
+    out = {
+        "metric": "cell-updates/s per RK3 step (FP64 GPU path)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, "
+                               "dt=0.5 s, MRI ratio %d (%d substeps/step)" % (cfg.nx, cfg.ny, cfg.nz,
+                                                                             cfg.mri_ratio, nsub),
+                   "cells": cells, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": kname.value.decode(), "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": kcnt.value},
+        "wall_s": wall,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg.nx, cfg.ny, cfg.nz, a.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

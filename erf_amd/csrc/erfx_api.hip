@@ -3,6 +3,7 @@
 #include "erfx_common.h"
 #include "erfx_internal.h"
 
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -105,7 +106,14 @@ struct erfx_sim {
     erfx_state S[5];                          // S_old, S_new, S_sum, S_scratch, F_slow
     double *prim = nullptr, *pi_stage = nullptr, *fast_coeffs = nullptr, *eddy = nullptr;
     std::vector<void*> allocs;
+    KTimer timer;
 };
+
+static void timer_free(KTimer& t) {
+    for (int i = 0; i < t.cap; ++i) (void)hipEventDestroy(t.ev[i]);
+    delete[] t.ev;
+    t = KTimer();
+}
 
 extern "C" {
 
@@ -332,6 +340,7 @@ int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
 int erfx_sim_destroy(erfx_sim* s) {
     if (!s) return ERFX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    timer_free(s->timer);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->ctx) erfx_ctx_destroy(s->ctx);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -496,6 +505,41 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
 int erfx_sim_sync(erfx_sim* s) {
     if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
     ERFX_HIP(hipStreamSynchronize(s->stream));
+    return ERFX_OK;
+}
+
+// Times the acoustic column solve, the step's dominant kernel, on up to
+// 4096 launches after the reset.
+int erfx_sim_timer_reset(erfx_sim* s) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    timer_free(s->timer);
+    KTimer& t = s->timer;
+    t.ev = new hipEvent_t[8192];
+    for (int i = 0; i < 8192; ++i) {
+        hipError_t e = hipEventCreate(&t.ev[i]);
+        if (e != hipSuccess) { timer_free(t); return hip_fail(e, "hipEventCreate"); }
+        t.cap = i + 1;
+    }
+    t.kernel = KT_FAST_COLUMN;
+    s->ctx->P.timer = &t;
+    return ERFX_OK;
+}
+
+int erfx_sim_timer_read(erfx_sim* s, char* name, int name_len, double* total_ms, long* launches, double* bytes) {
+    if (!s || !name || name_len < 1 || !total_ms || !launches || !bytes) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    const KTimer& t = s->timer;
+    double sum = 0;
+    for (int i = 0; i + 1 < t.used; i += 2) {
+        float ms = 0;
+        ERFX_HIP(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+        sum += ms;
+    }
+    std::snprintf(name, (size_t)name_len, "%s", t.name);
+    *total_ms = sum;
+    *launches = t.used / 2;
+    *bytes = t.bytes;
     return ERFX_OK;
 }
 

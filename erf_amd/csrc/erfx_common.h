@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../include/erfx.h"
+#include "erfx_pow.h"
 
 namespace erfx {
 
@@ -27,10 +28,10 @@ constexpr double ip_0 = 1. / p_0;
 
 // ---- EOS (Source/Utils/ERF_EOS.H:89-93, 130-135), dry ---------------------
 __device__ __forceinline__ double eos_p_of_rt(double rt) {
-    return p_0 * pow(R_d * rt * (1.0 + (R_v / R_d) * 0.0) * ip_0, Gamma);
+    return p_0 * erfx_pow(R_d * rt * (1.0 + (R_v / R_d) * 0.0) * ip_0, Gamma);
 }
 __device__ __forceinline__ double eos_exner_of_rt(double rt, double rdOcp) {
-    return pow(R_d * rt * (1.0 + R_v / R_d * 0.0) * ip_0, Gamma * rdOcp);
+    return erfx_pow(R_d * rt * (1.0 + R_v / R_d * 0.0) * ip_0, Gamma * rdOcp);
 }
 
 // ---- index space ------------------------------------------------------------
@@ -93,6 +94,29 @@ inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) 
 // ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
 enum BCK { INT_DIR = 0, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_ODD, NEUMANN_INT };
 
+// Per-launch timing of one chosen kernel with HIP events recorded on the
+// kernel's own stream around each of its launches (host side only).
+enum { KT_FAST_COLUMN = 0 };
+struct KTimer {
+    int kernel = -1;                 // KT_* id being timed, -1 = off
+    hipEvent_t* ev = nullptr;        // start/stop pairs
+    int cap = 0;                     // events available
+    int used = 0;                    // events recorded
+    double bytes = 0;                // algorithmic HBM bytes per launch
+    const char* name = "";
+};
+inline void kt_start(KTimer* t, int id, hipStream_t s) {
+    if (t && t->kernel == id && t->used + 2 <= t->cap) (void)hipEventRecord(t->ev[t->used], s);
+}
+inline void kt_stop(KTimer* t, int id, hipStream_t s, double bytes, const char* name) {
+    if (t && t->kernel == id && t->used + 2 <= t->cap) {
+        (void)hipEventRecord(t->ev[t->used + 1], s);
+        t->used += 2;
+        t->bytes = bytes;
+        t->name = name;
+    }
+}
+
 // Per-run constants derived from erfx_config.
 struct Params {
     erfx_config cfg;
@@ -104,6 +128,7 @@ struct Params {
     double vel_extdir[3][6];
     bool periodic[3];
     bool use_diff, use_turb, consA;
+    KTimer* timer = nullptr;  // optional launch timing (host pointer, never read on the device)
 };
 
 void make_params(const erfx_config& cfg, const erfx_layout& lay, Params& P);

@@ -162,13 +162,16 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     const double* sr1 = slow_rhs.cons + ne;
     const double* pr0 = prev.cons;
     const double* st0 = stage.cons;
+    kt_start(P.timer, KT_FAST_COLUMN, s);
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // bottom row
         double rhs0 = prev.zmom[q0] - stage.zmom[q0] + dtau * slow_rhs.zmom[q0];
+        // every RHS reads prev.zmom (== data.zmom for step > 0), so no rho*w is
+        // written before the forward sweep is complete
         double sol_prev = rhs0 * iB[q0];
         soln[q0] = sol_prev;
-        data.zmom[q0] = stage.zmom[q0] + sol_prev;
+        const double w0 = stage.zmom[q0] + sol_prev;
         for (int k = 1; k <= nz; ++k) {
             const long q = q0 + k * sz;
             double RHS;
@@ -201,6 +204,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
             soln[q] = sol;
             sol_prev = sol;
         }
+        data.zmom[q0] = w0;
         if (implicit) {
             const long qt = q0 + nz * sz;
             data.zmom[qt] = stage.zmom[qt] + sol_prev;
@@ -221,6 +225,10 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
             }
         }
     });
+    // algorithmic traffic per z-face: 17 fields read once (A, 1/B, C, P, Q, prim,
+    // prev/stage zmom, prev/stage rho and rho*theta, slow rho, rho*theta and zmom,
+    // the two horizontal divergences), soln and the new zmom written once
+    kt_stop(P.timer, KT_FAST_COLUMN, s, 19.0 * 8.0 * nx * ny * (nz + 1), "fast_rhs_N column solve");
 
     // K5 (:518-536 and :566-592)
     double* cc0 = data.cons;

@@ -155,7 +155,7 @@ void turbulent_viscosity(const Params& P, const SlowWork& W, const double* cons,
         double SmnSmn = s11bar * s11bar + s22bar * s22bar + s33bar * s33bar + 2.0 * s12bar * s12bar +
                         2.0 * s13bar * s13bar + 2.0 * s23bar * s23bar;
         double cellVolMsf = 1.0 / (dxi * 1.0 * dyi * 1.0 * dzi);
-        double DeltaMsf = pow(cellVolMsf, 1.0 / 3.0);
+        double DeltaMsf = erfx_pow(cellVolMsf, 1.0 / 3.0);
         double CsDeltaSqrMsf = Cs * Cs * DeltaMsf * DeltaMsf;
         double m = CsDeltaSqrMsf * cons[q] * sqrt(2.0 * SmnSmn);
         mu_h[q] = m;

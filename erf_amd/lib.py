@@ -56,6 +56,9 @@ SIGNATURES = {
     "erfx_sim_step_async": (_I, [_P, _D, _D, _I]),
     "erfx_sim_sync": (_I, [_P]),
     "erfx_sim_stream": (_P, [_P]),
+    "erfx_sim_timer_reset": (_I, [_P]),
+    "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
+                                 ctypes.POINTER(_D)]),
 }
 
 

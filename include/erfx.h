@@ -249,6 +249,13 @@ int  erfx_sim_step_async(erfx_sim* sim, double t0, double dt, int nsteps);
 int  erfx_sim_sync(erfx_sim* sim);
 /* The sim's stream (a hipStream_t) for event timing. */
 void* erfx_sim_stream(erfx_sim* sim);
+/* Per-launch timing of the step's dominant kernel with HIP events on the sim's
+ * stream (the role of ERF's BL_PROFILE regions): reset, run steps, then read
+ * the kernel name, summed milliseconds, launch count and algorithmic HBM bytes
+ * per launch. */
+int  erfx_sim_timer_reset(erfx_sim* sim);
+int  erfx_sim_timer_read(erfx_sim* sim, char* name, int name_len, double* total_ms, long* launches,
+                         double* bytes_per_launch);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,11 @@ extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
 
+// out[i] = x[i]**y[i] with the EOS's pow (for the accuracy test against libm).
+void orc_pow_n(const double* x, const double* y, double* out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = erfx_pow(x[i], y[i]);
+}
+
 void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
     auto* L = new Level();
     try { L->init(*cfg, *lay); }

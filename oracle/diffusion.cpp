@@ -155,7 +155,7 @@ void Level::turbulent_viscosity(Arr c, Arr u, Arr v, Arr w) {
         double SmnSmn = smn_smn(i, j, k, t11, t22, t33, t12, t13, t23);
         double dxI = dxInv[0], dyI = dxInv[1], dzI = dxInv[2];
         double cellVolMsf = 1.0 / (dxI * 1.0 * dyI * 1.0 * dzI);
-        double DeltaMsf = std::pow(cellVolMsf, 1.0 / 3.0);
+        double DeltaMsf = erfx_pow(cellVolMsf, 1.0 / 3.0);
         double CsDeltaSqrMsf = Cs * Cs * DeltaMsf * DeltaMsf;
         mu(i, j, k, EddyDiff::Mom_h) = CsDeltaSqrMsf * c(i, j, k, 0) * std::sqrt(2.0 * SmnSmn);
         mu(i, j, k, EddyDiff::Mom_v) = mu(i, j, k, EddyDiff::Mom_h);

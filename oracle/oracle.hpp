@@ -20,6 +20,10 @@
 #include <stdexcept>
 
 #include "../include/erfx.h"
+// The library's reproducible pow (correctly rounded to within ~0.5 ulp, checked
+// against the C library's pow by tests/test_oracle_cpu.py) stands in for
+// std::pow, so that the oracle and the GPU evaluate the EOS bit for bit alike.
+#include "../erf_amd/csrc/erfx_pow.h"
 
 namespace orc {
 
@@ -38,20 +42,20 @@ constexpr double iGamma   = 1. / Gamma;
 
 // ---- EOS (Source/Utils/ERF_EOS.H) -------------------------------------------
 inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_EOS.H:89-93
-    return p_0 * std::pow(R_d * rhotheta * (1.0 + (R_v / R_d) * qv) * ip_0, Gamma);
+    return p_0 * erfx_pow(R_d * rhotheta * (1.0 + (R_v / R_d) * qv) * ip_0, Gamma);
 }
 inline double getExnergivenRTh(double rhotheta, double rdOcp, double qv = 0.0) { // ERF_EOS.H:130-135
-    return std::pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma * rdOcp);
+    return erfx_pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma * rdOcp);
 }
 inline double getRhoThetagivenP(double p, double qv = 0.0) {              // ERF_EOS.H:144-149
-    return std::pow(p * std::pow(p_0, Gamma - 1), iGamma) * iR_d / (1.0 + R_v / R_d * qv);
+    return erfx_pow(p * erfx_pow(p_0, Gamma - 1), iGamma) * iR_d / (1.0 + R_v / R_d * qv);
 }
 inline double getRhogivenThetaPress(double theta, double p, double rdOcp, double qv = 0.0) {
-    return std::pow(p_0, rdOcp) * std::pow(p, iGamma) / (R_d * theta * (1.0 + R_v / R_d * qv));
+    return erfx_pow(p_0, rdOcp) * erfx_pow(p, iGamma) / (R_d * theta * (1.0 + R_v / R_d * qv));
 }
 inline double getdPdRgivenConstantTheta(double rho, double theta, double qv = 0.0) {
-    return Gamma * p_0 * std::pow((R_d * theta * (1.0 + R_v / R_d * qv) * ip_0), Gamma) *
-           std::pow(rho, Gamma - 1.0);
+    return Gamma * p_0 * erfx_pow((R_d * theta * (1.0 + R_v / R_d * qv) * ip_0), Gamma) *
+           erfx_pow(rho, Gamma - 1.0);
 }
 
 // ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
