@@ -35,7 +35,22 @@ def parse():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB (3 launches), the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counter readings")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 27                                   # 2^27 doubles = 1 GiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "launches": 3}))
 
 
 def cpu_baseline(nx, ny, nz, budget_s):
@@ -70,6 +85,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the x-y domain decomposition is a design note, not code)", file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     import numpy as np
     import torch
     from erf_amd import config as C, problems as P

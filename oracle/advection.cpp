@@ -45,6 +45,11 @@ static inline double interp_line(int type, double sp2, double sp1, double s, dou
     throw std::runtime_error("bad adv type");
 }
 
+double face_interp(int type, double sp2, double sp1, double s, double sm1, double sm2, double sm3,
+                   double upw_in, double upw_frac) {
+    return interp_line(type, sp2, sp1, s, sm1, sm2, sm3, upw_in, upw_frac);
+}
+
 static inline bool wide(int t) { return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6; }
 static inline bool narrow(int t) { return t == ERFX_ADV_CENTERED2; }
 

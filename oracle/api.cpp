@@ -54,6 +54,19 @@ void orc_pow_n(const double* x, const double* y, double* out, long n) {
     for (long i = 0; i < n; ++i) out[i] = erfx_pow(x[i], y[i]);
 }
 
+// ERF's column solve on faces 0..n: factor (A, B, C) as make_fast_coeffs does,
+// then the Thomas sweep of erf_fast_rhs_N.  B is overwritten with the factors.
+void orc_column_solve(int n, const double* A, double* B, const double* C, const double* rhs, double* soln) {
+    column_factor(n, A, B, C, 1);
+    column_solve(n, A, B, C, rhs, soln, 1);
+}
+
+// Face value of the centred/upwind interpolation family from the six cell
+// values s[0..5] at i-3..i+2 (lo face of cell i), advecting velocity sign `upw`.
+double orc_face_interp(int type, const double* s, double upw, double upw_frac) {
+    return face_interp(type, s[5], s[4], s[3], s[2], s[1], s[0], upw, upw_frac);
+}
+
 void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
     auto* L = new Level();
     try { L->init(*cfg, *lay); }

@@ -40,16 +40,32 @@ void Level::make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double 
         fc(i, j, 0, 0) = 0.0; fc(i, j, 0, 1) = 1.0; fc(i, j, 0, 2) = 0.0;
         fc(i, j, nz, 0) = 0.0; fc(i, j, nz, 1) = 1.0; fc(i, j, nz, 2) = 0.0;
         if (top_outflow) fc(i, j, nz, 0) = -1.0;
-        double bet = fc(i, j, 0, 1);
-        for (int k = 1; k <= nz; k++) {
-            double gam = fc(i, j, k - 1, 2) / bet;
-            bet = fc(i, j, k, 1) - fc(i, j, k, 0) * gam;
-            fc(i, j, k, 1) = bet;
-        }
+        column_factor(nz, &fc(i, j, 0, 0), &fc(i, j, 0, 1), &fc(i, j, 0, 2), L.sz);
     });
-    loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
-        fc(i, j, k, 1) = 1.0 / fc(i, j, k, 1);
-    });
+}
+
+// Column recurrence of make_fast_coeffs, GPU branch (ERF_make_fast_coeffs.cpp:214-242)
+// followed by the inversion (:287-290): on faces 0..n with stride `st`, B[k]
+// becomes bet_k and then 1/bet_k on the interior faces 1..n-1; B[0] and B[n]
+// keep bet_0 and bet_n (ERF does not invert them).
+void column_factor(int n, const double* A, double* B, const double* C, long st) {
+    double bet = B[0];
+    for (int k = 1; k <= n; k++) {
+        double gam = C[(k - 1) * st] / bet;
+        bet = B[k * st] - A[k * st] * gam;
+        B[k * st] = bet;
+    }
+    for (int k = 1; k <= n - 1; k++) B[k * st] = 1.0 / B[k * st];
+}
+
+// Thomas sweep of erf_fast_rhs_N, GPU branch (ERF_fast_rhs_N.cpp:435-453), on the
+// factors column_factor leaves: forward soln_k = (rhs_k - A_k soln_{k-1}) iB_k,
+// backward soln_k -= (C_k iB_k) soln_{k+1}.
+void column_solve(int n, const double* A, const double* iB, const double* C, const double* rhs,
+                  double* soln, long st) {
+    soln[0] = rhs[0] * iB[0];
+    for (int k = 1; k <= n; k++) soln[k * st] = (rhs[k * st] - A[k * st] * soln[(k - 1) * st]) * iB[k * st];
+    for (int k = n - 1; k >= 0; k--) soln[k * st] -= (C[k * st] * iB[k * st]) * soln[(k + 1) * st];
 }
 
 // erf_fast_rhs_N (Source/TimeIntegration/ERF_fast_rhs_N.cpp:36-614), GPU
@@ -164,15 +180,8 @@ void Level::fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA s
     Arr cz = data.zmom;
     if (implicit) {
         loop(Box(0, 0, 0, nx - 1, ny - 1, 0), [&](int i, int j, int) {
-            soln(i, j, 0) = RHS(i, j, 0) * fc(i, j, 0, 1);
-            cz(i, j, 0) = stage.zmom(i, j, 0) + soln(i, j, 0);
-            for (int k = 1; k <= nz; k++)
-                soln(i, j, k) = (RHS(i, j, k) - fc(i, j, k, 0) * soln(i, j, k - 1)) * fc(i, j, k, 1);
-            cz(i, j, nz) = stage.zmom(i, j, nz) + soln(i, j, nz);
-            for (int k = nz - 1; k >= 0; k--) {
-                soln(i, j, k) -= (fc(i, j, k, 2) * fc(i, j, k, 1)) * soln(i, j, k + 1);
-                cz(i, j, k) = stage.zmom(i, j, k) + soln(i, j, k);
-            }
+            column_solve(nz, &fc(i, j, 0, 0), &fc(i, j, 0, 1), &fc(i, j, 0, 2), &RHS(i, j, 0), &soln(i, j, 0), L.sz);
+            for (int k = 0; k <= nz; k++) cz(i, j, k) = stage.zmom(i, j, k) + soln(i, j, k);
         });
     } else {
         loop(Box(0, 0, 0, nx - 1, ny - 1, 0), [&](int i, int j, int) {

@@ -58,6 +58,15 @@ inline double getdPdRgivenConstantTheta(double rho, double theta, double qv = 0.
            erfx_pow(rho, Gamma - 1.0);
 }
 
+// ---- column tridiagonal factor / solve (fast.cpp) ----------------------------
+void column_factor(int n, const double* A, double* B, const double* C, long st);
+void column_solve(int n, const double* A, const double* iB, const double* C, const double* rhs,
+                  double* soln, long st);
+
+// face value of the centred/upwind family (advection.cpp)
+double face_interp(int type, double sp2, double sp1, double s, double sm1, double sm2, double sm3,
+                   double upw_in, double upw_frac);
+
 // ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
 enum BCK { INT_DIR = 0, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_ODD, NEUMANN_INT };
 

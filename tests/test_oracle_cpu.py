@@ -1,0 +1,220 @@
+"""CPU checks that pin the oracle (test infrastructure) without a GPU:
+closed forms, library routines, exactness on polynomials and physical
+invariants, each chosen so that a plausible mistake (a wrong weight, sign or
+index, a dropped term) fails."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from erf_amd import config as C, problems as P
+from helpers import valid
+from oracle_lib import Oracle, lib as olib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def libm_pow(x, y):
+    """The C library's pow (math.pow calls it; numpy's vectorised power does not)."""
+    y = np.broadcast_to(y, np.shape(x))
+    return np.array([math.pow(a, b) for a, b in zip(np.ravel(x), np.ravel(y))])
+
+
+def orc_pow(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    f = olib().orc_pow_n
+    f.argtypes = [_dp, _dp, _dp, ctypes.c_long]
+    f(_ptr(x), _ptr(y), _ptr(out), x.size)
+    return out
+
+
+# ---- pow used by the EOS -------------------------------------------------------
+
+@pytest.mark.parametrize("y", [1.4, 0.4 / 1.4, (287.0 / 1004.5) * 1.4, 1.0 / 3.0, 1.0 / 1.4, 0.4, -1.7, 2.5])
+def test_pow_against_libm(y):
+    """Within 1 ulp of the C library's pow everywhere, identical on >= 99.5%."""
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0.2, 5.0, 200000)
+    a = orc_pow(x, y)
+    b = libm_pow(x, y)
+    ulps = np.abs(a.view(np.int64) - b.view(np.int64))
+    assert ulps.max() <= 1
+    assert (ulps == 0).mean() >= 0.995
+
+
+def test_pow_wide_range_and_exact_cases():
+    rng = np.random.default_rng(1)
+    x = np.exp2(rng.uniform(-300, 300, 200000))
+    y = rng.uniform(-2, 2, 200000)
+    a = orc_pow(x, y)
+    b = libm_pow(x, y)
+    assert np.abs(a.view(np.int64) - b.view(np.int64)).max() <= 1
+    ex = orc_pow(np.array([4.0, 2.0, 9.0, 1.0, 0.0, 1e-310]), np.array([0.5, 10.0, 0.5, 3.7, 1.4, 0.5]))
+    assert ex[0] == 2.0 and ex[1] == 1024.0 and ex[2] == 3.0 and ex[3] == 1.0 and ex[4] == 0.0
+    assert abs(ex[5] / math.sqrt(1e-310) - 1) < 1e-15
+    assert np.isnan(orc_pow(np.array([-1.0]), 1.4))[0]
+
+
+def test_eos_round_trip():
+    """p(rho*theta(p)) = p (ERF_EOS.H:89-93 and :144-149) through the EOS pow."""
+    p = np.linspace(2.0e4, 1.1e5, 2001)
+    g = C.Gamma
+    rt = orc_pow(p * orc_pow(np.array([C.p_0]), g - 1)[0], 1.0 / g) * (1.0 / C.R_d)
+    p2 = C.p_0 * orc_pow(C.R_d * rt * (1.0 / C.p_0), g)
+    assert np.max(np.abs(p2 / p - 1.0)) < 1e-14
+
+
+# ---- column tridiagonal solve -----------------------------------------------------
+
+def _column_solve(A, B, C_, rhs):
+    n = len(A) - 1
+    f = olib().orc_column_solve
+    f.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    Bw = np.array(B, dtype=np.float64)
+    C_ = np.ascontiguousarray(C_, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    out = np.empty(n + 1)
+    f(n, _ptr(A), _ptr(Bw), _ptr(C_), _ptr(rhs), _ptr(out))
+    return out
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (16, 2), (96, 3), (128, 4)])
+def test_column_solve_matches_banded_solver(n, seed):
+    """The factor + Thomas sweep that make_fast_coeffs and erf_fast_rhs_N use
+    solves the wall-bounded system (rows 0 and n are identity rows) exactly as
+    a banded LU solve does."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-0.4, 0.1, n + 1)
+    C_ = rng.uniform(-0.4, 0.1, n + 1)
+    B = 1.0 + rng.uniform(0.5, 1.0, n + 1)
+    A[0] = C_[0] = 0.0
+    B[0] = 1.0
+    A[n] = C_[n] = 0.0
+    B[n] = 1.0
+    rhs = rng.normal(size=n + 1)
+    ab = np.zeros((3, n + 1))
+    ab[0, 1:] = C_[:-1]
+    ab[1, :] = B
+    ab[2, :-1] = A[1:]
+    ref = scipy.linalg.solve_banded((1, 1), ab, rhs)
+    got = _column_solve(A, B, C_, rhs)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * max(1.0, np.abs(ref).max())
+
+
+# ---- face interpolation: exactness on polynomials ------------------------------------
+
+def _face(type_, s, upw, frac=1.0):
+    f = olib().orc_face_interp
+    f.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double]
+    f.restype = ctypes.c_double
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return f(type_, _ptr(s), upw, frac)
+
+
+@pytest.mark.parametrize("type_,degree", [(C.ADV_CENTERED2, 1), (C.ADV_UPWIND3, 2), (C.ADV_CENTERED4, 3),
+                                          (C.ADV_UPWIND5, 4), (C.ADV_CENTERED6, 5)])
+def test_face_interp_polynomial_order(type_, degree):
+    """Cell averages of a polynomial of the scheme's degree reconstruct the exact
+    face value (ERF_Interpolation_UPW.H weights); one degree higher does not."""
+    x0 = 0.3
+    centres = np.arange(-3, 3) + 0.5                 # cells i-3 .. i+2, face at 0
+    for p in range(degree + 2):
+        avg = (((centres + 0.5 + x0) ** (p + 1)) - ((centres - 0.5 + x0) ** (p + 1))) / (p + 1)
+        exact = x0 ** p
+        for upw in (1.0, -1.0):
+            e = abs(_face(type_, avg, upw) - exact)
+            if p <= degree:
+                assert e < 1e-13, (p, upw, e)
+            else:
+                assert e > 1e-6, (p, upw, e)
+
+
+# ---- whole-step invariants on the oracle ---------------------------------------------
+
+def _theta(a):
+    return a[C.RHOTHETA] / a[C.RHO]
+
+
+def test_periodic_mass_conservation():
+    cfg, f = P.rising_thermal(16, 12, 16, seed=2)
+    orc = Oracle(cfg)
+    orc.load(f)
+    r0 = valid(orc.get(C.F_CONS), cfg, orc.L, C.F_CONS)[C.RHO]
+    m0 = math.fsum(r0.ravel())
+    orc.step(0.0, 1.0, 3)
+    r1 = valid(orc.get(C.F_CONS), cfg, orc.L, C.F_CONS)[C.RHO]
+    m1 = math.fsum(r1.ravel())
+    assert abs(m1 - m0) / m0 < 1e-14
+    assert np.abs(r1 - r0).max() > 1e-9          # the state did evolve
+
+
+def _exact_rest_state(cfg):
+    """theta = 300 K at rest with rho, rho*theta, r0, p0 and pi0 consistent to
+    roundoff: iterate r -> p (the discrete HSE recurrence of erf_enforce_hse) ->
+    rho*theta(p) -> r = rho*theta / theta to a fixed point."""
+    L, f = P.empty_fields(cfg)
+    nz, ng, g = cfg.nz, L.ng, C.CONST_GRAV
+    rdOcp = C.R_d / cfg.c_p
+    # level by level (each level's fixed point contracts with factor ~ g dz / c^2)
+    r = np.full(nz + 2, C.p_0 / (C.R_d * 300.0))   # k = -1 .. nz
+    hz = 0.5 * cfg.dz
+    for _ in range(100):
+        r[1] = float(P._rt_of_p(np.array([C.p_0 - hz * r[1] * g]))[0]) / 300.0
+    pk = C.p_0 - hz * r[1] * g
+    for k in range(2, nz + 1):
+        r[k] = r[k - 1]
+        for _ in range(100):
+            pn = pk - cfg.dz * (0.5 * (r[k] + r[k - 1])) * g
+            r[k] = float(P._rt_of_p(np.array([pn]))[0]) / 300.0
+        pk = pk - cfg.dz * (0.5 * (r[k] + r[k - 1])) * g
+    r[0] = r[1]
+    r[nz + 1] = r[nz]
+    p, pi = P.enforce_hse(r, cfg.dz, rdOcp, g)
+    for fid, col in ((C.F_R0, r), (C.F_P0, p), (C.F_PI0, pi)):
+        f[fid][0][ng - 1:ng + nz + 1] = col[:, None, None]
+    rt = P._rt_of_p(f[C.F_P0][0])
+    f[C.F_CONS][C.RHOTHETA][ng:ng + nz] = rt[ng:ng + nz]
+    f[C.F_CONS][C.RHO][ng:ng + nz] = rt[ng:ng + nz] / 300.0
+    return f
+
+
+def test_hydrostatic_rest_stays_at_rest():
+    """An atmosphere in discrete hydrostatic balance at rest stays at rest:
+    buoyancy and the perturbation-pressure gradient cancel to roundoff (a sign
+    or averaging error in either term makes w grow at O(1e-3) m/s per step)."""
+    cfg, _ = P.rising_thermal(12, 10, 16, seed=0)
+    f = _exact_rest_state(cfg)
+    orc = Oracle(cfg)
+    orc.load(f)
+    orc.step(0.0, 1.0, 3)
+    for fid in (C.F_XVEL, C.F_YVEL, C.F_ZVEL):
+        v = valid(orc.get(fid), cfg, orc.L, fid)
+        assert np.abs(v).max() < 1e-11, (fid, np.abs(v).max())
+
+
+def test_mirror_symmetry_of_centred_bubble():
+    """A bubble centred in x with no noise stays mirror-symmetric: theta(i) =
+    theta(nx-1-i) and u(i) = -u(nx-i) to roundoff (a wrong stencil offset or
+    sign breaks it at O(1e-6))."""
+    cfg, f = P.rising_thermal(16, 8, 16, seed=0)
+    f = dict(f)
+    for fid in (C.F_XVEL, C.F_YVEL, C.F_ZVEL):
+        f[fid] = np.zeros_like(f[fid])
+    orc = Oracle(cfg)
+    orc.load(f)
+    orc.step(0.0, 1.0, 3)
+    th = _theta(valid(orc.get(C.F_CONS), cfg, orc.L, C.F_CONS))
+    u = valid(orc.get(C.F_XVEL), cfg, orc.L, C.F_XVEL)[0]
+    assert np.abs(th - th[:, :, ::-1]).max() < 1e-10
+    umax = np.abs(u).max()
+    assert umax > 1e-6
+    assert np.abs(u + u[:, :, ::-1]).max() < 1e-10 * max(umax, 1.0)
