@@ -2,15 +2,13 @@
 // erf_fast_rhs_N as gfx950 kernels.
 //
 // Launch structure per substep:
-//   K1  cells grown by 1 + z-faces: Delta(rho theta), extrapolated theta,
-//       lagged update, Delta(rho w) of the previous substep
+//   K1  cells grown by 1: Delta(rho theta), extrapolated theta, lagged update
 //   K2  x- and y-faces (one launch): horizontal momentum update, written
 //       straight into the current state (each lane reads prev and writes cur
 //       at its own index, so the S_prev == S_data aliasing is safe)
-//   K3  cells: horizontal flux divergence of the increments
-//   K4  columns (one lane per (i, j), k loop): tridiagonal RHS, Thomas sweep,
-//       new rho*w
-//   K5  cells: vertical fluxes, time-averaged rho*w, rho and rho*theta update
+//   K3-5 columns (one lane per (i, j), k loops): horizontal divergence of the
+//       increments, tridiagonal RHS, Thomas sweep, new rho*w, vertical
+//       fluxes, time-averaged rho*w, rho and rho*theta update
 // The arithmetic of every point keeps the reference's operation order.
 #include "erfx_common.h"
 #include "erfx_internal.h"
@@ -102,7 +100,6 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double d = prt[q] - srt[q];
         extrap[q] = (step == 0) ? d : d + beta_d * (d - lagged[q]);
         lagged[q] = d;
-        if (k >= 0) drw[q] = prev.zmom[q] - stage.zmom[q];
     });
 
     // K2 (:161-257)
@@ -138,20 +135,13 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         }
     });
 
-    // K3 (:328-357)
-    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        double xflux_lo = data.xmom[q] - stage.xmom[q];
-        double xflux_hi = data.xmom[q + 1] - stage.xmom[q + 1];
-        double yflux_lo = data.ymom[q] - stage.ymom[q];
-        double yflux_hi = data.ymom[q + sy] - stage.ymom[q + sy];
-        double pc = prim[q];
-        tr0[q] = (xflux_hi - xflux_lo) * dxi + (yflux_hi - yflux_lo) * dyi;
-        tr1[q] = ((xflux_hi * (pc + prim[q + 1]) - xflux_lo * (pc + prim[q - 1])) * dxi +
-                  (yflux_hi * (pc + prim[q + sy]) - yflux_lo * (pc + prim[q - sy])) * dyi) * 0.5;
-    });
-
-    // K4 (:374-453): RHS on interior faces, boundary rows, Thomas sweep (GPU branch)
+    // K3 + K4 + K5 fused per column (:328-357, :374-453, :518-536, :566-592).
+    // One lane per (i, j) column.  Forward sweep: horizontal divergence of the
+    // increments in cell k, tridiagonal RHS on face k, elimination.  Backward
+    // sweep: substitution, new rho*w, vertical fluxes and the rho / rho*theta
+    // update of cell k.  The loads of the next level are issued before the
+    // arithmetic of the current one, so each lane keeps two levels of loads in
+    // flight.  Every expression keeps the reference's operation order.
     const double halfg = fabs(0.5 * (-P.gravity));
     const double* A = fc;
     const double* iB = fc + ne;
@@ -162,96 +152,129 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     const double* sr1 = slow_rhs.cons + ne;
     const double* pr0 = prev.cons;
     const double* st0 = stage.cons;
+    double* cc0 = data.cons;
+    double* cc1 = data.cons + ne;
     kt_start(P.timer, KT_FAST_COLUMN, s);
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
-        // bottom row
-        double rhs0 = prev.zmom[q0] - stage.zmom[q0] + dtau * slow_rhs.zmom[q0];
-        // every RHS reads prev.zmom (== data.zmom for step > 0), so no rho*w is
-        // written before the forward sweep is complete
-        double sol_prev = rhs0 * iB[q0];
+        // per-level loads of the forward sweep (face k and cell k)
+        struct Fw {
+            double P, Q, A, iB, ws, pkp1, pz, szz, pr, st, prt_, srt_, s0, s1;
+            double dxl, dxh, sxl, sxh, dyl, dyh, syl, syh, pc, pxm, pxp, pym, pyp;
+        };
+        auto fload = [&](long q, Fw& v) {
+            v.P = Pc[q]; v.Q = Qc[q]; v.A = A[q]; v.iB = iB[q]; v.ws = slow_rhs.zmom[q];
+            v.pkp1 = prim[q + sz]; v.pz = prev.zmom[q + sz]; v.szz = stage.zmom[q + sz];
+            v.pr = pr0[q]; v.st = st0[q]; v.prt_ = prt[q]; v.srt_ = srt[q]; v.s0 = sr0[q]; v.s1 = sr1[q];
+            v.dxl = data.xmom[q]; v.dxh = data.xmom[q + 1]; v.sxl = stage.xmom[q]; v.sxh = stage.xmom[q + 1];
+            v.dyl = data.ymom[q]; v.dyh = data.ymom[q + sy]; v.syl = stage.ymom[q]; v.syh = stage.ymom[q + sy];
+            v.pc = prim[q]; v.pxm = prim[q - 1]; v.pxp = prim[q + 1]; v.pym = prim[q - sy]; v.pyp = prim[q + sy];
+        };
+        // horizontal flux divergence of the increments (K3)
+        auto hdiv = [&](const Fw& v, double& t0, double& t1) {
+            double xflux_lo = v.dxl - v.sxl;
+            double xflux_hi = v.dxh - v.sxh;
+            double yflux_lo = v.dyl - v.syl;
+            double yflux_hi = v.dyh - v.syh;
+            double pc = v.pc;
+            t0 = (xflux_hi - xflux_lo) * dxi + (yflux_hi - yflux_lo) * dyi;
+            t1 = ((xflux_hi * (pc + v.pxp) - xflux_lo * (pc + v.pxm)) * dxi +
+                  (yflux_hi * (pc + v.pyp) - yflux_lo * (pc + v.pym)) * dyi) * 0.5;
+        };
+
+        Fw cur, nxt;
+        fload(q0, cur);
+        // face 0 (bottom row, :423-432) and cell 0
+        double Om_km1 = prev.zmom[q0] - stage.zmom[q0];
+        double rhs0 = Om_km1 + dtau * cur.ws;
+        double sol_prev = implicit ? rhs0 * cur.iB : rhs0;
         soln[q0] = sol_prev;
-        const double w0 = stage.zmom[q0] + sol_prev;
-        for (int k = 1; k <= nz; ++k) {
+        double t0_km1, t1_km1;
+        hdiv(cur, t0_km1, t1_km1);
+        tr0[q0] = t0_km1;
+        tr1[q0] = t1_km1;
+        double drho_km1 = cur.pr - cur.st;
+        double drt_km1 = cur.prt_ - cur.srt_;
+        double s0_km1 = cur.s0, s1_km1 = cur.s1;
+        double p_km2 = prim[q0 - sz], p_km1 = cur.pc, p_k = cur.pkp1;
+        double Om_k = cur.pz - cur.szz;
+        if (nz > 1) fload(q0 + sz, nxt);
+        for (int k = 1; k <= nz - 1; ++k) {
             const long q = q0 + k * sz;
-            double RHS;
-            if (k <= nz - 1) {
-                double coeff_P = Pc[q];
-                double coeff_Q = Qc[q];
-                double theta_t_lo = 0.5 * (prim[q - 2 * sz] + prim[q - sz]);
-                double theta_t_mid = 0.5 * (prim[q - sz] + prim[q]);
-                double theta_t_hi = 0.5 * (prim[q] + prim[q + sz]);
-                double Omega_kp1 = prev.zmom[q + sz] - stage.zmom[q + sz];
-                double Omega_k = prev.zmom[q] - stage.zmom[q];
-                double Omega_km1 = prev.zmom[q - sz] - stage.zmom[q - sz];
-                double old_drho_k = pr0[q] - st0[q];
-                double old_drho_km1 = pr0[q - sz] - st0[q - sz];
-                double drt_k = prt[q] - srt[q];
-                double drt_km1 = prt[q - sz] - srt[q - sz];
-                double R0_tmp = coeff_P * drt_k + coeff_Q * drt_km1 - halfg * (old_drho_k + old_drho_km1);
-                double R1_tmp = halfg * (-sr0[q] - sr0[q - sz] + tr0[q] + tr0[q - sz]) +
-                                (coeff_P * (sr1[q] - tr1[q]) + coeff_Q * (sr1[q - sz] - tr1[q - sz]));
-                R1_tmp += beta_1 * dzi * ((Omega_kp1 - Omega_km1) * halfg
-                                          - (Omega_kp1 * theta_t_hi - Omega_k * theta_t_mid) * coeff_P
-                                          - (Omega_k * theta_t_mid - Omega_km1 * theta_t_lo) * coeff_Q);
-                RHS = Omega_k + dtau * (slow_rhs.zmom[q] + R0_tmp + dtau * beta_2 * R1_tmp);
-            } else {
-                RHS = prev.zmom[q] - stage.zmom[q] + dtau * slow_rhs.zmom[q];
-            }
-            double sol;
-            if (implicit) sol = (RHS - A[q] * sol_prev) * iB[q];
-            else sol = RHS;
+            cur = nxt;
+            if (k + 1 <= nz - 1) fload(q + sz, nxt);
+            double t0_k, t1_k;
+            hdiv(cur, t0_k, t1_k);
+            tr0[q] = t0_k;
+            tr1[q] = t1_k;
+            double Om_kp1 = cur.pz - cur.szz;
+            double drho_k = cur.pr - cur.st;
+            double drt_k = cur.prt_ - cur.srt_;
+            double coeff_P = cur.P, coeff_Q = cur.Q;
+            double theta_t_lo = 0.5 * (p_km2 + p_km1);
+            double theta_t_mid = 0.5 * (p_km1 + p_k);
+            double theta_t_hi = 0.5 * (p_k + cur.pkp1);
+            double R0_tmp = coeff_P * drt_k + coeff_Q * drt_km1 - halfg * (drho_k + drho_km1);
+            double R1_tmp = halfg * (-cur.s0 - s0_km1 + t0_k + t0_km1) +
+                            (coeff_P * (cur.s1 - t1_k) + coeff_Q * (s1_km1 - t1_km1));
+            R1_tmp += beta_1 * dzi * ((Om_kp1 - Om_km1) * halfg
+                                      - (Om_kp1 * theta_t_hi - Om_k * theta_t_mid) * coeff_P
+                                      - (Om_k * theta_t_mid - Om_km1 * theta_t_lo) * coeff_Q);
+            double RHS = Om_k + dtau * (cur.ws + R0_tmp + dtau * beta_2 * R1_tmp);
+            double sol = implicit ? (RHS - cur.A * sol_prev) * cur.iB : RHS;
             soln[q] = sol;
             sol_prev = sol;
+            p_km2 = p_km1; p_km1 = p_k; p_k = cur.pkp1;
+            Om_km1 = Om_k; Om_k = Om_kp1;
+            drho_km1 = drho_k; drt_km1 = drt_k;
+            s0_km1 = cur.s0; s1_km1 = cur.s1;
+            t0_km1 = t0_k; t1_km1 = t1_k;
         }
-        data.zmom[q0] = w0;
-        if (implicit) {
-            const long qt = q0 + nz * sz;
-            data.zmom[qt] = stage.zmom[qt] + sol_prev;
-            double sol_next = sol_prev;
-            for (int k = nz - 1; k >= 0; --k) {
-                const long q = q0 + k * sz;
-                double sol = soln[q];
-                sol -= (Cc[q] * iB[q]) * sol_next;
-                soln[q] = sol;
-                data.zmom[q] = stage.zmom[q] + sol;
-                sol_next = sol;
-            }
-        } else {
-            soln[q0] = rhs0;
-            for (int k = 0; k <= nz; ++k) {
-                const long q = q0 + k * sz;
-                data.zmom[q] = stage.zmom[q] + soln[q];
-            }
-        }
-    });
-    // algorithmic traffic per z-face: 17 fields read once (A, 1/B, C, P, Q, prim,
-    // prev/stage zmom, prev/stage rho and rho*theta, slow rho, rho*theta and zmom,
-    // the two horizontal divergences), soln and the new zmom written once
-    kt_stop(P.timer, KT_FAST_COLUMN, s, 19.0 * 8.0 * nx * ny * (nz + 1), "fast_rhs_N column solve");
+        // top face (:423-432): Om_k now holds face nz
+        const long qt = q0 + nz * sz;
+        double rhs_t = Om_k + dtau * slow_rhs.zmom[qt];
+        double sol_t = implicit ? (rhs_t - A[qt] * sol_prev) * iB[qt] : rhs_t;
+        data.zmom[qt] = stage.zmom[qt] + sol_t;
 
-    // K5 (:518-536 and :566-592)
-    double* cc0 = data.cons;
-    double* cc1 = data.cons + ne;
-    const double* pc0 = prev.cons;
-    const double* pc1 = prev.cons + ne;
-    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        double zflux_lo = beta_2 * soln[q] + beta_1 * drw[q];
-        double zflux_hi = beta_2 * soln[q + sz] + beta_1 * drw[q + sz];
-        scratch.zmom[q] += facinv * zflux_lo / (1.0 * 1.0);
-        if (k == nz - 1) scratch.zmom[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
-        double pcq = prim[q];
-        double t0 = tr0[q] + dzi * (zflux_hi - zflux_lo);
-        double t1 = tr1[q] + 0.5 * dzi * (zflux_hi * (pcq + prim[q + sz]) - zflux_lo * (pcq + prim[q - sz]));
-        if (step == 0) {
-            cc0[q] = pc0[q] + dtau * (sr0[q] - t0);
-            cc1[q] = pc1[q] + dtau * (sr1[q] - t1);
-        } else {
-            cc0[q] += dtau * (sr0[q] - t0);
-            cc1[q] += dtau * (sr1[q] - t1);
+        // backward sweep with the cell update (K5)
+        struct Bw { double so, C, iB, pz, szz, t0, t1, pc, pm, s0, s1, c0, c1; };
+        auto bload = [&](long q, Bw& v) {
+            v.so = soln[q]; v.C = Cc[q]; v.iB = iB[q]; v.pz = prev.zmom[q]; v.szz = stage.zmom[q];
+            v.t0 = tr0[q]; v.t1 = tr1[q]; v.pc = prim[q]; v.pm = prim[q - sz]; v.s0 = sr0[q]; v.s1 = sr1[q];
+            v.c0 = step == 0 ? prev.cons[q] : cc0[q];
+            v.c1 = step == 0 ? prev.cons[q + ne] : cc1[q];
+        };
+        double sol_next = sol_t;
+        double drw_next = Om_k;
+        double p_next = prim[qt];
+        Bw bc, bn;
+        bload(qt - sz, bn);
+        for (int k = nz - 1; k >= 0; --k) {
+            const long q = q0 + k * sz;
+            bc = bn;
+            if (k >= 1) bload(q - sz, bn);
+            double sol = implicit ? bc.so - (bc.C * bc.iB) * sol_next : bc.so;
+            double drw_k = bc.pz - bc.szz;
+            data.zmom[q] = bc.szz + sol;
+            double zflux_lo = beta_2 * sol + beta_1 * drw_k;
+            double zflux_hi = beta_2 * sol_next + beta_1 * drw_next;
+            scratch.zmom[q] += facinv * zflux_lo / (1.0 * 1.0);
+            if (k == nz - 1) scratch.zmom[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
+            double pcq = bc.pc;
+            double t0 = bc.t0 + dzi * (zflux_hi - zflux_lo);
+            double t1 = bc.t1 + 0.5 * dzi * (zflux_hi * (pcq + p_next) - zflux_lo * (pcq + bc.pm));
+            cc0[q] = bc.c0 + dtau * (bc.s0 - t0);
+            cc1[q] = bc.c1 + dtau * (bc.s1 - t1);
+            sol_next = sol;
+            drw_next = drw_k;
+            p_next = pcq;
         }
     });
+    // algorithmic traffic per z-face: 22 fields read once (A, 1/B, C, P, Q, theta,
+    // prev/stage rho*w, prev/stage rho and rho*theta, slow rho, rho*theta and
+    // rho*w, new and stage x- and y-momenta, rho and rho*theta of the state being
+    // updated, averaged rho*w) and 4 written (rho*w, averaged rho*w, rho, rho*theta)
+    kt_stop(P.timer, KT_FAST_COLUMN, s, 26.0 * 8.0 * nx * ny * (nz + 1), "fast_rhs_N column (K3+K4+K5)");
 }
 
 }  // namespace erfx
