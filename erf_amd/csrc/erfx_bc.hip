@@ -260,6 +260,148 @@ void velocity_to_momentum(const Params& P, const double* xv, const double* yv, c
     });
 }
 
+// The per-substep apply_bcs (fast_only, unsynced, ng_cons = ng_vel = 1) on a
+// box periodic in x and y with wall / outflow z faces, in two launches instead
+// of the ~30 of the generic sequence below.  It writes exactly the values that
+// sequence leaves behind (M2V on valid faces; FillBoundary + foextrap of rho on
+// 2 and of rho*theta on 1 vertical ghost layer; velocity fills; V2M on the box
+// grown by 1; FillBoundary of the momenta), point for point:
+//   KA  cons ghosts and every velocity the sequence defines, each computed
+//       from valid data (periodic images, same-column values), so no lane
+//       reads a ghost another lane writes;
+//   KB  momenta: V2M where the sequence applies it, the periodic image's V2M
+//       on the FillBoundary layers.
+// Points the sequence does not touch are not written.
+bool fast_bcs_fusable(const Params& P) {
+    if (!P.periodic[0] || !P.periodic[1]) return false;
+    const int G = P.ng_cons > P.ng_vel ? P.ng_cons : P.ng_vel;
+    if (P.L.nx < G + 1 || P.L.ny < G + 1 || P.L.nz < 3) return false;
+    for (int v = 0; v < 2; ++v)
+        for (int o : {2, 5})
+            if (P.bck[v][o] != EXT_DIR && P.bck[v][o] != FOEXTRAP) return false;
+    if (P.bck[2][5] != EXT_DIR && P.bck[2][5] != NEUMANN_INT) return false;
+    return true;
+}
+
+void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv,
+                             hipStream_t s) {
+    const Lay L = P.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int ngC = P.ng_cons, ngV = P.ng_vel;
+    const int G = ngC > ngV ? ngC : ngV;
+    const long ne = L.ne, sy = L.sy, sz = L.sz;
+    const int nc = P.cfg.ncons;
+    double* c = S.cons;
+    double* xm = S.xmom;
+    double* ym = S.ymom;
+    double* zm = S.zmom;
+    const int xtl = P.bck[0][2], xth = P.bck[0][5], ytl = P.bck[1][2], yth = P.bck[1][5];
+    const double xvl = P.vel_extdir[0][2], xvh = P.vel_extdir[0][5];
+    const double yvl = P.vel_extdir[1][2], yvh = P.vel_extdir[1][5];
+    const double zvl = P.vel_extdir[2][2], zvh = P.vel_extdir[2][5];
+    const int zth = P.bck[2][5];
+    const Box grown(-G, -G, -2, nx + G, ny + G, nz + 1);
+
+    launch_box(grown, s, [=] __device__(int i, int j, int k) {
+        // periodic images: cell-centred and face-centred along x / y
+        const int ic = i < 0 ? i + nx : (i > nx - 1 ? i - nx : i);
+        const int jc = j < 0 ? j + ny : (j > ny - 1 ? j - ny : j);
+        const int if_ = i < 0 ? i + nx : (i > nx ? i - nx : i);
+        const int jf = j < 0 ? j + ny : (j > ny ? j - ny : j);
+        const long q = L.ix(i, j, k);
+        const bool kval = k >= 0 && k <= nz - 1;
+        // ---- cons ghosts
+        if (kval && (i < 0 || i > nx - 1 || j < 0 || j > ny - 1) && i >= -ngC && i <= nx - 1 + ngC &&
+            j >= -ngC && j <= ny - 1 + ngC) {
+            const long src = L.ix(ic, jc, k);
+            for (int n = 0; n < nc; ++n) c[q + n * ne] = c[src + n * ne];
+        }
+        if (!kval) {
+            const long src = L.ix(ic, jc, k < 0 ? 0 : nz - 1);
+            if (i >= -2 && i <= nx + 1 && j >= -2 && j <= ny + 1) c[q] = c[src];
+            if ((k == -1 || k == nz) && i >= -1 && i <= nx && j >= -1 && j <= ny) c[q + ne] = c[src + ne];
+        }
+        // rho after the fills at a cell of valid k (any lateral index)
+        auto rho_v = [&](int ii, int jj, int kk) {
+            int a = ii < 0 ? ii + nx : (ii > nx - 1 ? ii - nx : ii);
+            int b = jj < 0 ? jj + ny : (jj > ny - 1 ? jj - ny : jj);
+            return c[L.ix(a, b, kk)];
+        };
+        // M2V of a valid face (ERF_MomentumToVelocity.cpp:24-96)
+        auto u_of = [&](int fi, int jj, int kk) {
+            long f = L.ix(fi, jj, kk);
+            return xm[f] * 2.0 / (rho_v(fi, jj, kk) + rho_v(fi - 1, jj, kk));
+        };
+        auto v_of = [&](int ii, int fj, int kk) {
+            long f = L.ix(ii, fj, kk);
+            return ym[f] * 2.0 / (rho_v(ii, fj, kk) + rho_v(ii, fj - 1, kk));
+        };
+        auto w_of = [&](int ii, int jj, int fk) {
+            long f = L.ix(ii, jj, fk);
+            int kl = fk - 1 < 0 ? 0 : fk - 1, kh = fk > nz - 1 ? nz - 1 : fk;
+            return zm[f] * 2.0 / (rho_v(ii, jj, kh) + rho_v(ii, jj, kl));
+        };
+        // ---- x-velocity: valid faces, FillBoundary layers, vertical ghosts
+        if (kval && i >= -ngV && i <= nx + ngV && j >= -ngV && j <= ny - 1 + ngV) {
+            xv[q] = u_of(if_, jc, k);
+        } else if ((k == -1 || k == nz) && i >= -1 && i <= nx + 1 && j >= -1 && j <= ny) {
+            bool lo = k < 0;
+            int t = lo ? xtl : xth;
+            xv[q] = t == EXT_DIR ? (lo ? xvl : xvh) : u_of(if_, jc, lo ? 0 : nz - 1);
+        }
+        // ---- y-velocity
+        if (kval && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV && j <= ny + ngV) {
+            yv[q] = v_of(ic, jf, k);
+        } else if ((k == -1 || k == nz) && i >= -1 && i <= nx && j >= -1 && j <= ny + 1) {
+            bool lo = k < 0;
+            int t = lo ? ytl : yth;
+            yv[q] = t == EXT_DIR ? (lo ? yvl : yvh) : v_of(ic, jf, lo ? 0 : nz - 1);
+        }
+        // ---- z-velocity (k = 0 takes the zlo value on the valid box grown by 1)
+        if (k >= 0 && k <= nz && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV && j <= ny - 1 + ngV) {
+            const bool near = i >= -1 && i <= nx && j >= -1 && j <= ny;
+            double w;
+            if (near && k == 0) w = zvl;
+            else if (near && k == nz && zth == EXT_DIR) w = zvh;
+            else if (near && k == nz) w = (4.0 * w_of(ic, jc, nz - 1) - w_of(ic, jc, nz - 2)) / 3.0;
+            else w = w_of(ic, jc, k);
+            zv[q] = w;
+        }
+    });
+
+    launch_box(grown, s, [=] __device__(int i, int j, int k) {
+        const int ic = i < 0 ? i + nx : (i > nx - 1 ? i - nx : i);
+        const int jc = j < 0 ? j + ny : (j > ny - 1 ? j - ny : j);
+        const int if_ = i < 0 ? i + nx : (i > nx ? i - nx : i);
+        const int jf = j < 0 ? j + ny : (j > ny ? j - ny : j);
+        const long q = L.ix(i, j, k);
+        const bool kval = k >= 0 && k <= nz - 1;
+        // VelocityToMomentum (ERF_VelocityToMomentum.cpp:24-114) at face index f
+        auto xm_at = [&](long f) { return xv[f] * 0.5 * (c[f] + c[f - 1]); };
+        auto ym_at = [&](long f) { return yv[f] * 0.5 * (c[f] + c[f - sy]); };
+        auto zm_at = [&](long f) { return zv[f] * 0.5 * (c[f] + c[f - sz]); };
+        // x-momentum: FillBoundary layers take the image's V2M
+        if (kval && (i < 0 || i > nx || j < 0 || j > ny - 1) && i >= -ngV && i <= nx + ngV && j >= -ngV &&
+            j <= ny - 1 + ngV) {
+            xm[q] = xm_at(L.ix(if_, jc, k));
+        } else if (i >= -1 && i <= nx + 1 && j >= -1 && j <= ny && k >= -1 && k <= nz) {
+            xm[q] = xm_at(q);
+        }
+        if (kval && (i < 0 || i > nx - 1 || j < 0 || j > ny) && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV &&
+            j <= ny + ngV) {
+            ym[q] = ym_at(L.ix(ic, jf, k));
+        } else if (i >= -1 && i <= nx && j >= -1 && j <= ny + 1 && k >= -1 && k <= nz) {
+            ym[q] = ym_at(q);
+        }
+        if (k >= 0 && k <= nz && (i < 0 || i > nx - 1 || j < 0 || j > ny - 1) && i >= -ngV && i <= nx - 1 + ngV &&
+            j >= -ngV && j <= ny - 1 + ngV) {
+            zm[q] = zm_at(L.ix(ic, jc, k));
+        } else if (i >= -1 && i <= nx && j >= -1 && j <= ny && k >= 0 && k <= nz) {
+            zm[q] = zm_at(q);
+        }
+    });
+}
+
 // ERF::FillIntermediatePatch, level 0 (ERF_FillIntermediatePatch.cpp:27-261).
 void fill_intermediate_patch(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv,
                              int ngc, int ngv, bool cons_only, int icomp, int ncomp, hipStream_t s) {
@@ -280,6 +422,10 @@ void fill_intermediate_patch(const Params& P, const erfx_state& S, double* xv, d
 // apply_bcs (Source/TimeIntegration/ERF_TI_utils.H:53-150).
 void apply_bcs(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv, int ngc, int ngv,
                bool fast_only, bool synced, hipStream_t s) {
+    if (fast_only && !synced && ngc == 1 && ngv == 1 && fast_bcs_fusable(P)) {
+        apply_bcs_fast_periodic(P, S, xv, yv, zv, s);
+        return;
+    }
     int scomp, ncomp, ngu;
     if (!synced) {
         ngu = ngc > ngv + 1 ? ngc : ngv + 1;
