@@ -454,6 +454,7 @@ static int sim_one_step(erfx_sim* s, double time, double dt) {
     ERFX_HIP(hipMemcpyAsync(Sn.zmom, So.zmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int ratio = P.cfg.mri_ratio;
     const double sub_timestep = dt / ratio;
+    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
     double time_stage = time;
     for (int nrk = 0; nrk < 3; ++nrk) {
@@ -473,15 +474,19 @@ static int sim_one_step(erfx_sim* s, double time, double dt) {
         slow_rhs_pre(P, c->sw, nrk, slow_dt, Fs, Sn, s->prim, Sc, un, vn, wn, s->eddy, s->r0, s->p0, st);
         const double inv_fac = 1.0 / static_cast<double>(nsub);
         for (int ks = 0; ks < nsub; ++ks) {
-            if (ks == 0) {
-                make_fast_coeffs(P, s->fast_coeffs, Sn, s->prim, s->pi_stage, s->r0, s->pi0, dtau, beta_s, st);
-                fast_rhs_N(P, c->fw, ks, nrk, Fs, So, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc, dtau, beta_s,
-                           inv_fac, st);
-            } else {
-                fast_rhs_N(P, c->fw, ks, nrk, Fs, Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc, dtau, beta_s,
-                           inv_fac, st);
-            }
-            apply_bcs(P, Ss, un, vn, wn, 1, 1, true, false, st);
+            // On a laterally periodic box the apply_bcs between two substeps is
+            // folded into the next substep (round trip of the momenta on read,
+            // ghost layer 1 of rho and rho*theta written by the column kernel);
+            // the full fill runs after the stage's last substep.
+            FastChain ch;
+            ch.rt_prev = chain && ks > 0;
+            ch.ghost_writes = chain && ks < nsub - 1;
+            ch.w_lo = P.vel_extdir[2][2];
+            ch.w_hi = P.vel_extdir[2][5];
+            if (ks == 0) make_fast_coeffs(P, s->fast_coeffs, Sn, s->prim, s->pi_stage, s->r0, s->pi0, dtau, beta_s, st);
+            fast_rhs_N(P, c->fw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc,
+                       dtau, beta_s, inv_fac, st, ch);
+            if (!chain || ks == nsub - 1) apply_bcs(P, Ss, un, vn, wn, 1, 1, true, false, st);
         }
         slow_rhs_post(P, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);
         apply_bcs(P, Sn, un, vn, wn, P.ng_cons, P.ng_vel, false, false, st);

@@ -83,11 +83,13 @@ __global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int
     f(i, j);
 }
 
+// Work-groups are 256 lanes along x, so each field-level access of a group is
+// one contiguous 2 KB row segment (one DRAM-friendly burst per field and level).
 template <class F>
 inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
     if (ilo > ihi || jlo > jhi) return;
-    dim3 blk(64, 4, 1);
-    dim3 grd((unsigned)((ihi - ilo + 1 + 63) / 64), (unsigned)((jhi - jlo + 1 + 3) / 4), 1);
+    dim3 blk(256, 1, 1);
+    dim3 grd((unsigned)((ihi - ilo + 1 + 255) / 256), (unsigned)(jhi - jlo + 1), 1);
     hipLaunchKernelGGL(col_kernel<F>, grd, blk, 0, s, ilo, ihi, jlo, jhi, f);
 }
 

@@ -38,45 +38,67 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
     const double D = dtau * dtau * beta_2 * beta_2 * dzi;
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
-        // interior faces k = 1..nz-1
-        for (int k = 1; k <= nz - 1; ++k) {
-            const long q = q0 + k * sz;
-            double rhobar_lo = r0[q - sz], rhobar_hi = r0[q];
-            double pibar_lo = pi0[q - sz], pibar_hi = pi0[q];
-            double pi_c = 0.5 * (pi_stage[q - sz] + pi_stage[q]);
-            double coeff_P = -Gamma * R_d * dzi + halfg * R_d * rhobar_hi / (c_v * pibar_hi * rt[q]);
-            coeff_P *= pi_c;
-            double coeff_Q = Gamma * R_d * dzi + halfg * R_d * rhobar_lo / (c_v * pibar_lo * rt[q - sz]);
-            coeff_Q *= pi_c;
-            Pc[q] = coeff_P;
-            Qc[q] = coeff_Q;
-            double theta_t_lo = 0.5 * (prim[q - 2 * sz] + prim[q - sz]);
-            double theta_t_mid = 0.5 * (prim[q - sz] + prim[q]);
-            double theta_t_hi = 0.5 * (prim[q] + prim[q + sz]);
-            A[q] = D * (halfg - coeff_Q * theta_t_lo);
-            Cc[q] = D * (-halfg + coeff_P * theta_t_hi);
-            B[q] = 1.0 + D * (coeff_Q - coeff_P) * theta_t_mid;
-        }
+        // bottom face, then the interior faces with the recurrence carried along
         A[q0] = 0.0; B[q0] = 1.0; Cc[q0] = 0.0;
-        const long qt = q0 + nz * sz;
-        A[qt] = top_outflow ? -1.0 : 0.0; B[qt] = 1.0; Cc[qt] = 0.0;
-        double bet = B[q0];
-        double c_prev = Cc[q0];
-        for (int k = 1; k <= nz; ++k) {
-            const long q = q0 + k * sz;
-            double gam = c_prev / bet;
-            bet = B[q] - A[q] * gam;
-            c_prev = Cc[q];
-            // store 1/B on interior faces only; the top keeps B itself (:287-290)
-            B[q] = (k <= nz - 1) ? 1.0 / bet : bet;
+        double bet = 1.0;
+        double c_prev = 0.0;
+        double r_lo = r0[q0], pib_lo = pi0[q0], pis_lo = pi_stage[q0], rt_lo = rt[q0];
+        double pr_km2 = prim[q0 - sz], pr_km1 = prim[q0], pr_k = prim[q0 + sz];
+        struct Lv { double r, pib, pis, rt_, pkp1; };
+        auto load = [&](long q, Lv& v) {
+            v.r = r0[q]; v.pib = pi0[q]; v.pis = pi_stage[q]; v.rt_ = rt[q]; v.pkp1 = prim[q + sz];
+        };
+        constexpr int RD = 4;
+        Lv ring[RD];
+#pragma unroll
+        for (int d = 0; d < RD; ++d)
+            if (1 + d <= nz - 1) load(q0 + (1 + d) * sz, ring[d]);
+        for (int kb = 1; kb <= nz - 1; kb += RD) {
+#pragma unroll
+            for (int d = 0; d < RD; ++d) {
+                const int k = kb + d;
+                if (k > nz - 1) break;
+                const long q = q0 + k * sz;
+                Lv v = ring[d];
+                if (k + RD <= nz - 1) load(q + RD * sz, ring[d]);
+                double rhobar_lo = r_lo, rhobar_hi = v.r;
+                double pibar_lo = pib_lo, pibar_hi = v.pib;
+                double pi_c = 0.5 * (pis_lo + v.pis);
+                double coeff_P = -Gamma * R_d * dzi + halfg * R_d * rhobar_hi / (c_v * pibar_hi * v.rt_);
+                coeff_P *= pi_c;
+                double coeff_Q = Gamma * R_d * dzi + halfg * R_d * rhobar_lo / (c_v * pibar_lo * rt_lo);
+                coeff_Q *= pi_c;
+                Pc[q] = coeff_P;
+                Qc[q] = coeff_Q;
+                double theta_t_lo = 0.5 * (pr_km2 + pr_km1);
+                double theta_t_mid = 0.5 * (pr_km1 + pr_k);
+                double theta_t_hi = 0.5 * (pr_k + v.pkp1);
+                double a = D * (halfg - coeff_Q * theta_t_lo);
+                double c = D * (-halfg + coeff_P * theta_t_hi);
+                double b = 1.0 + D * (coeff_Q - coeff_P) * theta_t_mid;
+                A[q] = a;
+                Cc[q] = c;
+                double gam = c_prev / bet;
+                bet = b - a * gam;
+                c_prev = c;
+                B[q] = 1.0 / bet;                        // interior faces store 1/B (:287-290)
+                r_lo = v.r; pib_lo = v.pib; pis_lo = v.pis; rt_lo = v.rt_;
+                pr_km2 = pr_km1; pr_km1 = pr_k; pr_k = v.pkp1;
+            }
         }
+        // top face keeps B itself
+        const long qt = q0 + nz * sz;
+        const double at = top_outflow ? -1.0 : 0.0;
+        A[qt] = at; Cc[qt] = 0.0;
+        double gam = c_prev / bet;
+        B[qt] = 1.0 - at * gam;
     });
 }
 
 void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erfx_state& slow_rhs,
                 const erfx_state& prev, const erfx_state& stage, const double* prim, const double* pi_stage,
                 const double* fc, const erfx_state& data, const erfx_state& scratch, double dtau,
-                double beta_s, double facinv, hipStream_t s) {
+                double beta_s, double facinv, hipStream_t s, const FastChain& ch) {
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sy = L.sy, sz = L.sz;
@@ -102,33 +124,49 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         lagged[q] = d;
     });
 
-    // K2 (:161-257)
+    // K2 (:161-257).  With rt_prev the previous substep's apply_bcs is folded in:
+    // its momentum -> velocity -> momentum round trip (ERF_MomentumToVelocity.cpp,
+    // ERF_VelocityToMomentum.cpp) is applied to prev as it is read.
     const bool first = (nrk == 0 && step == 0);
+    const bool rt = ch.rt_prev && step > 0;
+    const double* prho = prev.cons;
     const double gr = -Gamma * R_d;
     launch_box(Box(0, 0, 0, nx, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (j <= ny - 1) {
+            double pm = prev.xmom[q];
+            if (rt) {
+                double d = prho[q] + prho[q - 1];
+                double u = pm * 2.0 / d;
+                pm = u * 0.5 * d;
+            }
             double new_drho_u;
             if (first) {
-                new_drho_u = prev.xmom[q] - stage.xmom[q] + dtau * slow_rhs.xmom[q];
+                new_drho_u = pm - stage.xmom[q] + dtau * slow_rhs.xmom[q];
             } else {
                 double gpx = (extrap[q] - extrap[q - 1]) * dxi;
                 double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
                 double fast_rhs_rho_u = gr * pi_c * gpx;
-                new_drho_u = prev.xmom[q] - stage.xmom[q] + dtau * fast_rhs_rho_u + dtau * slow_rhs.xmom[q];
+                new_drho_u = pm - stage.xmom[q] + dtau * fast_rhs_rho_u + dtau * slow_rhs.xmom[q];
             }
             scratch.xmom[q] += facinv * new_drho_u;
             data.xmom[q] = stage.xmom[q] + new_drho_u;
         }
         if (i <= nx - 1) {
+            double pm = prev.ymom[q];
+            if (rt) {
+                double d = prho[q] + prho[q - sy];
+                double v = pm * 2.0 / d;
+                pm = v * 0.5 * d;
+            }
             double new_drho_v;
             if (first) {
-                new_drho_v = prev.ymom[q] - stage.ymom[q] + dtau * slow_rhs.ymom[q];
+                new_drho_v = pm - stage.ymom[q] + dtau * slow_rhs.ymom[q];
             } else {
                 double gpy = (extrap[q] - extrap[q - sy]) * dyi;
                 double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
                 double fast_rhs_rho_v = gr * pi_c * gpy;
-                new_drho_v = prev.ymom[q] - stage.ymom[q] + dtau * fast_rhs_rho_v + dtau * slow_rhs.ymom[q];
+                new_drho_v = pm - stage.ymom[q] + dtau * fast_rhs_rho_v + dtau * slow_rhs.ymom[q];
             }
             scratch.ymom[q] += facinv * new_drho_v;
             data.ymom[q] = stage.ymom[q] + new_drho_v;
@@ -154,18 +192,21 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     const double* st0 = stage.cons;
     double* cc0 = data.cons;
     double* cc1 = data.cons + ne;
+    const bool gw = ch.ghost_writes;
+    const double ch_w_lo = ch.w_lo, ch_w_hi = ch.w_hi;
     kt_start(P.timer, KT_FAST_COLUMN, s);
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // per-level loads of the forward sweep (face k and cell k)
         struct Fw {
-            double P, Q, A, iB, ws, pkp1, pz, szz, pr, st, prt_, srt_, s0, s1;
+            double P, Q, A, iB, ws, pkp1, pz, szz, pr, rkp1, st, prt_, srt_, s0, s1;
             double dxl, dxh, sxl, sxh, dyl, dyh, syl, syh, pc, pxm, pxp, pym, pyp;
         };
         auto fload = [&](long q, Fw& v) {
             v.P = Pc[q]; v.Q = Qc[q]; v.A = A[q]; v.iB = iB[q]; v.ws = slow_rhs.zmom[q];
             v.pkp1 = prim[q + sz]; v.pz = prev.zmom[q + sz]; v.szz = stage.zmom[q + sz];
-            v.pr = pr0[q]; v.st = st0[q]; v.prt_ = prt[q]; v.srt_ = srt[q]; v.s0 = sr0[q]; v.s1 = sr1[q];
+            v.pr = pr0[q]; v.rkp1 = rt ? pr0[q + sz] : 0.0;
+            v.st = st0[q]; v.prt_ = prt[q]; v.srt_ = srt[q]; v.s0 = sr0[q]; v.s1 = sr1[q];
             v.dxl = data.xmom[q]; v.dxh = data.xmom[q + 1]; v.sxl = stage.xmom[q]; v.sxh = stage.xmom[q + 1];
             v.dyl = data.ymom[q]; v.dyh = data.ymom[q + sy]; v.syl = stage.ymom[q]; v.syh = stage.ymom[q + sy];
             v.pc = prim[q]; v.pxm = prim[q - 1]; v.pxp = prim[q + 1]; v.pym = prim[q - sy]; v.pyp = prim[q + sy];
@@ -182,10 +223,22 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                   (yflux_hi * (pc + v.pyp) - yflux_lo * (pc + v.pym)) * dyi) * 0.5;
         };
 
-        Fw cur, nxt;
+        // prev rho*w at the face above a level's cell, after the round trip when
+        // rt (interior: rho of the two cells; top: the Dirichlet w with the
+        // foextrap ghost rho)
+        auto w_above = [&](const Fw& v, int kf) {
+            if (!rt) return v.pz;
+            if (kf >= nz) return ch_w_hi * 0.5 * (v.pr + v.pr);
+            double d = v.rkp1 + v.pr;
+            double w = v.pz * 2.0 / d;
+            return w * 0.5 * d;
+        };
+        Fw cur;
         fload(q0, cur);
         // face 0 (bottom row, :423-432) and cell 0
-        double Om_km1 = prev.zmom[q0] - stage.zmom[q0];
+        double pz0 = rt ? ch_w_lo * 0.5 * (cur.pr + cur.pr) : prev.zmom[q0];
+        double Om_km1 = pz0 - stage.zmom[q0];
+        drw[q0] = Om_km1;
         double rhs0 = Om_km1 + dtau * cur.ws;
         double sol_prev = implicit ? rhs0 * cur.iB : rhs0;
         soln[q0] = sol_prev;
@@ -197,49 +250,61 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double drt_km1 = cur.prt_ - cur.srt_;
         double s0_km1 = cur.s0, s1_km1 = cur.s1;
         double p_km2 = prim[q0 - sz], p_km1 = cur.pc, p_k = cur.pkp1;
-        double Om_k = cur.pz - cur.szz;
-        if (nz > 1) fload(q0 + sz, nxt);
-        for (int k = 1; k <= nz - 1; ++k) {
-            const long q = q0 + k * sz;
-            cur = nxt;
-            if (k + 1 <= nz - 1) fload(q + sz, nxt);
-            double t0_k, t1_k;
-            hdiv(cur, t0_k, t1_k);
-            tr0[q] = t0_k;
-            tr1[q] = t1_k;
-            double Om_kp1 = cur.pz - cur.szz;
-            double drho_k = cur.pr - cur.st;
-            double drt_k = cur.prt_ - cur.srt_;
-            double coeff_P = cur.P, coeff_Q = cur.Q;
-            double theta_t_lo = 0.5 * (p_km2 + p_km1);
-            double theta_t_mid = 0.5 * (p_km1 + p_k);
-            double theta_t_hi = 0.5 * (p_k + cur.pkp1);
-            double R0_tmp = coeff_P * drt_k + coeff_Q * drt_km1 - halfg * (drho_k + drho_km1);
-            double R1_tmp = halfg * (-cur.s0 - s0_km1 + t0_k + t0_km1) +
-                            (coeff_P * (cur.s1 - t1_k) + coeff_Q * (s1_km1 - t1_km1));
-            R1_tmp += beta_1 * dzi * ((Om_kp1 - Om_km1) * halfg
-                                      - (Om_kp1 * theta_t_hi - Om_k * theta_t_mid) * coeff_P
-                                      - (Om_k * theta_t_mid - Om_km1 * theta_t_lo) * coeff_Q);
-            double RHS = Om_k + dtau * (cur.ws + R0_tmp + dtau * beta_2 * R1_tmp);
-            double sol = implicit ? (RHS - cur.A * sol_prev) * cur.iB : RHS;
-            soln[q] = sol;
-            sol_prev = sol;
-            p_km2 = p_km1; p_km1 = p_k; p_k = cur.pkp1;
-            Om_km1 = Om_k; Om_k = Om_kp1;
-            drho_km1 = drho_k; drt_km1 = drt_k;
-            s0_km1 = cur.s0; s1_km1 = cur.s1;
-            t0_km1 = t0_k; t1_km1 = t1_k;
+        double Om_k = w_above(cur, 1) - cur.szz;
+        // ring of FD levels of loads in flight (slots indexed statically after unrolling)
+        constexpr int FD = 3;
+        Fw ring[FD];
+#pragma unroll
+        for (int d = 0; d < FD; ++d)
+            if (1 + d <= nz - 1) fload(q0 + (1 + d) * sz, ring[d]);
+        for (int kb = 1; kb <= nz - 1; kb += FD) {
+#pragma unroll
+            for (int d = 0; d < FD; ++d) {
+                const int k = kb + d;
+                if (k > nz - 1) break;
+                const long q = q0 + k * sz;
+                Fw c = ring[d];
+                if (k + FD <= nz - 1) fload(q + FD * sz, ring[d]);
+                double t0_k, t1_k;
+                hdiv(c, t0_k, t1_k);
+                tr0[q] = t0_k;
+                tr1[q] = t1_k;
+                drw[q] = Om_k;
+                double Om_kp1 = w_above(c, k + 1) - c.szz;
+                double drho_k = c.pr - c.st;
+                double drt_k = c.prt_ - c.srt_;
+                double coeff_P = c.P, coeff_Q = c.Q;
+                double theta_t_lo = 0.5 * (p_km2 + p_km1);
+                double theta_t_mid = 0.5 * (p_km1 + p_k);
+                double theta_t_hi = 0.5 * (p_k + c.pkp1);
+                double R0_tmp = coeff_P * drt_k + coeff_Q * drt_km1 - halfg * (drho_k + drho_km1);
+                double R1_tmp = halfg * (-c.s0 - s0_km1 + t0_k + t0_km1) +
+                                (coeff_P * (c.s1 - t1_k) + coeff_Q * (s1_km1 - t1_km1));
+                R1_tmp += beta_1 * dzi * ((Om_kp1 - Om_km1) * halfg
+                                          - (Om_kp1 * theta_t_hi - Om_k * theta_t_mid) * coeff_P
+                                          - (Om_k * theta_t_mid - Om_km1 * theta_t_lo) * coeff_Q);
+                double RHS = Om_k + dtau * (c.ws + R0_tmp + dtau * beta_2 * R1_tmp);
+                double sol = implicit ? (RHS - c.A * sol_prev) * c.iB : RHS;
+                soln[q] = sol;
+                sol_prev = sol;
+                p_km2 = p_km1; p_km1 = p_k; p_k = c.pkp1;
+                Om_km1 = Om_k; Om_k = Om_kp1;
+                drho_km1 = drho_k; drt_km1 = drt_k;
+                s0_km1 = c.s0; s1_km1 = c.s1;
+                t0_km1 = t0_k; t1_km1 = t1_k;
+            }
         }
         // top face (:423-432): Om_k now holds face nz
         const long qt = q0 + nz * sz;
         double rhs_t = Om_k + dtau * slow_rhs.zmom[qt];
         double sol_t = implicit ? (rhs_t - A[qt] * sol_prev) * iB[qt] : rhs_t;
         data.zmom[qt] = stage.zmom[qt] + sol_t;
+        drw[qt] = Om_k;
 
         // backward sweep with the cell update (K5)
-        struct Bw { double so, C, iB, pz, szz, t0, t1, pc, pm, s0, s1, c0, c1; };
+        struct Bw { double so, C, iB, dw, szz, t0, t1, pc, pm, s0, s1, c0, c1; };
         auto bload = [&](long q, Bw& v) {
-            v.so = soln[q]; v.C = Cc[q]; v.iB = iB[q]; v.pz = prev.zmom[q]; v.szz = stage.zmom[q];
+            v.so = soln[q]; v.C = Cc[q]; v.iB = iB[q]; v.dw = drw[q]; v.szz = stage.zmom[q];
             v.t0 = tr0[q]; v.t1 = tr1[q]; v.pc = prim[q]; v.pm = prim[q - sz]; v.s0 = sr0[q]; v.s1 = sr1[q];
             v.c0 = step == 0 ? prev.cons[q] : cc0[q];
             v.c1 = step == 0 ? prev.cons[q + ne] : cc1[q];
@@ -247,27 +312,58 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double sol_next = sol_t;
         double drw_next = Om_k;
         double p_next = prim[qt];
-        Bw bc, bn;
-        bload(qt - sz, bn);
-        for (int k = nz - 1; k >= 0; --k) {
-            const long q = q0 + k * sz;
-            bc = bn;
-            if (k >= 1) bload(q - sz, bn);
-            double sol = implicit ? bc.so - (bc.C * bc.iB) * sol_next : bc.so;
-            double drw_k = bc.pz - bc.szz;
-            data.zmom[q] = bc.szz + sol;
-            double zflux_lo = beta_2 * sol + beta_1 * drw_k;
-            double zflux_hi = beta_2 * sol_next + beta_1 * drw_next;
-            scratch.zmom[q] += facinv * zflux_lo / (1.0 * 1.0);
-            if (k == nz - 1) scratch.zmom[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
-            double pcq = bc.pc;
-            double t0 = bc.t0 + dzi * (zflux_hi - zflux_lo);
-            double t1 = bc.t1 + 0.5 * dzi * (zflux_hi * (pcq + p_next) - zflux_lo * (pcq + bc.pm));
-            cc0[q] = bc.c0 + dtau * (bc.s0 - t0);
-            cc1[q] = bc.c1 + dtau * (bc.s1 - t1);
-            sol_next = sol;
-            drw_next = drw_k;
-            p_next = pcq;
+        constexpr int BD = 3;
+        Bw bring[BD];
+#pragma unroll
+        for (int d = 0; d < BD; ++d)
+            if (nz - 1 - d >= 0) bload(q0 + (nz - 1 - d) * sz, bring[d]);
+        for (int kb = nz - 1; kb >= 0; kb -= BD) {
+#pragma unroll
+            for (int d = 0; d < BD; ++d) {
+                const int k = kb - d;
+                if (k < 0) break;
+                const long q = q0 + k * sz;
+                Bw bc = bring[d];
+                if (k - BD >= 0) bload(q - BD * sz, bring[d]);
+                double sol = implicit ? bc.so - (bc.C * bc.iB) * sol_next : bc.so;
+                double drw_k = bc.dw;
+                data.zmom[q] = bc.szz + sol;
+                double zflux_lo = beta_2 * sol + beta_1 * drw_k;
+                double zflux_hi = beta_2 * sol_next + beta_1 * drw_next;
+                scratch.zmom[q] += facinv * zflux_lo / (1.0 * 1.0);
+                if (k == nz - 1) scratch.zmom[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
+                double pcq = bc.pc;
+                double t0 = bc.t0 + dzi * (zflux_hi - zflux_lo);
+                double t1 = bc.t1 + 0.5 * dzi * (zflux_hi * (pcq + p_next) - zflux_lo * (pcq + bc.pm));
+                double r_new = bc.c0 + dtau * (bc.s0 - t0);
+                double rt_new = bc.c1 + dtau * (bc.s1 - t1);
+                cc0[q] = r_new;
+                cc1[q] = rt_new;
+                if (gw) {
+                    // ghost layer 1 as the skipped apply_bcs would leave it: the
+                    // periodic images of this cell, and at k = 0 / nz-1 the
+                    // foextrap rows below / above, images included
+                    const int ia[2] = {i, i == 0 ? nx : (i == nx - 1 ? -1 : i)};
+                    const int ja[2] = {j, j == 0 ? ny : (j == ny - 1 ? -1 : j)};
+                    const int nk = (k == 0 ? 1 : 0) + (k == nz - 1 ? 1 : 0);
+                    const int ka[3] = {k, k == 0 ? -1 : nz, nz};
+                    for (int a = 0; a < 2; ++a) {
+                        if (a == 1 && ia[1] == i) continue;
+                        for (int b = 0; b < 2; ++b) {
+                            if (b == 1 && ja[1] == j) continue;
+                            for (int c3 = 0; c3 <= nk; ++c3) {
+                                if (a == 0 && b == 0 && c3 == 0) continue;
+                                long g = L.ix(ia[a], ja[b], ka[c3]);
+                                cc0[g] = r_new;
+                                cc1[g] = rt_new;
+                            }
+                        }
+                    }
+                }
+                sol_next = sol;
+                drw_next = drw_k;
+                p_next = pcq;
+            }
         }
     });
     // algorithmic traffic per z-face: 22 fields read once (A, 1/B, C, P, Q, theta,

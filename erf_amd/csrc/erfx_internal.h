@@ -32,10 +32,20 @@ struct FastWork {
 void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, const double* prim,
                       const double* pi_stage, const double* r0, const double* pi0, double dtau, double beta_s,
                       hipStream_t s);
+// Options of the sim driver's substep chain; all off for the C-ABI hook.
+struct FastChain {
+    // prev is S_sum of the previous substep without its apply_bcs: apply the
+    // momentum -> velocity -> momentum round trip of that apply_bcs on read
+    bool rt_prev = false;
+    // write ghost layer 1 of rho and rho*theta (periodic images, foextrap rows)
+    bool ghost_writes = false;
+    double w_lo = 0.0, w_hi = 0.0;   // Dirichlet w on the bottom and top faces
+};
 void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erfx_state& slow_rhs,
                 const erfx_state& prev, const erfx_state& stage, const double* prim, const double* pi_stage,
                 const double* fc, const erfx_state& data, const erfx_state& scratch, double dtau,
-                double beta_s, double facinv, hipStream_t s);
+                double beta_s, double facinv, hipStream_t s, const FastChain& ch = FastChain());
+bool fast_bcs_fusable(const Params& P);
 
 // erfx_slow.hip
 struct SlowWork {
