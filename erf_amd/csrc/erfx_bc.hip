@@ -260,12 +260,12 @@ void velocity_to_momentum(const Params& P, const double* xv, const double* yv, c
     });
 }
 
-// The per-substep apply_bcs (fast_only, unsynced, ng_cons = ng_vel = 1) on a
-// box periodic in x and y with wall / outflow z faces, in two launches instead
-// of the ~30 of the generic sequence below.  It writes exactly the values that
-// sequence leaves behind (M2V on valid faces; FillBoundary + foextrap of rho on
-// 2 and of rho*theta on 1 vertical ghost layer; velocity fills; V2M on the box
-// grown by 1; FillBoundary of the momenta), point for point:
+// apply_bcs with unsynced momenta on a box periodic in x and y with wall /
+// outflow z faces, in two launches instead of the ~30 of the generic sequence
+// below.  It writes exactly the values that sequence leaves behind
+// (FillIntermediatePatch for rho first, then M2V on the valid faces, the cons
+// fills of the remaining components, the velocity fills, V2M on the box grown
+// by 1 and FillBoundary of the momenta), point for point:
 //   KA  cons ghosts and every velocity the sequence defines, each computed
 //       from valid data (periodic images, same-column values), so no lane
 //       reads a ghost another lane writes;
@@ -275,22 +275,35 @@ void velocity_to_momentum(const Params& P, const double* xv, const double* yv, c
 bool fast_bcs_fusable(const Params& P) {
     if (!P.periodic[0] || !P.periodic[1]) return false;
     const int G = P.ng_cons > P.ng_vel ? P.ng_cons : P.ng_vel;
-    if (P.L.nx < G + 1 || P.L.ny < G + 1 || P.L.nz < 3) return false;
+    if (P.L.nx < G + 1 || P.L.ny < G + 1 || P.L.nz < 2 * G) return false;
     for (int v = 0; v < 2; ++v)
         for (int o : {2, 5})
             if (P.bck[v][o] != EXT_DIR && P.bck[v][o] != FOEXTRAP) return false;
     if (P.bck[2][5] != EXT_DIR && P.bck[2][5] != NEUMANN_INT) return false;
-    return true;
+    return P.cfg.ncons <= 8;
 }
 
-void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv,
-                             hipStream_t s) {
+struct ConsRows {
+    int r[8];   // foextrap rows filled below and above, per component (lateral reach the same)
+};
+
+void apply_bcs_periodic(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv, int ngc,
+                        int ngv, bool fast_only, hipStream_t s) {
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int ngC = P.ng_cons, ngV = P.ng_vel;
     const int G = ngC > ngV ? ngC : ngV;
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const int nc = P.cfg.ncons;
+    // rows per component: rho by the first FillIntermediatePatch, the others by the second
+    ConsRows R;
+    const int ngu1 = ngc > ngv + 1 ? ngc : ngv + 1;
+    const int ncomp2 = fast_only ? 1 : nc - 1;
+    int rmax = 0;
+    for (int n = 0; n < 8; ++n) {
+        R.r[n] = n == 0 ? ngu1 : (n < 1 + ncomp2 && n < nc ? ngc : 0);
+        if (R.r[n] > rmax) rmax = R.r[n];
+    }
     double* c = S.cons;
     double* xm = S.xmom;
     double* ym = S.ymom;
@@ -300,9 +313,11 @@ void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, d
     const double yvl = P.vel_extdir[1][2], yvh = P.vel_extdir[1][5];
     const double zvl = P.vel_extdir[2][2], zvh = P.vel_extdir[2][5];
     const int zth = P.bck[2][5];
-    const Box grown(-G, -G, -2, nx + G, ny + G, nz + 1);
+    const int kz = rmax > ngv ? rmax : (ngv > 1 ? ngv : 1);
+    const Box grownA(-G, -G, -kz, nx + G, ny + G, nz - 1 + kz);
+    const Box grownB(-G, -G, -1, nx + G, ny + G, nz);
 
-    launch_box(grown, s, [=] __device__(int i, int j, int k) {
+    launch_box(grownA, s, [=] __device__(int i, int j, int k) {
         // periodic images: cell-centred and face-centred along x / y
         const int ic = i < 0 ? i + nx : (i > nx - 1 ? i - nx : i);
         const int jc = j < 0 ? j + ny : (j > ny - 1 ? j - ny : j);
@@ -310,16 +325,20 @@ void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, d
         const int jf = j < 0 ? j + ny : (j > ny ? j - ny : j);
         const long q = L.ix(i, j, k);
         const bool kval = k >= 0 && k <= nz - 1;
+        const int depth = k < 0 ? -k : (k > nz - 1 ? k - nz + 1 : 0);   // vertical ghost row
         // ---- cons ghosts
         if (kval && (i < 0 || i > nx - 1 || j < 0 || j > ny - 1) && i >= -ngC && i <= nx - 1 + ngC &&
             j >= -ngC && j <= ny - 1 + ngC) {
             const long src = L.ix(ic, jc, k);
             for (int n = 0; n < nc; ++n) c[q + n * ne] = c[src + n * ne];
         }
-        if (!kval) {
+        if (depth > 0) {
             const long src = L.ix(ic, jc, k < 0 ? 0 : nz - 1);
-            if (i >= -2 && i <= nx + 1 && j >= -2 && j <= ny + 1) c[q] = c[src];
-            if ((k == -1 || k == nz) && i >= -1 && i <= nx && j >= -1 && j <= ny) c[q + ne] = c[src + ne];
+            for (int n = 0; n < nc; ++n) {
+                const int rr = R.r[n];
+                if (depth <= rr && i >= -rr && i <= nx - 1 + rr && j >= -rr && j <= ny - 1 + rr)
+                    c[q + n * ne] = c[src + n * ne];
+            }
         }
         // rho after the fills at a cell of valid k (any lateral index)
         auto rho_v = [&](int ii, int jj, int kk) {
@@ -341,10 +360,10 @@ void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, d
             int kl = fk - 1 < 0 ? 0 : fk - 1, kh = fk > nz - 1 ? nz - 1 : fk;
             return zm[f] * 2.0 / (rho_v(ii, jj, kh) + rho_v(ii, jj, kl));
         };
-        // ---- x-velocity: valid faces, FillBoundary layers, vertical ghosts
+        // ---- x-velocity: valid faces, FillBoundary layers, vertical ghost rows
         if (kval && i >= -ngV && i <= nx + ngV && j >= -ngV && j <= ny - 1 + ngV) {
             xv[q] = u_of(if_, jc, k);
-        } else if ((k == -1 || k == nz) && i >= -1 && i <= nx + 1 && j >= -1 && j <= ny) {
+        } else if (depth > 0 && depth <= ngv && i >= -ngv && i <= nx + ngv && j >= -ngv && j <= ny - 1 + ngv) {
             bool lo = k < 0;
             int t = lo ? xtl : xth;
             xv[q] = t == EXT_DIR ? (lo ? xvl : xvh) : u_of(if_, jc, lo ? 0 : nz - 1);
@@ -352,14 +371,14 @@ void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, d
         // ---- y-velocity
         if (kval && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV && j <= ny + ngV) {
             yv[q] = v_of(ic, jf, k);
-        } else if ((k == -1 || k == nz) && i >= -1 && i <= nx && j >= -1 && j <= ny + 1) {
+        } else if (depth > 0 && depth <= ngv && i >= -ngv && i <= nx - 1 + ngv && j >= -ngv && j <= ny + ngv) {
             bool lo = k < 0;
             int t = lo ? ytl : yth;
             yv[q] = t == EXT_DIR ? (lo ? yvl : yvh) : v_of(ic, jf, lo ? 0 : nz - 1);
         }
-        // ---- z-velocity (k = 0 takes the zlo value on the valid box grown by 1)
+        // ---- z-velocity (k = 0 takes the zlo value on the valid box grown by ngv)
         if (k >= 0 && k <= nz && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV && j <= ny - 1 + ngV) {
-            const bool near = i >= -1 && i <= nx && j >= -1 && j <= ny;
+            const bool near = i >= -ngv && i <= nx - 1 + ngv && j >= -ngv && j <= ny - 1 + ngv;
             double w;
             if (near && k == 0) w = zvl;
             else if (near && k == nz && zth == EXT_DIR) w = zvh;
@@ -369,7 +388,7 @@ void apply_bcs_fast_periodic(const Params& P, const erfx_state& S, double* xv, d
         }
     });
 
-    launch_box(grown, s, [=] __device__(int i, int j, int k) {
+    launch_box(grownB, s, [=] __device__(int i, int j, int k) {
         const int ic = i < 0 ? i + nx : (i > nx - 1 ? i - nx : i);
         const int jc = j < 0 ? j + ny : (j > ny - 1 ? j - ny : j);
         const int if_ = i < 0 ? i + nx : (i > nx ? i - nx : i);
@@ -422,8 +441,8 @@ void fill_intermediate_patch(const Params& P, const erfx_state& S, double* xv, d
 // apply_bcs (Source/TimeIntegration/ERF_TI_utils.H:53-150).
 void apply_bcs(const Params& P, const erfx_state& S, double* xv, double* yv, double* zv, int ngc, int ngv,
                bool fast_only, bool synced, hipStream_t s) {
-    if (fast_only && !synced && ngc == 1 && ngv == 1 && fast_bcs_fusable(P)) {
-        apply_bcs_fast_periodic(P, S, xv, yv, zv, s);
+    if (!synced && ngc >= 1 && ngv >= 1 && fast_bcs_fusable(P)) {
+        apply_bcs_periodic(P, S, xv, yv, zv, ngc, ngv, fast_only, s);
         return;
     }
     int scomp, ncomp, ngu;
