@@ -23,6 +23,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fast_rhs_N_column_cbl256x256x128_traffic.json")
+
+
+def measured_traffic(kernel, bytes_per_launch):
+    """HBM bytes per launch of the timed kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement, if it was taken on this kernel and
+    workload (same name and algorithmic byte count); otherwise None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or abs(t.get("algorithmic_bytes_per_launch", -1) - bytes_per_launch) > 0.5:
+        return None
+    return t["traffic_bytes_per_launch"]
 
 
 def parse():
@@ -156,8 +171,9 @@ def main():
                                                                              cfg.mri_ratio, nsub),
                    "cells": cells, "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": kname.value.decode(), "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": avg_ms, "launches": kcnt.value},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kname.value.decode(), kbytes.value),
+                     "algorithmic_bytes": kbytes.value, "avg_launch_ms": avg_ms, "launches": kcnt.value},
         "wall_s": wall,
     }
     if not a.no_cpu:
