@@ -10,6 +10,8 @@ import math
 # ---- enums (include/erfx.h) -------------------------------------------------
 BC_PERIODIC, BC_SLIP_WALL, BC_NO_SLIP_WALL, BC_SYMMETRY, BC_OUTFLOW = range(5)
 ADV_CENTERED2, ADV_UPWIND3, ADV_CENTERED4, ADV_UPWIND5, ADV_CENTERED6 = range(5)
+# WENO family (scalars only): ERF_Interpolation_WENO.H, ERF_Interpolation_WENO_Z.H
+ADV_WENO3, ADV_WENOZ3, ADV_WENOMZQ3, ADV_WENO5, ADV_WENOZ5 = range(5, 10)
 MOLEC_NONE, MOLEC_CONSTANT, MOLEC_CONSTANT_ALPHA = range(3)
 LES_NONE, LES_SMAGORINSKY = range(2)
 RHO, RHOTHETA, RHOKE, RHOQKE, RHOSCALAR = range(5)
@@ -117,7 +119,7 @@ def default_config(nx, ny, nz, dx, dy, dz):
 
 def ghost_cells(cfg):
     """ERF::ComputeGhostCells (Source/ERF.H:1081-1125) + 1 for cons."""
-    wide = (ADV_UPWIND5, ADV_CENTERED6)
+    wide = (ADV_UPWIND5, ADV_CENTERED6, ADV_WENO5, ADV_WENOZ5)
     g = 3 if any(t in wide for t in (cfg.dycore_horiz_adv, cfg.dycore_vert_adv,
                                       cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv)) else 2
     return g + 1

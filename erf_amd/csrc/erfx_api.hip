@@ -64,9 +64,14 @@ static int validate(const erfx_config& c) {
     if (c.bc[2] == ERFX_BC_SYMMETRY || c.bc[2] == ERFX_BC_OUTFLOW || c.bc[5] == ERFX_BC_SYMMETRY) {
         set_error("vertical faces must be slip or no-slip walls (or an outflow top)"); return ERFX_ERR_UNSUPPORTED;
     }
-    int advs[4] = {c.dycore_horiz_adv, c.dycore_vert_adv, c.dryscal_horiz_adv, c.dryscal_vert_adv};
-    for (int a : advs)
-        if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_CENTERED6) { set_error("unsupported advection scheme"); return ERFX_ERR_UNSUPPORTED; }
+    int dyc[2] = {c.dycore_horiz_adv, c.dycore_vert_adv};
+    for (int a : dyc)
+        if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_CENTERED6) {
+            set_error("unsupported dycore advection scheme (WENO is for scalars only)"); return ERFX_ERR_UNSUPPORTED;
+        }
+    int scal[2] = {c.dryscal_horiz_adv, c.dryscal_vert_adv};
+    for (int a : scal)
+        if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_WENOZ5) { set_error("unsupported scalar advection scheme"); return ERFX_ERR_UNSUPPORTED; }
     if (c.molec_diff_type < 0 || c.molec_diff_type > 2 || c.les_type < 0 || c.les_type > 1) {
         set_error("bad diffusion choice"); return ERFX_ERR_ARG;
     }
@@ -147,7 +152,9 @@ void erfx_config_defaults(erfx_config* c) {
 
 int erfx_ghost_cells(const erfx_config* c) {
     if (!c) return -1;
-    auto wide = [](int t) { return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6; };
+    auto wide = [](int t) {
+        return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6 || t == ERFX_ADV_WENO5 || t == ERFX_ADV_WENOZ5;
+    };
     int g = (wide(c->dycore_horiz_adv) || wide(c->dycore_vert_adv) || wide(c->dryscal_horiz_adv) ||
              wide(c->dryscal_vert_adv)) ? 3 : 2;
     return g + 1;

@@ -16,6 +16,91 @@ namespace erfx {
 // Value on the lo face of point s along a line; sm3..sp2 are the values at
 // offsets -3..+2.  CENTERED2 :9-87, UPWIND3 :92-201, CENTERED4 :207-295,
 // UPWIND5 :300-395, CENTERED6 :400-480.
+// WENO family for scalars (Source/Utils/ERF_Interpolation_WENO.H: WENO3 :9-127,
+// WENO5 :132-262; ERF_Interpolation_WENO_Z.H: WENO_Z3 :9-126, WENO_MZQ3
+// :130-252, WENO_Z5 :256-389).  The stencil is passed from the upwind side.
+namespace weno {
+constexpr double eps = 1.0e-6, tol = 1.0e-12;
+__device__ __forceinline__ double w3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (2.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double w1 = g1 / ((eps + b1) * (eps + b1));
+    double w2 = g2 / ((eps + b2) * (eps + b2));
+    double wsum = w1 + w2;
+    double v1 = -sm1 + 3.0 * s;
+    double v2 = s + sp1;
+    return ((w1 * v1 + w2 * v2) / (2.0 * wsum));
+}
+__device__ __forceinline__ double z3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (2.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double t5 = fabs(b2 - b1);
+    double w1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    double w2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    double wsum = w1 + w2;
+    double v1 = -sm1 + 3.0 * s;
+    double v2 = s + sp1;
+    return ((w1 * v1 + w2 * v2) / (2.0 * wsum));
+}
+__device__ __forceinline__ double mzq3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (1.0 / 3.0), g3 = (1.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double b3 = ((13.0 / 12.0) * ((sm1 - 2.0 * s + sp1) * (sm1 - 2.0 * s + sp1))) + (((sp1 - sm1) * (sp1 - sm1)) / 4.0);
+    double t5 = (fabs(b3 - b1) + fabs(b3 - b2)) / 32.0;
+    double a1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    double a2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    double a3 = g3 * (1.0 + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    double asum = a1 + a2 + a3;
+    double w1 = a1 / asum;
+    double w2 = a2 / asum;
+    double w3_ = a3 / asum;
+    double v1 = (-sm1 + 3.0 * s) / 2.0;
+    double v2 = (s + sp1) / 2.0;
+    double v3 = (-sm1 + 5.0 * s + 2.0 * sp1) / 6.0;
+    return ((w3_ / g3) * (v3 - g1 * v1 - g2 * v2) + w1 * v1 + w2 * v2);
+}
+__device__ __forceinline__ double w5(double sm2, double sm1, double s, double sp1, double sp2, bool z) {
+    const double c1 = (13.0 / 12.0);
+    const double g1 = (1.0 / 10.0), g2 = (3.0 / 5.0), g3 = (3.0 / 10.0);
+    double b1 = c1 * (sm2 - 2.0 * sm1 + s) * (sm2 - 2.0 * sm1 + s) +
+                0.25 * (sm2 - 4.0 * sm1 + 3.0 * s) * (sm2 - 4.0 * sm1 + 3.0 * s);
+    double b2 = c1 * (sm1 - 2.0 * s + sp1) * (sm1 - 2.0 * s + sp1) + 0.25 * (sm1 - sp1) * (sm1 - sp1);
+    double b3 = c1 * (s - 2.0 * sp1 + sp2) * (s - 2.0 * sp1 + sp2) +
+                0.25 * (3.0 * s - 4.0 * sp1 + sp2) * (3.0 * s - 4.0 * sp1 + sp2);
+    double w1, w2, w3_;
+    if (z) {
+        double t5 = fabs(b3 - b1);
+        w1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+        w2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+        w3_ = g3 * (1.0 + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    } else {
+        w1 = g1 / ((eps + b1) * (eps + b1));
+        w2 = g2 / ((eps + b2) * (eps + b2));
+        w3_ = g3 / ((eps + b3) * (eps + b3));
+    }
+    double wsum = w1 + w2 + w3_;
+    double v1 = 2.0 * sm2 - 7.0 * sm1 + 11.0 * s;
+    double v2 = -sm1 + 5.0 * s + 2.0 * sp1;
+    double v3 = 2.0 * s + 5.0 * sp1 - sp2;
+    return ((w1 * v1 + w2 * v2 + w3_ * v3) / (6.0 * wsum));
+}
+__device__ __forceinline__ double face(int t, double sp2, double sp1, double s, double sm1, double sm2, double sm3,
+                                       double upw) {
+    if (!(upw > tol) && !(upw < -tol)) return 0.5 * (s + sm1);
+    const bool pos = upw > tol;
+    if (t == ERFX_ADV_WENO3) return pos ? w3(sm2, sm1, s) : w3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENOZ3) return pos ? z3(sm2, sm1, s) : z3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENOMZQ3) return pos ? mzq3(sm2, sm1, s) : mzq3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENO5) return pos ? w5(sm3, sm2, sm1, s, sp1, false) : w5(sp2, sp1, s, sm1, sm2, false);
+    return pos ? w5(sm3, sm2, sm1, s, sp1, true) : w5(sp2, sp1, s, sm1, sm2, true);
+}
+}  // namespace weno
+
+__host__ __device__ __forceinline__ bool is_weno(int t) { return t >= ERFX_ADV_WENO3 && t <= ERFX_ADV_WENOZ5; }
+
 __device__ __forceinline__ double interp_line(int t, double sp2, double sp1, double s, double sm1, double sm2,
                                               double sm3, double upw, double upw_frac) {
     if (t == ERFX_ADV_CENTERED2) {
@@ -56,6 +141,15 @@ __device__ __forceinline__ double interp_at(int t, const double* f, long q, long
     double sp2 = wide_stencil(t) ? f[q + 2 * st] : 0.;
     double sm3 = wide_stencil(t) ? f[q - 3 * st] : 0.;
     return interp_line(t, sp2, f[q + st], f[q], f[q - st], f[q - 2 * st], sm3, upw, fr);
+}
+
+// scalar interpolation: the WENO family or the centred/upwind family
+__device__ __forceinline__ double interp_scalar_at(int t, const double* f, long q, long st, double upw, double fr) {
+    if (is_weno(t)) {
+        double sp2 = f[q + 2 * st], sm3 = f[q - 3 * st];
+        return weno::face(t, sp2, f[q + st], f[q], f[q - st], f[q - 2 * st], sm3, upw);
+    }
+    return interp_at(t, f, q, st, upw, fr);
 }
 
 // UPWINDALL::InterpolateInZ: wall-reduced vertical interpolation for w.
@@ -563,7 +657,9 @@ void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& r
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
     const int ncons = P.cfg.ncons;
-    const int ht = P.cfg.dryscal_horiz_adv, vt = P.cfg.dryscal_vert_adv;
+    const int ht = P.cfg.dryscal_horiz_adv;
+    // a WENO horizontal type is used in z as well (ERF_AdvectionSrcForState.cpp:228-262)
+    const int vt = is_weno(ht) ? ht : P.cfg.dryscal_vert_adv;
     const double hf = P.cfg.dryscal_horiz_upw_frac, vf = P.cfg.dryscal_vert_upw_frac;
     const bool c2c2 = (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2);
     const bool use_diff = P.use_diff, consA = P.consA, turb = P.use_turb;
@@ -592,12 +688,12 @@ void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& r
             gz0 = fz0 * (0.5 * (ps[q] + ps[q - sz]));
             gz1 = fz1 * (0.5 * (ps[q + sz] + ps[q]));
         } else {
-            gx0 = fx0 * interp_at(ht, ps, q, 1, fx0, hf);
-            gx1 = fx1 * interp_at(ht, ps, q + 1, 1, fx1, hf);
-            gy0 = fy0 * interp_at(ht, ps, q, sy, fy0, hf);
-            gy1 = fy1 * interp_at(ht, ps, q + sy, sy, fy1, hf);
-            gz0 = fz0 * interp_at(vt, ps, q, sz, fz0, vf);
-            gz1 = fz1 * interp_at(vt, ps, q + sz, sz, fz1, vf);
+            gx0 = fx0 * interp_scalar_at(ht, ps, q, 1, fx0, hf);
+            gx1 = fx1 * interp_scalar_at(ht, ps, q + 1, 1, fx1, hf);
+            gy0 = fy0 * interp_scalar_at(ht, ps, q, sy, fy0, hf);
+            gy1 = fy1 * interp_scalar_at(ht, ps, q + sy, sy, fy1, hf);
+            gz0 = fz0 * interp_scalar_at(vt, ps, q, sz, fz0, vf);
+            gz1 = fz1 * interp_scalar_at(vt, ps, q + sz, sz, fz1, vf);
         }
         double r = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
         if (use_diff) {

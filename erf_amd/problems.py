@@ -130,14 +130,16 @@ def _fill_cons_from_theta(cfg, L, f, theta3):
     cons[C.RHO][sl] = rt[sl] / theta3[sl]
 
 
-def rising_thermal(nx=64, ny=64, nz=64, seed=0, adv=C.ADV_CENTERED2):
+def rising_thermal(nx=64, ny=64, nz=64, seed=0, adv=C.ADV_CENTERED2, scal_adv=None):
     """Dry rising thermal: 20x20x10 km, periodic x/y, slip walls, theta = 300 K
     isentropic, +2 K cos^2 bubble of radius 2 km at (10, 10, 2) km, uniform
     noise of +-0.01 m/s on u, v, w; centred 2nd-order advection, no diffusion,
-    dt = 1 s, MRI ratio 6."""
+    dt = 1 s, MRI ratio 6.  `scal_adv` picks another scheme (e.g. WENO) for the
+    passive scalars."""
     cfg = C.default_config(nx, ny, nz, 20000.0 / nx, 20000.0 / ny, 10000.0 / nz)
     cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
-    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    sa = adv if scal_adv is None else scal_adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = sa
     cfg.mri_ratio = 6
     L, f = empty_fields(cfg)
     _base_state(cfg, L, f, np.full(nz, 300.0))

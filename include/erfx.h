@@ -59,13 +59,20 @@ enum {
     ERFX_BC_OUTFLOW      = 4
 };
 
-/* ---- advection schemes (AdvType, Source/DataStructs/ERF_AdvStruct.H) ---- */
+/* ---- advection schemes (AdvType, Source/DataStructs/ERF_AdvStruct.H) ----
+ * The WENO family (ERF_Interpolation_WENO.H, ERF_Interpolation_WENO_Z.H) is
+ * for scalars only; ERF applies the horizontal WENO type in z as well. */
 enum {
     ERFX_ADV_CENTERED2 = 0,
     ERFX_ADV_UPWIND3   = 1,
     ERFX_ADV_CENTERED4 = 2,
     ERFX_ADV_UPWIND5   = 3,
-    ERFX_ADV_CENTERED6 = 4
+    ERFX_ADV_CENTERED6 = 4,
+    ERFX_ADV_WENO3     = 5,
+    ERFX_ADV_WENOZ3    = 6,
+    ERFX_ADV_WENOMZQ3  = 7,
+    ERFX_ADV_WENO5     = 8,
+    ERFX_ADV_WENOZ5    = 9
 };
 
 /* ---- diffusion (MolecDiffType / LESType, ERF_DiffStruct.H, ERF_TurbStruct.H) */

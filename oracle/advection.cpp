@@ -7,8 +7,101 @@ namespace orc {
 // Face interpolation of the centred/upwind family, lo face of cell i along a
 // line (sm3..sp2 = values at i-3..i+2).  ERF_Interpolation_UPW.H: CENTERED2
 // :9-87, UPWIND3 :92-201, CENTERED4 :207-295, UPWIND5 :300-395, CENTERED6 :400-480.
+// WENO family (Source/Utils/ERF_Interpolation_WENO.H: WENO3 :9-127, WENO5 :132-262;
+// ERF_Interpolation_WENO_Z.H: WENO_Z3 :9-126, WENO_MZQ3 :130-252, WENO_Z5 :256-389).
+// Evaluate() takes the stencil ordered from the upwind side.
+namespace weno {
+constexpr double eps = 1.0e-6, tol = 1.0e-12;
+static inline double w3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (2.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double w1 = g1 / ((eps + b1) * (eps + b1));
+    double w2 = g2 / ((eps + b2) * (eps + b2));
+    double wsum = w1 + w2;
+    double v1 = -sm1 + 3.0 * s;
+    double v2 = s + sp1;
+    return ((w1 * v1 + w2 * v2) / (2.0 * wsum));
+}
+static inline double z3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (2.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double t5 = std::abs(b2 - b1);
+    double w1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    double w2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    double wsum = w1 + w2;
+    double v1 = -sm1 + 3.0 * s;
+    double v2 = s + sp1;
+    return ((w1 * v1 + w2 * v2) / (2.0 * wsum));
+}
+static inline double mzq3(double sm1, double s, double sp1) {
+    const double g1 = (1.0 / 3.0), g2 = (1.0 / 3.0), g3 = (1.0 / 3.0);
+    double b1 = (s - sm1) * (s - sm1);
+    double b2 = (sp1 - s) * (sp1 - s);
+    double b3 = ((13.0 / 12.0) * ((sm1 - 2.0 * s + sp1) * (sm1 - 2.0 * s + sp1))) + (((sp1 - sm1) * (sp1 - sm1)) / 4.0);
+    double t5 = (std::abs(b3 - b1) + std::abs(b3 - b2)) / 32.0;
+    double a1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    double a2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    double a3 = g3 * (1.0 + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    double asum = a1 + a2 + a3;
+    double w1 = a1 / asum;
+    double w2 = a2 / asum;
+    double w3_ = a3 / asum;
+    double v1 = (-sm1 + 3.0 * s) / 2.0;
+    double v2 = (s + sp1) / 2.0;
+    double v3 = (-sm1 + 5.0 * s + 2.0 * sp1) / 6.0;
+    return ((w3_ / g3) * (v3 - g1 * v1 - g2 * v2) + w1 * v1 + w2 * v2);
+}
+static inline void beta5(double sm2, double sm1, double s, double sp1, double sp2, double& b1, double& b2, double& b3) {
+    const double c1 = (13.0 / 12.0);
+    b1 = c1 * (sm2 - 2.0 * sm1 + s) * (sm2 - 2.0 * sm1 + s) +
+         0.25 * (sm2 - 4.0 * sm1 + 3.0 * s) * (sm2 - 4.0 * sm1 + 3.0 * s);
+    b2 = c1 * (sm1 - 2.0 * s + sp1) * (sm1 - 2.0 * s + sp1) + 0.25 * (sm1 - sp1) * (sm1 - sp1);
+    b3 = c1 * (s - 2.0 * sp1 + sp2) * (s - 2.0 * sp1 + sp2) +
+         0.25 * (3.0 * s - 4.0 * sp1 + sp2) * (3.0 * s - 4.0 * sp1 + sp2);
+}
+static inline double w5(double sm2, double sm1, double s, double sp1, double sp2, bool z) {
+    const double g1 = (1.0 / 10.0), g2 = (3.0 / 5.0), g3 = (3.0 / 10.0);
+    double b1, b2, b3;
+    beta5(sm2, sm1, s, sp1, sp2, b1, b2, b3);
+    double w1, w2, w3_;
+    if (z) {
+        double t5 = std::abs(b3 - b1);
+        w1 = g1 * (1.0 + (t5 * t5) / ((eps + b1) * (eps + b1)));
+        w2 = g2 * (1.0 + (t5 * t5) / ((eps + b2) * (eps + b2)));
+        w3_ = g3 * (1.0 + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    } else {
+        w1 = g1 / ((eps + b1) * (eps + b1));
+        w2 = g2 / ((eps + b2) * (eps + b2));
+        w3_ = g3 / ((eps + b3) * (eps + b3));
+    }
+    double wsum = w1 + w2 + w3_;
+    double v1 = 2.0 * sm2 - 7.0 * sm1 + 11.0 * s;
+    double v2 = -sm1 + 5.0 * s + 2.0 * sp1;
+    double v3 = 2.0 * s + 5.0 * sp1 - sp2;
+    return ((w1 * v1 + w2 * v2 + w3_ * v3) / (6.0 * wsum));
+}
+// lo face of s; sm3..sp2 at offsets -3..+2
+static inline double face(int type, double sp2, double sp1, double s, double sm1, double sm2, double sm3, double upw) {
+    if (!(upw > tol) && !(upw < -tol)) return 0.5 * (s + sm1);
+    const bool pos = upw > tol;
+    switch (type) {
+        case ERFX_ADV_WENO3: return pos ? w3(sm2, sm1, s) : w3(sp1, s, sm1);
+        case ERFX_ADV_WENOZ3: return pos ? z3(sm2, sm1, s) : z3(sp1, s, sm1);
+        case ERFX_ADV_WENOMZQ3: return pos ? mzq3(sm2, sm1, s) : mzq3(sp1, s, sm1);
+        case ERFX_ADV_WENO5: return pos ? w5(sm3, sm2, sm1, s, sp1, false) : w5(sp2, sp1, s, sm1, sm2, false);
+        case ERFX_ADV_WENOZ5: return pos ? w5(sm3, sm2, sm1, s, sp1, true) : w5(sp2, sp1, s, sm1, sm2, true);
+    }
+    throw std::runtime_error("bad WENO type");
+}
+}  // namespace weno
+
+static inline bool is_weno(int t) { return t >= ERFX_ADV_WENO3 && t <= ERFX_ADV_WENOZ5; }
+
 static inline double interp_line(int type, double sp2, double sp1, double s, double sm1,
                                  double sm2, double sm3, double upw_in, double upw_frac) {
+    if (is_weno(type)) return weno::face(type, sp2, sp1, s, sm1, sm2, sm3, upw_in);
     switch (type) {
         case ERFX_ADV_CENTERED2: {
             double a1 = (s + sm1);
@@ -50,7 +143,9 @@ double face_interp(int type, double sp2, double sp1, double s, double sm1, doubl
     return interp_line(type, sp2, sp1, s, sm1, sm2, sm3, upw_in, upw_frac);
 }
 
-static inline bool wide(int t) { return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6; }
+static inline bool wide(int t) {
+    return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6 || t == ERFX_ADV_WENO5 || t == ERFX_ADV_WENOZ5;
+}
 static inline bool narrow(int t) { return t == ERFX_ADV_CENTERED2; }
 
 static inline double interp_x(int t, Arr q, int i, int j, int k, int n, double upw, double fr) {
@@ -74,6 +169,8 @@ static inline double interp_z(int t, Arr q, int i, int j, int k, int n, double u
 void Level::advection_src_scalars(const Box& bx, int icomp, int ncomp, Arr ax, Arr ay, Arr az,
                                   Arr cprim, Arr rhs, int hadv, int vadv, double hfrac, double vfrac) {
     const int nx = bx.hi[0], ny = bx.hi[1], nz = bx.hi[2];
+    // a WENO horizontal type is used in all three directions (:228-262)
+    if (is_weno(hadv)) vadv = hadv;
     Field fx, fy, fz;
     fx.alloc(L, 1); fy.alloc(L, 1); fz.alloc(L, 1);
     Arr Fx = fx.a(L), Fy = fy.a(L), Fz = fz.a(L);

@@ -84,3 +84,18 @@ def test_explicit_substepping():
     cfg.implicit_substepping = 0
     cfg.mri_ratio = 12
     _compare(cfg, f, 0.5, 2, 1e-11)
+
+
+@pytest.mark.parametrize("adv", [C.ADV_WENO3, C.ADV_WENOZ3, C.ADV_WENOMZQ3, C.ADV_WENO5, C.ADV_WENOZ5])
+def test_weno_scalar_advection(adv):
+    """Passive scalar (RhoScalar) advected with the WENO family; a tracer blob
+    with a sharp edge so that the nonlinear weights are exercised."""
+    cfg, f = P.rising_thermal(16, 16, 16, seed=17, adv=C.ADV_UPWIND3, scal_adv=adv)
+    L = C.layout_for(cfg)
+    ng = L.ng
+    rho = f[C.F_CONS][C.RHO]
+    kk, jj, ii = np.meshgrid(np.arange(L.shape3()[0]), np.arange(L.shape3()[1]), np.arange(L.shape3()[2]),
+                             indexing="ij")
+    blob = ((ii - ng - 8) ** 2 + (jj - ng - 8) ** 2 + (kk - ng - 6) ** 2 < 16).astype(float)
+    f[C.F_CONS][C.RHOSCALAR] = rho * blob
+    _compare(cfg, f, 1.0, 2, 1e-11, exact=True)

@@ -218,3 +218,77 @@ def test_mirror_symmetry_of_centred_bubble():
     umax = np.abs(u).max()
     assert umax > 1e-6
     assert np.abs(u + u[:, :, ::-1]).max() < 1e-10 * max(umax, 1.0)
+
+
+# ---- WENO family (scalars) against the published formulas -------------------------------
+# Jiang & Shu (1996) WENO5 / WENO3, Borges et al. (2008) WENO-Z; reconstruction at the
+# face between v[i] and v[i+1] from the upwind side, written from the papers.
+
+def _js5(v, z):
+    vm2, vm1, v0, vp1, vp2 = v
+    q = [(2 * vm2 - 7 * vm1 + 11 * v0) / 6, (-vm1 + 5 * v0 + 2 * vp1) / 6, (2 * v0 + 5 * vp1 - vp2) / 6]
+    b = [13 / 12 * (vm2 - 2 * vm1 + v0) ** 2 + 0.25 * (vm2 - 4 * vm1 + 3 * v0) ** 2,
+         13 / 12 * (vm1 - 2 * v0 + vp1) ** 2 + 0.25 * (vm1 - vp1) ** 2,
+         13 / 12 * (v0 - 2 * vp1 + vp2) ** 2 + 0.25 * (3 * v0 - 4 * vp1 + vp2) ** 2]
+    d = [0.1, 0.6, 0.3]
+    eps = 1e-6
+    if z:
+        tau = abs(b[0] - b[2])
+        a = [d[k] * (1 + (tau / (eps + b[k])) ** 2) for k in range(3)]
+    else:
+        a = [d[k] / (eps + b[k]) ** 2 for k in range(3)]
+    return sum(a[k] * q[k] for k in range(3)) / sum(a)
+
+
+def _js3(v, z):
+    vm1, v0, vp1 = v
+    q = [(-vm1 + 3 * v0) / 2, (v0 + vp1) / 2]
+    b = [(v0 - vm1) ** 2, (vp1 - v0) ** 2]
+    d = [1 / 3, 2 / 3]
+    eps = 1e-6
+    if z:
+        tau = abs(b[1] - b[0])
+        a = [d[k] * (1 + (tau / (eps + b[k])) ** 2) for k in range(2)]
+    else:
+        a = [d[k] / (eps + b[k]) ** 2 for k in range(2)]
+    return sum(a[k] * q[k] for k in range(2)) / sum(a)
+
+
+@pytest.mark.parametrize("type_,ref", [(C.ADV_WENO5, lambda v: _js5(v, False)), (C.ADV_WENOZ5, lambda v: _js5(v, True)),
+                                       (C.ADV_WENO3, lambda v: _js3(v[1:4], False)),
+                                       (C.ADV_WENOZ3, lambda v: _js3(v[1:4], True))])
+def test_weno_against_published_formulas(type_, ref):
+    """s[0..5] = cells i-3..i+2, lo face of cell i: with positive velocity the
+    upwind stencil is cells i-3..i+1 (i-2..i for the 3rd order), with negative
+    velocity the mirror image."""
+    rng = np.random.default_rng(5)
+    for trial in range(400):
+        scale = 10.0 ** rng.uniform(-3, 2)
+        s = rng.uniform(-1, 1, 6) * scale + (rng.uniform(-1, 1) if trial % 2 else 0.0)
+        if trial % 5 == 0:
+            s = np.sin(0.3 * np.arange(6) + trial) * scale          # smooth data
+        got_p = _face(type_, s, 1.0)
+        got_m = _face(type_, s, -1.0)
+        ref_p = ref(list(s[0:5]))                                   # i-3 .. i+1
+        ref_m = ref(list(s[5:0:-1]))                                # i+2 .. i-2 (mirrored)
+        tol = 1e-13 * max(1.0, np.abs(s).max())
+        assert abs(got_p - ref_p) < tol, (trial, got_p, ref_p)
+        assert abs(got_m - ref_m) < tol, (trial, got_m, ref_m)
+        assert _face(type_, s, 0.0) == 0.5 * (s[3] + s[2])          # no flow: centred average
+
+
+@pytest.mark.parametrize("type_,degree", [(C.ADV_WENO3, 1), (C.ADV_WENOZ3, 1), (C.ADV_WENOMZQ3, 1),
+                                          (C.ADV_WENO5, 2), (C.ADV_WENOZ5, 2)])
+def test_weno_exact_on_low_degree_polynomials(type_, degree):
+    """Each candidate stencil is exact up to the scheme's sub-stencil degree, so
+    any convex weighting is too; a step is reconstructed without overshoot."""
+    centres = np.arange(-3, 3) + 0.5
+    x0 = 0.3
+    for p in range(degree + 1):
+        avg = (((centres + 0.5 + x0) ** (p + 1)) - ((centres - 0.5 + x0) ** (p + 1))) / (p + 1)
+        for upw in (1.0, -1.0):
+            assert abs(_face(type_, avg, upw) - x0 ** p) < 1e-12
+    step = np.array([0.0, 0.0, 0.0, 1.0, 1.0, 1.0])
+    for upw in (1.0, -1.0):
+        v = _face(type_, step, upw)
+        assert -1e-12 <= v <= 1.0 + 1e-12
