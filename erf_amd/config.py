@@ -65,6 +65,9 @@ class Config(ctypes.Structure):
         ("mri_ratio", ctypes.c_int),
         ("force_stage1_single_substep", ctypes.c_int),
         ("implicit_substepping", ctypes.c_int),
+        ("rayleigh_damp_U", ctypes.c_int), ("rayleigh_damp_V", ctypes.c_int),
+        ("rayleigh_damp_W", ctypes.c_int), ("rayleigh_damp_T", ctypes.c_int),
+        ("rayleigh_dampcoef", ctypes.c_double), ("rayleigh_zdamp", ctypes.c_double),
     ]
 
 
@@ -114,6 +117,9 @@ def default_config(nx, ny, nz, dx, dy, dz):
     c.mri_ratio = 6
     c.force_stage1_single_substep = 1
     c.implicit_substepping = 1
+    c.rayleigh_damp_U = c.rayleigh_damp_V = c.rayleigh_damp_W = c.rayleigh_damp_T = 0
+    c.rayleigh_dampcoef = 0.2
+    c.rayleigh_zdamp = 500.0
     return c
 
 

@@ -76,6 +76,9 @@ static int validate(const erfx_config& c) {
         set_error("bad diffusion choice"); return ERFX_ERR_ARG;
     }
     if (c.les_type == ERFX_LES_SMAGORINSKY && !(c.Cs > 0)) { set_error("Smagorinsky needs Cs > 0"); return ERFX_ERR_ARG; }
+    if ((c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T) && !(c.rayleigh_zdamp > 0)) {
+        set_error("Rayleigh damping needs rayleigh_zdamp > 0"); return ERFX_ERR_ARG;
+    }
     if (!(c.mri_ratio > 1 && c.mri_ratio % 2 == 0)) { set_error("mri_ratio must be even and > 1 (ERF_MRI.H:205-207)"); return ERFX_ERR_ARG; }
     if (!c.force_stage1_single_substep && c.mri_ratio % 3 != 0) {
         set_error("without the single first-stage substep the ratio must be a multiple of 6"); return ERFX_ERR_ARG;
@@ -96,6 +99,7 @@ struct erfx_ctx {
     const double* pi0 = nullptr;
     FastWork fw{};
     SlowWork sw{};
+    double* ray = nullptr;    // Rayleigh profiles and per-level factors (RayleighTable layout)
     std::vector<void*> allocs;
 };
 
@@ -148,6 +152,8 @@ void erfx_config_defaults(erfx_config* c) {
     c->mri_ratio = 6;
     c->force_stage1_single_substep = 1;
     c->implicit_substepping = 1;
+    c->rayleigh_dampcoef = 0.2;
+    c->rayleigh_zdamp = 500.0;
 }
 
 int erfx_ghost_cells(const erfx_config* c) {
@@ -225,10 +231,58 @@ int erfx_set_base_state(erfx_ctx* c, const double* r0, const double* p0, const d
     return ERFX_OK;
 }
 
+// Per-level Rayleigh table: ubar, vbar, wbar, thetabar, then the factor
+// dampcoef * sin^2(pi/2 zfrac) and a 0/1 activity flag at cell centres and at
+// z-faces, each nz+1 long.  The factors depend on k only, so they are computed
+// here on the host with the C library's sin, exactly as the reference
+// evaluates them per cell (ERF_make_sources.cpp:213-222, ERF_make_mom_sources.cpp:255-296).
+int erfx_set_rayleigh(erfx_ctx* c, const double* ub, const double* vb, const double* wb, const double* tb) {
+    if (!c || !ub || !vb || !wb || !tb) { set_error("null argument"); return ERFX_ERR_ARG; }
+    const erfx_config& cf = c->P.cfg;
+    const int n1 = cf.nz + 1;
+    std::vector<double> h((size_t)RAY_NFIELDS * n1, 0.0);
+    for (int k = 0; k < n1; ++k) {
+        h[RAY_UBAR * n1 + k] = ub[k];
+        h[RAY_VBAR * n1 + k] = vb[k];
+        h[RAY_WBAR * n1 + k] = wb[k];
+        h[RAY_THETABAR * n1 + k] = tb[k];
+    }
+    const double zlo = cf.prob_lo[2], dz = cf.dz;
+    const double ztop = zlo + cf.nz * dz;
+    const double zdamp = cf.rayleigh_zdamp, dampcoef = cf.rayleigh_dampcoef;
+    for (int k = 0; k < n1; ++k) {
+        double zcc = zlo + (k + 0.5) * dz;
+        double zfrac = 1 - (ztop - zcc) / zdamp;
+        if (zfrac > 0) {
+            double sinefac = std::sin(PIoTwo * zfrac);
+            h[RAY_FAC_CC * n1 + k] = dampcoef * sinefac * sinefac;
+            h[RAY_ACT_CC * n1 + k] = 1.0;
+        }
+        double zstag = zlo + k * dz;
+        double zfrac_n = 1 - (ztop - zstag) / zdamp;
+        if (zfrac_n > 0) {
+            double sinefac = std::sin(PIoTwo * zfrac_n);
+            h[RAY_FAC_ND * n1 + k] = dampcoef * sinefac * sinefac;
+            h[RAY_ACT_ND * n1 + k] = 1.0;
+        }
+    }
+    if (!c->ray) {
+        int rc = dalloc(c->allocs, &c->ray, h.size());
+        if (rc) return rc;
+    }
+    ERFX_HIP(hipMemcpy(c->ray, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->P.ray = c->ray;
+    return ERFX_OK;
+}
+
 static int launch_status() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return ERFX_OK;
+}
+
+static bool rayleigh_on(const erfx_config& c) {
+    return c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T;
 }
 
 static bool bad_state(const erfx_state* s) { return !s || !s->cons || !s->xmom || !s->ymom || !s->zmom; }
@@ -289,6 +343,7 @@ int erfx_slow_rhs_pre(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, con
     }
     if (c->P.use_turb && !eddy) { set_error("LES model needs the eddy viscosity"); return ERFX_ERR_ARG; }
     if (!c->r0) { set_error("base state not set"); return ERFX_ERR_STATE; }
+    if (rayleigh_on(c->P.cfg) && !c->ray) { set_error("Rayleigh damping profiles not set (erfx_set_rayleigh)"); return ERFX_ERR_STATE; }
     slow_rhs_pre(c->P, c->sw, nrk, slow_dt, *rhs, *data, prim, *scratch, xv, yv, zv, eddy, c->r0, c->p0, c->stream);
     return launch_status();
 }
@@ -506,6 +561,9 @@ static int sim_one_step(erfx_sim* s, double time, double dt) {
 
 int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
     if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (rayleigh_on(s->ctx->P.cfg) && !s->ctx->ray) {
+        set_error("Rayleigh damping profiles not set (erfx_set_rayleigh on the sim's context)"); return ERFX_ERR_STATE;
+    }
     if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
     for (int n = 0; n < nsteps; ++n) {
         int rc = sim_one_step(s, t0 + n * dt, dt);

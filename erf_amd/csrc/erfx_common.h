@@ -18,6 +18,7 @@ namespace erfx {
 
 // ---- physical constants (Source/ERF_Constants.H:7-20) ---------------------
 constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double PIoTwo = PI / 2.0;
 constexpr double R_d = 287.0;
 constexpr double R_v = 461.505;
 constexpr double Cp_d = 1004.5;
@@ -131,7 +132,11 @@ struct Params {
     bool periodic[3];
     bool use_diff, use_turb, consA;
     KTimer* timer = nullptr;  // optional launch timing (host pointer, never read on the device)
+    const double* ray = nullptr;  // Rayleigh table (device), RAY_* blocks of nz+1 values
 };
+
+// blocks of the Rayleigh table
+enum { RAY_UBAR = 0, RAY_VBAR, RAY_WBAR, RAY_THETABAR, RAY_FAC_CC, RAY_FAC_ND, RAY_ACT_CC, RAY_ACT_ND, RAY_NFIELDS };
 
 void make_params(const erfx_config& cfg, const erfx_layout& lay, Params& P);
 

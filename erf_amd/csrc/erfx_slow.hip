@@ -415,6 +415,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const double sflux_val = P.cfg.surface_heat_flux;
     double* cr0 = rhs.cons;
     double* cr1 = rhs.cons + ne;
+    const double* ray = P.ray;
+    const int n1 = nz + 1;
+    const bool rayT = P.cfg.rayleigh_damp_T && ray;
+    const bool rayU = P.cfg.rayleigh_damp_U && ray, rayV = P.cfg.rayleigh_damp_V && ray;
+    const bool rayW = P.cfg.rayleigh_damp_W && ray;
     const double* ru = data.xmom;
     const double* rv = data.ymom;
     const double* rw = data.zmom;
@@ -449,6 +454,14 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             double dfz1 = diff_flux(c, prim, mu_th_v, q + sz, sz, dzi, ae_theta, consA, turb);
             r1 -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
         }
+        // cell source of rho*theta (make_sources: Rayleigh damping, ERF_make_sources.cpp:203-225),
+        // added the way ERF adds cc_src (ERF_slow_rhs_pre.cpp:521-526)
+        double src = 0.0;
+        if (rayT && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+            double theta = prim[q];
+            src -= ray[RAY_FAC_CC * n1 + k] * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
+        }
+        r1 += src;
         cr1[q] = r1;
     });
 
@@ -516,6 +529,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 double rho_w_loc = 0.25 * (rw[q + sz] + rw[q] + rw[q - sy + sz] + rw[q - sy]);
                 xs += cf * (rho_v_loc * sinphi - rho_w_loc * cosphi);
             }
+            if (rayU && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+                double rho_on_u_face = 0.5 * (cons[q] + cons[q - 1]);
+                double uu = ru[q] / rho_on_u_face;
+                xs -= ray[RAY_FAC_CC * n1 + k] * (uu - ray[RAY_UBAR * n1 + k]) * rho_on_u_face;
+            }
             xs += 0.5 * (cons[q] + cons[q - 1]) * g0;
             double gpx = dxi * (pp[q] - pp[q - 1]);
             gpx *= 1.0;
@@ -560,6 +578,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             if (cor) {
                 double rho_u_loc = 0.25 * (ru[q + 1] + ru[q] + ru[q + 1 - sy] + ru[q - sy]);
                 ys += -cf * rho_u_loc * sinphi;
+            }
+            if (rayV && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+                double rho_on_v_face = 0.5 * (cons[q] + cons[q - sy]);
+                double vv = rv[q] / rho_on_v_face;
+                ys -= ray[RAY_FAC_CC * n1 + k] * (vv - ray[RAY_VBAR * n1 + k]) * rho_on_v_face;
             }
             ys += 0.5 * (cons[q] + cons[q - sy]) * g1;
             double gpy = dyi * (pp[q] - pp[q - sy]);
@@ -636,6 +659,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             if (cor) {
                 double rho_u_loc = 0.25 * (ru[q + 1] + ru[q] + ru[q + 1 - sz] + ru[q - sz]);
                 zs += cf * rho_u_loc * cosphi;
+            }
+            if (rayW && ray[RAY_ACT_ND * n1 + k] != 0.0) {
+                double rho_on_w_face = 0.5 * (cons[q] + cons[q - sz]);
+                double ww = rw[q] / rho_on_w_face;
+                zs -= ray[RAY_FAC_ND * n1 + k] * (ww - ray[RAY_WBAR * n1 + k]) * rho_on_w_face;
             }
             zs += 0.5 * (cons[q] + cons[q - sz]) * g2;
             double met_h_zeta = 1;

@@ -34,6 +34,7 @@ SIGNATURES = {
     "erfx_ctx_destroy": (_I, [_P]),
     "erfx_ctx_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
     "erfx_set_base_state": (_I, [_P, _P, _P, _P]),
+    "erfx_set_rayleigh": (_I, [_P, _P, _P, _P, _P]),
     "erfx_cons_to_prim": (_I, [_P, _P, _P, _P, _I]),
     "erfx_apply_bcs": (_I, [_P, ctypes.POINTER(State), _P, _P, _P, _I, _I, _I, _I]),
     "erfx_make_fast_coeffs": (_I, [_P, _P, ctypes.POINTER(State), _P, _P, _D, _D]),
@@ -139,6 +140,14 @@ class Sim:
     def state(self, s):
         """erfx_state of integrator buffer s (config.S_OLD ... config.F_SLOW)."""
         return State(*(self.ptr(C.state_id(s, c)) for c in range(4)))
+
+    def set_rayleigh(self, ubar, vbar, wbar, thetabar):
+        """Rayleigh reference profiles (nz+1 values each) on the sim's context."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ubar, vbar, wbar, thetabar)]
+        for a in arrs:
+            if a.size != self.cfg.nz + 1:
+                raise ValueError("Rayleigh profiles need nz+1 values")
+        check(lib().erfx_set_rayleigh(self.ctx, *(a.ctypes.data for a in arrs)))
 
     @property
     def ctx(self):

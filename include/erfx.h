@@ -123,6 +123,15 @@ typedef struct erfx_config {
     int    mri_ratio;             /* fixed_mri_dt_ratio, even, > 1                   */
     int    force_stage1_single_substep;
     int    implicit_substepping;  /* 1: beta_s = 0.1, 0: beta_s = -1 (explicit)      */
+
+    /* Rayleigh damping layer below the domain top (ERF_DataStruct.H:567-573;
+     * ERF_make_sources.cpp:203-225, ERF_make_mom_sources.cpp:247-296): rate
+     * dampcoef * sin^2(pi/2 * zfrac), zfrac = 1 - (ztop - z) / zdamp, towards
+     * the profiles given to erfx_set_rayleigh; ztop is the domain top
+     * (ERF_init_rayleigh.cpp:17). */
+    int    rayleigh_damp_U, rayleigh_damp_V, rayleigh_damp_W, rayleigh_damp_T;
+    double rayleigh_dampcoef;     /* 1/s, default 0.2                                */
+    double rayleigh_zdamp;        /* m, default 500                                  */
 } erfx_config;
 
 /* ---- layout shared by every field of a context ------------------------- */
@@ -162,6 +171,13 @@ int erfx_ctx_layout(const erfx_ctx* ctx, erfx_layout* out);
 /* Base state {r0, p0, pi0} on cells with ghosts (BaseState, ERF_IndexDefines.H:63-71).
  * The pointers are kept and read by later calls. */
 int erfx_set_base_state(erfx_ctx* ctx, const double* r0, const double* p0, const double* pi0);
+
+/* Reference profiles of the Rayleigh damping (ERF's Rayleigh::ubar, vbar,
+ * wbar, thetabar; ERF::initRayleigh, ERF_init_rayleigh.cpp:14-45): host arrays
+ * of nz+1 values indexed by k (cell k for ubar, vbar and thetabar, z-face k for
+ * wbar), copied.  Needed when any rayleigh_damp_* flag is set. */
+int erfx_set_rayleigh(erfx_ctx* ctx, const double* ubar, const double* vbar, const double* wbar,
+                      const double* thetabar);
 
 /* cons_to_prim (ERF_TI_utils.H:4-45): prim(n) = cons(n+1)/rho on the box grown
  * by `ng`; pi_stage = Exner(rho*theta) on the box grown by 1. */

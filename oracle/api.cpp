@@ -67,6 +67,17 @@ double orc_face_interp(int type, const double* s, double upw, double upw_frac) {
     return face_interp(type, s[5], s[4], s[3], s[2], s[1], s[0], upw, upw_frac);
 }
 
+// Rayleigh reference profiles (nz+1 values each)
+int orc_set_rayleigh(void* h, const double* ub, const double* vb, const double* wb, const double* tb) {
+    Level& L = *static_cast<Level*>(h);
+    const int n1 = L.L.nz + 1;
+    L.ray_ubar.assign(ub, ub + n1);
+    L.ray_vbar.assign(vb, vb + n1);
+    L.ray_wbar.assign(wb, wb + n1);
+    L.ray_thetabar.assign(tb, tb + n1);
+    return 0;
+}
+
 void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
     auto* L = new Level();
     try { L->init(*cfg, *lay); }
@@ -137,7 +148,7 @@ int orc_turbulent_viscosity(void* h) {
 int orc_slow_rhs_pre(void* h, int nrk, double slow_dt) {
     Level& L = *static_cast<Level*>(h);
     return guard([&] {
-        std::fill(L.cc_src.v.begin(), L.cc_src.v.end(), 0.0);
+        L.make_sources(L.A(L.S_new), L.A(L.prim));
         L.make_mom_sources(L.A(L.S_new));
         L.slow_rhs_pre(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_new), L.A(L.prim), L.A(L.S_scratch),
                        L.A(L.xvel), L.A(L.yvel), L.A(L.zvel));

@@ -48,10 +48,9 @@ void Level::advance_dycore(double old_time, double dt) {
         if (nrk == 1) { nsubsteps = ratio / 2; dtau = sub_timestep; time_stage = old_time + timestep / 2.0; }
         if (nrk == 2) { nsubsteps = ratio; dtau = sub_timestep; time_stage = old_time + timestep; }
         if (nrk > 0) cons_to_prim(Sn.cons, ng_cons);   // pre_update
-        // slow_rhs_fun_pre: make_sources (cc_src = 0 for the supported forcing),
-        // make_mom_sources, erf_slow_rhs_pre
+        // slow_rhs_fun_pre: make_sources, make_mom_sources, erf_slow_rhs_pre
         double slow_dt = time_stage - old_time;
-        std::fill(cc_src.v.begin(), cc_src.v.end(), 0.0);
+        make_sources(Sn, A(prim));
         make_mom_sources(Sn);
         slow_rhs_pre(nrk, slow_dt, Fs, Sn, A(prim), Sc, xv, yv, zv);
         double inv_fac = 1.0 / static_cast<double>(nsubsteps);

@@ -29,7 +29,7 @@ namespace orc {
 
 // ---- constants (Source/ERF_Constants.H:7-20) -------------------------------
 constexpr double PI       = 3.14159265358979323846264338327950288;
-constexpr double PIoTwo   = PI / 2.0;
+constexpr double PIoTwo   = PI / 2.0;   // ERF_Constants.H
 constexpr double R_d      = 287.0;
 constexpr double R_v      = 461.505;
 constexpr double Cp_d     = 1004.5;
@@ -182,6 +182,9 @@ struct Level {
     void turbulent_viscosity(Arr cons_in, Arr u, Arr v, Arr w);
     void make_tau_terms(StateA data, Arr u, Arr v, Arr w);
     void make_mom_sources(StateA data);
+    void make_sources(StateA data, Arr prim_);
+    // Rayleigh damping reference profiles (nz+1 values each), set by orc_set_rayleigh
+    std::vector<double> ray_ubar, ray_vbar, ray_wbar, ray_thetabar;
     void slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_, StateA scratch,
                       Arr u, Arr v, Arr w);
     void slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA snew, StateA data,

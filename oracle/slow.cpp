@@ -38,6 +38,50 @@ void Level::make_mom_sources(StateA data) {
             zs(i, j, k) += cf * rho_u_loc * cosphi;
         });
     }
+    // Rayleigh damping (ERF_make_mom_sources.cpp:247-296)
+    if ((cfg.rayleigh_damp_U || cfg.rayleigh_damp_V || cfg.rayleigh_damp_W) && (int)ray_wbar.size() < nz + 1)
+        throw std::runtime_error("Rayleigh damping profiles not set");
+    {
+        const double zlo = cfg.prob_lo[2], dz = cfg.dz;
+        const double ztop = zlo + nz * dz;          // ERF_init_rayleigh.cpp:17
+        const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
+        if (cfg.rayleigh_damp_U) {
+            loop(tbx, [&](int i, int j, int k) {
+                double zcc = zlo + (k + 0.5) * dz;
+                double zfrac = 1 - (ztop - zcc) / zdamp;
+                if (zfrac > 0) {
+                    double rho_on_u_face = 0.5 * (c(i, j, k) + c(i - 1, j, k));
+                    double uu = ru(i, j, k) / rho_on_u_face;
+                    double sinefac = std::sin(PIoTwo * zfrac);
+                    xs(i, j, k) -= dampcoef * sinefac * sinefac * (uu - ray_ubar[k]) * rho_on_u_face;
+                }
+            });
+        }
+        if (cfg.rayleigh_damp_V) {
+            loop(tby, [&](int i, int j, int k) {
+                double zcc = zlo + (k + 0.5) * dz;
+                double zfrac = 1 - (ztop - zcc) / zdamp;
+                if (zfrac > 0) {
+                    double rho_on_v_face = 0.5 * (c(i, j, k) + c(i, j - 1, k));
+                    double vv = rv(i, j, k) / rho_on_v_face;
+                    double sinefac = std::sin(PIoTwo * zfrac);
+                    ys(i, j, k) -= dampcoef * sinefac * sinefac * (vv - ray_vbar[k]) * rho_on_v_face;
+                }
+            });
+        }
+        if (cfg.rayleigh_damp_W) {
+            loop(tbz, [&](int i, int j, int k) {
+                double zstag = zlo + k * dz;
+                double zfrac = 1 - (ztop - zstag) / zdamp;
+                if (zfrac > 0) {
+                    double rho_on_w_face = 0.5 * (c(i, j, k) + c(i, j, k - 1));
+                    double ww = rw(i, j, k) / rho_on_w_face;
+                    double sinefac = std::sin(PIoTwo * zfrac);
+                    zs(i, j, k) -= dampcoef * sinefac * sinefac * (ww - ray_wbar[k]) * rho_on_w_face;
+                }
+            });
+        }
+    }
     const double* g = cfg.abl_geo_forcing;
     loop(tbx, [&](int i, int j, int k) {
         double rho_on_u_face = 0.5 * (c(i, j, k) + c(i - 1, j, k));
@@ -50,6 +94,28 @@ void Level::make_mom_sources(StateA data) {
     loop(tbz, [&](int i, int j, int k) {
         double rho_on_w_face = 0.5 * (c(i, j, k) + c(i, j, k - 1));
         zs(i, j, k) += rho_on_w_face * g[2];
+    });
+}
+
+// make_sources (Source/SourceTerms/ERF_make_sources.cpp:32-418) for the dry
+// flat step: cc_src is zero except the Rayleigh damping of rho*theta (:203-225).
+void Level::make_sources(StateA data, Arr prim_) {
+    std::fill(cc_src.v.begin(), cc_src.v.end(), 0.0);
+    if (!cfg.rayleigh_damp_T) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    if ((int)ray_thetabar.size() < nz + 1) throw std::runtime_error("Rayleigh damping profiles not set");
+    Arr cs = A(cc_src), c = data.cons;
+    const double zlo = cfg.prob_lo[2], dz = cfg.dz;
+    const double ztop = zlo + nz * dz;
+    const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
+        double zcc = zlo + (k + 0.5) * dz;
+        double zfrac = 1 - (ztop - zcc) / zdamp;
+        if (zfrac > 0) {
+            double theta = prim_(i, j, k, 0);
+            double sinefac = std::sin(PIoTwo * zfrac);
+            cs(i, j, k, 1) -= dampcoef * sinefac * sinefac * (theta - ray_thetabar[k]) * c(i, j, k, 0);
+        }
     });
 }
 

@@ -33,6 +33,7 @@ def lib():
         l.orc_turbulent_viscosity.argtypes = [ctypes.c_void_p]
         l.orc_slow_rhs_pre.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         l.orc_slow_rhs_post.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        l.orc_set_rayleigh.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
         _lib = l
     return _lib
 
@@ -69,6 +70,12 @@ class Oracle:
     def load(self, fields):
         for fid, a in fields.items():
             self.set(fid, a)
+
+    def set_rayleigh(self, ubar, vbar, wbar, thetabar):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ubar, vbar, wbar, thetabar)]
+        assert all(a.size == self.cfg.nz + 1 for a in arrs)
+        self._keep = arrs
+        self._chk(lib().orc_set_rayleigh(self.h, *(a.ctypes.data for a in arrs)))
 
     def step(self, t0, dt, n):
         self._chk(lib().orc_step(self.h, t0, dt, n))

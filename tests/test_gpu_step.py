@@ -23,13 +23,16 @@ def _sim():
     return Sim
 
 
-def _compare(cfg, f, dt, nsteps, tol, exact=False):
+def _compare(cfg, f, dt, nsteps, tol, exact=False, rayleigh=None):
     Sim = _sim()
     sim = Sim(cfg)
     sim.load(f)
-    sim.step(0.0, dt, nsteps)
     orc = Oracle(cfg)
     orc.load(f)
+    if rayleigh is not None:
+        sim.set_rayleigh(*rayleigh)
+        orc.set_rayleigh(*rayleigh)
+    sim.step(0.0, dt, nsteps)
     orc.step(0.0, dt, nsteps)
     worst = {}
     ndiff = {}
@@ -99,3 +102,14 @@ def test_weno_scalar_advection(adv):
     blob = ((ii - ng - 8) ** 2 + (jj - ng - 8) ** 2 + (kk - ng - 6) ** 2 < 16).astype(float)
     f[C.F_CONS][C.RHOSCALAR] = rho * blob
     _compare(cfg, f, 1.0, 2, 1e-11, exact=True)
+
+
+def test_rayleigh_damping_layer():
+    """u, v, w and theta damped in the top 4 km towards reference profiles."""
+    cfg, f = P.rising_thermal(16, 16, 16, seed=19, adv=C.ADV_UPWIND3)
+    cfg.rayleigh_damp_U = cfg.rayleigh_damp_V = cfg.rayleigh_damp_W = cfg.rayleigh_damp_T = 1
+    cfg.rayleigh_zdamp = 4000.0
+    cfg.rayleigh_dampcoef = 0.05
+    n1 = cfg.nz + 1
+    prof = (np.full(n1, 0.3), np.full(n1, -0.2), np.zeros(n1), np.full(n1, 300.4))
+    _compare(cfg, f, 1.0, 2, 1e-11, exact=True, rayleigh=prof)

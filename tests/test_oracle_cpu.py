@@ -292,3 +292,63 @@ def test_weno_exact_on_low_degree_polynomials(type_, degree):
     for upw in (1.0, -1.0):
         v = _face(type_, step, upw)
         assert -1e-12 <= v <= 1.0 + 1e-12
+
+
+# ---- Rayleigh damping: closed form of the source terms ------------------------------------
+
+def test_rayleigh_damping_source_closed_form():
+    """The momentum and rho*theta RHS with damping minus without equals
+    -dampcoef sin^2(pi/2 zfrac) (phi - phibar) rho on the damped levels
+    (zfrac = 1 - (ztop - z)/zdamp, z at cell centres for u, v, theta and at
+    faces for w) and nothing below the layer."""
+    from helpers import ALL_IDS
+    cfg, f = P.rising_thermal(8, 8, 16, seed=4)
+    nz, dz = cfg.nz, cfg.dz
+    ub, vb, wb, tb = (np.full(nz + 1, 1.5), np.full(nz + 1, -0.7), np.full(nz + 1, 0.2), np.full(nz + 1, 299.0))
+    cfg.rayleigh_zdamp = 3000.0
+    cfg.rayleigh_dampcoef = 0.05
+    orc = Oracle(cfg)
+    orc.load(f)
+    orc.step(0.0, 1.0, 1)                          # a realistic mid-run state in every buffer
+    state = {fid: orc.get(fid) for fid in ALL_IDS}
+    res = {}
+    for on in (0, 1):
+        cfg.rayleigh_damp_U = cfg.rayleigh_damp_V = cfg.rayleigh_damp_W = cfg.rayleigh_damp_T = on
+        o2 = Oracle(cfg)
+        o2.set_rayleigh(ub, vb, wb, tb)
+        for fid, a in state.items():
+            o2.set(fid, a)
+        o2.call("orc_slow_rhs_pre", 1, 0.5)
+        res[on] = (o2.get(C.state_id(C.F_SLOW, 1))[0], o2.get(C.state_id(C.F_SLOW, 3))[0],
+                   o2.get(C.state_id(C.F_SLOW, 0))[C.RHOTHETA], state)
+    st = res[1][3]
+    L = C.layout_for(cfg)
+    ng = L.ng
+    rho = st[C.state_id(C.S_NEW, 0)][C.RHO]
+    rhomom_u = st[C.state_id(C.S_NEW, 1)][0]
+    rhomom_w = st[C.state_id(C.S_NEW, 3)][0]
+    theta = st[C.D_PRIM][0]
+    ztop = nz * dz
+    for k in range(nz):
+        kk = k + ng
+        zfrac = 1 - (ztop - (k + 0.5) * dz) / cfg.rayleigh_zdamp
+        fac = cfg.rayleigh_dampcoef * np.sin(np.pi / 2 * zfrac) ** 2 if zfrac > 0 else 0.0
+        sl = (kk, slice(ng, ng + cfg.ny), slice(ng, ng + cfg.nx))
+        rface = 0.5 * (rho[kk, ng:ng + cfg.ny, ng:ng + cfg.nx] + rho[kk, ng:ng + cfg.ny, ng - 1:ng + cfg.nx - 1])
+        du = res[1][0][sl] - res[0][0][sl]
+        exp_u = -fac * (rhomom_u[sl] / rface - ub[k]) * rface
+        assert np.allclose(du, exp_u, rtol=1e-9, atol=1e-12), k
+        dth = res[1][2][sl] - res[0][2][sl]
+        exp_t = -fac * (theta[sl] - tb[k]) * rho[sl]
+        assert np.allclose(dth, exp_t, rtol=1e-9, atol=1e-12), k
+        if zfrac <= 0:
+            assert np.all(du == 0) and np.all(dth == 0)
+    for k in range(1, nz):
+        kk = k + ng
+        zfrac = 1 - (ztop - k * dz) / cfg.rayleigh_zdamp
+        fac = cfg.rayleigh_dampcoef * np.sin(np.pi / 2 * zfrac) ** 2 if zfrac > 0 else 0.0
+        sl = (kk, slice(ng, ng + cfg.ny), slice(ng, ng + cfg.nx))
+        rface = 0.5 * (rho[sl] + rho[kk - 1, ng:ng + cfg.ny, ng:ng + cfg.nx])
+        dw = res[1][1][sl] - res[0][1][sl]
+        exp_w = -fac * (rhomom_w[sl] / rface - wb[k]) * rface
+        assert np.allclose(dw, exp_w, rtol=1e-9, atol=1e-12), k
