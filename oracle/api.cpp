@@ -78,6 +78,12 @@ int orc_set_rayleigh(void* h, const double* ub, const double* vb, const double* 
     return 0;
 }
 
+// initial state of the reference's DensityCurrent regression case
+int orc_init_density_current(void* h, double x_c, double z_c, double x_r, double z_r, double T_pert) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.init_density_current(x_c, z_c, x_r, z_r, T_pert); });
+}
+
 void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
     auto* L = new Level();
     try { L->init(*cfg, *lay); }

@@ -1,1 +1,114 @@
-// init.cpp -- placeholder
+// init.cpp -- TEST INFRASTRUCTURE ONLY (oracle).  Initial states of the
+// reference's regression cases, restated so that the oracle can be checked
+// against the reference's gold plotfiles.
+#include "oracle.hpp"
+
+namespace orc {
+
+// HSEutils::init_isentropic_hse (Source/Utils/ERF_HSE_utils.H:88-181), flat
+// grid with klo = 0: Newton iteration per level for a density that satisfies
+// the EOS at constant theta and the discrete hydrostatic balance.
+static void isentropic_hse(double r_sfc, double theta, std::vector<double>& r, std::vector<double>& p,
+                           double dz, int khi) {
+    const int MAX_ITER = 10;           // ERF_HSE_utils.H:13-14
+    const double TOL = 1.e-8;
+    const double half_dz = 0.5 * dz;
+    r[0] = r_sfc;
+    p[0] = p_0 - half_dz * r[0] * CONST_GRAV;
+    for (int iter = 0; iter < MAX_ITER; iter++) {
+        double p_hse = p_0 - half_dz * r[0] * CONST_GRAV;
+        double p_eos = getPgivenRTh(r[0] * theta);
+        double A = p_hse - p_eos;
+        double dpdr = getdPdRgivenConstantTheta(r[0], theta);
+        double drho = A / (dpdr + half_dz * CONST_GRAV);
+        r[0] = r[0] + drho;
+        p[0] = getPgivenRTh(r[0] * theta);
+        if (std::abs(drho) < TOL) break;
+    }
+    for (int k = 1; k <= khi; k++) {
+        r[k] = r[k - 1];
+        for (int iter = 0; iter < MAX_ITER; iter++) {
+            double r_avg = 0.5 * (r[k - 1] + r[k]);
+            double p_hse = p[k - 1] - dz * r_avg * CONST_GRAV;
+            double p_eos = getPgivenRTh(r[k] * theta);
+            double A = p_hse - p_eos;
+            double dpdr = getdPdRgivenConstantTheta(r[k], theta);
+            double drho = A / (dpdr + dz * CONST_GRAV);
+            r[k] = r[k] + drho;
+            p[k] = getPgivenRTh(r[k] * theta);
+            if (std::abs(drho) < TOL * r[k - 1]) break;
+        }
+    }
+    r[khi + 1] = r[khi];
+}
+
+// Exec/RegTests/DensityCurrent: ERF::init_only (Source/ERF.cpp:1258-1346) with
+// the default init type: initHSE (erf_init_dens_hse of
+// Source/Prob/ERF_init_density_hse_dry.H:11-120, T_sfc = 300 K, and
+// erf_enforce_hse, Source/Initialization/ERF_init1d.cpp:153-313), init_from_hse
+// (ERF_init_from_hse.cpp:30-50) and the problem's perturbation
+// (Exec/RegTests/DensityCurrent/ERF_prob.cpp:34-160): a cold cos^2 bubble
+// entering rho only; velocities zero (U_0 = 0).
+void Level::init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
+    const double dz = cfg.dz, dx = cfg.dx;
+    // base density
+    const double T_sfc = 300.;
+    const double rho_sfc = p_0 / (R_d * T_sfc);
+    const double Thetabar = T_sfc;
+    std::vector<double> r(nz + 2), p(nz + 2);
+    isentropic_hse(rho_sfc, Thetabar, r, p, dz, nz - 1);
+    // rho_hse on cells k = -1..nz: r[max(k, 0)]
+    std::vector<double> rh(nz + 2), ph(nz + 2), pih(nz + 2);
+    for (int k = -1; k <= nz; ++k) rh[k + 1] = r[std::max(k, 0)];
+    // erf_enforce_hse
+    const double g = gravity;
+    const double hz = 0.5 * dz;
+    ph[1] = p_0 - hz * rh[1] * g;
+    pih[1] = getExnergivenP(ph[1], rdOcp);
+    ph[0] = p_0 + hz * rh[1] * g;
+    pih[0] = getExnergivenP(ph[0], rdOcp);
+    for (int k = 1; k <= nz; k++) {
+        double dens_interp = 0.5 * (rh[k + 1] + rh[k]);
+        ph[k + 1] = ph[k] - dz * dens_interp * g;
+        pih[k + 1] = getExnergivenP(ph[k + 1], rdOcp);
+    }
+    Arr r0a = A(r0), p0a = A(p0), pi0a = A(pi0), c = A(cons);
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -ng; j <= ny + ng; ++j)
+            for (int i = -ng; i <= nx + ng; ++i) {
+                r0a(i, j, k) = rh[k + 1];
+                p0a(i, j, k) = ph[k + 1];
+                pi0a(i, j, k) = pih[k + 1];
+            }
+    // init_from_hse on the box grown by one; zero elsewhere
+    std::fill(cons.v.begin(), cons.v.end(), 0.0);
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -1; j <= ny; ++j)
+            for (int i = -1; i <= nx; ++i) {
+                c(i, j, k, 0) = r0a(i, j, k);
+                c(i, j, k, 1) = getRhoThetagivenP(p0a(i, j, k));
+            }
+    // perturbation of rho on the valid cells
+    const double rdOcp_l = rdOcp;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = cfg.prob_lo[0] + (i + 0.5) * dx;
+                const double z = cfg.prob_lo[2] + (k + 0.5) * dz;
+                double Lr = std::sqrt(std::pow((x - x_c) / x_r, 2) + std::pow((z - z_c) / z_r, 2));
+                double pert = 0.0;
+                if (Lr <= 1.0) {
+                    double dT = T_pert * (std::cos(PI * Lr) + 1.0) / 2.0;
+                    double Tbar_hse = p0a(i, j, k) / (R_d * r0a(i, j, k));
+                    double theta_perturbed = (Tbar_hse + dT) * std::pow(p_0 / p0a(i, j, k), rdOcp_l);
+                    pert = getRhoThetagivenP(p0a(i, j, k)) / theta_perturbed - r0a(i, j, k);
+                }
+                c(i, j, k, 0) += pert;
+            }
+    std::fill(xvel.v.begin(), xvel.v.end(), 0.0);
+    std::fill(yvel.v.begin(), yvel.v.end(), 0.0);
+    std::fill(zvel.v.begin(), zvel.v.end(), 0.0);
+}
+
+}  // namespace orc

@@ -47,6 +47,9 @@ inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_
 inline double getExnergivenRTh(double rhotheta, double rdOcp, double qv = 0.0) { // ERF_EOS.H:130-135
     return erfx_pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma * rdOcp);
 }
+inline double getExnergivenP(double P, double rdOcp) {                    // ERF_EOS.H:117-121
+    return erfx_pow(P * ip_0, rdOcp);
+}
 inline double getRhoThetagivenP(double p, double qv = 0.0) {              // ERF_EOS.H:144-149
     return erfx_pow(p * erfx_pow(p_0, Gamma - 1), iGamma) * iR_d / (1.0 + R_v / R_d * qv);
 }
@@ -183,6 +186,7 @@ struct Level {
     void make_tau_terms(StateA data, Arr u, Arr v, Arr w);
     void make_mom_sources(StateA data);
     void make_sources(StateA data, Arr prim_);
+    void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
     // Rayleigh damping reference profiles (nz+1 values each), set by orc_set_rayleigh
     std::vector<double> ray_ubar, ray_vbar, ray_wbar, ray_thetabar;
     void slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_, StateA scratch,

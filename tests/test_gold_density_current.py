@@ -1,0 +1,101 @@
+"""The oracle against the reference's own regression gold file.
+
+Tests/ERFGoldFiles/DensityCurrent is the plotfile ERF writes at step 10 of
+Exec/RegTests/DensityCurrent with Tests/test_files/DensityCurrent/DensityCurrent.i
+(256 x 4 x 64 cells of 100 x 25 x 100 m, Symmetry at xlo, Outflow at xhi,
+periodic y, slip walls, Upwind_3rd advection by default, ConstantAlpha
+diffusion with K = 75 m^2/s, c_p = 1004, dt = 1 s, fast dt = 0.25 s).  ERF's
+CTest compares it with fcompare at rtol = atol = 2e-10
+(Tests/CMakeLists.txt:3-4, Tests/CTestList.cmake:45-47).  The fixture holds the
+j = 0 plane of the gold data (the case is y-invariant), extracted by
+tests/golden/make_density_current_fixture.py.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from erf_amd import config as C
+from helpers import valid
+from oracle_lib import Oracle, lib as olib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "density_current_step10.npz")
+
+
+def density_current_config():
+    cfg = C.default_config(256, 4, 64, 100.0, 25.0, 100.0)
+    cfg.prob_lo[0], cfg.prob_lo[1], cfg.prob_lo[2] = -12800.0, 0.0, 0.0
+    cfg.bc[0] = C.BC_SYMMETRY
+    cfg.bc[3] = C.BC_OUTFLOW
+    cfg.bc[1] = cfg.bc[4] = C.BC_PERIODIC
+    cfg.bc[2] = cfg.bc[5] = C.BC_SLIP_WALL
+    cfg.c_p = 1004.0
+    cfg.molec_diff_type = C.MOLEC_CONSTANT_ALPHA
+    cfg.rho0_trans = 1.0
+    cfg.dynamic_viscosity = 75.0
+    cfg.alpha_T = 75.0
+    cfg.alpha_C = 0.0
+    cfg.mri_ratio = 4                  # fixed_dt / fixed_fast_dt
+    return cfg
+
+
+def run_oracle(nsteps):
+    cfg = density_current_config()
+    orc = Oracle(cfg)
+    f = olib().orc_init_density_current
+    f.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5
+    orc._chk(f(orc.h, 0.0, 3000.0, 4000.0, 2000.0, -15.0))   # ERF_prob.H defaults
+    orc.step(0.0, 1.0, nsteps)
+    return cfg, orc
+
+
+def plot_vars(cfg, orc):
+    """The eight plotted variables on the j = 0 plane, as ERF derives them
+    (IO/ERF_Plotfile.cpp: face velocities averaged to cell centres, theta =
+    rho*theta / rho, pressure from the EOS)."""
+    L = orc.L
+    cons = valid(orc.get(C.F_CONS), cfg, L, C.F_CONS)
+    u = valid(orc.get(C.F_XVEL), cfg, L, C.F_XVEL)[0]
+    v = valid(orc.get(C.F_YVEL), cfg, L, C.F_YVEL)[0]
+    w = valid(orc.get(C.F_ZVEL), cfg, L, C.F_ZVEL)[0]
+    r0 = valid(orc.get(C.F_R0), cfg, L, C.F_R0)[0]
+    p0 = valid(orc.get(C.F_P0), cfg, L, C.F_P0)[0]
+    rho, rt = cons[C.RHO], cons[C.RHOTHETA]
+    out = {
+        "density": rho,
+        "x_velocity": 0.5 * (u[:, :, :-1] + u[:, :, 1:]),
+        "y_velocity": 0.5 * (v[:, :-1, :] + v[:, 1:, :]),
+        "z_velocity": 0.5 * (w[:-1] + w[1:]),
+        "theta": rt / rho,
+        "pres_hse": p0,
+        "dens_hse": r0,
+        "pressure": C.p_0 * np.power(C.R_d * rt * (1.0 / C.p_0), C.Gamma),
+    }
+    return {k: a[:, 0, :] for k, a in out.items()}
+
+
+def test_fixture_is_the_gold_plotfile():
+    z = np.load(FIXTURE)
+    assert list(z["names"]) == ["density", "x_velocity", "y_velocity", "z_velocity", "theta", "pres_hse",
+                                "dens_hse", "pressure"]
+    assert int(z["nstep"]) == 10 and float(z["time"]) == 10.0
+    assert z["plane"].shape == (8, 64, 256)
+
+
+def test_oracle_matches_density_current_gold():
+    z = np.load(FIXTURE)
+    gold = dict(zip(list(z["names"]), z["plane"]))
+    cfg, orc = run_oracle(10)
+    mine = plot_vars(cfg, orc)
+    report = {}
+    for name, g in gold.items():
+        a = mine[name]
+        aerr = float(np.abs(a - g).max())
+        scale = float(np.abs(g).max())
+        rerr = aerr / scale if scale > 0 else aerr
+        report[name] = (aerr, rerr)
+    # fcompare -r 2e-10 --abs_tol 2e-10: a variable fails when both errors exceed
+    bad = {k: v for k, v in report.items() if v[0] > 2e-10 and v[1] > 2e-10}
+    assert not bad, "beyond fcompare tolerance: %s\nall: %s" % (bad, report)

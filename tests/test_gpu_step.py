@@ -113,3 +113,19 @@ def test_rayleigh_damping_layer():
     n1 = cfg.nz + 1
     prof = (np.full(n1, 0.3), np.full(n1, -0.2), np.zeros(n1), np.full(n1, 300.4))
     _compare(cfg, f, 1.0, 2, 1e-11, exact=True, rayleigh=prof)
+
+
+def test_density_current_gold_case():
+    """The reference's DensityCurrent regression case (whose gold file the
+    oracle reproduces, tests/test_gold_density_current.py), 10 steps on the
+    GPU from the oracle's initial state: bit-identical to the oracle."""
+    import ctypes
+    from oracle_lib import lib as olib
+    from test_gold_density_current import density_current_config
+    cfg = density_current_config()
+    orc0 = Oracle(cfg)
+    f = olib().orc_init_density_current
+    f.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5
+    orc0._chk(f(orc0.h, 0.0, 3000.0, 4000.0, 2000.0, -15.0))
+    fields = {fid: orc0.get(fid) for fid in (C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0, C.F_PI0)}
+    _compare(cfg, fields, 1.0, 10, 1e-11, exact=True)
