@@ -84,6 +84,48 @@ int orc_init_density_current(void* h, double x_c, double z_c, double x_r, double
     return guard([&] { L.init_density_current(x_c, z_c, x_r, z_r, T_pert); });
 }
 
+// initial states of the ScalarAdvDiff and TaylorGreenVortex regression cases
+int orc_init_scalar_advdiff(void* h, const double* params) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.init_scalar_advdiff(params); });
+}
+int orc_init_taylor_green(void* h, double rho_0, double V_0, double M_0) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.init_taylor_green(rho_0, V_0, M_0); });
+}
+
+// ERF::Evolve's time loop (Source/ERF.cpp:346-399) with ComputeDt
+// (ERF_ComputeTimestep.cpp:9-44) before every step: dt from estTimeStep,
+// limited to change_max times the previous dt, init_shrink on the first step,
+// cut to land on stop_time; the MRI ratio from estTimeStep unless
+// fixed_mri_ratio > 0 (ERF_advance_dycore.cpp:299).  *t_end gets the final time.
+int orc_run(void* h, double t0, int nsteps, double cfl, double fixed_dt, double fixed_fast_dt,
+            int fixed_mri_ratio, double change_max, double init_shrink, double stop_time, double* t_end) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        double t = t0;
+        double dt_prev = 1.e100;
+        const int ratio0 = L.cfg.mri_ratio;
+        for (int step = 0; step < nsteps && t < stop_time; ++step) {
+            long ratio = 1;
+            double dt_tmp = L.est_time_step(cfl, fixed_dt, fixed_fast_dt, ratio);
+            double dt_0 = dt_tmp;
+            dt_tmp = std::min(dt_tmp, change_max * dt_prev);
+            dt_0 = std::min(dt_0, 1 * dt_tmp);
+            if (step == 0) dt_0 *= init_shrink;
+            const double eps = 1.e-3 * dt_0;
+            if (t + dt_0 > stop_time - eps) dt_0 = stop_time - t;
+            L.cfg.mri_ratio = fixed_mri_ratio > 0 ? fixed_mri_ratio : (int)ratio;
+            L.advance(t, dt_0);
+            t += dt_0;
+            dt_prev = dt_0;
+            if (t >= stop_time - 1.e-6 * dt_0) break;
+        }
+        L.cfg.mri_ratio = ratio0;
+        if (t_end) *t_end = t;
+    });
+}
+
 void* orc_create(const erfx_config* cfg, const erfx_layout* lay) {
     auto* L = new Level();
     try { L->init(*cfg, *lay); }

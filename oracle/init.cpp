@@ -112,3 +112,148 @@ void Level::init_density_current(double x_c, double z_c, double x_r, double z_r,
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// Constant-density base state without gravity (Source/Prob/ERF_init_constant_density_hse.H
+// with erf_enforce_hse at g = 0): r0 = rho_0, p0 = p_0, pi0 = Exner(p_0).
+static void constant_base_state(Level& Lv, double rho_0) {
+    const erfx_layout& L = Lv.L;
+    Arr r0a = Lv.A(Lv.r0), p0a = Lv.A(Lv.p0), pi0a = Lv.A(Lv.pi0);
+    const double g = Lv.gravity;
+    const double hz = 0.5 * Lv.cfg.dz;
+    for (int k = -1; k <= L.nz; ++k)
+        for (int j = -L.ng; j <= L.ny + L.ng; ++j)
+            for (int i = -L.ng; i <= L.nx + L.ng; ++i) {
+                // the hydrostatic recurrence of erf_enforce_hse with rho = rho_0
+                double p;
+                if (k == -1) p = p_0 + hz * rho_0 * g;
+                else {
+                    p = p_0 - hz * rho_0 * g;
+                    for (int kk = 1; kk <= k; ++kk) p = p - Lv.cfg.dz * (0.5 * (rho_0 + rho_0)) * g;
+                }
+                r0a(i, j, k) = rho_0;
+                p0a(i, j, k) = p;
+                pi0a(i, j, k) = getExnergivenP(p, Lv.rdOcp);
+            }
+}
+
+// Exec/RegTests/ScalarAdvDiff with init_type = uniform: init_uniform
+// (ERF_prob_common.H:450-463: rho = rho_0, rho*theta = rho_0 T_0 on the box grown
+// by one), constant-density base state, and the perturbation of ERF_prob.cpp:54-150:
+// the scalar of prob_type 10 (A_0 exp(-10 r^2) + B_0 sin x), 11 (cos^2 disc in
+// x-y), 12 (cos^2 disc in x-z), times rho_0; x-velocity u_0 plus the log profile
+// uRef ln((z+z0)/z0)/ln((zRef+z0)/z0), y-velocity v_0, z-velocity w_0.
+void Level::init_scalar_advdiff(const double* pr) {
+    const double rho_0 = pr[0], T_0 = pr[1], A_0 = pr[2], B_0 = pr[3], u_0 = pr[4], v_0 = pr[5], w_0 = pr[6];
+    const double rad_0 = pr[7], z0 = pr[8], zRef = pr[9], uRef = pr[10];
+    const double xc_frac = pr[11], yc_frac = pr[12], zc_frac = pr[13];
+    const int prob_type = (int)pr[14];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    constant_base_state(*this, rho_0);
+    std::fill(cons.v.begin(), cons.v.end(), 0.0);
+    Arr c = A(cons);
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -1; j <= ny; ++j)
+            for (int i = -1; i <= nx; ++i) {
+                c(i, j, k, 0) = rho_0;
+                c(i, j, k, 1) = rho_0 * T_0;
+            }
+    const double* lo = cfg.prob_lo;
+    const double hi[3] = {lo[0] + nx * cfg.dx, lo[1] + ny * cfg.dy, lo[2] + nz * cfg.dz};
+    const double dxv[3] = {cfg.dx, cfg.dy, cfg.dz};
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = lo[0] + (i + 0.5) * dxv[0];
+                const double y = lo[1] + (j + 0.5) * dxv[1];
+                const double z = lo[2] + (k + 0.5) * dxv[2];
+                const double xc = xc_frac * (lo[0] + hi[0]);
+                const double yc = yc_frac * (lo[1] + hi[1]);
+                const double zc = zc_frac * (lo[2] + hi[2]);
+                const double r0 = rad_0 * (hi[0] - lo[0]);
+                const double r3d = std::sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc));
+                const double r2d_xy = std::sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc));
+                const double r2d_xz = std::sqrt((x - xc) * (x - xc) + (z - zc) * (z - zc));
+                double sc;
+                if (prob_type == 10) sc = A_0 * std::exp(-10. * r3d * r3d) + B_0 * std::sin(x);
+                else if (prob_type == 11) sc = A_0 * 0.25 * (1.0 + std::cos(PI * std::min(r2d_xy, r0) / r0));
+                else if (prob_type == 12) sc = A_0 * 0.25 * (1.0 + std::cos(PI * std::min(r2d_xz, r0) / r0));
+                else throw std::runtime_error("unsupported prob_type");
+                sc *= rho_0;
+                c(i, j, k, ERFX_RHOSCALAR) += sc;
+            }
+    Arr u = A(xvel), v = A(yvel), w = A(zvel);
+    std::fill(xvel.v.begin(), xvel.v.end(), 0.0);
+    std::fill(yvel.v.begin(), yvel.v.end(), 0.0);
+    std::fill(zvel.v.begin(), zvel.v.end(), 0.0);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const double z = lo[2] + (k + 0.5) * dxv[2];
+                if (uRef != 0.0)
+                    u(i, j, k) = u_0 + uRef * std::log((z + z0) / z0) / std::log((zRef + z0) / z0);
+                else
+                    u(i, j, k) = u_0;
+            }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i < nx; ++i) v(i, j, k) = v_0;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) w(i, j, k) = w_0;
+}
+
+// Exec/RegTests/TaylorGreenVortex (ERF_prob.cpp:22-100) with the default init
+// type: constant-density base state, init_from_hse, then rho*theta from the
+// vortex pressure, RhoScalar = rho_0, u = V_0 sin x cos y cos z, v = -V_0 cos x
+// sin y cos z, w = 0.
+void Level::init_taylor_green(double rho_0, double V_0, double M_0) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    constant_base_state(*this, rho_0);
+    std::fill(cons.v.begin(), cons.v.end(), 0.0);
+    Arr c = A(cons), p0a = A(p0), r0a = A(r0);
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -1; j <= ny; ++j)
+            for (int i = -1; i <= nx; ++i) {
+                c(i, j, k, 0) = r0a(i, j, k);
+                c(i, j, k, 1) = getRhoThetagivenP(p0a(i, j, k));
+            }
+    const double* lo = cfg.prob_lo;
+    const double dxv[3] = {cfg.dx, cfg.dy, cfg.dz};
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = lo[0] + (i + 0.5) * dxv[0];
+                const double y = lo[1] + (j + 0.5) * dxv[1];
+                const double z = lo[2] + (k + 0.5) * dxv[2];
+                const double p = rho_0 * V_0 * V_0 *
+                                 (1.0 / (Gamma * M_0 * M_0) +
+                                  (1.0 / 16.0) * (std::cos(2 * x) + std::cos(2 * y)) * (std::cos(2 * z) + 2));
+                c(i, j, k, 1) += getRhoThetagivenP(p) - getRhoThetagivenP(p0a(i, j, k));
+                c(i, j, k, ERFX_RHOSCALAR) += 1.0 * rho_0;
+            }
+    Arr u = A(xvel), v = A(yvel), w = A(zvel);
+    std::fill(xvel.v.begin(), xvel.v.end(), 0.0);
+    std::fill(yvel.v.begin(), yvel.v.end(), 0.0);
+    std::fill(zvel.v.begin(), zvel.v.end(), 0.0);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const double x = lo[0] + (i + 0.0) * dxv[0];
+                const double y = lo[1] + (j + 0.5) * dxv[1];
+                const double z = lo[2] + (k + 0.5) * dxv[2];
+                u(i, j, k) = V_0 * std::sin(x) * std::cos(y) * std::cos(z);
+            }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = lo[0] + (i + 0.5) * dxv[0];
+                const double y = lo[1] + (j + 0.0) * dxv[1];
+                const double z = lo[2] + (k + 0.5) * dxv[2];
+                v(i, j, k) = -V_0 * std::cos(x) * std::sin(y) * std::cos(z);
+            }
+    (void)w;
+}
+
+}  // namespace orc

@@ -87,3 +87,58 @@ void Level::advance(double time, double dt) {
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// ERF::estTimeStep (Source/TimeIntegration/ERF_ComputeTimestep.cpp:49-233),
+// one level, flat grid, compressible with substepping: the acoustic CFL
+// estimate cfl / max((|u|+c)/dx, (|v|+c)/dy) (implicit substepping leaves w
+// out), the advective estimate cfl / max(|u|/dx, |v|/dy, |w|/dz), and the MRI
+// ratio from their quotient (or from fixed_dt / fixed_fast_dt), made even.
+double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, long& ratio) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    Arr c = A(cons), u = A(xvel), v = A(yvel), w = A(zvel);
+    const bool implicit = cfg.implicit_substepping != 0;
+    double comp_inv = -1.e100, lowM_inv = -1.e100;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                // average_face_to_cellcenter
+                double uc = 0.5 * (u(i, j, k) + u(i + 1, j, k));
+                double vc = 0.5 * (v(i, j, k) + v(i, j + 1, k));
+                double wc = 0.5 * (w(i, j, k) + w(i, j, k + 1));
+                const double rho = c(i, j, k, 0);
+                const double rhotheta = c(i, j, k, 1);
+                double pressure = getPgivenRTh(rhotheta);
+                double cs = std::sqrt(Gamma * pressure / rho);
+                double a = (std::abs(uc) + cs) * dxInv[0];
+                double b = (std::abs(vc) + cs) * dxInv[1];
+                double e = (std::abs(wc) + cs) * dxInv[2];
+                if (implicit) {
+                    if (nx > 1 && ny > 1) comp_inv = std::max({a, b, comp_inv});
+                    else if (nx > 1) comp_inv = std::max(a, comp_inv);
+                    else comp_inv = std::max(b, comp_inv);
+                } else {
+                    if (nx > 1 && ny > 1) comp_inv = std::max({a, b, e, comp_inv});
+                    else if (nx > 1) comp_inv = std::max({a, e, comp_inv});
+                    else if (ny > 1) comp_inv = std::max({b, e, comp_inv});
+                    else comp_inv = std::max(e, comp_inv);
+                }
+                lowM_inv = std::max({std::abs(uc) * dxInv[0], std::abs(vc) * dxInv[1], std::abs(wc) * dxInv[2],
+                                     lowM_inv});
+            }
+    double estdt_comp = cfl / comp_inv;
+    double estdt_lowM = 1.e20;
+    if (lowM_inv > 0.0) estdt_lowM = cfl / lowM_inv;
+    if (fixed_dt > 0. && fixed_fast_dt > 0.) ratio = static_cast<long>(fixed_dt / fixed_fast_dt);
+    else if (fixed_dt > 0.) ratio = static_cast<long>(std::ceil((fixed_dt / estdt_comp)));
+    else ratio = (lowM_inv > 0.0) ? static_cast<long>(std::ceil((estdt_lowM / estdt_comp))) : 1;
+    if (cfg.force_stage1_single_substep) {
+        if (ratio % 2 != 0) ratio += 1;
+    } else {
+        if (ratio % 6 != 0) ratio = static_cast<int>(std::ceil(ratio / 6.0) * 6);
+    }
+    return fixed_dt > 0.0 ? fixed_dt : estdt_comp;
+}
+
+}  // namespace orc

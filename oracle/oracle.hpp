@@ -187,6 +187,10 @@ struct Level {
     void make_mom_sources(StateA data);
     void make_sources(StateA data, Arr prim_);
     void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
+    void init_scalar_advdiff(const double* params);
+    void init_taylor_green(double rho_0, double V_0, double M_0);
+    // ERF::estTimeStep (ERF_ComputeTimestep.cpp:49-233): returns dt, sets the MRI ratio
+    double est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, long& ratio);
     // Rayleigh damping reference profiles (nz+1 values each), set by orc_set_rayleigh
     std::vector<double> ray_ubar, ray_vbar, ray_wbar, ray_thetabar;
     void slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_, StateA scratch,

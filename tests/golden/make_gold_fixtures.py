@@ -1,10 +1,11 @@
-"""Extract the DensityCurrent gold plotfile of the reference's regression tests
-(Tests/ERFGoldFiles/DensityCurrent, AMReX plotfile written at step 10) into a
-small fixture: the y-invariant j = 0 plane of all eight plotted variables.
+"""Extract gold plotfiles of the reference's regression tests
+(Tests/ERFGoldFiles/<case>, AMReX plotfiles) into small fixtures:
+  density_current_step10.npz  the y-invariant j = 0 plane of DensityCurrent;
+  <case>.npz                  the full arrays of the small cases listed in SMALL.
 
 Run from the repository root in a container that has the reference mounted:
-    python tests/golden/make_density_current_fixture.py /root/reference
-The plotfile is read as data (text headers, raw FP64 FABs); nothing of the
+    python tests/golden/make_gold_fixtures.py /root/reference
+The plotfiles are read as data (text headers, raw FP64 FABs); nothing of the
 reference is imported or executed.
 """
 import os
@@ -49,8 +50,23 @@ def read_plotfile(path):
     return dict(names=names, time=time, lo=lo, hi=hi, n=n, nstep=nstep, data=data, cell_h=ch)
 
 
+SMALL = ["ScalarDiffusionSine", "ScalarDiffusionGaussian", "ScalarAdvectionUniformU", "ScalarAdvectionShearedU",
+         "TaylorGreenAdvecting", "TaylorGreenAdvectingDiffusing"]
+
+
+def save_small(ref, case):
+    pf = read_plotfile(os.path.join(ref, "Tests", "ERFGoldFiles", case))
+    assert np.all(np.isfinite(pf["data"]))
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), case + ".npz"),
+                        names=np.array(pf["names"]), time=pf["time"], nstep=pf["nstep"],
+                        lo=np.array(pf["lo"]), hi=np.array(pf["hi"]), n=np.array(pf["n"]), data=pf["data"])
+    print("wrote fixture", case, pf["names"], pf["data"].shape, "time", pf["time"], "steps", pf["nstep"])
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    for case in SMALL:
+        save_small(ref, case)
     pf = read_plotfile(os.path.join(ref, "Tests", "ERFGoldFiles", "DensityCurrent"))
     d = pf["data"]
     assert np.all(np.isfinite(d))

@@ -8,7 +8,7 @@ diffusion with K = 75 m^2/s, c_p = 1004, dt = 1 s, fast dt = 0.25 s).  ERF's
 CTest compares it with fcompare at rtol = atol = 2e-10
 (Tests/CMakeLists.txt:3-4, Tests/CTestList.cmake:45-47).  The fixture holds the
 j = 0 plane of the gold data (the case is y-invariant), extracted by
-tests/golden/make_density_current_fixture.py.
+tests/golden/make_gold_fixtures.py.
 """
 import ctypes
 import math
