@@ -71,6 +71,18 @@ class Config(ctypes.Structure):
     ]
 
 
+class DtControl(ctypes.Structure):
+    """erfx_dt_control: ERF's ComputeDt / estTimeStep settings (ERF.cpp:27-30 defaults)."""
+    _fields_ = [("cfl", ctypes.c_double), ("fixed_dt", ctypes.c_double), ("fixed_fast_dt", ctypes.c_double),
+                ("fixed_mri_ratio", ctypes.c_int), ("change_max", ctypes.c_double),
+                ("init_shrink", ctypes.c_double), ("stop_time", ctypes.c_double)]
+
+
+def dt_control(cfl=0.8, fixed_dt=-1.0, fixed_fast_dt=-1.0, fixed_mri_ratio=0, change_max=1.1, init_shrink=1.0,
+               stop_time=1.e100):
+    return DtControl(cfl, fixed_dt, fixed_fast_dt, fixed_mri_ratio, change_max, init_shrink, stop_time)
+
+
 class Layout(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
                 ("sy", ctypes.c_long), ("sz", ctypes.c_long), ("nelem", ctypes.c_long)]

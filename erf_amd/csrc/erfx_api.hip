@@ -3,6 +3,8 @@
 #include "erfx_common.h"
 #include "erfx_internal.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -116,6 +118,8 @@ struct erfx_sim {
     double *prim = nullptr, *pi_stage = nullptr, *fast_coeffs = nullptr, *eddy = nullptr;
     std::vector<void*> allocs;
     KTimer timer;
+    double* red = nullptr;    // partial maxima of the time-step estimate (EST_BLOCKS x 2)
+    double* red_host = nullptr;
 };
 
 static void timer_free(KTimer& t) {
@@ -389,6 +393,10 @@ int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
     if (!rc) rc = A(&s->prim, (nc - 1) * ne);
     if (!rc) rc = A(&s->pi_stage, ne);
     if (!rc) rc = A(&s->fast_coeffs, 5 * ne);
+    if (!rc) rc = A(&s->red, 2 * EST_BLOCKS + 2);
+    if (!rc && hipHostMalloc((void**)&s->red_host, 2 * sizeof(double)) != hipSuccess) {
+        set_error("hipHostMalloc failed"); rc = ERFX_ERR_HIP;
+    }
     if (!rc && P.use_turb) rc = A(&s->eddy, 13 * ne);
     if (rc) { erfx_sim_destroy(s); return rc; }
     // S_old and S_new cons alias vars_old / vars_new (ERF_Advance.cpp:127-160)
@@ -403,6 +411,7 @@ int erfx_sim_destroy(erfx_sim* s) {
     if (!s) return ERFX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     timer_free(s->timer);
+    if (s->red_host) (void)hipHostFree(s->red_host);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->ctx) erfx_ctx_destroy(s->ctx);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -481,7 +490,7 @@ int erfx_sim_download(erfx_sim* s, int id, double* host) {
 
 // One step: ERF::Advance (ERF_Advance.cpp:20-233), advance_dycore
 // (ERF_advance_dycore.cpp:37-305) and MRISplitIntegrator::advance (ERF_MRI.H:187-334).
-static int sim_one_step(erfx_sim* s, double time, double dt) {
+static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     erfx_ctx* c = s->ctx;
     const Params& P = c->P;
     hipStream_t st = s->stream;
@@ -514,7 +523,6 @@ static int sim_one_step(erfx_sim* s, double time, double dt) {
     ERFX_HIP(hipMemcpyAsync(Sn.xmom, So.xmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
     ERFX_HIP(hipMemcpyAsync(Sn.ymom, So.ymom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
     ERFX_HIP(hipMemcpyAsync(Sn.zmom, So.zmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    const int ratio = P.cfg.mri_ratio;
     const double sub_timestep = dt / ratio;
     const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
@@ -566,9 +574,70 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
     }
     if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
     for (int n = 0; n < nsteps; ++n) {
-        int rc = sim_one_step(s, t0 + n * dt, dt);
+        int rc = sim_one_step(s, t0 + n * dt, dt, s->ctx->P.cfg.mri_ratio);
         if (rc) return rc;
     }
+    return ERFX_OK;
+}
+
+// ERF::estTimeStep (ERF_ComputeTimestep.cpp:49-233): the two maxima come from
+// est_dt_maxima on the GPU; the rest is host arithmetic in ERF's order.
+int erfx_sim_est_dt(erfx_sim* s, const erfx_dt_control* d, double* dt, int* mri_ratio) {
+    if (!s || !d || !dt || !mri_ratio) { set_error("null argument"); return ERFX_ERR_ARG; }
+    const Params& P = s->ctx->P;
+    est_dt_maxima(P, s->cons[s->cur], s->vel[s->cur][0], s->vel[s->cur][1], s->vel[s->cur][2], s->red,
+                  s->red_host, s->stream);
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    const double comp_inv = s->red_host[0], lowM_inv = s->red_host[1];
+    const double estdt_comp = d->cfl / comp_inv;
+    double estdt_lowM = 1.e20;
+    if (lowM_inv > 0.0) estdt_lowM = d->cfl / lowM_inv;
+    long ratio;
+    if (d->fixed_dt > 0. && d->fixed_fast_dt > 0.) ratio = static_cast<long>(d->fixed_dt / d->fixed_fast_dt);
+    else if (d->fixed_dt > 0.) ratio = static_cast<long>(std::ceil((d->fixed_dt / estdt_comp)));
+    else ratio = (lowM_inv > 0.0) ? static_cast<long>(std::ceil((estdt_lowM / estdt_comp))) : 1;
+    if (P.cfg.force_stage1_single_substep) {
+        if (ratio % 2 != 0) ratio += 1;
+    } else {
+        if (ratio % 6 != 0) ratio = static_cast<int>(std::ceil(ratio / 6.0) * 6);
+    }
+    *mri_ratio = d->fixed_mri_ratio > 0 ? d->fixed_mri_ratio : (int)ratio;
+    *dt = d->fixed_dt > 0.0 ? d->fixed_dt : estdt_comp;
+    return launch_status();
+}
+
+// ERF::Evolve (ERF.cpp:346-399) with ComputeDt (ERF_ComputeTimestep.cpp:9-44).
+int erfx_sim_run(erfx_sim* s, double t0, int max_steps, const erfx_dt_control* d, double* t_end, int* steps_done) {
+    if (!s || !d) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!(d->cfl > 0) && !(d->fixed_dt > 0)) { set_error("need cfl > 0 or fixed_dt > 0"); return ERFX_ERR_ARG; }
+    if (rayleigh_on(s->ctx->P.cfg) && !s->ctx->ray) {
+        set_error("Rayleigh damping profiles not set (erfx_set_rayleigh on the sim's context)"); return ERFX_ERR_STATE;
+    }
+    double t = t0;
+    double dt_prev = 1.e100;
+    int done = 0;
+    for (int step = 0; step < max_steps && t < d->stop_time; ++step) {
+        double dt_tmp;
+        int ratio;
+        int rc = erfx_sim_est_dt(s, d, &dt_tmp, &ratio);
+        if (rc) return rc;
+        if (ratio < 2 || ratio % 2 != 0) { set_error("MRI ratio must be even and >= 2"); return ERFX_ERR_ARG; }
+        double dt_0 = dt_tmp;
+        dt_tmp = std::min(dt_tmp, d->change_max * dt_prev);
+        dt_0 = std::min(dt_0, 1 * dt_tmp);
+        if (step == 0) dt_0 *= d->init_shrink;
+        const double eps = 1.e-3 * dt_0;
+        if (t + dt_0 > d->stop_time - eps) dt_0 = d->stop_time - t;
+        rc = sim_one_step(s, t, dt_0, ratio);
+        if (rc) return rc;
+        t += dt_0;
+        dt_prev = dt_0;
+        ++done;
+        if (t >= d->stop_time - 1.e-6 * dt_0) break;
+    }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    if (t_end) *t_end = t;
+    if (steps_done) *steps_done = done;
     return ERFX_OK;
 }
 

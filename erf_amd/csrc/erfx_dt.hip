@@ -1,0 +1,83 @@
+// erfx_dt.hip -- the two maxima of ERF::estTimeStep
+// (Source/TimeIntegration/ERF_ComputeTimestep.cpp:66-163) as a two-pass
+// max-reduction over the valid cells.  A max is exact and independent of the
+// order of reduction, so the result equals the serial loop's bit for bit.
+#include "erfx_common.h"
+#include "erfx_internal.h"
+
+namespace erfx {
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// block max of two values into out[0..1]; blockDim.x == 256
+__device__ __forceinline__ void block_max2(double a, double b, double* out) {
+    __shared__ double sa[4], sb[4];
+    a = wave_max(a);
+    b = wave_max(b);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = fmax(fmax(sa[0], sa[1]), fmax(sa[2], sa[3]));
+        out[1] = fmax(fmax(sb[0], sb[1]), fmax(sb[2], sb[3]));
+    }
+}
+
+__global__ void __launch_bounds__(256) est_dt_pass1(Lay L, const double* cons, const double* u, const double* v,
+                                                    const double* w, double dxi, double dyi, double dzi,
+                                                    bool implicit, double* partial) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long n = (long)nx * ny * nz;
+    double comp = -1.e100, lowM = -1.e100;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const int j = (int)((t / nx) % ny);
+        const int k = (int)(t / ((long)nx * ny));
+        const long q = L.ix(i, j, k);
+        const double uc = 0.5 * (u[q] + u[q + 1]);
+        const double vc = 0.5 * (v[q] + v[q + L.sy]);
+        const double wc = 0.5 * (w[q] + w[q + L.sz]);
+        const double rho = cons[q];
+        const double rhotheta = cons[q + L.ne];
+        const double pressure = eos_p_of_rt(rhotheta);
+        const double c = sqrt(Gamma * pressure / rho);
+        const double a = (fabs(uc) + c) * dxi;
+        const double b = (fabs(vc) + c) * dyi;
+        const double e = (fabs(wc) + c) * dzi;
+        if (implicit) {
+            if (nx > 1 && ny > 1) comp = fmax(fmax(a, b), comp);
+            else if (nx > 1) comp = fmax(a, comp);
+            else comp = fmax(b, comp);
+        } else {
+            if (nx > 1 && ny > 1) comp = fmax(fmax(fmax(a, b), e), comp);
+            else if (nx > 1) comp = fmax(fmax(a, e), comp);
+            else if (ny > 1) comp = fmax(fmax(b, e), comp);
+            else comp = fmax(e, comp);
+        }
+        lowM = fmax(fmax(fmax(fabs(uc) * dxi, fabs(vc) * dyi), fabs(wc) * dzi), lowM);
+    }
+    block_max2(comp, lowM, partial + 2 * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) est_dt_pass2(const double* partial, int nblocks, double* out) {
+    double a = -1.e100, b = -1.e100;
+    for (int t = threadIdx.x; t < nblocks; t += blockDim.x) {
+        a = fmax(a, partial[2 * t]);
+        b = fmax(b, partial[2 * t + 1]);
+    }
+    block_max2(a, b, out);
+}
+
+void est_dt_maxima(const Params& P, const double* cons, const double* u, const double* v, const double* w,
+                   double* partial, double* host, hipStream_t s) {
+    hipLaunchKernelGGL(est_dt_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, u, v, w, P.dxInv[0], P.dxInv[1],
+                       P.dxInv[2], P.cfg.implicit_substepping != 0, partial);
+    hipLaunchKernelGGL(est_dt_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
+                       partial + 2 * EST_BLOCKS);
+    (void)hipMemcpyAsync(host, partial + 2 * EST_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+}
+
+}  // namespace erfx

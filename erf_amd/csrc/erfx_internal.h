@@ -47,6 +47,12 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 double beta_s, double facinv, hipStream_t s, const FastChain& ch = FastChain());
 bool fast_bcs_fusable(const Params& P);
 
+// time-step estimate: max over valid cells of the acoustic and the advective
+// inverse time scales into host[0], host[1] (stream-ordered; sync before use)
+constexpr int EST_BLOCKS = 1024;
+void est_dt_maxima(const Params& P, const double* cons, const double* u, const double* v, const double* w,
+                   double* partial, double* host, hipStream_t s);
+
 // erfx_slow.hip
 struct SlowWork {
     double* pp;                   // perturbational pressure, cells grown by 1

@@ -57,6 +57,8 @@ SIGNATURES = {
     "erfx_sim_step_async": (_I, [_P, _D, _D, _I]),
     "erfx_sim_sync": (_I, [_P]),
     "erfx_sim_stream": (_P, [_P]),
+    "erfx_sim_est_dt": (_I, [_P, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
+    "erfx_sim_run": (_I, [_P, _D, _I, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_timer_reset": (_I, [_P]),
     "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(_D)]),
@@ -127,6 +129,19 @@ class Sim:
 
     def step(self, t0, dt, n):
         check(lib().erfx_sim_step(self.h, t0, dt, n))
+
+    def run(self, t0, max_steps, dtc):
+        """ERF's time loop with its dt control; returns (t_end, steps_done)."""
+        t = _D()
+        n = _I()
+        check(lib().erfx_sim_run(self.h, t0, max_steps, ctypes.byref(dtc), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def est_dt(self, dtc):
+        dt = _D()
+        r = _I()
+        check(lib().erfx_sim_est_dt(self.h, ctypes.byref(dtc), ctypes.byref(dt), ctypes.byref(r)))
+        return dt.value, r.value
 
     def step_async(self, t0, dt, n):
         check(lib().erfx_sim_step_async(self.h, t0, dt, n))

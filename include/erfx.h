@@ -272,6 +272,30 @@ int  erfx_sim_step_async(erfx_sim* sim, double t0, double dt, int nsteps);
 int  erfx_sim_sync(erfx_sim* sim);
 /* The sim's stream (a hipStream_t) for event timing. */
 void* erfx_sim_stream(erfx_sim* sim);
+/* Time-step control of ERF's time loop (ERF::ComputeDt / ERF::estTimeStep,
+ * Source/TimeIntegration/ERF_ComputeTimestep.cpp:9-233; defaults in ERF.cpp:27-30):
+ * fixed_dt <= 0 selects the CFL time step; the MRI ratio comes from
+ * fixed_mri_ratio when > 0, from fixed_dt / fixed_fast_dt when both are > 0,
+ * and from the CFL estimates otherwise (made even). */
+typedef struct erfx_dt_control {
+    double cfl;                   /* 0.8 in ERF                                      */
+    double fixed_dt;              /* > 0: fixed slow time step                       */
+    double fixed_fast_dt;         /* > 0 with fixed_dt: fixed acoustic time step     */
+    int    fixed_mri_ratio;       /* > 0: fixed_mri_dt_ratio                         */
+    double change_max;            /* 1.1 in ERF                                      */
+    double init_shrink;           /* 1.0 in ERF                                      */
+    double stop_time;             /* run ends there                                  */
+} erfx_dt_control;
+
+/* estTimeStep on the sim's current state: the slow time step and the MRI ratio
+ * (an FP64 max-reduction on the GPU, one host read). */
+int  erfx_sim_est_dt(erfx_sim* sim, const erfx_dt_control* dtc, double* dt, int* mri_ratio);
+/* ERF::Evolve's loop: up to max_steps steps from t0, each with ComputeDt first
+ * (limited to change_max times the previous step, init_shrink on the first,
+ * cut to land on stop_time).  Synchronous; *t_end and *steps_done report. */
+int  erfx_sim_run(erfx_sim* sim, double t0, int max_steps, const erfx_dt_control* dtc, double* t_end,
+                  int* steps_done);
+
 /* Per-launch timing of the step's dominant kernel with HIP events on the sim's
  * stream (the role of ERF's BL_PROFILE regions): reset, run steps, then read
  * the kernel name, summed milliseconds, launch count and algorithmic HBM bytes
