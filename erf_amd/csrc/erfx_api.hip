@@ -393,8 +393,8 @@ int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
     if (!rc) rc = A(&s->prim, (nc - 1) * ne);
     if (!rc) rc = A(&s->pi_stage, ne);
     if (!rc) rc = A(&s->fast_coeffs, 5 * ne);
-    if (!rc) rc = A(&s->red, 2 * EST_BLOCKS + 2);
-    if (!rc && hipHostMalloc((void**)&s->red_host, 2 * sizeof(double)) != hipSuccess) {
+    if (!rc) rc = A(&s->red, 6 * EST_BLOCKS + 6);
+    if (!rc && hipHostMalloc((void**)&s->red_host, 6 * sizeof(double)) != hipSuccess) {
         set_error("hipHostMalloc failed"); rc = ERFX_ERR_HIP;
     }
     if (!rc && P.use_turb) rc = A(&s->eddy, 13 * ne);
@@ -639,6 +639,18 @@ int erfx_sim_run(erfx_sim* s, double t0, int max_steps, const erfx_dt_control* d
     if (t_end) *t_end = t;
     if (steps_done) *steps_done = done;
     return ERFX_OK;
+}
+
+// ERF::sum_integrated_quantities' volume-weighted sums (volWgtSumMF,
+// ERF_WriteScalarProfiles.cpp:11-60, 378-410): total mass, rho*theta and scalar.
+int erfx_sim_integrals(erfx_sim* s, double* mass, double* rhotheta, double* scalar) {
+    if (!s || !mass || !rhotheta || !scalar) { set_error("null argument"); return ERFX_ERR_ARG; }
+    volume_sums(s->ctx->P, s->cons[s->cur], s->red, s->red_host, s->stream);
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    *mass = s->red_host[0] + s->red_host[1];
+    *rhotheta = s->red_host[2] + s->red_host[3];
+    *scalar = s->red_host[4] + s->red_host[5];
+    return launch_status();
 }
 
 int erfx_sim_sync(erfx_sim* s) {

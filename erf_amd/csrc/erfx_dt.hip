@@ -1,7 +1,10 @@
-// erfx_dt.hip -- the two maxima of ERF::estTimeStep
-// (Source/TimeIntegration/ERF_ComputeTimestep.cpp:66-163) as a two-pass
-// max-reduction over the valid cells.  A max is exact and independent of the
-// order of reduction, so the result equals the serial loop's bit for bit.
+// erfx_dt.hip -- global reductions of the step:
+//  * the two maxima of ERF::estTimeStep (Source/TimeIntegration/ERF_ComputeTimestep.cpp:66-163),
+//    a two-pass max-reduction (a max is exact and independent of the order of
+//    reduction, so the result equals the serial loop's bit for bit);
+//  * the volume-weighted sums of ERF::volWgtSumMF (Source/IO/ERF_WriteScalarProfiles.cpp:378-410),
+//    kept in double-double all the way, so the result is the exactly rounded
+//    sum of the per-cell products whatever the reduction order.
 #include "erfx_common.h"
 #include "erfx_internal.h"
 
@@ -78,6 +81,72 @@ void est_dt_maxima(const Params& P, const double* cons, const double* u, const d
     hipLaunchKernelGGL(est_dt_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
                        partial + 2 * EST_BLOCKS);
     (void)hipMemcpyAsync(host, partial + 2 * EST_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+}
+
+// ---- double-double sums ---------------------------------------------------------
+struct DD { double hi, lo; };
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    double s = a.hi + b.hi;
+    double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    double h = s + e;
+    return {h, e - (h - s)};
+}
+__device__ __forceinline__ DD dd_shfl(DD v, int off) { return {__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)}; }
+
+// block sum of NS double-double accumulators into out[2*c], out[2*c+1]
+template <int NS>
+__device__ __forceinline__ void block_sum(DD* acc, double* out) {
+    __shared__ double sh[4][NS][2];
+    for (int c = 0; c < NS; ++c)
+        for (int off = 32; off > 0; off >>= 1) acc[c] = dd_add(acc[c], dd_shfl(acc[c], off));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0)
+        for (int c = 0; c < NS; ++c) { sh[w][c][0] = acc[c].hi; sh[w][c][1] = acc[c].lo; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NS; ++c) {
+            DD t = {sh[0][c][0], sh[0][c][1]};
+            for (int ww = 1; ww < 4; ++ww) t = dd_add(t, DD{sh[ww][c][0], sh[ww][c][1]});
+            out[2 * c] = t.hi;
+            out[2 * c + 1] = t.lo;
+        }
+}
+
+// sums of (cons comp / mf^2) * cell volume over the valid cells for rho,
+// rho*theta and RhoScalar (the quantities of ERF::sum_integrated_quantities)
+__global__ void __launch_bounds__(256) sums_pass1(Lay L, const double* cons, double cell_vol, double* partial) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long n = (long)nx * ny * nz;
+    const int comps[3] = {ERFX_RHO, ERFX_RHOTHETA, ERFX_RHOSCALAR};
+    DD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const int j = (int)((t / nx) % ny);
+        const int k = (int)(t / ((long)nx * ny));
+        const long q = L.ix(i, j, k);
+        for (int c = 0; c < 3; ++c) {
+            double phi = cons[q + comps[c] * L.ne] / (1.0 * 1.0);
+            acc[c] = dd_add(acc[c], DD{phi * cell_vol, 0.0});
+        }
+    }
+    block_sum<3>(acc, partial + 6 * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) sums_pass2(const double* partial, int nblocks, double* out) {
+    DD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int t = threadIdx.x; t < nblocks; t += blockDim.x)
+        for (int c = 0; c < 3; ++c) acc[c] = dd_add(acc[c], DD{partial[6 * t + 2 * c], partial[6 * t + 2 * c + 1]});
+    block_sum<3>(acc, out);
+}
+
+void volume_sums(const Params& P, const double* cons, double* partial, double* host, hipStream_t s) {
+    const double cell_vol = P.cfg.dx * P.cfg.dy * P.cfg.dz;
+    hipLaunchKernelGGL(sums_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, cell_vol, partial);
+    hipLaunchKernelGGL(sums_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
+                       partial + 6 * EST_BLOCKS);
+    (void)hipMemcpyAsync(host, partial + 6 * EST_BLOCKS, 6 * sizeof(double), hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace erfx

@@ -52,6 +52,9 @@ bool fast_bcs_fusable(const Params& P);
 constexpr int EST_BLOCKS = 1024;
 void est_dt_maxima(const Params& P, const double* cons, const double* u, const double* v, const double* w,
                    double* partial, double* host, hipStream_t s);
+// double-double volume sums of rho, rho*theta, RhoScalar: host[0..5] = (hi, lo) x 3
+// (partial needs 6 * EST_BLOCKS + 6 doubles)
+void volume_sums(const Params& P, const double* cons, double* partial, double* host, hipStream_t s);
 
 // erfx_slow.hip
 struct SlowWork {

@@ -59,6 +59,7 @@ SIGNATURES = {
     "erfx_sim_stream": (_P, [_P]),
     "erfx_sim_est_dt": (_I, [_P, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_run": (_I, [_P, _D, _I, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
+    "erfx_sim_integrals": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "erfx_sim_timer_reset": (_I, [_P]),
     "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(_D)]),
@@ -136,6 +137,12 @@ class Sim:
         n = _I()
         check(lib().erfx_sim_run(self.h, t0, max_steps, ctypes.byref(dtc), ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
+
+    def integrals(self):
+        """(mass, rho*theta, scalar) volume sums, exactly rounded."""
+        m, r, q = _D(), _D(), _D()
+        check(lib().erfx_sim_integrals(self.h, ctypes.byref(m), ctypes.byref(r), ctypes.byref(q)))
+        return m.value, r.value, q.value
 
     def est_dt(self, dtc):
         dt = _D()

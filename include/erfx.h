@@ -296,6 +296,12 @@ int  erfx_sim_est_dt(erfx_sim* sim, const erfx_dt_control* dtc, double* dt, int*
 int  erfx_sim_run(erfx_sim* sim, double t0, int max_steps, const erfx_dt_control* dtc, double* t_end,
                   int* steps_done);
 
+/* The volume-weighted sums of ERF::sum_integrated_quantities (volWgtSumMF,
+ * ERF_WriteScalarProfiles.cpp:11-60, 378-410) over the valid cells: total mass,
+ * rho*theta and RhoScalar.  Summed in double-double, so the result is the
+ * exactly rounded sum of the per-cell terms, independent of reduction order. */
+int  erfx_sim_integrals(erfx_sim* sim, double* mass, double* rhotheta, double* scalar);
+
 /* Per-launch timing of the step's dominant kernel with HIP events on the sim's
  * stream (the role of ERF's BL_PROFILE regions): reset, run steps, then read
  * the kernel name, summed milliseconds, launch count and algorithmic HBM bytes

@@ -129,3 +129,28 @@ def test_density_current_gold_case():
     orc0._chk(f(orc0.h, 0.0, 3000.0, 4000.0, 2000.0, -15.0))
     fields = {fid: orc0.get(fid) for fid in (C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0, C.F_PI0)}
     _compare(cfg, fields, 1.0, 10, 1e-11, exact=True)
+
+
+def test_volume_integrals_exactly_rounded():
+    """Mass, rho*theta and scalar sums (volWgtSumMF) equal the exactly rounded
+    sum of the per-cell terms (math.fsum), before and after steps; mass is
+    conserved by the step to roundoff of the sum."""
+    import math
+    Sim = _sim()
+    cfg, f = P.rising_thermal(24, 20, 16, seed=3)
+    f[C.F_CONS][C.RHOSCALAR] = f[C.F_CONS][C.RHO] * 0.5
+    sim = Sim(cfg)
+    sim.load(f)
+    vol = cfg.dx * cfg.dy * cfg.dz
+    for nsteps in (0, 2):
+        if nsteps:
+            sim.step(0.0, 1.0, nsteps)
+        cons = valid(sim.download(C.F_CONS), cfg, sim.L, C.F_CONS)
+        got = sim.integrals()
+        for c, g in zip((C.RHO, C.RHOTHETA, C.RHOSCALAR), got):
+            exact = math.fsum((cons[c] / (1.0 * 1.0) * vol).ravel())
+            assert abs(g - exact) <= abs(exact) * 2.3e-16, (c, g, exact)
+        if nsteps == 0:
+            m0 = got[0]
+    assert abs(got[0] - m0) <= 1e-14 * m0
+    sim.close()
