@@ -54,14 +54,30 @@ struct Box {
     }
 };
 
+// Work-groups are dispatched to the 8 XCDs of the chip round-robin by
+// linear id, and every XCD has its own L2.  xcd_remap turns the hardware id w
+// into a logical id so that each XCD receives one contiguous range of logical
+// ids (a bijection on [0, nb)): tiles that share halos (the next x-tile, the
+// next rows, the planes above and below) then run on the same XCD and share
+// its L2 instead of each fetching the halo from memory.
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned w, unsigned nb) {
+    const unsigned x = w % NUM_XCD, local = w / NUM_XCD;
+    const unsigned q = nb / NUM_XCD, r = nb % NUM_XCD;
+    return x * q + (x < r ? x : r) + local;
+}
+
 // Generic box launch: x-lanes along i (64-wide waves), 4 rows of j per block,
-// one z-slab per grid.z.  Every lane checks its point against the box, so any
+// tiles ordered x, y, z; every lane checks its point against the box, so any
 // grid covering the box is safe.
 template <class F>
-__global__ void __launch_bounds__(256) box_kernel(Box b, F f) {
-    int i = b.lo[0] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    int j = b.lo[1] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    int k = b.lo[2] + (int)blockIdx.z;
+__global__ void __launch_bounds__(256) box_kernel(Box b, unsigned gx, unsigned gy, F f) {
+    const unsigned nb = gridDim.x;
+    const unsigned t = xcd_remap(blockIdx.x, nb);
+    const unsigned bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    int i = b.lo[0] + (int)(bx * 64 + threadIdx.x);
+    int j = b.lo[1] + (int)(by * 4 + threadIdx.y);
+    int k = b.lo[2] + (int)bz;
     if (i > b.hi[0] || j > b.hi[1] || k > b.hi[2]) return;
     f(i, j, k);
 }
@@ -70,28 +86,30 @@ template <class F>
 inline void launch_box(const Box& b, hipStream_t s, F f) {
     if (!b.ok()) return;
     dim3 blk(64, 4, 1);
-    dim3 grd((unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64), (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4),
-             (unsigned)(b.hi[2] - b.lo[2] + 1));
-    hipLaunchKernelGGL(box_kernel<F>, grd, blk, 0, s, b, f);
+    const unsigned gx = (unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64);
+    const unsigned gy = (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4);
+    const unsigned gz = (unsigned)(b.hi[2] - b.lo[2] + 1);
+    hipLaunchKernelGGL(box_kernel<F>, dim3(gx * gy * gz), blk, 0, s, b, gx, gy, f);
 }
 
 // Column launch over a 2-D (i, j) range; the functor loops over k itself.
+// Work-groups are 256 lanes along x, so each field-level access of a group is
+// one contiguous 2 KB row segment (one DRAM-friendly burst per field and level).
 template <class F>
-__global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int jhi, F f) {
-    int i = ilo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    int j = jlo + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+__global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int jhi, unsigned gx, F f) {
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    int i = ilo + (int)((t % gx) * 256 + threadIdx.x);
+    int j = jlo + (int)(t / gx);
     if (i > ihi || j > jhi) return;
     f(i, j);
 }
 
-// Work-groups are 256 lanes along x, so each field-level access of a group is
-// one contiguous 2 KB row segment (one DRAM-friendly burst per field and level).
 template <class F>
 inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
     if (ilo > ihi || jlo > jhi) return;
-    dim3 blk(256, 1, 1);
-    dim3 grd((unsigned)((ihi - ilo + 1 + 255) / 256), (unsigned)(jhi - jlo + 1), 1);
-    hipLaunchKernelGGL(col_kernel<F>, grd, blk, 0, s, ilo, ihi, jlo, jhi, f);
+    const unsigned gx = (unsigned)((ihi - ilo + 1 + 255) / 256);
+    const unsigned gy = (unsigned)(jhi - jlo + 1);
+    hipLaunchKernelGGL(col_kernel<F>, dim3(gx * gy), dim3(256, 1, 1), 0, s, ilo, ihi, jlo, jhi, gx, f);
 }
 
 // ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
