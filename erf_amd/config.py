@@ -15,11 +15,15 @@ ADV_WENO3, ADV_WENOZ3, ADV_WENOMZQ3, ADV_WENO5, ADV_WENOZ5 = range(5, 10)
 MOLEC_NONE, MOLEC_CONSTANT, MOLEC_CONSTANT_ALPHA = range(3)
 LES_NONE, LES_SMAGORINSKY = range(2)
 RHO, RHOTHETA, RHOKE, RHOQKE, RHOSCALAR = range(5)
+RHOQ1, RHOQ2, RHOQ3 = 5, 6, 7          # vapour, cloud water, rain (Kessler)
+NCONS_DRY, NCONS_MOIST = 5, 8
+MOIST_NONE, MOIST_KESSLER, MOIST_KESSLER_NORAIN = range(3)
 
 F_CONS, F_XVEL, F_YVEL, F_ZVEL, F_R0, F_P0, F_PI0 = range(7)
 D_PRIM, D_PI_STAGE, D_FAST_COEFFS, D_EDDY = 16, 17, 18, 19
 D_XVEL_OLD, D_YVEL_OLD, D_ZVEL_OLD = 20, 21, 22
 D_XMOM_SRC, D_YMOM_SRC, D_ZMOM_SRC = 23, 24, 25
+D_RAIN_ACCUM = 26
 D_STATE_BASE = 32
 S_OLD, S_NEW, S_SUM, S_SCRATCH, F_SLOW = range(5)
 C_CONS, C_XMOM, C_YMOM, C_ZMOM = range(4)
@@ -68,6 +72,9 @@ class Config(ctypes.Structure):
         ("rayleigh_damp_U", ctypes.c_int), ("rayleigh_damp_V", ctypes.c_int),
         ("rayleigh_damp_W", ctypes.c_int), ("rayleigh_damp_T", ctypes.c_int),
         ("rayleigh_dampcoef", ctypes.c_double), ("rayleigh_zdamp", ctypes.c_double),
+        ("moisture_type", ctypes.c_int), ("buoyancy_type", ctypes.c_int),
+        ("moistscal_horiz_adv", ctypes.c_int), ("moistscal_vert_adv", ctypes.c_int),
+        ("moistscal_horiz_upw_frac", ctypes.c_double), ("moistscal_vert_upw_frac", ctypes.c_double),
     ]
 
 
@@ -132,14 +139,33 @@ def default_config(nx, ny, nz, dx, dy, dz):
     c.rayleigh_damp_U = c.rayleigh_damp_V = c.rayleigh_damp_W = c.rayleigh_damp_T = 0
     c.rayleigh_dampcoef = 0.2
     c.rayleigh_zdamp = 500.0
+    c.moisture_type = MOIST_NONE
+    c.buoyancy_type = 1
+    c.moistscal_horiz_adv = c.moistscal_vert_adv = ADV_WENO3
+    c.moistscal_horiz_upw_frac = c.moistscal_vert_upw_frac = 1.0
+    return c
+
+
+def set_moisture(c, moisture_type=MOIST_KESSLER, buoyancy_type=None):
+    """Switch a config to a Kessler model: 8 cons comps and ERF's moist default
+    buoyancy (type 2 with rain, type 1 without; ERF_DataStruct.H:126-137)."""
+    c.moisture_type = moisture_type
+    c.ncons = NCONS_MOIST
+    if buoyancy_type is None:
+        buoyancy_type = 2 if moisture_type == MOIST_KESSLER else 1
+    c.buoyancy_type = buoyancy_type
     return c
 
 
 def ghost_cells(cfg):
-    """ERF::ComputeGhostCells (Source/ERF.H:1081-1125) + 1 for cons."""
+    """ERF::ComputeGhostCells (Source/ERF.H:1081-1125) + 1 for cons.  As in ERF,
+    the moist vertical type counts only for WENO5/5Z."""
     wide = (ADV_UPWIND5, ADV_CENTERED6, ADV_WENO5, ADV_WENOZ5)
     g = 3 if any(t in wide for t in (cfg.dycore_horiz_adv, cfg.dycore_vert_adv,
-                                      cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv)) else 2
+                                      cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv,
+                                      cfg.moistscal_horiz_adv)) else 2
+    if cfg.moistscal_vert_adv in (ADV_WENO5, ADV_WENOZ5):
+        g = 3
     return g + 1
 
 
@@ -164,3 +190,11 @@ def ncomp_of(cfg, fid):
     if fid == D_EDDY:
         return 13
     return 1
+
+
+def field_size(cfg, fid):
+    """Doubles a host buffer for field fid holds."""
+    L = layout_for(cfg)
+    if fid == D_RAIN_ACCUM:
+        return L.sz
+    return ncomp_of(cfg, fid) * L.nelem

@@ -50,12 +50,28 @@ void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
     P.consA = c.molec_diff_type == ERFX_MOLEC_CONSTANT_ALPHA;
     P.use_turb = c.les_type == ERFX_LES_SMAGORINSKY;
     P.use_diff = c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE;
+    P.moist = c.moisture_type != ERFX_MOIST_NONE;
+    P.n_qstate = P.moist ? 3 : 0;
 }
 
 static int validate(const erfx_config& c) {
     if (c.nx < 4 || c.ny < 4 || c.nz < 4) { set_error("box must be at least 4 cells in each direction"); return ERFX_ERR_ARG; }
     if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) { set_error("cell sizes must be positive"); return ERFX_ERR_ARG; }
-    if (c.ncons != ERFX_NCONS_DRY) { set_error("only the dry state (ncons = 5) is supported"); return ERFX_ERR_UNSUPPORTED; }
+    if (c.moisture_type < ERFX_MOIST_NONE || c.moisture_type > ERFX_MOIST_KESSLER_NORAIN) {
+        set_error("bad moisture_type"); return ERFX_ERR_ARG;
+    }
+    const int want_ncons = c.moisture_type == ERFX_MOIST_NONE ? ERFX_NCONS_DRY : ERFX_NCONS_MOIST;
+    if (c.ncons != want_ncons) {
+        set_error("ncons must be 5 without moisture and 8 with a Kessler model"); return ERFX_ERR_ARG;
+    }
+    if (c.moisture_type == ERFX_MOIST_NONE && c.buoyancy_type != 1) {
+        set_error("only buoyancy_type 1 is implemented for the dry state"); return ERFX_ERR_UNSUPPORTED;
+    }
+    if (c.buoyancy_type < 1 || c.buoyancy_type > 4) { set_error("buoyancy_type must be 1, 2, 3 or 4"); return ERFX_ERR_ARG; }
+    if (c.moisture_type == ERFX_MOIST_KESSLER_NORAIN && c.buoyancy_type != 1) {
+        // asserted by ERF (ERF_make_buoyancy.cpp:171-175)
+        set_error("Kessler_NoRain requires buoyancy_type 1"); return ERFX_ERR_ARG;
+    }
     for (int o = 0; o < 6; ++o) {
         if (c.bc[o] < ERFX_BC_PERIODIC || c.bc[o] > ERFX_BC_OUTFLOW) { set_error("bad bc type"); return ERFX_ERR_ARG; }
         if ((c.bc[o] == ERFX_BC_PERIODIC) != (c.bc[o % 3 == o ? o + 3 : o - 3] == ERFX_BC_PERIODIC)) {
@@ -71,7 +87,7 @@ static int validate(const erfx_config& c) {
         if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_CENTERED6) {
             set_error("unsupported dycore advection scheme (WENO is for scalars only)"); return ERFX_ERR_UNSUPPORTED;
         }
-    int scal[2] = {c.dryscal_horiz_adv, c.dryscal_vert_adv};
+    int scal[4] = {c.dryscal_horiz_adv, c.dryscal_vert_adv, c.moistscal_horiz_adv, c.moistscal_vert_adv};
     for (int a : scal)
         if (a < ERFX_ADV_CENTERED2 || a > ERFX_ADV_WENOZ5) { set_error("unsupported scalar advection scheme"); return ERFX_ERR_UNSUPPORTED; }
     if (c.molec_diff_type < 0 || c.molec_diff_type > 2 || c.les_type < 0 || c.les_type > 1) {
@@ -102,6 +118,8 @@ struct erfx_ctx {
     FastWork fw{};
     SlowWork sw{};
     double* ray = nullptr;    // Rayleigh profiles and per-level factors (RayleighTable layout)
+    double* kfz = nullptr;    // Kessler sedimentation flux on z-faces (moist only)
+    int* kbad = nullptr;      // device word: a Kessler temperature left the saturation fit's range
     std::vector<void*> allocs;
 };
 
@@ -120,6 +138,7 @@ struct erfx_sim {
     KTimer timer;
     double* red = nullptr;    // partial maxima of the time-step estimate (EST_BLOCKS x 2)
     double* red_host = nullptr;
+    double* rain_accum = nullptr;   // Kessler surface rain, k = 0 plane (sz doubles)
 };
 
 static void timer_free(KTimer& t) {
@@ -158,6 +177,10 @@ void erfx_config_defaults(erfx_config* c) {
     c->implicit_substepping = 1;
     c->rayleigh_dampcoef = 0.2;
     c->rayleigh_zdamp = 500.0;
+    c->moisture_type = ERFX_MOIST_NONE;
+    c->buoyancy_type = 1;
+    c->moistscal_horiz_adv = c->moistscal_vert_adv = ERFX_ADV_WENO3;
+    c->moistscal_horiz_upw_frac = c->moistscal_vert_upw_frac = 1.0;
 }
 
 int erfx_ghost_cells(const erfx_config* c) {
@@ -166,7 +189,9 @@ int erfx_ghost_cells(const erfx_config* c) {
         return t == ERFX_ADV_UPWIND5 || t == ERFX_ADV_CENTERED6 || t == ERFX_ADV_WENO5 || t == ERFX_ADV_WENOZ5;
     };
     int g = (wide(c->dycore_horiz_adv) || wide(c->dycore_vert_adv) || wide(c->dryscal_horiz_adv) ||
-             wide(c->dryscal_vert_adv)) ? 3 : 2;
+             wide(c->dryscal_vert_adv) || wide(c->moistscal_horiz_adv)) ? 3 : 2;
+    // ERF tests the moist vertical type only against WENO5/5Z
+    if (c->moistscal_vert_adv == ERFX_ADV_WENO5 || c->moistscal_vert_adv == ERFX_ADV_WENOZ5) g = 3;
     return g + 1;
 }
 
@@ -211,6 +236,18 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
         if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
     if ((rc = dalloc(c->allocs, &c->fw.trhs, 2 * ne))) { erfx_ctx_destroy(c); return rc; }
     c->sw.xsrc = c->sw.ysrc = c->sw.zsrc = nullptr;
+    if (c->P.moist) {
+        const size_t nz = (size_t)lay.nz;
+        double* kb = nullptr;
+        if ((rc = dalloc(c->allocs, &c->kfz, ne)) ||
+            (rc = dalloc(c->allocs, &c->sw.pavg, (size_t)PLANE_NOUT * nz)) ||
+            (rc = dalloc(c->allocs, &c->sw.pavg_part, nz * PLANE_SPLIT_MAX * 2 * PLANE_ROWS)) ||
+            (rc = dalloc(c->allocs, &kb, 1))) {
+            erfx_ctx_destroy(c);
+            return rc;
+        }
+        c->kbad = reinterpret_cast<int*>(kb);
+    }
     *out = c;
     return ERFX_OK;
 }
@@ -291,6 +328,19 @@ static bool rayleigh_on(const erfx_config& c) {
 
 static bool bad_state(const erfx_state* s) { return !s || !s->cons || !s->xmom || !s->ymom || !s->zmom; }
 
+// after a stream sync: report (and clear) a temperature out of the saturation fit's range
+static int kessler_status(erfx_ctx* c) {
+    if (!c->kbad) return ERFX_OK;
+    int h = 0;
+    ERFX_HIP(hipMemcpy(&h, c->kbad, sizeof(int), hipMemcpyDeviceToHost));
+    if (h) {
+        ERFX_HIP(hipMemset(c->kbad, 0, sizeof(int)));
+        set_error("Kessler: temperature below -85 C, outside the saturation fit (ERF_Microphysics_Utils.H:62)");
+        return ERFX_ERR_STATE;
+    }
+    return ERFX_OK;
+}
+
 int erfx_cons_to_prim(erfx_ctx* c, const double* cons, double* prim, double* pi_stage, int ng) {
     if (!c || !cons || !prim || !pi_stage) { set_error("null argument"); return ERFX_ERR_ARG; }
     if (ng < 1 || ng > c->P.ng_cons) { set_error("ng out of range"); return ERFX_ERR_ARG; }
@@ -352,6 +402,18 @@ int erfx_slow_rhs_pre(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, con
     return launch_status();
 }
 
+int erfx_kessler_advance(erfx_ctx* c, double* cons, double dt, double* rain_accum) {
+    if (!c || !cons) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!c->P.moist) { set_error("no moisture model configured"); return ERFX_ERR_ARG; }
+    if (!(dt > 0)) { set_error("dt must be positive"); return ERFX_ERR_ARG; }
+    kessler_advance(c->P, cons, c->kfz, rain_accum, dt, c->stream, c->kbad);
+    int rc = launch_status();
+    if (rc) return rc;
+    // the range check of the saturation fit needs the result (ERF aborts there)
+    ERFX_HIP(hipStreamSynchronize(c->stream));
+    return kessler_status(c);
+}
+
 int erfx_slow_rhs_post(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, const erfx_state* old,
                        erfx_state* snew, erfx_state* data, const double* prim, const erfx_state* scratch,
                        const double* xv, const double* yv, const double* eddy) {
@@ -398,6 +460,7 @@ int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
         set_error("hipHostMalloc failed"); rc = ERFX_ERR_HIP;
     }
     if (!rc && P.use_turb) rc = A(&s->eddy, 13 * ne);
+    if (!rc && P.moist) rc = A(&s->rain_accum, (size_t)P.L.sz);
     if (rc) { erfx_sim_destroy(s); return rc; }
     // S_old and S_new cons alias vars_old / vars_new (ERF_Advance.cpp:127-160)
     s->S[0].cons = s->cons[1 - s->cur];
@@ -427,10 +490,15 @@ int erfx_sim_layout(const erfx_sim* s, erfx_layout* out) {
 erfx_ctx* erfx_sim_ctx(erfx_sim* s) { return s ? s->ctx : nullptr; }
 void* erfx_sim_stream(erfx_sim* s) { return s ? (void*)s->stream : nullptr; }
 
-static double* field_ptr(erfx_sim* s, int id, int* ncomp) {
+// device pointer of a sim field and its size in doubles
+static double* field_ptr(erfx_sim* s, int id, long* count) {
     const int nc = s->ctx->P.cfg.ncons;
     int n = 1;
     double* p = nullptr;
+    if (id == ERFX_D_RAIN_ACCUM) {
+        if (count) *count = s->rain_accum ? (long)s->ctx->P.L.sz : 0;
+        return s->rain_accum;
+    }
     switch (id) {
         case ERFX_F_CONS: p = s->cons[s->cur]; n = nc; break;
         case ERFX_F_XVEL: p = s->vel[s->cur][0]; break;
@@ -455,7 +523,7 @@ static double* field_ptr(erfx_sim* s, int id, int* ncomp) {
                 n = c == 0 ? nc : 1;
             }
     }
-    if (ncomp) *ncomp = n;
+    if (count) *count = (long)n * s->ctx->P.L.ne;
     return p;
 }
 
@@ -463,27 +531,27 @@ double* erfx_sim_field_ptr(erfx_sim* s, int id) { return s ? field_ptr(s, id, nu
 
 long erfx_sim_field_size(const erfx_sim* s, int id) {
     if (!s) return -1;
-    int n = 0;
+    long n = 0;
     double* p = field_ptr(const_cast<erfx_sim*>(s), id, &n);
-    return p ? (long)n * s->ctx->P.L.ne : -1;
+    return p ? n : -1;
 }
 
 int erfx_sim_upload(erfx_sim* s, int id, const double* host) {
     if (!s || !host) { set_error("null argument"); return ERFX_ERR_ARG; }
-    int n = 0;
+    long n = 0;
     double* p = field_ptr(s, id, &n);
     if (!p) { set_error("unknown field id"); return ERFX_ERR_ARG; }
-    ERFX_HIP(hipMemcpyAsync(p, host, (size_t)n * s->ctx->P.L.ne * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    ERFX_HIP(hipMemcpyAsync(p, host, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s->stream));
     ERFX_HIP(hipStreamSynchronize(s->stream));
     return ERFX_OK;
 }
 
 int erfx_sim_download(erfx_sim* s, int id, double* host) {
     if (!s || !host) { set_error("null argument"); return ERFX_ERR_ARG; }
-    int n = 0;
+    long n = 0;
     double* p = field_ptr(s, id, &n);
     if (!p) { set_error("unknown field id"); return ERFX_ERR_ARG; }
-    ERFX_HIP(hipMemcpyAsync(host, p, (size_t)n * s->ctx->P.L.ne * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    ERFX_HIP(hipMemcpyAsync(host, p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     ERFX_HIP(hipStreamSynchronize(s->stream));
     return ERFX_OK;
 }
@@ -561,6 +629,8 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
         slow_rhs_post(P, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);
         apply_bcs(P, Sn, un, vn, wn, P.ng_cons, P.ng_vel, false, false, st);
     }
+    // advance_microphysics on the new state (ERF_Advance.cpp:172-174)
+    if (P.moist) kessler_advance(P, Sn.cons, c->kfz, s->rain_accum, dt, st, c->kbad);
     s->cur = nw;
     s->S[0].cons = s->cons[1 - s->cur];
     s->S[1].cons = s->cons[s->cur];
@@ -638,7 +708,7 @@ int erfx_sim_run(erfx_sim* s, double t0, int max_steps, const erfx_dt_control* d
     ERFX_HIP(hipStreamSynchronize(s->stream));
     if (t_end) *t_end = t;
     if (steps_done) *steps_done = done;
-    return ERFX_OK;
+    return kessler_status(s->ctx);
 }
 
 // ERF::sum_integrated_quantities' volume-weighted sums (volWgtSumMF,
@@ -656,7 +726,7 @@ int erfx_sim_integrals(erfx_sim* s, double* mass, double* rhotheta, double* scal
 int erfx_sim_sync(erfx_sim* s) {
     if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
     ERFX_HIP(hipStreamSynchronize(s->stream));
-    return ERFX_OK;
+    return kessler_status(s->ctx);
 }
 
 // Times the acoustic column solve, the step's dominant kernel, on up to

@@ -26,6 +26,11 @@ constexpr double p_0 = 1.0e5;
 constexpr double Gamma = 1.4;
 constexpr double CONST_GRAV = 9.81;
 constexpr double ip_0 = 1. / p_0;
+constexpr double L_v = 2.5e6;          // ERF_Constants.H:16
+constexpr double lcond = 2.501e6;      // ERF_Constants.H:66
+constexpr double qcw0 = 1.e-3;         // ERF_Constants.H:46
+constexpr double alphaelq = 1.e-3;     // ERF_Constants.H:48
+constexpr double Rd_on_Rv = R_d / R_v; // ERF_Constants.H:87
 
 // ---- EOS (Source/Utils/ERF_EOS.H:89-93, 130-135), dry ---------------------
 __device__ __forceinline__ double eos_p_of_rt(double rt) {
@@ -33,6 +38,15 @@ __device__ __forceinline__ double eos_p_of_rt(double rt) {
 }
 __device__ __forceinline__ double eos_exner_of_rt(double rt, double rdOcp) {
     return erfx_pow(R_d * rt * (1.0 + R_v / R_d * 0.0) * ip_0, Gamma * rdOcp);
+}
+// moist forms: the pressure of the moist potential temperature theta (1 + R_v/R_d qv)
+// (ERF_EOS.H:76-79) and the temperature (ERF_EOS.H:42-51)
+__device__ __forceinline__ double eos_p_of_rt_qv(double rt, double qv) {
+    return p_0 * erfx_pow(R_d * rt * (1.0 + (R_v / R_d) * qv) * ip_0, Gamma);
+}
+__device__ __forceinline__ double eos_T(double rho, double rt, double qv) {
+    double p_loc = p_0 * erfx_pow(R_d * rt * (1.0 + R_v / R_d * qv) * ip_0, Gamma);
+    return p_loc / (R_d * rho * (1.0 + R_v / R_d * qv));
 }
 
 // ---- index space ------------------------------------------------------------
@@ -151,7 +165,18 @@ struct Params {
     bool use_diff, use_turb, consA;
     KTimer* timer = nullptr;  // optional launch timing (host pointer, never read on the device)
     const double* ray = nullptr;  // Rayleigh table (device), RAY_* blocks of nz+1 values
+    bool moist = false;       // a Kessler model: cons comps 5..7 are RhoQ1..RhoQ3
+    int n_qstate = 0;         // 3 with Kessler (ERF_Kessler.H:172)
 };
+
+// prim components of the moisture species (prim(n) = cons(n+1) / rho)
+constexpr int PRIM_Q1 = ERFX_RHOQ1 - 1, PRIM_Q2 = ERFX_RHOQ2 - 1, PRIM_Q3 = ERFX_RHOQ3 - 1;
+// 1 + qv + qc on the face between cells a and b, the moist divisor of the
+// pressure gradients (ERF_slow_rhs_pre.cpp:622-626; ERF_fast_rhs_N.cpp:215-219;
+// ERF_make_fast_coeffs.cpp:183-188): q = 0.5 (qv_a + qv_b + qc_a + qc_b)
+__device__ __forceinline__ double face_q(const double* pq1, const double* pq2, long a, long b) {
+    return 0.5 * (pq1[a] + pq1[b] + pq2[a] + pq2[b]);
+}
 
 // blocks of the Rayleigh table
 enum { RAY_UBAR = 0, RAY_VBAR, RAY_WBAR, RAY_THETABAR, RAY_FAC_CC, RAY_FAC_ND, RAY_ACT_CC, RAY_ACT_ND, RAY_NFIELDS };

@@ -4,7 +4,9 @@
 //    reduction, so the result equals the serial loop's bit for bit);
 //  * the volume-weighted sums of ERF::volWgtSumMF (Source/IO/ERF_WriteScalarProfiles.cpp:378-410),
 //    kept in double-double all the way, so the result is the exactly rounded
-//    sum of the per-cell products whatever the reduction order.
+//    sum of the per-cell products whatever the reduction order;
+//  * the plane averages of the moist buoyancy (PlaneAverage, Source/Utils/ERF_PlaneAverage.H:184-239),
+//    per level, summed the same way.
 #include "erfx_common.h"
 #include "erfx_internal.h"
 
@@ -147,6 +149,64 @@ void volume_sums(const Params& P, const double* cons, double* partial, double* h
     hipLaunchKernelGGL(sums_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
                        partial + 6 * EST_BLOCKS);
     (void)hipMemcpyAsync(host, partial + 6 * EST_BLOCKS, 6 * sizeof(double), hipMemcpyDeviceToHost, s);
+}
+
+// ---- plane averages ------------------------------------------------------------
+// block (b, k) sums the plane terms value * denom of its share of level k
+__global__ void __launch_bounds__(256) plane_pass1(Lay L, const double* cons, const double* prim, double denom,
+                                                   int nq, int nsplit, double* partial) {
+    const int nx = L.nx, ny = L.ny;
+    const int k = blockIdx.y;
+    const long n = (long)nx * ny;
+    DD acc[PLANE_ROWS];
+    for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = DD{0.0, 0.0};
+    const double* pth = prim;
+    const double* pq[3] = {prim + (ERFX_RHOQ1 - 1) * L.ne, prim + (ERFX_RHOQ2 - 1) * L.ne,
+                           prim + (ERFX_RHOQ3 - 1) * L.ne};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)nsplit * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)(t / nx);
+        const long q = L.ix(i, j, k);
+        const double fac = 1.0;
+        double v[PLANE_ROWS] = {cons[q], pth[q], 0.0, 0.0, 0.0};
+        for (int c = 0; c < nq && c < 3; ++c) v[2 + c] = pq[c][q];
+        for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = dd_add(acc[r], DD{v[r] * denom * fac, 0.0});
+    }
+    block_sum<PLANE_ROWS>(acc, partial + ((long)k * nsplit + blockIdx.x) * 2 * PLANE_ROWS);
+}
+
+// block k folds the nsplit partials of level k, then the temperature of the
+// averages (getTgivenRandRTh(rho_d, rho_d theta_d, qv_d), ERF_make_buoyancy.cpp:274-275)
+__global__ void __launch_bounds__(256) plane_pass2(const double* partial, int nsplit, int nz, double* out) {
+    const int k = blockIdx.x;
+    DD acc[PLANE_ROWS];
+    for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = DD{0.0, 0.0};
+    const double* p = partial + (long)k * nsplit * 2 * PLANE_ROWS;
+    for (int t = threadIdx.x; t < nsplit; t += blockDim.x)
+        for (int r = 0; r < PLANE_ROWS; ++r)
+            acc[r] = dd_add(acc[r], DD{p[t * 2 * PLANE_ROWS + 2 * r], p[t * 2 * PLANE_ROWS + 2 * r + 1]});
+    __shared__ double res[2 * PLANE_ROWS];
+    block_sum<PLANE_ROWS>(acc, res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < PLANE_ROWS; ++r) out[r * nz + k] = res[2 * r];
+        const double rho_d = res[0], theta_d = res[2], qv_d = res[4];
+        out[PLANE_T * nz + k] = eos_T(rho_d, rho_d * theta_d, qv_d);
+    }
+}
+
+void plane_averages(const Params& P, const double* cons, const double* prim, double* partial, double* out,
+                    hipStream_t s) {
+    const int nz = P.L.nz;
+    const long nplane = (long)P.L.nx * P.L.ny;
+    int nsplit = (2048 + nz - 1) / nz;
+    if (nsplit > PLANE_SPLIT_MAX) nsplit = PLANE_SPLIT_MAX;
+    const long need = (nplane + 255) / 256;
+    if (nsplit > need) nsplit = (int)need;
+    if (nsplit < 1) nsplit = 1;
+    const double denom = 1.0 / (double)nplane;
+    hipLaunchKernelGGL(plane_pass1, dim3(nsplit, nz), dim3(256), 0, s, P.L, cons, prim, denom, P.n_qstate, nsplit,
+                       partial);
+    hipLaunchKernelGGL(plane_pass2, dim3(nz), dim3(256), 0, s, (const double*)partial, nsplit, nz, out);
 }
 
 }  // namespace erfx

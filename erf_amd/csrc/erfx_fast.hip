@@ -36,6 +36,9 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
     double* Pc = fc + 3 * ne;
     double* Qc = fc + 4 * ne;
     const double D = dtau * dtau * beta_2 * beta_2 * dzi;
+    const bool moist = P.moist;
+    const double* pq1 = prim + PRIM_Q1 * ne;
+    const double* pq2 = prim + PRIM_Q2 * ne;
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // bottom face, then the interior faces with the recurrence carried along
@@ -44,9 +47,11 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
         double c_prev = 0.0;
         double r_lo = r0[q0], pib_lo = pi0[q0], pis_lo = pi_stage[q0], rt_lo = rt[q0];
         double pr_km2 = prim[q0 - sz], pr_km1 = prim[q0], pr_k = prim[q0 + sz];
-        struct Lv { double r, pib, pis, rt_, pkp1; };
+        double q1_lo = moist ? pq1[q0] : 0.0, q2_lo = moist ? pq2[q0] : 0.0;
+        struct Lv { double r, pib, pis, rt_, pkp1, q1, q2; };
         auto load = [&](long q, Lv& v) {
             v.r = r0[q]; v.pib = pi0[q]; v.pis = pi_stage[q]; v.rt_ = rt[q]; v.pkp1 = prim[q + sz];
+            if (moist) { v.q1 = pq1[q]; v.q2 = pq2[q]; } else { v.q1 = 0.0; v.q2 = 0.0; }
         };
         constexpr int RD = 4;
         Lv ring[RD];
@@ -70,6 +75,11 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
                 coeff_Q *= pi_c;
                 Pc[q] = coeff_P;
                 Qc[q] = coeff_Q;
+                if (moist) {   // ERF_make_fast_coeffs.cpp:183-188
+                    double qf = 0.5 * (v.q1 + q1_lo + v.q2 + q2_lo);
+                    coeff_P /= (1.0 + qf);
+                    coeff_Q /= (1.0 + qf);
+                }
                 double theta_t_lo = 0.5 * (pr_km2 + pr_km1);
                 double theta_t_mid = 0.5 * (pr_km1 + pr_k);
                 double theta_t_hi = 0.5 * (pr_k + v.pkp1);
@@ -84,6 +94,7 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
                 B[q] = 1.0 / bet;                        // interior faces store 1/B (:287-290)
                 r_lo = v.r; pib_lo = v.pib; pis_lo = v.pis; rt_lo = v.rt_;
                 pr_km2 = pr_km1; pr_km1 = pr_k; pr_k = v.pkp1;
+                q1_lo = v.q1; q2_lo = v.q2;
             }
         }
         // top face keeps B itself
@@ -131,6 +142,9 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     const bool rt = ch.rt_prev && step > 0;
     const double* prho = prev.cons;
     const double gr = -Gamma * R_d;
+    const bool moist = P.moist;
+    const double* pq1 = prim + PRIM_Q1 * ne;
+    const double* pq2 = prim + PRIM_Q2 * ne;
     launch_box(Box(0, 0, 0, nx, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (j <= ny - 1) {
@@ -145,6 +159,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 new_drho_u = pm - stage.xmom[q] + dtau * slow_rhs.xmom[q];
             } else {
                 double gpx = (extrap[q] - extrap[q - 1]) * dxi;
+                if (moist) gpx /= (1.0 + face_q(pq1, pq2, q, q - 1));   // :215-219
                 double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
                 double fast_rhs_rho_u = gr * pi_c * gpx;
                 new_drho_u = pm - stage.xmom[q] + dtau * fast_rhs_rho_u + dtau * slow_rhs.xmom[q];
@@ -164,6 +179,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 new_drho_v = pm - stage.ymom[q] + dtau * slow_rhs.ymom[q];
             } else {
                 double gpy = (extrap[q] - extrap[q - sy]) * dyi;
+                if (moist) gpy /= (1.0 + face_q(pq1, pq2, q, q - sy));  // :238-242
                 double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
                 double fast_rhs_rho_v = gr * pi_c * gpy;
                 new_drho_v = pm - stage.ymom[q] + dtau * fast_rhs_rho_v + dtau * slow_rhs.ymom[q];
@@ -201,6 +217,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         struct Fw {
             double P, Q, A, iB, ws, pkp1, pz, szz, pr, rkp1, st, prt_, srt_, s0, s1;
             double dxl, dxh, sxl, sxh, dyl, dyh, syl, syh, pc, pxm, pxp, pym, pyp;
+            double q1, q2;   // qv, qc of cell k (moist only)
         };
         auto fload = [&](long q, Fw& v) {
             v.P = Pc[q]; v.Q = Qc[q]; v.A = A[q]; v.iB = iB[q]; v.ws = slow_rhs.zmom[q];
@@ -210,6 +227,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
             v.dxl = data.xmom[q]; v.dxh = data.xmom[q + 1]; v.sxl = stage.xmom[q]; v.sxh = stage.xmom[q + 1];
             v.dyl = data.ymom[q]; v.dyh = data.ymom[q + sy]; v.syl = stage.ymom[q]; v.syh = stage.ymom[q + sy];
             v.pc = prim[q]; v.pxm = prim[q - 1]; v.pxp = prim[q + 1]; v.pym = prim[q - sy]; v.pyp = prim[q + sy];
+            if (moist) { v.q1 = pq1[q]; v.q2 = pq2[q]; } else { v.q1 = 0.0; v.q2 = 0.0; }
         };
         // horizontal flux divergence of the increments (K3)
         auto hdiv = [&](const Fw& v, double& t0, double& t1) {
@@ -250,6 +268,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double drt_km1 = cur.prt_ - cur.srt_;
         double s0_km1 = cur.s0, s1_km1 = cur.s1;
         double p_km2 = prim[q0 - sz], p_km1 = cur.pc, p_k = cur.pkp1;
+        double q1_km1 = cur.q1, q2_km1 = cur.q2;
         double Om_k = w_above(cur, 1) - cur.szz;
         // ring of FD levels of loads in flight (slots indexed statically after unrolling)
         constexpr int FD = 3;
@@ -274,6 +293,11 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 double drho_k = c.pr - c.st;
                 double drt_k = c.prt_ - c.srt_;
                 double coeff_P = c.P, coeff_Q = c.Q;
+                if (moist) {   // :379-384
+                    double qf = 0.5 * (c.q1 + q1_km1 + c.q2 + q2_km1);
+                    coeff_P /= (1.0 + qf);
+                    coeff_Q /= (1.0 + qf);
+                }
                 double theta_t_lo = 0.5 * (p_km2 + p_km1);
                 double theta_t_mid = 0.5 * (p_km1 + p_k);
                 double theta_t_hi = 0.5 * (p_k + c.pkp1);
@@ -288,6 +312,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 soln[q] = sol;
                 sol_prev = sol;
                 p_km2 = p_km1; p_km1 = p_k; p_k = c.pkp1;
+                q1_km1 = c.q1; q2_km1 = c.q2;
                 Om_km1 = Om_k; Om_k = Om_kp1;
                 drho_km1 = drho_k; drt_km1 = drt_k;
                 s0_km1 = c.s0; s1_km1 = c.s1;
@@ -370,7 +395,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     // prev/stage rho*w, prev/stage rho and rho*theta, slow rho, rho*theta and
     // rho*w, new and stage x- and y-momenta, rho and rho*theta of the state being
     // updated, averaged rho*w) and 4 written (rho*w, averaged rho*w, rho, rho*theta)
-    kt_stop(P.timer, KT_FAST_COLUMN, s, 26.0 * 8.0 * nx * ny * (nz + 1), "fast_rhs_N column (K3+K4+K5)");
+    kt_stop(P.timer, KT_FAST_COLUMN, s, (moist ? 28.0 : 26.0) * 8.0 * nx * ny * (nz + 1), "fast_rhs_N column (K3+K4+K5)");
 }
 
 }  // namespace erfx

@@ -55,12 +55,27 @@ void est_dt_maxima(const Params& P, const double* cons, const double* u, const d
 // double-double volume sums of rho, rho*theta, RhoScalar: host[0..5] = (hi, lo) x 3
 // (partial needs 6 * EST_BLOCKS + 6 doubles)
 void volume_sums(const Params& P, const double* cons, double* partial, double* host, hipStream_t s);
+// PlaneAverage rows of the moist buoyancy (ERF_make_buoyancy.cpp:216-255) into
+// out[r * nz + k]: r = 0 rho, 1 theta, 2 qv, 3 qc, 4 qp (exactly rounded sums of
+// value / (nx ny) over level k), and PLANE_T = the temperature of those averages.
+constexpr int PLANE_ROWS = 5, PLANE_T = 5, PLANE_NOUT = 6;
+constexpr int PLANE_SPLIT_MAX = 64;
+// partial needs nz * PLANE_SPLIT_MAX * 2 * PLANE_ROWS doubles; out PLANE_NOUT * nz
+void plane_averages(const Params& P, const double* cons, const double* prim, double* partial, double* out,
+                    hipStream_t s);
+// Kessler microphysics on cons (erfx_kessler.hip); fz: z-face scratch field,
+// rain_accum: k = 0 plane or null; *bad (device) is set to 1 when a
+// temperature falls below the saturation fit's range (ERF aborts there)
+void kessler_advance(const Params& P, double* cons, double* fz, double* rain_accum, double dt, hipStream_t s,
+                     int* bad);
 
 // erfx_slow.hip
 struct SlowWork {
     double* pp;                   // perturbational pressure, cells grown by 1
     double* tau[6];               // 11, 22, 33, 12, 13, 23
     double* xsrc; double* ysrc; double* zsrc;
+    double* pavg = nullptr;       // moist buoyancy plane averages (PLANE_NOUT x nz)
+    double* pavg_part = nullptr;  // their partial sums
 };
 void turbulent_viscosity(const Params& P, const SlowWork& W, const double* cons, const double* u,
                          const double* v, const double* w, double* eddy, hipStream_t s);

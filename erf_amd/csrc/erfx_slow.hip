@@ -382,7 +382,31 @@ __device__ __forceinline__ double diff_flux(const double* cons, const double* cp
     return -rhoAlpha * (cp[q] - cp[q - st]) * dinv;
 }
 
-// erf_slow_rhs_pre with make_mom_sources (ERF_TI_slow_rhs_fun.H:216-241), flat, dry.
+// Departure of cell q (level k) from the level's plane averages in the moist
+// buoyancy of types 2, 3 and 4 (ERF_make_buoyancy.cpp:272-323, 347-372).
+// pavg rows of nz values: rho, theta, qv, qc, qp averages, then the
+// temperature of the averages (PLANE_T).
+__device__ __forceinline__ double moist_buoyancy_dev(int btype, int nq, const double* cons, const double* pq1,
+                                                     const double* pq2, const double* pq3, const double* pavg,
+                                                     int nz, long ne, long q, int k) {
+    const double rho_d = pavg[k], theta_d = pavg[nz + k], qv_d = pavg[2 * nz + k];
+    const double qc_d = pavg[3 * nz + k], qp_d = pavg[4 * nz + k];
+    const double temp1d = pavg[PLANE_T * nz + k];
+    const double rho = cons[q], rt = cons[q + ne];
+    const double temp3d = eos_T(rho, rt, cons[q + ERFX_RHOQ1 * ne] / rho);
+    const double qv = (nq >= 1) ? pq1[q] : 0.0;
+    const double qc = (nq >= 2) ? pq2[q] : 0.0;
+    const double qp = (nq >= 3) ? pq3[q] : 0.0;
+    (void)rho_d;
+    if (btype == 2)
+        return 0.61 * (qv - qv_d) - (qc - qc_d + qp - qp_d) +
+               (temp3d - temp1d) / temp1d * (1.0 + 0.61 * qv_d - qc_d - qp_d);
+    if (btype == 4)
+        return 0.61 * (qv - qv_d) - (qc - qc_d + qp - qp_d) + (rt / rho - theta_d) / theta_d;
+    return 0.61 * qv - (qc + qp) + (temp3d - temp1d) / temp1d;
+}
+
+// erf_slow_rhs_pre with make_mom_sources (ERF_TI_slow_rhs_fun.H:216-241), flat.
 void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, const erfx_state& rhs,
                   const erfx_state& data, const double* prim, const erfx_state& scratch, const double* u,
                   const double* v, const double* w, const double* eddy, const double* r0, const double* p0,
@@ -398,10 +422,23 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     // p' on cells grown by one, k >= 0 (ERF_slow_rhs_pre.cpp:337-358)
     double* pp = W.pp;
     const double* rt = data.cons + ne;
-    launch_box(Box(-1, -1, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        pp[q] = eos_p_of_rt(rt[q]) - p0[q];
-    });
+    const bool moist = P.moist;
+    if (!moist) {
+        launch_box(Box(-1, -1, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            pp[q] = eos_p_of_rt(rt[q]) - p0[q];
+        });
+    } else {
+        const double* rq1 = data.cons + ERFX_RHOQ1 * ne;
+        const double* rho = data.cons;
+        launch_box(Box(-1, -1, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            double qv_for_p = rq1[q] / rho[q];
+            pp[q] = eos_p_of_rt_qv(rt[q], qv_for_p) - p0[q];
+        });
+        // plane averages and the level temperatures of the moist buoyancy (types 2-4)
+        if (P.cfg.buoyancy_type != 1) plane_averages(P, data.cons, prim, W.pavg_part, W.pavg, s);
+    }
 
     // cell RHS of rho and rho*theta: AdvectionSrcForRho, AdvectionSrcForScalars
     // (theta, dycore scheme) and DiffusionSrcForState_N (theta)
@@ -488,6 +525,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     double* rvr = rhs.ymom;
     double* rwr = rhs.zmom;
     const double* om = rw;   // Omega = rho*w on a flat grid
+    const double* pq1 = prim + PRIM_Q1 * ne;
+    const double* pq2 = prim + PRIM_Q2 * ne;
+    const double* pq3 = prim + PRIM_Q3 * ne;
+    const int btype = moist ? P.cfg.buoyancy_type : 1;
+    const int nqs = P.n_qstate;
+    const double* pavg = W.pavg;
     launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         // ---- x-momentum on x-faces ----
@@ -537,7 +580,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             xs += 0.5 * (cons[q] + cons[q - 1]) * g0;
             double gpx = dxi * (pp[q] - pp[q - 1]);
             gpx *= 1.0;
-            r += (-gpx - 0.0) / (1.0 + 0.0) + xs;
+            double qf = moist ? face_q(pq1, pq2, q, q - 1) : 0.0;
+            r += (-gpx - 0.0) / (1.0 + qf) + xs;
             if ((zero_xlo && i == 0) || (zero_xhi && i == nx)) r = 0.;
             rur[q] = r;
         }
@@ -587,7 +631,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             ys += 0.5 * (cons[q] + cons[q - sy]) * g1;
             double gpy = dyi * (pp[q] - pp[q - sy]);
             gpy *= 1.0;
-            r += (-gpy - 0.0) / (1.0 + 0.0) + ys;
+            double qf = moist ? face_q(pq1, pq2, q, q - sy) : 0.0;
+            r += (-gpy - 0.0) / (1.0 + qf) + ys;
             if ((zero_ylo && j == 0) || (zero_yhi && j == ny)) r = 0.;
             rvr[q] = r;
         }
@@ -655,7 +700,28 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             if (use_diff)
                 r -= ((t13[q + 1] - t13[q]) * dxi * 1.0 + (t23[q + sy] - t23[q]) * dyi * 1.0 +
                       (t33[q] - t33[q - sz]) * dzi);
-            double zs = grav_z * 0.5 * ((cons[q] - r0[q]) + (cons[q - sz] - r0[q - sz]));
+            double zs;
+            if (!moist) {
+                zs = grav_z * 0.5 * ((cons[q] - r0[q]) + (cons[q - sz] - r0[q - sz]));
+            } else if (btype == 1) {
+                // ERF_make_buoyancy.cpp:195-208
+                const double* c1 = cons + ERFX_RHOQ1 * ne;
+                const double* c2 = cons + ERFX_RHOQ2 * ne;
+                double rhop_hi = cons[q] + c1[q] + c2[q] - r0[q];
+                double rhop_lo = cons[q - sz] + c1[q - sz] + c2[q - sz] - r0[q - sz];
+                for (int q_offset = 2; q_offset < nqs; ++q_offset) {
+                    rhop_hi += cons[q + (ERFX_RHOQ1 + q_offset) * ne];
+                    rhop_lo += cons[q - sz + (ERFX_RHOQ1 + q_offset) * ne];
+                }
+                zs = grav_z * 0.5 * (rhop_hi + rhop_lo);
+            } else {
+                // ERF_make_buoyancy.cpp:272-323 (types 2, 4) and :347-372 (type 3)
+                double qplus = moist_buoyancy_dev(btype, nqs, cons, pq1, pq2, pq3, pavg, nz, ne, q, k);
+                double qminus = moist_buoyancy_dev(btype, nqs, cons, pq1, pq2, pq3, pavg, nz, ne, q - sz, k - 1);
+                double qavg = 0.5 * (qplus + qminus);
+                double r0avg = 0.5 * (pavg[k] + pavg[k - 1]);
+                zs = -qavg * r0avg * grav_z;
+            }
             if (cor) {
                 double rho_u_loc = 0.25 * (ru[q + 1] + ru[q] + ru[q + 1 - sz] + ru[q - sz]);
                 zs += cf * rho_u_loc * cosphi;
@@ -668,74 +734,114 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             zs += 0.5 * (cons[q] + cons[q - sz]) * g2;
             double met_h_zeta = 1;
             double gpz = dzi * (pp[q] - pp[q - sz]) / met_h_zeta;
-            r += (zs - gpz - 0.0) / (1.0 + 0.0);
+            double qf = moist ? face_q(pq1, pq2, q, q - sz) : 0.0;
+            r += (zs - gpz - 0.0) / (1.0 + qf);
             rwr[q] = r;
         }
     });
 }
 
-// erf_slow_rhs_post (Source/TimeIntegration/ERF_slow_rhs_post.cpp:49-559), dry:
-// RhoScalar advected with the time-averaged momenta, then S_new <- S_data.
-void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& rhs, const erfx_state& old,
-                   const erfx_state& snew, const erfx_state& data, const double* prim,
-                   const erfx_state& scratch, const double* eddy, hipStream_t s) {
-    (void)nrk;
+// One block of slow scalars of erf_slow_rhs_post (ERF_slow_rhs_post.cpp:364-497):
+// advection with the time-averaged momenta (AdvectionSrcForScalars), diffusion
+// (DiffusionSrcForState_N), then cur = old + dt * rhs, clipped at 0 for the
+// moisture species.  copy_slow: first copy every slow comp of S_new into cur.
+struct ScalarBlock {
+    int n0, ncomp;            // cons comps [n0, n0 + ncomp)
+    int ht, vt;               // advection types (a WENO horizontal type also in z)
+    double hf, vf;
+    int mu_h, mu_v;           // EddyDiff comps
+    bool clip;
+};
+static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, double slow_dt, const erfx_state& rhs,
+                       const erfx_state& old, const erfx_state& snew, const erfx_state& data, const double* prim,
+                       const erfx_state& scratch, const double* eddy, hipStream_t s) {
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
     const int ncons = P.cfg.ncons;
-    const int ht = P.cfg.dryscal_horiz_adv;
-    // a WENO horizontal type is used in z as well (ERF_AdvectionSrcForState.cpp:228-262)
-    const int vt = is_weno(ht) ? ht : P.cfg.dryscal_vert_adv;
-    const double hf = P.cfg.dryscal_horiz_upw_frac, vf = P.cfg.dryscal_vert_upw_frac;
+    const int ht = B.ht;
+    const int vt = is_weno(ht) ? ht : B.vt;
+    const double hf = B.hf, vf = B.vf;
     const bool c2c2 = (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2);
     const bool use_diff = P.use_diff, consA = P.consA, turb = P.use_turb;
     const double ae = consA ? P.cfg.alpha_C : P.cfg.rho0_trans * P.cfg.alpha_C;
-    const double* mu_h = eddy ? eddy + 4 * ne : nullptr;   // Scalar_h
-    const double* mu_v = eddy ? eddy + 10 * ne : nullptr;  // Scalar_v
-    const int n = ERFX_RHOSCALAR;
-    const double* ps = prim + (n - 1) * ne;
+    const double* mu_h = eddy ? eddy + B.mu_h * ne : nullptr;
+    const double* mu_v = eddy ? eddy + B.mu_v * ne : nullptr;
+    const int n0 = B.n0, nc_b = B.ncomp;
+    const bool clip = B.clip;
     const double* ax = scratch.xmom;
     const double* ay = scratch.ymom;
     const double* az = scratch.zmom;
     double* cur = data.cons;
     double* nc = snew.cons;
     const double* oc = old.cons;
-    double* cr = rhs.cons + n * ne;
+    double* crb = rhs.cons;
     launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
-        for (int c = 2; c < ncons; ++c) cur[q + c * ne] = nc[q + c * ne];
+        if (copy_slow)
+            for (int c = 2; c < ncons; ++c) cur[q + c * ne] = nc[q + c * ne];
         double fx0 = ax[q], fx1 = ax[q + 1], fy0 = ay[q], fy1 = ay[q + sy], fz0 = az[q], fz1 = az[q + sz];
-        double gx0, gx1, gy0, gy1, gz0, gz1;
-        if (c2c2) {
-            gx0 = fx0 * (0.5 * (ps[q] + ps[q - 1]));
-            gx1 = fx1 * (0.5 * (ps[q + 1] + ps[q]));
-            gy0 = fy0 * (0.5 * (ps[q] + ps[q - sy]));
-            gy1 = fy1 * (0.5 * (ps[q + sy] + ps[q]));
-            gz0 = fz0 * (0.5 * (ps[q] + ps[q - sz]));
-            gz1 = fz1 * (0.5 * (ps[q + sz] + ps[q]));
-        } else {
-            gx0 = fx0 * interp_scalar_at(ht, ps, q, 1, fx0, hf);
-            gx1 = fx1 * interp_scalar_at(ht, ps, q + 1, 1, fx1, hf);
-            gy0 = fy0 * interp_scalar_at(ht, ps, q, sy, fy0, hf);
-            gy1 = fy1 * interp_scalar_at(ht, ps, q + sy, sy, fy1, hf);
-            gz0 = fz0 * interp_scalar_at(vt, ps, q, sz, fz0, vf);
-            gz1 = fz1 * interp_scalar_at(vt, ps, q + sz, sz, fz1, vf);
+        for (int m = 0; m < nc_b; ++m) {
+            const int n = n0 + m;
+            const double* ps = prim + (n - 1) * ne;
+            double gx0, gx1, gy0, gy1, gz0, gz1;
+            if (c2c2) {
+                gx0 = fx0 * (0.5 * (ps[q] + ps[q - 1]));
+                gx1 = fx1 * (0.5 * (ps[q + 1] + ps[q]));
+                gy0 = fy0 * (0.5 * (ps[q] + ps[q - sy]));
+                gy1 = fy1 * (0.5 * (ps[q + sy] + ps[q]));
+                gz0 = fz0 * (0.5 * (ps[q] + ps[q - sz]));
+                gz1 = fz1 * (0.5 * (ps[q + sz] + ps[q]));
+            } else {
+                gx0 = fx0 * interp_scalar_at(ht, ps, q, 1, fx0, hf);
+                gx1 = fx1 * interp_scalar_at(ht, ps, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_scalar_at(ht, ps, q, sy, fy0, hf);
+                gy1 = fy1 * interp_scalar_at(ht, ps, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_scalar_at(vt, ps, q, sz, fz0, vf);
+                gz1 = fz1 * interp_scalar_at(vt, ps, q + sz, sz, fz1, vf);
+            }
+            double r = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
+            if (use_diff) {
+                double dfx0 = diff_flux(nc, ps, mu_h, q, 1, dxi, ae, consA, turb) * 1.0;
+                double dfx1 = diff_flux(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb) * 1.0;
+                double dfy0 = diff_flux(nc, ps, mu_h, q, sy, dyi, ae, consA, turb) * 1.0;
+                double dfy1 = diff_flux(nc, ps, mu_h, q + sy, sy, dyi, ae, consA, turb) * 1.0;
+                double dfz0 = diff_flux(nc, ps, mu_v, q, sz, dzi, ae, consA, turb);
+                double dfz1 = diff_flux(nc, ps, mu_v, q + sz, sz, dzi, ae, consA, turb);
+                r -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
+            }
+            crb[q + n * ne] = r;
+            double v = oc[q + n * ne] + slow_dt * r;
+            if (clip) v = (v < 0.0) ? 0.0 : v;   // amrex::max(v, 0.0)
+            cur[q + n * ne] = v;
         }
-        double r = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
-        if (use_diff) {
-            double dfx0 = diff_flux(nc, ps, mu_h, q, 1, dxi, ae, consA, turb) * 1.0;
-            double dfx1 = diff_flux(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb) * 1.0;
-            double dfy0 = diff_flux(nc, ps, mu_h, q, sy, dyi, ae, consA, turb) * 1.0;
-            double dfy1 = diff_flux(nc, ps, mu_h, q + sy, sy, dyi, ae, consA, turb) * 1.0;
-            double dfz0 = diff_flux(nc, ps, mu_v, q, sz, dzi, ae, consA, turb);
-            double dfz1 = diff_flux(nc, ps, mu_v, q + sz, sz, dzi, ae, consA, turb);
-            r -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
-        }
-        cr[q] = r;
-        cur[q + n * ne] = oc[q + n * ne] + slow_dt * r;
     });
+}
+
+// erf_slow_rhs_post (Source/TimeIntegration/ERF_slow_rhs_post.cpp:49-559): the
+// slow variables are RhoScalar (:168-172) and, with moisture, the n_qstate
+// species as one block with the moist advection types (:173-175, :364-434);
+// then S_new <- S_data.
+void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& rhs, const erfx_state& old,
+                   const erfx_state& snew, const erfx_state& data, const double* prim,
+                   const erfx_state& scratch, const double* eddy, hipStream_t s) {
+    (void)nrk;
+    const Lay L = P.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long ne = L.ne;
+    const int ncons = P.cfg.ncons;
+    const erfx_config& c = P.cfg;
+    ScalarBlock scal{ERFX_RHOSCALAR, 1, c.dryscal_horiz_adv, c.dryscal_vert_adv, c.dryscal_horiz_upw_frac,
+                     c.dryscal_vert_upw_frac, 4 /* Scalar_h */, 10 /* Scalar_v */, false};
+    post_block(P, scal, true, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
+    if (P.moist) {
+        ScalarBlock moist{ERFX_RHOQ1, P.n_qstate, c.moistscal_horiz_adv, c.moistscal_vert_adv,
+                          c.moistscal_horiz_upw_frac, c.moistscal_vert_upw_frac, 5 /* Q_h */, 11 /* Q_v */, true};
+        post_block(P, moist, false, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
+    }
+    double* cur = data.cons;
+    double* nc = snew.cons;
     // S_new <- S_data after every cell is updated (neighbours read S_new's rho above)
     launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);

@@ -45,6 +45,7 @@ SIGNATURES = {
                                ctypes.POINTER(State), _P, _P, _P, _P, _P]),
     "erfx_slow_rhs_post": (_I, [_P, _I, _D, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
                                 ctypes.POINTER(State), _P, ctypes.POINTER(State), _P, _P, _P]),
+    "erfx_kessler_advance": (_I, [_P, _P, _D, _P]),
     "erfx_sim_create": (_I, [ctypes.POINTER(C.Config), ctypes.POINTER(_P)]),
     "erfx_sim_destroy": (_I, [_P]),
     "erfx_sim_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
@@ -120,7 +121,10 @@ class Sim:
         n = lib().erfx_sim_field_size(self.h, fid)
         if n < 0:
             raise ValueError("unknown field %d" % fid)
-        a = np.empty((n // self.L.nelem,) + self.L.shape3())
+        if fid == C.D_RAIN_ACCUM:      # one k = 0 plane: (j, i) with ghosts
+            a = np.empty((self.L.sz // self.L.sy, self.L.sy))
+        else:
+            a = np.empty((n // self.L.nelem,) + self.L.shape3())
         check(lib().erfx_sim_download(self.h, fid, a.ctypes.data))
         return a
 

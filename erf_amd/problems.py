@@ -154,6 +154,60 @@ def rising_thermal(nx=64, ny=64, nz=64, seed=0, adv=C.ADV_CENTERED2, scal_adv=No
     return cfg, f
 
 
+def _esat_bolton(T):
+    """Saturation vapour pressure over water [mbar], Bolton (1980), for
+    initial humidity only."""
+    tc = T - 273.15
+    return 6.112 * np.exp(17.67 * tc / (tc + 243.5))
+
+
+def moist_bubble(nx=24, ny=20, nz=16, seed=0, buoyancy_type=None, norain=False, adv=C.ADV_UPWIND3):
+    """Moist rising thermal with Kessler microphysics on the rising-thermal
+    domain (20x20x10 km, periodic x/y, slip walls, isentropic 300 K base state):
+    relative humidity 90% below 2 km decaying with a 1 km e-folding scale above;
+    inside the +2 K cos^2 bubble the air is 5% supersaturated and carries up
+    to 2 g/kg of cloud water and 0.5 g/kg of rain, so condensation,
+    autoconversion, accretion, rain evaporation and sedimentation all act in
+    the first step.  rho*theta keeps the base-state pressure with the moist
+    equation of state (getRhoThetagivenP with qv, ERF_EOS.H:144-149).
+    Dycore and dry scalars 3rd-order upwind, moist scalars WENO3 (ERF's
+    defaults), dt = 1 s, MRI ratio 6."""
+    cfg = C.default_config(nx, ny, nz, 20000.0 / nx, 20000.0 / ny, 10000.0 / nz)
+    cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    cfg.mri_ratio = 6
+    C.set_moisture(cfg, C.MOIST_KESSLER_NORAIN if norain else C.MOIST_KESSLER, buoyancy_type)
+    L, f = empty_fields(cfg)
+    _base_state(cfg, L, f, np.full(nz, 300.0))
+    ix, jy, kz = _cell_centres(cfg, L)
+    Lr = np.sqrt(((ix[None, None, :] - 10000.0) / 2000.0) ** 2 +
+                 ((jy[None, :, None] - 10000.0) / 2000.0) ** 2 +
+                 ((kz[:, None, None] - 2000.0) / 2000.0) ** 2)
+    shape = np.where(Lr <= 1.0, (np.cos(math.pi * Lr) + 1.0) / 2.0, 0.0)
+    theta = 300.0 + 2.0 * shape
+    rh_env = np.where(kz <= 2000.0, 0.9, 0.9 * np.exp(-(kz - 2000.0) / 1000.0))[:, None, None]
+    rh = rh_env + (1.05 - rh_env) * shape
+    p = f[C.F_P0][0]
+    rdOcp = C.R_d / cfg.c_p
+    T = theta * np.power(np.maximum(p, 1.0) / C.p_0, rdOcp)
+    es = _esat_bolton(T)
+    pmb = np.maximum(p, 1.0) / 100.0
+    qv = rh * (C.R_d / C.R_v) * es / (pmb - es)
+    qc = 2.0e-3 * shape
+    qp = 5.0e-4 * shape if not norain else 0.0 * shape
+    ng = L.ng
+    sl = (slice(ng, ng + nz),)
+    rt = _rt_of_p(p) / (1.0 + C.R_v / C.R_d * qv)
+    cons = f[C.F_CONS]
+    cons[C.RHOTHETA][sl] = rt[sl]
+    cons[C.RHO][sl] = rt[sl] / theta[sl]
+    for comp, q in ((C.RHOQ1, qv), (C.RHOQ2, qc), (C.RHOQ3, qp)):
+        cons[comp][sl] = (cons[C.RHO] * q)[sl]
+    rng = np.random.default_rng(seed)
+    _noise_vel(cfg, L, f, rng, 0.01, (0.0, 0.0, 0.0))
+    return cfg, f
+
+
 def _noise_vel(cfg, L, f, rng, amp, mean):
     ng, nx, ny, nz = L.ng, cfg.nx, cfg.ny, cfg.nz
     for fid, (ex, ey, ez), m in ((C.F_XVEL, (1, 0, 0), mean[0]), (C.F_YVEL, (0, 1, 0), mean[1]),

@@ -82,8 +82,13 @@ enum { ERFX_LES_NONE = 0, ERFX_LES_SMAGORINSKY = 1 };
 /* ---- conserved-variable components (ERF_IndexDefines.H:13-50) ---------- */
 enum {
     ERFX_RHO = 0, ERFX_RHOTHETA = 1, ERFX_RHOKE = 2, ERFX_RHOQKE = 3,
-    ERFX_RHOSCALAR = 4, ERFX_NCONS_DRY = 5
+    ERFX_RHOSCALAR = 4, ERFX_NCONS_DRY = 5,
+    /* moisture species of the Kessler models: vapour, cloud water, rain */
+    ERFX_RHOQ1 = 5, ERFX_RHOQ2 = 6, ERFX_RHOQ3 = 7, ERFX_NCONS_MOIST = 8
 };
+
+/* ---- moisture models (MoistureType, ERF_DataStruct.H) ------------------ */
+enum { ERFX_MOIST_NONE = 0, ERFX_MOIST_KESSLER = 1, ERFX_MOIST_KESSLER_NORAIN = 2 };
 
 /* ---- run configuration: the subset of SolverChoice the step reads ----- */
 typedef struct erfx_config {
@@ -91,7 +96,7 @@ typedef struct erfx_config {
     double dx, dy, dz;            /* uniform spacing [m]                             */
     double prob_lo[3];            /* domain origin [m]                               */
     int    bc[6];                 /* xlo, ylo, zlo, xhi, yhi, zhi (AMReX Orientation) */
-    int    ncons;                 /* 5 (dry)                                         */
+    int    ncons;                 /* 5 (dry) or 8 (Kessler: + RhoQ1..RhoQ3)          */
 
     /* advection (ERF_AdvStruct.H) */
     int    dycore_horiz_adv, dycore_vert_adv;
@@ -132,6 +137,17 @@ typedef struct erfx_config {
     int    rayleigh_damp_U, rayleigh_damp_V, rayleigh_damp_W, rayleigh_damp_T;
     double rayleigh_dampcoef;     /* 1/s, default 0.2                                */
     double rayleigh_zdamp;        /* m, default 500                                  */
+
+    /* moisture (ERF_DataStruct.H:100-140; Kessler, ERF_Kessler.H).  With
+     * moisture the three species ride in cons comps 5..7; the dycore divides
+     * the pressure gradients by 1 + qv + qc and evaluates p' with qv. */
+    int    moisture_type;         /* ERFX_MOIST_*                                    */
+    /* make_buoyancy expression (ERF_make_buoyancy.cpp:32-380): 1 = density
+     * perturbation (the only dry form here); 2, 3, 4 = the moist forms with
+     * plane averages.  ERF picks 2 for Kessler with rain, 1 otherwise. */
+    int    buoyancy_type;
+    int    moistscal_horiz_adv, moistscal_vert_adv;          /* WENO3 in ERF */
+    double moistscal_horiz_upw_frac, moistscal_vert_upw_frac;
 } erfx_config;
 
 /* ---- layout shared by every field of a context ------------------------- */
@@ -225,6 +241,17 @@ int erfx_slow_rhs_post(erfx_ctx* ctx, int nrk, double slow_dt,
                        erfx_state* data, const double* prim, const erfx_state* scratch,
                        const double* xvel, const double* yvel, const double* eddy);
 
+/* Kessler warm rain, once per step after the dycore (ERF::advance_microphysics,
+ * ERF_advance_microphysics.cpp:5-16): Copy_State_to_Micro
+ * (ERF_Init_Kessler.cpp:64-99), Kessler::AdvanceKessler with the step's dt
+ * (ERF_Kessler.cpp:11-250; sedimentation only for ERFX_MOIST_KESSLER),
+ * Copy_Micro_to_State and its FillBoundary (ERF_Update_Kessler.cpp:14-40).
+ * `rain_accum` (may be NULL) is the k = 0 plane of the layout,
+ * elem(i,j) = p[(i+ng) + (j+ng)*sy], and accumulates surface rain in mm.
+ * Synchronises the stream at return: ERF aborts when a temperature leaves the
+ * saturation fit's range (below -85 C), reported here as ERFX_ERR_STATE. */
+int erfx_kessler_advance(erfx_ctx* ctx, double* cons, double dt, double* rain_accum);
+
 /* ---- whole-step driver ------------------------------------------------- *
  * ERF::Advance + advance_dycore + MRISplitIntegrator::advance on one box
  * (ERF_Advance.cpp:20-233, ERF_advance_dycore.cpp:37-305, ERF_MRI.H:187-369).
@@ -246,6 +273,7 @@ enum {
     ERFX_D_PRIM = 16, ERFX_D_PI_STAGE = 17, ERFX_D_FAST_COEFFS = 18, ERFX_D_EDDY = 19,
     ERFX_D_XVEL_OLD = 20, ERFX_D_YVEL_OLD = 21, ERFX_D_ZVEL_OLD = 22,
     ERFX_D_XMOM_SRC = 23, ERFX_D_YMOM_SRC = 24, ERFX_D_ZMOM_SRC = 25,
+    ERFX_D_RAIN_ACCUM = 26,       /* Kessler surface rain [mm], one k = 0 plane (sz doubles) */
     ERFX_D_STATE_BASE = 32
 };
 

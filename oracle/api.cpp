@@ -136,11 +136,13 @@ void orc_destroy(void* h) { delete static_cast<Level*>(h); }
 
 long orc_field_size(void* h, int id) {
     Level& L = *static_cast<Level*>(h);
+    if (id == ERFX_D_RAIN_ACCUM) return (long)L.rain_accum.size();
     Field* f = field_by_id(L, id);
     return f ? (long)f->v.size() : -1;
 }
 int orc_set(void* h, int id, const double* src) {
     Level& L = *static_cast<Level*>(h);
+    if (id == ERFX_D_RAIN_ACCUM) { std::memcpy(L.rain_accum.data(), src, L.rain_accum.size() * sizeof(double)); return 0; }
     Field* f = field_by_id(L, id);
     if (!f) { g_err = "bad field id"; return -1; }
     std::memcpy(f->v.data(), src, f->v.size() * sizeof(double));
@@ -148,6 +150,7 @@ int orc_set(void* h, int id, const double* src) {
 }
 int orc_get(void* h, int id, double* dst) {
     Level& L = *static_cast<Level*>(h);
+    if (id == ERFX_D_RAIN_ACCUM) { std::memcpy(dst, L.rain_accum.data(), L.rain_accum.size() * sizeof(double)); return 0; }
     Field* f = field_by_id(L, id);
     if (!f) { g_err = "bad field id"; return -1; }
     std::memcpy(dst, f->v.data(), f->v.size() * sizeof(double));
@@ -197,7 +200,7 @@ int orc_slow_rhs_pre(void* h, int nrk, double slow_dt) {
     Level& L = *static_cast<Level*>(h);
     return guard([&] {
         L.make_sources(L.A(L.S_new), L.A(L.prim));
-        L.make_mom_sources(L.A(L.S_new));
+        L.make_mom_sources(L.A(L.S_new), L.A(L.prim));
         L.slow_rhs_pre(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_new), L.A(L.prim), L.A(L.S_scratch),
                        L.A(L.xvel), L.A(L.yvel), L.A(L.zvel));
     });
@@ -209,6 +212,29 @@ int orc_slow_rhs_post(void* h, int nrk, double slow_dt) {
         L.slow_rhs_post(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_old), L.A(L.S_new), L.A(L.S_sum),
                         L.A(L.prim), L.A(L.S_scratch), L.A(L.xvel), L.A(L.yvel));
     });
+}
+
+// Kessler on the level's vars_new cons with step dt (advance_microphysics)
+int orc_kessler(void* h, double dt) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.kessler_advance(L.A(L.cons), dt); });
+}
+
+// saturation functions (K, mbar) for the closed-form tests
+int orc_saturation(const double* t, const double* p, long n, double* es, double* des, double* qs, double* dqs) {
+    return guard([&] {
+        for (long i = 0; i < n; ++i) {
+            es[i] = esatw(t[i]);
+            des[i] = dtesatw(t[i]);
+            qs[i] = qsatw(t[i], p[i]);
+            dqs[i] = dtqsatw(t[i], p[i]);
+        }
+    });
+}
+
+// moist EOS: temperature from rho, rho*theta and qv (ERF_EOS.H:42-51)
+void orc_temperature(const double* rho, const double* rt, const double* qv, long n, double* T) {
+    for (long i = 0; i < n; ++i) T[i] = getTgivenRandRTh(rho[i], rt[i], qv[i]);
 }
 
 }  // extern "C"

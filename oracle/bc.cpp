@@ -10,6 +10,9 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     cfg = c;
     L = lay;
     ncons = c.ncons;
+    moist = c.moisture_type != ERFX_MOIST_NONE;
+    n_qstate = moist ? 3 : 0;
+    if (ncons != (moist ? ERFX_NCONS_MOIST : ERFX_NCONS_DRY)) throw std::runtime_error("ncons does not match moisture_type");
     ng_cons = lay.ng;
     ng_vel = lay.ng - 1;
     rdOcp = R_d / c.c_p;
@@ -51,6 +54,7 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     tau11.alloc(L, 1); tau22.alloc(L, 1); tau33.alloc(L, 1);
     tau12.alloc(L, 1); tau13.alloc(L, 1); tau23.alloc(L, 1);
     xmom_src.alloc(L, 1); ymom_src.alloc(L, 1); zmom_src.alloc(L, 1); cc_src.alloc(L, ncons);
+    rain_accum.assign((size_t)L.sz, 0.0);
 }
 
 // FabArray::FillBoundary on a single periodic box: every ghost point whose

@@ -336,12 +336,15 @@ void Level::diffusion_src_state(const Box& bx, int start_comp, int num_comp, Arr
     const bool consA = cfg.molec_diff_type == ERFX_MOLEC_CONSTANT_ALPHA;
     const bool turb = cfg.les_type == ERFX_LES_SMAGORINSKY;
     Arr mt = A(eddy);
+    // prim indices: theta, KE, QKE, scalar, then the moisture species (:103-186)
     double alpha_eff[8] = {0};
-    if (consA) { alpha_eff[0] = cfg.alpha_T; alpha_eff[3] = cfg.alpha_C; }
-    else { alpha_eff[0] = cfg.rho0_trans * cfg.alpha_T; alpha_eff[3] = cfg.rho0_trans * cfg.alpha_C; }
+    const double aT = consA ? cfg.alpha_T : cfg.rho0_trans * cfg.alpha_T;
+    const double aC = consA ? cfg.alpha_C : cfg.rho0_trans * cfg.alpha_C;
+    alpha_eff[0] = aT;
+    for (int p = 3; p < 8; ++p) alpha_eff[p] = aC;
     using namespace EddyDiff;
-    const int idx_h[4] = {Theta_h, KE_h, QKE_h, Scalar_h};
-    const int idx_v[4] = {Theta_v, KE_v, QKE_v, Scalar_v};
+    const int idx_h[8] = {Theta_h, KE_h, QKE_h, Scalar_h, Q_h, Q_h, Q_h, Q_h};
+    const int idx_v[8] = {Theta_v, KE_v, QKE_v, Scalar_v, Q_v, Q_v, Q_v, Q_v};
     const double dx_inv = dxInv[0], dy_inv = dxInv[1], dz_inv = dxInv[2];
     Field fx, fy, fz;
     fx.alloc(L, 1); fy.alloc(L, 1); fz.alloc(L, 1);

@@ -15,6 +15,7 @@ void Level::make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double 
     Arr r0a = A(r0), pi0a = A(pi0);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     Arr sc = stage.cons;
+    const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
     loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
         double rhobar_lo = r0a(i, j, k - 1), rhobar_hi = r0a(i, j, k);
         double pibar_lo = pi0a(i, j, k - 1), pibar_hi = pi0a(i, j, k);
@@ -27,6 +28,11 @@ void Level::make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double 
         coeff_Q *= pi_c;
         fc(i, j, k, 3) = coeff_P;
         fc(i, j, k, 4) = coeff_Q;
+        if (moist) {   // :183-188
+            double q = 0.5 * (prim_(i, j, k, q1) + prim_(i, j, k - 1, q1) + prim_(i, j, k, q2) + prim_(i, j, k - 1, q2));
+            coeff_P /= (1.0 + q);
+            coeff_Q /= (1.0 + q);
+        }
         double theta_t_lo  = 0.5 * (prim_(i, j, k - 2, 0) + prim_(i, j, k - 1, 0));
         double theta_t_mid = 0.5 * (prim_(i, j, k - 1, 0) + prim_(i, j, k, 0));
         double theta_t_hi  = 0.5 * (prim_(i, j, k, 0) + prim_(i, j, k + 1, 0));
@@ -79,6 +85,7 @@ void Level::fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA s
     const double dxi = dxInv[0], dyi = dxInv[1], dzi = dxInv[2];
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const bool implicit = cfg.implicit_substepping != 0;
+    const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
 
     // temporaries (valid-size plus what the stencils read)
     Field f_drt, f_drw, f_extrap, f_trhs, f_tx, f_ty, f_rhs, f_soln;
@@ -115,6 +122,10 @@ void Level::fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA s
     } else {
         loop(Box(0, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) {
             double gpx = (extrap(i, j, k) - extrap(i - 1, j, k)) * dxi;
+            if (moist) {   // :215-219
+                double q = 0.5 * (prim_(i, j, k, q1) + prim_(i - 1, j, k, q1) + prim_(i, j, k, q2) + prim_(i - 1, j, k, q2));
+                gpx /= (1.0 + q);
+            }
             double pi_c = 0.5 * (pi_st(i - 1, j, k) + pi_st(i, j, k));
             double fast_rhs_rho_u = -Gamma * R_d * pi_c * gpx;
             double new_drho_u = prev.xmom(i, j, k) - stage.xmom(i, j, k)
@@ -124,6 +135,10 @@ void Level::fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA s
         });
         loop(Box(0, 0, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) {
             double gpy = (extrap(i, j, k) - extrap(i, j - 1, k)) * dyi;
+            if (moist) {   // :238-242
+                double q = 0.5 * (prim_(i, j, k, q1) + prim_(i, j - 1, k, q1) + prim_(i, j, k, q2) + prim_(i, j - 1, k, q2));
+                gpy /= (1.0 + q);
+            }
             double pi_c = 0.5 * (pi_st(i, j - 1, k) + pi_st(i, j, k));
             double fast_rhs_rho_v = -Gamma * R_d * pi_c * gpy;
             double new_drho_v = prev.ymom(i, j, k) - stage.ymom(i, j, k)
@@ -153,6 +168,11 @@ void Level::fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA s
     loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
         double coeff_P = fc(i, j, k, 3);
         double coeff_Q = fc(i, j, k, 4);
+        if (moist) {   // :379-384
+            double q = 0.5 * (pr(i, j, k, q1) + pr(i, j, k - 1, q1) + pr(i, j, k, q2) + pr(i, j, k - 1, q2));
+            coeff_P /= (1.0 + q);
+            coeff_Q /= (1.0 + q);
+        }
         double theta_t_lo  = 0.5 * (pr(i, j, k - 2, 0) + pr(i, j, k - 1, 0));
         double theta_t_mid = 0.5 * (pr(i, j, k - 1, 0) + pr(i, j, k, 0));
         double theta_t_hi  = 0.5 * (pr(i, j, k, 0) + pr(i, j, k + 1, 0));

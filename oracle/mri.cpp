@@ -51,7 +51,7 @@ void Level::advance_dycore(double old_time, double dt) {
         // slow_rhs_fun_pre: make_sources, make_mom_sources, erf_slow_rhs_pre
         double slow_dt = time_stage - old_time;
         make_sources(Sn, A(prim));
-        make_mom_sources(Sn);
+        make_mom_sources(Sn, A(prim));
         slow_rhs_pre(nrk, slow_dt, Fs, Sn, A(prim), Sc, xv, yv, zv);
         double inv_fac = 1.0 / static_cast<double>(nsubsteps);
         for (int ks = 0; ks < nsubsteps; ++ks) {
@@ -70,7 +70,7 @@ void Level::advance_dycore(double old_time, double dt) {
     (void)use_diff;
 }
 
-// ERF::Advance, level 0, dycore only (Source/TimeIntegration/ERF_Advance.cpp:20-233):
+// ERF::Advance, level 0, dycore and microphysics (Source/TimeIntegration/ERF_Advance.cpp:20-233):
 // the old state is the previous new state; FillPatch + V2M of the old
 // velocities, then the dycore step writes the new cons and velocities.
 void Level::advance(double time, double dt) {
@@ -84,6 +84,7 @@ void Level::advance(double time, double dt) {
     copy_field(S_old.cons, cons_old_vars);   // cons_mf copy incl. ghosts (:127-128)
     advance_dycore(time, dt);
     copy_field(cons, S_new.cons);             // state_new cons aliases vars_new
+    kessler_advance(A(cons), dt);             // advance_microphysics (:172-174)
 }
 
 }  // namespace orc

@@ -1,6 +1,7 @@
 // oracle.hpp -- TEST INFRASTRUCTURE ONLY.
 //
-// CPU restatement of ERF's compressible dycore step (flat grid, dry), used as
+// CPU restatement of ERF's compressible dycore step (flat grid, dry or with a
+// Kessler moisture model), used as
 // the correctness oracle and as the CPU baseline.  Only tests/, the package's
 // smoke check and bench.py's cpu_baseline leg may load it; the product path
 // never calls it.
@@ -39,6 +40,11 @@ constexpr double CONST_GRAV = 9.81;
 constexpr double ip_0     = 1. / p_0;
 constexpr double iR_d     = 1. / R_d;
 constexpr double iGamma   = 1. / Gamma;
+constexpr double L_v      = 2.5e6;      // ERF_Constants.H:16
+constexpr double lcond    = 2.501e6;    // ERF_Constants.H:66
+constexpr double qcw0     = 1.e-3;      // ERF_Constants.H:46
+constexpr double alphaelq = 1.e-3;      // ERF_Constants.H:48
+constexpr double Rd_on_Rv = R_d / R_v;  // ERF_Constants.H:87
 
 // ---- EOS (Source/Utils/ERF_EOS.H) -------------------------------------------
 inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_EOS.H:89-93
@@ -46,6 +52,10 @@ inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_
 }
 inline double getExnergivenRTh(double rhotheta, double rdOcp, double qv = 0.0) { // ERF_EOS.H:130-135
     return erfx_pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma * rdOcp);
+}
+inline double getTgivenRandRTh(double rho, double rhotheta, double qv = 0.0) {  // ERF_EOS.H:42-51
+    double p_loc = p_0 * erfx_pow(R_d * rhotheta * (1.0 + R_v / R_d * qv) * ip_0, Gamma);
+    return p_loc / (R_d * rho * (1.0 + R_v / R_d * qv));
 }
 inline double getExnergivenP(double P, double rdOcp) {                    // ERF_EOS.H:117-121
     return erfx_pow(P * ip_0, rdOcp);
@@ -60,6 +70,12 @@ inline double getdPdRgivenConstantTheta(double rho, double theta, double qv = 0.
     return Gamma * p_0 * erfx_pow((R_d * theta * (1.0 + R_v / R_d * qv) * ip_0), Gamma) *
            erfx_pow(rho, Gamma - 1.0);
 }
+
+// ---- saturation over liquid water (kessler.cpp) ------------------------------
+double esatw(double t);     // mbar
+double dtesatw(double t);   // mbar / K
+double qsatw(double t, double p);
+double dtqsatw(double t, double p);
 
 // ---- column tridiagonal factor / solve (fast.cpp) ----------------------------
 void column_factor(int n, const double* A, double* B, const double* C, long st);
@@ -136,6 +152,8 @@ struct Level {
     erfx_layout L;
     int ng_cons = 0, ng_vel = 0;
     int ncons = 5;
+    bool moist = false;        // a Kessler model is on: cons comps 5..7 are RhoQ1..RhoQ3
+    int n_qstate = 0;          // Kessler::Qstate_Size (ERF_Kessler.H:172)
     double rdOcp = 0, gravity = 0;
     double dxInv[3];
     // BC kinds: [var][ori], var 0..2 = x/y/z velocity, 3 = cons; ori 0..5 = xlo,ylo,zlo,xhi,yhi,zhi
@@ -184,7 +202,13 @@ struct Level {
                           Arr t11, Arr t22, Arr t33, Arr t12, Arr t13, Arr t23);
     void turbulent_viscosity(Arr cons_in, Arr u, Arr v, Arr w);
     void make_tau_terms(StateA data, Arr u, Arr v, Arr w);
-    void make_mom_sources(StateA data);
+    void make_mom_sources(StateA data, Arr prim_);
+    // PlaneAverage of ERF_make_buoyancy.cpp:216-255: per level k of the valid
+    // box, the sums of value / (nx ny); rows rho, theta, qv, qc, qp
+    void plane_averages(StateA data, Arr prim_, std::vector<double>& avg);
+    // Kessler warm rain once per step (ERF_advance_microphysics.cpp:5-16)
+    void kessler_advance(Arr c, double dt);
+    std::vector<double> rain_accum;   // k = 0 plane of the layout (sz values)
     void make_sources(StateA data, Arr prim_);
     void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
     void init_scalar_advdiff(const double* params);

@@ -6,10 +6,11 @@
 
 namespace orc {
 
-// make_mom_sources (Source/SourceTerms/ERF_make_mom_sources.cpp:40-450) with
-// make_buoyancy type 1, dry (Source/SourceTerms/ERF_make_buoyancy.cpp:95-107):
-// buoyancy, Coriolis and geostrophic forcing.
-void Level::make_mom_sources(StateA data) {
+// make_mom_sources (Source/SourceTerms/ERF_make_mom_sources.cpp:40-450):
+// buoyancy (make_buoyancy, Source/SourceTerms/ERF_make_buoyancy.cpp:32-380:
+// type 1 dry :95-107 and moist :178-209, moist types 2-4 :211-376 on plane
+// averages), Coriolis, Rayleigh damping and geostrophic forcing.
+void Level::make_mom_sources(StateA data, Arr prim_) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     Arr xs = A(xmom_src), ys = A(ymom_src), zs = A(zmom_src);
     std::fill(xmom_src.v.begin(), xmom_src.v.end(), 0.0);
@@ -17,9 +18,56 @@ void Level::make_mom_sources(StateA data) {
     std::fill(zmom_src.v.begin(), zmom_src.v.end(), 0.0);
     Arr c = data.cons, r0a = A(r0);
     const double grav_z = -gravity;
-    loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
-        zs(i, j, k) = grav_z * 0.5 * ((c(i, j, k) - r0a(i, j, k)) + (c(i, j, k - 1) - r0a(i, j, k - 1)));
-    });
+    const Box faces_in(0, 0, 1, nx - 1, ny - 1, nz - 1);
+    if (!moist) {
+        loop(faces_in, [&](int i, int j, int k) {
+            zs(i, j, k) = grav_z * 0.5 * ((c(i, j, k) - r0a(i, j, k)) + (c(i, j, k - 1) - r0a(i, j, k - 1)));
+        });
+    } else if (cfg.buoyancy_type == 1) {
+        loop(faces_in, [&](int i, int j, int k) {
+            double rhop_hi = c(i, j, k, ERFX_RHO) + c(i, j, k, ERFX_RHOQ1) + c(i, j, k, ERFX_RHOQ2) - r0a(i, j, k);
+            double rhop_lo = c(i, j, k - 1, ERFX_RHO) + c(i, j, k - 1, ERFX_RHOQ1) + c(i, j, k - 1, ERFX_RHOQ2) -
+                             r0a(i, j, k - 1);
+            for (int q_offset = 2; q_offset < n_qstate; ++q_offset) {
+                rhop_hi += c(i, j, k, ERFX_RHOQ1 + q_offset);
+                rhop_lo += c(i, j, k - 1, ERFX_RHOQ1 + q_offset);
+            }
+            zs(i, j, k) = grav_z * 0.5 * (rhop_hi + rhop_lo);
+        });
+    } else {
+        std::vector<double> avg;
+        plane_averages(data, prim_, avg);
+        const double* rho_d = &avg[0];
+        const double* theta_d = &avg[(size_t)nz];
+        const double* qv_d = &avg[2 * (size_t)nz];
+        const double* qc_d = &avg[3 * (size_t)nz];
+        const double* qp_d = &avg[4 * (size_t)nz];
+        const int btype = cfg.buoyancy_type;
+        const int nq = n_qstate;
+        // the departure of one cell from the level's plane averages
+        auto qdev = [&](int i, int j, int k) {
+            double temp1d = getTgivenRandRTh(rho_d[k], rho_d[k] * theta_d[k], qv_d[k]);
+            double temp3d = getTgivenRandRTh(c(i, j, k, ERFX_RHO), c(i, j, k, ERFX_RHOTHETA),
+                                             c(i, j, k, ERFX_RHOQ1) / c(i, j, k, ERFX_RHO));
+            double qv = (nq >= 1) ? prim_(i, j, k, ERFX_RHOQ1 - 1) : 0.0;
+            double qc = (nq >= 2) ? prim_(i, j, k, ERFX_RHOQ2 - 1) : 0.0;
+            double qp = (nq >= 3) ? prim_(i, j, k, ERFX_RHOQ3 - 1) : 0.0;
+            if (btype == 2)
+                return 0.61 * (qv - qv_d[k]) - (qc - qc_d[k] + qp - qp_d[k]) +
+                       (temp3d - temp1d) / temp1d * (1.0 + 0.61 * qv_d[k] - qc_d[k] - qp_d[k]);
+            if (btype == 4)
+                return 0.61 * (qv - qv_d[k]) - (qc - qc_d[k] + qp - qp_d[k]) +
+                       (c(i, j, k, ERFX_RHOTHETA) / c(i, j, k, ERFX_RHO) - theta_d[k]) / theta_d[k];
+            return 0.61 * qv - (qc + qp) + (temp3d - temp1d) / temp1d;     // type 3
+        };
+        loop(faces_in, [&](int i, int j, int k) {
+            double qplus = qdev(i, j, k);
+            double qminus = qdev(i, j, k - 1);
+            double qavg = 0.5 * (qplus + qminus);
+            double r0avg = 0.5 * (rho_d[k] + rho_d[k - 1]);
+            zs(i, j, k) = -qavg * r0avg * grav_z;
+        });
+    }
     Box tbx(0, 0, 0, nx, ny - 1, nz - 1), tby(0, 0, 0, nx - 1, ny, nz - 1), tbz(0, 0, 0, nx - 1, ny - 1, nz - 1);
     Arr ru = data.xmom, rv = data.ymom, rw = data.zmom;
     if (cfg.use_coriolis) {
@@ -146,7 +194,7 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
 #pragma omp critical
             { if (!bad_theta) { bad_theta = 1; std::fprintf(stderr, "BAD THETA AT %d %d %d %e\n", i, j, k, c(i, j, k, 1)); } }
         }
-        double qv_for_p = 0.0;
+        double qv_for_p = moist ? c(i, j, k, ERFX_RHOQ1) / c(i, j, k, ERFX_RHO) : 0.0;
         pp(i, j, k) = getPgivenRTh(c(i, j, k, 1), qv_for_p) - p0a(i, j, k);
     });
     if (bad_theta) throw std::runtime_error("Bad theta in slow_rhs_pre");
@@ -213,11 +261,15 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     }
     Arr xs = A(xmom_src), ys = A(ymom_src), zs = A(zmom_src);
     const double apg0 = 0.0, apg1 = 0.0, apg2 = 0.0;
+    // q = qv + qc averaged to the face (:622-626, :671-675, :732-736)
+    const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
+    Arr cp = prim_;
     loop(tbx, [&](int i, int j, int k) {
         double gp_xi = dxi * (pp(i, j, k) - pp(i - 1, j, k));
         double gpx = gp_xi;
         gpx *= 1.0;
         double q = 0.0;
+        if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i - 1, j, k, q1) + cp(i, j, k, q2) + cp(i - 1, j, k, q2));
         rur(i, j, k) += (-gpx - apg0) / (1.0 + q) + xs(i, j, k);
     });
     loop(tby, [&](int i, int j, int k) {
@@ -225,6 +277,7 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
         double gpy = gp_eta;
         gpy *= 1.0;
         double q = 0.0;
+        if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i, j - 1, k, q1) + cp(i, j, k, q2) + cp(i, j - 1, k, q2));
         rvr(i, j, k) += (-gpy - apg1) / (1.0 + q) + ys(i, j, k);
     });
     // zero the rhs on Dirichlet lateral faces (:697-726)
@@ -236,13 +289,16 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
         double met_h_zeta = 1;
         double gpz = dzi * (pp(i, j, k) - pp(i, j, k - 1)) / met_h_zeta;
         double q = 0.0;
+        if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i, j, k - 1, q1) + cp(i, j, k, q2) + cp(i, j, k - 1, q2));
         rwr(i, j, k) += (zs(i, j, k) - gpz - apg2) / (1.0 + q);
     });
     (void)dt;
 }
 
-// erf_slow_rhs_post (Source/TimeIntegration/ERF_slow_rhs_post.cpp:49-559),
-// dry: RhoScalar is the only slow variable advanced (:161-166).
+// erf_slow_rhs_post (Source/TimeIntegration/ERF_slow_rhs_post.cpp:49-559):
+// the slow variables are RhoScalar and, with moisture, the n_qstate species
+// as one block with the moist advection types (:168-175, :364-434); the
+// species are clipped at 0 after the update (:485-497).
 void Level::slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA snew, StateA data,
                           Arr prim_, StateA scratch, Arr u, Arr v) {
     (void)nrk; (void)u; (void)v;
@@ -259,11 +315,26 @@ void Level::slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA sne
                           cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv, cfg.dryscal_horiz_upw_frac,
                           cfg.dryscal_vert_upw_frac);
     if (use_diff) diffusion_src_state(tbx, ivar, 1, ncons_, prim_, cr);
+    if (moist) {
+        advection_src_scalars(tbx, ERFX_RHOQ1, n_qstate, scratch.xmom, scratch.ymom, scratch.zmom, prim_, cr,
+                              cfg.moistscal_horiz_adv, cfg.moistscal_vert_adv, cfg.moistscal_horiz_upw_frac,
+                              cfg.moistscal_vert_upw_frac);
+        if (use_diff) diffusion_src_state(tbx, ERFX_RHOQ1, n_qstate, ncons_, prim_, cr);
+    }
     Arr cs = A(cc_src);
     loop(tbx, [&](int i, int j, int k) {
         cr(i, j, k, ivar) += cs(i, j, k, ivar);
         cur(i, j, k, ivar) = oc(i, j, k, ivar) + dt * cr(i, j, k, ivar);
     });
+    if (moist) {
+        loop(tbx, [&](int i, int j, int k) {
+            for (int n = ERFX_RHOQ1; n < ncons; ++n) {
+                cr(i, j, k, n) += cs(i, j, k, n);
+                cur(i, j, k, n) = oc(i, j, k, n) + dt * cr(i, j, k, n);
+                cur(i, j, k, n) = std::max(cur(i, j, k, n), 0.0);
+            }
+        });
+    }
     loop(tbx, [&](int i, int j, int k) {
         for (int n = 0; n < ncons; ++n) ncons_(i, j, k, n) = cur(i, j, k, n);
     });

@@ -10,6 +10,24 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 _lib = None
 
 
+def saturation(t, p):
+    """(esatw, dtesatw, qsatw, dtqsatw) of the oracle at temperatures t [K], pressures p [mbar]."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    p = np.ascontiguousarray(np.broadcast_to(p, t.shape), dtype=np.float64)
+    outs = [np.empty_like(t) for _ in range(4)]
+    rc = lib().orc_saturation(t.ctypes.data, p.ctypes.data, t.size, *(o.ctypes.data for o in outs))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return outs
+
+
+def temperature(rho, rt, qv):
+    rho, rt, qv = (np.ascontiguousarray(a, dtype=np.float64) for a in np.broadcast_arrays(rho, rt, qv))
+    T = np.empty_like(rho)
+    lib().orc_temperature(rho.ctypes.data, rt.ctypes.data, qv.ctypes.data, rho.size, T.ctypes.data)
+    return T
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -34,6 +52,9 @@ def lib():
         l.orc_slow_rhs_pre.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         l.orc_slow_rhs_post.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         l.orc_set_rayleigh.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        l.orc_kessler.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        l.orc_saturation.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 4
+        l.orc_temperature.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_long, ctypes.c_void_p]
         _lib = l
     return _lib
 
@@ -62,8 +83,11 @@ class Oracle:
         self._chk(lib().orc_set(self.h, fid, a.ctypes.data))
 
     def get(self, fid):
-        nc = C.ncomp_of(self.cfg, fid)
-        a = np.empty((nc,) + self.L.shape3())
+        if fid == C.D_RAIN_ACCUM:      # one k = 0 plane: (j, i) with ghosts
+            a = np.empty((self.L.sz // self.L.sy, self.L.sy))
+        else:
+            nc = C.ncomp_of(self.cfg, fid)
+            a = np.empty((nc,) + self.L.shape3())
         self._chk(lib().orc_get(self.h, fid, a.ctypes.data))
         return a
 
@@ -79,6 +103,10 @@ class Oracle:
 
     def step(self, t0, dt, n):
         self._chk(lib().orc_step(self.h, t0, dt, n))
+
+    def kessler(self, dt):
+        """Kessler on the level's cons (ERFX_F_CONS) with step dt."""
+        self._chk(lib().orc_kessler(self.h, dt))
 
     def call(self, name, *args):
         self._chk(getattr(lib(), name)(self.h, *args))

@@ -29,6 +29,12 @@ def _setup(kind):
     elif kind == "cbl":
         cfg, f = P.cbl(20, 18, 16, seed=5)
         dt = 0.5
+    elif kind.startswith("moist"):   # Kessler, buoyancy type moist1..moist4
+        cfg, f = P.moist_bubble(24, 20, 16, seed=13, buoyancy_type=int(kind[-1]))
+        dt = 1.0
+    elif kind == "norain":
+        cfg, f = P.moist_bubble(24, 20, 16, seed=17, norain=True)
+        dt = 1.0
     else:  # walls + molecular diffusion
         cfg, f = P.rising_thermal(24, 8, 16, seed=11, adv=C.ADV_UPWIND3)
         cfg.bc[0] = C.BC_SYMMETRY
@@ -42,6 +48,7 @@ def _setup(kind):
     orc.step(0.0, dt, 1)
     sim = Sim(cfg)
     ids = list(ALL_IDS) + ([C.D_EDDY] if cfg.les_type != C.LES_NONE else [])
+    ids += [C.D_RAIN_ACCUM] if cfg.moisture_type != C.MOIST_NONE else []
     for fid in ids:
         sim.upload(fid, orc.get(fid))
     return cfg, orc, sim, ids, dt
@@ -64,7 +71,8 @@ def _st(sim, s):
     return ctypes.byref(sim.state(s))
 
 
-KINDS = ["thermal", "cbl", "walls"]
+KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain"]
+MOIST_BUOYANCY = ["moist1", "moist2", "moist3", "moist4"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -132,7 +140,7 @@ def test_turbulent_viscosity(kind):
     assert _same(sim, orc, ids) == {}
 
 
-@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kind", KINDS + ["moist3", "moist4"])
 @pytest.mark.parametrize("nrk", [0, 2])
 def test_slow_rhs_pre(kind, nrk):
     from erf_amd.lib import lib, check
@@ -159,4 +167,14 @@ def test_slow_rhs_post(kind, nrk):
                                    _st(sim, C.S_SUM), sim.ptr(C.D_PRIM), _st(sim, C.S_SCRATCH), sim.ptr(C.F_XVEL),
                                    sim.ptr(C.F_YVEL), eddy))
     sim.sync()
+    assert _same(sim, orc, ids) == {}
+
+
+@pytest.mark.parametrize("kind", ["moist1", "norain"])
+@pytest.mark.parametrize("dt", [1.0, 3.0])
+def test_kessler(kind, dt):
+    from erf_amd.lib import lib, check
+    cfg, orc, sim, ids, _ = _setup(kind)
+    orc.kessler(dt)
+    check(lib().erfx_kessler_advance(sim.ctx, sim.ptr(C.F_CONS), dt, sim.ptr(C.D_RAIN_ACCUM)))
     assert _same(sim, orc, ids) == {}

@@ -45,6 +45,12 @@ def _compare(cfg, f, dt, nsteps, tol, exact=False, rayleigh=None):
             e = rel_err(a[c], b[c])
             worst[(fid, c)] = e
             ndiff[(fid, c)] = int((a[c] != b[c]).sum())
+    if cfg.moisture_type != C.MOIST_NONE:
+        ng = sim.L.ng
+        a = sim.download(C.D_RAIN_ACCUM)[ng:ng + cfg.ny, ng:ng + cfg.nx]
+        b = orc.get(C.D_RAIN_ACCUM)[ng:ng + cfg.ny, ng:ng + cfg.nx]
+        worst[("rain", 0)] = rel_err(a, b) if np.abs(b).max() > 0 else float(np.abs(a).max())
+        ndiff[("rain", 0)] = int((a != b).sum())
     sim.close()
     bad = {k: v for k, v in worst.items() if not v <= tol}
     assert not bad, "fields beyond %g: %s (all: %s)" % (tol, bad, worst)
@@ -154,3 +160,14 @@ def test_volume_integrals_exactly_rounded():
             m0 = got[0]
     assert abs(got[0] - m0) <= 1e-14 * m0
     sim.close()
+
+
+@pytest.mark.parametrize("btype,norain", [(1, False), (2, False), (3, False), (4, False), (1, True)])
+def test_moist_kessler_step_exact(btype, norain):
+    cfg, f = P.moist_bubble(24, 20, 16, seed=21, buoyancy_type=btype, norain=norain)
+    _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
+
+
+def test_moist_kessler_multistep():
+    cfg, f = P.moist_bubble(24, 20, 16, seed=23)
+    _compare(cfg, f, 1.0, 5, 1e-9)
