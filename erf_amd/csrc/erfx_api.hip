@@ -2,6 +2,7 @@
 // whole-step driver (ERF::Advance -> advance_dycore -> MRISplitIntegrator::advance).
 #include "erfx_common.h"
 #include "erfx_internal.h"
+#include "erfx_mlp.h"
 
 #include <algorithm>
 #include <cmath>
@@ -121,6 +122,14 @@ struct erfx_ctx {
     double* kfz = nullptr;    // Kessler sedimentation flux on z-faces (moist only)
     int* kbad = nullptr;      // device word: a Kessler temperature left the saturation fit's range
     std::vector<void*> allocs;
+};
+
+// ---- learned microphysics ------------------------------------------------------
+struct erfx_mlp {
+    Net net;
+    std::vector<void*> allocs;
+    double x_mean[mlp::NI], x_scale[mlp::NI], x_lo[mlp::NI], x_hi[mlp::NI], y_scale[mlp::NO];
+    bool scaled = false;
 };
 
 // ---- whole-step driver ---------------------------------------------------------
@@ -412,6 +421,127 @@ int erfx_kessler_advance(erfx_ctx* c, double* cons, double dt, double* rain_accu
     // the range check of the saturation fit needs the result (ERF aborts there)
     ERFX_HIP(hipStreamSynchronize(c->stream));
     return kessler_status(c);
+}
+
+static int balloc(std::vector<void*>& list, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    list.push_back(*p);
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMemset");
+    return ERFX_OK;
+}
+
+int erfx_mlp_create(int capacity, erfx_mlp** out) {
+    using namespace erfx::mlp;
+    if (!out || capacity < 1) { set_error("null argument or capacity < 1"); return ERFX_ERR_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return ERFX_ERR_NODEVICE; }
+    auto* m = new erfx_mlp();
+    Net& N = m->net;
+    N.cap = capacity;
+    const size_t cap = (size_t)capacity;
+    const size_t nblk = (cap + 63) / 64 + 1;     // per-block gradient partials + one block to stage the sum
+    struct { void** p; size_t bytes; } req[] = {
+        {(void**)&N.P.w, NPARAM * 4}, {(void**)&N.P.m, NPARAM * 4}, {(void**)&N.P.v, NPARAM * 4},
+        {(void**)&N.P.w1b, NI * NH * 2}, {(void**)&N.P.w2b, NH * NH * 2}, {(void**)&N.P.w2tb, NH * NH * 2},
+        {(void**)&N.P.w3b, NH * NO * 2},
+        {(void**)&N.T.z1, cap * NH * 4}, {(void**)&N.T.h1, cap * NH * 2}, {(void**)&N.T.z2, cap * NH * 4},
+        {(void**)&N.T.h2, cap * NH * 2}, {(void**)&N.T.dz2, cap * NH * 2}, {(void**)&N.T.dz1, cap * NH * 4},
+        {(void**)&N.T.dy, cap * NO * 4}, {(void**)&N.T.loss, cap * 4},
+        {(void**)&N.ybuf, cap * NO * 4}, {(void**)&N.partial, nblk * NPARAM * 4}, {(void**)&N.loss_dev, 4},
+    };
+    for (auto& r : req) {
+        int rc = balloc(m->allocs, r.p, r.bytes);
+        if (rc) { erfx_mlp_destroy(m); return rc; }
+    }
+    N.W = Weights{N.P.w1b, N.P.w + OFF_B1, N.P.w2b, N.P.w2tb, N.P.w + OFF_B2, N.P.w3b, N.P.w + OFF_B3};
+    for (int k = 0; k < NI; ++k) { m->x_mean[k] = 0.0; m->x_scale[k] = 1.0; m->x_lo[k] = -1e300; m->x_hi[k] = 1e300; }
+    for (int o = 0; o < NO; ++o) m->y_scale[o] = 1.0;
+    *out = m;
+    return ERFX_OK;
+}
+
+int erfx_mlp_destroy(erfx_mlp* m) {
+    if (!m) return ERFX_OK;
+    (void)hipDeviceSynchronize();
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+    return ERFX_OK;
+}
+
+int erfx_mlp_set_params(erfx_mlp* m, const float* h) {
+    if (!m || !h) { set_error("null argument"); return ERFX_ERR_ARG; }
+    Net& N = m->net;
+    ERFX_HIP(hipMemcpy(N.P.w, h, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    ERFX_HIP(hipMemset(N.P.m, 0, mlp::NPARAM * sizeof(float)));
+    ERFX_HIP(hipMemset(N.P.v, 0, mlp::NPARAM * sizeof(float)));
+    N.t = 0;
+    mlp_refresh(N, nullptr);
+    ERFX_HIP(hipDeviceSynchronize());
+    return launch_status();
+}
+
+int erfx_mlp_get_params(erfx_mlp* m, float* h) {
+    if (!m || !h) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipDeviceSynchronize());
+    ERFX_HIP(hipMemcpy(h, m->net.P.w, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return ERFX_OK;
+}
+
+int erfx_mlp_set_scaling(erfx_mlp* m, const double* xm, const double* xs, const double* xl, const double* xh,
+                         const double* ys) {
+    if (!m || !xm || !xs || !xl || !xh || !ys) { set_error("null argument"); return ERFX_ERR_ARG; }
+    for (int k = 0; k < mlp::NI; ++k) {
+        if (!(xs[k] > 0)) { set_error("x_scale must be positive"); return ERFX_ERR_ARG; }
+        m->x_mean[k] = xm[k]; m->x_scale[k] = xs[k]; m->x_lo[k] = xl[k]; m->x_hi[k] = xh[k];
+    }
+    for (int o = 0; o < mlp::NO; ++o) {
+        if (!(ys[o] > 0)) { set_error("y_scale must be positive"); return ERFX_ERR_ARG; }
+        m->y_scale[o] = ys[o];
+    }
+    m->scaled = true;
+    return ERFX_OK;
+}
+
+int erfx_mlp_forward(erfx_mlp* m, const float* x, float* y, int n) {
+    if (!m || (n > 0 && (!x || !y)) || n < 0) { set_error("bad argument"); return ERFX_ERR_ARG; }
+    mlp_forward(m->net, x, y, n, nullptr);
+    return launch_status();
+}
+
+static int mlp_pass(erfx_mlp* m, const float* x, const float* t, int n, bool update, double lr, float* grad_host,
+                    double* loss) {
+    if (!m || !x || !t || n < 1) { set_error("bad argument"); return ERFX_ERR_ARG; }
+    Net& N = m->net;
+    if (n > N.cap) { set_error("more samples than the capacity given at creation"); return ERFX_ERR_ARG; }
+    if (update) N.t += 1;
+    // the summed gradient is staged in the partial buffer's last block, past
+    // every block a pass of n <= cap samples uses
+    float* gstage = grad_host ? N.partial + (size_t)((N.cap + 63) / 64) * mlp::NPARAM : nullptr;
+    mlp_train_pass(N, x, t, n, update, (float)lr, gstage, nullptr);
+    int rc = launch_status();
+    if (rc) return rc;
+    if (grad_host || loss) {
+        ERFX_HIP(hipDeviceSynchronize());
+        if (grad_host) ERFX_HIP(hipMemcpy(grad_host, gstage, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) {
+            float ls = 0.0f;
+            ERFX_HIP(hipMemcpy(&ls, N.loss_dev, sizeof(float), hipMemcpyDeviceToHost));
+            *loss = (double)ls / ((double)n * mlp::NO);
+        }
+    }
+    return ERFX_OK;
+}
+
+int erfx_mlp_gradient(erfx_mlp* m, const float* x, const float* t, int n, float* grad_host, double* loss) {
+    if (!grad_host) { set_error("null argument"); return ERFX_ERR_ARG; }
+    return mlp_pass(m, x, t, n, false, 0.0, grad_host, loss);
+}
+
+int erfx_mlp_train_step(erfx_mlp* m, const float* x, const float* t, int n, double lr, double* loss) {
+    if (!(lr > 0)) { set_error("lr must be positive"); return ERFX_ERR_ARG; }
+    return mlp_pass(m, x, t, n, true, lr, nullptr, loss);
 }
 
 int erfx_slow_rhs_post(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, const erfx_state* old,

@@ -46,6 +46,14 @@ SIGNATURES = {
     "erfx_slow_rhs_post": (_I, [_P, _I, _D, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
                                 ctypes.POINTER(State), _P, ctypes.POINTER(State), _P, _P, _P]),
     "erfx_kessler_advance": (_I, [_P, _P, _D, _P]),
+    "erfx_mlp_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "erfx_mlp_destroy": (_I, [_P]),
+    "erfx_mlp_set_params": (_I, [_P, _P]),
+    "erfx_mlp_get_params": (_I, [_P, _P]),
+    "erfx_mlp_set_scaling": (_I, [_P, _P, _P, _P, _P, _P]),
+    "erfx_mlp_forward": (_I, [_P, _P, _P, _I]),
+    "erfx_mlp_gradient": (_I, [_P, _P, _P, _I, _P, ctypes.POINTER(_D)]),
+    "erfx_mlp_train_step": (_I, [_P, _P, _P, _I, _D, ctypes.POINTER(_D)]),
     "erfx_sim_create": (_I, [ctypes.POINTER(C.Config), ctypes.POINTER(_P)]),
     "erfx_sim_destroy": (_I, [_P]),
     "erfx_sim_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
@@ -182,3 +190,54 @@ class Sim:
     @property
     def stream(self):
         return lib().erfx_sim_stream(self.h)
+
+
+class MLP:
+    """The learned Kessler stand-in (erfx_mlp): 8 -> 128 -> 128 -> 4, GELU,
+    BF16 weights and activations with FP32 accumulation, trained on the GPU."""
+    NPARAM = 18180
+
+    def __init__(self, capacity):
+        h = _P()
+        check(lib().erfx_mlp_create(int(capacity), ctypes.byref(h)))
+        self.h = h
+        self.capacity = int(capacity)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erfx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.NPARAM
+        check(lib().erfx_mlp_set_params(self.h, a.ctypes.data))
+
+    def get_params(self):
+        a = np.empty(self.NPARAM, dtype=np.float32)
+        check(lib().erfx_mlp_get_params(self.h, a.ctypes.data))
+        return a
+
+    def set_scaling(self, x_mean, x_scale, x_lo, x_hi, y_scale):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x_mean, x_scale, x_lo, x_hi, y_scale)]
+        self._scaling = arrs
+        check(lib().erfx_mlp_set_scaling(self.h, *(a.ctypes.data for a in arrs)))
+
+    def forward(self, x_dev, y_dev, n):
+        """x_dev, y_dev: device pointers (e.g. torch tensor data_ptr())."""
+        check(lib().erfx_mlp_forward(self.h, x_dev, y_dev, int(n)))
+
+    def gradient(self, x_dev, t_dev, n):
+        g = np.empty(self.NPARAM, dtype=np.float32)
+        loss = _D()
+        check(lib().erfx_mlp_gradient(self.h, x_dev, t_dev, int(n), g.ctypes.data, ctypes.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, x_dev, t_dev, n, lr, want_loss=True):
+        loss = _D()
+        check(lib().erfx_mlp_train_step(self.h, x_dev, t_dev, int(n), float(lr),
+                                        ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None

@@ -252,6 +252,39 @@ int erfx_slow_rhs_post(erfx_ctx* ctx, int nrk, double slow_dt,
  * saturation fit's range (below -85 C), reported here as ERFX_ERR_STATE. */
 int erfx_kessler_advance(erfx_ctx* ctx, double* cons, double dt, double* rain_accum);
 
+/* ---- learned stand-in for the Kessler cell tendencies ------------------- *
+ * No counterpart in ERF: a multilayer perceptron 8 -> 128 -> 128 -> 4 with GELU
+ * activations, BF16 weights and activations, FP32 accumulation, trained on the
+ * GPU by the library's own forward, backward and Adam kernels on samples of
+ * Kessler's own tendencies (erfx_kessler_samples).  Inputs per cell:
+ * (T [K], p [mbar], rho, qv, qc, qr, w [m/s, cell centre], dt [s]); outputs:
+ * the local tendencies of one step (dqv, dqc, dqr, dtheta), sedimentation
+ * excluded.  Parameters (FP32 master copy, ERFX_MLP_NPARAM floats):
+ * W1 [8][128], b1 [128], W2 [128][128], b2 [128], W3 [128][4], b3 [4],
+ * row-major; a layer computes z_u = b_u + sum_k h_k W[k][u]. */
+typedef struct erfx_mlp erfx_mlp;
+enum { ERFX_MLP_NIN = 8, ERFX_MLP_NHID = 128, ERFX_MLP_NOUT = 4, ERFX_MLP_NPARAM = 18180 };
+
+/* `capacity`: largest number of samples per training pass. */
+int erfx_mlp_create(int capacity, erfx_mlp** out);
+int erfx_mlp_destroy(erfx_mlp* m);
+int erfx_mlp_set_params(erfx_mlp* m, const float* host_params);
+int erfx_mlp_get_params(erfx_mlp* m, float* host_params);
+/* Feature scaling: the network sees (x - x_mean) / x_scale and produces
+ * y / y_scale; x_lo, x_hi bound the raw inputs it was trained on (outside,
+ * Kessler itself is used).  8, 8, 8, 8 and 4 host doubles. */
+int erfx_mlp_set_scaling(erfx_mlp* m, const double* x_mean, const double* x_scale, const double* x_lo,
+                         const double* x_hi, const double* y_scale);
+/* Network on device arrays of scaled values: x [n][8] -> y [n][4] (FP32). */
+int erfx_mlp_forward(erfx_mlp* m, const float* x, float* y, int n);
+/* Gradient of the mean squared error over n <= capacity samples (device x
+ * [n][8], targets t [n][4], scaled) into grad_host (ERFX_MLP_NPARAM floats);
+ * *loss = the mean squared error.  Synchronous. */
+int erfx_mlp_gradient(erfx_mlp* m, const float* x, const float* t, int n, float* grad_host, double* loss);
+/* One Adam step (beta 0.9, 0.999; eps 1e-8) on a minibatch; *loss as above,
+ * before the update.  Synchronous when loss is non-NULL. */
+int erfx_mlp_train_step(erfx_mlp* m, const float* x, const float* t, int n, double lr, double* loss);
+
 /* ---- whole-step driver ------------------------------------------------- *
  * ERF::Advance + advance_dycore + MRISplitIntegrator::advance on one box
  * (ERF_Advance.cpp:20-233, ERF_advance_dycore.cpp:37-305, ERF_MRI.H:187-369).

@@ -39,6 +39,13 @@ Field* field_by_id(Level& L, int id) {
     return nullptr;
 }
 
+// The library's whole-step driver keeps vars_new's cons and the integrator's
+// S_new cons in one buffer (ERF aliases them, ERF_Advance.cpp:157); the
+// oracle keeps two copies, so a hook that writes one of them mirrors it into
+// the other.
+void mirror_snew_to_cons(Level& L) { L.cons.v = L.S_new.cons.v; }
+void mirror_cons_to_snew(Level& L) { L.S_new.cons.v = L.cons.v; }
+
 template <class F> int guard(F&& f) {
     try { f(); return 0; }
     catch (const std::exception& e) { g_err = e.what(); return -1; }
@@ -174,6 +181,7 @@ int orc_apply_bcs(void* h, int state_index, int ng_cons, int ng_vel, int fast_on
     return guard([&] {
         L.apply_bcs(L.A(*st[state_index]), L.A(L.xvel), L.A(L.yvel), L.A(L.zvel), ng_cons, ng_vel,
                     fast_only != 0, synced != 0);
+        if (state_index == 1) mirror_snew_to_cons(L);
     });
 }
 int orc_make_fast_coeffs(void* h, double dtau, double beta_s) {
@@ -211,13 +219,17 @@ int orc_slow_rhs_post(void* h, int nrk, double slow_dt) {
         std::fill(L.cc_src.v.begin(), L.cc_src.v.end(), 0.0);
         L.slow_rhs_post(nrk, slow_dt, L.A(L.F_slow), L.A(L.S_old), L.A(L.S_new), L.A(L.S_sum),
                         L.A(L.prim), L.A(L.S_scratch), L.A(L.xvel), L.A(L.yvel));
+        mirror_snew_to_cons(L);
     });
 }
 
 // Kessler on the level's vars_new cons with step dt (advance_microphysics)
 int orc_kessler(void* h, double dt) {
     Level& L = *static_cast<Level*>(h);
-    return guard([&] { L.kessler_advance(L.A(L.cons), dt); });
+    return guard([&] {
+        L.kessler_advance(L.A(L.cons), dt);
+        mirror_cons_to_snew(L);
+    });
 }
 
 // saturation functions (K, mbar) for the closed-form tests

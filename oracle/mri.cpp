@@ -83,8 +83,8 @@ void Level::advance(double time, double dt) {
     velocity_to_momentum(u, gv, v, gv, w, gv, c, So.xmom, So.ymom, So.zmom);
     copy_field(S_old.cons, cons_old_vars);   // cons_mf copy incl. ghosts (:127-128)
     advance_dycore(time, dt);
+    kessler_advance(A(S_new.cons), dt);       // advance_microphysics (:172-174) on vars_new
     copy_field(cons, S_new.cons);             // state_new cons aliases vars_new
-    kessler_advance(A(cons), dt);             // advance_microphysics (:172-174)
 }
 
 }  // namespace orc
