@@ -121,6 +121,14 @@ struct erfx_ctx {
     double* ray = nullptr;    // Rayleigh profiles and per-level factors (RayleighTable layout)
     double* kfz = nullptr;    // Kessler sedimentation flux on z-faces (moist only)
     int* kbad = nullptr;      // device word: a Kessler temperature left the saturation fit's range
+    // learned microphysics work space (allocated on first use): per-cell slot,
+    // network rows, their cells, two device counters and their pinned host copy
+    int* eslot = nullptr;
+    float* ex = nullptr;
+    float* ey = nullptr;
+    long* erows = nullptr;
+    unsigned long long* ecount = nullptr;
+    unsigned long long* ecount_host = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -128,7 +136,7 @@ struct erfx_ctx {
 struct erfx_mlp {
     Net net;
     std::vector<void*> allocs;
-    double x_mean[mlp::NI], x_scale[mlp::NI], x_lo[mlp::NI], x_hi[mlp::NI], y_scale[mlp::NO];
+    MlpScaling sc;
     bool scaled = false;
 };
 
@@ -148,6 +156,11 @@ struct erfx_sim {
     double* red = nullptr;    // partial maxima of the time-step estimate (EST_BLOCKS x 2)
     double* red_host = nullptr;
     double* rain_accum = nullptr;   // Kessler surface rain, k = 0 plane (sz doubles)
+    erfx_mlp* mlp = nullptr;        // learned microphysics in place of Kessler's local processes
+    long mlp_rows = 0, mlp_fallback = 0;
+    float* samples = nullptr;       // training rows harvested before each step's microphysics
+    long samples_max = 0;
+    unsigned long long* samples_count = nullptr;
 };
 
 static void timer_free(KTimer& t) {
@@ -263,6 +276,7 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
 
 int erfx_ctx_destroy(erfx_ctx* c) {
     if (!c) return ERFX_OK;
+    if (c->ecount_host) (void)hipHostFree(c->ecount_host);
     for (void* p : c->allocs) (void)hipFree(p);
     delete c;
     return ERFX_OK;
@@ -456,8 +470,8 @@ int erfx_mlp_create(int capacity, erfx_mlp** out) {
         if (rc) { erfx_mlp_destroy(m); return rc; }
     }
     N.W = Weights{N.P.w1b, N.P.w + OFF_B1, N.P.w2b, N.P.w2tb, N.P.w + OFF_B2, N.P.w3b, N.P.w + OFF_B3};
-    for (int k = 0; k < NI; ++k) { m->x_mean[k] = 0.0; m->x_scale[k] = 1.0; m->x_lo[k] = -1e300; m->x_hi[k] = 1e300; }
-    for (int o = 0; o < NO; ++o) m->y_scale[o] = 1.0;
+    for (int k = 0; k < NI; ++k) { m->sc.x_mean[k] = 0.0; m->sc.x_scale[k] = 1.0; m->sc.x_lo[k] = -1e300; m->sc.x_hi[k] = 1e300; }
+    for (int o = 0; o < NO; ++o) m->sc.y_scale[o] = 1.0;
     *out = m;
     return ERFX_OK;
 }
@@ -494,11 +508,11 @@ int erfx_mlp_set_scaling(erfx_mlp* m, const double* xm, const double* xs, const 
     if (!m || !xm || !xs || !xl || !xh || !ys) { set_error("null argument"); return ERFX_ERR_ARG; }
     for (int k = 0; k < mlp::NI; ++k) {
         if (!(xs[k] > 0)) { set_error("x_scale must be positive"); return ERFX_ERR_ARG; }
-        m->x_mean[k] = xm[k]; m->x_scale[k] = xs[k]; m->x_lo[k] = xl[k]; m->x_hi[k] = xh[k];
+        m->sc.x_mean[k] = xm[k]; m->sc.x_scale[k] = xs[k]; m->sc.x_lo[k] = xl[k]; m->sc.x_hi[k] = xh[k];
     }
     for (int o = 0; o < mlp::NO; ++o) {
         if (!(ys[o] > 0)) { set_error("y_scale must be positive"); return ERFX_ERR_ARG; }
-        m->y_scale[o] = ys[o];
+        m->sc.y_scale[o] = ys[o];
     }
     m->scaled = true;
     return ERFX_OK;
@@ -542,6 +556,55 @@ int erfx_mlp_gradient(erfx_mlp* m, const float* x, const float* t, int n, float*
 int erfx_mlp_train_step(erfx_mlp* m, const float* x, const float* t, int n, double lr, double* loss) {
     if (!(lr > 0)) { set_error("lr must be positive"); return ERFX_ERR_ARG; }
     return mlp_pass(m, x, t, n, true, lr, nullptr, loss);
+}
+
+static int emulator_space(erfx_ctx* c) {
+    if (c->eslot) return ERFX_OK;
+    const size_t ne = (size_t)c->P.L.ne;
+    const size_t ncell = (size_t)c->P.L.nx * c->P.L.ny * c->P.L.nz;
+    int rc;
+    if ((rc = balloc(c->allocs, (void**)&c->eslot, ne * sizeof(int))) ||
+        (rc = balloc(c->allocs, (void**)&c->ex, ncell * mlp::NI * sizeof(float))) ||
+        (rc = balloc(c->allocs, (void**)&c->ey, ncell * mlp::NO * sizeof(float))) ||
+        (rc = balloc(c->allocs, (void**)&c->erows, ncell * sizeof(long))) ||
+        (rc = balloc(c->allocs, (void**)&c->ecount, 2 * sizeof(unsigned long long))))
+        return rc;
+    if (hipHostMalloc((void**)&c->ecount_host, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        set_error("hipHostMalloc failed"); return ERFX_ERR_HIP;
+    }
+    return ERFX_OK;
+}
+
+int erfx_kessler_samples(erfx_ctx* c, const double* cons, const double* zvel, double dt, float* rows, long max_rows,
+                         long* count) {
+    if (!c || !cons || !zvel || !rows || !count || max_rows < 0) { set_error("bad argument"); return ERFX_ERR_ARG; }
+    if (c->P.cfg.moisture_type != ERFX_MOIST_KESSLER) { set_error("samples need the Kessler model with rain"); return ERFX_ERR_ARG; }
+    int rc = emulator_space(c);
+    if (rc) return rc;
+    ERFX_HIP(hipMemsetAsync(c->ecount, 0, sizeof(unsigned long long), c->stream));
+    kessler_samples(c->P, cons, zvel, dt, rows, max_rows, c->ecount, c->kbad, c->stream);
+    if ((rc = launch_status())) return rc;
+    ERFX_HIP(hipMemcpyAsync(c->ecount_host, c->ecount, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    ERFX_HIP(hipStreamSynchronize(c->stream));
+    *count = (long)c->ecount_host[0];
+    return kessler_status(c);
+}
+
+int erfx_kessler_mlp_advance(erfx_ctx* c, erfx_mlp* m, double* cons, const double* zvel, double dt,
+                             double* rain_accum, long* n_mlp, long* n_fallback) {
+    if (!c || !m || !cons || !zvel) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (c->P.cfg.moisture_type != ERFX_MOIST_KESSLER) { set_error("the learned step replaces Kessler with rain"); return ERFX_ERR_ARG; }
+    if (!m->scaled) { set_error("set the network's scaling first (erfx_mlp_set_scaling)"); return ERFX_ERR_STATE; }
+    if (!(dt > 0)) { set_error("dt must be positive"); return ERFX_ERR_ARG; }
+    int rc = emulator_space(c);
+    if (rc) return rc;
+    kessler_mlp_advance(c->P, m->net, m->sc, cons, zvel, c->kfz, rain_accum, dt, c->eslot, c->ex, c->ey, c->erows,
+                        c->ecount, c->ecount_host, c->kbad, c->stream);
+    if ((rc = launch_status())) return rc;
+    ERFX_HIP(hipStreamSynchronize(c->stream));
+    if (n_mlp) *n_mlp = (long)c->ecount_host[0];
+    if (n_fallback) *n_fallback = (long)c->ecount_host[1];
+    return kessler_status(c);
 }
 
 int erfx_slow_rhs_post(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, const erfx_state* old,
@@ -760,11 +823,65 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
         apply_bcs(P, Sn, un, vn, wn, P.ng_cons, P.ng_vel, false, false, st);
     }
     // advance_microphysics on the new state (ERF_Advance.cpp:172-174)
-    if (P.moist) kessler_advance(P, Sn.cons, c->kfz, s->rain_accum, dt, st, c->kbad);
+    if (P.moist) {
+        if (s->samples) {
+            kessler_samples(P, Sn.cons, wn, dt, s->samples + 0, s->samples_max, s->samples_count, c->kbad, st);
+        }
+        if (s->mlp) {
+            int rc = emulator_space(c);
+            if (rc) return rc;
+            kessler_mlp_advance(P, s->mlp->net, s->mlp->sc, Sn.cons, wn, c->kfz, s->rain_accum, dt, c->eslot, c->ex,
+                                c->ey, c->erows, c->ecount, c->ecount_host, c->kbad, st);
+            s->mlp_rows += (long)c->ecount_host[0];
+            s->mlp_fallback += (long)c->ecount_host[1];
+        } else {
+            kessler_advance(P, Sn.cons, c->kfz, s->rain_accum, dt, st, c->kbad);
+        }
+    }
     s->cur = nw;
     s->S[0].cons = s->cons[1 - s->cur];
     s->S[1].cons = s->cons[s->cur];
     return launch_status();
+}
+
+int erfx_sim_set_mlp(erfx_sim* s, erfx_mlp* m) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (m && s->ctx->P.cfg.moisture_type != ERFX_MOIST_KESSLER) { set_error("the learned step replaces Kessler with rain"); return ERFX_ERR_ARG; }
+    if (m && !m->scaled) { set_error("set the network's scaling first (erfx_mlp_set_scaling)"); return ERFX_ERR_STATE; }
+    s->mlp = m;
+    s->mlp_rows = s->mlp_fallback = 0;
+    return ERFX_OK;
+}
+
+int erfx_sim_mlp_counts(erfx_sim* s, long* n_mlp, long* n_fallback) {
+    if (!s || !n_mlp || !n_fallback) { set_error("null argument"); return ERFX_ERR_ARG; }
+    *n_mlp = s->mlp_rows;
+    *n_fallback = s->mlp_fallback;
+    return ERFX_OK;
+}
+
+int erfx_sim_set_sampling(erfx_sim* s, float* rows, long max_rows) {
+    if (!s || max_rows < 0 || (max_rows > 0 && !rows)) { set_error("bad argument"); return ERFX_ERR_ARG; }
+    if (s->ctx->P.cfg.moisture_type != ERFX_MOIST_KESSLER) { set_error("samples need the Kessler model with rain"); return ERFX_ERR_ARG; }
+    if (!s->samples_count) {
+        int rc = balloc(s->allocs, (void**)&s->samples_count, sizeof(unsigned long long));
+        if (rc) return rc;
+    }
+    ERFX_HIP(hipMemset(s->samples_count, 0, sizeof(unsigned long long)));
+    s->samples = max_rows > 0 ? rows : nullptr;
+    s->samples_max = max_rows;
+    return ERFX_OK;
+}
+
+int erfx_sim_sample_count(erfx_sim* s, long* count) {
+    if (!s || !count) { set_error("null argument"); return ERFX_ERR_ARG; }
+    unsigned long long h = 0;
+    if (s->samples_count) {
+        ERFX_HIP(hipStreamSynchronize(s->stream));
+        ERFX_HIP(hipMemcpy(&h, s->samples_count, sizeof(h), hipMemcpyDeviceToHost));
+    }
+    *count = (long)h;
+    return ERFX_OK;
 }
 
 int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {

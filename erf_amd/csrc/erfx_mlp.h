@@ -79,9 +79,22 @@ struct Net {
     long t = 0;                 // Adam step count
 };
 
+// feature scaling of the network and the raw range it was trained on
+struct MlpScaling {
+    double x_mean[mlp::NI], x_scale[mlp::NI], x_lo[mlp::NI], x_hi[mlp::NI], y_scale[mlp::NO];
+};
+
 void mlp_forward(const Net& N, const float* x, float* y, int n, hipStream_t s);
 void mlp_refresh(const Net& N, hipStream_t s);
 void mlp_train_pass(Net& N, const float* x, const float* tgt, int n, bool update, float lr, float* grad_out,
                     hipStream_t s);
+
+// erfx_kessler.hip: Kessler's local tendencies as training rows, and the
+// microphysics step with the network (see there)
+void kessler_samples(const Params& P, const double* cons, const double* w, double dt, float* out, long max_rows,
+                     unsigned long long* counter, int* bad, hipStream_t s);
+void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, double* cons, const double* w,
+                         double* fz, double* rain_accum, double dt, int* slot, float* x, float* y, long* rows_cell,
+                         unsigned long long* counter, unsigned long long* host_counts, int* bad, hipStream_t s);
 
 }  // namespace erfx

@@ -54,6 +54,13 @@ SIGNATURES = {
     "erfx_mlp_forward": (_I, [_P, _P, _P, _I]),
     "erfx_mlp_gradient": (_I, [_P, _P, _P, _I, _P, ctypes.POINTER(_D)]),
     "erfx_mlp_train_step": (_I, [_P, _P, _P, _I, _D, ctypes.POINTER(_D)]),
+    "erfx_kessler_samples": (_I, [_P, _P, _P, _D, _P, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
+    "erfx_kessler_mlp_advance": (_I, [_P, _P, _P, _P, _D, _P, ctypes.POINTER(ctypes.c_long),
+                                      ctypes.POINTER(ctypes.c_long)]),
+    "erfx_sim_set_mlp": (_I, [_P, _P]),
+    "erfx_sim_mlp_counts": (_I, [_P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
+    "erfx_sim_set_sampling": (_I, [_P, _P, ctypes.c_long]),
+    "erfx_sim_sample_count": (_I, [_P, ctypes.POINTER(ctypes.c_long)]),
     "erfx_sim_create": (_I, [ctypes.POINTER(C.Config), ctypes.POINTER(_P)]),
     "erfx_sim_destroy": (_I, [_P]),
     "erfx_sim_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
@@ -78,6 +85,14 @@ SIGNATURES = {
 def lib():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64, and a
+        # process that loads the system one first (through this library) leaves
+        # torch without a device.  Importing torch first makes the library bind
+        # to the runtime torch loaded, so both share buffers and streams.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("erf_amd/liberfx.so is not built: run __graft_entry__.build() "
                                "(make -C erf_amd/csrc)")
@@ -182,6 +197,24 @@ class Sim:
             if a.size != self.cfg.nz + 1:
                 raise ValueError("Rayleigh profiles need nz+1 values")
         check(lib().erfx_set_rayleigh(self.ctx, *(a.ctypes.data for a in arrs)))
+
+    def set_mlp(self, net):
+        """Microphysics by the learned stand-in (an MLP), or Kessler for None."""
+        self._mlp = net
+        check(lib().erfx_sim_set_mlp(self.h, net.h if net is not None else None))
+
+    def mlp_counts(self):
+        a, b = ctypes.c_long(), ctypes.c_long()
+        check(lib().erfx_sim_mlp_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_sampling(self, rows_dev, max_rows):
+        check(lib().erfx_sim_set_sampling(self.h, rows_dev, int(max_rows)))
+
+    def sample_count(self):
+        n = ctypes.c_long()
+        check(lib().erfx_sim_sample_count(self.h, ctypes.byref(n)))
+        return n.value
 
     @property
     def ctx(self):

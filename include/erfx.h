@@ -285,6 +285,25 @@ int erfx_mlp_gradient(erfx_mlp* m, const float* x, const float* t, int n, float*
  * before the update.  Synchronous when loss is non-NULL. */
 int erfx_mlp_train_step(erfx_mlp* m, const float* x, const float* t, int n, double lr, double* loss);
 
+/* Training rows from Kessler itself (rain model): for every cell whose local
+ * processes are active (supersaturated, cloudy, or rainy in subsaturated air;
+ * elsewhere Kessler's local tendencies are exactly 0), one row of 12 floats
+ * (T, p, rho, qv, qc, qr, w, dt, dqv, dqc, dqr, dtheta) on the device, appended
+ * in no fixed order; *count = active cells found (rows written: min(count,
+ * max_rows)).  cons and zvel as before the step's microphysics. Synchronous. */
+int erfx_kessler_samples(erfx_ctx* ctx, const double* cons, const double* zvel, double dt, float* rows,
+                         long max_rows, long* count);
+/* The microphysics step of erfx_kessler_advance (rain model) with the
+ * network's tendencies for the active cells inside its training range;
+ * inactive cells get Kessler's exact 0, active cells outside the range
+ * Kessler's own update.  The network's (dqv, dqc, dqr) are projected in FP64
+ * to conserve qv + qc + qr in each cell and keep every species >= 0; theta
+ * follows the latent heat of the vapour change, as in Kessler
+ * (ERF_Kessler.cpp:172-173); sedimentation is Kessler's flux form.
+ * *n_mlp, *n_fallback (may be NULL) count the two kinds of active cells. */
+int erfx_kessler_mlp_advance(erfx_ctx* ctx, erfx_mlp* m, double* cons, const double* zvel, double dt,
+                             double* rain_accum, long* n_mlp, long* n_fallback);
+
 /* ---- whole-step driver ------------------------------------------------- *
  * ERF::Advance + advance_dycore + MRISplitIntegrator::advance on one box
  * (ERF_Advance.cpp:20-233, ERF_advance_dycore.cpp:37-305, ERF_MRI.H:187-369).
@@ -362,6 +381,17 @@ int  erfx_sim_run(erfx_sim* sim, double t0, int max_steps, const erfx_dt_control
  * rho*theta and RhoScalar.  Summed in double-double, so the result is the
  * exactly rounded sum of the per-cell terms, independent of reduction order. */
 int  erfx_sim_integrals(erfx_sim* sim, double* mass, double* rhotheta, double* scalar);
+
+/* Microphysics of the sim's steps: the network (erfx_kessler_mlp_advance) when
+ * m is non-NULL, Kessler otherwise; counts of network and fallback cells
+ * summed over the steps since it was set. */
+int  erfx_sim_set_mlp(erfx_sim* sim, erfx_mlp* m);
+int  erfx_sim_mlp_counts(erfx_sim* sim, long* n_mlp, long* n_fallback);
+/* Harvest erfx_kessler_samples rows before every step's microphysics into a
+ * device buffer of max_rows x 12 floats (NULL / 0 stops); the count keeps
+ * growing past max_rows, rows beyond it are dropped. */
+int  erfx_sim_set_sampling(erfx_sim* sim, float* rows, long max_rows);
+int  erfx_sim_sample_count(erfx_sim* sim, long* count);
 
 /* Per-launch timing of the step's dominant kernel with HIP events on the sim's
  * stream (the role of ERF's BL_PROFILE regions): reset, run steps, then read
