@@ -138,5 +138,16 @@ def test_training_gain_and_emulated_run():
     # rho differs only through the dynamics' response to the heating
     assert errs["rho"] < 1e-4 and errs["rhotheta"] < 1e-3 and errs["qv"] < 1e-2
     assert errs["qc"] < 0.1 and errs["qr"] < 0.1
+    # microphysics cost per call on the last state, both ways (reported only)
+    from erf_amd.lib import lib, check
+    for name, call in (("kessler", lambda s: lib().erfx_kessler_advance(s.ctx, s.ptr(C.F_CONS), 1.0, None)),
+                       ("network", lambda s: lib().erfx_kessler_mlp_advance(s.ctx, net.h, s.ptr(C.F_CONS),
+                                                                            s.ptr(C.F_ZVEL), 1.0, None, None, None))):
+        check(call(ka))
+        t0 = time.perf_counter()
+        for _ in range(5):
+            check(call(ka))
+        print("%s microphysics on %dx%dx%d: %.3f ms per call" % (name, cfg.nx, cfg.ny, cfg.nz,
+                                                                   (time.perf_counter() - t0) / 5 * 1e3))
     ka.close()
     em.close()
