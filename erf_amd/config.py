@@ -20,6 +20,8 @@ NCONS_DRY, NCONS_MOIST = 5, 8
 MOIST_NONE, MOIST_KESSLER, MOIST_KESSLER_NORAIN = range(3)
 
 F_CONS, F_XVEL, F_YVEL, F_ZVEL, F_R0, F_P0, F_PI0 = range(7)
+F_Z_ND = 7          # node heights of a terrain-following grid
+D_DETJ, D_AX, D_AY, D_AZ, D_Z_CC = 27, 28, 29, 30, 31
 D_PRIM, D_PI_STAGE, D_FAST_COEFFS, D_EDDY = 16, 17, 18, 19
 D_XVEL_OLD, D_YVEL_OLD, D_ZVEL_OLD = 20, 21, 22
 D_XMOM_SRC, D_YMOM_SRC, D_ZMOM_SRC = 23, 24, 25
@@ -75,6 +77,7 @@ class Config(ctypes.Structure):
         ("moisture_type", ctypes.c_int), ("buoyancy_type", ctypes.c_int),
         ("moistscal_horiz_adv", ctypes.c_int), ("moistscal_vert_adv", ctypes.c_int),
         ("moistscal_horiz_upw_frac", ctypes.c_double), ("moistscal_vert_upw_frac", ctypes.c_double),
+        ("use_terrain", ctypes.c_int),
     ]
 
 
@@ -143,6 +146,7 @@ def default_config(nx, ny, nz, dx, dy, dz):
     c.buoyancy_type = 1
     c.moistscal_horiz_adv = c.moistscal_vert_adv = ADV_WENO3
     c.moistscal_horiz_upw_frac = c.moistscal_vert_upw_frac = 1.0
+    c.use_terrain = 0
     return c
 
 

@@ -69,6 +69,7 @@ static int validate(const erfx_config& c) {
         set_error("only buoyancy_type 1 is implemented for the dry state"); return ERFX_ERR_UNSUPPORTED;
     }
     if (c.buoyancy_type < 1 || c.buoyancy_type > 4) { set_error("buoyancy_type must be 1, 2, 3 or 4"); return ERFX_ERR_ARG; }
+    if (c.use_terrain) { set_error("terrain-following grids are not implemented on the GPU"); return ERFX_ERR_UNSUPPORTED; }
     if (c.moisture_type == ERFX_MOIST_KESSLER_NORAIN && c.buoyancy_type != 1) {
         // asserted by ERF (ERF_make_buoyancy.cpp:171-175)
         set_error("Kessler_NoRain requires buoyancy_type 1"); return ERFX_ERR_ARG;

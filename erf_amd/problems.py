@@ -91,6 +91,9 @@ def empty_fields(cfg):
     f = {}
     for fid in (C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0, C.F_PI0):
         f[fid] = np.zeros((C.ncomp_of(cfg, fid),) + shp)
+    if cfg.use_terrain:   # flat until a problem sets the real node heights
+        zeta = cfg.prob_lo[2] + (np.arange(shp[0]) - L.ng) * cfg.dz
+        f[C.F_Z_ND] = np.broadcast_to(zeta[:, None, None], shp)[None].copy()
     return L, f
 
 
@@ -205,6 +208,132 @@ def moist_bubble(nx=24, ny=20, nz=16, seed=0, buoyancy_type=None, norain=False, 
         cons[comp][sl] = (cons[C.RHO] * q)[sl]
     rng = np.random.default_rng(seed)
     _noise_vel(cfg, L, f, rng, 0.01, (0.0, 0.0, 0.0))
+    return cfg, f
+
+
+def terrain_nodes(cfg, L, height):
+    """Node heights of a terrain-following grid (Gal-Chen / basic terrain
+    following): z = zeta + h(x, y) (1 - zeta / ztop), zeta = k dz, on every node
+    of the layout.  `height(x, y)` gives the surface; lateral ghost nodes are
+    periodic images on periodic axes, and the vertical ghost nodes continue
+    the columns linearly."""
+    ng = L.ng
+    nzt, nyt, nxt = L.shape3()
+    ztop = cfg.nz * cfg.dz
+    ii = np.arange(nxt) - ng
+    jj = np.arange(nyt) - ng
+    if cfg.bc[0] == C.BC_PERIODIC:
+        ii = np.mod(ii, cfg.nx)
+    if cfg.bc[1] == C.BC_PERIODIC:
+        jj = np.mod(jj, cfg.ny)
+    x = cfg.prob_lo[0] + ii * cfg.dx
+    y = cfg.prob_lo[1] + jj * cfg.dy
+    h = height(x[None, :], y[:, None])                       # (ny_t, nx_t)
+    zeta = cfg.prob_lo[2] + (np.arange(nzt) - ng) * cfg.dz    # linear in k, ghosts included
+    z = zeta[:, None, None] + h[None, :, :] * (1.0 - zeta[:, None, None] / ztop)
+    return z[None]
+
+
+def hse_terrain(zc, zs, ps, theta, g=C.CONST_GRAV, rdOcp=C.R_d / C.Cp_d):
+    """Discrete hydrostatic balance on terrain-following columns, all columns
+    at once.  zc: cell-centre heights (k = -1 .. nz, first axis), zs, ps:
+    surface height and pressure of each column, theta: potential temperature
+    at zc.  Density from p(k) - p(k-1) = -g (rho(k) + rho(k-1)) / 2
+    (zc(k) - zc(k-1)) with the equation of state p = p(rho theta) (Newton per
+    level, as hse_density); the k = -1 layer copies rho(0) and extends p
+    hydrostatically.  Returns rho, p (the hydrostatic recursion) and Exner."""
+    nk = zc.shape[0]
+    r = np.zeros_like(zc)
+    p = np.zeros_like(zc)
+
+    def newton(k, p_ref, r_ref, w_ref, w_self):
+        # solve p_ref - g (w_ref r_ref + w_self rho) = p(rho theta) for rho
+        th = theta[k]
+        rk = np.full_like(p_ref, 1.0) if r_ref is None else r_ref.copy()
+        rr = 0.0 if r_ref is None else r_ref
+        for _ in range(30):
+            p_hse = p_ref - g * (w_ref * rr + w_self * rk)
+            A = p_hse - _p_of_rt(rk * th)
+            dpdr = C.Gamma * C.p_0 * np.power(C.R_d * th / C.p_0, C.Gamma) * np.power(rk, C.Gamma - 1.0)
+            rk = rk + A / (dpdr + g * w_self)
+        return rk, p_ref - g * (w_ref * rr + w_self * rk)
+
+    # k = 0: from the surface pressure, rho(0) over the half cell below the centre
+    r[1], p[1] = newton(1, ps, None, 0.0, zc[1] - zs)
+    for k in range(2, nk):
+        hdz = 0.5 * (zc[k] - zc[k - 1])
+        r[k], p[k] = newton(k, p[k - 1], r[k - 1], hdz, hdz)
+    r[0] = r[1]
+    p[0] = p[1] + g * r[1] * (zc[1] - zc[0])
+    return r, p, _exner_of_p(p, rdOcp)
+
+
+def hill_flow(nx=24, ny=16, nz=16, seed=0, h0=500.0, a=2000.0, u0=10.0, v0=0.0, moist=False,
+              adv=C.ADV_UPWIND3, lx=20000.0, ly=16000.0, noise=0.01, bubble=(0.3, 0.5), n2=0.0):
+    """Flow over a Gaussian hill on a terrain-following grid: lx x ly x 10 km
+    computational box, periodic laterally, slip walls; hill of height h0 and
+    half-width a at the domain centre; isentropic theta = 300 K base state at
+    the cells' true heights; uniform (u0, v0) plus seeded +-noise m/s and a
+    +1 K bubble at 3 km (at fractions `bubble` of the box, None for none).
+    n2 > 0 gives the initial state a stable stratification of
+    Brunt-Vaisala frequency squared n2 in hydrostatic balance (the base state
+    stays isentropic).  With `moist`, Kessler and a humid layer."""
+    cfg = C.default_config(nx, ny, nz, lx / nx, ly / ny, 10000.0 / nz)
+    cfg.use_terrain = 1
+    cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    cfg.mri_ratio = 6
+    if moist:
+        C.set_moisture(cfg, C.MOIST_KESSLER, 1)
+    L, f = empty_fields(cfg)
+    xc, yc = 0.5 * lx, 0.5 * ly
+    zn = terrain_nodes(cfg, L, lambda x, y: h0 * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / a ** 2))
+    f[C.F_Z_ND] = zn
+    z = zn[0]
+    ng = L.ng
+    # cell-centre and surface heights of every column (k = -1 .. nz)
+    zc = np.zeros_like(z)
+    zc[:-1, :-1, :-1] = 0.125 * (z[:-1, :-1, :-1] + z[:-1, :-1, 1:] + z[:-1, 1:, :-1] + z[:-1, 1:, 1:] +
+                                 z[1:, :-1, :-1] + z[1:, :-1, 1:] + z[1:, 1:, :-1] + z[1:, 1:, 1:])
+    zc[-1] = zc[-2]; zc[:, -1] = zc[:, -2]; zc[:, :, -1] = zc[:, :, -2]
+    zs = 0.25 * (z[ng, :-1, :-1] + z[ng, :-1, 1:] + z[ng, 1:, :-1] + z[ng, 1:, 1:])
+    zs = np.pad(zs, ((0, 1), (0, 1)), mode="edge")
+    kk = slice(ng - 1, ng + nz + 1)
+    g = C.CONST_GRAV
+    rdOcp = C.R_d / cfg.c_p
+    # surface pressures of the continuous profiles (isentropic; constant n2)
+    ps_isen = C.p_0 * np.power(1.0 - g * zs / (cfg.c_p * 300.0), 1.0 / rdOcp)
+    rho0, p0, pi0 = hse_terrain(zc[kk], zs, ps_isen, np.full_like(zc[kk], 300.0), g, rdOcp)
+    for fid, a3 in ((C.F_R0, rho0), (C.F_P0, p0), (C.F_PI0, pi0)):
+        f[fid][0][kk] = a3
+    ix, jy, kz = _cell_centres(cfg, L)
+    theta = np.full_like(zc, 300.0)
+    pr = f[C.F_P0][0]
+    if n2 > 0:   # theta = 300 exp(n2 z / g) in discrete balance of its own
+        theta = 300.0 * np.exp(n2 * zc / g)
+        pr = np.zeros_like(zc)
+        pis = 1.0 + g * g / (cfg.c_p * 300.0 * n2) * (np.exp(-n2 * zs / g) - 1.0)
+        pr[kk] = hse_terrain(zc[kk], zs, C.p_0 * np.power(pis, 1.0 / rdOcp), theta[kk], g, rdOcp)[1]
+    if bubble is not None:
+        Lr = np.sqrt(((ix[None, None, :] - bubble[0] * lx) / 2000.0) ** 2 +
+                     ((jy[None, :, None] - bubble[1] * ly) / 2000.0) ** 2 + ((zc - 3000.0) / 1500.0) ** 2)
+        theta = theta + np.where(Lr <= 1.0, (np.cos(math.pi * Lr) + 1.0) / 2.0, 0.0)
+    sl = (slice(ng, ng + nz),)
+    rt = _rt_of_p(np.maximum(pr, 1.0))
+    cons = f[C.F_CONS]
+    if moist:
+        T = theta * np.power(np.maximum(f[C.F_P0][0], 1.0) / C.p_0, rdOcp)
+        es = _esat_bolton(T)
+        pmb = np.maximum(f[C.F_P0][0], 1.0) / 100.0
+        rh = np.where(zc <= 2000.0, 0.9, 0.9 * np.exp(-(zc - 2000.0) / 1000.0))
+        qv = rh * (C.R_d / C.R_v) * es / (pmb - es)
+        rt = rt / (1.0 + C.R_v / C.R_d * qv)
+    cons[C.RHOTHETA][sl] = rt[sl]
+    cons[C.RHO][sl] = rt[sl] / theta[sl]
+    if moist:
+        cons[C.RHOQ1][sl] = (cons[C.RHO] * qv)[sl]
+    rng = np.random.default_rng(seed)
+    _noise_vel(cfg, L, f, rng, noise, (u0, v0, 0.0))
     return cfg, f
 
 

@@ -148,6 +148,12 @@ typedef struct erfx_config {
     int    buoyancy_type;
     int    moistscal_horiz_adv, moistscal_vert_adv;          /* WENO3 in ERF */
     double moistscal_horiz_upw_frac, moistscal_vert_upw_frac;
+
+    /* terrain-following grid (use_terrain, static terrain; ERF_DataStruct.H):
+     * the node heights come from erfx_set_terrain, the step uses
+     * erf_fast_rhs_T's acoustic substep and the metric terms of the slow RHS.
+     * dz is the computational spacing (detJ = height of a cell / dz). */
+    int    use_terrain;
 } erfx_config;
 
 /* ---- layout shared by every field of a context ------------------------- */
@@ -314,7 +320,9 @@ typedef struct erfx_sim erfx_sim;
 enum {
     ERFX_F_CONS = 0, ERFX_F_XVEL = 1, ERFX_F_YVEL = 2, ERFX_F_ZVEL = 3,
     ERFX_F_R0 = 4, ERFX_F_P0 = 5, ERFX_F_PI0 = 6,
-    ERFX_F_NUM = 7
+    ERFX_F_Z_ND = 7,   /* node heights z_phys_nd (use_terrain only); uploading them
+                          recomputes detJ, ax, ay, az and z_cc */
+    ERFX_F_NUM = 8
 };
 
 /* Diagnostic ids: the integrator's internal buffers, for hook-level testing.
@@ -326,6 +334,9 @@ enum {
     ERFX_D_XVEL_OLD = 20, ERFX_D_YVEL_OLD = 21, ERFX_D_ZVEL_OLD = 22,
     ERFX_D_XMOM_SRC = 23, ERFX_D_YMOM_SRC = 24, ERFX_D_ZMOM_SRC = 25,
     ERFX_D_RAIN_ACCUM = 26,       /* Kessler surface rain [mm], one k = 0 plane (sz doubles) */
+    /* terrain metrics (use_terrain): make_J, make_areas, make_zcc
+       (ERF_TerrainMetrics.cpp:583-715) */
+    ERFX_D_DETJ = 27, ERFX_D_AX = 28, ERFX_D_AY = 29, ERFX_D_AZ = 30, ERFX_D_Z_CC = 31,
     ERFX_D_STATE_BASE = 32
 };
 

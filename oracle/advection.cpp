@@ -165,7 +165,8 @@ static inline double interp_z(int t, Arr q, int i, int j, int k, int n, double u
 }
 
 // AdvectionSrcForScalars (Source/Advection/ERF_AdvectionSrcForState.cpp:118-395,
-// ERF_AdvectionSrcForScalars.H:7-110) with detJ = 1 and unit map factors.
+// ERF_AdvectionSrcForScalars.H:7-110), unit map factors; 1/detJ on a
+// terrain-following grid (:298, :361).
 void Level::advection_src_scalars(const Box& bx, int icomp, int ncomp, Arr ax, Arr ay, Arr az,
                                   Arr cprim, Arr rhs, int hadv, int vadv, double hfrac, double vfrac) {
     const int nx = bx.hi[0], ny = bx.hi[1], nz = bx.hi[2];
@@ -204,8 +205,9 @@ void Level::advection_src_scalars(const Box& bx, int icomp, int ncomp, Arr ax, A
             });
         }
         const double dxi = dxInv[0], dyi = dxInv[1], dzi = dxInv[2];
+        Arr Jc = A(detJ);
         loop(bx, [&](int i, int j, int k) {
-            double invdetJ = 1.;
+            double invdetJ = terrain ? ((Jc(i, j, k) > 0.) ? 1. / Jc(i, j, k) : 1.) : 1.;
             double mfsq = 1.0 * 1.0;
             rhs(i, j, k, ci) = -invdetJ * mfsq * ((Fx(i + 1, j, k) - Fx(i, j, k)) * dxi +
                                                    (Fy(i, j + 1, k) - Fy(i, j, k)) * dyi +
@@ -385,6 +387,196 @@ void Level::advection_src_mom(const Box& tbx, const Box& tby, const Box& tbz, Ar
         double mfsq = 1.0 * 1.0;
         double src = (xflux_hi - xflux_lo) * dxInv_ * mfsq + (yflux_hi - yflux_lo) * dyInv_ * mfsq
                    + (zflux_hi - zflux_lo) * dzInv_;
+        rw_rhs(i, j, k) = -src;
+    });
+}
+
+}  // namespace orc
+
+namespace orc {
+
+// AdvectionSrcForMom with terrain metrics (ERF_AdvectionSrcForMom.cpp:204-288
+// for C2/C2, ERF_AdvectionSrcForMom_T.H:22-330 otherwise), static terrain,
+// unit map factors: the flat fluxes weighted by the face areas (ax, ay, az),
+// the edge heights h_zeta, and divided by detJ averaged to the face.
+void Level::advection_src_mom_T(const Box& tbx, const Box& tby, const Box& tbz, Arr ru_rhs, Arr rv_rhs,
+                                Arr rw_rhs, Arr u, Arr v, Arr w, Arr rho_u, Arr rho_v, Arr Omega) {
+    const double dxInv_ = dxInv[0], dyInv_ = dxInv[1], dzInv_ = dxInv[2];
+    const int hz = L.nz, lz = 0;
+    const int ht = cfg.dycore_horiz_adv, vt = cfg.dycore_vert_adv;
+    const double hf = cfg.dycore_horiz_upw_frac, vf = cfg.dycore_vert_upw_frac;
+    Arr z = A(z_nd), Ax = A(ax), Ay = A(ay), Az = A(az), J = A(detJ);
+    const double dzi = dzInv_;
+    if (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2) {
+        loop(tbx, [&](int i, int j, int k) {
+            double xflux_hi = 0.25 * (rho_u(i, j, k) * 1.0 + rho_u(i + 1, j, k) * 1.0) * (u(i + 1, j, k) + u(i, j, k)) *
+                              0.5 * (Ax(i, j, k) + Ax(i + 1, j, k));
+            double xflux_lo = 0.25 * (rho_u(i, j, k) * 1.0 + rho_u(i - 1, j, k) * 1.0) * (u(i - 1, j, k) + u(i, j, k)) *
+                              0.5 * (Ax(i, j, k) + Ax(i - 1, j, k));
+            double met_h_zeta_yhi = met::h_zeta_EdgeK(z, i, j + 1, k, dzi);
+            double yflux_hi = 0.25 * (rho_v(i, j + 1, k) * 1.0 + rho_v(i - 1, j + 1, k) * 1.0) *
+                              (u(i, j + 1, k) + u(i, j, k)) * met_h_zeta_yhi;
+            double met_h_zeta_ylo = met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double yflux_lo = 0.25 * (rho_v(i, j, k) * 1.0 + rho_v(i - 1, j, k) * 1.0) *
+                              (u(i, j - 1, k) + u(i, j, k)) * met_h_zeta_ylo;
+            double zflux_hi = 0.25 * (Omega(i, j, k + 1) + Omega(i - 1, j, k + 1)) * (u(i, j, k + 1) + u(i, j, k)) *
+                              0.5 * (Az(i, j, k + 1) + Az(i - 1, j, k + 1));
+            double zflux_lo = 0.25 * (Omega(i, j, k) + Omega(i - 1, j, k)) * (u(i, j, k - 1) + u(i, j, k)) *
+                              0.5 * (Az(i, j, k) + Az(i - 1, j, k));
+            double mfsq = 1.0 * 1.0;
+            double src = (xflux_hi - xflux_lo) * dxInv_ * mfsq + (yflux_hi - yflux_lo) * dyInv_ * mfsq
+                       + (zflux_hi - zflux_lo) * dzInv_;
+            ru_rhs(i, j, k) = -src / (0.5 * (J(i, j, k) + J(i - 1, j, k)));
+        });
+        loop(tby, [&](int i, int j, int k) {
+            double met_h_zeta_xhi = met::h_zeta_EdgeK(z, i + 1, j, k, dzi);
+            double xflux_hi = 0.25 * (rho_u(i + 1, j, k) * 1.0 + rho_u(i + 1, j - 1, k) * 1.0) *
+                              (v(i + 1, j, k) + v(i, j, k)) * met_h_zeta_xhi;
+            double met_h_zeta_xlo = met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double xflux_lo = 0.25 * (rho_u(i, j, k) * 1.0 + rho_u(i, j - 1, k) * 1.0) *
+                              (v(i - 1, j, k) + v(i, j, k)) * met_h_zeta_xlo;
+            double yflux_hi = 0.25 * (rho_v(i, j + 1, k) * 1.0 + rho_v(i, j, k) * 1.0) * (v(i, j + 1, k) + v(i, j, k)) *
+                              0.5 * (Ay(i, j, k) + Ay(i, j + 1, k));
+            double yflux_lo = 0.25 * (rho_v(i, j, k) * 1.0 + rho_v(i, j - 1, k) * 1.0) * (v(i, j - 1, k) + v(i, j, k)) *
+                              0.5 * (Ay(i, j, k) + Ay(i, j - 1, k));
+            double zflux_hi = 0.25 * (Omega(i, j, k + 1) + Omega(i, j - 1, k + 1)) * (v(i, j, k + 1) + v(i, j, k)) *
+                              0.5 * (Az(i, j, k + 1) + Az(i, j - 1, k + 1));
+            double zflux_lo = 0.25 * (Omega(i, j, k) + Omega(i, j - 1, k)) * (v(i, j, k - 1) + v(i, j, k)) *
+                              0.5 * (Az(i, j, k) + Az(i, j - 1, k));
+            double mfsq = 1.0 * 1.0;
+            double src = (xflux_hi - xflux_lo) * dxInv_ * mfsq + (yflux_hi - yflux_lo) * dyInv_ * mfsq
+                       + (zflux_hi - zflux_lo) * dzInv_;
+            rv_rhs(i, j, k) = -src / (0.5 * (J(i, j, k) + J(i, j - 1, k)));
+        });
+        loop(tbz, [&](int i, int j, int k) {
+            double met_h_zeta_xhi = met::h_zeta_EdgeJ(z, i + 1, j, k, dzi);
+            double xflux_hi = 0.25 * (rho_u(i + 1, j, k) + rho_u(i + 1, j, k - 1)) * 1.0 *
+                              (w(i + 1, j, k) + w(i, j, k)) * met_h_zeta_xhi;
+            double met_h_zeta_xlo = met::h_zeta_EdgeJ(z, i, j, k, dzi);
+            double xflux_lo = 0.25 * (rho_u(i, j, k) + rho_u(i, j, k - 1)) * 1.0 *
+                              (w(i - 1, j, k) + w(i, j, k)) * met_h_zeta_xlo;
+            double met_h_zeta_yhi = met::h_zeta_EdgeI(z, i, j + 1, k, dzi);
+            double yflux_hi = 0.25 * (rho_v(i, j + 1, k) + rho_v(i, j + 1, k - 1)) * 1.0 *
+                              (w(i, j + 1, k) + w(i, j, k)) * met_h_zeta_yhi;
+            double met_h_zeta_ylo = met::h_zeta_EdgeI(z, i, j, k, dzi);
+            double yflux_lo = 0.25 * (rho_v(i, j, k) + rho_v(i, j, k - 1)) * 1.0 *
+                              (w(i, j - 1, k) + w(i, j, k)) * met_h_zeta_ylo;
+            double zflux_lo = 0.25 * (Omega(i, j, k) + Omega(i, j, k - 1)) * (w(i, j, k) + w(i, j, k - 1));
+            double zflux_hi = (k == hz) ? Omega(i, j, k) * w(i, j, k) * Az(i, j, k)
+                                        : 0.25 * (Omega(i, j, k) + Omega(i, j, k + 1)) * (w(i, j, k) + w(i, j, k + 1)) *
+                                              0.5 * (Az(i, j, k) + Az(i, j, k + 1));
+            double mfsq = 1.0 * 1.0;
+            double src = (xflux_hi - xflux_lo) * dxInv_ * mfsq + (yflux_hi - yflux_lo) * dyInv_ * mfsq
+                       + (zflux_hi - zflux_lo) * dzInv_;
+            rw_rhs(i, j, k) = -src / (0.5 * (J(i, j, k) + J(i, j, k - 1)));
+        });
+        return;
+    }
+    // higher order: AdvectionSrcForXMom / YMom / ZMom with metrics
+    loop(tbx, [&](int i, int j, int k) {
+        double rho_u_avg_hi = 0.5 * (rho_u(i + 1, j, k) * 1.0 + rho_u(i, j, k) * 1.0);
+        double interp_hi = interp_x(ht, u, i + 1, j, k, 0, rho_u_avg_hi, hf);
+        double rho_u_avg_lo = 0.5 * (rho_u(i - 1, j, k) * 1.0 + rho_u(i, j, k) * 1.0);
+        double interp_lo = interp_x(ht, u, i, j, k, 0, rho_u_avg_lo, hf);
+        double centFluxXXNext = rho_u_avg_hi * interp_hi * 0.5 * (Ax(i, j, k) + Ax(i + 1, j, k));
+        double centFluxXXPrev = rho_u_avg_lo * interp_lo * 0.5 * (Ax(i, j, k) + Ax(i - 1, j, k));
+        double rho_v_avg_hi = 0.5 * (rho_v(i, j + 1, k) * 1.0 + rho_v(i - 1, j + 1, k) * 1.0);
+        interp_hi = interp_y(ht, u, i, j + 1, k, 0, rho_v_avg_hi, hf);
+        double rho_v_avg_lo = 0.5 * (rho_v(i, j, k) * 1.0 + rho_v(i - 1, j, k) * 1.0);
+        interp_lo = interp_y(ht, u, i, j, k, 0, rho_v_avg_lo, hf);
+        double edgeFluxXYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i, j + 1, k, dzi);
+        double edgeFluxXYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
+        double Omega_avg_hi = 0.5 * (Omega(i, j, k + 1) + Omega(i - 1, j, k + 1)) * 0.5 * (Az(i, j, k + 1) + Az(i - 1, j, k + 1));
+        double Omega_avg_lo = 0.5 * (Omega(i, j, k) + Omega(i - 1, j, k)) * 0.5 * (Az(i, j, k) + Az(i - 1, j, k));
+        interp_hi = interp_z(vt, u, i, j, k + 1, 0, Omega_avg_hi, vf);
+        interp_lo = interp_z(vt, u, i, j, k, 0, Omega_avg_lo, vf);
+        double edgeFluxXZNext = Omega_avg_hi * interp_hi;
+        double edgeFluxXZPrev = Omega_avg_lo * interp_lo;
+        double mfsq = 1 / (1.0 * 1.0);
+        double src = (centFluxXXNext - centFluxXXPrev) * dxInv_ * mfsq + (edgeFluxXYNext - edgeFluxXYPrev) * dyInv_ * mfsq
+                   + (edgeFluxXZNext - edgeFluxXZPrev) * dzInv_;
+        src /= 0.5 * (J(i, j, k) + J(i - 1, j, k));
+        ru_rhs(i, j, k) = -src;
+    });
+    loop(tby, [&](int i, int j, int k) {
+        double rho_u_avg_hi = 0.5 * (rho_u(i + 1, j, k) * 1.0 + rho_u(i + 1, j - 1, k) * 1.0);
+        double rho_u_avg_lo = 0.5 * (rho_u(i, j, k) * 1.0 + rho_u(i, j - 1, k) * 1.0);
+        double interp_hi = interp_x(ht, v, i + 1, j, k, 0, rho_u_avg_hi, hf);
+        double interp_lo = interp_x(ht, v, i, j, k, 0, rho_u_avg_lo, hf);
+        double edgeFluxYXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i + 1, j, k, dzi);
+        double edgeFluxYXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
+        double rho_v_avg_hi = 0.5 * (rho_v(i, j, k) * 1.0 + rho_v(i, j + 1, k) * 1.0);
+        double rho_v_avg_lo = 0.5 * (rho_v(i, j, k) * 1.0 + rho_v(i, j - 1, k) * 1.0);
+        interp_hi = interp_y(ht, v, i, j + 1, k, 0, rho_v_avg_hi, hf);
+        interp_lo = interp_y(ht, v, i, j, k, 0, rho_v_avg_lo, hf);
+        double centFluxYYNext = rho_v_avg_hi * 0.5 * (Ay(i, j, k) + Ay(i, j + 1, k)) * interp_hi;
+        double centFluxYYPrev = rho_v_avg_lo * 0.5 * (Ay(i, j, k) + Ay(i, j - 1, k)) * interp_lo;
+        double Omega_avg_hi = 0.5 * (Omega(i, j, k + 1) + Omega(i, j - 1, k + 1)) * 0.5 * (Az(i, j, k + 1) + Az(i, j - 1, k + 1));
+        double Omega_avg_lo = 0.5 * (Omega(i, j, k) + Omega(i, j - 1, k)) * 0.5 * (Az(i, j, k) + Az(i, j - 1, k));
+        interp_hi = interp_z(vt, v, i, j, k + 1, 0, Omega_avg_hi, vf);
+        interp_lo = interp_z(vt, v, i, j, k, 0, Omega_avg_lo, vf);
+        double edgeFluxYZNext = Omega_avg_hi * interp_hi;
+        double edgeFluxYZPrev = Omega_avg_lo * interp_lo;
+        double mfsq = 1 / (1.0 * 1.0);
+        double src = (edgeFluxYXNext - edgeFluxYXPrev) * dxInv_ * mfsq + (centFluxYYNext - centFluxYYPrev) * dyInv_ * mfsq
+                   + (edgeFluxYZNext - edgeFluxYZPrev) * dzInv_;
+        src /= 0.5 * (J(i, j, k) + J(i, j - 1, k));
+        rv_rhs(i, j, k) = -src;
+    });
+    loop(tbz, [&](int i, int j, int k) {
+        double rho_u_avg_hi = 0.5 * (rho_u(i + 1, j, k) + rho_u(i + 1, j, k - 1)) * 1.0;
+        double rho_u_avg_lo = 0.5 * (rho_u(i, j, k) + rho_u(i, j, k - 1)) * 1.0;
+        double interp_hi = interp_x(ht, w, i + 1, j, k, 0, rho_u_avg_hi, hf);
+        double interp_lo = interp_x(ht, w, i, j, k, 0, rho_u_avg_lo, hf);
+        double edgeFluxZXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeJ(z, i + 1, j, k, dzi);
+        double edgeFluxZXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeJ(z, i, j, k, dzi);
+        double rho_v_avg_hi = 0.5 * (rho_v(i, j + 1, k) + rho_v(i, j + 1, k - 1)) * 1.0;
+        interp_hi = interp_y(ht, w, i, j + 1, k, 0, rho_v_avg_hi, hf);
+        double rho_v_avg_lo = 0.5 * (rho_v(i, j, k) + rho_v(i, j, k - 1)) * 1.0;
+        interp_lo = interp_y(ht, w, i, j, k, 0, rho_v_avg_lo, hf);
+        double edgeFluxZYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeI(z, i, j + 1, k, dzi);
+        double edgeFluxZYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeI(z, i, j, k, dzi);
+        double Omega_avg_hi = (k == hz) ? Omega(i, j, k) * Az(i, j, k)
+                                        : 0.25 * (Omega(i, j, k) + Omega(i, j, k + 1)) * (Az(i, j, k) + Az(i, j, k + 1));
+        double centFluxZZNext = Omega_avg_hi;
+        if (k == hz) {
+            centFluxZZNext *= w(i, j, k);
+        } else {
+            if (k == hz - 1) {
+                interp_hi = interp_z_all(w, i, j, k + 1, Omega_avg_hi, 0, ERFX_ADV_CENTERED2);
+            } else if (k == hz - 2 || k == lz + 1) {
+                if (vt != ERFX_ADV_CENTERED2 && vt != ERFX_ADV_UPWIND3)
+                    interp_hi = interp_z_all(w, i, j, k + 1, Omega_avg_hi, 0, ERFX_ADV_CENTERED4);
+                else
+                    interp_hi = interp_z_all(w, i, j, k + 1, Omega_avg_hi, vf, vt);
+            } else {
+                interp_hi = interp_z(vt, w, i, j, k + 1, 0, Omega_avg_hi, vf);
+            }
+            centFluxZZNext *= interp_hi;
+        }
+        double Omega_avg_lo = (k == 0) ? Omega(i, j, k) * Az(i, j, k)
+                                       : 0.25 * (Omega(i, j, k) + Omega(i, j, k - 1)) * (Az(i, j, k) + Az(i, j, k - 1));
+        double centFluxZZPrev = Omega_avg_lo;
+        if (k == 0) {
+            centFluxZZPrev *= w(i, j, k);
+        } else {
+            if (k == lz + 1) {
+                interp_lo = interp_z_all(w, i, j, k, Omega_avg_lo, 0, ERFX_ADV_CENTERED2);
+            } else if (k == lz + 2 || k == hz - 1) {
+                if (vt != ERFX_ADV_CENTERED2 && vt != ERFX_ADV_UPWIND3)
+                    interp_lo = interp_z_all(w, i, j, k, Omega_avg_lo, 0, ERFX_ADV_CENTERED4);
+                else
+                    interp_lo = interp_z_all(w, i, j, k, Omega_avg_lo, vf, vt);
+            } else {
+                interp_lo = interp_z(vt, w, i, j, k, 0, Omega_avg_lo, vf);
+            }
+            centFluxZZPrev *= interp_lo;
+        }
+        double mfsq = 1.0 * 1.0;
+        double src = (edgeFluxZXNext - edgeFluxZXPrev) * dxInv_ * mfsq + (edgeFluxZYNext - edgeFluxZYPrev) * dyInv_ * mfsq
+                   + (centFluxZZNext - centFluxZZPrev) * dzInv_;
+        double denom = 0.5 * (J(i, j, k) + J(i, j, k - 1));
+        src /= denom;
         rw_rhs(i, j, k) = -src;
     });
 }

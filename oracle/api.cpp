@@ -30,6 +30,15 @@ Field* field_by_id(Level& L, int id) {
         case ERFX_D_ZMOM_SRC: return &L.zmom_src;
         default: break;
     }
+    if (L.terrain) switch (id) {
+        case ERFX_F_Z_ND: return &L.z_nd;
+        case ERFX_D_DETJ: return &L.detJ;
+        case ERFX_D_AX: return &L.ax;
+        case ERFX_D_AY: return &L.ay;
+        case ERFX_D_AZ: return &L.az;
+        case ERFX_D_Z_CC: return &L.z_cc;
+        default: break;
+    }
     if (id >= ERFX_D_STATE_BASE && id < ERFX_D_STATE_BASE + 20) {
         int s = (id - ERFX_D_STATE_BASE) / 4, c = (id - ERFX_D_STATE_BASE) % 4;
         StateF* st[5] = {&L.S_old, &L.S_new, &L.S_sum, &L.S_scratch, &L.F_slow};
@@ -152,6 +161,7 @@ int orc_set(void* h, int id, const double* src) {
     if (id == ERFX_D_RAIN_ACCUM) { std::memcpy(L.rain_accum.data(), src, L.rain_accum.size() * sizeof(double)); return 0; }
     Field* f = field_by_id(L, id);
     if (!f) { g_err = "bad field id"; return -1; }
+    if (id == ERFX_F_Z_ND) return guard([&] { L.set_terrain(src); });
     std::memcpy(f->v.data(), src, f->v.size() * sizeof(double));
     return 0;
 }

@@ -13,6 +13,8 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     moist = c.moisture_type != ERFX_MOIST_NONE;
     n_qstate = moist ? 3 : 0;
     if (ncons != (moist ? ERFX_NCONS_MOIST : ERFX_NCONS_DRY)) throw std::runtime_error("ncons does not match moisture_type");
+    if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE))
+        throw std::runtime_error("terrain with diffusion is not restated");
     ng_cons = lay.ng;
     ng_vel = lay.ng - 1;
     rdOcp = R_d / c.c_p;
@@ -55,6 +57,15 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     tau12.alloc(L, 1); tau13.alloc(L, 1); tau23.alloc(L, 1);
     xmom_src.alloc(L, 1); ymom_src.alloc(L, 1); zmom_src.alloc(L, 1); cc_src.alloc(L, ncons);
     rain_accum.assign((size_t)L.sz, 0.0);
+    if (c.use_terrain) {
+        // flat node heights until set_terrain is given the real ones
+        std::vector<double> zf((size_t)L.nelem);
+        for (long q = 0; q < L.nelem; ++q) {
+            long kk = q / L.sz - L.ng;
+            zf[(size_t)q] = c.prob_lo[2] + (double)kk * c.dz;
+        }
+        set_terrain(zf.data());
+    }
 }
 
 // FabArray::FillBoundary on a single periodic box: every ghost point whose
@@ -289,7 +300,7 @@ void Level::fill_yvel(Arr v, int ng) {
 // ERF_BoundaryConditions_zvel.cpp:16-268 (flat grid).  The box is clipped to
 // the domain in z, so z-ghost faces are never filled; w at k = 0 always takes
 // the zlo Dirichlet value.
-void Level::fill_zvel(Arr w, int ng) {
+void Level::fill_zvel(Arr w, int ng, Arr u, Arr v) {
     fill_boundary_periodic(w, 1, ng_vel, 2);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     if (!periodic[0]) {
@@ -333,9 +344,16 @@ void Level::fill_zvel(Arr w, int ng) {
     {
         double vl = vel_extdir[2][2], vh = vel_extdir[2][5];
         Box lo(-ng, -ng, 0, nx - 1 + ng, ny - 1 + ng, 0), hi(-ng, -ng, nz, nx - 1 + ng, ny - 1 + ng, nz);
-        loop(lo, [&](int i, int j, int k) { w(i, j, k) = vl; });
+        // with static terrain the wall value is Omega's and w follows the
+        // slope (ERF_BoundaryConditions_zvel.cpp:220-257)
+        Arr zn = terrain ? A(z_nd) : Arr();
+        if (terrain) loop(lo, [&](int i, int j, int k) { w(i, j, k) = met::w_from_omega(i, j, k, vl, u, v, zn, dxInv); });
+        else loop(lo, [&](int i, int j, int k) { w(i, j, k) = vl; });
         int th = bck[2][5];
-        if (th == EXT_DIR) loop(hi, [&](int i, int j, int k) { w(i, j, k) = vh; });
+        if (th == EXT_DIR) {
+            if (terrain) loop(hi, [&](int i, int j, int k) { w(i, j, k) = met::w_from_omega(i, j, k, vh, u, v, zn, dxInv); });
+            else loop(hi, [&](int i, int j, int k) { w(i, j, k) = vh; });
+        }
         else if (th == NEUMANN_INT)
             loop(hi, [&](int i, int j, int k) { w(i, j, k) = (4.0 * w(i, j, nz - 1) - w(i, j, nz - 2)) / 3.0; });
     }
@@ -382,7 +400,7 @@ void Level::fill_intermediate_patch(StateA S, Arr xv, Arr yv, Arr zv, int ngc, i
     if (!cons_only) {
         fill_xvel(xv, ngv);
         fill_yvel(yv, ngv);
-        fill_zvel(zv, ngv);
+        fill_zvel(zv, ngv, xv, yv);
         int g1[3] = {1, 1, 1}, g0[3] = {1, 1, 0};
         velocity_to_momentum(xv, g1, yv, g1, zv, g0, S.cons, S.xmom, S.ymom, S.zmom);
     }
@@ -418,7 +436,7 @@ void Level::fill_patch(Arr c, Arr u, Arr v, Arr w) {
     fill_cons(c, 0, ncons, ng_cons);
     fill_xvel(u, ng_vel);
     fill_yvel(v, ng_vel);
-    fill_zvel(w, ng_vel);
+    fill_zvel(w, ng_vel, u, v);
 }
 
 // cons_to_prim (ERF_TI_utils.H:4-45)

@@ -16,15 +16,27 @@ void Level::make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double 
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     Arr sc = stage.cons;
     const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
+    Arr J = A(detJ);
     loop(Box(0, 0, 1, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
         double rhobar_lo = r0a(i, j, k - 1), rhobar_hi = r0a(i, j, k);
         double pibar_lo = pi0a(i, j, k - 1), pibar_hi = pi0a(i, j, k);
         double pi_c = 0.5 * (pi_st(i, j, k - 1) + pi_st(i, j, k));
-        double coeff_P = -Gamma * R_d * dzi
-                       + halfg * R_d * rhobar_hi / (c_v * pibar_hi * sc(i, j, k, 1));
+        double detJ_on_kface = 1.0;
+        double coeff_P, coeff_Q;
+        if (terrain) {   // :113-137
+            detJ_on_kface = 0.5 * (J(i, j, k) + J(i, j, k - 1));
+            double inv_detJ_on_kface = 1. / detJ_on_kface;
+            coeff_P = -Gamma * R_d * dzi * inv_detJ_on_kface
+                    + halfg * R_d * rhobar_hi / (c_v * pibar_hi * sc(i, j, k, 1));
+            coeff_Q = Gamma * R_d * dzi * inv_detJ_on_kface
+                    + halfg * R_d * rhobar_lo / (c_v * pibar_lo * sc(i, j, k - 1, 1));
+        } else {
+            coeff_P = -Gamma * R_d * dzi
+                    + halfg * R_d * rhobar_hi / (c_v * pibar_hi * sc(i, j, k, 1));
+            coeff_Q = Gamma * R_d * dzi
+                    + halfg * R_d * rhobar_lo / (c_v * pibar_lo * sc(i, j, k - 1, 1));
+        }
         coeff_P *= pi_c;
-        double coeff_Q = Gamma * R_d * dzi
-                       + halfg * R_d * rhobar_lo / (c_v * pibar_lo * sc(i, j, k - 1, 1));
         coeff_Q *= pi_c;
         fc(i, j, k, 3) = coeff_P;
         fc(i, j, k, 4) = coeff_Q;
@@ -39,7 +51,7 @@ void Level::make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double 
         double D = dtau * dtau * beta_2 * beta_2 * dzi;
         fc(i, j, k, 0) = D * (halfg - coeff_Q * theta_t_lo);
         fc(i, j, k, 2) = D * (-halfg + coeff_P * theta_t_hi);
-        fc(i, j, k, 1) = 1.0 + D * (coeff_Q - coeff_P) * theta_t_mid;
+        fc(i, j, k, 1) = (terrain ? detJ_on_kface : 1.0) + D * (coeff_Q - coeff_P) * theta_t_mid;
     });
     const bool top_outflow = (cfg.bc[5] == ERFX_BC_OUTFLOW);
     loop(Box(0, 0, 0, nx - 1, ny - 1, 0), [&](int i, int j, int) {

@@ -124,6 +124,7 @@ void Level::kessler_advance(Arr c, double dt) {
                 dq_clwater_to_rain = dt * (accrr * qcc + auto_r * (qcc - qcw0));
                 dq_clwater_to_rain = std::min(dq_clwater_to_rain, cw);
             }
+            if (terrain) dJinv = 1.0 / A(detJ)(i, j, k);
             double dq_sed = dt * dJinv * (1.0 / rh) * (fz(i, j, k + 1) - fz(i, j, k)) / dz;
             if (std::fabs(dq_sed) < 1e-14) dq_sed = 0.0;
             v += -dq_vapor_to_clwater + dq_clwater_to_vapor + dq_rain_to_vapor;

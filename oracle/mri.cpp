@@ -55,12 +55,13 @@ void Level::advance_dycore(double old_time, double dt) {
         slow_rhs_pre(nrk, slow_dt, Fs, Sn, A(prim), Sc, xv, yv, zv);
         double inv_fac = 1.0 / static_cast<double>(nsubsteps);
         for (int ks = 0; ks < nsubsteps; ++ks) {
-            if (ks == 0) {
-                make_fast_coeffs(A(fast_coeffs), Sn, A(prim), A(pi_stage), dtau, beta_s);
-                fast_rhs_N(ks, nrk, Fs, So, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
-            } else {
-                fast_rhs_N(ks, nrk, Fs, Ss, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
-            }
+            // ERF_TI_fast_rhs_fun.H:122-175: erf_fast_rhs_T on a static terrain grid
+            if (ks == 0) make_fast_coeffs(A(fast_coeffs), Sn, A(prim), A(pi_stage), dtau, beta_s);
+            StateA prev = (ks == 0) ? So : Ss;
+            if (terrain)
+                fast_rhs_T(ks, nrk, Fs, prev, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
+            else
+                fast_rhs_N(ks, nrk, Fs, prev, Sn, A(prim), A(pi_stage), A(fast_coeffs), Ss, Sc, dtau, beta_s, inv_fac);
             apply_bcs(Ss, xv, yv, zv, 1, 1, true, false);
         }
         slow_rhs_post(nrk, slow_dt, Fs, So, Sn, Ss, A(prim), Sc, xv, yv);

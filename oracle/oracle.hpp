@@ -135,6 +135,69 @@ template <class F> inline void loop_serial(const Box& b, F&& f) {
 }
 
 struct StateA { Arr cons, xmom, ymom, zmom; };
+
+// ---- terrain metrics (Source/Utils/ERF_TerrainMetrics.H:36-450) -------------
+// z is the node height field z_phys_nd; di = 1/dx etc.
+namespace met {
+inline double h_xi_Iface(Arr z, int i, int j, int k, double dxi) {       // :105-116
+    return 0.125 * dxi * (z(i + 1, j, k) + z(i + 1, j, k + 1) + z(i + 1, j + 1, k) + z(i + 1, j + 1, k + 1)
+                          - z(i - 1, j, k) - z(i - 1, j, k + 1) - z(i - 1, j + 1, k) - z(i - 1, j + 1, k + 1));
+}
+inline double h_zeta_Iface(Arr z, int i, int j, int k, double dzi) {     // :91-100
+    return 0.5 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
+}
+inline double h_eta_Jface(Arr z, int i, int j, int k, double dyi) {      // :159-170
+    return 0.125 * dyi * (z(i, j + 1, k) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k) + z(i + 1, j + 1, k + 1)
+                          - z(i, j - 1, k) - z(i, j - 1, k + 1) - z(i + 1, j - 1, k) - z(i + 1, j - 1, k + 1));
+}
+inline double h_zeta_Jface(Arr z, int i, int j, int k, double dzi) {     // :135-144
+    return 0.5 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
+}
+inline double h_zeta_Kface(Arr z, int i, int j, int k, double dzi) {     // :176-187
+    return 0.125 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1)
+                          - z(i, j, k - 1) - z(i + 1, j, k - 1) - z(i, j + 1, k - 1) - z(i + 1, j + 1, k - 1));
+}
+inline double h_zeta_EdgeK(Arr z, int i, int j, int k, double dzi) {     // :225-233
+    return dzi * (z(i, j, k + 1) - z(i, j, k));
+}
+inline double h_zeta_EdgeJ(Arr z, int i, int j, int k, double dzi) {     // :268-277
+    return 0.25 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k - 1) - z(i, j + 1, k - 1));
+}
+inline double h_zeta_EdgeI(Arr z, int i, int j, int k, double dzi) {     // :307-316
+    return 0.25 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k - 1) - z(i + 1, j, k - 1));
+}
+// slopes of the z-face (i, j, k) used by OmegaFromW / WFromOmega (:368-381)
+inline double zlo_xi(Arr z, int i, int j, int k, double dxi) {
+    return 0.5 * dxi * (z(i + 1, j + 1, k) + z(i + 1, j, k) - z(i, j + 1, k) - z(i, j, k));
+}
+inline double zlo_eta(Arr z, int i, int j, int k, double dyi) {
+    return 0.5 * dyi * (z(i + 1, j + 1, k) + z(i, j + 1, k) - z(i + 1, j, k) - z(i, j, k));
+}
+// u and v averaged to the z-face, extrapolated at k = 0 (:383-388)
+inline double u_at_zface(Arr u, int i, int j, int k) {
+    return (k == 0) ? 1.5 * (0.5 * (u(i, j, k) + u(i + 1, j, k))) - 0.5 * (0.5 * (u(i, j, k + 1) + u(i + 1, j, k + 1)))
+                    : 0.25 * (u(i, j, k - 1) + u(i + 1, j, k - 1) + u(i, j, k) + u(i + 1, j, k));
+}
+inline double v_at_zface(Arr v, int i, int j, int k) {
+    return (k == 0) ? 1.5 * (0.5 * (v(i, j, k) + v(i, j + 1, k))) - 0.5 * (0.5 * (v(i, j, k + 1) + v(i, j + 1, k + 1)))
+                    : 0.25 * (v(i, j, k - 1) + v(i, j + 1, k - 1) + v(i, j, k) + v(i, j + 1, k));
+}
+// OmegaFromW (:359-392) and WFromOmega (:397-445)
+inline double omega_from_w(int i, int j, int k, double w, Arr u, Arr v, Arr z, const double* di) {
+    double mx = zlo_xi(z, i, j, k, di[0]);
+    double my = zlo_eta(z, i, j, k, di[1]);
+    double uu = u_at_zface(u, i, j, k);
+    double vv = v_at_zface(v, i, j, k);
+    return w - mx * uu - my * vv;
+}
+inline double w_from_omega(int i, int j, int k, double omega, Arr u, Arr v, Arr z, const double* di) {
+    double uu = u_at_zface(u, i, j, k);
+    double vv = v_at_zface(v, i, j, k);
+    double mx = zlo_xi(z, i, j, k, di[0]);
+    double my = zlo_eta(z, i, j, k, di[1]);
+    return omega + mx * uu + my * vv;
+}
+}  // namespace met
 struct StateF {
     Field cons, xmom, ymom, zmom;
     void alloc(const erfx_layout& L, int nc) { cons.alloc(L, nc); xmom.alloc(L,1); ymom.alloc(L,1); zmom.alloc(L,1); }
@@ -153,6 +216,7 @@ struct Level {
     int ng_cons = 0, ng_vel = 0;
     int ncons = 5;
     bool moist = false;        // a Kessler model is on: cons comps 5..7 are RhoQ1..RhoQ3
+    bool terrain = false;      // terrain-following grid: z_phys_nd and its metrics below
     int n_qstate = 0;          // Kessler::Qstate_Size (ERF_Kessler.H:172)
     double rdOcp = 0, gravity = 0;
     double dxInv[3];
@@ -182,7 +246,7 @@ struct Level {
     void fill_cons(Arr c, int icomp, int ncomp, int ng);
     void fill_xvel(Arr u, int ng);
     void fill_yvel(Arr v, int ng);
-    void fill_zvel(Arr w, int ng);
+    void fill_zvel(Arr w, int ng, Arr u, Arr v);   // u, v: the filled velocities (terrain walls)
     void fill_boundary_periodic(Arr a, int ncomp, int ng, int nodal_dir);
     void momentum_to_velocity(Arr xv, Arr yv, Arr zv, Arr dens, Arr xm, Arr ym, Arr zm);
     void velocity_to_momentum(Arr xv, int ngu[3], Arr yv, int ngv[3], Arr zv, int ngw[3], Arr dens,
@@ -209,6 +273,13 @@ struct Level {
     // Kessler warm rain once per step (ERF_advance_microphysics.cpp:5-16)
     void kessler_advance(Arr c, double dt);
     std::vector<double> rain_accum;   // k = 0 plane of the layout (sz values)
+    // terrain (use_terrain): node heights and the metrics of make_J, make_areas
+    // and make_zcc (Source/Utils/ERF_TerrainMetrics.cpp:583-715); with a flat
+    // grid they hold detJ = ax = ay = az = 1
+    Field z_nd, detJ, ax, ay, az, z_cc;
+    void set_terrain(const double* z_nodes);
+    void fast_rhs_T(int step, int nrk, StateA slow_rhs, StateA prev, StateA stage, Arr prim_, Arr pi_st,
+                    Arr fc, StateA data, StateA scratch, double dtau, double beta_s, double facinv);
     void make_sources(StateA data, Arr prim_);
     void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
     void init_scalar_advdiff(const double* params);
@@ -227,6 +298,8 @@ struct Level {
                              Arr cell_prim, Arr rhs);
     void advection_src_mom(const Box& tbx, const Box& tby, const Box& tbz, Arr ru_rhs, Arr rv_rhs,
                            Arr rw_rhs, Arr u, Arr v, Arr w, Arr rho_u, Arr rho_v, Arr Omega);
+    void advection_src_mom_T(const Box& tbx, const Box& tby, const Box& tbz, Arr ru_rhs, Arr rv_rhs,
+                             Arr rw_rhs, Arr u, Arr v, Arr w, Arr rho_u, Arr rho_v, Arr Omega);
 
     void advance_dycore(double old_time, double dt);
     void advance(double time, double dt);   // ERF::Advance (lev 0, dycore only)

@@ -91,11 +91,11 @@ void Level::make_mom_sources(StateA data, Arr prim_) {
         throw std::runtime_error("Rayleigh damping profiles not set");
     {
         const double zlo = cfg.prob_lo[2], dz = cfg.dz;
-        const double ztop = zlo + nz * dz;          // ERF_init_rayleigh.cpp:17
+        const double ztop = terrain ? A(z_nd)(0, 0, nz) : zlo + nz * dz;   // ERF_init_rayleigh.cpp:17 (flat top)
         const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
         if (cfg.rayleigh_damp_U) {
             loop(tbx, [&](int i, int j, int k) {
-                double zcc = zlo + (k + 0.5) * dz;
+                double zcc = terrain ? A(z_cc)(i, j, k) : zlo + (k + 0.5) * dz;
                 double zfrac = 1 - (ztop - zcc) / zdamp;
                 if (zfrac > 0) {
                     double rho_on_u_face = 0.5 * (c(i, j, k) + c(i - 1, j, k));
@@ -107,7 +107,7 @@ void Level::make_mom_sources(StateA data, Arr prim_) {
         }
         if (cfg.rayleigh_damp_V) {
             loop(tby, [&](int i, int j, int k) {
-                double zcc = zlo + (k + 0.5) * dz;
+                double zcc = terrain ? A(z_cc)(i, j, k) : zlo + (k + 0.5) * dz;
                 double zfrac = 1 - (ztop - zcc) / zdamp;
                 if (zfrac > 0) {
                     double rho_on_v_face = 0.5 * (c(i, j, k) + c(i, j - 1, k));
@@ -119,7 +119,7 @@ void Level::make_mom_sources(StateA data, Arr prim_) {
         }
         if (cfg.rayleigh_damp_W) {
             loop(tbz, [&](int i, int j, int k) {
-                double zstag = zlo + k * dz;
+                double zstag = terrain ? A(z_nd)(i, j, k) : zlo + k * dz;
                 double zfrac = 1 - (ztop - zstag) / zdamp;
                 if (zfrac > 0) {
                     double rho_on_w_face = 0.5 * (c(i, j, k) + c(i, j, k - 1));
@@ -154,10 +154,10 @@ void Level::make_sources(StateA data, Arr prim_) {
     if ((int)ray_thetabar.size() < nz + 1) throw std::runtime_error("Rayleigh damping profiles not set");
     Arr cs = A(cc_src), c = data.cons;
     const double zlo = cfg.prob_lo[2], dz = cfg.dz;
-    const double ztop = zlo + nz * dz;
+    const double ztop = terrain ? A(z_nd)(0, 0, nz) : zlo + nz * dz;
     const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
     loop(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
-        double zcc = zlo + (k + 0.5) * dz;
+        double zcc = terrain ? A(z_cc)(i, j, k) : zlo + (k + 0.5) * dz;
         double zfrac = 1 - (ztop - zcc) / zdamp;
         if (zfrac > 0) {
             double theta = prim_(i, j, k, 0);
@@ -176,6 +176,7 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     (void)nrk;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const bool use_diff = cfg.molec_diff_type != ERFX_MOLEC_NONE || cfg.les_type != ERFX_LES_NONE;
+    if (use_diff && terrain) throw std::runtime_error("diffusion on a terrain-following grid is not restated");
     if (use_diff) make_tau_terms(data, u, v, w);
     Arr rwr = rhs.zmom;
     // S_rhs[zmom].setVal(0.0) (:232)
@@ -198,33 +199,51 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
         pp(i, j, k) = getPgivenRTh(c(i, j, k, 1), qv_for_p) - p0a(i, j, k);
     });
     if (bad_theta) throw std::runtime_error("Bad theta in slow_rhs_pre");
-    // Omega = rho*w on z-faces grown by one (:410-414)
+    // Omega on z-faces grown by one (:367-414): rho*w on a flat grid; with
+    // terrain 0 at and below the bottom face, rho*w above the top face and
+    // OmegaFromW in between (the top face included)
     Arr om = A(omega);
-    loop(Box(-1, -1, -1, nx, ny, nz + 1), [&](int i, int j, int k) { om(i, j, k) = data.zmom(i, j, k); });
+    if (!terrain) {
+        loop(Box(-1, -1, -1, nx, ny, nz + 1), [&](int i, int j, int k) { om(i, j, k) = data.zmom(i, j, k); });
+    } else {
+        Arr zn = A(z_nd);
+        loop(Box(-1, -1, -1, nx, ny, nz + 1), [&](int i, int j, int k) {
+            if (k <= 0) om(i, j, k) = 0.;
+            else if (k == nz + 1) om(i, j, k) = data.zmom(i, j, k);
+            else om(i, j, k) = met::omega_from_w(i, j, k, data.zmom(i, j, k), data.xmom, data.ymom, zn, dxInv);
+        });
+    }
+    Arr Ax = A(ax), Ay = A(ay), Az = A(az), Jc = A(detJ);
     // AdvectionSrcForRho
     Arr ax = scratch.xmom, ay = scratch.ymom, az = scratch.zmom;
     const double dxi = dxInv[0], dyi = dxInv[1], dzi = dxInv[2];
     Field ffx, ffy, ffz; ffx.alloc(L, 1); ffy.alloc(L, 1); ffz.alloc(L, 1);
     Arr fx = ffx.a(L), fy = ffy.a(L), fz = ffz.a(L);
+    // AdvectionSrcForRho (ERF_AdvectionSrcForState.cpp:29-93): area-weighted
+    // fluxes (areas and detJ are 1 on a flat grid)
     loop(Box(0, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) {
-        fx(i, j, k) = 1.0 * data.xmom(i, j, k) / 1.0;
+        fx(i, j, k) = (terrain ? Ax(i, j, k) : 1.0) * data.xmom(i, j, k) / 1.0;
         ax(i, j, k) = fx(i, j, k);
     });
     loop(Box(0, 0, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) {
-        fy(i, j, k) = 1.0 * data.ymom(i, j, k) / 1.0;
+        fy(i, j, k) = (terrain ? Ay(i, j, k) : 1.0) * data.ymom(i, j, k) / 1.0;
         ay(i, j, k) = fy(i, j, k);
     });
     loop(Box(0, 0, 0, nx - 1, ny - 1, nz), [&](int i, int j, int k) {
         double mfsq = 1.0 * 1.0;
-        fz(i, j, k) = 1.0 * om(i, j, k) / mfsq;
+        fz(i, j, k) = (terrain ? Az(i, j, k) : 1.0) * om(i, j, k) / mfsq;
         az(i, j, k) = fz(i, j, k);
     });
     Arr cr = rhs.cons;
     loop(bx, [&](int i, int j, int k) {
         double mfsq = 1.0 * 1.0;
-        cr(i, j, k, 0) = -mfsq / 1.0 * ((fx(i + 1, j, k) - fx(i, j, k)) * dxi +
-                                        (fy(i, j + 1, k) - fy(i, j, k)) * dyi +
-                                        (fz(i, j, k + 1) - fz(i, j, k)) * dzi);
+        const double dJ = terrain ? Jc(i, j, k) : 1.0;
+        if (dJ > 0.)
+            cr(i, j, k, 0) = -mfsq / dJ * ((fx(i + 1, j, k) - fx(i, j, k)) * dxi +
+                                          (fy(i, j + 1, k) - fy(i, j, k)) * dyi +
+                                          (fz(i, j, k + 1) - fz(i, j, k)) * dzi);
+        else
+            cr(i, j, k, 0) = 0.;
     });
     advection_src_scalars(bx, 1, 1, ax, ay, az, prim_, cr, cfg.dycore_horiz_adv, cfg.dycore_vert_adv,
                           cfg.dycore_horiz_upw_frac, cfg.dycore_vert_upw_frac);
@@ -237,7 +256,8 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     // momentum
     Box tbx(0, 0, 0, nx, ny - 1, nz - 1), tby(0, 0, 0, nx - 1, ny, nz - 1), tbz(0, 0, 1, nx - 1, ny - 1, nz - 1);
     Arr rur = rhs.xmom, rvr = rhs.ymom;
-    advection_src_mom(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
+    if (terrain) advection_src_mom_T(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
+    else advection_src_mom(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
     if (use_diff) {
         Arr t11 = A(tau11), t22 = A(tau22), t33 = A(tau33), t12 = A(tau12), t13 = A(tau13), t23 = A(tau23);
         loop(tbx, [&](int i, int j, int k) {
@@ -264,9 +284,22 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     // q = qv + qc averaged to the face (:622-626, :671-675, :732-736)
     const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
     Arr cp = prim_;
+    Arr zn = A(z_nd);
     loop(tbx, [&](int i, int j, int k) {
         double gp_xi = dxi * (pp(i, j, k) - pp(i - 1, j, k));
         double gpx = gp_xi;
+        if (terrain) {   // :600-620
+            double met_h_xi = met::h_xi_Iface(zn, i, j, k, dxi);
+            double met_h_zeta = met::h_zeta_Iface(zn, i, j, k, dzi);
+            double gp_zeta_on_iface;
+            if (k == 0)
+                gp_zeta_on_iface = 0.5 * dzi * (pp(i - 1, j, k + 1) + pp(i, j, k + 1) - pp(i - 1, j, k) - pp(i, j, k));
+            else if (k == nz - 1)
+                gp_zeta_on_iface = 0.5 * dzi * (pp(i - 1, j, k) + pp(i, j, k) - pp(i - 1, j, k - 1) - pp(i, j, k - 1));
+            else
+                gp_zeta_on_iface = 0.25 * dzi * (pp(i - 1, j, k + 1) + pp(i, j, k + 1) - pp(i - 1, j, k - 1) - pp(i, j, k - 1));
+            gpx -= (met_h_xi / met_h_zeta) * gp_zeta_on_iface;
+        }
         gpx *= 1.0;
         double q = 0.0;
         if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i - 1, j, k, q1) + cp(i, j, k, q2) + cp(i - 1, j, k, q2));
@@ -275,6 +308,18 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     loop(tby, [&](int i, int j, int k) {
         double gp_eta = dyi * (pp(i, j, k) - pp(i, j - 1, k));
         double gpy = gp_eta;
+        if (terrain) {   // :650-669
+            double met_h_eta = met::h_eta_Jface(zn, i, j, k, dyi);
+            double met_h_zeta = met::h_zeta_Jface(zn, i, j, k, dzi);
+            double gp_zeta_on_jface;
+            if (k == 0)
+                gp_zeta_on_jface = 0.5 * dzi * (pp(i, j, k + 1) + pp(i, j - 1, k + 1) - pp(i, j, k) - pp(i, j - 1, k));
+            else if (k == nz - 1)
+                gp_zeta_on_jface = 0.5 * dzi * (pp(i, j, k) + pp(i, j - 1, k) - pp(i, j, k - 1) - pp(i, j - 1, k - 1));
+            else
+                gp_zeta_on_jface = 0.25 * dzi * (pp(i, j, k + 1) + pp(i, j - 1, k + 1) - pp(i, j, k - 1) - pp(i, j - 1, k - 1));
+            gpy -= (met_h_eta / met_h_zeta) * gp_zeta_on_jface;
+        }
         gpy *= 1.0;
         double q = 0.0;
         if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i, j - 1, k, q1) + cp(i, j, k, q2) + cp(i, j - 1, k, q2));
@@ -286,7 +331,7 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     if (bck[1][1] == EXT_DIR) loop(Box(0, 0, 0, nx - 1, 0, nz - 1), [&](int i, int j, int k) { rvr(i, j, k) = 0.; });
     if (bck[1][4] == EXT_DIR) loop(Box(0, ny, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) { rvr(i, j, k) = 0.; });
     loop(tbz, [&](int i, int j, int k) {
-        double met_h_zeta = 1;
+        double met_h_zeta = terrain ? met::h_zeta_Kface(zn, i, j, k, dzi) : 1;
         double gpz = dzi * (pp(i, j, k) - pp(i, j, k - 1)) / met_h_zeta;
         double q = 0.0;
         if (moist) q = 0.5 * (cp(i, j, k, q1) + cp(i, j, k - 1, q1) + cp(i, j, k, q2) + cp(i, j, k - 1, q2));

@@ -15,11 +15,11 @@ def valid(a, cfg, L, fid, grow=0):
     """Slice of the valid region (faces included for staggered fields)."""
     ng = L.ng
     ex = ey = ez = 0
-    if fid in (C.F_XVEL, C.D_XVEL_OLD) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 1):
+    if fid in (C.F_XVEL, C.D_XVEL_OLD, C.F_Z_ND) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 1):
         ex = 1
-    if fid in (C.F_YVEL, C.D_YVEL_OLD) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 2):
+    if fid in (C.F_YVEL, C.D_YVEL_OLD, C.F_Z_ND) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 2):
         ey = 1
-    if fid in (C.F_ZVEL, C.D_ZVEL_OLD) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 3):
+    if fid in (C.F_ZVEL, C.D_ZVEL_OLD, C.F_Z_ND) or (fid >= C.D_STATE_BASE and (fid - C.D_STATE_BASE) % 4 == 3):
         ez = 1
     g = grow
     return a[:, ng - g:ng + cfg.nz + ez + g, ng - g:ng + cfg.ny + ey + g, ng - g:ng + cfg.nx + ex + g]
