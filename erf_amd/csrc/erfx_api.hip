@@ -53,6 +53,8 @@ void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
     P.use_diff = c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE;
     P.moist = c.moisture_type != ERFX_MOIST_NONE;
     P.n_qstate = P.moist ? 3 : 0;
+    P.terrain = c.use_terrain != 0;
+    P.dz_min = c.dz;
 }
 
 static int validate(const erfx_config& c) {
@@ -69,7 +71,12 @@ static int validate(const erfx_config& c) {
         set_error("only buoyancy_type 1 is implemented for the dry state"); return ERFX_ERR_UNSUPPORTED;
     }
     if (c.buoyancy_type < 1 || c.buoyancy_type > 4) { set_error("buoyancy_type must be 1, 2, 3 or 4"); return ERFX_ERR_ARG; }
-    if (c.use_terrain) { set_error("terrain-following grids are not implemented on the GPU"); return ERFX_ERR_UNSUPPORTED; }
+    if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE)) {
+        set_error("diffusion on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
+    }
+    if (c.use_terrain && (c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T)) {
+        set_error("Rayleigh damping on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
+    }
     if (c.moisture_type == ERFX_MOIST_KESSLER_NORAIN && c.buoyancy_type != 1) {
         // asserted by ERF (ERF_make_buoyancy.cpp:171-175)
         set_error("Kessler_NoRain requires buoyancy_type 1"); return ERFX_ERR_ARG;
@@ -122,6 +129,10 @@ struct erfx_ctx {
     double* ray = nullptr;    // Rayleigh profiles and per-level factors (RayleighTable layout)
     double* kfz = nullptr;    // Kessler sedimentation flux on z-faces (moist only)
     int* kbad = nullptr;      // device word: a Kessler temperature left the saturation fit's range
+    // terrain-following grid: node heights and metrics (the Params pointers
+    // refer to these) and the work fields of erf_fast_rhs_T
+    double *z_nd = nullptr, *detJ = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *z_cc = nullptr;
+    TerrainWork tw{};
     // learned microphysics work space (allocated on first use): per-cell slot,
     // network rows, their cells, two device counters and their pinned host copy
     int* eslot = nullptr;
@@ -229,6 +240,12 @@ int erfx_layout_for(const erfx_config* c, erfx_layout* out) {
     return ERFX_OK;
 }
 
+static int launch_status() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return ERFX_OK;
+}
+
 static int dalloc(std::vector<void*>& list, double** p, size_t n) {
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, n * sizeof(double));
@@ -271,7 +288,45 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
         }
         c->kbad = reinterpret_cast<int*>(kb);
     }
+    if (c->P.terrain) {
+        double** tb[] = {&c->z_nd, &c->detJ, &c->ax, &c->ay, &c->az, &c->z_cc, &c->sw.omega, &c->tw.du, &c->tw.dv,
+                         &c->tw.dw, &c->tw.drho, &c->tw.drt, &c->tw.nu, &c->tw.nv, &c->tw.om};
+        for (double** b : tb)
+            if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
+        c->P.z_nd = c->z_nd; c->P.detJ = c->detJ; c->P.ax = c->ax; c->P.ay = c->ay; c->P.az = c->az;
+        c->P.z_cc = c->z_cc;
+        // flat node heights until erfx_set_terrain gives the real ones
+        std::vector<double> zf(ne);
+        for (size_t q = 0; q < ne; ++q) zf[q] = cfg->prob_lo[2] + (double)((long)(q / lay.sz) - lay.ng) * cfg->dz;
+        hipError_t e = hipMemcpy(c->z_nd, zf.data(), ne * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { erfx_ctx_destroy(c); return hip_fail(e, "hipMemcpy"); }
+        terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->stream);
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { erfx_ctx_destroy(c); return hip_fail(e, "terrain metrics"); }
+    }
     *out = c;
+    return ERFX_OK;
+}
+
+int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
+    if (!c || !z_nd) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!c->P.terrain) { set_error("the context was made without use_terrain"); return ERFX_ERR_STATE; }
+    if (z_nd != c->z_nd)
+        ERFX_HIP(hipMemcpyAsync(c->z_nd, z_nd, (size_t)c->P.L.ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->stream);
+    int rc = launch_status();
+    if (rc) return rc;
+    // dz_min of the time-step estimate (ERF.cpp:898-906): dz times the smallest
+    // detJ of the valid cells
+    const Lay L = c->P.L;
+    std::vector<double> J((size_t)L.ne);
+    ERFX_HIP(hipMemcpyAsync(J.data(), c->detJ, (size_t)L.ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ERFX_HIP(hipStreamSynchronize(c->stream));
+    double jmin = 1.e300;
+    for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < L.ny; ++j)
+            for (int i = 0; i < L.nx; ++i) jmin = std::min(jmin, J[(size_t)L.ix(i, j, k)]);
+    c->P.dz_min = c->P.cfg.dz * jmin;
     return ERFX_OK;
 }
 
@@ -340,12 +395,6 @@ int erfx_set_rayleigh(erfx_ctx* c, const double* ub, const double* vb, const dou
     return ERFX_OK;
 }
 
-static int launch_status() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "kernel launch");
-    return ERFX_OK;
-}
-
 static bool rayleigh_on(const erfx_config& c) {
     return c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T;
 }
@@ -398,8 +447,23 @@ int erfx_fast_rhs_N(erfx_ctx* c, int step, int nrk, const erfx_state* slow_rhs, 
         set_error("null argument"); return ERFX_ERR_ARG;
     }
     if (!(nrk > 0 || step == 0)) { set_error("the first RK stage takes a single substep (ERF_fast_rhs_N.cpp:106)"); return ERFX_ERR_ARG; }
+    if (c->P.terrain) { set_error("a terrain-following grid takes erf_fast_rhs_T"); return ERFX_ERR_STATE; }
     fast_rhs_N(c->P, c->fw, step, nrk, *slow_rhs, *prev, *stage, prim, pi_stage, fc, *data, *scratch, dtau, beta_s,
                facinv, c->stream);
+    return launch_status();
+}
+
+int erfx_fast_rhs_T(erfx_ctx* c, int step, int nrk, const erfx_state* slow_rhs, const erfx_state* prev,
+                    const erfx_state* stage, const double* prim, const double* pi_stage, const double* fc,
+                    erfx_state* data, erfx_state* scratch, double dtau, double beta_s, double facinv) {
+    if (!c || bad_state(slow_rhs) || bad_state(prev) || bad_state(stage) || bad_state(data) || bad_state(scratch) ||
+        !prim || !pi_stage || !fc) {
+        set_error("null argument"); return ERFX_ERR_ARG;
+    }
+    if (!c->P.terrain) { set_error("erf_fast_rhs_T needs a terrain-following grid (use_terrain)"); return ERFX_ERR_STATE; }
+    if (!(nrk > 0 || step == 0)) { set_error("the first RK stage takes a single substep (ERF_fast_rhs_T.cpp:129)"); return ERFX_ERR_ARG; }
+    fast_rhs_T(c->P, c->fw, c->tw, step, nrk, *slow_rhs, *prev, *stage, prim, pi_stage, fc, *data, *scratch, dtau,
+               beta_s, facinv, c->stream);
     return launch_status();
 }
 
@@ -708,6 +772,12 @@ static double* field_ptr(erfx_sim* s, int id, long* count) {
         case ERFX_D_XVEL_OLD: p = s->vel[1 - s->cur][0]; break;
         case ERFX_D_YVEL_OLD: p = s->vel[1 - s->cur][1]; break;
         case ERFX_D_ZVEL_OLD: p = s->vel[1 - s->cur][2]; break;
+        case ERFX_F_Z_ND: p = s->ctx->z_nd; break;
+        case ERFX_D_DETJ: p = s->ctx->detJ; break;
+        case ERFX_D_AX: p = s->ctx->ax; break;
+        case ERFX_D_AY: p = s->ctx->ay; break;
+        case ERFX_D_AZ: p = s->ctx->az; break;
+        case ERFX_D_Z_CC: p = s->ctx->z_cc; break;
         default:
             if (id >= ERFX_D_STATE_BASE && id < ERFX_D_STATE_BASE + 20) {
                 int st = (id - ERFX_D_STATE_BASE) / 4, c = (id - ERFX_D_STATE_BASE) % 4;
@@ -736,6 +806,10 @@ int erfx_sim_upload(erfx_sim* s, int id, const double* host) {
     double* p = field_ptr(s, id, &n);
     if (!p) { set_error("unknown field id"); return ERFX_ERR_ARG; }
     ERFX_HIP(hipMemcpyAsync(p, host, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    if (id == ERFX_F_Z_ND) {   // new node heights: remake the metrics
+        int rc = erfx_set_terrain(s->ctx, p);
+        if (rc) return rc;
+    }
     ERFX_HIP(hipStreamSynchronize(s->stream));
     return ERFX_OK;
 }
@@ -816,8 +890,12 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
             ch.w_lo = P.vel_extdir[2][2];
             ch.w_hi = P.vel_extdir[2][5];
             if (ks == 0) make_fast_coeffs(P, s->fast_coeffs, Sn, s->prim, s->pi_stage, s->r0, s->pi0, dtau, beta_s, st);
-            fast_rhs_N(P, c->fw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc,
-                       dtau, beta_s, inv_fac, st, ch);
+            if (P.terrain)
+                fast_rhs_T(P, c->fw, c->tw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs,
+                           Ss, Sc, dtau, beta_s, inv_fac, st);
+            else
+                fast_rhs_N(P, c->fw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc,
+                           dtau, beta_s, inv_fac, st, ch);
             if (!chain || ks == nsub - 1) apply_bcs(P, Ss, un, vn, wn, 1, 1, true, false, st);
         }
         slow_rhs_post(P, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);

@@ -6,6 +6,7 @@
 // (ERF_PhysBCFunct.cpp:16-367).  Each stage is one launch over its ghost slabs.
 #include "erfx_common.h"
 #include "erfx_internal.h"
+#include "erfx_terrain.h"
 
 namespace erfx {
 
@@ -194,9 +195,10 @@ void fill_xvel(const Params& P, double* a, int ng, hipStream_t s) { fill_hvel(P,
 void fill_yvel(const Params& P, double* a, int ng, hipStream_t s) { fill_hvel(P, a, 1, ng, s); }
 
 // ERFPhysBCFunct_w::operator() (ERF_PhysBCFunct.cpp:271-367) and
-// ERF_BoundaryConditions_zvel.cpp:16-268, flat grid: the box is clipped to
-// k in [0, nz]; w(k=0) always takes the zlo Dirichlet value.
-void fill_zvel(const Params& P, double* a, int ng, hipStream_t s) {
+// ERF_BoundaryConditions_zvel.cpp:16-268: the box is clipped to k in [0, nz];
+// w(k=0) always takes the zlo Dirichlet value, which on a terrain grid is
+// Omega's, w following the slope of the filled u, v (:220-257).
+void fill_zvel(const Params& P, double* a, int ng, hipStream_t s, const double* uf, const double* vf) {
     fill_boundary_periodic(P, a, 1, P.ng_vel, 2, s);
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -221,6 +223,17 @@ void fill_zvel(const Params& P, double* a, int ng, hipStream_t s) {
     const int th = P.bck[2][5];
     // both planes in one launch: kp = 0 -> k = 0, kp = 1 -> k = nz
     Box b(-ng, -ng, 0, nx - 1 + ng, ny - 1 + ng, 1);
+    if (P.terrain) {
+        const Zn z{P.z_nd, L};
+        const Fld u{uf, L}, v{vf, L};
+        const double dxi = P.dxInv[0], dyi = P.dxInv[1];
+        launch_box(b, s, [=] __device__(int i, int j, int kp) {
+            if (kp == 0) { a[L.ix(i, j, 0)] = met::w_from_omega(i, j, 0, vl, u, v, z, dxi, dyi); return; }
+            if (th == EXT_DIR) a[L.ix(i, j, nz)] = met::w_from_omega(i, j, nz, vh, u, v, z, dxi, dyi);
+            else if (th == NEUMANN_INT) a[L.ix(i, j, nz)] = (4.0 * a[L.ix(i, j, nz - 1)] - a[L.ix(i, j, nz - 2)]) / 3.0;
+        });
+        return;
+    }
     launch_box(b, s, [=] __device__(int i, int j, int kp) {
         if (kp == 0) { a[L.ix(i, j, 0)] = vl; return; }
         if (th == EXT_DIR) a[L.ix(i, j, nz)] = vh;
@@ -273,6 +286,8 @@ void velocity_to_momentum(const Params& P, const double* xv, const double* yv, c
 //       on the FillBoundary layers.
 // Points the sequence does not touch are not written.
 bool fast_bcs_fusable(const Params& P) {
+    // the terrain wall value of w reads the filled u, v: kept in sequence
+    if (P.terrain) return false;
     if (!P.periodic[0] || !P.periodic[1]) return false;
     const int G = P.ng_cons > P.ng_vel ? P.ng_cons : P.ng_vel;
     if (P.L.nx < G + 1 || P.L.ny < G + 1 || P.L.nz < 2 * G) return false;
@@ -429,7 +444,7 @@ void fill_intermediate_patch(const Params& P, const erfx_state& S, double* xv, d
     if (!cons_only) {
         fill_xvel(P, xv, ngv, s);
         fill_yvel(P, yv, ngv, s);
-        fill_zvel(P, zv, ngv, s);
+        fill_zvel(P, zv, ngv, s, xv, yv);
         velocity_to_momentum(P, xv, yv, zv, 1, 0, S.cons, S.xmom, S.ymom, S.zmom, s);
     }
     fill_boundary_periodic(P, S.cons, P.cfg.ncons, P.ng_cons, -1, s);
@@ -467,7 +482,7 @@ void fill_patch(const Params& P, double* c, double* u, double* v, double* w, hip
     fill_cons(P, c, 0, P.cfg.ncons, P.ng_cons, s);
     fill_xvel(P, u, P.ng_vel, s);
     fill_yvel(P, v, P.ng_vel, s);
-    fill_zvel(P, w, P.ng_vel, s);
+    fill_zvel(P, w, P.ng_vel, s, u, v);
 }
 
 // cons_to_prim (ERF_TI_utils.H:4-45): prim on the box grown by ng, pi_stage

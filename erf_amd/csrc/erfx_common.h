@@ -167,6 +167,17 @@ struct Params {
     const double* ray = nullptr;  // Rayleigh table (device), RAY_* blocks of nz+1 values
     bool moist = false;       // a Kessler model: cons comps 5..7 are RhoQ1..RhoQ3
     int n_qstate = 0;         // 3 with Kessler (ERF_Kessler.H:172)
+    // terrain-following grid (use_terrain): node heights z_phys_nd and the
+    // metrics detJ, face areas ax, ay, az and cell-centre heights (device
+    // fields of the layout); null on a flat grid
+    bool terrain = false;
+    const double* z_nd = nullptr;
+    const double* detJ = nullptr;
+    const double* ax = nullptr;
+    const double* ay = nullptr;
+    const double* az = nullptr;
+    const double* z_cc = nullptr;
+    double dz_min = 0.0;      // dz, times the smallest detJ of the valid cells on a terrain grid
 };
 
 // prim components of the moisture species (prim(n) = cons(n+1) / rho)

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) est_dt_pass2(const double* partial, int n
 void est_dt_maxima(const Params& P, const double* cons, const double* u, const double* v, const double* w,
                    double* partial, double* host, hipStream_t s) {
     hipLaunchKernelGGL(est_dt_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, u, v, w, P.dxInv[0], P.dxInv[1],
-                       P.dxInv[2], P.cfg.implicit_substepping != 0, partial);
+                       1.0 / P.dz_min, P.cfg.implicit_substepping != 0, partial);   // ERF_ComputeTimestep.cpp:66
     hipLaunchKernelGGL(est_dt_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
                        partial + 2 * EST_BLOCKS);
     (void)hipMemcpyAsync(host, partial + 2 * EST_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, s);

@@ -39,6 +39,7 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
     const bool moist = P.moist;
     const double* pq1 = prim + PRIM_Q1 * ne;
     const double* pq2 = prim + PRIM_Q2 * ne;
+    const double* J = P.terrain ? P.detJ : nullptr;
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // bottom face, then the interior faces with the recurrence carried along
@@ -69,9 +70,17 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
                 double rhobar_lo = r_lo, rhobar_hi = v.r;
                 double pibar_lo = pib_lo, pibar_hi = v.pib;
                 double pi_c = 0.5 * (pis_lo + v.pis);
-                double coeff_P = -Gamma * R_d * dzi + halfg * R_d * rhobar_hi / (c_v * pibar_hi * v.rt_);
+                double coeff_P, coeff_Q, detJ_on_kface = 1.0;
+                if (J) {   // terrain (ERF_make_fast_coeffs.cpp:113-137)
+                    detJ_on_kface = 0.5 * (J[q] + J[q - sz]);
+                    double inv_detJ_on_kface = 1. / detJ_on_kface;
+                    coeff_P = -Gamma * R_d * dzi * inv_detJ_on_kface + halfg * R_d * rhobar_hi / (c_v * pibar_hi * v.rt_);
+                    coeff_Q = Gamma * R_d * dzi * inv_detJ_on_kface + halfg * R_d * rhobar_lo / (c_v * pibar_lo * rt_lo);
+                } else {
+                    coeff_P = -Gamma * R_d * dzi + halfg * R_d * rhobar_hi / (c_v * pibar_hi * v.rt_);
+                    coeff_Q = Gamma * R_d * dzi + halfg * R_d * rhobar_lo / (c_v * pibar_lo * rt_lo);
+                }
                 coeff_P *= pi_c;
-                double coeff_Q = Gamma * R_d * dzi + halfg * R_d * rhobar_lo / (c_v * pibar_lo * rt_lo);
                 coeff_Q *= pi_c;
                 Pc[q] = coeff_P;
                 Qc[q] = coeff_Q;
@@ -85,7 +94,7 @@ void make_fast_coeffs(const Params& P, double* fc, const erfx_state& stage, cons
                 double theta_t_hi = 0.5 * (pr_k + v.pkp1);
                 double a = D * (halfg - coeff_Q * theta_t_lo);
                 double c = D * (-halfg + coeff_P * theta_t_hi);
-                double b = 1.0 + D * (coeff_Q - coeff_P) * theta_t_mid;
+                double b = (J ? detJ_on_kface : 1.0) + D * (coeff_Q - coeff_P) * theta_t_mid;
                 A[q] = a;
                 Cc[q] = c;
                 double gam = c_prev / bet;

@@ -10,7 +10,9 @@ void fill_boundary_periodic(const Params& P, double* a, int ncomp, int ng, int n
 void fill_cons(const Params& P, double* c, int icomp, int ncomp, int ng, hipStream_t s);
 void fill_xvel(const Params& P, double* a, int ng, hipStream_t s);
 void fill_yvel(const Params& P, double* a, int ng, hipStream_t s);
-void fill_zvel(const Params& P, double* a, int ng, hipStream_t s);
+// u, v: the filled velocities, read for the wall value of w on a terrain grid
+void fill_zvel(const Params& P, double* a, int ng, hipStream_t s, const double* u = nullptr,
+               const double* v = nullptr);
 void momentum_to_velocity(const Params& P, double* xv, double* yv, double* zv, const double* dens,
                           const double* xm, const double* ym, const double* zm, hipStream_t s);
 void velocity_to_momentum(const Params& P, const double* xv, const double* yv, const double* zv, int gx,
@@ -46,6 +48,22 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 const double* fc, const erfx_state& data, const erfx_state& scratch, double dtau,
                 double beta_s, double facinv, hipStream_t s, const FastChain& ch = FastChain());
 bool fast_bcs_fusable(const Params& P);
+// erf_fast_rhs_T on a static terrain grid (erfx_terrain.hip): increments of the
+// previous substep and the updated horizontal momenta on faces grown by one
+struct TerrainWork {
+    double *du, *dv, *dw;     // momentum increments prev - stage
+    double *drho, *drt;       // rho and rho*theta increments, cells grown by 1
+    double *nu, *nv;          // new horizontal momentum increments
+    double *om;               // Omega of the old increments on z-faces
+};
+void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int step, int nrk,
+                const erfx_state& slow_rhs, const erfx_state& prev, const erfx_state& stage, const double* prim,
+                const double* pi_stage, const double* fc, const erfx_state& data, const erfx_state& scratch,
+                double dtau, double beta_s, double facinv, hipStream_t s);
+// make_J, make_areas, make_zcc (ERF_TerrainMetrics.cpp:583-715) on the whole
+// layout: detJ, ax, ay, az = 1 and z_cc = 0 outside the boxes they are made on
+void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* ax, double* ay, double* az,
+                     double* z_cc, hipStream_t s);
 
 // time-step estimate: max over valid cells of the acoustic and the advective
 // inverse time scales into host[0], host[1] (stream-ordered; sync before use)
@@ -76,6 +94,7 @@ struct SlowWork {
     double* xsrc; double* ysrc; double* zsrc;
     double* pavg = nullptr;       // moist buoyancy plane averages (PLANE_NOUT x nz)
     double* pavg_part = nullptr;  // their partial sums
+    double* omega = nullptr;      // Omega on z-faces grown by one (terrain grids)
 };
 void turbulent_viscosity(const Params& P, const SlowWork& W, const double* cons, const double* u,
                          const double* v, const double* w, double* eddy, hipStream_t s);

@@ -107,8 +107,9 @@ __device__ __forceinline__ Rates rates_rain(const Micro& m, double qv, double qc
 }
 
 // sedimentation tendency of rain in a cell from the face fluxes (:160-165)
-__device__ __forceinline__ double sed_of(const double* fz, long q, long sz, double r, double dt, double dz) {
-    double dJinv = 1.0;
+__device__ __forceinline__ double sed_of(const double* fz, long q, long sz, double r, double dt, double dz,
+                                         const double* J) {
+    double dJinv = J ? 1.0 / J[q] : 1.0;   // 1/detJ on a terrain grid (ERF_Kessler.cpp:82-94)
     double dq_sed = dt * dJinv * (1.0 / r) * (fz[q + sz] - fz[q]) / dz;
     if (fabs(dq_sed) < 1e-14) dq_sed = 0.0;
     return dq_sed;
@@ -151,6 +152,7 @@ void kessler_advance(const Params& P, double* cons, double* fz, double* rain_acc
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sz = L.sz;
     const double dz = P.cfg.dz;
+    const double* Jt = P.terrain ? P.detJ : nullptr;
     const double fac_cond = lcond / P.cfg.c_p;    // Kessler::Define (ERF_Kessler.H:65)
     const bool rain = P.cfg.moisture_type == ERFX_MOIST_KESSLER;
     if (rain) sed_flux(P, cons, fz, rain_accum, dt, s);
@@ -166,7 +168,7 @@ void kessler_advance(const Params& P, double* cons, double* fz, double* rain_acc
             qc = smax(0.0, qc);
             qp = smax(0.0, qp);
             const Rates R = rates_rain(m, qv, qc, qp, dt, &flag);
-            const double dq_sed = sed_of(fz, q, sz, m.r, dt, dz);
+            const double dq_sed = sed_of(fz, q, sz, m.r, dt, dz, Jt);
             qv += -R.v2c + R.c2v + R.r2v;
             qc += R.v2c - R.c2v - R.c2r;
             qp += dq_sed + R.c2r - R.r2v;
@@ -258,6 +260,7 @@ void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, dou
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sz = L.sz;
     const double dz = P.cfg.dz;
+    const double* Jt = P.terrain ? P.detJ : nullptr;
     const double fac_cond = lcond / P.cfg.c_p;
     sed_flux(P, cons, fz, rain_accum, dt, s);
     (void)hipMemsetAsync(counter, 0, 2 * sizeof(unsigned long long), s);
@@ -297,7 +300,7 @@ void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, dou
         double th = m.th;
         double qv = smax(0.0, m.qv), qc = smax(0.0, m.qc), qp = smax(0.0, m.qp);
         int flag = 0;
-        const double dq_sed = sed_of(fz, q, sz, m.r, dt, dz);
+        const double dq_sed = sed_of(fz, q, sz, m.r, dt, dz, Jt);
         const int sl = slot[q];
         if (sl == -2) {                       // Kessler's own update
             const Rates R = rates_rain(m, qv, qc, qp, dt, &flag);

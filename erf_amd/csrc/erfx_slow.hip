@@ -9,6 +9,7 @@
 // gradient and sources of one momentum component.
 #include "erfx_common.h"
 #include "erfx_internal.h"
+#include "erfx_terrain.h"
 
 namespace erfx {
 
@@ -170,6 +171,175 @@ __device__ __forceinline__ double interp_z_all(const double* f, long q, long sz,
     else if (t == ERFX_ADV_CENTERED6) iv = g1_5_6 * a1 - g2_5_6 * a2 + g3_5_6 * a3;
     return iv;
 }
+
+// ---- momentum advection on a terrain-following grid ---------------------
+// AdvectionSrcForMom with metrics (ERF_AdvectionSrcForMom.cpp:204-288 for
+// C2/C2, ERF_AdvectionSrcForMom_T.H:22-330 otherwise), static terrain, unit
+// map factors: the flat fluxes weighted by the face areas and the edge
+// heights h_zeta, divided by detJ averaged to the face.  Each returns that
+// quotient; the right-hand side is its negative.
+struct TAdv {
+    const double *ru, *rv, *u, *v, *w, *om, *Ax, *Ay, *Az, *J;
+    Zn z;
+    long sy, sz;
+    double dxi, dyi, dzi, hf, vf;
+    int ht, vt, nz;
+    bool c2c2;
+
+    __device__ double x(int i, int j, int k, long q) const {
+        double src;
+        if (c2c2) {
+            double xflux_hi = 0.25 * (ru[q] * 1.0 + ru[q + 1] * 1.0) * (u[q + 1] + u[q]) * 0.5 * (Ax[q] + Ax[q + 1]);
+            double xflux_lo = 0.25 * (ru[q] * 1.0 + ru[q - 1] * 1.0) * (u[q - 1] + u[q]) * 0.5 * (Ax[q] + Ax[q - 1]);
+            double met_h_zeta_yhi = met::h_zeta_EdgeK(z, i, j + 1, k, dzi);
+            double yflux_hi = 0.25 * (rv[q + sy] * 1.0 + rv[q - 1 + sy] * 1.0) * (u[q + sy] + u[q]) * met_h_zeta_yhi;
+            double met_h_zeta_ylo = met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double yflux_lo = 0.25 * (rv[q] * 1.0 + rv[q - 1] * 1.0) * (u[q - sy] + u[q]) * met_h_zeta_ylo;
+            double zflux_hi = 0.25 * (om[q + sz] + om[q - 1 + sz]) * (u[q + sz] + u[q]) * 0.5 * (Az[q + sz] + Az[q - 1 + sz]);
+            double zflux_lo = 0.25 * (om[q] + om[q - 1]) * (u[q - sz] + u[q]) * 0.5 * (Az[q] + Az[q - 1]);
+            double mfsq = 1.0 * 1.0;
+            src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq + (zflux_hi - zflux_lo) * dzi;
+        } else {
+            double rho_u_avg_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q] * 1.0);
+            double interp_hi = interp_at(ht, u, q + 1, 1, rho_u_avg_hi, hf);
+            double rho_u_avg_lo = 0.5 * (ru[q - 1] * 1.0 + ru[q] * 1.0);
+            double interp_lo = interp_at(ht, u, q, 1, rho_u_avg_lo, hf);
+            double centFluxXXNext = rho_u_avg_hi * interp_hi * 0.5 * (Ax[q] + Ax[q + 1]);
+            double centFluxXXPrev = rho_u_avg_lo * interp_lo * 0.5 * (Ax[q] + Ax[q - 1]);
+            double rho_v_avg_hi = 0.5 * (rv[q + sy] * 1.0 + rv[q - 1 + sy] * 1.0);
+            interp_hi = interp_at(ht, u, q + sy, sy, rho_v_avg_hi, hf);
+            double rho_v_avg_lo = 0.5 * (rv[q] * 1.0 + rv[q - 1] * 1.0);
+            interp_lo = interp_at(ht, u, q, sy, rho_v_avg_lo, hf);
+            double edgeFluxXYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i, j + 1, k, dzi);
+            double edgeFluxXYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double Omega_avg_hi = 0.5 * (om[q + sz] + om[q - 1 + sz]) * 0.5 * (Az[q + sz] + Az[q - 1 + sz]);
+            double Omega_avg_lo = 0.5 * (om[q] + om[q - 1]) * 0.5 * (Az[q] + Az[q - 1]);
+            interp_hi = interp_at(vt, u, q + sz, sz, Omega_avg_hi, vf);
+            interp_lo = interp_at(vt, u, q, sz, Omega_avg_lo, vf);
+            double edgeFluxXZNext = Omega_avg_hi * interp_hi;
+            double edgeFluxXZPrev = Omega_avg_lo * interp_lo;
+            double mfsq = 1 / (1.0 * 1.0);
+            src = (centFluxXXNext - centFluxXXPrev) * dxi * mfsq + (edgeFluxXYNext - edgeFluxXYPrev) * dyi * mfsq
+                + (edgeFluxXZNext - edgeFluxXZPrev) * dzi;
+        }
+        src /= 0.5 * (J[q] + J[q - 1]);
+        return src;
+    }
+
+    __device__ double y(int i, int j, int k, long q) const {
+        double src;
+        if (c2c2) {
+            double met_h_zeta_xhi = met::h_zeta_EdgeK(z, i + 1, j, k, dzi);
+            double xflux_hi = 0.25 * (ru[q + 1] * 1.0 + ru[q + 1 - sy] * 1.0) * (v[q + 1] + v[q]) * met_h_zeta_xhi;
+            double met_h_zeta_xlo = met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double xflux_lo = 0.25 * (ru[q] * 1.0 + ru[q - sy] * 1.0) * (v[q - 1] + v[q]) * met_h_zeta_xlo;
+            double yflux_hi = 0.25 * (rv[q + sy] * 1.0 + rv[q] * 1.0) * (v[q + sy] + v[q]) * 0.5 * (Ay[q] + Ay[q + sy]);
+            double yflux_lo = 0.25 * (rv[q] * 1.0 + rv[q - sy] * 1.0) * (v[q - sy] + v[q]) * 0.5 * (Ay[q] + Ay[q - sy]);
+            double zflux_hi = 0.25 * (om[q + sz] + om[q - sy + sz]) * (v[q + sz] + v[q]) * 0.5 * (Az[q + sz] + Az[q - sy + sz]);
+            double zflux_lo = 0.25 * (om[q] + om[q - sy]) * (v[q - sz] + v[q]) * 0.5 * (Az[q] + Az[q - sy]);
+            double mfsq = 1.0 * 1.0;
+            src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq + (zflux_hi - zflux_lo) * dzi;
+        } else {
+            double rho_u_avg_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q + 1 - sy] * 1.0);
+            double rho_u_avg_lo = 0.5 * (ru[q] * 1.0 + ru[q - sy] * 1.0);
+            double interp_hi = interp_at(ht, v, q + 1, 1, rho_u_avg_hi, hf);
+            double interp_lo = interp_at(ht, v, q, 1, rho_u_avg_lo, hf);
+            double edgeFluxYXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i + 1, j, k, dzi);
+            double edgeFluxYXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
+            double rho_v_avg_hi = 0.5 * (rv[q] * 1.0 + rv[q + sy] * 1.0);
+            double rho_v_avg_lo = 0.5 * (rv[q] * 1.0 + rv[q - sy] * 1.0);
+            interp_hi = interp_at(ht, v, q + sy, sy, rho_v_avg_hi, hf);
+            interp_lo = interp_at(ht, v, q, sy, rho_v_avg_lo, hf);
+            double centFluxYYNext = rho_v_avg_hi * 0.5 * (Ay[q] + Ay[q + sy]) * interp_hi;
+            double centFluxYYPrev = rho_v_avg_lo * 0.5 * (Ay[q] + Ay[q - sy]) * interp_lo;
+            double Omega_avg_hi = 0.5 * (om[q + sz] + om[q - sy + sz]) * 0.5 * (Az[q + sz] + Az[q - sy + sz]);
+            double Omega_avg_lo = 0.5 * (om[q] + om[q - sy]) * 0.5 * (Az[q] + Az[q - sy]);
+            interp_hi = interp_at(vt, v, q + sz, sz, Omega_avg_hi, vf);
+            interp_lo = interp_at(vt, v, q, sz, Omega_avg_lo, vf);
+            double edgeFluxYZNext = Omega_avg_hi * interp_hi;
+            double edgeFluxYZPrev = Omega_avg_lo * interp_lo;
+            double mfsq = 1 / (1.0 * 1.0);
+            src = (edgeFluxYXNext - edgeFluxYXPrev) * dxi * mfsq + (centFluxYYNext - centFluxYYPrev) * dyi * mfsq
+                + (edgeFluxYZNext - edgeFluxYZPrev) * dzi;
+        }
+        src /= 0.5 * (J[q] + J[q - sy]);
+        return src;
+    }
+
+    // interior z-faces 1 <= k <= nz-1
+    __device__ double zf(int i, int j, int k, long q) const {
+        const int hz = nz, lz = 0;
+        double src;
+        if (c2c2) {
+            double met_h_zeta_xhi = met::h_zeta_EdgeJ(z, i + 1, j, k, dzi);
+            double xflux_hi = 0.25 * (ru[q + 1] + ru[q + 1 - sz]) * 1.0 * (w[q + 1] + w[q]) * met_h_zeta_xhi;
+            double met_h_zeta_xlo = met::h_zeta_EdgeJ(z, i, j, k, dzi);
+            double xflux_lo = 0.25 * (ru[q] + ru[q - sz]) * 1.0 * (w[q - 1] + w[q]) * met_h_zeta_xlo;
+            double met_h_zeta_yhi = met::h_zeta_EdgeI(z, i, j + 1, k, dzi);
+            double yflux_hi = 0.25 * (rv[q + sy] + rv[q + sy - sz]) * 1.0 * (w[q + sy] + w[q]) * met_h_zeta_yhi;
+            double met_h_zeta_ylo = met::h_zeta_EdgeI(z, i, j, k, dzi);
+            double yflux_lo = 0.25 * (rv[q] + rv[q - sz]) * 1.0 * (w[q - sy] + w[q]) * met_h_zeta_ylo;
+            double zflux_lo = 0.25 * (om[q] + om[q - sz]) * (w[q] + w[q - sz]);
+            double zflux_hi = (k == hz) ? om[q] * w[q] * Az[q]
+                                        : 0.25 * (om[q] + om[q + sz]) * (w[q] + w[q + sz]) * 0.5 * (Az[q] + Az[q + sz]);
+            double mfsq = 1.0 * 1.0;
+            src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq + (zflux_hi - zflux_lo) * dzi;
+        } else {
+            double rho_u_avg_hi = 0.5 * (ru[q + 1] + ru[q + 1 - sz]) * 1.0;
+            double rho_u_avg_lo = 0.5 * (ru[q] + ru[q - sz]) * 1.0;
+            double interp_hi = interp_at(ht, w, q + 1, 1, rho_u_avg_hi, hf);
+            double interp_lo = interp_at(ht, w, q, 1, rho_u_avg_lo, hf);
+            double edgeFluxZXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeJ(z, i + 1, j, k, dzi);
+            double edgeFluxZXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeJ(z, i, j, k, dzi);
+            double rho_v_avg_hi = 0.5 * (rv[q + sy] + rv[q + sy - sz]) * 1.0;
+            interp_hi = interp_at(ht, w, q + sy, sy, rho_v_avg_hi, hf);
+            double rho_v_avg_lo = 0.5 * (rv[q] + rv[q - sz]) * 1.0;
+            interp_lo = interp_at(ht, w, q, sy, rho_v_avg_lo, hf);
+            double edgeFluxZYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeI(z, i, j + 1, k, dzi);
+            double edgeFluxZYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeI(z, i, j, k, dzi);
+            double Omega_avg_hi = (k == hz) ? om[q] * Az[q] : 0.25 * (om[q] + om[q + sz]) * (Az[q] + Az[q + sz]);
+            double centFluxZZNext = Omega_avg_hi;
+            if (k == hz) {
+                centFluxZZNext *= w[q];
+            } else {
+                if (k == hz - 1) {
+                    interp_hi = interp_z_all(w, q + sz, sz, Omega_avg_hi, 0, ERFX_ADV_CENTERED2);
+                } else if (k == hz - 2 || k == lz + 1) {
+                    if (vt != ERFX_ADV_CENTERED2 && vt != ERFX_ADV_UPWIND3)
+                        interp_hi = interp_z_all(w, q + sz, sz, Omega_avg_hi, 0, ERFX_ADV_CENTERED4);
+                    else
+                        interp_hi = interp_z_all(w, q + sz, sz, Omega_avg_hi, vf, vt);
+                } else {
+                    interp_hi = interp_at(vt, w, q + sz, sz, Omega_avg_hi, vf);
+                }
+                centFluxZZNext *= interp_hi;
+            }
+            double Omega_avg_lo = (k == 0) ? om[q] * Az[q] : 0.25 * (om[q] + om[q - sz]) * (Az[q] + Az[q - sz]);
+            double centFluxZZPrev = Omega_avg_lo;
+            if (k == 0) {
+                centFluxZZPrev *= w[q];
+            } else {
+                if (k == lz + 1) {
+                    interp_lo = interp_z_all(w, q, sz, Omega_avg_lo, 0, ERFX_ADV_CENTERED2);
+                } else if (k == lz + 2 || k == hz - 1) {
+                    if (vt != ERFX_ADV_CENTERED2 && vt != ERFX_ADV_UPWIND3)
+                        interp_lo = interp_z_all(w, q, sz, Omega_avg_lo, 0, ERFX_ADV_CENTERED4);
+                    else
+                        interp_lo = interp_z_all(w, q, sz, Omega_avg_lo, vf, vt);
+                } else {
+                    interp_lo = interp_at(vt, w, q, sz, Omega_avg_lo, vf);
+                }
+                centFluxZZPrev *= interp_lo;
+            }
+            double mfsq = 1.0 * 1.0;
+            src = (edgeFluxZXNext - edgeFluxZXPrev) * dxi * mfsq + (edgeFluxZYNext - edgeFluxZYPrev) * dyi * mfsq
+                + (centFluxZZNext - centFluxZZPrev) * dzi;
+        }
+        double denom = 0.5 * (J[q] + J[q - sz]);
+        src /= denom;
+        return src;
+    }
+};
 
 // ---- strain (ComputeStrain_N, Source/Diffusion/ERF_ComputeStrain_N.cpp:28-229) ----
 struct StrainBC {
@@ -460,55 +630,113 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const double* ru = data.xmom;
     const double* rv = data.ymom;
     const double* rw = data.zmom;
-    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        double fx0 = ru[q], fx1 = ru[q + 1], fy0 = rv[q], fy1 = rv[q + sy], fz0 = rw[q], fz1 = rw[q + sz];
-        cr0[q] = -(1.0 * 1.0) / 1.0 * ((fx1 - fx0) * dxi + (fy1 - fy0) * dyi + (fz1 - fz0) * dzi);
-        double gx0, gx1, gy0, gy1, gz0, gz1;
-        if (c2c2) {
-            gx0 = fx0 * (0.5 * (prim[q] + prim[q - 1]));
-            gx1 = fx1 * (0.5 * (prim[q + 1] + prim[q]));
-            gy0 = fy0 * (0.5 * (prim[q] + prim[q - sy]));
-            gy1 = fy1 * (0.5 * (prim[q + sy] + prim[q]));
-            gz0 = fz0 * (0.5 * (prim[q] + prim[q - sz]));
-            gz1 = fz1 * (0.5 * (prim[q + sz] + prim[q]));
-        } else {
-            gx0 = fx0 * interp_at(ht, prim, q, 1, fx0, hf);
-            gx1 = fx1 * interp_at(ht, prim, q + 1, 1, fx1, hf);
-            gy0 = fy0 * interp_at(ht, prim, q, sy, fy0, hf);
-            gy1 = fy1 * interp_at(ht, prim, q + sy, sy, fy1, hf);
-            gz0 = fz0 * interp_at(vt, prim, q, sz, fz0, vf);
-            gz1 = fz1 * interp_at(vt, prim, q + sz, sz, fz1, vf);
-        }
-        double r1 = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
-        if (use_diff) {
-            const double* c = data.cons;
-            double dfx0 = diff_flux(c, prim, mu_th_h, q, 1, dxi, ae_theta, consA, turb) * 1.0;
-            double dfx1 = diff_flux(c, prim, mu_th_h, q + 1, 1, dxi, ae_theta, consA, turb) * 1.0;
-            double dfy0 = diff_flux(c, prim, mu_th_h, q, sy, dyi, ae_theta, consA, turb) * 1.0;
-            double dfy1 = diff_flux(c, prim, mu_th_h, q + sy, sy, dyi, ae_theta, consA, turb) * 1.0;
-            double dfz0 = (sflux && k == 0) ? sflux_val : diff_flux(c, prim, mu_th_v, q, sz, dzi, ae_theta, consA, turb);
-            double dfz1 = diff_flux(c, prim, mu_th_v, q + sz, sz, dzi, ae_theta, consA, turb);
-            r1 -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
-        }
-        // cell source of rho*theta (make_sources: Rayleigh damping, ERF_make_sources.cpp:203-225),
-        // added the way ERF adds cc_src (ERF_slow_rhs_pre.cpp:521-526)
-        double src = 0.0;
-        if (rayT && ray[RAY_ACT_CC * n1 + k] != 0.0) {
-            double theta = prim[q];
-            src -= ray[RAY_FAC_CC * n1 + k] * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
-        }
-        r1 += src;
-        cr1[q] = r1;
-    });
+    if (P.terrain) {
+        // Omega on z-faces grown by one (ERF_slow_rhs_pre.cpp:367-414): 0 at and
+        // below the bottom face, rho*w above the top face, OmegaFromW between
+        // (the top face included); the area-weighted fluxes of
+        // AdvectionSrcForRho (ERF_AdvectionSrcForState.cpp:29-93) seed the
+        // time-averaged momenta
+        const Zn z{P.z_nd, L};
+        const Fld fru{ru, L}, frv{rv, L};
+        const double* Ax = P.ax; const double* Ay = P.ay; const double* Az = P.az; const double* Jc = P.detJ;
+        double* omw = W.omega;
+        double* sxm = scratch.xmom; double* sym = scratch.ymom; double* szm = scratch.zmom;
+        launch_box(Box(-1, -1, -1, nx, ny, nz + 1), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            double o;
+            if (k <= 0) o = 0.;
+            else if (k == nz + 1) o = rw[q];
+            else o = met::omega_from_w(i, j, k, rw[q], fru, frv, z, dxi, dyi);
+            omw[q] = o;
+            if (i >= 0 && j >= 0 && j <= ny - 1 && k >= 0 && k <= nz - 1) sxm[q] = Ax[q] * ru[q] / 1.0;
+            if (i >= 0 && i <= nx - 1 && j >= 0 && k >= 0 && k <= nz - 1) sym[q] = Ay[q] * rv[q] / 1.0;
+            if (i >= 0 && i <= nx - 1 && j >= 0 && j <= ny - 1 && k >= 0 && k <= nz) szm[q] = Az[q] * o / (1.0 * 1.0);
+        });
+        // cell RHS of rho (AdvectionSrcForRho) and rho*theta (AdvectionSrcForScalars
+        // with 1/detJ, :298), from the area-weighted fluxes
+        launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            double fx0 = sxm[q], fx1 = sxm[q + 1], fy0 = sym[q], fy1 = sym[q + sy], fz0 = szm[q], fz1 = szm[q + sz];
+            const double mfsq = 1.0 * 1.0;
+            const double dJ = Jc[q];
+            if (dJ > 0.)
+                cr0[q] = -mfsq / dJ * ((fx1 - fx0) * dxi + (fy1 - fy0) * dyi + (fz1 - fz0) * dzi);
+            else
+                cr0[q] = 0.;
+            double gx0, gx1, gy0, gy1, gz0, gz1;
+            if (c2c2) {
+                gx0 = fx0 * (0.5 * (prim[q] + prim[q - 1]));
+                gx1 = fx1 * (0.5 * (prim[q + 1] + prim[q]));
+                gy0 = fy0 * (0.5 * (prim[q] + prim[q - sy]));
+                gy1 = fy1 * (0.5 * (prim[q + sy] + prim[q]));
+                gz0 = fz0 * (0.5 * (prim[q] + prim[q - sz]));
+                gz1 = fz1 * (0.5 * (prim[q + sz] + prim[q]));
+            } else {
+                gx0 = fx0 * interp_at(ht, prim, q, 1, fx0, hf);
+                gx1 = fx1 * interp_at(ht, prim, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_at(ht, prim, q, sy, fy0, hf);
+                gy1 = fy1 * interp_at(ht, prim, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_at(vt, prim, q, sz, fz0, vf);
+                gz1 = fz1 * interp_at(vt, prim, q + sz, sz, fz1, vf);
+            }
+            double invdetJ = (dJ > 0.) ? 1. / dJ : 1.;
+            double r1 = -invdetJ * mfsq * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
+            double src = 0.0;
+            r1 += src;
+            cr1[q] = r1;
+        });
+    } else {
+        launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            double fx0 = ru[q], fx1 = ru[q + 1], fy0 = rv[q], fy1 = rv[q + sy], fz0 = rw[q], fz1 = rw[q + sz];
+            cr0[q] = -(1.0 * 1.0) / 1.0 * ((fx1 - fx0) * dxi + (fy1 - fy0) * dyi + (fz1 - fz0) * dzi);
+            double gx0, gx1, gy0, gy1, gz0, gz1;
+            if (c2c2) {
+                gx0 = fx0 * (0.5 * (prim[q] + prim[q - 1]));
+                gx1 = fx1 * (0.5 * (prim[q + 1] + prim[q]));
+                gy0 = fy0 * (0.5 * (prim[q] + prim[q - sy]));
+                gy1 = fy1 * (0.5 * (prim[q + sy] + prim[q]));
+                gz0 = fz0 * (0.5 * (prim[q] + prim[q - sz]));
+                gz1 = fz1 * (0.5 * (prim[q + sz] + prim[q]));
+            } else {
+                gx0 = fx0 * interp_at(ht, prim, q, 1, fx0, hf);
+                gx1 = fx1 * interp_at(ht, prim, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_at(ht, prim, q, sy, fy0, hf);
+                gy1 = fy1 * interp_at(ht, prim, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_at(vt, prim, q, sz, fz0, vf);
+                gz1 = fz1 * interp_at(vt, prim, q + sz, sz, fz1, vf);
+            }
+            double r1 = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
+            if (use_diff) {
+                const double* c = data.cons;
+                double dfx0 = diff_flux(c, prim, mu_th_h, q, 1, dxi, ae_theta, consA, turb) * 1.0;
+                double dfx1 = diff_flux(c, prim, mu_th_h, q + 1, 1, dxi, ae_theta, consA, turb) * 1.0;
+                double dfy0 = diff_flux(c, prim, mu_th_h, q, sy, dyi, ae_theta, consA, turb) * 1.0;
+                double dfy1 = diff_flux(c, prim, mu_th_h, q + sy, sy, dyi, ae_theta, consA, turb) * 1.0;
+                double dfz0 = (sflux && k == 0) ? sflux_val : diff_flux(c, prim, mu_th_v, q, sz, dzi, ae_theta, consA, turb);
+                double dfz1 = diff_flux(c, prim, mu_th_v, q + sz, sz, dzi, ae_theta, consA, turb);
+                r1 -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
+            }
+            // cell source of rho*theta (make_sources: Rayleigh damping, ERF_make_sources.cpp:203-225),
+            // added the way ERF adds cc_src (ERF_slow_rhs_pre.cpp:521-526)
+            double src = 0.0;
+            if (rayT && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+                double theta = prim[q];
+                src -= ray[RAY_FAC_CC * n1 + k] * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
+            }
+            r1 += src;
+            cr1[q] = r1;
+        });
 
-    // time-averaged momenta seeded with the stage momenta (ERF_AdvectionSrcForState.cpp:56-71)
-    launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        if (j <= ny - 1 && k <= nz - 1) scratch.xmom[q] = 1.0 * ru[q] / 1.0;
-        if (i <= nx - 1 && k <= nz - 1) scratch.ymom[q] = 1.0 * rv[q] / 1.0;
-        if (i <= nx - 1 && j <= ny - 1) scratch.zmom[q] = 1.0 * rw[q] / (1.0 * 1.0);
-    });
+        // time-averaged momenta seeded with the stage momenta (ERF_AdvectionSrcForState.cpp:56-71)
+        launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+            const long q = L.ix(i, j, k);
+            if (j <= ny - 1 && k <= nz - 1) scratch.xmom[q] = 1.0 * ru[q] / 1.0;
+            if (i <= nx - 1 && k <= nz - 1) scratch.ymom[q] = 1.0 * rv[q] / 1.0;
+            if (i <= nx - 1 && j <= ny - 1) scratch.zmom[q] = 1.0 * rw[q] / (1.0 * 1.0);
+        });
+
+    }
 
     // momentum RHS (AdvectionSrcForMom flat, DiffusionSrcForMom_N, pressure
     // gradient, make_mom_sources), one launch over the three face boxes
@@ -524,7 +752,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     double* rur = rhs.xmom;
     double* rvr = rhs.ymom;
     double* rwr = rhs.zmom;
-    const double* om = rw;   // Omega = rho*w on a flat grid
+    const double* om = P.terrain ? W.omega : rw;   // Omega = rho*w on a flat grid
+    const bool terr = P.terrain;
+    const TAdv TA{ru, rv, u, v, w, om, P.ax, P.ay, P.az, P.detJ, Zn{P.z_nd, L}, sy, sz, dxi, dyi, dzi, hf, vf,
+                  ht, vt, nz, c2c2};
+    const Zn zt{P.z_nd, L};
     const double* pq1 = prim + PRIM_Q1 * ne;
     const double* pq2 = prim + PRIM_Q2 * ne;
     const double* pq3 = prim + PRIM_Q3 * ne;
@@ -536,7 +768,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
         // ---- x-momentum on x-faces ----
         if (j <= ny - 1 && k <= nz - 1) {
             double src;
-            if (c2c2) {
+            if (terr) {
+                src = TA.x(i, j, k, q);
+            } else if (c2c2) {
                 double xflux_hi = 0.25 * (ru[q] * 1.0 + ru[q + 1] * 1.0) * (u[q + 1] + u[q]);
                 double xflux_lo = 0.25 * (ru[q] * 1.0 + ru[q - 1] * 1.0) * (u[q - 1] + u[q]);
                 double yflux_hi = 0.25 * (rv[q + sy] * 1.0 + rv[q - 1 + sy] * 1.0) * (u[q + sy] + u[q]);
@@ -579,6 +813,18 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             }
             xs += 0.5 * (cons[q] + cons[q - 1]) * g0;
             double gpx = dxi * (pp[q] - pp[q - 1]);
+            if (terr) {   // ERF_slow_rhs_pre.cpp:600-620
+                double met_h_xi = met::h_xi_Iface(zt, i, j, k, dxi);
+                double met_h_zeta = met::h_zeta_Iface(zt, i, j, k, dzi);
+                double gp_zeta_on_iface;
+                if (k == 0)
+                    gp_zeta_on_iface = 0.5 * dzi * (pp[q - 1 + sz] + pp[q + sz] - pp[q - 1] - pp[q]);
+                else if (k == nz - 1)
+                    gp_zeta_on_iface = 0.5 * dzi * (pp[q - 1] + pp[q] - pp[q - 1 - sz] - pp[q - sz]);
+                else
+                    gp_zeta_on_iface = 0.25 * dzi * (pp[q - 1 + sz] + pp[q + sz] - pp[q - 1 - sz] - pp[q - sz]);
+                gpx -= (met_h_xi / met_h_zeta) * gp_zeta_on_iface;
+            }
             gpx *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - 1) : 0.0;
             r += (-gpx - 0.0) / (1.0 + qf) + xs;
@@ -588,7 +834,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
         // ---- y-momentum on y-faces ----
         if (i <= nx - 1 && k <= nz - 1) {
             double src;
-            if (c2c2) {
+            if (terr) {
+                src = TA.y(i, j, k, q);
+            } else if (c2c2) {
                 double xflux_hi = 0.25 * (ru[q + 1] * 1.0 + ru[q + 1 - sy] * 1.0) * (v[q + 1] + v[q]);
                 double xflux_lo = 0.25 * (ru[q] * 1.0 + ru[q - sy] * 1.0) * (v[q - 1] + v[q]);
                 double yflux_hi = 0.25 * (rv[q + sy] * 1.0 + rv[q] * 1.0) * (v[q + sy] + v[q]);
@@ -630,6 +878,18 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             }
             ys += 0.5 * (cons[q] + cons[q - sy]) * g1;
             double gpy = dyi * (pp[q] - pp[q - sy]);
+            if (terr) {   // ERF_slow_rhs_pre.cpp:650-669
+                double met_h_eta = met::h_eta_Jface(zt, i, j, k, dyi);
+                double met_h_zeta = met::h_zeta_Jface(zt, i, j, k, dzi);
+                double gp_zeta_on_jface;
+                if (k == 0)
+                    gp_zeta_on_jface = 0.5 * dzi * (pp[q + sz] + pp[q - sy + sz] - pp[q] - pp[q - sy]);
+                else if (k == nz - 1)
+                    gp_zeta_on_jface = 0.5 * dzi * (pp[q] + pp[q - sy] - pp[q - sz] - pp[q - sy - sz]);
+                else
+                    gp_zeta_on_jface = 0.25 * dzi * (pp[q + sz] + pp[q - sy + sz] - pp[q - sz] - pp[q - sy - sz]);
+                gpy -= (met_h_eta / met_h_zeta) * gp_zeta_on_jface;
+            }
             gpy *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - sy) : 0.0;
             r += (-gpy - 0.0) / (1.0 + qf) + ys;
@@ -641,7 +901,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             if (k == 0 || k == nz) { rwr[q] = 0.0; return; }
             const int hz = nz, lz = 0;
             double src;
-            if (c2c2) {
+            if (terr) {
+                src = TA.zf(i, j, k, q);
+            } else if (c2c2) {
                 double xflux_hi = 0.25 * (ru[q + 1] + ru[q + 1 - sz]) * 1.0 * (w[q + 1] + w[q]);
                 double xflux_lo = 0.25 * (ru[q] + ru[q - sz]) * 1.0 * (w[q - 1] + w[q]);
                 double yflux_hi = 0.25 * (rv[q + sy] + rv[q + sy - sz]) * 1.0 * (w[q + sy] + w[q]);
@@ -732,7 +994,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 zs -= ray[RAY_FAC_ND * n1 + k] * (ww - ray[RAY_WBAR * n1 + k]) * rho_on_w_face;
             }
             zs += 0.5 * (cons[q] + cons[q - sz]) * g2;
-            double met_h_zeta = 1;
+            double met_h_zeta = terr ? met::h_zeta_Kface(zt, i, j, k, dzi) : 1;
             double gpz = dzi * (pp[q] - pp[q - sz]) / met_h_zeta;
             double qf = moist ? face_q(pq1, pq2, q, q - sz) : 0.0;
             r += (zs - gpz - 0.0) / (1.0 + qf);
@@ -760,6 +1022,7 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
     const int ncons = P.cfg.ncons;
+    const double* Jc = P.terrain ? P.detJ : nullptr;   // 1/detJ on a terrain grid (:298, :361)
     const int ht = B.ht;
     const int vt = is_weno(ht) ? ht : B.vt;
     const double hf = B.hf, vf = B.vf;
@@ -801,7 +1064,8 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
                 gz0 = fz0 * interp_scalar_at(vt, ps, q, sz, fz0, vf);
                 gz1 = fz1 * interp_scalar_at(vt, ps, q + sz, sz, fz1, vf);
             }
-            double r = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
+            double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
+            double r = -invdetJ * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
             if (use_diff) {
                 double dfx0 = diff_flux(nc, ps, mu_h, q, 1, dxi, ae, consA, turb) * 1.0;
                 double dfx1 = diff_flux(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb) * 1.0;

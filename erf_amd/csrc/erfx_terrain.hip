@@ -1,0 +1,269 @@
+// erfx_terrain.hip -- terrain-following grid: the metrics of the node heights
+// and the acoustic substep erf_fast_rhs_T (static terrain, unit map factors).
+//
+// The substep runs as four launches: increments on the grown box; the
+// horizontal momenta; the horizontal flux divergences with Omega of the old
+// increments; and one thread per column for the implicit vertical solve and
+// the rho, rho*theta and rho*w updates.  Every expression keeps the
+// reference's operation order (FP contraction is off), so each point carries
+// the reference's rounding.
+#include "erfx_internal.h"
+#include "erfx_terrain.h"
+
+namespace erfx {
+
+// one lane per element of the layout, padding included
+template <class F>
+__global__ void __launch_bounds__(256) lin_kernel(long n, F g) {
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) g(q);
+}
+
+// make_J, make_areas, make_zcc (Source/Utils/ERF_TerrainMetrics.cpp:583-715):
+// detJ and z_cc on cells (-1..nx, -1..ny, 0..nz), ax on x-faces (-1..nx+1),
+// ay on y-faces (-1..ny+1); 1 (0 for z_cc) everywhere else, az = 1.
+void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* ax, double* ay, double* az,
+                     double* z_cc, hipStream_t s) {
+    const Lay L = P.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
+    const long ne = L.ne, sy = L.sy, sz = L.sz;
+    const double dzInv = 1.0 / P.cfg.dz;
+    const Zn z{z_nd, L};
+    const unsigned nb = (unsigned)((ne + 255) / 256);
+    auto f = [=] __device__(long q) {
+        const int k = (int)(q / sz) - ng;
+        const long rem = q % sz;
+        const int j = (int)(rem / sy) - ng, i = (int)(rem % sy) - ng;
+        const bool kin = k >= 0 && k <= nz;
+        double J = 1.0, Zc = 0.0, Ax = 1.0, Ay = 1.0;
+        if (kin && i >= -1 && i <= nx && j >= -1 && j <= ny) {
+            J = .25 * dzInv * (z(i, j, k + 1) + z(i + 1, j, k + 1) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1)
+                               - z(i, j, k) - z(i + 1, j, k) - z(i, j + 1, k) - z(i + 1, j + 1, k));
+            Zc = .125 * (z(i, j, k) + z(i + 1, j, k) + z(i, j + 1, k) + z(i + 1, j + 1, k)
+                         + z(i, j, k + 1) + z(i + 1, j, k + 1) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1));
+        }
+        if (kin && i >= -1 && i <= nx + 1 && j >= -1 && j <= ny)
+            Ax = .5 * dzInv * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
+        if (kin && i >= -1 && i <= nx && j >= -1 && j <= ny + 1)
+            Ay = .5 * dzInv * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
+        detJ[q] = J; z_cc[q] = Zc; ax[q] = Ax; ay[q] = Ay; az[q] = 1.0;
+    };
+    hipLaunchKernelGGL(lin_kernel<decltype(f)>, dim3(nb), dim3(256), 0, s, ne, f);
+}
+
+// erf_fast_rhs_T (Source/TimeIntegration/ERF_fast_rhs_T.cpp:39-704), GPU
+// branch of the column solve.  For step > 0 prev and data alias (S_sum).
+void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int step, int nrk,
+                const erfx_state& slow_rhs, const erfx_state& prev, const erfx_state& stage, const double* prim,
+                const double* pi_stage, const double* fc, const erfx_state& data, const erfx_state& scratch,
+                double dtau, double beta_s, double facinv, hipStream_t s) {
+    (void)nrk;
+    const Lay L = P.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long ne = L.ne, sy = L.sy, sz = L.sz;
+    const double beta_1 = 0.5 * (1.0 - beta_s);
+    const double beta_2 = 0.5 * (1.0 + beta_s);
+    const double beta_d = 0.1;
+    const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
+    const bool moist = P.moist;
+    const Zn z{P.z_nd, L};
+    const double* J = P.detJ;
+    double* cc0 = data.cons;
+    double* cc1 = data.cons + ne;
+    const double* pc0 = prev.cons;
+    const double* pc1 = prev.cons + ne;
+    const double* sc0 = stage.cons;
+    const double* sc1 = stage.cons + ne;
+    double* lagged1 = scratch.cons + ne;
+    double* du = T.du; double* dv = T.dv; double* dw = T.dw;
+    double* drho = T.drho; double* drt = T.drt;
+    double* nu = T.nu; double* nv = T.nv; double* om = T.om;
+    double* extrap = W.extrap;
+    double* trhs0 = W.trhs;
+    double* trhs1 = W.trhs + ne;
+    double* soln = W.soln;
+    const double* pxm = prev.xmom; const double* pym = prev.ymom; const double* pzm = prev.zmom;
+    const double* sxm = stage.xmom; const double* sym = stage.ymom; const double* szm = stage.zmom;
+    const bool first = step == 0;
+
+    // :139-204: rho, rho*theta copied from prev on the grown box (step 0); the
+    // increments, theta extrapolated forward and the lagged increment; the
+    // momentum increments on faces grown by one laterally (the top-of-box
+    // layer of du, dv copied into the layer above)
+    launch_box(Box(-1, -1, -1, nx + 1, ny + 1, nz), s, [=] __device__(int i, int j, int k) {
+        const long q = L.ix(i, j, k);
+        if (i <= nx && j <= ny) {
+            if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
+            double dr = cc0[q] - sc0[q];
+            double dt_ = cc1[q] - sc1[q];
+            drho[q] = dr;
+            drt[q] = dt_;
+            extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
+            lagged1[q] = dt_;
+        }
+        if (k >= 0 && j <= ny) {
+            if (k <= nz - 1) du[q] = pxm[q] - sxm[q];
+            else du[q] = pxm[q - sz] - sxm[q - sz];
+        }
+        if (k >= 0 && i <= nx) {
+            if (k <= nz - 1) dv[q] = pym[q] - sym[q];
+            else dv[q] = pym[q - sz] - sym[q - sz];
+        }
+        if (k >= 0 && i <= nx && j <= ny) dw[q] = pzm[q] - szm[q];
+    });
+
+    // :262-331 horizontal momenta with the terrain-corrected pressure gradient
+    const double* pq1 = prim + PRIM_Q1 * ne;
+    const double* pq2 = prim + PRIM_Q2 * ne;
+    const double* srx = slow_rhs.xmom;
+    const double* sry = slow_rhs.ymom;
+    double* axm = scratch.xmom;
+    double* aym = scratch.ymom;
+    double* dxm = data.xmom;
+    double* dym = data.ymom;
+    launch_box(Box(0, 0, 0, nx, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
+        const long q = L.ix(i, j, k);
+        if (j <= ny - 1) {
+            double met_h_xi = met::h_xi_Iface(z, i, j, k, dxi);
+            double met_h_zeta = met::h_zeta_Iface(z, i, j, k, dzi);
+            double gp_xi = (extrap[q] - extrap[q - 1]) * dxi;
+            double gp_zeta_on_iface = (k == 0)
+                ? 0.5 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1] - extrap[q])
+                : 0.25 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1 - sz] - extrap[q - sz]);
+            double gpx = gp_xi - (met_h_xi / met_h_zeta) * gp_zeta_on_iface;
+            gpx *= 1.0;
+            if (moist) gpx /= (1.0 + face_q(pq1, pq2, q, q - 1));
+            double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
+            double fast_rhs_rho_u = -Gamma * R_d * pi_c * gpx;
+            double n_ = du[q] + dtau * fast_rhs_rho_u + dtau * srx[q];
+            nu[q] = n_;
+            if (k == nz - 1) nu[q + sz] = n_;
+            axm[q] += facinv * n_;
+            dxm[q] = sxm[q] + n_;
+        }
+        if (i <= nx - 1) {
+            double met_h_eta = met::h_eta_Jface(z, i, j, k, dyi);
+            double met_h_zeta = met::h_zeta_Jface(z, i, j, k, dzi);
+            double gp_eta = (extrap[q] - extrap[q - sy]) * dyi;
+            double gp_zeta_on_jface = (k == 0)
+                ? 0.5 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q] - extrap[q - sy])
+                : 0.25 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q - sz] - extrap[q - sy - sz]);
+            double gpy = gp_eta - (met_h_eta / met_h_zeta) * gp_zeta_on_jface;
+            gpy *= 1.0;
+            if (moist) gpy /= (1.0 + face_q(pq1, pq2, q, q - sy));
+            double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
+            double fast_rhs_rho_v = -Gamma * R_d * pi_c * gpy;
+            double n_ = dv[q] + dtau * fast_rhs_rho_v + dtau * sry[q];
+            nv[q] = n_;
+            if (k == nz - 1) nv[q + sz] = n_;
+            aym[q] += facinv * n_;
+            dym[q] = sym[q] + n_;
+        }
+    });
+
+    // :414-457 horizontal divergence of the area-weighted increments (cells)
+    // and :460-484 Omega of the old increments (z-faces of the valid box)
+    const Fld fdu{du, L}, fdv{dv, L}, fnu{nu, L}, fnv{nv, L};
+    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz), s, [=] __device__(int i, int j, int k) {
+        const long q = L.ix(i, j, k);
+        if (k <= nz - 1) {
+            double h_zeta_cc_xface_hi = 0.5 * dzi * (z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1) - z(i + 1, j, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_xface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
+            double h_zeta_cc_yface_hi = 0.5 * dzi * (z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1) - z(i, j + 1, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_yface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
+            double xflux_lo = nu[q] * h_zeta_cc_xface_lo / 1.0;
+            double xflux_hi = nu[q + 1] * h_zeta_cc_xface_hi / 1.0;
+            double yflux_lo = nv[q] * h_zeta_cc_yface_lo / 1.0;
+            double yflux_hi = nv[q + sy] * h_zeta_cc_yface_hi / 1.0;
+            double mfsq = 1.0 * 1.0;
+            trhs0[q] = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq;
+            trhs1[q] = ((xflux_hi * (prim[q] + prim[q + 1]) - xflux_lo * (prim[q] + prim[q - 1])) * dxi * mfsq +
+                        (yflux_hi * (prim[q] + prim[q + sy]) - yflux_lo * (prim[q] + prim[q - sy])) * dyi * mfsq) * 0.5;
+        }
+        if (k == 0) om[q] = 0.;
+        else if (k == nz) om[q] = dw[q];
+        else om[q] = met::omega_from_w(i, j, k, dw[q], fdu, fdv, z, dxi, dyi);
+    });
+
+    // :499-669 per column: the tridiagonal right-hand side on the interior
+    // faces, the solve, rho*w (the stage value plus the increment converted
+    // from Omega on the interior faces; faces 0 and nz keep the stage value),
+    // the time-averaged Omega and the rho, rho*theta updates
+    const double halfg = fabs(0.5 * (-P.gravity));
+    const double* sr0 = slow_rhs.cons;
+    const double* sr1 = slow_rhs.cons + ne;
+    const double* srz = slow_rhs.zmom;
+    const double* fA = fc;
+    const double* fB = fc + ne;
+    const double* fC = fc + 2 * ne;
+    const double* fP = fc + 3 * ne;
+    const double* fQ = fc + 4 * ne;
+    double* cz = data.zmom;
+    double* azm = scratch.zmom;
+    launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
+        const long q0 = L.ix(i, j, 0);
+        // forward sweep with the right-hand side made on the fly
+        double s_prev = (dtau * srz[q0]) * fB[q0];
+        soln[q0] = s_prev;
+        for (int k = 1; k <= nz; ++k) {
+            const long q = q0 + k * sz;
+            double rhs;
+            if (k == nz) {
+                rhs = dtau * srz[q];
+            } else {
+                double detJ_on_kface = 0.5 * (J[q] + J[q - sz]);
+                double coeff_P = fP[q];
+                double coeff_Q = fQ[q];
+                if (moist) {
+                    double qf = 0.5 * (pq1[q] + pq1[q - sz] + pq2[q] + pq2[q - sz]);
+                    coeff_P /= (1.0 + qf);
+                    coeff_Q /= (1.0 + qf);
+                }
+                double theta_t_lo = 0.5 * (prim[q - 2 * sz] + prim[q - sz]);
+                double theta_t_mid = 0.5 * (prim[q - sz] + prim[q]);
+                double theta_t_hi = 0.5 * (prim[q] + prim[q + sz]);
+                double R0_tmp = (-halfg * drho[q] + coeff_P * drt[q]) * J[q]
+                              + (-halfg * drho[q - sz] + coeff_Q * drt[q - sz]) * J[q - sz];
+                double R1_tmp = -halfg * (sr0[q] * J[q] + sr0[q - sz] * J[q - sz])
+                              + (coeff_P * sr1[q] * J[q] + coeff_Q * sr1[q - sz] * J[q - sz]);
+                double Omega_kp1 = om[q + sz];
+                double Omega_k = om[q];
+                double Omega_km1 = om[q - sz];
+                R1_tmp += (halfg) * (beta_1 * dzi * (Omega_kp1 - Omega_km1) + trhs0[q] + trhs0[q - sz]);
+                R1_tmp += -(coeff_P * (beta_1 * dzi * (Omega_kp1 * theta_t_hi - Omega_k * theta_t_mid) + trhs1[q]) +
+                            coeff_Q * (beta_1 * dzi * (Omega_k * theta_t_mid - Omega_km1 * theta_t_lo) + trhs1[q - sz]));
+                rhs = detJ_on_kface * dw[q] + dtau * (detJ_on_kface * srz[q] + R0_tmp + dtau * beta_2 * R1_tmp);
+                rhs += detJ_on_kface * met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
+            }
+            s_prev = (rhs - fA[q] * s_prev) * fB[q];
+            soln[q] = s_prev;
+        }
+        for (int k = nz - 1; k >= 0; --k) {
+            const long q = q0 + k * sz;
+            double v = soln[q] - (fC[q] * fB[q]) * s_prev;
+            soln[q] = v;
+            s_prev = v;
+        }
+        cz[q0] = szm[q0];
+        cz[q0 + nz * sz] = szm[q0 + nz * sz];
+        for (int k = 1; k <= nz - 1; ++k) {
+            const long q = q0 + k * sz;
+            double wpp = met::w_from_omega(i, j, k, soln[q], fnu, fnv, z, dxi, dyi);
+            cz[q] = szm[q] + wpp;
+        }
+        for (int k = 0; k <= nz - 1; ++k) {
+            const long q = q0 + k * sz;
+            double zflux_lo = beta_2 * soln[q] + beta_1 * om[q];
+            double zflux_hi = beta_2 * soln[q + sz] + beta_1 * om[q + sz];
+            azm[q] += facinv * zflux_lo / (1.0 * 1.0);
+            if (k == nz - 1) azm[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
+            double fast_rhs_rho = -(trhs0[q] + (zflux_hi - zflux_lo) * dzi) / J[q];
+            cc0[q] += dtau * (sr0[q] + fast_rhs_rho);
+            double fast_rhs_rhotheta = -(trhs1[q] + 0.5 * (zflux_hi * (prim[q] + prim[q + sz]) -
+                                                           zflux_lo * (prim[q] + prim[q - sz])) * dzi) / J[q];
+            cc1[q] += dtau * (sr1[q] + fast_rhs_rhotheta);
+        }
+    });
+}
+
+}  // namespace erfx

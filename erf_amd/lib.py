@@ -35,10 +35,13 @@ SIGNATURES = {
     "erfx_ctx_layout": (_I, [_P, ctypes.POINTER(C.Layout)]),
     "erfx_set_base_state": (_I, [_P, _P, _P, _P]),
     "erfx_set_rayleigh": (_I, [_P, _P, _P, _P, _P]),
+    "erfx_set_terrain": (_I, [_P, _P]),
     "erfx_cons_to_prim": (_I, [_P, _P, _P, _P, _I]),
     "erfx_apply_bcs": (_I, [_P, ctypes.POINTER(State), _P, _P, _P, _I, _I, _I, _I]),
     "erfx_make_fast_coeffs": (_I, [_P, _P, ctypes.POINTER(State), _P, _P, _D, _D]),
     "erfx_fast_rhs_N": (_I, [_P, _I, _I, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
+                             _P, _P, _P, ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
+    "erfx_fast_rhs_T": (_I, [_P, _I, _I, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
                              _P, _P, _P, ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
     "erfx_turbulent_viscosity": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "erfx_slow_rhs_pre": (_I, [_P, _I, _D, ctypes.POINTER(State), ctypes.POINTER(State), _P,

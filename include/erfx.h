@@ -194,6 +194,14 @@ int erfx_ctx_layout(const erfx_ctx* ctx, erfx_layout* out);
  * The pointers are kept and read by later calls. */
 int erfx_set_base_state(erfx_ctx* ctx, const double* r0, const double* p0, const double* pi0);
 
+/* Node heights z_phys_nd of a terrain-following grid (use_terrain; the
+ * reference's z_phys_nd, ERF_TerrainMetrics.H:21 init_terrain_grid): a device
+ * field of the layout holding every node, ghosts included (periodic images on
+ * periodic axes), copied.  The metrics detJ, ax, ay, az and z_cc (make_J,
+ * make_areas, make_zcc; ERF_Utils.H:27-44) are made from it on the context's
+ * stream.  Until it is called the grid is flat (z = prob_lo[2] + k dz). */
+int erfx_set_terrain(erfx_ctx* ctx, const double* z_nd);
+
 /* Reference profiles of the Rayleigh damping (ERF's Rayleigh::ubar, vbar,
  * wbar, thetabar; ERF::initRayleigh, ERF_init_rayleigh.cpp:14-45): host arrays
  * of nz+1 values indexed by k (cell k for ubar, vbar and thetabar, z-face k for
@@ -220,6 +228,15 @@ int erfx_make_fast_coeffs(erfx_ctx* ctx, double* fast_coeffs, const erfx_state* 
  * one acoustic substep on a flat grid.  For step > 0 the reference passes
  * S_prev == S_data (same memory); pass the same pointers here. */
 int erfx_fast_rhs_N(erfx_ctx* ctx, int step, int nrk,
+                    const erfx_state* slow_rhs, const erfx_state* prev,
+                    const erfx_state* stage, const double* prim, const double* pi_stage,
+                    const double* fast_coeffs, erfx_state* data, erfx_state* scratch,
+                    double dtau, double beta_s, double facinv);
+
+/* erf_fast_rhs_T (ERF_fast_rhs_T.cpp:39-704; ERF_TI_fast_headers.H:47-71):
+ * one acoustic substep on a static terrain grid (use_terrain), with the
+ * arguments of erfx_fast_rhs_N. */
+int erfx_fast_rhs_T(erfx_ctx* ctx, int step, int nrk,
                     const erfx_state* slow_rhs, const erfx_state* prev,
                     const erfx_state* stage, const double* prim, const double* pi_stage,
                     const double* fast_coeffs, erfx_state* data, erfx_state* scratch,

@@ -208,6 +208,15 @@ int orc_fast_rhs_N(void* h, int step, int nrk, double dtau, double beta_s, doubl
                      L.A(L.fast_coeffs), L.A(L.S_sum), L.A(L.S_scratch), dtau, beta_s, facinv);
     });
 }
+int orc_fast_rhs_T(void* h, int step, int nrk, double dtau, double beta_s, double facinv) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] {
+        if (!L.terrain) throw std::runtime_error("erf_fast_rhs_T needs a terrain grid");
+        StateA prev = step == 0 ? L.A(L.S_old) : L.A(L.S_sum);
+        L.fast_rhs_T(step, nrk, L.A(L.F_slow), prev, L.A(L.S_new), L.A(L.prim), L.A(L.pi_stage),
+                     L.A(L.fast_coeffs), L.A(L.S_sum), L.A(L.S_scratch), dtau, beta_s, facinv);
+    });
+}
 int orc_turbulent_viscosity(void* h) {
     Level& L = *static_cast<Level*>(h);
     return guard([&] {

@@ -20,6 +20,7 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     rdOcp = R_d / c.c_p;
     gravity = c.use_gravity ? CONST_GRAV : 0.0;
     dxInv[0] = 1.0 / c.dx; dxInv[1] = 1.0 / c.dy; dxInv[2] = 1.0 / c.dz;
+    dz_min = c.dz;
     for (int d = 0; d < 3; ++d) periodic[d] = (c.bc[d] == ERFX_BC_PERIODIC);
     for (int ori = 0; ori < 6; ++ori) {
         int dir = ori % 3;

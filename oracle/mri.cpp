@@ -101,6 +101,7 @@ double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, l
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     Arr c = A(cons), u = A(xvel), v = A(yvel), w = A(zvel);
     const bool implicit = cfg.implicit_substepping != 0;
+    const double dzinv = 1.0 / dz_min;   // ERF_ComputeTimestep.cpp:66
     double comp_inv = -1.e100, lowM_inv = -1.e100;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
@@ -115,7 +116,7 @@ double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, l
                 double cs = std::sqrt(Gamma * pressure / rho);
                 double a = (std::abs(uc) + cs) * dxInv[0];
                 double b = (std::abs(vc) + cs) * dxInv[1];
-                double e = (std::abs(wc) + cs) * dxInv[2];
+                double e = (std::abs(wc) + cs) * dzinv;
                 if (implicit) {
                     if (nx > 1 && ny > 1) comp_inv = std::max({a, b, comp_inv});
                     else if (nx > 1) comp_inv = std::max(a, comp_inv);
@@ -126,7 +127,7 @@ double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, l
                     else if (ny > 1) comp_inv = std::max({b, e, comp_inv});
                     else comp_inv = std::max(e, comp_inv);
                 }
-                lowM_inv = std::max({std::abs(uc) * dxInv[0], std::abs(vc) * dxInv[1], std::abs(wc) * dxInv[2],
+                lowM_inv = std::max({std::abs(uc) * dxInv[0], std::abs(vc) * dxInv[1], std::abs(wc) * dzinv,
                                      lowM_inv});
             }
     double estdt_comp = cfl / comp_inv;

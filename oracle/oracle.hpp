@@ -277,6 +277,9 @@ struct Level {
     // and make_zcc (Source/Utils/ERF_TerrainMetrics.cpp:583-715); with a flat
     // grid they hold detJ = ax = ay = az = 1
     Field z_nd, detJ, ax, ay, az, z_cc;
+    // smallest cell height: dz, times the smallest detJ of the valid cells on
+    // a terrain grid (ERF.cpp:898-906), for the time-step estimate
+    double dz_min = 0.0;
     void set_terrain(const double* z_nodes);
     void fast_rhs_T(int step, int nrk, StateA slow_rhs, StateA prev, StateA stage, Arr prim_, Arr pi_st,
                     Arr fc, StateA data, StateA scratch, double dtau, double beta_s, double facinv);

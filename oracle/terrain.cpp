@@ -33,6 +33,9 @@ void Level::set_terrain(const double* z_nodes) {
     loop(Box(-1, -1, 0, nx, ny + 1, nz), [&](int i, int j, int k) {
         Ay(i, j, k) = .5 * dzInv * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
     });
+    double jmin = 1.e300;
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) { jmin = std::min(jmin, J(i, j, k)); });
+    dz_min = cfg.dz * jmin;
     terrain = true;
 }
 

@@ -48,6 +48,8 @@ def lib():
         l.orc_make_fast_coeffs.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         l.orc_fast_rhs_N.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        l.orc_fast_rhs_T.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double]
         l.orc_turbulent_viscosity.argtypes = [ctypes.c_void_p]
         l.orc_slow_rhs_pre.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         l.orc_slow_rhs_post.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]

@@ -35,6 +35,10 @@ def _setup(kind):
     elif kind == "norain":
         cfg, f = P.moist_bubble(24, 20, 16, seed=17, norain=True)
         dt = 1.0
+    elif kind.startswith("hill"):   # terrain-following grid over a Gaussian hill
+        adv = C.ADV_CENTERED2 if kind == "hill_c2" else C.ADV_UPWIND3
+        cfg, f = P.hill_flow(24, 20, 16, seed=7, adv=adv, moist=(kind == "hill_moist"))
+        dt = 1.0
     else:  # walls + molecular diffusion
         cfg, f = P.rising_thermal(24, 8, 16, seed=11, adv=C.ADV_UPWIND3)
         cfg.bc[0] = C.BC_SYMMETRY
@@ -49,6 +53,7 @@ def _setup(kind):
     sim = Sim(cfg)
     ids = list(ALL_IDS) + ([C.D_EDDY] if cfg.les_type != C.LES_NONE else [])
     ids += [C.D_RAIN_ACCUM] if cfg.moisture_type != C.MOIST_NONE else []
+    ids += TERRAIN_IDS if cfg.use_terrain else []
     for fid in ids:
         sim.upload(fid, orc.get(fid))
     return cfg, orc, sim, ids, dt
@@ -71,7 +76,9 @@ def _st(sim, s):
     return ctypes.byref(sim.state(s))
 
 
-KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain"]
+TERRAIN_IDS = [C.F_Z_ND, C.D_DETJ, C.D_AX, C.D_AY, C.D_AZ, C.D_Z_CC]
+HILLS = ["hill", "hill_c2", "hill_moist"]
+KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain"] + HILLS
 MOIST_BUOYANCY = ["moist1", "moist2", "moist3", "moist4"]
 
 
@@ -120,9 +127,10 @@ def test_fast_rhs_N(kind, step, nrk):
     dtau = dt / cfg.mri_ratio
     nsub = cfg.mri_ratio
     beta_s = 0.1 if cfg.implicit_substepping else -1.0
-    orc.call("orc_fast_rhs_N", step, nrk, dtau, beta_s, 1.0 / nsub)
+    T = "T" if cfg.use_terrain else "N"       # erf_fast_rhs_T on a terrain grid
+    orc.call("orc_fast_rhs_" + T, step, nrk, dtau, beta_s, 1.0 / nsub)
     prev = C.S_OLD if step == 0 else C.S_SUM
-    check(lib().erfx_fast_rhs_N(sim.ctx, step, nrk, _st(sim, C.F_SLOW), _st(sim, prev), _st(sim, C.S_NEW),
+    check(getattr(lib(), "erfx_fast_rhs_" + T)(sim.ctx, step, nrk, _st(sim, C.F_SLOW), _st(sim, prev), _st(sim, C.S_NEW),
                                 sim.ptr(C.D_PRIM), sim.ptr(C.D_PI_STAGE), sim.ptr(C.D_FAST_COEFFS),
                                 _st(sim, C.S_SUM), _st(sim, C.S_SCRATCH), dtau, beta_s, 1.0 / nsub))
     sim.sync()
@@ -170,7 +178,7 @@ def test_slow_rhs_post(kind, nrk):
     assert _same(sim, orc, ids) == {}
 
 
-@pytest.mark.parametrize("kind", ["moist1", "norain"])
+@pytest.mark.parametrize("kind", ["moist1", "norain", "hill_moist"])
 @pytest.mark.parametrize("dt", [1.0, 3.0])
 def test_kessler(kind, dt):
     from erf_amd.lib import lib, check
@@ -178,3 +186,27 @@ def test_kessler(kind, dt):
     orc.kessler(dt)
     check(lib().erfx_kessler_advance(sim.ctx, sim.ptr(C.F_CONS), dt, sim.ptr(C.D_RAIN_ACCUM)))
     assert _same(sim, orc, ids) == {}
+
+
+@pytest.mark.parametrize("h0", [0.0, 500.0])
+def test_terrain_metrics(h0):
+    """make_J / make_areas / make_zcc from uploaded node heights (the whole
+    layout, padding included) and the dz_min of the time-step estimate."""
+    import torch
+    assert torch.cuda.is_available()
+    from erf_amd.lib import Sim
+    cfg, f = P.hill_flow(24, 20, 16, h0=h0)
+    orc = Oracle(cfg)
+    orc.load(f)
+    sim = Sim(cfg)
+    sim.upload(C.F_Z_ND, f[C.F_Z_ND])
+    assert _same(sim, orc, TERRAIN_IDS) == {}
+
+
+def test_fast_rhs_N_rejects_terrain():
+    from erf_amd.lib import lib
+    cfg, orc, sim, ids, dt = _setup("hill")
+    rc = lib().erfx_fast_rhs_N(sim.ctx, 0, 0, _st(sim, C.F_SLOW), _st(sim, C.S_OLD), _st(sim, C.S_NEW),
+                               sim.ptr(C.D_PRIM), sim.ptr(C.D_PI_STAGE), sim.ptr(C.D_FAST_COEFFS),
+                               _st(sim, C.S_SUM), _st(sim, C.S_SCRATCH), 1.0 / 6, 0.1, 1.0)
+    assert rc != 0

@@ -171,3 +171,47 @@ def test_moist_kessler_step_exact(btype, norain):
 def test_moist_kessler_multistep():
     cfg, f = P.moist_bubble(24, 20, 16, seed=23)
     _compare(cfg, f, 1.0, 5, 1e-9)
+
+
+# ---- terrain-following grid (Gaussian hill; erf_fast_rhs_T) ----------------
+@pytest.mark.parametrize("adv", [C.ADV_CENTERED2, C.ADV_UPWIND3, C.ADV_UPWIND5])
+def test_terrain_hill_step_exact(adv):
+    cfg, f = P.hill_flow(24, 20, 16, seed=31, adv=adv)
+    _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
+
+
+def test_terrain_hill_multistep():
+    cfg, f = P.hill_flow(24, 20, 16, seed=33)
+    _compare(cfg, f, 1.0, 5, 1e-9)
+
+
+def test_terrain_hill_moist_kessler():
+    cfg, f = P.hill_flow(24, 20, 16, seed=35, moist=True)
+    _compare(cfg, f, 1.0, 3, 1e-9)
+
+
+def test_terrain_cfl_time_loop():
+    """ERF's time loop with the CFL estimate on a terrain grid: dz_min is dz
+    times the smallest detJ (ERF.cpp:898-906)."""
+    import ctypes
+    from oracle_lib import lib as olib
+    Sim = _sim()
+    cfg, f = P.hill_flow(24, 20, 16, seed=37, h0=800.0)
+    nsteps, cfl = 3, 0.8
+    sim = Sim(cfg)
+    sim.load(f)
+    t_gpu, n_gpu = sim.run(0.0, nsteps, C.dt_control(cfl=cfl))
+    orc = Oracle(cfg)
+    orc.load(f)
+    l = olib()
+    l.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                          ctypes.POINTER(ctypes.c_double)]
+    t_end = ctypes.c_double()
+    orc._chk(l.orc_run(orc.h, 0.0, nsteps, cfl, -1.0, -1.0, 0, 1.1, 1.0, 1.e100, ctypes.byref(t_end)))
+    assert n_gpu == nsteps and t_gpu == t_end.value, (t_gpu, t_end.value)
+    for fid in FIELDS:
+        a = valid(sim.download(fid), cfg, sim.L, fid)
+        b = valid(orc.get(fid), cfg, orc.L, fid)
+        assert int((a != b).sum()) == 0, fid
+    sim.close()
