@@ -74,9 +74,6 @@ static int validate(const erfx_config& c) {
     if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE)) {
         set_error("diffusion on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
     }
-    if (c.use_terrain && (c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T)) {
-        set_error("Rayleigh damping on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
-    }
     if (c.moisture_type == ERFX_MOIST_KESSLER_NORAIN && c.buoyancy_type != 1) {
         // asserted by ERF (ERF_make_buoyancy.cpp:171-175)
         set_error("Kessler_NoRain requires buoyancy_type 1"); return ERFX_ERR_ARG;
@@ -132,6 +129,7 @@ struct erfx_ctx {
     // terrain-following grid: node heights and metrics (the Params pointers
     // refer to these) and the work fields of erf_fast_rhs_T
     double *z_nd = nullptr, *detJ = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *z_cc = nullptr;
+    double* ray_pt = nullptr;   // per-point Rayleigh factors on a terrain grid (RPT_* blocks)
     TerrainWork tw{};
     // learned microphysics work space (allocated on first use): per-cell slot,
     // network rows, their cells, two device counters and their pinned host copy
@@ -308,6 +306,44 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
     return ERFX_OK;
 }
 
+// Per-point Rayleigh factors of a terrain grid: the reference evaluates the
+// damping at each cell's z_cc and each z-node's height, with ztop the height
+// of node (0, 0, nz) (ERF_make_sources.cpp:203-225, ERF_make_mom_sources.cpp:247-296,
+// ERF_init_rayleigh.cpp:17); computed here on the host with the C library's sin.
+static int build_ray_points(erfx_ctx* c) {
+    if (!c->P.terrain || !c->ray) return ERFX_OK;
+    const Lay L = c->P.L;
+    const size_t ne = (size_t)L.ne;
+    std::vector<double> zc(ne), zn(ne), h((size_t)RPT_NFIELDS * ne, 0.0);
+    ERFX_HIP(hipMemcpyAsync(zc.data(), c->z_cc, ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ERFX_HIP(hipMemcpyAsync(zn.data(), c->z_nd, ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ERFX_HIP(hipStreamSynchronize(c->stream));
+    const erfx_config& cf = c->P.cfg;
+    const double ztop = zn[(size_t)L.ix(0, 0, L.nz)];
+    const double zdamp = cf.rayleigh_zdamp, dampcoef = cf.rayleigh_dampcoef;
+    for (size_t q = 0; q < ne; ++q) {
+        double zfrac = 1 - (ztop - zc[q]) / zdamp;
+        if (zfrac > 0) {
+            double sinefac = std::sin(PIoTwo * zfrac);
+            h[RPT_FAC_CC * ne + q] = dampcoef * sinefac * sinefac;
+            h[RPT_ACT_CC * ne + q] = 1.0;
+        }
+        double zfrac_n = 1 - (ztop - zn[q]) / zdamp;
+        if (zfrac_n > 0) {
+            double sinefac = std::sin(PIoTwo * zfrac_n);
+            h[RPT_FAC_ND * ne + q] = dampcoef * sinefac * sinefac;
+            h[RPT_ACT_ND * ne + q] = 1.0;
+        }
+    }
+    if (!c->ray_pt) {
+        int rc = dalloc(c->allocs, &c->ray_pt, h.size());
+        if (rc) return rc;
+    }
+    ERFX_HIP(hipMemcpy(c->ray_pt, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->P.ray_pt = c->ray_pt;
+    return ERFX_OK;
+}
+
 int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
     if (!c || !z_nd) { set_error("null argument"); return ERFX_ERR_ARG; }
     if (!c->P.terrain) { set_error("the context was made without use_terrain"); return ERFX_ERR_STATE; }
@@ -327,7 +363,7 @@ int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
         for (int j = 0; j < L.ny; ++j)
             for (int i = 0; i < L.nx; ++i) jmin = std::min(jmin, J[(size_t)L.ix(i, j, k)]);
     c->P.dz_min = c->P.cfg.dz * jmin;
-    return ERFX_OK;
+    return build_ray_points(c);
 }
 
 int erfx_ctx_destroy(erfx_ctx* c) {
@@ -392,7 +428,7 @@ int erfx_set_rayleigh(erfx_ctx* c, const double* ub, const double* vb, const dou
     }
     ERFX_HIP(hipMemcpy(c->ray, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     c->P.ray = c->ray;
-    return ERFX_OK;
+    return build_ray_points(c);
 }
 
 static bool rayleigh_on(const erfx_config& c) {

@@ -178,6 +178,9 @@ struct Params {
     const double* az = nullptr;
     const double* z_cc = nullptr;
     double dz_min = 0.0;      // dz, times the smallest detJ of the valid cells on a terrain grid
+    // terrain grid with Rayleigh damping: factors and flags per point (the
+    // heights vary along a level), RPT_* blocks of ne values
+    const double* ray_pt = nullptr;
 };
 
 // prim components of the moisture species (prim(n) = cons(n+1) / rho)
@@ -191,6 +194,28 @@ __device__ __forceinline__ double face_q(const double* pq1, const double* pq2, l
 
 // blocks of the Rayleigh table
 enum { RAY_UBAR = 0, RAY_VBAR, RAY_WBAR, RAY_THETABAR, RAY_FAC_CC, RAY_FAC_ND, RAY_ACT_CC, RAY_ACT_ND, RAY_NFIELDS };
+enum { RPT_FAC_CC = 0, RPT_FAC_ND, RPT_ACT_CC, RPT_ACT_ND, RPT_NFIELDS };
+
+// Rayleigh factor and activity at cell / z-node height of point q on level k:
+// from the per-level table on a flat grid, per point on a terrain grid
+struct RayAt {
+    const double* ray;
+    const double* pt;
+    int n1;
+    long ne;
+    __device__ __forceinline__ bool act_cc(long q, int k) const {
+        return pt ? pt[RPT_ACT_CC * ne + q] != 0.0 : ray[RAY_ACT_CC * n1 + k] != 0.0;
+    }
+    __device__ __forceinline__ double fac_cc(long q, int k) const {
+        return pt ? pt[RPT_FAC_CC * ne + q] : ray[RAY_FAC_CC * n1 + k];
+    }
+    __device__ __forceinline__ bool act_nd(long q, int k) const {
+        return pt ? pt[RPT_ACT_ND * ne + q] != 0.0 : ray[RAY_ACT_ND * n1 + k] != 0.0;
+    }
+    __device__ __forceinline__ double fac_nd(long q, int k) const {
+        return pt ? pt[RPT_FAC_ND * ne + q] : ray[RAY_FAC_ND * n1 + k];
+    }
+};
 
 void make_params(const erfx_config& cfg, const erfx_layout& lay, Params& P);
 

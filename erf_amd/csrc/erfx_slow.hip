@@ -627,6 +627,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const bool rayT = P.cfg.rayleigh_damp_T && ray;
     const bool rayU = P.cfg.rayleigh_damp_U && ray, rayV = P.cfg.rayleigh_damp_V && ray;
     const bool rayW = P.cfg.rayleigh_damp_W && ray;
+    const RayAt RA{ray, P.terrain ? P.ray_pt : nullptr, n1, ne};
     const double* ru = data.xmom;
     const double* rv = data.ymom;
     const double* rw = data.zmom;
@@ -682,6 +683,10 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             double invdetJ = (dJ > 0.) ? 1. / dJ : 1.;
             double r1 = -invdetJ * mfsq * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
             double src = 0.0;
+            if (rayT && RA.act_cc(q, k)) {   // ERF_make_sources.cpp:203-225 with z_cc
+                double theta = prim[q];
+                src -= RA.fac_cc(q, k) * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
+            }
             r1 += src;
             cr1[q] = r1;
         });
@@ -720,9 +725,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             // cell source of rho*theta (make_sources: Rayleigh damping, ERF_make_sources.cpp:203-225),
             // added the way ERF adds cc_src (ERF_slow_rhs_pre.cpp:521-526)
             double src = 0.0;
-            if (rayT && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+            if (rayT && RA.act_cc(q, k)) {
                 double theta = prim[q];
-                src -= ray[RAY_FAC_CC * n1 + k] * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
+                src -= RA.fac_cc(q, k) * (theta - ray[RAY_THETABAR * n1 + k]) * data.cons[q];
             }
             r1 += src;
             cr1[q] = r1;
@@ -806,10 +811,10 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 double rho_w_loc = 0.25 * (rw[q + sz] + rw[q] + rw[q - sy + sz] + rw[q - sy]);
                 xs += cf * (rho_v_loc * sinphi - rho_w_loc * cosphi);
             }
-            if (rayU && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+            if (rayU && RA.act_cc(q, k)) {
                 double rho_on_u_face = 0.5 * (cons[q] + cons[q - 1]);
                 double uu = ru[q] / rho_on_u_face;
-                xs -= ray[RAY_FAC_CC * n1 + k] * (uu - ray[RAY_UBAR * n1 + k]) * rho_on_u_face;
+                xs -= RA.fac_cc(q, k) * (uu - ray[RAY_UBAR * n1 + k]) * rho_on_u_face;
             }
             xs += 0.5 * (cons[q] + cons[q - 1]) * g0;
             double gpx = dxi * (pp[q] - pp[q - 1]);
@@ -871,10 +876,10 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 double rho_u_loc = 0.25 * (ru[q + 1] + ru[q] + ru[q + 1 - sy] + ru[q - sy]);
                 ys += -cf * rho_u_loc * sinphi;
             }
-            if (rayV && ray[RAY_ACT_CC * n1 + k] != 0.0) {
+            if (rayV && RA.act_cc(q, k)) {
                 double rho_on_v_face = 0.5 * (cons[q] + cons[q - sy]);
                 double vv = rv[q] / rho_on_v_face;
-                ys -= ray[RAY_FAC_CC * n1 + k] * (vv - ray[RAY_VBAR * n1 + k]) * rho_on_v_face;
+                ys -= RA.fac_cc(q, k) * (vv - ray[RAY_VBAR * n1 + k]) * rho_on_v_face;
             }
             ys += 0.5 * (cons[q] + cons[q - sy]) * g1;
             double gpy = dyi * (pp[q] - pp[q - sy]);
@@ -988,10 +993,10 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 double rho_u_loc = 0.25 * (ru[q + 1] + ru[q] + ru[q + 1 - sz] + ru[q - sz]);
                 zs += cf * rho_u_loc * cosphi;
             }
-            if (rayW && ray[RAY_ACT_ND * n1 + k] != 0.0) {
+            if (rayW && RA.act_nd(q, k)) {
                 double rho_on_w_face = 0.5 * (cons[q] + cons[q - sz]);
                 double ww = rw[q] / rho_on_w_face;
-                zs -= ray[RAY_FAC_ND * n1 + k] * (ww - ray[RAY_WBAR * n1 + k]) * rho_on_w_face;
+                zs -= RA.fac_nd(q, k) * (ww - ray[RAY_WBAR * n1 + k]) * rho_on_w_face;
             }
             zs += 0.5 * (cons[q] + cons[q - sz]) * g2;
             double met_h_zeta = terr ? met::h_zeta_Kface(zt, i, j, k, dzi) : 1;

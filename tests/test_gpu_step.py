@@ -215,3 +215,15 @@ def test_terrain_cfl_time_loop():
         b = valid(orc.get(fid), cfg, orc.L, fid)
         assert int((a != b).sum()) == 0, fid
     sim.close()
+
+
+def test_terrain_rayleigh_damping_layer():
+    """Rayleigh damping over the hill: factors from each cell's and node's
+    true height (per point on the GPU)."""
+    cfg, f = P.hill_flow(24, 20, 16, seed=39)
+    cfg.rayleigh_damp_U = cfg.rayleigh_damp_V = cfg.rayleigh_damp_W = cfg.rayleigh_damp_T = 1
+    cfg.rayleigh_zdamp = 4000.0
+    cfg.rayleigh_dampcoef = 0.05
+    n1 = cfg.nz + 1
+    prof = (np.full(n1, 9.0), np.full(n1, -0.2), np.zeros(n1), np.full(n1, 300.4))
+    _compare(cfg, f, 1.0, 2, 1e-11, exact=True, rayleigh=prof)

@@ -54,12 +54,18 @@ def test_metrics_closed_form():
     assert np.abs((J * cfg.dz).sum(axis=0) - (ztop - h4)).max() < 1e-9
 
 
-@pytest.mark.parametrize("adv", [C.ADV_CENTERED2, C.ADV_UPWIND3, C.ADV_WENO5])
-def test_flat_stretched_equals_flat(adv):
-    """z = 2 zeta in terrain form is the plain grid with 2 dz."""
+@pytest.mark.parametrize("adv,rayleigh", [(C.ADV_CENTERED2, False), (C.ADV_UPWIND3, False), (C.ADV_WENO5, False),
+                                          (C.ADV_UPWIND3, True)])
+def test_flat_stretched_equals_flat(adv, rayleigh):
+    """z = 2 zeta in terrain form is the plain grid with 2 dz (with a Rayleigh
+    damping layer: its heights come from z_cc and the nodes on the terrain grid)."""
     cfgA, fA = P.rising_thermal(16, 12, 16, adv=adv)
     fA[C.F_XVEL][0] += 5.0
     fA[C.F_YVEL][0] -= 3.0
+    if rayleigh:
+        cfgA.rayleigh_damp_U = cfgA.rayleigh_damp_V = cfgA.rayleigh_damp_W = cfgA.rayleigh_damp_T = 1
+        cfgA.rayleigh_zdamp = 4000.0
+        cfgA.rayleigh_dampcoef = 0.05
     cfgB = copy.deepcopy(cfgA)
     cfgB.dz = cfgA.dz / 2
     cfgB.use_terrain = 1
@@ -71,6 +77,11 @@ def test_flat_stretched_equals_flat(adv):
     oA, oB = Oracle(cfgA), Oracle(cfgB)
     oA.load(fA)
     oB.load(fB)
+    if rayleigh:
+        n1 = cfgA.nz + 1
+        prof = (np.full(n1, 4.0), np.full(n1, -2.0), np.zeros(n1), np.full(n1, 300.4))
+        oA.set_rayleigh(*prof)
+        oB.set_rayleigh(*prof)
     assert np.all(_valid(cfgB, L, oB.get(C.D_DETJ)[0]) == 2.0)
     oA.step(0.0, 1.0, 4)
     oB.step(0.0, 1.0, 4)
