@@ -211,26 +211,57 @@ def moist_bubble(nx=24, ny=20, nz=16, seed=0, buoyancy_type=None, norain=False, 
     return cfg, f
 
 
-def terrain_nodes(cfg, L, height):
-    """Node heights of a terrain-following grid (Gal-Chen / basic terrain
-    following): z = zeta + h(x, y) (1 - zeta / ztop), zeta = k dz, on every node
-    of the layout.  `height(x, y)` gives the surface; lateral ghost nodes are
-    periodic images on periodic axes, and the vertical ghost nodes continue
-    the columns linearly."""
+def z_levels_uniform(nz, dz, zsurf=0.0):
+    """init_zlevels without stretching (ERF_TerrainMetrics.cpp:27-33): k dz."""
+    return zsurf + np.arange(nz + 1) * dz
+
+
+def z_levels_stretched(nz, dz0, ratio, zsurf=0.0, dz_max=None):
+    """init_zlevels with grid_stretching_ratio (ERF_TerrainMetrics.cpp:34-44):
+    z_0 = zsurf, z_k = z_{k-1} + dz, dz *= ratio after each level; `dz_max`
+    caps the spacing (an explicit terrain_z_levels list of that shape)."""
+    z = np.zeros(nz + 1)
+    z[0] = zsurf
+    dz = dz0
+    for k in range(1, nz + 1):
+        z[k] = z[k - 1] + dz
+        dz *= ratio
+        if dz_max is not None and dz > dz_max:
+            dz = dz_max
+    return z
+
+
+def terrain_nodes(cfg, L, height, z_levels=None):
+    """Node heights z_phys_nd of the BTF terrain-following grid
+    (init_which_terrain_grid case 0, ERF_TerrainMetrics.cpp:256-336; Klemp 2011):
+    z = ((h - z_lev[0]) z_top + (z_top - h) z_lev[k]) / (z_top - z_lev[0]) on
+    nodes k = 0..nz, with h the surface height `height(x, y)` at the node's
+    column; z(-1) = 2 z(0) - z(1).  Lateral ghost nodes are periodic images on
+    periodic axes and copies of the edge column otherwise (the reference's
+    clamped index); the vertical ghost nodes the reference leaves unset
+    continue the columns linearly (nothing reads them).  `z_levels` defaults
+    to k dz."""
     ng = L.ng
     nzt, nyt, nxt = L.shape3()
-    ztop = cfg.nz * cfg.dz
+    nz = cfg.nz
+    zl = z_levels_uniform(nz, cfg.dz, cfg.prob_lo[2]) if z_levels is None else np.asarray(z_levels, float)
+    assert zl.size == nz + 1
     ii = np.arange(nxt) - ng
     jj = np.arange(nyt) - ng
-    if cfg.bc[0] == C.BC_PERIODIC:
-        ii = np.mod(ii, cfg.nx)
-    if cfg.bc[1] == C.BC_PERIODIC:
-        jj = np.mod(jj, cfg.ny)
+    ii = np.mod(ii, cfg.nx) if cfg.bc[0] == C.BC_PERIODIC else np.clip(ii, 0, cfg.nx)
+    jj = np.mod(jj, cfg.ny) if cfg.bc[1] == C.BC_PERIODIC else np.clip(jj, 0, cfg.ny)
     x = cfg.prob_lo[0] + ii * cfg.dx
     y = cfg.prob_lo[1] + jj * cfg.dy
-    h = height(x[None, :], y[:, None])                       # (ny_t, nx_t)
-    zeta = cfg.prob_lo[2] + (np.arange(nzt) - ng) * cfg.dz    # linear in k, ghosts included
-    z = zeta[:, None, None] + h[None, :, :] * (1.0 - zeta[:, None, None] / ztop)
+    h = height(x[None, :], y[:, None]) * np.ones((nyt, nxt))   # (ny_t, nx_t)
+    z_top, z_lev_sfc = zl[-1], zl[0]
+    z = np.zeros((nzt, nyt, nxt))
+    for k in range(nz + 1):
+        z[ng + k] = ((h - z_lev_sfc) * z_top + (z_top - h) * zl[k]) / (z_top - z_lev_sfc)
+    z[ng - 1] = 2.0 * z[ng] - z[ng + 1]
+    for kk in range(ng - 2, -1, -1):
+        z[kk] = 2.0 * z[kk + 1] - z[kk + 2]
+    for kk in range(ng + nz + 1, nzt):
+        z[kk] = 2.0 * z[kk - 1] - z[kk - 2]
     return z[None]
 
 
@@ -269,7 +300,8 @@ def hse_terrain(zc, zs, ps, theta, g=C.CONST_GRAV, rdOcp=C.R_d / C.Cp_d):
 
 
 def hill_flow(nx=24, ny=16, nz=16, seed=0, h0=500.0, a=2000.0, u0=10.0, v0=0.0, moist=False,
-              adv=C.ADV_UPWIND3, lx=20000.0, ly=16000.0, noise=0.01, bubble=(0.3, 0.5), n2=0.0):
+              adv=C.ADV_UPWIND3, lx=20000.0, ly=16000.0, noise=0.01, bubble=(0.3, 0.5), n2=0.0,
+              ztop=10000.0, z_levels=None, theta_noise=0.0, scal_adv=None):
     """Flow over a Gaussian hill on a terrain-following grid: lx x ly x 10 km
     computational box, periodic laterally, slip walls; hill of height h0 and
     half-width a at the domain centre; isentropic theta = 300 K base state at
@@ -277,17 +309,22 @@ def hill_flow(nx=24, ny=16, nz=16, seed=0, h0=500.0, a=2000.0, u0=10.0, v0=0.0, 
     +1 K bubble at 3 km (at fractions `bubble` of the box, None for none).
     n2 > 0 gives the initial state a stable stratification of
     Brunt-Vaisala frequency squared n2 in hydrostatic balance (the base state
-    stays isentropic).  With `moist`, Kessler and a humid layer."""
-    cfg = C.default_config(nx, ny, nz, lx / nx, ly / ny, 10000.0 / nz)
+    stays isentropic).  With `moist`, Kessler and a humid layer.  The
+    computational box is ztop high (dz = ztop / nz); `z_levels` (nz+1 node
+    levels, default k dz) stretch the grid; theta_noise adds seeded uniform
+    +-theta_noise K; `scal_adv` picks the scalar scheme (default: adv)."""
+    cfg = C.default_config(nx, ny, nz, lx / nx, ly / ny, ztop / nz)
     cfg.use_terrain = 1
     cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
-    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    sa = adv if scal_adv is None else scal_adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = sa
     cfg.mri_ratio = 6
     if moist:
         C.set_moisture(cfg, C.MOIST_KESSLER, 1)
+        cfg.moistscal_horiz_adv = cfg.moistscal_vert_adv = sa if scal_adv is not None else cfg.moistscal_horiz_adv
     L, f = empty_fields(cfg)
     xc, yc = 0.5 * lx, 0.5 * ly
-    zn = terrain_nodes(cfg, L, lambda x, y: h0 * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / a ** 2))
+    zn = terrain_nodes(cfg, L, lambda x, y: h0 * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / a ** 2), z_levels)
     f[C.F_Z_ND] = zn
     z = zn[0]
     ng = L.ng
@@ -314,6 +351,9 @@ def hill_flow(nx=24, ny=16, nz=16, seed=0, h0=500.0, a=2000.0, u0=10.0, v0=0.0, 
         pr = np.zeros_like(zc)
         pis = 1.0 + g * g / (cfg.c_p * 300.0 * n2) * (np.exp(-n2 * zs / g) - 1.0)
         pr[kk] = hse_terrain(zc[kk], zs, C.p_0 * np.power(pis, 1.0 / rdOcp), theta[kk], g, rdOcp)[1]
+    rng = np.random.default_rng(seed)
+    if theta_noise > 0:
+        theta = theta + rng.uniform(-theta_noise, theta_noise, theta.shape)
     if bubble is not None:
         Lr = np.sqrt(((ix[None, None, :] - bubble[0] * lx) / 2000.0) ** 2 +
                      ((jy[None, :, None] - bubble[1] * ly) / 2000.0) ** 2 + ((zc - 3000.0) / 1500.0) ** 2)
@@ -332,9 +372,48 @@ def hill_flow(nx=24, ny=16, nz=16, seed=0, h0=500.0, a=2000.0, u0=10.0, v0=0.0, 
     cons[C.RHO][sl] = rt[sl] / theta[sl]
     if moist:
         cons[C.RHOQ1][sl] = (cons[C.RHO] * qv)[sl]
-    rng = np.random.default_rng(seed)
     _noise_vel(cfg, L, f, rng, noise, (u0, v0, 0.0))
     return cfg, f
+
+
+def moist_terrain(nx=512, ny=512, nz=96, seed=0, h0=500.0, a=2000.0):
+    """Moist flow over terrain with Kessler on the terrain-following path:
+    dx = dy = 200 m, levels stretched from 20 m at the surface to 400 m
+    (geometric growth capped at 400 m) with the top at 15 km, Gaussian hill
+    h0 = 500 m, a = 2 km at the centre; u = 10 m/s, N = 0.01 1/s, RH 90 %
+    below 2 km decaying with a 1 km e-fold scale above, seeded +-0.1 K theta
+    noise; WENO5 scalars (dry and moist), 3rd-order upwind momentum, Rayleigh
+    damping above 12 km towards the initial wind, dt = 1 s, 6 substeps.
+    Laterally periodic (the inflow is the periodic image of the outflow).
+    The nz+1 levels reach 15 km exactly: the growth ratio is solved for."""
+    lo, hi = 1.0, 1.2
+    for _ in range(200):   # bisection on the growth ratio for the 15 km top
+        r = 0.5 * (lo + hi)
+        top = z_levels_stretched(nz, 20.0, r, dz_max=400.0)[-1]
+        lo, hi = (r, hi) if top < 15000.0 else (lo, r)
+    zl = z_levels_stretched(nz, 20.0, 0.5 * (lo + hi), dz_max=400.0)
+    zl = zl * (15000.0 / zl[-1])
+    cfg, f = hill_flow(nx, ny, nz, seed=seed, h0=h0, a=a, u0=10.0, moist=True, adv=C.ADV_UPWIND3,
+                       lx=200.0 * nx, ly=200.0 * ny, noise=0.0, bubble=None, n2=1e-4, ztop=15000.0,
+                       z_levels=zl, theta_noise=0.1, scal_adv=C.ADV_WENO5)
+    cfg.rayleigh_damp_U = cfg.rayleigh_damp_V = cfg.rayleigh_damp_W = cfg.rayleigh_damp_T = 1
+    cfg.rayleigh_zdamp = 3000.0
+    cfg.rayleigh_dampcoef = 0.2
+    return cfg, f
+
+
+def rayleigh_profiles_uniform(cfg, u=10.0, v=0.0, theta=None, f=None):
+    """Rayleigh reference profiles: the uniform wind, w = 0, and theta of the
+    initial state's column-mean per level (k = 0..nz, the top copies nz-1)."""
+    n1 = cfg.nz + 1
+    if theta is None:
+        L = C.layout_for(cfg)
+        ng = L.ng
+        c = f[C.F_CONS]
+        th = (c[C.RHOTHETA] / np.where(c[C.RHO] > 0, c[C.RHO], 1.0))[ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
+        tm = th.mean(axis=(1, 2))
+        theta = np.concatenate([tm, tm[-1:]])
+    return np.full(n1, u), np.full(n1, v), np.zeros(n1), np.asarray(theta, float)
 
 
 def _noise_vel(cfg, L, f, rng, amp, mean):

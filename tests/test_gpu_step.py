@@ -227,3 +227,10 @@ def test_terrain_rayleigh_damping_layer():
     n1 = cfg.nz + 1
     prof = (np.full(n1, 9.0), np.full(n1, -0.2), np.zeros(n1), np.full(n1, 300.4))
     _compare(cfg, f, 1.0, 2, 1e-11, exact=True, rayleigh=prof)
+
+
+def test_moist_terrain_case_exact():
+    """The moist terrain case at a reduced size (stretched levels, WENO5 dry and
+    moist scalars, U3 momentum, Kessler, Rayleigh layer): bit-identical steps."""
+    cfg, f = P.moist_terrain(24, 20, 16, seed=41)
+    _compare(cfg, f, 1.0, 2, 1e-12, exact=True, rayleigh=P.rayleigh_profiles_uniform(cfg, f=f))

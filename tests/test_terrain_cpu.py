@@ -225,3 +225,42 @@ def test_terrain_with_diffusion_rejected():
     cfg.molec_diff_type = C.MOLEC_CONSTANT
     with pytest.raises(RuntimeError):
         Oracle(cfg)
+
+
+def test_stretched_levels_and_btf_grid():
+    """init_zlevels with a stretching ratio (capped) and the BTF grid: the
+    stretched column over flat ground is the level list itself, and over the
+    hill the lid stays flat at z_top."""
+    zl = P.z_levels_stretched(96, 20.0, 1.035883433721025, dz_max=400.0)
+    dz = np.diff(zl)
+    assert dz[0] == 20.0 and dz.max() == 400.0 and abs(zl[-1] - 15000.0) < 1e-6
+    assert np.all(np.diff(dz) >= 0.0)
+    cfg, f = P.moist_terrain(24, 20, 16, a=500.0)     # a narrow hill: flat corners
+    L = C.layout_for(cfg)
+    ng, nz = L.ng, cfg.nz
+    z = f[C.F_Z_ND][0]
+    assert np.allclose(z[ng + nz], 15000.0, rtol=0, atol=1e-9)          # flat lid
+    corner = z[ng:ng + nz + 1, ng, ng]                                   # far from the hill: h ~ 0
+    assert corner[0] < 1e-3 and np.all(np.diff(corner) > 0)
+    assert z[ng, ng + cfg.ny // 2, ng + cfg.nx // 2] > 400.0             # hill top near 500 m
+    assert np.all(z[ng - 1] == 2.0 * z[ng] - z[ng + 1])                 # k = -1 extrapolated
+
+
+def test_moist_terrain_case_runs():
+    """The moist terrain case (stretched levels, WENO5 scalars, Kessler,
+    Rayleigh layer) at a reduced size: finite, mass of dry air conserved,
+    water species non-negative, rain reaching the ground is non-negative."""
+    cfg, f = P.moist_terrain(24, 20, 16)
+    o = Oracle(cfg)
+    o.load(f)
+    o.set_rayleigh(*P.rayleigh_profiles_uniform(cfg, f=f))
+    L = C.layout_for(cfg)
+    m0 = _integrals(o, cfg, L)
+    o.step(0.0, 1.0, 3)
+    m1 = _integrals(o, cfg, L)
+    assert abs(m1[C.RHO] - m0[C.RHO]) <= 1e-13 * abs(m0[C.RHO])
+    c = _valid(cfg, L, o.get(C.F_CONS))
+    assert np.all(np.isfinite(c))
+    for n in (C.RHOQ1, C.RHOQ2, C.RHOQ3):
+        assert c[n].min() >= 0.0
+    assert o.get(C.D_RAIN_ACCUM).min() >= 0.0
