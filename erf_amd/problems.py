@@ -282,7 +282,7 @@ def hse_terrain(zc, zs, ps, theta, g=C.CONST_GRAV, rdOcp=C.R_d / C.Cp_d):
         th = theta[k]
         rk = np.full_like(p_ref, 1.0) if r_ref is None else r_ref.copy()
         rr = 0.0 if r_ref is None else r_ref
-        for _ in range(30):
+        for _ in range(15):   # quadratic convergence: ~6 iterations reach rounding
             p_hse = p_ref - g * (w_ref * rr + w_self * rk)
             A = p_hse - _p_of_rt(rk * th)
             dpdr = C.Gamma * C.p_0 * np.power(C.R_d * th / C.p_0, C.Gamma) * np.power(rk, C.Gamma - 1.0)
