@@ -288,7 +288,7 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
     }
     if (c->P.terrain) {
         double** tb[] = {&c->z_nd, &c->detJ, &c->ax, &c->ay, &c->az, &c->z_cc, &c->sw.omega, &c->tw.du, &c->tw.dv,
-                         &c->tw.dw, &c->tw.drho, &c->tw.drt, &c->tw.nu, &c->tw.nv, &c->tw.om};
+                         &c->tw.nu, &c->tw.nv, &c->tw.om};
         for (double** b : tb)
             if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
         c->P.z_nd = c->z_nd; c->P.detJ = c->detJ; c->P.ax = c->ax; c->P.ay = c->ay; c->P.az = c->az;
@@ -885,18 +885,23 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     velocity_to_momentum(P, uo, vo, wo, P.ng_vel, P.ng_vel, cons_old, So.xmom, So.ymom, So.zmom, st);
     // advance_dycore
     if (P.use_turb) turbulent_viscosity(P, c->sw, cons_old, uo, vo, wo, s->eddy, st);
-    ERFX_HIP(hipMemcpyAsync(un, uo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    ERFX_HIP(hipMemcpyAsync(vn, vo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    ERFX_HIP(hipMemcpyAsync(wn, wo, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    {
+        CopyList cl;
+        cl.add(un, uo, (long)ne); cl.add(vn, vo, (long)ne); cl.add(wn, wo, (long)ne);
+        copy_fields(cl, st);
+    }
     apply_bcs(P, So, un, vn, wn, P.ng_cons, P.ng_vel, false, true, st);
     cons_to_prim(P, So.cons, s->prim, s->pi_stage, P.ng_cons, st);
     // MRI: S_new <- S_old (ERF_MRI.H:225-230)
-    ERFX_HIP(hipMemcpyAsync(Sn.cons, So.cons, nc * ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    ERFX_HIP(hipMemcpyAsync(Sn.xmom, So.xmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    ERFX_HIP(hipMemcpyAsync(Sn.ymom, So.ymom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
-    ERFX_HIP(hipMemcpyAsync(Sn.zmom, So.zmom, ne * sizeof(double), hipMemcpyDeviceToDevice, st));
+    {
+        CopyList cl;
+        cl.add(Sn.cons, So.cons, (long)(nc * ne)); cl.add(Sn.xmom, So.xmom, (long)ne);
+        cl.add(Sn.ymom, So.ymom, (long)ne); cl.add(Sn.zmom, So.zmom, (long)ne);
+        copy_fields(cl, st);
+    }
     const double sub_timestep = dt / ratio;
-    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR;
+    // the substep chain folds apply_bcs into erf_fast_rhs_N's kernels (flat grids only)
+    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !P.terrain;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
     double time_stage = time;
     for (int nrk = 0; nrk < 3; ++nrk) {

@@ -286,8 +286,6 @@ void velocity_to_momentum(const Params& P, const double* xv, const double* yv, c
 //       on the FillBoundary layers.
 // Points the sequence does not touch are not written.
 bool fast_bcs_fusable(const Params& P) {
-    // the terrain wall value of w reads the filled u, v: kept in sequence
-    if (P.terrain) return false;
     if (!P.periodic[0] || !P.periodic[1]) return false;
     const int G = P.ng_cons > P.ng_vel ? P.ng_cons : P.ng_vel;
     if (P.L.nx < G + 1 || P.L.ny < G + 1 || P.L.nz < 2 * G) return false;
@@ -330,6 +328,9 @@ void apply_bcs_periodic(const Params& P, const erfx_state& S, double* xv, double
     const int zth = P.bck[2][5];
     const int kz = rmax > ngv ? rmax : (ngv > 1 ? ngv : 1);
     const Box grownA(-G, -G, -kz, nx + G, ny + G, nz - 1 + kz);
+    const bool terr = P.terrain;
+    const Zn zt{P.z_nd, L};
+    const double dxi = P.dxInv[0], dyi = P.dxInv[1];
     const Box grownB(-G, -G, -1, nx + G, ny + G, nz);
 
     launch_box(grownA, s, [=] __device__(int i, int j, int k) {
@@ -395,7 +396,25 @@ void apply_bcs_periodic(const Params& P, const erfx_state& S, double* xv, double
         if (k >= 0 && k <= nz && i >= -ngV && i <= nx - 1 + ngV && j >= -ngV && j <= ny - 1 + ngV) {
             const bool near = i >= -ngv && i <= nx - 1 + ngv && j >= -ngv && j <= ny - 1 + ngv;
             double w;
-            if (near && k == 0) w = zvl;
+            if (terr && near && (k == 0 || (k == nz && zth == EXT_DIR))) {
+                // terrain wall: WFromOmega with u, v as this launch leaves them
+                // (the x / y branches above, evaluated again here)
+                auto uf = [&](int ii, int jj, int kk) {
+                    const int fi = ii < 0 ? ii + nx : (ii > nx ? ii - nx : ii);
+                    const int cj = jj < 0 ? jj + ny : (jj > ny - 1 ? jj - ny : jj);
+                    if (kk >= 0 && kk <= nz - 1) return u_of(fi, cj, kk);
+                    const bool lo = kk < 0;
+                    return (lo ? xtl : xth) == EXT_DIR ? (lo ? xvl : xvh) : u_of(fi, cj, lo ? 0 : nz - 1);
+                };
+                auto vf = [&](int ii, int jj, int kk) {
+                    const int ci = ii < 0 ? ii + nx : (ii > nx - 1 ? ii - nx : ii);
+                    const int fj = jj < 0 ? jj + ny : (jj > ny ? jj - ny : jj);
+                    if (kk >= 0 && kk <= nz - 1) return v_of(ci, fj, kk);
+                    const bool lo = kk < 0;
+                    return (lo ? ytl : yth) == EXT_DIR ? (lo ? yvl : yvh) : v_of(ci, fj, lo ? 0 : nz - 1);
+                };
+                w = met::w_from_omega(i, j, k, k == 0 ? zvl : zvh, uf, vf, zt, dxi, dyi);
+            } else if (near && k == 0) w = zvl;
             else if (near && k == nz && zth == EXT_DIR) w = zvh;
             else if (near && k == nz) w = (4.0 * w_of(ic, jc, nz - 1) - w_of(ic, jc, nz - 2)) / 3.0;
             else w = w_of(ic, jc, k);
@@ -503,6 +522,25 @@ void cons_to_prim(const Params& P, const double* c, double* prim, double* pi_sta
         if (i >= -1 && i <= nx && j >= -1 && j <= ny && k >= -1 && k <= nz)
             pi_stage[q] = eos_exner_of_rt(rho_theta, rdOcp);
     });
+}
+
+}  // namespace erfx
+
+namespace erfx {
+
+__global__ void __launch_bounds__(256) copy_kernel(CopyList c) {
+    const long stride = (long)gridDim.x * 256;
+    for (int a = 0; a < c.count; ++a) {
+        const long n2 = c.n[a] / 2;
+        const double2* src = reinterpret_cast<const double2*>(c.src[a]);
+        double2* dst = reinterpret_cast<double2*>(c.dst[a]);
+        for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n2; t += stride) dst[t] = src[t];
+    }
+}
+
+void copy_fields(const CopyList& c, hipStream_t s) {
+    if (c.count == 0) return;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, c);
 }
 
 }  // namespace erfx

@@ -24,6 +24,19 @@ void apply_bcs(const Params& P, const erfx_state& S, double* xv, double* yv, dou
 void fill_patch(const Params& P, double* c, double* u, double* v, double* w, hipStream_t s);
 void cons_to_prim(const Params& P, const double* c, double* prim, double* pi_stage, int ng, hipStream_t s);
 
+// Device-to-device copy of up to COPY_MAX arrays in one launch over the whole
+// chip (the runtime's copy engine splits a large copy into many small blit
+// dispatches that reach a fraction of the HBM bandwidth).  n: doubles, even.
+constexpr int COPY_MAX = 4;
+struct CopyList {
+    double* dst[COPY_MAX];
+    const double* src[COPY_MAX];
+    long n[COPY_MAX];
+    int count = 0;
+    void add(double* d, const double* s_, long n_) { dst[count] = d; src[count] = s_; n[count] = n_; ++count; }
+};
+void copy_fields(const CopyList& c, hipStream_t s);
+
 // erfx_fast.hip
 struct FastWork {
     double* extrap;   // theta extrapolated forward, cells grown by 1
@@ -51,8 +64,7 @@ bool fast_bcs_fusable(const Params& P);
 // erf_fast_rhs_T on a static terrain grid (erfx_terrain.hip): increments of the
 // previous substep and the updated horizontal momenta on faces grown by one
 struct TerrainWork {
-    double *du, *dv, *dw;     // momentum increments prev - stage
-    double *drho, *drt;       // rho and rho*theta increments, cells grown by 1
+    double *du, *dv;          // horizontal momentum increments prev - stage
     double *nu, *nv;          // new horizontal momentum increments
     double *om;               // Omega of the old increments on z-faces
 };

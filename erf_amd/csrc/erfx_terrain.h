@@ -58,16 +58,19 @@ __device__ __forceinline__ double zlo_eta(const Zn& z, int i, int j, int k, doub
     return 0.5 * dyi * (z(i + 1, j + 1, k) + z(i, j + 1, k) - z(i + 1, j, k) - z(i, j, k));
 }
 // u and v averaged to the z-face, extrapolated at k = 0 (:383-388)
-__device__ __forceinline__ double u_at_zface(const Fld& u, int i, int j, int k) {
+template <class U>
+__device__ __forceinline__ double u_at_zface(const U& u, int i, int j, int k) {
     return (k == 0) ? 1.5 * (0.5 * (u(i, j, k) + u(i + 1, j, k))) - 0.5 * (0.5 * (u(i, j, k + 1) + u(i + 1, j, k + 1)))
                     : 0.25 * (u(i, j, k - 1) + u(i + 1, j, k - 1) + u(i, j, k) + u(i + 1, j, k));
 }
-__device__ __forceinline__ double v_at_zface(const Fld& v, int i, int j, int k) {
+template <class V>
+__device__ __forceinline__ double v_at_zface(const V& v, int i, int j, int k) {
     return (k == 0) ? 1.5 * (0.5 * (v(i, j, k) + v(i, j + 1, k))) - 0.5 * (0.5 * (v(i, j, k + 1) + v(i, j + 1, k + 1)))
                     : 0.25 * (v(i, j, k - 1) + v(i, j + 1, k - 1) + v(i, j, k) + v(i, j + 1, k));
 }
-// OmegaFromW (:359-392) and WFromOmega (:397-445)
-__device__ __forceinline__ double omega_from_w(int i, int j, int k, double w, const Fld& u, const Fld& v,
+// OmegaFromW (:359-392) and WFromOmega (:397-445); u, v: anything read as u(i, j, k)
+template <class U, class V>
+__device__ __forceinline__ double omega_from_w(int i, int j, int k, double w, const U& u, const V& v,
                                                const Zn& z, double dxi, double dyi) {
     double mx = zlo_xi(z, i, j, k, dxi);
     double my = zlo_eta(z, i, j, k, dyi);
@@ -75,7 +78,8 @@ __device__ __forceinline__ double omega_from_w(int i, int j, int k, double w, co
     double vv = v_at_zface(v, i, j, k);
     return w - mx * uu - my * vv;
 }
-__device__ __forceinline__ double w_from_omega(int i, int j, int k, double omega, const Fld& u, const Fld& v,
+template <class U, class V>
+__device__ __forceinline__ double w_from_omega(int i, int j, int k, double omega, const U& u, const V& v,
                                                const Zn& z, double dxi, double dyi) {
     double uu = u_at_zface(u, i, j, k);
     double vv = v_at_zface(v, i, j, k);

@@ -1,12 +1,13 @@
 // erfx_terrain.hip -- terrain-following grid: the metrics of the node heights
 // and the acoustic substep erf_fast_rhs_T (static terrain, unit map factors).
 //
-// The substep runs as four launches: increments on the grown box; the
-// horizontal momenta; the horizontal flux divergences with Omega of the old
-// increments; and one thread per column for the implicit vertical solve and
-// the rho, rho*theta and rho*w updates.  Every expression keeps the
-// reference's operation order (FP contraction is off), so each point carries
-// the reference's rounding.
+// The substep runs as three phases: theta extrapolation and the horizontal
+// momentum increments on the grown box; the new horizontal momenta; and one
+// lane per column that forms Omega of the old increments and the horizontal
+// divergence of the new ones as it sweeps up, solves the implicit vertical
+// system and updates rho, rho*theta and rho*w on the way down.  Every
+// expression keeps the reference's operation order (FP contraction is off),
+// so each point carries the reference's rounding.
 #include "erfx_internal.h"
 #include "erfx_terrain.h"
 
@@ -75,8 +76,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     const double* sc0 = stage.cons;
     const double* sc1 = stage.cons + ne;
     double* lagged1 = scratch.cons + ne;
-    double* du = T.du; double* dv = T.dv; double* dw = T.dw;
-    double* drho = T.drho; double* drt = T.drt;
+    double* du = T.du; double* dv = T.dv;
     double* nu = T.nu; double* nv = T.nv; double* om = T.om;
     double* extrap = W.extrap;
     double* trhs0 = W.trhs;
@@ -86,31 +86,31 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     const double* sxm = stage.xmom; const double* sym = stage.ymom; const double* szm = stage.zmom;
     const bool first = step == 0;
 
-    // :139-204: rho, rho*theta copied from prev on the grown box (step 0); the
-    // increments, theta extrapolated forward and the lagged increment; the
-    // momentum increments on faces grown by one laterally (the top-of-box
-    // layer of du, dv copied into the layer above)
-    launch_box(Box(-1, -1, -1, nx + 1, ny + 1, nz), s, [=] __device__(int i, int j, int k) {
+    // :139-204: rho, rho*theta copied from prev on the grown box (step 0);
+    // theta extrapolated forward and the lagged increment; the horizontal
+    // momentum increments on faces grown by one laterally, kept because the
+    // new momenta overwrite prev when it aliases data (the rho, rho*theta and
+    // rho*w increments are formed where they are used, from prev and stage)
+    launch_box(Box(-1, -1, -1, nx + 1, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
-        if (i <= nx && j <= ny) {
+        if (i <= nx) {
             if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
-            double dr = cc0[q] - sc0[q];
             double dt_ = cc1[q] - sc1[q];
-            drho[q] = dr;
-            drt[q] = dt_;
             extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
             lagged1[q] = dt_;
         }
-        if (k >= 0 && j <= ny) {
-            if (k <= nz - 1) du[q] = pxm[q] - sxm[q];
-            else du[q] = pxm[q - sz] - sxm[q - sz];
-        }
-        if (k >= 0 && i <= nx) {
-            if (k <= nz - 1) dv[q] = pym[q] - sym[q];
-            else dv[q] = pym[q - sz] - sym[q - sz];
-        }
-        if (k >= 0 && i <= nx && j <= ny) dw[q] = pzm[q] - szm[q];
+        if (k >= 0) du[q] = pxm[q] - sxm[q];
+        if (k >= 0 && i <= nx) dv[q] = pym[q] - sym[q];
     });
+    // the top layer of cells (k = nz)
+    launch_box(Box(-1, -1, nz, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+        const long q = L.ix(i, j, k);
+        if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
+        double dt_ = cc1[q] - sc1[q];
+        extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
+        lagged1[q] = dt_;
+    });
+
 
     // :262-331 horizontal momenta with the terrain-corrected pressure gradient
     const double* pq1 = prim + PRIM_Q1 * ne;
@@ -161,34 +161,15 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         }
     });
 
-    // :414-457 horizontal divergence of the area-weighted increments (cells)
-    // and :460-484 Omega of the old increments (z-faces of the valid box)
+    // :414-669, one lane per (i, j) column.  The forward sweep makes, level by
+    // level, Omega of the old increments (:460-484) on the face above, the
+    // horizontal divergence of the area-weighted new increments (:414-457) in
+    // the cell, the tridiagonal right-hand side on the face (:499-552) and the
+    // elimination (:561-581, GPU branch); Omega, the divergences and the
+    // solution are kept for the backward sweep: substitution, rho*w = stage +
+    // WFromOmega on the interior faces (faces 0 and nz keep the stage value,
+    // :620-633), the time-averaged Omega and the rho, rho*theta update (:640-669).
     const Fld fdu{du, L}, fdv{dv, L}, fnu{nu, L}, fnv{nv, L};
-    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        if (k <= nz - 1) {
-            double h_zeta_cc_xface_hi = 0.5 * dzi * (z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1) - z(i + 1, j, k) - z(i + 1, j + 1, k));
-            double h_zeta_cc_xface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
-            double h_zeta_cc_yface_hi = 0.5 * dzi * (z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1) - z(i, j + 1, k) - z(i + 1, j + 1, k));
-            double h_zeta_cc_yface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
-            double xflux_lo = nu[q] * h_zeta_cc_xface_lo / 1.0;
-            double xflux_hi = nu[q + 1] * h_zeta_cc_xface_hi / 1.0;
-            double yflux_lo = nv[q] * h_zeta_cc_yface_lo / 1.0;
-            double yflux_hi = nv[q + sy] * h_zeta_cc_yface_hi / 1.0;
-            double mfsq = 1.0 * 1.0;
-            trhs0[q] = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq;
-            trhs1[q] = ((xflux_hi * (prim[q] + prim[q + 1]) - xflux_lo * (prim[q] + prim[q - 1])) * dxi * mfsq +
-                        (yflux_hi * (prim[q] + prim[q + sy]) - yflux_lo * (prim[q] + prim[q - sy])) * dyi * mfsq) * 0.5;
-        }
-        if (k == 0) om[q] = 0.;
-        else if (k == nz) om[q] = dw[q];
-        else om[q] = met::omega_from_w(i, j, k, dw[q], fdu, fdv, z, dxi, dyi);
-    });
-
-    // :499-669 per column: the tridiagonal right-hand side on the interior
-    // faces, the solve, rho*w (the stage value plus the increment converted
-    // from Omega on the interior faces; faces 0 and nz keep the stage value),
-    // the time-averaged Omega and the rho, rho*theta updates
     const double halfg = fabs(0.5 * (-P.gravity));
     const double* sr0 = slow_rhs.cons;
     const double* sr1 = slow_rhs.cons + ne;
@@ -202,41 +183,90 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     double* azm = scratch.zmom;
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
-        // forward sweep with the right-hand side made on the fly
+        // Omega of the old increments on face kf (0 at the bottom, the rho*w
+        // increment at the top, OmegaFromW between)
+        auto omega_old = [&](int kf) -> double {
+            const long q = q0 + (long)kf * sz;
+            if (kf == 0) return 0.;
+            const double dwk = pzm[q] - szm[q];
+            if (kf == nz) return dwk;
+            return met::omega_from_w(i, j, kf, dwk, fdu, fdv, z, dxi, dyi);
+        };
+        // horizontal divergence of the new increments in cell k
+        auto trhs_at = [&](int k, double& t0, double& t1) {
+            const long q = q0 + (long)k * sz;
+            double h_zeta_cc_xface_hi = 0.5 * dzi * (z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1) - z(i + 1, j, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_xface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
+            double h_zeta_cc_yface_hi = 0.5 * dzi * (z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1) - z(i, j + 1, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_yface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
+            double xflux_lo = nu[q] * h_zeta_cc_xface_lo / 1.0;
+            double xflux_hi = nu[q + 1] * h_zeta_cc_xface_hi / 1.0;
+            double yflux_lo = nv[q] * h_zeta_cc_yface_lo / 1.0;
+            double yflux_hi = nv[q + sy] * h_zeta_cc_yface_hi / 1.0;
+            double mfsq = 1.0 * 1.0;
+            t0 = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq;
+            t1 = ((xflux_hi * (prim[q] + prim[q + 1]) - xflux_lo * (prim[q] + prim[q - 1])) * dxi * mfsq +
+                  (yflux_hi * (prim[q] + prim[q + sy]) - yflux_lo * (prim[q] + prim[q - sy])) * dyi * mfsq) * 0.5;
+        };
+        // face 0 and cell 0
         double s_prev = (dtau * srz[q0]) * fB[q0];
         soln[q0] = s_prev;
-        for (int k = 1; k <= nz; ++k) {
-            const long q = q0 + k * sz;
-            double rhs;
-            if (k == nz) {
-                rhs = dtau * srz[q];
-            } else {
-                double detJ_on_kface = 0.5 * (J[q] + J[q - sz]);
-                double coeff_P = fP[q];
-                double coeff_Q = fQ[q];
-                if (moist) {
-                    double qf = 0.5 * (pq1[q] + pq1[q - sz] + pq2[q] + pq2[q - sz]);
-                    coeff_P /= (1.0 + qf);
-                    coeff_Q /= (1.0 + qf);
-                }
-                double theta_t_lo = 0.5 * (prim[q - 2 * sz] + prim[q - sz]);
-                double theta_t_mid = 0.5 * (prim[q - sz] + prim[q]);
-                double theta_t_hi = 0.5 * (prim[q] + prim[q + sz]);
-                double R0_tmp = (-halfg * drho[q] + coeff_P * drt[q]) * J[q]
-                              + (-halfg * drho[q - sz] + coeff_Q * drt[q - sz]) * J[q - sz];
-                double R1_tmp = -halfg * (sr0[q] * J[q] + sr0[q - sz] * J[q - sz])
-                              + (coeff_P * sr1[q] * J[q] + coeff_Q * sr1[q - sz] * J[q - sz]);
-                double Omega_kp1 = om[q + sz];
-                double Omega_k = om[q];
-                double Omega_km1 = om[q - sz];
-                R1_tmp += (halfg) * (beta_1 * dzi * (Omega_kp1 - Omega_km1) + trhs0[q] + trhs0[q - sz]);
-                R1_tmp += -(coeff_P * (beta_1 * dzi * (Omega_kp1 * theta_t_hi - Omega_k * theta_t_mid) + trhs1[q]) +
-                            coeff_Q * (beta_1 * dzi * (Omega_k * theta_t_mid - Omega_km1 * theta_t_lo) + trhs1[q - sz]));
-                rhs = detJ_on_kface * dw[q] + dtau * (detJ_on_kface * srz[q] + R0_tmp + dtau * beta_2 * R1_tmp);
-                rhs += detJ_on_kface * met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
+        double om_km1 = 0.0, om_k = omega_old(1);
+        om[q0] = om_km1;
+        double t0_km1, t1_km1;
+        trhs_at(0, t0_km1, t1_km1);
+        trhs0[q0] = t0_km1;
+        trhs1[q0] = t1_km1;
+        double drho_km1 = cc0[q0] - sc0[q0], drt_km1 = cc1[q0] - sc1[q0];
+        double J_km1 = J[q0], s0_km1 = sr0[q0], s1_km1 = sr1[q0];
+        double p_km2 = prim[q0 - sz], p_km1 = prim[q0];
+        double q1_km1 = moist ? pq1[q0] : 0.0, q2_km1 = moist ? pq2[q0] : 0.0;
+        for (int k = 1; k <= nz - 1; ++k) {
+            const long q = q0 + (long)k * sz;
+            const double om_kp1 = omega_old(k + 1);
+            double t0_k, t1_k;
+            trhs_at(k, t0_k, t1_k);
+            om[q] = om_k;
+            trhs0[q] = t0_k;
+            trhs1[q] = t1_k;
+            const double Jk = J[q], s0_k = sr0[q], s1_k = sr1[q], p_k = prim[q], p_kp1 = prim[q + sz];
+            const double drho_k = cc0[q] - sc0[q], drt_k = cc1[q] - sc1[q];
+            const double q1_k = moist ? pq1[q] : 0.0, q2_k = moist ? pq2[q] : 0.0;
+            double detJ_on_kface = 0.5 * (Jk + J_km1);
+            double coeff_P = fP[q];
+            double coeff_Q = fQ[q];
+            if (moist) {
+                double qf = 0.5 * (q1_k + q1_km1 + q2_k + q2_km1);
+                coeff_P /= (1.0 + qf);
+                coeff_Q /= (1.0 + qf);
             }
+            double theta_t_lo = 0.5 * (p_km2 + p_km1);
+            double theta_t_mid = 0.5 * (p_km1 + p_k);
+            double theta_t_hi = 0.5 * (p_k + p_kp1);
+            double R0_tmp = (-halfg * drho_k + coeff_P * drt_k) * Jk
+                          + (-halfg * drho_km1 + coeff_Q * drt_km1) * J_km1;
+            double R1_tmp = -halfg * (s0_k * Jk + s0_km1 * J_km1)
+                          + (coeff_P * s1_k * Jk + coeff_Q * s1_km1 * J_km1);
+            R1_tmp += (halfg) * (beta_1 * dzi * (om_kp1 - om_km1) + t0_k + t0_km1);
+            R1_tmp += -(coeff_P * (beta_1 * dzi * (om_kp1 * theta_t_hi - om_k * theta_t_mid) + t1_k) +
+                        coeff_Q * (beta_1 * dzi * (om_k * theta_t_mid - om_km1 * theta_t_lo) + t1_km1));
+            const double dw_k = pzm[q] - szm[q];
+            double rhs = detJ_on_kface * dw_k + dtau * (detJ_on_kface * srz[q] + R0_tmp + dtau * beta_2 * R1_tmp);
+            rhs += detJ_on_kface * met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
             s_prev = (rhs - fA[q] * s_prev) * fB[q];
             soln[q] = s_prev;
+            om_km1 = om_k; om_k = om_kp1;
+            t0_km1 = t0_k; t1_km1 = t1_k;
+            drho_km1 = drho_k; drt_km1 = drt_k;
+            J_km1 = Jk; s0_km1 = s0_k; s1_km1 = s1_k;
+            p_km2 = p_km1; p_km1 = p_k;
+            q1_km1 = q1_k; q2_km1 = q2_k;
+        }
+        {   // top face
+            const long qt = q0 + (long)nz * sz;
+            s_prev = (dtau * srz[qt] - fA[qt] * s_prev) * fB[qt];
+            soln[qt] = s_prev;
+            om[qt] = om_k;
         }
         for (int k = nz - 1; k >= 0; --k) {
             const long q = q0 + k * sz;
