@@ -49,9 +49,11 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--workload", choices=["cbl", "moist_terrain"], default="cbl",
+    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal"], default="cbl",
                     help="cbl (default): the dry CBL LES; moist_terrain: moist flow over a hill on the "
-                         "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s")
+                         "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s; "
+                         "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage")
+    ap.add_argument("--graph", action="store_true", help="replay the steps from a captured hipGraph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (3 launches), the reference kernel for "
@@ -113,7 +115,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    if a.workload == "moist_terrain":
+    if a.workload == "thermal":
+        nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (64, 64, 64)
+        cfg, f = P.rising_thermal(nx, ny, nz, seed=0)
+        sim = Sim(cfg)
+        sim.load(f)
+        dt = 1.0
+    elif a.workload == "moist_terrain":
         nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (512, 512, 96)
         print("building the moist terrain case %dx%dx%d" % (nx, ny, nz), file=sys.stderr, flush=True)
         cfg, f = P.moist_terrain(nx, ny, nz, seed=0)
@@ -130,12 +138,15 @@ def main():
         dt = 0.5
     stream = torch.cuda.ExternalStream(sim.stream)
 
+    if a.graph:
+        sim.set_graph(True)
     sim.step(0.0, dt, a.warmup)
     t = a.warmup * dt
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    check(lib().erfx_sim_timer_reset(sim.h))
+    if a.workload == "cbl" and not a.graph:
+        check(lib().erfx_sim_timer_reset(sim.h))
     with torch.cuda.stream(stream):
         e0.record(stream)
     wall0 = time.perf_counter()
@@ -167,17 +178,20 @@ def main():
     assert np.all(np.isfinite(rv)) and rv.min() > 0.05, "state blew up"
     sim.close()
 
-    if a.workload == "moist_terrain":
+    if a.workload != "cbl" or a.graph:
+        wl = {"thermal": "dry rising thermal %dx%dx%d, C2 advection, dt=1 s, MRI ratio %d (%d substeps/step)",
+              "moist_terrain": "moist flow over a 500 m Gaussian hill %dx%dx%d, dx=200 m, levels stretched "
+                               "20-400 m to a 15 km top, WENO5 scalars, U3 momentum, Kessler, Rayleigh above "
+                               "12 km, dt=1 s, MRI ratio %d (%d substeps/step)",
+              "cbl": "dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, "
+                     "MRI ratio %d (%d substeps/step)"}[a.workload]
         out = {
             "metric": "cell-updates/s per RK3 step (FP64 GPU path)", "value": value, "unit": "cell-updates/s",
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms / a.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded)",
-            "config": {"workload": "moist flow over a 500 m Gaussian hill %dx%dx%d, dx=200 m, levels stretched "
-                                   "20-400 m to a 15 km top, WENO5 scalars, U3 momentum, Kessler, Rayleigh above "
-                                   "12 km, dt=1 s, MRI ratio %d (%d substeps/step)" % (cfg.nx, cfg.ny, cfg.nz,
-                                                                                      cfg.mri_ratio, nsub),
-                       "cells": cells, "precision": "fp64"},
+            "config": {"workload": wl % (cfg.nx, cfg.ny, cfg.nz, cfg.mri_ratio, nsub),
+                       "cells": cells, "precision": "fp64", "graph": bool(a.graph)},
             "wall_s": wall,
         }
         print(json.dumps(out))

@@ -171,7 +171,18 @@ struct erfx_sim {
     float* samples = nullptr;       // training rows harvested before each step's microphysics
     long samples_max = 0;
     unsigned long long* samples_count = nullptr;
+    // two captured steps replayed as one graph (erfx_sim_set_graph)
+    bool graph_on = false;
+    hipGraphExec_t gexec = nullptr;
+    double g_dt = 0.0;
+    int g_ratio = 0, g_cur = -1;
 };
+
+static void graph_free(erfx_sim* s) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+    s->g_cur = -1;
+}
 
 static void timer_free(KTimer& t) {
     for (int i = 0; i < t.cap; ++i) (void)hipEventDestroy(t.ev[i]);
@@ -767,6 +778,7 @@ int erfx_sim_create(const erfx_config* cfg, erfx_sim** out) {
 int erfx_sim_destroy(erfx_sim* s) {
     if (!s) return ERFX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    graph_free(s);
     timer_free(s->timer);
     if (s->red_host) (void)hipHostFree(s->red_host);
     for (void* p : s->allocs) (void)hipFree(p);
@@ -1010,10 +1022,41 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
         set_error("Rayleigh damping profiles not set (erfx_set_rayleigh on the sim's context)"); return ERFX_ERR_STATE;
     }
     if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
-    for (int n = 0; n < nsteps; ++n) {
-        int rc = sim_one_step(s, t0 + n * dt, dt, s->ctx->P.cfg.mri_ratio);
+    const int ratio = s->ctx->P.cfg.mri_ratio;
+    int n = 0;
+    const bool graphable = s->graph_on && !s->mlp && !s->samples && !s->ctx->P.timer;
+    while (graphable && n + 2 <= nsteps) {
+        if (!s->gexec || s->g_dt != dt || s->g_ratio != ratio || s->g_cur != s->cur) {
+            graph_free(s);
+            const int cur0 = s->cur;
+            ERFX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+            int rc = sim_one_step(s, t0 + n * dt, dt, ratio);
+            if (!rc) rc = sim_one_step(s, t0 + (n + 1) * dt, dt, ratio);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { s->gexec = nullptr; return hip_fail(e, "hipGraphInstantiate"); }
+            s->g_dt = dt; s->g_ratio = ratio; s->g_cur = cur0;
+            // capture recorded the two steps without running them
+        }
+        ERFX_HIP(hipGraphLaunch(s->gexec, s->stream));
+        n += 2;
+    }
+    for (; n < nsteps; ++n) {
+        int rc = sim_one_step(s, t0 + n * dt, dt, ratio);
         if (rc) return rc;
     }
+    return ERFX_OK;
+}
+
+int erfx_sim_set_graph(erfx_sim* s, int on) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    s->graph_on = on != 0;
+    if (!s->graph_on) graph_free(s);
     return ERFX_OK;
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "erfx_sim_step": (_I, [_P, _D, _D, _I]),
     "erfx_sim_step_async": (_I, [_P, _D, _D, _I]),
     "erfx_sim_sync": (_I, [_P]),
+    "erfx_sim_set_graph": (_I, [_P, _I]),
     "erfx_sim_stream": (_P, [_P]),
     "erfx_sim_est_dt": (_I, [_P, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_run": (_I, [_P, _D, _I, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
@@ -185,6 +186,10 @@ class Sim:
 
     def sync(self):
         check(lib().erfx_sim_sync(self.h))
+
+    def set_graph(self, on=True):
+        """Replay the step sequence from a captured hipGraph (two steps per graph)."""
+        check(lib().erfx_sim_set_graph(self.h, 1 if on else 0))
 
     def ptr(self, fid):
         return lib().erfx_sim_field_ptr(self.h, fid)

@@ -378,6 +378,14 @@ int  erfx_sim_step(erfx_sim* sim, double t0, double dt, int nsteps);
 /* Same, asynchronous on the sim's stream (for timing with events). */
 int  erfx_sim_step_async(erfx_sim* sim, double t0, double dt, int nsteps);
 int  erfx_sim_sync(erfx_sim* sim);
+/* Replay the step sequence from a hipGraph (no counterpart in the reference;
+ * for small boxes, where launch overhead dominates).  When on, erfx_sim_step*
+ * captures two whole steps (the state buffers alternate, so two steps return
+ * every pointer to its place) into a graph on first use, re-captures when dt
+ * or the MRI ratio changes, and replays it; an odd last step runs directly.
+ * The results are those of the direct path.  Not used while the learned
+ * microphysics, row sampling or the kernel timer is on. */
+int  erfx_sim_set_graph(erfx_sim* sim, int on);
 /* The sim's stream (a hipStream_t) for event timing. */
 void* erfx_sim_stream(erfx_sim* sim);
 /* Time-step control of ERF's time loop (ERF::ComputeDt / ERF::estTimeStep,
