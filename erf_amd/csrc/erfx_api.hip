@@ -171,12 +171,45 @@ struct erfx_sim {
     float* samples = nullptr;       // training rows harvested before each step's microphysics
     long samples_max = 0;
     unsigned long long* samples_count = nullptr;
+    ColTuner* tuner = nullptr;      // column-kernel autotuner (erfx_sim_autotune)
     // two captured steps replayed as one graph (erfx_sim_set_graph)
     bool graph_on = false;
     hipGraphExec_t gexec = nullptr;
     double g_dt = 0.0;
     int g_ratio = 0, g_cur = -1;
 };
+
+static void tuner_free(erfx_sim* s) {
+    if (!s->tuner) return;
+    for (hipEvent_t& e : s->tuner->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete s->tuner;
+    s->tuner = nullptr;
+    s->ctx->P.tuner = nullptr;
+}
+
+// after the last timed launch: keep the fastest variant
+static int tuner_poll(erfx_sim* s) {
+    ColTuner* t = s->tuner;
+    if (!t || t->chosen >= 0 || t->next < COL_NV * ColTuner::SAMPLES) return ERFX_OK;
+    ERFX_HIP(hipEventSynchronize(t->ev[2 * t->next - 1]));
+    int best = 0;
+    for (int v = 0; v < COL_NV; ++v) {
+        float m = 1e30f;
+        for (int r = 0; r < ColTuner::SAMPLES; ++r) {
+            const int tag = r * COL_NV + v;
+            float x = 0.f;
+            ERFX_HIP(hipEventElapsedTime(&x, t->ev[2 * tag], t->ev[2 * tag + 1]));
+            m = std::min(m, x);
+        }
+        t->ms[v] = m;
+        if (m < t->ms[best]) best = v;
+    }
+    t->chosen = best;
+    s->ctx->P.col_variant = best;
+    s->ctx->P.tuner = nullptr;
+    return ERFX_OK;
+}
 
 static void graph_free(erfx_sim* s) {
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -779,6 +812,7 @@ int erfx_sim_destroy(erfx_sim* s) {
     if (!s) return ERFX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     graph_free(s);
+    tuner_free(s);
     timer_free(s->timer);
     if (s->red_host) (void)hipHostFree(s->red_host);
     for (void* p : s->allocs) (void)hipFree(p);
@@ -1024,7 +1058,11 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
     if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
     const int ratio = s->ctx->P.cfg.mri_ratio;
     int n = 0;
-    const bool graphable = s->graph_on && !s->mlp && !s->samples && !s->ctx->P.timer;
+    {
+        int rc = tuner_poll(s);
+        if (rc) return rc;
+    }
+    const bool graphable = s->graph_on && !s->mlp && !s->samples && !s->ctx->P.timer && !s->ctx->P.tuner;
     while (graphable && n + 2 <= nsteps) {
         if (!s->gexec || s->g_dt != dt || s->g_ratio != ratio || s->g_cur != s->cur) {
             graph_free(s);
@@ -1049,6 +1087,42 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
         int rc = sim_one_step(s, t0 + n * dt, dt, ratio);
         if (rc) return rc;
     }
+    return ERFX_OK;
+}
+
+int erfx_sim_autotune(erfx_sim* s, int on) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    tuner_free(s);
+    if (!on) return ERFX_OK;
+    s->tuner = new ColTuner();
+    for (hipEvent_t& e : s->tuner->ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) { tuner_free(s); return hip_fail(r, "hipEventCreate"); }
+    }
+    s->ctx->P.tuner = s->tuner;
+    graph_free(s);
+    return ERFX_OK;
+}
+
+int erfx_sim_column_variant(erfx_sim* s, int* variant, float* ms) {
+    if (!s || !variant) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    int rc = tuner_poll(s);
+    if (rc) return rc;
+    *variant = (s->tuner && s->tuner->chosen < 0) ? -1 : s->ctx->P.col_variant;
+    if (ms)
+        for (int v = 0; v < COL_NV; ++v) ms[v] = s->tuner ? s->tuner->ms[v] : 0.f;
+    return ERFX_OK;
+}
+
+int erfx_sim_set_column_variant(erfx_sim* s, int variant) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (variant < 0 || variant >= COL_NV) { set_error("column variant must be 0..8"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    tuner_free(s);
+    s->ctx->P.col_variant = variant;
+    graph_free(s);
     return ERFX_OK;
 }
 

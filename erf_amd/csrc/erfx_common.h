@@ -107,24 +107,40 @@ inline void launch_box(const Box& b, hipStream_t s, F f) {
 }
 
 // Column launch over a 2-D (i, j) range; the functor loops over k itself.
-// Work-groups are 256 lanes along x, so each field-level access of a group is
-// one contiguous 2 KB row segment (one DRAM-friendly burst per field and level).
+// Work-groups are `width` lanes along x (256 by default, so each field-level
+// access of a group is one contiguous 2 KB row segment).
 template <class F>
 __global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int jhi, unsigned gx, F f) {
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
-    int i = ilo + (int)((t % gx) * 256 + threadIdx.x);
+    int i = ilo + (int)((t % gx) * blockDim.x + threadIdx.x);
     int j = jlo + (int)(t / gx);
     if (i > ihi || j > jhi) return;
     f(i, j);
 }
 
 template <class F>
-inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
+inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f, int width = 256) {
     if (ilo > ihi || jlo > jhi) return;
-    const unsigned gx = (unsigned)((ihi - ilo + 1 + 255) / 256);
+    const unsigned gx = (unsigned)((ihi - ilo + 1 + width - 1) / width);
     const unsigned gy = (unsigned)(jhi - jlo + 1);
-    hipLaunchKernelGGL(col_kernel<F>, dim3(gx * gy), dim3(256, 1, 1), 0, s, ilo, ihi, jlo, jhi, gx, f);
+    hipLaunchKernelGGL(col_kernel<F>, dim3(gx * gy), dim3((unsigned)width, 1, 1), 0, s, ilo, ihi, jlo, jhi, gx, f);
 }
+
+// Variants of the acoustic column kernel (erf_fast_rhs_N): levels of loads
+// kept in flight per lane and work-group width.  Every variant computes the
+// same values; the sim's autotuner times them on the first launches of a run
+// and keeps the fastest.
+constexpr int COL_NV = 9;
+constexpr int COL_DEPTH[COL_NV] = {2, 3, 4, 2, 3, 4, 2, 3, 4};
+constexpr int COL_WIDTH[COL_NV] = {64, 64, 64, 128, 128, 128, 256, 256, 256};
+constexpr int COL_DEFAULT = 7;   // depth 3, 256 lanes
+struct ColTuner {
+    static constexpr int SAMPLES = 3;
+    hipEvent_t ev[2 * COL_NV * SAMPLES] = {};
+    int next = 0;          // launches timed so far
+    int chosen = -1;       // variant picked, -1 while timing
+    float ms[COL_NV] = {}; // best time per variant
+};
 
 // ---- boundary-condition kinds (AMReX BCType + ERFBCType) -------------------
 enum BCK { INT_DIR = 0, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_ODD, NEUMANN_INT };
@@ -181,6 +197,8 @@ struct Params {
     // terrain grid with Rayleigh damping: factors and flags per point (the
     // heights vary along a level), RPT_* blocks of ne values
     const double* ray_pt = nullptr;
+    int col_variant = COL_DEFAULT;   // acoustic column kernel variant (COL_DEPTH / COL_WIDTH)
+    ColTuner* tuner = nullptr;       // when timing variants (host pointer)
 };
 
 // prim components of the moisture species (prim(n) = cons(n+1) / rho)

@@ -12,6 +12,7 @@
 // The arithmetic of every point keeps the reference's operation order.
 #include "erfx_common.h"
 #include "erfx_internal.h"
+#include <type_traits>
 
 namespace erfx {
 
@@ -219,8 +220,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     double* cc1 = data.cons + ne;
     const bool gw = ch.ghost_writes;
     const double ch_w_lo = ch.w_lo, ch_w_hi = ch.w_hi;
-    kt_start(P.timer, KT_FAST_COLUMN, s);
-    launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
+    auto column = [=] __device__(int i, int j, auto depth) {
         const long q0 = L.ix(i, j, 0);
         // per-level loads of the forward sweep (face k and cell k)
         struct Fw {
@@ -280,7 +280,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double q1_km1 = cur.q1, q2_km1 = cur.q2;
         double Om_k = w_above(cur, 1) - cur.szz;
         // ring of FD levels of loads in flight (slots indexed statically after unrolling)
-        constexpr int FD = 3;
+        constexpr int FD = decltype(depth)::value;
         Fw ring[FD];
 #pragma unroll
         for (int d = 0; d < FD; ++d)
@@ -346,7 +346,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         double sol_next = sol_t;
         double drw_next = Om_k;
         double p_next = prim[qt];
-        constexpr int BD = 3;
+        constexpr int BD = decltype(depth)::value;
         Bw bring[BD];
 #pragma unroll
         for (int d = 0; d < BD; ++d)
@@ -399,7 +399,32 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 p_next = pcq;
             }
         }
-    });
+    };
+    // the variant: the sim's choice, or the next one to time while tuning
+    int var = P.col_variant;
+    int tag = -1;
+    if (P.tuner && P.tuner->chosen < 0 && P.tuner->next < COL_NV * ColTuner::SAMPLES) {
+        tag = P.tuner->next++;
+        var = tag % COL_NV;
+        (void)hipEventRecord(P.tuner->ev[2 * tag], s);
+    }
+    const int width = COL_WIDTH[var];
+    kt_start(P.timer, KT_FAST_COLUMN, s);
+    switch (COL_DEPTH[var]) {
+        case 2:
+            launch_cols(0, nx - 1, 0, ny - 1, s,
+                        [=] __device__(int i, int j) { column(i, j, std::integral_constant<int, 2>{}); }, width);
+            break;
+        case 4:
+            launch_cols(0, nx - 1, 0, ny - 1, s,
+                        [=] __device__(int i, int j) { column(i, j, std::integral_constant<int, 4>{}); }, width);
+            break;
+        default:
+            launch_cols(0, nx - 1, 0, ny - 1, s,
+                        [=] __device__(int i, int j) { column(i, j, std::integral_constant<int, 3>{}); }, width);
+            break;
+    }
+    if (tag >= 0) (void)hipEventRecord(P.tuner->ev[2 * tag + 1], s);
     // algorithmic traffic per z-face: 22 fields read once (A, 1/B, C, P, Q, theta,
     // prev/stage rho*w, prev/stage rho and rho*theta, slow rho, rho*theta and
     // rho*w, new and stage x- and y-momenta, rho and rho*theta of the state being

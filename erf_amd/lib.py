@@ -76,6 +76,9 @@ SIGNATURES = {
     "erfx_sim_step_async": (_I, [_P, _D, _D, _I]),
     "erfx_sim_sync": (_I, [_P]),
     "erfx_sim_set_graph": (_I, [_P, _I]),
+    "erfx_sim_autotune": (_I, [_P, _I]),
+    "erfx_sim_column_variant": (_I, [_P, ctypes.POINTER(_I), _P]),
+    "erfx_sim_set_column_variant": (_I, [_P, _I]),
     "erfx_sim_stream": (_P, [_P]),
     "erfx_sim_est_dt": (_I, [_P, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_run": (_I, [_P, _D, _I, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
@@ -186,6 +189,20 @@ class Sim:
 
     def sync(self):
         check(lib().erfx_sim_sync(self.h))
+
+    def autotune(self, on=True):
+        """Time the acoustic column kernel's variants on the next launches and keep the fastest."""
+        check(lib().erfx_sim_autotune(self.h, 1 if on else 0))
+
+    def column_variant(self):
+        """(variant in use or -1 while timing, best ms per variant)."""
+        v = _I()
+        ms = (ctypes.c_float * 9)()
+        check(lib().erfx_sim_column_variant(self.h, ctypes.byref(v), ms))
+        return v.value, list(ms)
+
+    def set_column_variant(self, v):
+        check(lib().erfx_sim_set_column_variant(self.h, int(v)))
 
     def set_graph(self, on=True):
         """Replay the step sequence from a captured hipGraph (two steps per graph)."""

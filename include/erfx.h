@@ -386,6 +386,17 @@ int  erfx_sim_sync(erfx_sim* sim);
  * The results are those of the direct path.  Not used while the learned
  * microphysics, row sampling or the kernel timer is on. */
 int  erfx_sim_set_graph(erfx_sim* sim, int on);
+/* Autotuner of the acoustic column kernel (no counterpart in the reference):
+ * with `on`, the next 27 launches of the erf_fast_rhs_N column kernel cycle
+ * through its 9 variants (levels of loads in flight 2/3/4 x work-group width
+ * 64/128/256), each timed with HIP events, and the fastest is kept for the
+ * rest of the run.  Every variant computes the same values. */
+int  erfx_sim_autotune(erfx_sim* sim, int on);
+/* The column variant in use (0..8; -1 if still timing) and, when ms is not
+ * NULL, the best time of each variant in ms (9 values, 0 if not timed). */
+int  erfx_sim_column_variant(erfx_sim* sim, int* variant, float* ms);
+/* Fix the column variant (0..8) and stop any tuning. */
+int  erfx_sim_set_column_variant(erfx_sim* sim, int variant);
 /* The sim's stream (a hipStream_t) for event timing. */
 void* erfx_sim_stream(erfx_sim* sim);
 /* Time-step control of ERF's time loop (ERF::ComputeDt / ERF::estTimeStep,

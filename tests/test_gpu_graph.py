@@ -1,7 +1,11 @@
-"""The step sequence replayed from a captured hipGraph (erfx_sim_set_graph)
-against the same steps launched directly: bit-identical after every replay,
-with eager work and host synchronisation between the replays, for dry, moist
-(Kessler) and terrain cases; a change of dt re-captures the graph."""
+"""Launch-level options of the sim against the plain path:
+
+* the step sequence replayed from a captured hipGraph (erfx_sim_set_graph):
+  bit-identical after every replay, with eager work and host synchronisation
+  between the replays, for dry, moist (Kessler) and terrain cases; a change of
+  dt re-captures the graph;
+* every variant of the acoustic column kernel (erfx_sim_set_column_variant)
+  and the autotuner that picks one (erfx_sim_autotune): bit-identical."""
 import numpy as np
 import pytest
 
@@ -54,6 +58,35 @@ def test_graph_replay_matches_direct(case):
     assert _same(a, b) == []
     a.step(t, 0.5, 2)                         # new dt: captured again
     b.step(t, 0.5, 2)
+    assert _same(a, b) == []
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("variant", list(range(9)))
+def test_column_variants_identical(variant):
+    cfg, f = P.rising_thermal(40, 24, 16, seed=11, adv=C.ADV_UPWIND3)
+    a, b = _pair(cfg, f)
+    a.set_graph(False)
+    a.set_column_variant(variant)
+    a.step(0.0, 1.0, 3)
+    b.step(0.0, 1.0, 3)
+    assert _same(a, b) == []
+    a.close()
+    b.close()
+
+
+def test_autotune_picks_a_variant_and_keeps_results():
+    cfg, f = P.moist_bubble(24, 20, 16, seed=13)
+    a, b = _pair(cfg, f)
+    a.set_graph(False)
+    a.autotune(True)
+    a.step(0.0, 1.0, 3)          # 3 steps x 10 substeps: every variant timed 3 times
+    b.step(0.0, 1.0, 3)
+    v, ms = a.column_variant()
+    assert 0 <= v < 9 and all(m > 0 for m in ms) and ms[v] == min(ms)
+    a.step(3.0, 1.0, 2)          # with the chosen variant
+    b.step(3.0, 1.0, 2)
     assert _same(a, b) == []
     a.close()
     b.close()
