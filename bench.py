@@ -54,6 +54,9 @@ def parse():
                          "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s; "
                          "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage")
     ap.add_argument("--graph", action="store_true", help="replay the steps from a captured hipGraph")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 face values of the advected quantities on perturbations; the same steps "
+                         "are then run in FP64 too and the relative L2 / Linf differences are reported")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default acoustic column kernel instead of timing its variants in the warm-up")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -117,27 +120,32 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    ray = None
     if a.workload == "thermal":
         nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (64, 64, 64)
         cfg, f = P.rising_thermal(nx, ny, nz, seed=0)
-        sim = Sim(cfg)
-        sim.load(f)
         dt = 1.0
     elif a.workload == "moist_terrain":
         nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (512, 512, 96)
         print("building the moist terrain case %dx%dx%d" % (nx, ny, nz), file=sys.stderr, flush=True)
         cfg, f = P.moist_terrain(nx, ny, nz, seed=0)
-        sim = Sim(cfg)
-        sim.load(f)
-        sim.set_rayleigh(*P.rayleigh_profiles_uniform(cfg, f=f))
-        del f
-        print("state uploaded", file=sys.stderr, flush=True)
+        ray = P.rayleigh_profiles_uniform(cfg, f=f)
         dt = 1.0
     else:
         cfg, f = P.cbl(a.nx, a.ny, a.nz, seed=0)
-        sim = Sim(cfg)
-        sim.load(f)
         dt = 0.5
+    if a.precision == "mixed":
+        cfg.precision = C.PREC_MIXED
+
+    def make_sim(c):
+        s_ = Sim(c)
+        s_.load(f)
+        if ray is not None:
+            s_.set_rayleigh(*ray)
+        return s_
+
+    sim = make_sim(cfg)
+    print("state uploaded", file=sys.stderr, flush=True)
     stream = torch.cuda.ExternalStream(sim.stream)
 
     if a.graph:
@@ -187,9 +195,32 @@ def main():
     ng = sim.L.ng
     rv = rho[ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
     assert np.all(np.isfinite(rv)) and rv.min() > 0.05, "state blew up"
+    errors = None
+    if a.precision == "mixed":
+        # the same steps in FP64 from the same state: relative L2 / Linf of the difference
+        c64 = C.Config.from_buffer_copy(cfg)
+        c64.precision = C.PREC_FP64
+        ref = make_sim(c64)
+        ref.step(0.0, dt, a.warmup + extra + a.steps)
+        errors = {}
+        fields = [(C.F_CONS, C.RHO, "rho"), (C.F_CONS, C.RHOTHETA, "rhotheta"), (C.F_XVEL, 0, "u"),
+                  (C.F_YVEL, 0, "v"), (C.F_ZVEL, 0, "w")]
+        if cfg.moisture_type:
+            fields += [(C.F_CONS, C.RHOQ1, "qv"), (C.F_CONS, C.RHOQ2, "qc"), (C.F_CONS, C.RHOQ3, "qr")]
+        got = {}
+        for fid, comp, name in fields:
+            if fid not in got:
+                got[fid] = (sim.download(fid), ref.download(fid))
+            x = got[fid][0][comp, ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
+            y = got[fid][1][comp, ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
+            r = y - y.mean() if name in ("rho", "rhotheta") else y
+            nrm = float(np.sqrt((r ** 2).mean()))
+            errors[name] = {"rel_l2": float(np.sqrt(((x - y) ** 2).mean())) / nrm if nrm > 0 else 0.0,
+                            "rel_linf": float(np.abs(x - y).max() / np.abs(r).max()) if np.abs(r).max() > 0 else 0.0}
+        ref.close()
     sim.close()
 
-    if a.workload != "cbl" or a.graph:
+    if a.workload != "cbl" or a.graph or a.precision == "mixed":
         wl = {"thermal": "dry rising thermal %dx%dx%d, C2 advection, dt=1 s, MRI ratio %d (%d substeps/step)",
               "moist_terrain": "moist flow over a 500 m Gaussian hill %dx%dx%d, dx=200 m, levels stretched "
                                "20-400 m to a 15 km top, WENO5 scalars, U3 momentum, Kessler, Rayleigh above "
@@ -197,14 +228,18 @@ def main():
               "cbl": "dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, "
                      "MRI ratio %d (%d substeps/step)"}[a.workload]
         out = {
-            "metric": "cell-updates/s per RK3 step (FP64 GPU path)", "value": value, "unit": "cell-updates/s",
+            "metric": "cell-updates/s per RK3 step (%s GPU path)" % ("mixed" if a.precision == "mixed" else "FP64"),
+            "value": value, "unit": "cell-updates/s",
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms / a.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64+f32" if a.precision == "mixed" else "f64",
             "data": "synthetic (seeded)",
             "config": {"workload": wl % (cfg.nx, cfg.ny, cfg.nz, cfg.mri_ratio, nsub),
-                       "cells": cells, "precision": "fp64", "graph": bool(a.graph)},
+                       "cells": cells, "precision": a.precision, "graph": bool(a.graph)},
             "wall_s": wall,
         }
+        if errors is not None:
+            out["error_vs_fp64"] = errors
         print(json.dumps(out))
         return
 

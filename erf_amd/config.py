@@ -21,6 +21,7 @@ MOIST_NONE, MOIST_KESSLER, MOIST_KESSLER_NORAIN = range(3)
 
 F_CONS, F_XVEL, F_YVEL, F_ZVEL, F_R0, F_P0, F_PI0 = range(7)
 F_Z_ND = 7          # node heights of a terrain-following grid
+PREC_FP64, PREC_MIXED = 0, 1
 D_DETJ, D_AX, D_AY, D_AZ, D_Z_CC = 27, 28, 29, 30, 31
 D_PRIM, D_PI_STAGE, D_FAST_COEFFS, D_EDDY = 16, 17, 18, 19
 D_XVEL_OLD, D_YVEL_OLD, D_ZVEL_OLD = 20, 21, 22
@@ -78,6 +79,7 @@ class Config(ctypes.Structure):
         ("moistscal_horiz_adv", ctypes.c_int), ("moistscal_vert_adv", ctypes.c_int),
         ("moistscal_horiz_upw_frac", ctypes.c_double), ("moistscal_vert_upw_frac", ctypes.c_double),
         ("use_terrain", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 
@@ -147,6 +149,7 @@ def default_config(nx, ny, nz, dx, dy, dz):
     c.moistscal_horiz_adv = c.moistscal_vert_adv = ADV_WENO3
     c.moistscal_horiz_upw_frac = c.moistscal_vert_upw_frac = 1.0
     c.use_terrain = 0
+    c.precision = PREC_FP64
     return c
 
 

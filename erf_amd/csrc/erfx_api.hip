@@ -71,6 +71,7 @@ static int validate(const erfx_config& c) {
         set_error("only buoyancy_type 1 is implemented for the dry state"); return ERFX_ERR_UNSUPPORTED;
     }
     if (c.buoyancy_type < 1 || c.buoyancy_type > 4) { set_error("buoyancy_type must be 1, 2, 3 or 4"); return ERFX_ERR_ARG; }
+    if (c.precision != ERFX_PREC_FP64 && c.precision != ERFX_PREC_MIXED) { set_error("bad precision"); return ERFX_ERR_ARG; }
     if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE)) {
         set_error("diffusion on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
     }

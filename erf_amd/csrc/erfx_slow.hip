@@ -10,6 +10,7 @@
 #include "erfx_common.h"
 #include "erfx_internal.h"
 #include "erfx_terrain.h"
+#include <type_traits>
 
 namespace erfx {
 
@@ -153,6 +154,116 @@ __device__ __forceinline__ double interp_scalar_at(int t, const double* f, long 
     return interp_at(t, f, q, st, upw, fr);
 }
 
+// ---- mixed precision: face values in FP32 on perturbations -------------------
+// The same stencils and WENO weights in FP32, applied to the deviations of the
+// stencil values from an FP64 reference c at the face (the mean of its two
+// cells); the face value is c plus the FP32 result.  The interpolants are
+// consistent (weights summing to one) and the smoothness indicators depend on
+// differences only, so the shift changes nothing in exact arithmetic; the
+// FP32 error scales with the local variation, not with the value itself.
+namespace wenof {
+constexpr float eps = 1.0e-6f;
+__device__ __forceinline__ float w3(float sm1, float s, float sp1) {
+    const float g1 = (1.0f / 3.0f), g2 = (2.0f / 3.0f);
+    float b1 = (s - sm1) * (s - sm1);
+    float b2 = (sp1 - s) * (sp1 - s);
+    float w1 = g1 / ((eps + b1) * (eps + b1));
+    float w2 = g2 / ((eps + b2) * (eps + b2));
+    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) / (2.0f * (w1 + w2)));
+}
+__device__ __forceinline__ float z3(float sm1, float s, float sp1) {
+    const float g1 = (1.0f / 3.0f), g2 = (2.0f / 3.0f);
+    float b1 = (s - sm1) * (s - sm1);
+    float b2 = (sp1 - s) * (sp1 - s);
+    float t5 = fabsf(b2 - b1);
+    float w1 = g1 * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    float w2 = g2 * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) / (2.0f * (w1 + w2)));
+}
+__device__ __forceinline__ float mzq3(float sm1, float s, float sp1) {
+    const float g = (1.0f / 3.0f);
+    float b1 = (s - sm1) * (s - sm1);
+    float b2 = (sp1 - s) * (sp1 - s);
+    float b3 = ((13.0f / 12.0f) * ((sm1 - 2.0f * s + sp1) * (sm1 - 2.0f * s + sp1))) + (((sp1 - sm1) * (sp1 - sm1)) / 4.0f);
+    float t5 = (fabsf(b3 - b1) + fabsf(b3 - b2)) / 32.0f;
+    float a1 = g * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
+    float a2 = g * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
+    float a3 = g * (1.0f + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    float asum = a1 + a2 + a3;
+    float v1 = (-sm1 + 3.0f * s) / 2.0f, v2 = (s + sp1) / 2.0f, v3 = (-sm1 + 5.0f * s + 2.0f * sp1) / 6.0f;
+    return ((a3 / asum / g) * (v3 - g * v1 - g * v2) + (a1 / asum) * v1 + (a2 / asum) * v2);
+}
+__device__ __forceinline__ float w5(float sm2, float sm1, float s, float sp1, float sp2, bool z) {
+    const float c1 = (13.0f / 12.0f);
+    const float g1 = (1.0f / 10.0f), g2 = (3.0f / 5.0f), g3 = (3.0f / 10.0f);
+    float b1 = c1 * (sm2 - 2.0f * sm1 + s) * (sm2 - 2.0f * sm1 + s) +
+               0.25f * (sm2 - 4.0f * sm1 + 3.0f * s) * (sm2 - 4.0f * sm1 + 3.0f * s);
+    float b2 = c1 * (sm1 - 2.0f * s + sp1) * (sm1 - 2.0f * s + sp1) + 0.25f * (sm1 - sp1) * (sm1 - sp1);
+    float b3 = c1 * (s - 2.0f * sp1 + sp2) * (s - 2.0f * sp1 + sp2) +
+               0.25f * (3.0f * s - 4.0f * sp1 + sp2) * (3.0f * s - 4.0f * sp1 + sp2);
+    float w1, w2, w3_;
+    if (z) {
+        float t5 = fabsf(b3 - b1);
+        w1 = g1 * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
+        w2 = g2 * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
+        w3_ = g3 * (1.0f + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    } else {
+        w1 = g1 / ((eps + b1) * (eps + b1));
+        w2 = g2 / ((eps + b2) * (eps + b2));
+        w3_ = g3 / ((eps + b3) * (eps + b3));
+    }
+    float v1 = 2.0f * sm2 - 7.0f * sm1 + 11.0f * s;
+    float v2 = -sm1 + 5.0f * s + 2.0f * sp1;
+    float v3 = 2.0f * s + 5.0f * sp1 - sp2;
+    return ((w1 * v1 + w2 * v2 + w3_ * v3) / (6.0f * (w1 + w2 + w3_)));
+}
+__device__ __forceinline__ float face(int t, float sp2, float sp1, float s, float sm1, float sm2, float sm3, double upw) {
+    if (!(upw > weno::tol) && !(upw < -weno::tol)) return 0.5f * (s + sm1);
+    const bool pos = upw > weno::tol;
+    if (t == ERFX_ADV_WENO3) return pos ? w3(sm2, sm1, s) : w3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENOZ3) return pos ? z3(sm2, sm1, s) : z3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENOMZQ3) return pos ? mzq3(sm2, sm1, s) : mzq3(sp1, s, sm1);
+    if (t == ERFX_ADV_WENO5) return pos ? w5(sm3, sm2, sm1, s, sp1, false) : w5(sp2, sp1, s, sm1, sm2, false);
+    return pos ? w5(sm3, sm2, sm1, s, sp1, true) : w5(sp2, sp1, s, sm1, sm2, true);
+}
+}  // namespace wenof
+
+__device__ __forceinline__ float interp_line_f(int t, float sp2, float sp1, float s, float sm1, float sm2, float sm3,
+                                               double upw, double upw_frac) {
+    if (t == ERFX_ADV_CENTERED2) return 0.5f * (s + sm1);
+    float u = (upw != 0.) ? ((upw > 0) ? 1.0f : -1.0f) : 0.0f;
+    u *= (float)upw_frac;
+    if (t == ERFX_ADV_UPWIND3)
+        return (7.0f / 12.0f) * (s + sm1) - (1.0f / 12.0f) * (sp1 + sm2) + u * (1.0f / 12.0f) * ((sp1 - sm2) - 3.0f * (s - sm1));
+    if (t == ERFX_ADV_CENTERED4) return (7.0f / 12.0f) * (s + sm1) - (1.0f / 12.0f) * (sp1 + sm2);
+    const float c5 = (37.0f / 60.0f) * (s + sm1) - (2.0f / 15.0f) * (sp1 + sm2) + (1.0f / 60.0f) * (sp2 + sm3);
+    if (t == ERFX_ADV_UPWIND5)
+        return c5 - u * (1.0f / 60.0f) * ((sp2 - sm3) - 5.0f * (sp1 - sm2) + 10.0f * (s - sm1));
+    return c5;   // CENTERED6
+}
+
+// face value of f on the lo face of q along stride st, mixed precision
+__device__ __forceinline__ double interp_mixed(int t, const double* f, long q, long st, double upw, double fr) {
+    const double c = 0.5 * (f[q] + f[q - st]);
+    if (t == ERFX_ADV_CENTERED2) return c;
+    const bool wide = t == ERFX_ADV_WENO5 || t == ERFX_ADV_WENOZ5 || t == ERFX_ADV_WENO3 || t == ERFX_ADV_WENOZ3 ||
+                      t == ERFX_ADV_WENOMZQ3 || wide_stencil(t);
+    const float sp1 = (float)(f[q + st] - c), s = (float)(f[q] - c), sm1 = (float)(f[q - st] - c);
+    const float sm2 = (float)(f[q - 2 * st] - c);
+    const float sp2 = wide ? (float)(f[q + 2 * st] - c) : 0.f, sm3 = wide ? (float)(f[q - 3 * st] - c) : 0.f;
+    const float v = is_weno(t) ? wenof::face(t, sp2, sp1, s, sm1, sm2, sm3, upw)
+                               : interp_line_f(t, sp2, sp1, s, sm1, sm2, sm3, upw, fr);
+    return c + (double)v;
+}
+// the FP64 or the mixed face value
+__device__ __forceinline__ double interp_pick(bool mixed, int t, const double* f, long q, long st, double upw, double fr) {
+    return mixed ? interp_mixed(t, f, q, st, upw, fr) : interp_at(t, f, q, st, upw, fr);
+}
+__device__ __forceinline__ double interp_scalar_pick(bool mixed, int t, const double* f, long q, long st, double upw,
+                                                     double fr) {
+    return mixed ? interp_mixed(t, f, q, st, upw, fr) : interp_scalar_at(t, f, q, st, upw, fr);
+}
+
 // UPWINDALL::InterpolateInZ: wall-reduced vertical interpolation for w.
 __device__ __forceinline__ double interp_z_all(const double* f, long q, long sz, double upw_lo, double upw_frac, int t) {
     if (t == ERFX_ADV_CENTERED2) return 0.5 * (f[q] + f[q - sz]);
@@ -172,19 +283,31 @@ __device__ __forceinline__ double interp_z_all(const double* f, long q, long sz,
     return iv;
 }
 
+// One box launch per precision: the functor takes (i, j, k) and a
+// std::bool_constant saying whether face values are taken in mixed precision,
+// so each form is its own kernel (the FP64 kernel carries no FP32 code).
+template <class F>
+inline void launch_box_prec(bool mixed, const Box& b, hipStream_t s, F f) {
+    if (mixed) launch_box(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::true_type{}); });
+    else launch_box(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::false_type{}); });
+}
+
 // ---- momentum advection on a terrain-following grid ---------------------
 // AdvectionSrcForMom with metrics (ERF_AdvectionSrcForMom.cpp:204-288 for
 // C2/C2, ERF_AdvectionSrcForMom_T.H:22-330 otherwise), static terrain, unit
 // map factors: the flat fluxes weighted by the face areas and the edge
 // heights h_zeta, divided by detJ averaged to the face.  Each returns that
 // quotient; the right-hand side is its negative.
-struct TAdv {
+struct TAdvData {
     const double *ru, *rv, *u, *v, *w, *om, *Ax, *Ay, *Az, *J;
     Zn z;
     long sy, sz;
     double dxi, dyi, dzi, hf, vf;
     int ht, vt, nz;
     bool c2c2;
+};
+template <bool mixed>      // FP32 face values (mixed precision)
+struct TAdv : TAdvData {
 
     __device__ double x(int i, int j, int k, long q) const {
         double src;
@@ -201,21 +324,21 @@ struct TAdv {
             src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq + (zflux_hi - zflux_lo) * dzi;
         } else {
             double rho_u_avg_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q] * 1.0);
-            double interp_hi = interp_at(ht, u, q + 1, 1, rho_u_avg_hi, hf);
+            double interp_hi = interp_pick(mixed, ht, u, q + 1, 1, rho_u_avg_hi, hf);
             double rho_u_avg_lo = 0.5 * (ru[q - 1] * 1.0 + ru[q] * 1.0);
-            double interp_lo = interp_at(ht, u, q, 1, rho_u_avg_lo, hf);
+            double interp_lo = interp_pick(mixed, ht, u, q, 1, rho_u_avg_lo, hf);
             double centFluxXXNext = rho_u_avg_hi * interp_hi * 0.5 * (Ax[q] + Ax[q + 1]);
             double centFluxXXPrev = rho_u_avg_lo * interp_lo * 0.5 * (Ax[q] + Ax[q - 1]);
             double rho_v_avg_hi = 0.5 * (rv[q + sy] * 1.0 + rv[q - 1 + sy] * 1.0);
-            interp_hi = interp_at(ht, u, q + sy, sy, rho_v_avg_hi, hf);
+            interp_hi = interp_pick(mixed, ht, u, q + sy, sy, rho_v_avg_hi, hf);
             double rho_v_avg_lo = 0.5 * (rv[q] * 1.0 + rv[q - 1] * 1.0);
-            interp_lo = interp_at(ht, u, q, sy, rho_v_avg_lo, hf);
+            interp_lo = interp_pick(mixed, ht, u, q, sy, rho_v_avg_lo, hf);
             double edgeFluxXYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i, j + 1, k, dzi);
             double edgeFluxXYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
             double Omega_avg_hi = 0.5 * (om[q + sz] + om[q - 1 + sz]) * 0.5 * (Az[q + sz] + Az[q - 1 + sz]);
             double Omega_avg_lo = 0.5 * (om[q] + om[q - 1]) * 0.5 * (Az[q] + Az[q - 1]);
-            interp_hi = interp_at(vt, u, q + sz, sz, Omega_avg_hi, vf);
-            interp_lo = interp_at(vt, u, q, sz, Omega_avg_lo, vf);
+            interp_hi = interp_pick(mixed, vt, u, q + sz, sz, Omega_avg_hi, vf);
+            interp_lo = interp_pick(mixed, vt, u, q, sz, Omega_avg_lo, vf);
             double edgeFluxXZNext = Omega_avg_hi * interp_hi;
             double edgeFluxXZPrev = Omega_avg_lo * interp_lo;
             double mfsq = 1 / (1.0 * 1.0);
@@ -242,20 +365,20 @@ struct TAdv {
         } else {
             double rho_u_avg_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q + 1 - sy] * 1.0);
             double rho_u_avg_lo = 0.5 * (ru[q] * 1.0 + ru[q - sy] * 1.0);
-            double interp_hi = interp_at(ht, v, q + 1, 1, rho_u_avg_hi, hf);
-            double interp_lo = interp_at(ht, v, q, 1, rho_u_avg_lo, hf);
+            double interp_hi = interp_pick(mixed, ht, v, q + 1, 1, rho_u_avg_hi, hf);
+            double interp_lo = interp_pick(mixed, ht, v, q, 1, rho_u_avg_lo, hf);
             double edgeFluxYXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeK(z, i + 1, j, k, dzi);
             double edgeFluxYXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeK(z, i, j, k, dzi);
             double rho_v_avg_hi = 0.5 * (rv[q] * 1.0 + rv[q + sy] * 1.0);
             double rho_v_avg_lo = 0.5 * (rv[q] * 1.0 + rv[q - sy] * 1.0);
-            interp_hi = interp_at(ht, v, q + sy, sy, rho_v_avg_hi, hf);
-            interp_lo = interp_at(ht, v, q, sy, rho_v_avg_lo, hf);
+            interp_hi = interp_pick(mixed, ht, v, q + sy, sy, rho_v_avg_hi, hf);
+            interp_lo = interp_pick(mixed, ht, v, q, sy, rho_v_avg_lo, hf);
             double centFluxYYNext = rho_v_avg_hi * 0.5 * (Ay[q] + Ay[q + sy]) * interp_hi;
             double centFluxYYPrev = rho_v_avg_lo * 0.5 * (Ay[q] + Ay[q - sy]) * interp_lo;
             double Omega_avg_hi = 0.5 * (om[q + sz] + om[q - sy + sz]) * 0.5 * (Az[q + sz] + Az[q - sy + sz]);
             double Omega_avg_lo = 0.5 * (om[q] + om[q - sy]) * 0.5 * (Az[q] + Az[q - sy]);
-            interp_hi = interp_at(vt, v, q + sz, sz, Omega_avg_hi, vf);
-            interp_lo = interp_at(vt, v, q, sz, Omega_avg_lo, vf);
+            interp_hi = interp_pick(mixed, vt, v, q + sz, sz, Omega_avg_hi, vf);
+            interp_lo = interp_pick(mixed, vt, v, q, sz, Omega_avg_lo, vf);
             double edgeFluxYZNext = Omega_avg_hi * interp_hi;
             double edgeFluxYZPrev = Omega_avg_lo * interp_lo;
             double mfsq = 1 / (1.0 * 1.0);
@@ -287,14 +410,14 @@ struct TAdv {
         } else {
             double rho_u_avg_hi = 0.5 * (ru[q + 1] + ru[q + 1 - sz]) * 1.0;
             double rho_u_avg_lo = 0.5 * (ru[q] + ru[q - sz]) * 1.0;
-            double interp_hi = interp_at(ht, w, q + 1, 1, rho_u_avg_hi, hf);
-            double interp_lo = interp_at(ht, w, q, 1, rho_u_avg_lo, hf);
+            double interp_hi = interp_pick(mixed, ht, w, q + 1, 1, rho_u_avg_hi, hf);
+            double interp_lo = interp_pick(mixed, ht, w, q, 1, rho_u_avg_lo, hf);
             double edgeFluxZXNext = rho_u_avg_hi * interp_hi * met::h_zeta_EdgeJ(z, i + 1, j, k, dzi);
             double edgeFluxZXPrev = rho_u_avg_lo * interp_lo * met::h_zeta_EdgeJ(z, i, j, k, dzi);
             double rho_v_avg_hi = 0.5 * (rv[q + sy] + rv[q + sy - sz]) * 1.0;
-            interp_hi = interp_at(ht, w, q + sy, sy, rho_v_avg_hi, hf);
+            interp_hi = interp_pick(mixed, ht, w, q + sy, sy, rho_v_avg_hi, hf);
             double rho_v_avg_lo = 0.5 * (rv[q] + rv[q - sz]) * 1.0;
-            interp_lo = interp_at(ht, w, q, sy, rho_v_avg_lo, hf);
+            interp_lo = interp_pick(mixed, ht, w, q, sy, rho_v_avg_lo, hf);
             double edgeFluxZYNext = rho_v_avg_hi * interp_hi * met::h_zeta_EdgeI(z, i, j + 1, k, dzi);
             double edgeFluxZYPrev = rho_v_avg_lo * interp_lo * met::h_zeta_EdgeI(z, i, j, k, dzi);
             double Omega_avg_hi = (k == hz) ? om[q] * Az[q] : 0.25 * (om[q] + om[q + sz]) * (Az[q] + Az[q + sz]);
@@ -310,7 +433,7 @@ struct TAdv {
                     else
                         interp_hi = interp_z_all(w, q + sz, sz, Omega_avg_hi, vf, vt);
                 } else {
-                    interp_hi = interp_at(vt, w, q + sz, sz, Omega_avg_hi, vf);
+                    interp_hi = interp_pick(mixed, vt, w, q + sz, sz, Omega_avg_hi, vf);
                 }
                 centFluxZZNext *= interp_hi;
             }
@@ -327,7 +450,7 @@ struct TAdv {
                     else
                         interp_lo = interp_z_all(w, q, sz, Omega_avg_lo, vf, vt);
                 } else {
-                    interp_lo = interp_at(vt, w, q, sz, Omega_avg_lo, vf);
+                    interp_lo = interp_pick(mixed, vt, w, q, sz, Omega_avg_lo, vf);
                 }
                 centFluxZZPrev *= interp_lo;
             }
@@ -615,6 +738,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const int ht = P.cfg.dycore_horiz_adv, vt = P.cfg.dycore_vert_adv;
     const double hf = P.cfg.dycore_horiz_upw_frac, vf = P.cfg.dycore_vert_upw_frac;
     const bool c2c2 = (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2);
+    const bool mixed_rt = P.cfg.precision == ERFX_PREC_MIXED;
     double ae_theta = consA ? P.cfg.alpha_T : P.cfg.rho0_trans * P.cfg.alpha_T;
     const double* mu_th_h = eddy ? eddy + 1 * ne : nullptr;
     const double* mu_th_v = eddy ? eddy + 7 * ne : nullptr;
@@ -655,7 +779,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
         });
         // cell RHS of rho (AdvectionSrcForRho) and rho*theta (AdvectionSrcForScalars
         // with 1/detJ, :298), from the area-weighted fluxes
-        launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
+        launch_box_prec(mixed_rt, Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k, auto mc) {
+            constexpr bool mixed = decltype(mc)::value;
             const long q = L.ix(i, j, k);
             double fx0 = sxm[q], fx1 = sxm[q + 1], fy0 = sym[q], fy1 = sym[q + sy], fz0 = szm[q], fz1 = szm[q + sz];
             const double mfsq = 1.0 * 1.0;
@@ -673,12 +798,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 gz0 = fz0 * (0.5 * (prim[q] + prim[q - sz]));
                 gz1 = fz1 * (0.5 * (prim[q + sz] + prim[q]));
             } else {
-                gx0 = fx0 * interp_at(ht, prim, q, 1, fx0, hf);
-                gx1 = fx1 * interp_at(ht, prim, q + 1, 1, fx1, hf);
-                gy0 = fy0 * interp_at(ht, prim, q, sy, fy0, hf);
-                gy1 = fy1 * interp_at(ht, prim, q + sy, sy, fy1, hf);
-                gz0 = fz0 * interp_at(vt, prim, q, sz, fz0, vf);
-                gz1 = fz1 * interp_at(vt, prim, q + sz, sz, fz1, vf);
+                gx0 = fx0 * interp_pick(mixed, ht, prim, q, 1, fx0, hf);
+                gx1 = fx1 * interp_pick(mixed, ht, prim, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_pick(mixed, ht, prim, q, sy, fy0, hf);
+                gy1 = fy1 * interp_pick(mixed, ht, prim, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_pick(mixed, vt, prim, q, sz, fz0, vf);
+                gz1 = fz1 * interp_pick(mixed, vt, prim, q + sz, sz, fz1, vf);
             }
             double invdetJ = (dJ > 0.) ? 1. / dJ : 1.;
             double r1 = -invdetJ * mfsq * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
@@ -691,7 +816,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             cr1[q] = r1;
         });
     } else {
-        launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
+        launch_box_prec(mixed_rt, Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k, auto mc) {
+            constexpr bool mixed = decltype(mc)::value;
             const long q = L.ix(i, j, k);
             double fx0 = ru[q], fx1 = ru[q + 1], fy0 = rv[q], fy1 = rv[q + sy], fz0 = rw[q], fz1 = rw[q + sz];
             cr0[q] = -(1.0 * 1.0) / 1.0 * ((fx1 - fx0) * dxi + (fy1 - fy0) * dyi + (fz1 - fz0) * dzi);
@@ -704,12 +830,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                 gz0 = fz0 * (0.5 * (prim[q] + prim[q - sz]));
                 gz1 = fz1 * (0.5 * (prim[q + sz] + prim[q]));
             } else {
-                gx0 = fx0 * interp_at(ht, prim, q, 1, fx0, hf);
-                gx1 = fx1 * interp_at(ht, prim, q + 1, 1, fx1, hf);
-                gy0 = fy0 * interp_at(ht, prim, q, sy, fy0, hf);
-                gy1 = fy1 * interp_at(ht, prim, q + sy, sy, fy1, hf);
-                gz0 = fz0 * interp_at(vt, prim, q, sz, fz0, vf);
-                gz1 = fz1 * interp_at(vt, prim, q + sz, sz, fz1, vf);
+                gx0 = fx0 * interp_pick(mixed, ht, prim, q, 1, fx0, hf);
+                gx1 = fx1 * interp_pick(mixed, ht, prim, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_pick(mixed, ht, prim, q, sy, fy0, hf);
+                gy1 = fy1 * interp_pick(mixed, ht, prim, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_pick(mixed, vt, prim, q, sz, fz0, vf);
+                gz1 = fz1 * interp_pick(mixed, vt, prim, q + sz, sz, fz1, vf);
             }
             double r1 = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
             if (use_diff) {
@@ -759,8 +885,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     double* rwr = rhs.zmom;
     const double* om = P.terrain ? W.omega : rw;   // Omega = rho*w on a flat grid
     const bool terr = P.terrain;
-    const TAdv TA{ru, rv, u, v, w, om, P.ax, P.ay, P.az, P.detJ, Zn{P.z_nd, L}, sy, sz, dxi, dyi, dzi, hf, vf,
-                  ht, vt, nz, c2c2};
+    const TAdvData TD{ru, rv, u, v, w, om, P.ax, P.ay, P.az, P.detJ, Zn{P.z_nd, L}, sy, sz, dxi, dyi, dzi, hf, vf,
+                      ht, vt, nz, c2c2};
     const Zn zt{P.z_nd, L};
     const double* pq1 = prim + PRIM_Q1 * ne;
     const double* pq2 = prim + PRIM_Q2 * ne;
@@ -768,7 +894,11 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const int btype = moist ? P.cfg.buoyancy_type : 1;
     const int nqs = P.n_qstate;
     const double* pavg = W.pavg;
+    // momentum face values stay FP64 in both forms: this kernel streams at
+    // near the HBM peak, and FP32 face values only add registers to it
     launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+        constexpr bool mixed = false;
+        const TAdv<mixed> TA{TD};
         const long q = L.ix(i, j, k);
         // ---- x-momentum on x-faces ----
         if (j <= ny - 1 && k <= nz - 1) {
@@ -786,17 +916,17 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             } else {
                 double a_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q] * 1.0);
-                double xflux_hi = a_hi * interp_at(ht, u, q + 1, 1, a_hi, hf);
+                double xflux_hi = a_hi * interp_pick(mixed, ht, u, q + 1, 1, a_hi, hf);
                 double a_lo = 0.5 * (ru[q - 1] * 1.0 + ru[q] * 1.0);
-                double xflux_lo = a_lo * interp_at(ht, u, q, 1, a_lo, hf);
+                double xflux_lo = a_lo * interp_pick(mixed, ht, u, q, 1, a_lo, hf);
                 double b_hi = 0.5 * (rv[q + sy] * 1.0 + rv[q - 1 + sy] * 1.0);
-                double yflux_hi = b_hi * interp_at(ht, u, q + sy, sy, b_hi, hf);
+                double yflux_hi = b_hi * interp_pick(mixed, ht, u, q + sy, sy, b_hi, hf);
                 double b_lo = 0.5 * (rv[q] * 1.0 + rv[q - 1] * 1.0);
-                double yflux_lo = b_lo * interp_at(ht, u, q, sy, b_lo, hf);
+                double yflux_lo = b_lo * interp_pick(mixed, ht, u, q, sy, b_lo, hf);
                 double c_hi = 0.5 * (om[q + sz] + om[q - 1 + sz]);
-                double zflux_hi = c_hi * interp_at(vt, u, q + sz, sz, c_hi, vf);
+                double zflux_hi = c_hi * interp_pick(mixed, vt, u, q + sz, sz, c_hi, vf);
                 double c_lo = 0.5 * (om[q] + om[q - 1]);
-                double zflux_lo = c_lo * interp_at(vt, u, q, sz, c_lo, vf);
+                double zflux_lo = c_lo * interp_pick(mixed, vt, u, q, sz, c_lo, vf);
                 double mfsq = 1 / (1.0 * 1.0);
                 src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq +
                       (zflux_hi - zflux_lo) * dzi;
@@ -852,17 +982,17 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             } else {
                 double a_hi = 0.5 * (ru[q + 1] * 1.0 + ru[q + 1 - sy] * 1.0);
-                double xflux_hi = a_hi * interp_at(ht, v, q + 1, 1, a_hi, hf);
+                double xflux_hi = a_hi * interp_pick(mixed, ht, v, q + 1, 1, a_hi, hf);
                 double a_lo = 0.5 * (ru[q] * 1.0 + ru[q - sy] * 1.0);
-                double xflux_lo = a_lo * interp_at(ht, v, q, 1, a_lo, hf);
+                double xflux_lo = a_lo * interp_pick(mixed, ht, v, q, 1, a_lo, hf);
                 double b_hi = 0.5 * (rv[q] * 1.0 + rv[q + sy] * 1.0);
-                double yflux_hi = b_hi * interp_at(ht, v, q + sy, sy, b_hi, hf);
+                double yflux_hi = b_hi * interp_pick(mixed, ht, v, q + sy, sy, b_hi, hf);
                 double b_lo = 0.5 * (rv[q] * 1.0 + rv[q - sy] * 1.0);
-                double yflux_lo = b_lo * interp_at(ht, v, q, sy, b_lo, hf);
+                double yflux_lo = b_lo * interp_pick(mixed, ht, v, q, sy, b_lo, hf);
                 double c_hi = 0.5 * (om[q + sz] + om[q - sy + sz]);
-                double zflux_hi = c_hi * interp_at(vt, v, q + sz, sz, c_hi, vf);
+                double zflux_hi = c_hi * interp_pick(mixed, vt, v, q + sz, sz, c_hi, vf);
                 double c_lo = 0.5 * (om[q] + om[q - sy]);
-                double zflux_lo = c_lo * interp_at(vt, v, q, sz, c_lo, vf);
+                double zflux_lo = c_lo * interp_pick(mixed, vt, v, q, sz, c_lo, vf);
                 double mfsq = 1 / (1.0 * 1.0);
                 src = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq +
                       (zflux_hi - zflux_lo) * dzi;
@@ -919,13 +1049,13 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             } else {
                 double a_hi = 0.5 * (ru[q + 1] + ru[q + 1 - sz]) * 1.0;
-                double xflux_hi = a_hi * interp_at(ht, w, q + 1, 1, a_hi, hf);
+                double xflux_hi = a_hi * interp_pick(mixed, ht, w, q + 1, 1, a_hi, hf);
                 double a_lo = 0.5 * (ru[q] + ru[q - sz]) * 1.0;
-                double xflux_lo = a_lo * interp_at(ht, w, q, 1, a_lo, hf);
+                double xflux_lo = a_lo * interp_pick(mixed, ht, w, q, 1, a_lo, hf);
                 double b_hi = 0.5 * (rv[q + sy] + rv[q + sy - sz]) * 1.0;
-                double yflux_hi = b_hi * interp_at(ht, w, q + sy, sy, b_hi, hf);
+                double yflux_hi = b_hi * interp_pick(mixed, ht, w, q + sy, sy, b_hi, hf);
                 double b_lo = 0.5 * (rv[q] + rv[q - sz]) * 1.0;
-                double yflux_lo = b_lo * interp_at(ht, w, q, sy, b_lo, hf);
+                double yflux_lo = b_lo * interp_pick(mixed, ht, w, q, sy, b_lo, hf);
                 double zflux_hi, zflux_lo;
                 if (k == hz) {
                     zflux_hi = om[q] * w[q];
@@ -939,7 +1069,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                         else
                             ih = interp_z_all(w, q + sz, sz, c_hi, vf, vt);
                     } else {
-                        ih = interp_at(vt, w, q + sz, sz, c_hi, vf);
+                        ih = interp_pick(mixed, vt, w, q + sz, sz, c_hi, vf);
                     }
                     zflux_hi = c_hi * ih;
                 }
@@ -955,7 +1085,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                         else
                             il = interp_z_all(w, q, sz, c_lo, vf, vt);
                     } else {
-                        il = interp_at(vt, w, q, sz, c_lo, vf);
+                        il = interp_pick(mixed, vt, w, q, sz, c_lo, vf);
                     }
                     zflux_lo = c_lo * il;
                 }
@@ -1032,6 +1162,7 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     const int vt = is_weno(ht) ? ht : B.vt;
     const double hf = B.hf, vf = B.vf;
     const bool c2c2 = (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2);
+    const bool mixed_rt = P.cfg.precision == ERFX_PREC_MIXED;
     const bool use_diff = P.use_diff, consA = P.consA, turb = P.use_turb;
     const double ae = consA ? P.cfg.alpha_C : P.cfg.rho0_trans * P.cfg.alpha_C;
     const double* mu_h = eddy ? eddy + B.mu_h * ne : nullptr;
@@ -1045,7 +1176,8 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     double* nc = snew.cons;
     const double* oc = old.cons;
     double* crb = rhs.cons;
-    launch_box(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k) {
+    launch_box_prec(mixed_rt, Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k, auto mc) {
+        constexpr bool mixed = decltype(mc)::value;
         const long q = L.ix(i, j, k);
         if (copy_slow)
             for (int c = 2; c < ncons; ++c) cur[q + c * ne] = nc[q + c * ne];
@@ -1062,12 +1194,12 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
                 gz0 = fz0 * (0.5 * (ps[q] + ps[q - sz]));
                 gz1 = fz1 * (0.5 * (ps[q + sz] + ps[q]));
             } else {
-                gx0 = fx0 * interp_scalar_at(ht, ps, q, 1, fx0, hf);
-                gx1 = fx1 * interp_scalar_at(ht, ps, q + 1, 1, fx1, hf);
-                gy0 = fy0 * interp_scalar_at(ht, ps, q, sy, fy0, hf);
-                gy1 = fy1 * interp_scalar_at(ht, ps, q + sy, sy, fy1, hf);
-                gz0 = fz0 * interp_scalar_at(vt, ps, q, sz, fz0, vf);
-                gz1 = fz1 * interp_scalar_at(vt, ps, q + sz, sz, fz1, vf);
+                gx0 = fx0 * interp_scalar_pick(mixed, ht, ps, q, 1, fx0, hf);
+                gx1 = fx1 * interp_scalar_pick(mixed, ht, ps, q + 1, 1, fx1, hf);
+                gy0 = fy0 * interp_scalar_pick(mixed, ht, ps, q, sy, fy0, hf);
+                gy1 = fy1 * interp_scalar_pick(mixed, ht, ps, q + sy, sy, fy1, hf);
+                gz0 = fz0 * interp_scalar_pick(mixed, vt, ps, q, sz, fz0, vf);
+                gz1 = fz1 * interp_scalar_pick(mixed, vt, ps, q + sz, sz, fz1, vf);
             }
             double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
             double r = -invdetJ * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);

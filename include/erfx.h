@@ -154,7 +154,17 @@ typedef struct erfx_config {
      * erf_fast_rhs_T's acoustic substep and the metric terms of the slow RHS.
      * dz is the computational spacing (detJ = height of a cell / dz). */
     int    use_terrain;
+
+    /* ERFX_PREC_FP64 (default) or ERFX_PREC_MIXED: the mixed form evaluates
+     * the face values of the advected scalars and of theta in FP32, as
+     * perturbations about an FP64 reference taken at each face, and keeps
+     * the flux products, their divergence, momentum advection and everything
+     * else (acoustic solve, averaged mass fluxes, base state, sums) in FP64.
+     * No counterpart in the reference, whose results the FP64 form reproduces. */
+    int    precision;
 } erfx_config;
+
+enum { ERFX_PREC_FP64 = 0, ERFX_PREC_MIXED = 1 };
 
 /* ---- layout shared by every field of a context ------------------------- */
 typedef struct erfx_layout {
