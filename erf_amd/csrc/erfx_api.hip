@@ -130,6 +130,7 @@ struct erfx_ctx {
     // terrain-following grid: node heights and metrics (the Params pointers
     // refer to these) and the work fields of erf_fast_rhs_T
     double *z_nd = nullptr, *detJ = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *z_cc = nullptr;
+    double *mrx = nullptr, *mry = nullptr;
     double* ray_pt = nullptr;   // per-point Rayleigh factors on a terrain grid (RPT_* blocks)
     TerrainWork tw{};
     // learned microphysics work space (allocated on first use): per-cell slot,
@@ -332,18 +333,20 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
         c->kbad = reinterpret_cast<int*>(kb);
     }
     if (c->P.terrain) {
-        double** tb[] = {&c->z_nd, &c->detJ, &c->ax, &c->ay, &c->az, &c->z_cc, &c->sw.omega, &c->tw.du, &c->tw.dv,
-                         &c->tw.nu, &c->tw.nv, &c->tw.om};
+        double** tb[] = {&c->z_nd, &c->detJ, &c->ax, &c->ay, &c->az, &c->z_cc, &c->mrx, &c->mry, &c->sw.omega,
+                         &c->tw.du, &c->tw.dv, &c->tw.nu, &c->tw.nv, &c->tw.om};
         for (double** b : tb)
             if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
         c->P.z_nd = c->z_nd; c->P.detJ = c->detJ; c->P.ax = c->ax; c->P.ay = c->ay; c->P.az = c->az;
         c->P.z_cc = c->z_cc;
+        c->P.mrx = c->mrx;
+        c->P.mry = c->mry;
         // flat node heights until erfx_set_terrain gives the real ones
         std::vector<double> zf(ne);
         for (size_t q = 0; q < ne; ++q) zf[q] = cfg->prob_lo[2] + (double)((long)(q / lay.sz) - lay.ng) * cfg->dz;
         hipError_t e = hipMemcpy(c->z_nd, zf.data(), ne * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) { erfx_ctx_destroy(c); return hip_fail(e, "hipMemcpy"); }
-        terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->stream);
+        terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->mrx, c->mry, c->stream);
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) { erfx_ctx_destroy(c); return hip_fail(e, "terrain metrics"); }
     }
@@ -394,7 +397,7 @@ int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
     if (!c->P.terrain) { set_error("the context was made without use_terrain"); return ERFX_ERR_STATE; }
     if (z_nd != c->z_nd)
         ERFX_HIP(hipMemcpyAsync(c->z_nd, z_nd, (size_t)c->P.L.ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->stream);
+    terrain_metrics(c->P, c->z_nd, c->detJ, c->ax, c->ay, c->az, c->z_cc, c->mrx, c->mry, c->stream);
     int rc = launch_status();
     if (rc) return rc;
     // dz_min of the time-step estimate (ERF.cpp:898-906): dz times the smallest

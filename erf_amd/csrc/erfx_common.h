@@ -193,6 +193,10 @@ struct Params {
     const double* ay = nullptr;
     const double* az = nullptr;
     const double* z_cc = nullptr;
+    // h_xi / h_zeta on x-faces and h_eta / h_zeta on y-faces of the valid box,
+    // the factors of the pressure-gradient metric term (static terrain)
+    const double* mrx = nullptr;
+    const double* mry = nullptr;
     double dz_min = 0.0;      // dz, times the smallest detJ of the valid cells on a terrain grid
     // terrain grid with Rayleigh damping: factors and flags per point (the
     // heights vary along a level), RPT_* blocks of ne values

@@ -74,8 +74,10 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
                 double dtau, double beta_s, double facinv, hipStream_t s);
 // make_J, make_areas, make_zcc (ERF_TerrainMetrics.cpp:583-715) on the whole
 // layout: detJ, ax, ay, az = 1 and z_cc = 0 outside the boxes they are made on
+// (and mrx, mry: the ratios h_xi / h_zeta, h_eta / h_zeta of the pressure
+// gradient's metric term on the x- and y-faces of the valid box, 0 elsewhere)
 void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* ax, double* ay, double* az,
-                     double* z_cc, hipStream_t s);
+                     double* z_cc, double* mrx, double* mry, hipStream_t s);
 
 // time-step estimate: max over valid cells of the acoustic and the advective
 // inverse time scales into host[0], host[1] (stream-ordered; sync before use)

@@ -888,6 +888,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const TAdvData TD{ru, rv, u, v, w, om, P.ax, P.ay, P.az, P.detJ, Zn{P.z_nd, L}, sy, sz, dxi, dyi, dzi, hf, vf,
                       ht, vt, nz, c2c2};
     const Zn zt{P.z_nd, L};
+    const double* mrx = P.mrx;
+    const double* mry = P.mry;
     const double* pq1 = prim + PRIM_Q1 * ne;
     const double* pq2 = prim + PRIM_Q2 * ne;
     const double* pq3 = prim + PRIM_Q3 * ne;
@@ -949,8 +951,6 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             xs += 0.5 * (cons[q] + cons[q - 1]) * g0;
             double gpx = dxi * (pp[q] - pp[q - 1]);
             if (terr) {   // ERF_slow_rhs_pre.cpp:600-620
-                double met_h_xi = met::h_xi_Iface(zt, i, j, k, dxi);
-                double met_h_zeta = met::h_zeta_Iface(zt, i, j, k, dzi);
                 double gp_zeta_on_iface;
                 if (k == 0)
                     gp_zeta_on_iface = 0.5 * dzi * (pp[q - 1 + sz] + pp[q + sz] - pp[q - 1] - pp[q]);
@@ -958,7 +958,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                     gp_zeta_on_iface = 0.5 * dzi * (pp[q - 1] + pp[q] - pp[q - 1 - sz] - pp[q - sz]);
                 else
                     gp_zeta_on_iface = 0.25 * dzi * (pp[q - 1 + sz] + pp[q + sz] - pp[q - 1 - sz] - pp[q - sz]);
-                gpx -= (met_h_xi / met_h_zeta) * gp_zeta_on_iface;
+                gpx -= mrx[q] * gp_zeta_on_iface;   // (h_xi / h_zeta) made once
             }
             gpx *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - 1) : 0.0;
@@ -1014,8 +1014,6 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             ys += 0.5 * (cons[q] + cons[q - sy]) * g1;
             double gpy = dyi * (pp[q] - pp[q - sy]);
             if (terr) {   // ERF_slow_rhs_pre.cpp:650-669
-                double met_h_eta = met::h_eta_Jface(zt, i, j, k, dyi);
-                double met_h_zeta = met::h_zeta_Jface(zt, i, j, k, dzi);
                 double gp_zeta_on_jface;
                 if (k == 0)
                     gp_zeta_on_jface = 0.5 * dzi * (pp[q + sz] + pp[q - sy + sz] - pp[q] - pp[q - sy]);
@@ -1023,7 +1021,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                     gp_zeta_on_jface = 0.5 * dzi * (pp[q] + pp[q - sy] - pp[q - sz] - pp[q - sy - sz]);
                 else
                     gp_zeta_on_jface = 0.25 * dzi * (pp[q + sz] + pp[q - sy + sz] - pp[q - sz] - pp[q - sy - sz]);
-                gpy -= (met_h_eta / met_h_zeta) * gp_zeta_on_jface;
+                gpy -= mry[q] * gp_zeta_on_jface;   // (h_eta / h_zeta) made once
             }
             gpy *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - sy) : 0.0;

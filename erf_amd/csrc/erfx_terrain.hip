@@ -24,11 +24,12 @@ __global__ void __launch_bounds__(256) lin_kernel(long n, F g) {
 // detJ and z_cc on cells (-1..nx, -1..ny, 0..nz), ax on x-faces (-1..nx+1),
 // ay on y-faces (-1..ny+1); 1 (0 for z_cc) everywhere else, az = 1.
 void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* ax, double* ay, double* az,
-                     double* z_cc, hipStream_t s) {
+                     double* z_cc, double* mrx, double* mry, hipStream_t s) {
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const double dzInv = 1.0 / P.cfg.dz;
+    const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
     const Zn z{z_nd, L};
     const unsigned nb = (unsigned)((ne + 255) / 256);
     auto f = [=] __device__(long q) {
@@ -48,6 +49,11 @@ void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* 
         if (kin && i >= -1 && i <= nx && j >= -1 && j <= ny + 1)
             Ay = .5 * dzInv * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
         detJ[q] = J; z_cc[q] = Zc; ax[q] = Ax; ay[q] = Ay; az[q] = 1.0;
+        const bool kv = k >= 0 && k <= nz - 1;
+        mrx[q] = (kv && i >= 0 && i <= nx && j >= 0 && j <= ny - 1)
+                     ? met::h_xi_Iface(z, i, j, k, dxi) / met::h_zeta_Iface(z, i, j, k, dzi) : 0.0;
+        mry[q] = (kv && i >= 0 && i <= nx - 1 && j >= 0 && j <= ny)
+                     ? met::h_eta_Jface(z, i, j, k, dyi) / met::h_zeta_Jface(z, i, j, k, dzi) : 0.0;
     };
     hipLaunchKernelGGL(lin_kernel<decltype(f)>, dim3(nb), dim3(256), 0, s, ne, f);
 }
@@ -121,16 +127,16 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     double* aym = scratch.ymom;
     double* dxm = data.xmom;
     double* dym = data.ymom;
+    const double* mrx = P.mrx;
+    const double* mry = P.mry;
     launch_box(Box(0, 0, 0, nx, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (j <= ny - 1) {
-            double met_h_xi = met::h_xi_Iface(z, i, j, k, dxi);
-            double met_h_zeta = met::h_zeta_Iface(z, i, j, k, dzi);
             double gp_xi = (extrap[q] - extrap[q - 1]) * dxi;
             double gp_zeta_on_iface = (k == 0)
                 ? 0.5 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1] - extrap[q])
                 : 0.25 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1 - sz] - extrap[q - sz]);
-            double gpx = gp_xi - (met_h_xi / met_h_zeta) * gp_zeta_on_iface;
+            double gpx = gp_xi - mrx[q] * gp_zeta_on_iface;   // (h_xi / h_zeta) made once
             gpx *= 1.0;
             if (moist) gpx /= (1.0 + face_q(pq1, pq2, q, q - 1));
             double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
@@ -142,13 +148,11 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             dxm[q] = sxm[q] + n_;
         }
         if (i <= nx - 1) {
-            double met_h_eta = met::h_eta_Jface(z, i, j, k, dyi);
-            double met_h_zeta = met::h_zeta_Jface(z, i, j, k, dzi);
             double gp_eta = (extrap[q] - extrap[q - sy]) * dyi;
             double gp_zeta_on_jface = (k == 0)
                 ? 0.5 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q] - extrap[q - sy])
                 : 0.25 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q - sz] - extrap[q - sy - sz]);
-            double gpy = gp_eta - (met_h_eta / met_h_zeta) * gp_zeta_on_jface;
+            double gpy = gp_eta - mry[q] * gp_zeta_on_jface;   // (h_eta / h_zeta) made once
             gpy *= 1.0;
             if (moist) gpy /= (1.0 + face_q(pq1, pq2, q, q - sy));
             double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
@@ -262,36 +266,36 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             p_km2 = p_km1; p_km1 = p_k;
             q1_km1 = q1_k; q2_km1 = q2_k;
         }
-        {   // top face
-            const long qt = q0 + (long)nz * sz;
-            s_prev = (dtau * srz[qt] - fA[qt] * s_prev) * fB[qt];
-            soln[qt] = s_prev;
-            om[qt] = om_k;
-        }
+        // top face: the last elimination step (no substitution there)
+        const long qt = q0 + (long)nz * sz;
+        double sol_hi = (dtau * srz[qt] - fA[qt] * s_prev) * fB[qt];
+        cz[qt] = szm[qt];
+        // one downward sweep: substitution, rho*w of face k, then the update
+        // of cell k from its two faces
+        double om_hi = om_k, p_hi = prim[qt];
         for (int k = nz - 1; k >= 0; --k) {
             const long q = q0 + k * sz;
-            double v = soln[q] - (fC[q] * fB[q]) * s_prev;
-            soln[q] = v;
-            s_prev = v;
-        }
-        cz[q0] = szm[q0];
-        cz[q0 + nz * sz] = szm[q0 + nz * sz];
-        for (int k = 1; k <= nz - 1; ++k) {
-            const long q = q0 + k * sz;
-            double wpp = met::w_from_omega(i, j, k, soln[q], fnu, fnv, z, dxi, dyi);
-            cz[q] = szm[q] + wpp;
-        }
-        for (int k = 0; k <= nz - 1; ++k) {
-            const long q = q0 + k * sz;
-            double zflux_lo = beta_2 * soln[q] + beta_1 * om[q];
-            double zflux_hi = beta_2 * soln[q + sz] + beta_1 * om[q + sz];
+            const double sol = soln[q] - (fC[q] * fB[q]) * sol_hi;
+            if (k >= 1) {
+                double wpp = met::w_from_omega(i, j, k, sol, fnu, fnv, z, dxi, dyi);
+                cz[q] = szm[q] + wpp;
+            } else {
+                cz[q] = szm[q];
+            }
+            const double om_lo = om[q];
+            double zflux_lo = beta_2 * sol + beta_1 * om_lo;
+            double zflux_hi = beta_2 * sol_hi + beta_1 * om_hi;
             azm[q] += facinv * zflux_lo / (1.0 * 1.0);
             if (k == nz - 1) azm[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
-            double fast_rhs_rho = -(trhs0[q] + (zflux_hi - zflux_lo) * dzi) / J[q];
+            const double Jq = J[q], pq = prim[q];
+            double fast_rhs_rho = -(trhs0[q] + (zflux_hi - zflux_lo) * dzi) / Jq;
             cc0[q] += dtau * (sr0[q] + fast_rhs_rho);
-            double fast_rhs_rhotheta = -(trhs1[q] + 0.5 * (zflux_hi * (prim[q] + prim[q + sz]) -
-                                                           zflux_lo * (prim[q] + prim[q - sz])) * dzi) / J[q];
+            double fast_rhs_rhotheta = -(trhs1[q] + 0.5 * (zflux_hi * (pq + p_hi) -
+                                                           zflux_lo * (pq + prim[q - sz])) * dzi) / Jq;
             cc1[q] += dtau * (sr1[q] + fast_rhs_rhotheta);
+            sol_hi = sol;
+            om_hi = om_lo;
+            p_hi = pq;
         }
     });
 }
