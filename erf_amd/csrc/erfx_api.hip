@@ -15,6 +15,7 @@ namespace erfx {
 static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+thread_local Recorder* g_rec = nullptr;
 
 int hip_fail(hipError_t e, const char* what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -178,6 +179,7 @@ struct erfx_sim {
     bool graph_on = false;
     hipGraphExec_t gexec = nullptr;
     double g_dt = 0.0;
+    double g_sdt[6] = {};   // stage time steps of the two captured steps
     int g_ratio = 0, g_cur = -1;
 };
 
@@ -912,6 +914,15 @@ int erfx_sim_download(erfx_sim* s, int id, double* host) {
 
 // One step: ERF::Advance (ERF_Advance.cpp:20-233), advance_dycore
 // (ERF_advance_dycore.cpp:37-305) and MRISplitIntegrator::advance (ERF_MRI.H:187-334).
+// slow_dt of the three RK stages as the MRI loop forms it, time_stage - time
+// with time_stage = time + dt/3, time + dt/2, time + dt (ERF_MRI.H:241-299):
+// the sum rounds to the spacing of time, so the values vary with time
+static void stage_dts(double time, double dt, double out[3]) {
+    out[0] = (time + dt / 3.0) - time;
+    out[1] = (time + dt / 2.0) - time;
+    out[2] = (time + dt) - time;
+}
+
 static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     erfx_ctx* c = s->ctx;
     const Params& P = c->P;
@@ -953,21 +964,21 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     // the substep chain folds apply_bcs into erf_fast_rhs_N's kernels (flat grids only)
     const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !P.terrain;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
-    double time_stage = time;
+    double sdt[3];
+    stage_dts(time, dt, sdt);
     for (int nrk = 0; nrk < 3; ++nrk) {
         int nsub = 1;
         double dtau = 0;
         if (nrk == 0) {
             if (P.cfg.force_stage1_single_substep) { nsub = 1; dtau = dt / 3.0; }
             else { nsub = ratio / 3; dtau = sub_timestep; }
-            time_stage = time + dt / 3.0;
         } else if (nrk == 1) {
-            nsub = ratio / 2; dtau = sub_timestep; time_stage = time + dt / 2.0;
+            nsub = ratio / 2; dtau = sub_timestep;
         } else {
-            nsub = ratio; dtau = sub_timestep; time_stage = time + dt;
+            nsub = ratio; dtau = sub_timestep;
         }
         if (nrk > 0) cons_to_prim(P, Sn.cons, s->prim, s->pi_stage, P.ng_cons, st);
-        const double slow_dt = time_stage - time;
+        const double slow_dt = sdt[nrk];
         slow_rhs_pre(P, c->sw, nrk, slow_dt, Fs, Sn, s->prim, Sc, un, vn, wn, s->eddy, s->r0, s->p0, st);
         const double inv_fac = 1.0 / static_cast<double>(nsub);
         for (int ks = 0; ks < nsub; ++ks) {
@@ -1068,7 +1079,11 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
     }
     const bool graphable = s->graph_on && !s->mlp && !s->samples && !s->ctx->P.timer && !s->ctx->P.tuner;
     while (graphable && n + 2 <= nsteps) {
-        if (!s->gexec || s->g_dt != dt || s->g_ratio != ratio || s->g_cur != s->cur) {
+        double sdt[6];
+        stage_dts(t0 + n * dt, dt, sdt);
+        stage_dts(t0 + (n + 1) * dt, dt, sdt + 3);
+        if (!s->gexec || s->g_dt != dt || s->g_ratio != ratio || s->g_cur != s->cur ||
+            std::memcmp(sdt, s->g_sdt, sizeof(sdt)) != 0) {
             graph_free(s);
             const int cur0 = s->cur;
             ERFX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
@@ -1082,6 +1097,7 @@ int erfx_sim_step_async(erfx_sim* s, double t0, double dt, int nsteps) {
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) { s->gexec = nullptr; return hip_fail(e, "hipGraphInstantiate"); }
             s->g_dt = dt; s->g_ratio = ratio; s->g_cur = cur0;
+            std::memcpy(s->g_sdt, sdt, sizeof(sdt));
             // capture recorded the two steps without running them
         }
         ERFX_HIP(hipGraphLaunch(s->gexec, s->stream));
@@ -1256,6 +1272,304 @@ int erfx_sim_step(erfx_sim* s, double t0, double dt, int nsteps) {
     int rc = erfx_sim_step_async(s, t0, dt, nsteps);
     if (rc) return rc;
     return erfx_sim_sync(s);
+}
+
+}  // extern "C"
+
+// ---- batched ensembles ---------------------------------------------------------
+// A member's step is recorded by running the sim driver with a Recorder
+// active (every launch appends its work item); the n-th items of all members
+// form one batched launch.  A recorded step is valid for the dt, stage time
+// steps, buffer parities and column variant it was recorded with.
+namespace {
+struct BatchSlot {
+    BatchLaunch fn;
+    unsigned width, total;
+    int n;
+    size_t off;
+};
+struct BatchStep {
+    double dt = 0, sdt[3] = {};
+    int variant = 0;
+    std::vector<int> cur;            // members' buffer parity at the step's start
+    std::vector<BatchSlot> slots;
+    void* items = nullptr;           // device copy of every slot's items
+    hipGraphExec_t gexec = nullptr;
+};
+void batch_free(BatchStep* b) {
+    if (!b) return;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->items) (void)hipFree(b->items);
+    delete b;
+}
+constexpr size_t ENS_KEEP = 4;       // recorded steps kept (two parities x two stage-dt sets)
+}  // namespace
+
+struct erfx_ensemble {
+    std::vector<erfx_sim*> m;
+    hipStream_t stream = nullptr;
+    std::vector<BatchStep*> rec;     // most recently used last
+    bool graph_on = false;
+    int variant = COL_DEFAULT;
+    int tune_next = -1;              // next variant to time, -1 when not tuning
+    float tune_ms[COL_NV] = {};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long cells = 0;
+    long launches = 0;
+};
+
+// the members' parity after one step
+static void sim_flip(erfx_sim* s) {
+    s->cur = 1 - s->cur;
+    s->S[0].cons = s->cons[1 - s->cur];
+    s->S[1].cons = s->cons[s->cur];
+}
+
+static int ens_record(erfx_ensemble* e, double time, double dt, int variant, BatchStep** out) {
+    const int n = (int)e->m.size();
+    std::vector<std::vector<RecItem>> seq((size_t)n);
+    auto b = new BatchStep();
+    b->dt = dt;
+    stage_dts(time, dt, b->sdt);
+    b->variant = variant;
+    for (int k = 0; k < n; ++k) {
+        erfx_sim* s = e->m[(size_t)k];
+        Params& P = s->ctx->P;
+        const int cur = s->cur;
+        double* c0 = s->S[0].cons;
+        double* c1 = s->S[1].cons;
+        KTimer* tm = P.timer;
+        ColTuner* tu = P.tuner;
+        const int var = P.col_variant;
+        P.timer = nullptr;
+        P.tuner = nullptr;
+        P.col_variant = variant;
+        Recorder r;
+        g_rec = &r;
+        int rc = sim_one_step(s, time, dt, s->ctx->P.cfg.mri_ratio);
+        g_rec = nullptr;
+        s->cur = cur;
+        s->S[0].cons = c0;
+        s->S[1].cons = c1;
+        P.timer = tm;
+        P.tuner = tu;
+        P.col_variant = var;
+        b->cur.push_back(cur);
+        if (!rc && r.refused) {
+            set_error(std::string("not batchable: ") + r.refused);
+            rc = ERFX_ERR_UNSUPPORTED;
+        }
+        if (rc) { batch_free(b); return rc; }
+        seq[(size_t)k] = std::move(r.items);
+    }
+    const size_t ns = seq[0].size();
+    for (int k = 1; k < n; ++k) {
+        bool same = seq[(size_t)k].size() == ns;
+        for (size_t i = 0; same && i < ns; ++i) {
+            const RecItem& a = seq[0][i];
+            const RecItem& c = seq[(size_t)k][i];
+            same = a.launch == c.launch && a.width == c.width && a.size == c.size;
+        }
+        if (!same) {
+            batch_free(b);
+            set_error("ensemble members record different kernel sequences (scheme options or MRI ratio differ)");
+            return ERFX_ERR_UNSUPPORTED;
+        }
+    }
+    // item arrays, one per slot, each 256-byte aligned; t0 made cumulative
+    size_t bytes = 0;
+    for (size_t i = 0; i < ns; ++i) {
+        BatchSlot sl{seq[0][i].launch, seq[0][i].width, 0u, n, (bytes + 255) / 256 * 256};
+        bytes = sl.off + (size_t)n * seq[0][i].size;
+        b->slots.push_back(sl);
+    }
+    std::vector<unsigned char> host(bytes);
+    for (size_t i = 0; i < ns; ++i) {
+        BatchSlot& sl = b->slots[i];
+        unsigned long long t0 = 0;
+        for (int k = 0; k < n; ++k) {
+            const RecItem& it = seq[(size_t)k][i];
+            BHdr h;
+            std::memcpy(&h, it.bytes.data(), sizeof(BHdr));
+            h.t0 = (unsigned)t0;
+            t0 += h.tiles;
+            unsigned char* dst = host.data() + sl.off + (size_t)k * it.size;
+            std::memcpy(dst, it.bytes.data(), it.size);
+            std::memcpy(dst, &h, sizeof(BHdr));
+        }
+        if (t0 * sl.width >= (1ull << 32)) {
+            batch_free(b);
+            set_error("batched launch exceeds the grid limit");
+            return ERFX_ERR_UNSUPPORTED;
+        }
+        sl.total = (unsigned)t0;
+    }
+    hipError_t err = hipMalloc(&b->items, bytes > 0 ? bytes : 1);
+    if (err != hipSuccess) { b->items = nullptr; batch_free(b); return hip_fail(err, "hipMalloc"); }
+    err = hipMemcpy(b->items, host.data(), bytes, hipMemcpyHostToDevice);
+    if (err != hipSuccess) { batch_free(b); return hip_fail(err, "hipMemcpy"); }
+    *out = b;
+    return ERFX_OK;
+}
+
+static void ens_launch(const BatchStep* b, hipStream_t st) {
+    for (const BatchSlot& sl : b->slots)
+        if (sl.total) sl.fn(static_cast<const unsigned char*>(b->items) + sl.off, sl.n, sl.total, sl.width, st);
+}
+
+static int ens_one_step(erfx_ensemble* e, double time, double dt) {
+    int variant = e->tune_next >= 0 ? e->tune_next : e->variant;
+    double sdt[3];
+    stage_dts(time, dt, sdt);
+    BatchStep* b = nullptr;
+    for (size_t i = 0; i < e->rec.size() && !b; ++i) {
+        BatchStep* r = e->rec[i];
+        bool hit = r->dt == dt && r->variant == variant && std::memcmp(r->sdt, sdt, sizeof(sdt)) == 0;
+        for (size_t k = 0; hit && k < e->m.size(); ++k) hit = r->cur[k] == e->m[k]->cur;
+        if (hit) {
+            b = r;
+            e->rec.erase(e->rec.begin() + (long)i);
+            e->rec.push_back(b);
+        }
+    }
+    if (!b) {
+        int rc = ens_record(e, time, dt, variant, &b);
+        if (rc) return rc;
+        if (e->rec.size() >= ENS_KEEP) {
+            batch_free(e->rec.front());
+            e->rec.erase(e->rec.begin());
+        }
+        e->rec.push_back(b);
+    }
+    long nl = 0;
+    for (const BatchSlot& sl : b->slots) nl += sl.total ? 1 : 0;
+    e->launches = nl;
+    const bool timing = e->tune_next >= 0;
+    if (timing) ERFX_HIP(hipEventRecord(e->ev[0], e->stream));
+    if (e->graph_on) {
+        if (!b->gexec) {
+            ERFX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+            ens_launch(b, e->stream);
+            hipGraph_t g = nullptr;
+            hipError_t err = hipStreamEndCapture(e->stream, &g);
+            if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
+            err = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (err != hipSuccess) { b->gexec = nullptr; return hip_fail(err, "hipGraphInstantiate"); }
+        }
+        ERFX_HIP(hipGraphLaunch(b->gexec, e->stream));
+    } else {
+        ens_launch(b, e->stream);
+    }
+    int rc = launch_status();
+    if (rc) return rc;
+    if (timing) {
+        ERFX_HIP(hipEventRecord(e->ev[1], e->stream));
+        ERFX_HIP(hipEventSynchronize(e->ev[1]));
+        float ms = 0.f;
+        ERFX_HIP(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        e->tune_ms[e->tune_next] = ms;
+        if (++e->tune_next == COL_NV) {
+            int best = 0;
+            for (int v = 1; v < COL_NV; ++v)
+                if (e->tune_ms[v] < e->tune_ms[best]) best = v;
+            e->variant = best;
+            e->tune_next = -1;
+        }
+    }
+    for (erfx_sim* s : e->m) sim_flip(s);
+    return ERFX_OK;
+}
+
+static void ens_clear(erfx_ensemble* e) {
+    for (BatchStep* b : e->rec) batch_free(b);
+    e->rec.clear();
+}
+
+extern "C" {
+
+int erfx_ensemble_create(erfx_sim* const* members, int n, erfx_ensemble** out) {
+    if (!members || n < 1 || !out) { set_error("null argument or no members"); return ERFX_ERR_ARG; }
+    *out = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (!members[k]) { set_error("null member"); return ERFX_ERR_ARG; }
+        for (int l = 0; l < k; ++l)
+            if (members[l] == members[k]) { set_error("a sim appears twice in the ensemble"); return ERFX_ERR_ARG; }
+    }
+    auto e = new erfx_ensemble();
+    e->m.assign(members, members + n);
+    for (erfx_sim* s : e->m) e->cells += (long)s->ctx->P.L.nx * s->ctx->P.L.ny * s->ctx->P.L.nz;
+    hipError_t err = hipStreamCreate(&e->stream);
+    if (err == hipSuccess) err = hipEventCreate(&e->ev[0]);
+    if (err == hipSuccess) err = hipEventCreate(&e->ev[1]);
+    if (err != hipSuccess) { erfx_ensemble_destroy(e); return hip_fail(err, "ensemble stream/events"); }
+    *out = e;
+    return ERFX_OK;
+}
+
+int erfx_ensemble_destroy(erfx_ensemble* e) {
+    if (!e) return ERFX_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_clear(e);
+    for (hipEvent_t& v : e->ev)
+        if (v) (void)hipEventDestroy(v);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return ERFX_OK;
+}
+
+int erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps) {
+    if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (!(dt > 0) || nsteps < 0) { set_error("dt must be positive and nsteps >= 0"); return ERFX_ERR_ARG; }
+    for (erfx_sim* s : e->m) {
+        if (rayleigh_on(s->ctx->P.cfg) && !s->ctx->ray) {
+            set_error("Rayleigh damping profiles not set on a member"); return ERFX_ERR_STATE;
+        }
+        if (s->mlp || s->samples) { set_error("not batchable: learned microphysics or row sampling on a member"); return ERFX_ERR_UNSUPPORTED; }
+        ERFX_HIP(hipStreamSynchronize(s->stream));   // members' own uploads and steps done
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = ens_one_step(e, t0 + n * dt, dt);
+        if (rc) { (void)hipStreamSynchronize(e->stream); return rc; }
+    }
+    ERFX_HIP(hipStreamSynchronize(e->stream));
+    return ERFX_OK;
+}
+
+int erfx_ensemble_set_graph(erfx_ensemble* e, int on) {
+    if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
+    e->graph_on = on != 0;
+    return ERFX_OK;
+}
+
+int erfx_ensemble_autotune(erfx_ensemble* e, int on) {
+    if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
+    e->tune_next = on ? 0 : -1;
+    for (float& v : e->tune_ms) v = 0.f;
+    return ERFX_OK;
+}
+
+int erfx_ensemble_set_column_variant(erfx_ensemble* e, int variant) {
+    if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (variant < 0 || variant >= COL_NV) { set_error("column variant out of range"); return ERFX_ERR_ARG; }
+    e->variant = variant;
+    e->tune_next = -1;
+    return ERFX_OK;
+}
+
+int erfx_ensemble_column_variant(erfx_ensemble* e, int* variant, float* ms) {
+    if (!e || !variant) { set_error("null argument"); return ERFX_ERR_ARG; }
+    *variant = e->tune_next >= 0 ? -1 : e->variant;
+    if (ms)
+        for (int v = 0; v < COL_NV; ++v) ms[v] = e->tune_ms[v];
+    return ERFX_OK;
+}
+
+int erfx_ensemble_info(erfx_ensemble* e, long* launches_per_step, long* cells) {
+    if (!e || !launches_per_step || !cells) { set_error("null argument"); return ERFX_ERR_ARG; }
+    *launches_per_step = e->launches;
+    *cells = e->cells;
+    return ERFX_OK;
 }
 
 }  // extern "C"

@@ -538,7 +538,37 @@ __global__ void __launch_bounds__(256) copy_kernel(CopyList c) {
     }
 }
 
+// batched form: each member's lists copied by its own range of work-groups
+struct CopyItem {
+    BHdr h;
+    CopyList c;
+};
+__global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* it, int n) {
+    const CopyItem& m = batch_member(it, n, blockIdx.x);
+    const long stride = (long)m.h.tiles * 256;
+    const long first = (long)(blockIdx.x - m.h.t0) * 256 + threadIdx.x;
+    for (int a = 0; a < m.c.count; ++a) {
+        const long n2 = m.c.n[a] / 2;
+        const double2* src = reinterpret_cast<const double2*>(m.c.src[a]);
+        double2* dst = reinterpret_cast<double2*>(m.c.dst[a]);
+        for (long t = first; t < n2; t += stride) dst[t] = src[t];
+    }
+}
+static void copy_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+    (void)width;
+    hipLaunchKernelGGL(copy_kernel_batch, dim3(total), dim3(256), 0, s, static_cast<const CopyItem*>(items), n);
+}
+
 void copy_fields(const CopyList& c, hipStream_t s) {
+    if (g_rec) {
+        long nmax = 0;
+        for (int a = 0; a < c.count; ++a) nmax = c.n[a] > nmax ? c.n[a] : nmax;
+        const long want = (nmax / 2 + 255) / 256;
+        CopyItem it{{{0, 0, 0}, {0, 0, 0}, 1u, 1u, 0u, (unsigned)(want < 1024 ? want : 1024)}, c};
+        if (c.count == 0) it.h.tiles = 0;
+        rec_push(&copy_batch, 256u, it);
+        return;
+    }
     if (c.count == 0) return;
     hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, c);
 }

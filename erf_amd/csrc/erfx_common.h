@@ -9,7 +9,10 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "../../include/erfx.h"
 #include "erfx_pow.h"
@@ -84,6 +87,65 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned w, unsigned nb) {
 // Generic box launch: x-lanes along i (64-wide waves), 4 rows of j per block,
 // tiles ordered x, y, z; every lane checks its point against the box, so any
 // grid covering the box is safe.
+// ---- batched launches (many boxes in one kernel) ---------------------------
+// While a Recorder is active on the calling thread, launch_box, launch_cols and
+// copy_fields append their work item (the tile ranges and the functor, by
+// value) to it instead of launching.  A batch then launches the n-th item of
+// every member box in ONE kernel: the work-group ids are split into
+// consecutive ranges, one per member, and each work-group finds its member by
+// a binary search over the ranges' first ids (a uniform value, so the search
+// and the functor's captures are scalar loads).
+struct BHdr {
+    int lo[3], hi[3];
+    unsigned gx, gy;      // tiles along x and y (column launches: along x only)
+    unsigned t0, tiles;   // the member's first work-group id in the batch, and its count
+};
+// launches `total` work-groups of `width` lanes over n items stored at `items`
+using BatchLaunch = void (*)(const void* items, int n, unsigned total, unsigned width, hipStream_t s);
+struct RecItem {
+    BatchLaunch launch;   // one instance per item type, so it also identifies the kernel
+    unsigned width;       // work-group size
+    size_t size, align;   // of the item type
+    std::vector<unsigned char> bytes;
+};
+struct Recorder {
+    std::vector<RecItem> items;
+    const char* refused = nullptr;   // set by a launch that cannot be recorded
+};
+// the calling thread's active recorder (host only), or null
+extern thread_local Recorder* g_rec;
+// for host functions whose launches cannot be batched: true (and the recorder
+// marked) when recording
+inline bool rec_refuse(const char* what) {
+    if (!g_rec) return false;
+    if (!g_rec->refused) g_rec->refused = what;
+    return true;
+}
+template <class It>
+inline void rec_push(BatchLaunch fn, unsigned width, const It& item) {
+    static_assert(std::is_trivially_copyable<It>::value, "batched items are copied bytewise");
+    RecItem r;
+    r.launch = fn;
+    r.width = width;
+    r.size = sizeof(It);
+    r.align = alignof(It);
+    r.bytes.resize(sizeof(It));
+    std::memcpy(r.bytes.data(), &item, sizeof(It));
+    g_rec->items.push_back(std::move(r));
+}
+// the item of work-group t: the last one whose range starts at or before t
+// (members with no work-groups share their start with the next and are skipped)
+template <class It>
+__device__ __forceinline__ const It& batch_member(const It* it, int n, unsigned t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (it[mid].h.t0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return it[lo];
+}
+
 template <class F>
 __global__ void __launch_bounds__(256) box_kernel(Box b, unsigned gx, unsigned gy, F f) {
     const unsigned nb = gridDim.x;
@@ -97,12 +159,42 @@ __global__ void __launch_bounds__(256) box_kernel(Box b, unsigned gx, unsigned g
 }
 
 template <class F>
+struct BoxItem {
+    BHdr h;
+    F f;
+};
+template <class F>
+__global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* it, int n) {
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const BoxItem<F>& m = batch_member(it, n, t);
+    const unsigned u = t - m.h.t0;
+    const unsigned bx = u % m.h.gx, by = (u / m.h.gx) % m.h.gy, bz = u / (m.h.gx * m.h.gy);
+    int i = m.h.lo[0] + (int)(bx * 64 + threadIdx.x);
+    int j = m.h.lo[1] + (int)(by * 4 + threadIdx.y);
+    int k = m.h.lo[2] + (int)bz;
+    if (i > m.h.hi[0] || j > m.h.hi[1] || k > m.h.hi[2]) return;
+    m.f(i, j, k);
+}
+template <class F>
+void launch_box_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+    (void)width;
+    hipLaunchKernelGGL(box_kernel_batch<F>, dim3(total), dim3(64, 4, 1), 0, s, static_cast<const BoxItem<F>*>(items),
+                       n);
+}
+
+template <class F>
 inline void launch_box(const Box& b, hipStream_t s, F f) {
-    if (!b.ok()) return;
+    const bool ok = b.ok();
+    const unsigned gx = ok ? (unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64) : 1u;
+    const unsigned gy = ok ? (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4) : 1u;
+    const unsigned gz = ok ? (unsigned)(b.hi[2] - b.lo[2] + 1) : 0u;
+    if (g_rec) {   // recorded even when empty, so every member records the same sequence
+        BoxItem<F> it{{{b.lo[0], b.lo[1], b.lo[2]}, {b.hi[0], b.hi[1], b.hi[2]}, gx, gy, 0u, gx * gy * gz}, f};
+        rec_push(&launch_box_batch<F>, 256u, it);
+        return;
+    }
+    if (!ok) return;
     dim3 blk(64, 4, 1);
-    const unsigned gx = (unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64);
-    const unsigned gy = (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4);
-    const unsigned gz = (unsigned)(b.hi[2] - b.lo[2] + 1);
     hipLaunchKernelGGL(box_kernel<F>, dim3(gx * gy * gz), blk, 0, s, b, gx, gy, f);
 }
 
@@ -118,11 +210,70 @@ __global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int
     f(i, j);
 }
 
+// Work-group launch for block-level reductions: f(u) runs on all 256 lanes of
+// work-group u of a 1-D grid.
+template <class F>
+__global__ void __launch_bounds__(256) grid_kernel(F f) {
+    f(blockIdx.x);
+}
+template <class F>
+struct GridItem {
+    BHdr h;
+    F f;
+};
+template <class F>
+__global__ void __launch_bounds__(256) grid_kernel_batch(const GridItem<F>* it, int n) {
+    const GridItem<F>& m = batch_member(it, n, blockIdx.x);
+    m.f(blockIdx.x - m.h.t0);
+}
+template <class F>
+void launch_grid_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+    (void)width;
+    hipLaunchKernelGGL(grid_kernel_batch<F>, dim3(total), dim3(256), 0, s, static_cast<const GridItem<F>*>(items), n);
+}
+template <class F>
+inline void launch_grid(unsigned blocks, hipStream_t s, F f) {
+    if (g_rec) {
+        GridItem<F> it{{{0, 0, 0}, {0, 0, 0}, 1u, 1u, 0u, blocks}, f};
+        rec_push(&launch_grid_batch<F>, 256u, it);
+        return;
+    }
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(grid_kernel<F>, dim3(blocks), dim3(256), 0, s, f);
+}
+
+template <class F>
+struct ColItem {
+    BHdr h;
+    F f;
+};
+template <class F>
+__global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* it, int n) {
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const ColItem<F>& m = batch_member(it, n, t);
+    const unsigned u = t - m.h.t0;
+    int i = m.h.lo[0] + (int)((u % m.h.gx) * blockDim.x + threadIdx.x);
+    int j = m.h.lo[1] + (int)(u / m.h.gx);
+    if (i > m.h.hi[0] || j > m.h.hi[1]) return;
+    m.f(i, j);
+}
+template <class F>
+void launch_cols_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+    hipLaunchKernelGGL(col_kernel_batch<F>, dim3(total), dim3(width, 1, 1), 0, s, static_cast<const ColItem<F>*>(items),
+                       n);
+}
+
 template <class F>
 inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f, int width = 256) {
-    if (ilo > ihi || jlo > jhi) return;
-    const unsigned gx = (unsigned)((ihi - ilo + 1 + width - 1) / width);
-    const unsigned gy = (unsigned)(jhi - jlo + 1);
+    const bool ok = ilo <= ihi && jlo <= jhi;
+    const unsigned gx = ok ? (unsigned)((ihi - ilo + 1 + width - 1) / width) : 1u;
+    const unsigned gy = ok ? (unsigned)(jhi - jlo + 1) : 0u;
+    if (g_rec) {
+        ColItem<F> it{{{ilo, jlo, 0}, {ihi, jhi, 0}, gx, 1u, 0u, gx * gy}, f};
+        rec_push(&launch_cols_batch<F>, (unsigned)width, it);
+        return;
+    }
+    if (!ok) return;
     hipLaunchKernelGGL(col_kernel<F>, dim3(gx * gy), dim3((unsigned)width, 1, 1), 0, s, ilo, ihi, jlo, jhi, gx, f);
 }
 

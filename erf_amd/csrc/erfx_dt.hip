@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(256) est_dt_pass2(const double* partial, int n
 
 void est_dt_maxima(const Params& P, const double* cons, const double* u, const double* v, const double* w,
                    double* partial, double* host, hipStream_t s) {
+    if (rec_refuse("time-step estimate")) return;
     hipLaunchKernelGGL(est_dt_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, u, v, w, P.dxInv[0], P.dxInv[1],
                        1.0 / P.dz_min, P.cfg.implicit_substepping != 0, partial);   // ERF_ComputeTimestep.cpp:66
     hipLaunchKernelGGL(est_dt_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
@@ -144,6 +145,7 @@ __global__ void __launch_bounds__(256) sums_pass2(const double* partial, int nbl
 }
 
 void volume_sums(const Params& P, const double* cons, double* partial, double* host, hipStream_t s) {
+    if (rec_refuse("volume sums")) return;
     const double cell_vol = P.cfg.dx * P.cfg.dy * P.cfg.dz;
     hipLaunchKernelGGL(sums_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, cell_vol, partial);
     hipLaunchKernelGGL(sums_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
@@ -152,18 +154,17 @@ void volume_sums(const Params& P, const double* cons, double* partial, double* h
 }
 
 // ---- plane averages ------------------------------------------------------------
-// block (b, k) sums the plane terms value * denom of its share of level k
-__global__ void __launch_bounds__(256) plane_pass1(Lay L, const double* cons, const double* prim, double denom,
-                                                   int nq, int nsplit, double* partial) {
+// work-group (b, k) sums the plane terms value * denom of its share of level k
+__device__ __forceinline__ void plane_pass1(unsigned b, int k, Lay L, const double* cons, const double* prim,
+                                            double denom, int nq, int nsplit, double* partial) {
     const int nx = L.nx, ny = L.ny;
-    const int k = blockIdx.y;
     const long n = (long)nx * ny;
     DD acc[PLANE_ROWS];
     for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = DD{0.0, 0.0};
     const double* pth = prim;
     const double* pq[3] = {prim + (ERFX_RHOQ1 - 1) * L.ne, prim + (ERFX_RHOQ2 - 1) * L.ne,
                            prim + (ERFX_RHOQ3 - 1) * L.ne};
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)nsplit * blockDim.x) {
+    for (long t = (long)b * blockDim.x + threadIdx.x; t < n; t += (long)nsplit * blockDim.x) {
         const int i = (int)(t % nx), j = (int)(t / nx);
         const long q = L.ix(i, j, k);
         const double fac = 1.0;
@@ -171,13 +172,12 @@ __global__ void __launch_bounds__(256) plane_pass1(Lay L, const double* cons, co
         for (int c = 0; c < nq && c < 3; ++c) v[2 + c] = pq[c][q];
         for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = dd_add(acc[r], DD{v[r] * denom * fac, 0.0});
     }
-    block_sum<PLANE_ROWS>(acc, partial + ((long)k * nsplit + blockIdx.x) * 2 * PLANE_ROWS);
+    block_sum<PLANE_ROWS>(acc, partial + ((long)k * nsplit + b) * 2 * PLANE_ROWS);
 }
 
 // block k folds the nsplit partials of level k, then the temperature of the
 // averages (getTgivenRandRTh(rho_d, rho_d theta_d, qv_d), ERF_make_buoyancy.cpp:274-275)
-__global__ void __launch_bounds__(256) plane_pass2(const double* partial, int nsplit, int nz, double* out) {
-    const int k = blockIdx.x;
+__device__ __forceinline__ void plane_pass2(int k, const double* partial, int nsplit, int nz, double* out) {
     DD acc[PLANE_ROWS];
     for (int r = 0; r < PLANE_ROWS; ++r) acc[r] = DD{0.0, 0.0};
     const double* p = partial + (long)k * nsplit * 2 * PLANE_ROWS;
@@ -204,9 +204,13 @@ void plane_averages(const Params& P, const double* cons, const double* prim, dou
     if (nsplit > need) nsplit = (int)need;
     if (nsplit < 1) nsplit = 1;
     const double denom = 1.0 / (double)nplane;
-    hipLaunchKernelGGL(plane_pass1, dim3(nsplit, nz), dim3(256), 0, s, P.L, cons, prim, denom, P.n_qstate, nsplit,
-                       partial);
-    hipLaunchKernelGGL(plane_pass2, dim3(nz), dim3(256), 0, s, (const double*)partial, nsplit, nz, out);
+    const Lay L = P.L;
+    const int nq = P.n_qstate;
+    launch_grid((unsigned)(nsplit * nz), s, [=] __device__(unsigned u) {
+        plane_pass1(u % (unsigned)nsplit, (int)(u / (unsigned)nsplit), L, cons, prim, denom, nq, nsplit, partial);
+    });
+    const double* part = partial;
+    launch_grid((unsigned)nz, s, [=] __device__(unsigned u) { plane_pass2((int)u, part, nsplit, nz, out); });
 }
 
 }  // namespace erfx

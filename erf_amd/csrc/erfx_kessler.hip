@@ -215,6 +215,7 @@ __device__ __forceinline__ bool active_cell(double qv, double qc, double qp, dou
 // w, dt, dqv, dqc, dqr, dtheta) rows, appended at the device counter.
 void kessler_samples(const Params& P, const double* cons, const double* w, double dt, float* out, long max_rows,
                      unsigned long long* counter, int* bad, hipStream_t s) {
+    if (rec_refuse("training-row harvest")) return;
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sz = L.sz;
@@ -255,6 +256,7 @@ void kessler_samples(const Params& P, const double* cons, const double* w, doubl
 void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, double* cons, const double* w,
                          double* fz, double* rain_accum, double dt, int* slot, float* x, float* y, long* rows_cell,
                          unsigned long long* counter, unsigned long long* host_counts, int* bad, hipStream_t s) {
+    if (rec_refuse("learned microphysics")) return;
     using namespace mlp;
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;

@@ -25,6 +25,7 @@ __global__ void __launch_bounds__(256) lin_kernel(long n, F g) {
 // ay on y-faces (-1..ny+1); 1 (0 for z_cc) everywhere else, az = 1.
 void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* ax, double* ay, double* az,
                      double* z_cc, double* mrx, double* mry, hipStream_t s) {
+    if (rec_refuse("terrain metrics")) return;
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
     const long ne = L.ne, sy = L.sy, sz = L.sz;

@@ -86,6 +86,14 @@ SIGNATURES = {
     "erfx_sim_timer_reset": (_I, [_P]),
     "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(_D)]),
+    "erfx_ensemble_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "erfx_ensemble_destroy": (_I, [_P]),
+    "erfx_ensemble_step": (_I, [_P, _D, _D, _I]),
+    "erfx_ensemble_set_graph": (_I, [_P, _I]),
+    "erfx_ensemble_autotune": (_I, [_P, _I]),
+    "erfx_ensemble_set_column_variant": (_I, [_P, _I]),
+    "erfx_ensemble_column_variant": (_I, [_P, ctypes.POINTER(_I), _P]),
+    "erfx_ensemble_info": (_I, [_P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
 }
 
 
@@ -248,6 +256,50 @@ class Sim:
     @property
     def stream(self):
         return lib().erfx_sim_stream(self.h)
+
+
+class Ensemble:
+    """Sims advanced together, every kernel of the step launched once for all
+    members (erfx_ensemble).  The members stay owned by the caller."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (_P * len(self.sims))(*[s.h for s in self.sims])
+        h = _P()
+        check(lib().erfx_ensemble_create(arr, len(self.sims), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().erfx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, t0, dt, n):
+        check(lib().erfx_ensemble_step(self.h, t0, dt, n))
+
+    def set_graph(self, on=True):
+        check(lib().erfx_ensemble_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, on=True):
+        check(lib().erfx_ensemble_autotune(self.h, 1 if on else 0))
+
+    def set_column_variant(self, v):
+        check(lib().erfx_ensemble_set_column_variant(self.h, int(v)))
+
+    def column_variant(self):
+        v = _I()
+        ms = (ctypes.c_float * 9)()
+        check(lib().erfx_ensemble_column_variant(self.h, ctypes.byref(v), ms))
+        return v.value, list(ms)
+
+    def info(self):
+        """(kernel launches per step, total cells of the members)"""
+        a, b = ctypes.c_long(), ctypes.c_long()
+        check(lib().erfx_ensemble_info(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 class MLP:

@@ -433,11 +433,12 @@ def _noise_vel(cfg, L, f, rng, amp, mean):
     f[C.F_ZVEL][0][ng + nz, :, :] = 0.0
 
 
-def cbl(nx=256, ny=256, nz=128, seed=0):
+def cbl(nx=256, ny=256, nz=128, seed=0, ug0=10.0):
     """Dry convective boundary layer LES: dx = dy = 20 m, dz = 10 m, periodic
     laterally, no-slip bottom with a surface heat flux of 0.1 K m/s, slip top;
     theta = 300 K below 500 m then +0.003 K/m; uniform theta noise of +-0.25 K
-    in the lowest 200 m; geostrophic wind (10, 0) m/s with f = 1e-4 s^-1;
+    in the lowest 200 m; geostrophic wind (ug0, 0) m/s, ug0 = 10 by default,
+    with f = 1e-4 s^-1;
     5th-order upwind advection, Smagorinsky Cs = 0.1, dt = 0.5 s.
 
     The MRI ratio is 16 (1 + 8 + 16 acoustic substeps per step), not 4: at
@@ -460,7 +461,7 @@ def cbl(nx=256, ny=256, nz=128, seed=0):
     lat = 90.0 * (math.pi / 180.)
     cfg.sinphi = math.sin(lat)
     cfg.cosphi = math.cos(lat)
-    ug = (10.0, 0.0, 0.0)
+    ug = (float(ug0), 0.0, 0.0)
     cfg.abl_geo_forcing[0] = -cfg.coriolis_factor * (ug[1] * cfg.sinphi - ug[2] * cfg.cosphi)
     cfg.abl_geo_forcing[1] = cfg.coriolis_factor * ug[0] * cfg.sinphi
     cfg.abl_geo_forcing[2] = -cfg.coriolis_factor * ug[0] * cfg.cosphi
@@ -485,6 +486,22 @@ def cbl(nx=256, ny=256, nz=128, seed=0):
     _noise_vel(cfg, L, f, rng, 0.0, ug)
     f[C.F_XVEL][0][ng:ng + nz, ng:ng + ny, ng:ng + nx + 1] = ug[0]
     return cfg, f
+
+
+ENSEMBLE_SIZES = ((48, 48, 48), (64, 64, 96), (96, 96, 64), (128, 128, 80), (160, 160, 80))
+
+
+def cbl_ensemble(n=48, seed=0, sizes=ENSEMBLE_SIZES):
+    """Members of the batched ensemble: independent dry CBL boxes (cbl), each
+    with its own noise seed, a geostrophic speed drawn uniformly from
+    [5, 15] m/s and a box size drawn uniformly from `sizes`.  Yields
+    (cfg, fields) one member at a time."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(sizes), n)
+    speed = rng.uniform(5.0, 15.0, n)
+    for m in range(n):
+        nx, ny, nz = sizes[int(pick[m])]
+        yield cbl(nx, ny, nz, seed=1000 * seed + m + 1, ug0=float(speed[m]))
 
 
 def small_case(kind, nx, ny, nz, seed=0, **kw):

@@ -392,7 +392,9 @@ int  erfx_sim_sync(erfx_sim* sim);
  * for small boxes, where launch overhead dominates).  When on, erfx_sim_step*
  * captures two whole steps (the state buffers alternate, so two steps return
  * every pointer to its place) into a graph on first use, re-captures when dt
- * or the MRI ratio changes, and replays it; an odd last step runs directly.
+ * or the MRI ratio changes, or the stage time steps of the reference's
+ * MRI loop ((t + dt/3) - t etc., which round differently as t grows) differ
+ * from the captured ones, and replays it; an odd last step runs directly.
  * The results are those of the direct path.  Not used while the learned
  * microphysics, row sampling or the kernel timer is on. */
 int  erfx_sim_set_graph(erfx_sim* sim, int on);
@@ -457,6 +459,38 @@ int  erfx_sim_sample_count(erfx_sim* sim, long* count);
 int  erfx_sim_timer_reset(erfx_sim* sim);
 int  erfx_sim_timer_read(erfx_sim* sim, char* name, int name_len, double* total_ms, long* launches,
                          double* bytes_per_launch);
+
+/* ---- batched ensembles (no counterpart in the reference) --------------------
+ * ERF advances each box of a level with its own kernel launches (one
+ * ParallelFor per box, MFIter loops in ERF_slow_rhs_pre.cpp:244,
+ * ERF_fast_rhs_N.cpp:115-566, ...).  An ensemble advances n independent sims (boxes
+ * of any sizes, or ensemble members) together: every kernel of the step runs
+ * ONCE for all members, its work-groups split into one range per member.
+ * Members keep their own buffers, configs and scalars (seed, wind, dt-derived
+ * constants); they must record the same sequence of kernels (same scheme
+ * options, same MRI ratio; sizes may differ), else erfx_ensemble_step returns
+ * ERFX_ERR_UNSUPPORTED.  Not with the learned microphysics or row sampling.
+ * The results are those of stepping each member alone. */
+typedef struct erfx_ensemble erfx_ensemble;
+/* members stay owned by the caller and must outlive the ensemble */
+int  erfx_ensemble_create(erfx_sim* const* members, int n, erfx_ensemble** out);
+int  erfx_ensemble_destroy(erfx_ensemble* e);
+/* nsteps steps of size dt from t0 for every member (synchronous at return).
+ * The step's batched launch list is recorded on first use per (dt, stage time
+ * steps, members' buffer parity, column variant) and reused. */
+int  erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps);
+/* replay each recorded step as a hipGraph (one graph launch per step) */
+int  erfx_ensemble_set_graph(erfx_ensemble* e, int on);
+/* With `on`, the next 9 steps each use one variant of the acoustic column
+ * kernel for all members, timed with HIP events; the fastest is kept.
+ * Every variant computes the same values. */
+int  erfx_ensemble_autotune(erfx_ensemble* e, int on);
+int  erfx_ensemble_set_column_variant(erfx_ensemble* e, int variant);
+/* the variant in use (-1 while timing) and, when ms is not NULL, the step
+ * time in ms of each variant timed (9 values) */
+int  erfx_ensemble_column_variant(erfx_ensemble* e, int* variant, float* ms);
+/* kernel launches per step (batched) and the sum of the members' cells */
+int  erfx_ensemble_info(erfx_ensemble* e, long* launches_per_step, long* cells);
 
 #ifdef __cplusplus
 }
