@@ -49,10 +49,15 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal"], default="cbl",
+    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal", "ensemble"], default="cbl",
                     help="cbl (default): the dry CBL LES; moist_terrain: moist flow over a hill on the "
                          "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s; "
-                         "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage")
+                         "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage; "
+                         "ensemble: dry CBL members of uneven box sizes advanced as one batch")
+    ap.add_argument("--members", type=int, default=48, help="ensemble members (--workload ensemble)")
+    ap.add_argument("--ens-streams", type=int, default=2, help="ensemble: member groups, one stream each")
+    ap.add_argument("--unbatched-steps", type=int, default=2,
+                    help="ensemble: steps timed with every member launched on its own, for comparison")
     ap.add_argument("--graph", action="store_true", help="replay the steps from a captured hipGraph")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 face values of the advected quantities on perturbations; the same steps "
@@ -104,6 +109,127 @@ def cpu_baseline(nx, ny, nz, budget_s):
             "sample": "%d steps of the CBL case on a %dx%dx%d block (OpenMP C++ oracle, -O2)" % (n, sx, sy, nz)}
 
 
+def run_ensemble(a):
+    """The batched ensemble: a.members dry CBL members (own seed, geostrophic
+    speed in [5, 15] m/s, box size drawn from five shapes), every kernel of the
+    step launched once for all members.  Timed batched, then with every member
+    launched on its own; with --precision mixed the same steps run in FP64 and
+    the per-member errors are reported."""
+    import numpy as np
+    import torch
+    from erf_amd import config as C, problems as P
+    from erf_amd.lib import Sim, Ensemble
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    dt = 0.5
+    mixed = a.precision == "mixed"
+    sims, refs, shapes = [], [], []
+    for cfg, f in P.cbl_ensemble(a.members, seed=0):
+        shapes.append((cfg.nx, cfg.ny, cfg.nz))
+        if mixed:
+            c64 = C.Config.from_buffer_copy(cfg)
+            r = Sim(c64)
+            r.load(f)
+            refs.append(r)
+            cfg.precision = C.PREC_MIXED
+        s = Sim(cfg)
+        s.load(f)
+        sims.append(s)
+        del f
+    print("%d members uploaded" % len(sims), file=sys.stderr, flush=True)
+    ens = Ensemble(sims)
+    ens.set_streams(a.ens_streams)
+    if a.graph:
+        ens.set_graph(True)
+    warm = a.warmup
+    if not a.no_autotune:
+        ens.autotune(True)           # one warm-up step per column variant
+        warm = max(warm, 9)
+    ens.step(0.0, dt, warm)
+    t = warm * dt
+    variant, var_ms = ens.column_variant()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ens.step(t, dt, a.steps)
+    wall = time.perf_counter() - w0
+    t += a.steps * dt
+    launches, cells = ens.info()
+    value = cells * a.steps / wall
+    # the same members, each launched on its own (the launch pattern of one box per kernel)
+    un = None
+    if a.unbatched_steps > 0:
+        for s in sims:
+            s.set_column_variant(variant)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        for n in range(a.unbatched_steps):
+            for s in sims:
+                s.step_async(t + n * dt, dt, 1)
+        for s in sims:
+            s.sync()
+        uw = time.perf_counter() - w0
+        t += a.unbatched_steps * dt
+        un = {"value": cells * a.unbatched_steps / uw, "steps": a.unbatched_steps,
+              "ms_per_step": 1e3 * uw / a.unbatched_steps}
+    total_steps = warm + a.steps + (a.unbatched_steps if un else 0)
+    # sanity and per-member errors against FP64
+    worst_rho = 1e30
+    errs = {}
+    if mixed:
+        ens64 = Ensemble(refs)
+        ens64.step(0.0, dt, total_steps)
+        ens64.close()
+    for m, s in enumerate(sims):
+        nx, ny, nz = shapes[m]
+        ng = s.L.ng
+        sl = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+        cons = s.download(C.F_CONS)
+        worst_rho = min(worst_rho, float(cons[0][sl].min()))
+        if not mixed:
+            continue
+        ref = {C.F_CONS: refs[m].download(C.F_CONS)}
+        got = {C.F_CONS: cons}
+        for fid, comp, name in [(C.F_CONS, C.RHO, "rho"), (C.F_CONS, C.RHOTHETA, "rhotheta"), (C.F_XVEL, 0, "u"),
+                                (C.F_YVEL, 0, "v"), (C.F_ZVEL, 0, "w")]:
+            if fid not in got:
+                got[fid] = s.download(fid)
+                ref[fid] = refs[m].download(fid)
+            x = got[fid][comp][sl]
+            y = ref[fid][comp][sl]
+            r = y - y.mean() if name in ("rho", "rhotheta") else y
+            nrm = float(np.sqrt((r ** 2).mean()))
+            errs.setdefault(name, []).append(float(np.sqrt(((x - y) ** 2).mean())) / nrm if nrm > 0 else 0.0)
+    assert np.isfinite(worst_rho) and worst_rho > 0.05, "state blew up"
+    ens.close()
+    for s in sims + refs:
+        s.close()
+    hist = {}
+    for sh in shapes:
+        k = "%dx%dx%d" % sh
+        hist[k] = hist.get(k, 0) + 1
+    out = {
+        "metric": "cell-updates/s per RK3 step (batched ensemble, %s GPU path)" % ("mixed" if mixed else "FP64"),
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": warm,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if mixed else "f64", "data": "synthetic (seeded)",
+        "config": {"workload": "%d dry CBL LES members (U5, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, MRI ratio 16), "
+                               "own seeds, geostrophic speed in [5, 15] m/s, box sizes drawn from "
+                               "48^3, 64x64x96, 96x96x64, 128x128x80, 160x160x80" % len(shapes),
+                   "members": hist, "cells": cells, "precision": a.precision, "graph": bool(a.graph),
+                   "streams": a.ens_streams},
+        "launches_per_step": launches,
+        "column_variant": {"variant": variant, "step_ms": var_ms},
+        "unbatched": un,
+        "wall_s": wall,
+    }
+    if un:
+        out["batched_over_unbatched"] = value / un["value"]
+    if mixed:
+        out["error_vs_fp64_per_member"] = {k: {"max": max(v), "median": float(np.median(v))} for k, v in errs.items()}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -112,6 +238,9 @@ def main():
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy()
+        return
+    if a.workload == "ensemble":
+        run_ensemble(a)
         return
     import numpy as np
     import torch

@@ -5,6 +5,7 @@
 #include "erfx_mlp.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1305,15 +1306,22 @@ void batch_free(BatchStep* b) {
 constexpr size_t ENS_KEEP = 4;       // recorded steps kept (two parities x two stage-dt sets)
 }  // namespace
 
-struct erfx_ensemble {
+// Members are split into groups of about equal cell count, each batched on a
+// stream of its own: the groups' launches overlap, so the tail of one batched
+// kernel is filled by another group's work.
+struct EnsGroup {
     std::vector<erfx_sim*> m;
     hipStream_t stream = nullptr;
     std::vector<BatchStep*> rec;     // most recently used last
+};
+
+struct erfx_ensemble {
+    std::vector<erfx_sim*> m;
+    std::vector<EnsGroup> g;
     bool graph_on = false;
     int variant = COL_DEFAULT;
     int tune_next = -1;              // next variant to time, -1 when not tuning
     float tune_ms[COL_NV] = {};
-    hipEvent_t ev[2] = {nullptr, nullptr};
     long cells = 0;
     long launches = 0;
 };
@@ -1325,15 +1333,17 @@ static void sim_flip(erfx_sim* s) {
     s->S[1].cons = s->cons[s->cur];
 }
 
-static int ens_record(erfx_ensemble* e, double time, double dt, int variant, BatchStep** out) {
-    const int n = (int)e->m.size();
+static long sim_cells(const erfx_sim* s) { return (long)s->ctx->P.L.nx * s->ctx->P.L.ny * s->ctx->P.L.nz; }
+
+static int ens_record(const std::vector<erfx_sim*>& mem, double time, double dt, int variant, BatchStep** out) {
+    const int n = (int)mem.size();
     std::vector<std::vector<RecItem>> seq((size_t)n);
     auto b = new BatchStep();
     b->dt = dt;
     stage_dts(time, dt, b->sdt);
     b->variant = variant;
     for (int k = 0; k < n; ++k) {
-        erfx_sim* s = e->m[(size_t)k];
+        erfx_sim* s = mem[(size_t)k];
         Params& P = s->ctx->P;
         const int cur = s->cur;
         double* c0 = s->S[0].cons;
@@ -1417,58 +1427,75 @@ static void ens_launch(const BatchStep* b, hipStream_t st) {
         if (sl.total) sl.fn(static_cast<const unsigned char*>(b->items) + sl.off, sl.n, sl.total, sl.width, st);
 }
 
-static int ens_one_step(erfx_ensemble* e, double time, double dt) {
-    int variant = e->tune_next >= 0 ? e->tune_next : e->variant;
+// one group's step: the recorded launch list for (dt, stage dts, parities,
+// variant), recorded on a miss, replayed directly or as a graph
+static int ens_group_step(EnsGroup& G, bool graph_on, double time, double dt, int variant, long* launches) {
     double sdt[3];
     stage_dts(time, dt, sdt);
     BatchStep* b = nullptr;
-    for (size_t i = 0; i < e->rec.size() && !b; ++i) {
-        BatchStep* r = e->rec[i];
+    for (size_t i = 0; i < G.rec.size() && !b; ++i) {
+        BatchStep* r = G.rec[i];
         bool hit = r->dt == dt && r->variant == variant && std::memcmp(r->sdt, sdt, sizeof(sdt)) == 0;
-        for (size_t k = 0; hit && k < e->m.size(); ++k) hit = r->cur[k] == e->m[k]->cur;
+        for (size_t k = 0; hit && k < G.m.size(); ++k) hit = r->cur[k] == G.m[k]->cur;
         if (hit) {
             b = r;
-            e->rec.erase(e->rec.begin() + (long)i);
-            e->rec.push_back(b);
+            G.rec.erase(G.rec.begin() + (long)i);
+            G.rec.push_back(b);
         }
     }
     if (!b) {
-        int rc = ens_record(e, time, dt, variant, &b);
+        int rc = ens_record(G.m, time, dt, variant, &b);
         if (rc) return rc;
-        if (e->rec.size() >= ENS_KEEP) {
-            batch_free(e->rec.front());
-            e->rec.erase(e->rec.begin());
+        if (G.rec.size() >= ENS_KEEP) {
+            batch_free(G.rec.front());
+            G.rec.erase(G.rec.begin());
         }
-        e->rec.push_back(b);
+        G.rec.push_back(b);
     }
-    long nl = 0;
-    for (const BatchSlot& sl : b->slots) nl += sl.total ? 1 : 0;
-    e->launches = nl;
-    const bool timing = e->tune_next >= 0;
-    if (timing) ERFX_HIP(hipEventRecord(e->ev[0], e->stream));
-    if (e->graph_on) {
+    for (const BatchSlot& sl : b->slots) *launches += sl.total ? 1 : 0;
+    if (graph_on) {
         if (!b->gexec) {
-            ERFX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
-            ens_launch(b, e->stream);
+            ERFX_HIP(hipStreamBeginCapture(G.stream, hipStreamCaptureModeRelaxed));
+            ens_launch(b, G.stream);
             hipGraph_t g = nullptr;
-            hipError_t err = hipStreamEndCapture(e->stream, &g);
+            hipError_t err = hipStreamEndCapture(G.stream, &g);
             if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
             err = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (err != hipSuccess) { b->gexec = nullptr; return hip_fail(err, "hipGraphInstantiate"); }
         }
-        ERFX_HIP(hipGraphLaunch(b->gexec, e->stream));
+        ERFX_HIP(hipGraphLaunch(b->gexec, G.stream));
     } else {
-        ens_launch(b, e->stream);
+        ens_launch(b, G.stream);
     }
-    int rc = launch_status();
-    if (rc) return rc;
+    return launch_status();
+}
+
+static int ens_sync(erfx_ensemble* e) {
+    for (EnsGroup& G : e->g) ERFX_HIP(hipStreamSynchronize(G.stream));
+    return ERFX_OK;
+}
+
+static int ens_one_step(erfx_ensemble* e, double time, double dt) {
+    const int variant = e->tune_next >= 0 ? e->tune_next : e->variant;
+    const bool timing = e->tune_next >= 0;
+    std::chrono::steady_clock::time_point t0;
     if (timing) {
-        ERFX_HIP(hipEventRecord(e->ev[1], e->stream));
-        ERFX_HIP(hipEventSynchronize(e->ev[1]));
-        float ms = 0.f;
-        ERFX_HIP(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
-        e->tune_ms[e->tune_next] = ms;
+        int rc = ens_sync(e);
+        if (rc) return rc;
+        t0 = std::chrono::steady_clock::now();
+    }
+    long nl = 0;
+    for (EnsGroup& G : e->g) {
+        int rc = ens_group_step(G, e->graph_on, time, dt, variant, &nl);
+        if (rc) return rc;
+    }
+    e->launches = nl;
+    if (timing) {
+        int rc = ens_sync(e);
+        if (rc) return rc;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        e->tune_ms[e->tune_next] = (float)ms;
         if (++e->tune_next == COL_NV) {
             int best = 0;
             for (int v = 1; v < COL_NV; ++v)
@@ -1482,8 +1509,37 @@ static int ens_one_step(erfx_ensemble* e, double time, double dt) {
 }
 
 static void ens_clear(erfx_ensemble* e) {
-    for (BatchStep* b : e->rec) batch_free(b);
-    e->rec.clear();
+    for (EnsGroup& G : e->g) {
+        for (BatchStep* b : G.rec) batch_free(b);
+        G.rec.clear();
+    }
+}
+
+// split the members into k groups: largest first, each to the group with the
+// fewest cells so far
+static int ens_regroup(erfx_ensemble* e, int k) {
+    int rc = ens_sync(e);
+    if (rc) return rc;
+    ens_clear(e);
+    for (EnsGroup& G : e->g)
+        if (G.stream) (void)hipStreamDestroy(G.stream);
+    e->g.clear();
+    k = std::max(1, std::min(k, (int)e->m.size()));
+    e->g.resize((size_t)k);
+    for (EnsGroup& G : e->g) {
+        hipError_t err = hipStreamCreate(&G.stream);
+        if (err != hipSuccess) { G.stream = nullptr; return hip_fail(err, "hipStreamCreate"); }
+    }
+    std::vector<erfx_sim*> order = e->m;
+    std::stable_sort(order.begin(), order.end(),
+                     [](const erfx_sim* a, const erfx_sim* b) { return sim_cells(a) > sim_cells(b); });
+    std::vector<long> load((size_t)k, 0);
+    for (erfx_sim* s : order) {
+        const size_t g = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        e->g[g].m.push_back(s);
+        load[g] += sim_cells(s);
+    }
+    return ERFX_OK;
 }
 
 extern "C" {
@@ -1498,24 +1554,27 @@ int erfx_ensemble_create(erfx_sim* const* members, int n, erfx_ensemble** out) {
     }
     auto e = new erfx_ensemble();
     e->m.assign(members, members + n);
-    for (erfx_sim* s : e->m) e->cells += (long)s->ctx->P.L.nx * s->ctx->P.L.ny * s->ctx->P.L.nz;
-    hipError_t err = hipStreamCreate(&e->stream);
-    if (err == hipSuccess) err = hipEventCreate(&e->ev[0]);
-    if (err == hipSuccess) err = hipEventCreate(&e->ev[1]);
-    if (err != hipSuccess) { erfx_ensemble_destroy(e); return hip_fail(err, "ensemble stream/events"); }
+    for (erfx_sim* s : e->m) e->cells += sim_cells(s);
+    int rc = ens_regroup(e, 2);
+    if (rc) { erfx_ensemble_destroy(e); return rc; }
     *out = e;
     return ERFX_OK;
 }
 
 int erfx_ensemble_destroy(erfx_ensemble* e) {
     if (!e) return ERFX_OK;
-    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    (void)ens_sync(e);
     ens_clear(e);
-    for (hipEvent_t& v : e->ev)
-        if (v) (void)hipEventDestroy(v);
-    if (e->stream) (void)hipStreamDestroy(e->stream);
+    for (EnsGroup& G : e->g)
+        if (G.stream) (void)hipStreamDestroy(G.stream);
     delete e;
     return ERFX_OK;
+}
+
+int erfx_ensemble_set_streams(erfx_ensemble* e, int nstreams) {
+    if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (nstreams < 1 || nstreams > 8) { set_error("streams must be 1..8"); return ERFX_ERR_ARG; }
+    return ens_regroup(e, nstreams);
 }
 
 int erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps) {
@@ -1530,10 +1589,9 @@ int erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps) {
     }
     for (int n = 0; n < nsteps; ++n) {
         int rc = ens_one_step(e, t0 + n * dt, dt);
-        if (rc) { (void)hipStreamSynchronize(e->stream); return rc; }
+        if (rc) { (void)ens_sync(e); return rc; }
     }
-    ERFX_HIP(hipStreamSynchronize(e->stream));
-    return ERFX_OK;
+    return ens_sync(e);
 }
 
 int erfx_ensemble_set_graph(erfx_ensemble* e, int on) {

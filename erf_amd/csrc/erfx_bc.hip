@@ -543,14 +543,16 @@ struct CopyItem {
     BHdr h;
     CopyList c;
 };
-__global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* it, int n) {
+__global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* __restrict__ it, int n) {
     const CopyItem& m = batch_member(it, n, blockIdx.x);
-    const long stride = (long)m.h.tiles * 256;
-    const long first = (long)(blockIdx.x - m.h.t0) * 256 + threadIdx.x;
-    for (int a = 0; a < m.c.count; ++a) {
-        const long n2 = m.c.n[a] / 2;
-        const double2* src = reinterpret_cast<const double2*>(m.c.src[a]);
-        double2* dst = reinterpret_cast<double2*>(m.c.dst[a]);
+    const BHdr h = m.h;
+    const CopyList& c = m.c;
+    const long stride = (long)h.tiles * 256;
+    const long first = (long)(blockIdx.x - h.t0) * 256 + threadIdx.x;
+    for (int a = 0; a < c.count; ++a) {
+        const long n2 = c.n[a] / 2;
+        const double2* src = reinterpret_cast<const double2*>(c.src[a]);
+        double2* dst = reinterpret_cast<double2*>(c.dst[a]);
         for (long t = first; t < n2; t += stride) dst[t] = src[t];
     }
 }

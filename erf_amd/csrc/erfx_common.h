@@ -134,7 +134,13 @@ inline void rec_push(BatchLaunch fn, unsigned width, const It& item) {
     g_rec->items.push_back(std::move(r));
 }
 // the item of work-group t: the last one whose range starts at or before t
-// (members with no work-groups share their start with the next and are skipped)
+// (members with no work-groups share their start with the next and are skipped).
+// The items are read through a __restrict__ pointer to const: no store of the
+// kernel can reach them, so the captures are loaded as scalar (uniform) loads
+// where the functor uses them, as from a kernel argument.  (A local copy of
+// the functor would move captured arrays indexed at run time into scratch.)
+// Every functor call is forced inline: with two kernels per functor the
+// inliner would otherwise leave large bodies as calls, with their spills.
 template <class It>
 __device__ __forceinline__ const It& batch_member(const It* it, int n, unsigned t) {
     int lo = 0, hi = n - 1;
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(256) box_kernel(Box b, unsigned gx, unsigned g
     int j = b.lo[1] + (int)(by * 4 + threadIdx.y);
     int k = b.lo[2] + (int)bz;
     if (i > b.hi[0] || j > b.hi[1] || k > b.hi[2]) return;
-    f(i, j, k);
+    [[clang::always_inline]] f(i, j, k);
 }
 
 template <class F>
@@ -164,16 +170,18 @@ struct BoxItem {
     F f;
 };
 template <class F>
-__global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* it, int n) {
+__global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* __restrict__ it, int n) {
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
     const BoxItem<F>& m = batch_member(it, n, t);
-    const unsigned u = t - m.h.t0;
-    const unsigned bx = u % m.h.gx, by = (u / m.h.gx) % m.h.gy, bz = u / (m.h.gx * m.h.gy);
-    int i = m.h.lo[0] + (int)(bx * 64 + threadIdx.x);
-    int j = m.h.lo[1] + (int)(by * 4 + threadIdx.y);
-    int k = m.h.lo[2] + (int)bz;
-    if (i > m.h.hi[0] || j > m.h.hi[1] || k > m.h.hi[2]) return;
-    m.f(i, j, k);
+    const BHdr h = m.h;
+    const F& f = m.f;
+    const unsigned u = t - h.t0;
+    const unsigned bx = u % h.gx, by = (u / h.gx) % h.gy, bz = u / (h.gx * h.gy);
+    int i = h.lo[0] + (int)(bx * 64 + threadIdx.x);
+    int j = h.lo[1] + (int)(by * 4 + threadIdx.y);
+    int k = h.lo[2] + (int)bz;
+    if (i > h.hi[0] || j > h.hi[1] || k > h.hi[2]) return;
+    [[clang::always_inline]] f(i, j, k);
 }
 template <class F>
 void launch_box_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
@@ -207,14 +215,14 @@ __global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int
     int i = ilo + (int)((t % gx) * blockDim.x + threadIdx.x);
     int j = jlo + (int)(t / gx);
     if (i > ihi || j > jhi) return;
-    f(i, j);
+    [[clang::always_inline]] f(i, j);
 }
 
 // Work-group launch for block-level reductions: f(u) runs on all 256 lanes of
 // work-group u of a 1-D grid.
 template <class F>
 __global__ void __launch_bounds__(256) grid_kernel(F f) {
-    f(blockIdx.x);
+    [[clang::always_inline]] f(blockIdx.x);
 }
 template <class F>
 struct GridItem {
@@ -222,9 +230,11 @@ struct GridItem {
     F f;
 };
 template <class F>
-__global__ void __launch_bounds__(256) grid_kernel_batch(const GridItem<F>* it, int n) {
+__global__ void __launch_bounds__(256) grid_kernel_batch(const GridItem<F>* __restrict__ it, int n) {
     const GridItem<F>& m = batch_member(it, n, blockIdx.x);
-    m.f(blockIdx.x - m.h.t0);
+    const unsigned t0 = m.h.t0;
+    const F& f = m.f;
+    [[clang::always_inline]] f(blockIdx.x - t0);
 }
 template <class F>
 void launch_grid_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
@@ -247,15 +257,20 @@ struct ColItem {
     BHdr h;
     F f;
 };
+// Batched columns run over the member's (i, j) range flattened, i fastest, so
+// a box narrower than the work-group fills its lanes with the next rows.
 template <class F>
-__global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* it, int n) {
+__global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* __restrict__ it, int n) {
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
     const ColItem<F>& m = batch_member(it, n, t);
-    const unsigned u = t - m.h.t0;
-    int i = m.h.lo[0] + (int)((u % m.h.gx) * blockDim.x + threadIdx.x);
-    int j = m.h.lo[1] + (int)(u / m.h.gx);
-    if (i > m.h.hi[0] || j > m.h.hi[1]) return;
-    m.f(i, j);
+    const BHdr h = m.h;
+    const F& f = m.f;
+    const unsigned nxr = (unsigned)(h.hi[0] - h.lo[0] + 1);
+    const unsigned c = (t - h.t0) * blockDim.x + threadIdx.x;
+    if (c >= nxr * (unsigned)(h.hi[1] - h.lo[1] + 1)) return;
+    const int i = h.lo[0] + (int)(c % nxr);
+    const int j = h.lo[1] + (int)(c / nxr);
+    [[clang::always_inline]] f(i, j);
 }
 template <class F>
 void launch_cols_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
@@ -269,7 +284,8 @@ inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f, 
     const unsigned gx = ok ? (unsigned)((ihi - ilo + 1 + width - 1) / width) : 1u;
     const unsigned gy = ok ? (unsigned)(jhi - jlo + 1) : 0u;
     if (g_rec) {
-        ColItem<F> it{{{ilo, jlo, 0}, {ihi, jhi, 0}, gx, 1u, 0u, gx * gy}, f};
+        const unsigned long long cols = ok ? (unsigned long long)(ihi - ilo + 1) * (unsigned long long)(jhi - jlo + 1) : 0;
+        ColItem<F> it{{{ilo, jlo, 0}, {ihi, jhi, 0}, gx, 1u, 0u, (unsigned)((cols + width - 1) / width)}, f};
         rec_push(&launch_cols_batch<F>, (unsigned)width, it);
         return;
     }

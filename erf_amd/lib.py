@@ -90,6 +90,7 @@ SIGNATURES = {
     "erfx_ensemble_destroy": (_I, [_P]),
     "erfx_ensemble_step": (_I, [_P, _D, _D, _I]),
     "erfx_ensemble_set_graph": (_I, [_P, _I]),
+    "erfx_ensemble_set_streams": (_I, [_P, _I]),
     "erfx_ensemble_autotune": (_I, [_P, _I]),
     "erfx_ensemble_set_column_variant": (_I, [_P, _I]),
     "erfx_ensemble_column_variant": (_I, [_P, ctypes.POINTER(_I), _P]),
@@ -282,6 +283,9 @@ class Ensemble:
 
     def set_graph(self, on=True):
         check(lib().erfx_ensemble_set_graph(self.h, 1 if on else 0))
+
+    def set_streams(self, n):
+        check(lib().erfx_ensemble_set_streams(self.h, int(n)))
 
     def autotune(self, on=True):
         check(lib().erfx_ensemble_autotune(self.h, 1 if on else 0))

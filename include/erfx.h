@@ -465,11 +465,14 @@ int  erfx_sim_timer_read(erfx_sim* sim, char* name, int name_len, double* total_
  * ParallelFor per box, MFIter loops in ERF_slow_rhs_pre.cpp:244,
  * ERF_fast_rhs_N.cpp:115-566, ...).  An ensemble advances n independent sims (boxes
  * of any sizes, or ensemble members) together: every kernel of the step runs
- * ONCE for all members, its work-groups split into one range per member.
+ * ONCE for all members of a group, its work-groups split into one range per
+ * member.  The members are split into groups of about equal cell count (2 by
+ * default), each group on a stream of its own so that the groups' kernels
+ * overlap.
  * Members keep their own buffers, configs and scalars (seed, wind, dt-derived
- * constants); they must record the same sequence of kernels (same scheme
- * options, same MRI ratio; sizes may differ), else erfx_ensemble_step returns
- * ERFX_ERR_UNSUPPORTED.  Not with the learned microphysics or row sampling.
+ * constants); the members of a group must record the same sequence of kernels
+ * (same scheme options, same MRI ratio; sizes may differ), else
+ * erfx_ensemble_step returns ERFX_ERR_UNSUPPORTED.  Not with the learned microphysics or row sampling.
  * The results are those of stepping each member alone. */
 typedef struct erfx_ensemble erfx_ensemble;
 /* members stay owned by the caller and must outlive the ensemble */
@@ -479,7 +482,9 @@ int  erfx_ensemble_destroy(erfx_ensemble* e);
  * The step's batched launch list is recorded on first use per (dt, stage time
  * steps, members' buffer parity, column variant) and reused. */
 int  erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps);
-/* replay each recorded step as a hipGraph (one graph launch per step) */
+/* number of member groups / streams (1..8; 2 by default) */
+int  erfx_ensemble_set_streams(erfx_ensemble* e, int nstreams);
+/* replay each recorded step as a hipGraph (one graph launch per group and step) */
 int  erfx_ensemble_set_graph(erfx_ensemble* e, int on);
 /* With `on`, the next 9 steps each use one variant of the acoustic column
  * kernel for all members, timed with HIP events; the fastest is kept.
@@ -489,7 +494,7 @@ int  erfx_ensemble_set_column_variant(erfx_ensemble* e, int variant);
 /* the variant in use (-1 while timing) and, when ms is not NULL, the step
  * time in ms of each variant timed (9 values) */
 int  erfx_ensemble_column_variant(erfx_ensemble* e, int* variant, float* ms);
-/* kernel launches per step (batched) and the sum of the members' cells */
+/* kernel launches per step (batched, all groups) and the sum of the members' cells */
 int  erfx_ensemble_info(erfx_ensemble* e, long* launches_per_step, long* cells);
 
 #ifdef __cplusplus

@@ -72,6 +72,23 @@ def test_ensemble_matches_members_alone(kind, graph):
         s.close()
 
 
+@pytest.mark.parametrize("streams", [1, 3])
+def test_ensemble_stream_groups(streams):
+    """members split over 1 or 3 groups (streams): the same results"""
+    from erf_amd.lib import Ensemble
+    cases = _members("cbl")
+    a, b = _pairs(cases)
+    ens = Ensemble(a)
+    ens.set_streams(streams)
+    ens.step(0.0, 0.5, 3)
+    for s in b:
+        s.step(0.0, 0.5, 3)
+    assert _diff(a, b) == []
+    ens.close()
+    for s in a + b:
+        s.close()
+
+
 def test_ensemble_member_against_oracle():
     from erf_amd.lib import Ensemble
     cases = _members("cbl")
@@ -116,6 +133,7 @@ def test_ensemble_refuses_different_sequences():
     c2.mri_ratio = 4                   # a different substep count: other kernels
     a, _ = _pairs([(c1, f1), (c2, f2)])
     ens = Ensemble(a)
+    ens.set_streams(1)                 # both in one group (one batch)
     with pytest.raises(ErfxError):
         ens.step(0.0, 1.0, 1)
     ens.close()
