@@ -56,6 +56,8 @@ def parse():
                          "ensemble: dry CBL members of uneven box sizes advanced as one batch")
     ap.add_argument("--members", type=int, default=48, help="ensemble members (--workload ensemble)")
     ap.add_argument("--ens-streams", type=int, default=2, help="ensemble: member groups, one stream each")
+    ap.add_argument("--ens-box", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
+                    help="ensemble: every member of this one box size instead of the five drawn sizes")
     ap.add_argument("--unbatched-steps", type=int, default=2,
                     help="ensemble: steps timed with every member launched on its own, for comparison")
     ap.add_argument("--graph", action="store_true", help="replay the steps from a captured hipGraph")
@@ -125,7 +127,8 @@ def run_ensemble(a):
     dt = 0.5
     mixed = a.precision == "mixed"
     sims, refs, shapes = [], [], []
-    for cfg, f in P.cbl_ensemble(a.members, seed=0):
+    sizes = (tuple(a.ens_box),) if a.ens_box else P.ENSEMBLE_SIZES
+    for cfg, f in P.cbl_ensemble(a.members, seed=0, sizes=sizes):
         shapes.append((cfg.nx, cfg.ny, cfg.nz))
         if mixed:
             c64 = C.Config.from_buffer_copy(cfg)
@@ -214,8 +217,8 @@ def run_ensemble(a):
         "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32" if mixed else "f64", "data": "synthetic (seeded)",
         "config": {"workload": "%d dry CBL LES members (U5, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, MRI ratio 16), "
-                               "own seeds, geostrophic speed in [5, 15] m/s, box sizes drawn from "
-                               "48^3, 64x64x96, 96x96x64, 128x128x80, 160x160x80" % len(shapes),
+                               "own seeds, geostrophic speed in [5, 15] m/s, box sizes drawn from %s"
+                               % (len(shapes), ", ".join("%dx%dx%d" % s for s in sizes)),
                    "members": hist, "cells": cells, "precision": a.precision, "graph": bool(a.graph),
                    "streams": a.ens_streams},
         "launches_per_step": launches,

@@ -916,7 +916,8 @@ int erfx_sim_download(erfx_sim* s, int id, double* host) {
 // One step: ERF::Advance (ERF_Advance.cpp:20-233), advance_dycore
 // (ERF_advance_dycore.cpp:37-305) and MRISplitIntegrator::advance (ERF_MRI.H:187-334).
 // slow_dt of the three RK stages as the MRI loop forms it, time_stage - time
-// with time_stage = time + dt/3, time + dt/2, time + dt (ERF_MRI.H:241-299):
+// with time_stage = time + dt/3, time + dt/2, time + dt (ERF_MRI.H:241-299,
+// ERF_TI_slow_rhs_fun.H:19):
 // the sum rounds to the spacing of time, so the values vary with time
 static void stage_dts(double time, double dt, double out[3]) {
     out[0] = (time + dt / 3.0) - time;
