@@ -49,11 +49,18 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal", "ensemble"], default="cbl",
+    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal", "ensemble", "supercell"], default="cbl",
                     help="cbl (default): the dry CBL LES; moist_terrain: moist flow over a hill on the "
                          "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s; "
                          "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage; "
-                         "ensemble: dry CBL members of uneven box sizes advanced as one batch")
+                         "ensemble: dry CBL members of uneven box sizes advanced as one batch; "
+                         "supercell: Weisman-Klemp deep convection, the learned Kessler stand-in trained on "
+                         "the plain method's runs and run against plain Kessler")
+    ap.add_argument("--sc-seeds", type=int, default=2, help="supercell: training runs (seeds 0..n-1)")
+    ap.add_argument("--sc-steps", type=int, default=600, help="supercell: steps of each training and evaluation run")
+    ap.add_argument("--sc-every", type=int, default=2, help="supercell: rows harvested every n-th step")
+    ap.add_argument("--sc-rows", type=int, default=4_000_000, help="supercell: row cap per training run")
+    ap.add_argument("--sc-fit", type=int, default=3000, help="supercell: Adam steps")
     ap.add_argument("--members", type=int, default=48, help="ensemble members (--workload ensemble)")
     ap.add_argument("--ens-streams", type=int, default=2, help="ensemble: member groups, one stream each")
     ap.add_argument("--ens-box", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
@@ -233,6 +240,107 @@ def run_ensemble(a):
     print(json.dumps(out))
 
 
+def run_supercell(a):
+    """Config of the learned microphysics: Weisman-Klemp deep convection
+    (256x256x80, dx = 1 km, dz = 250 m, dt = 2 s).  Rows of Kessler's own
+    tendencies are harvested from the plain method's runs (seeds 0..n-1), the
+    network is trained on the GPU, and the held-out run (seed 8) is advanced
+    with plain Kessler and with the network from the same state: both timed,
+    the network's run compared with Kessler's."""
+    import numpy as np
+    import torch
+    from erf_amd import config as C, problems as P, emulator as E
+    from erf_amd.lib import Sim, MLP
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (256, 256, 80)
+    dt = 2.0
+    case = lambda seed: P.supercell(nx, ny, nz, seed=seed)
+    w0 = time.perf_counter()
+    rows = E.harvest(seeds=tuple(range(a.sc_seeds)), nsteps=a.sc_steps, dt=dt, case=case, every=a.sc_every,
+                     max_rows=a.sc_rows)
+    held = E.harvest(seeds=(8,), nsteps=a.sc_steps, dt=dt, case=case, every=4 * a.sc_every,
+                     max_rows=min(a.sc_rows, 1_000_000))
+    t_harvest = time.perf_counter() - w0
+    print("rows: %d training, %d held out (%.0f s)" % (rows.shape[0], held.shape[0], t_harvest),
+          file=sys.stderr, flush=True)
+    sc = E.Scaling(rows)
+    untrained = MLP(64)
+    untrained.set_params(E.init_params(0))
+    sc.apply(untrained)
+    mse0 = E.mse(untrained, sc, held)
+    w0 = time.perf_counter()
+    net, sc, losses = E.fit(rows, steps=a.sc_fit, batch=8192, lr=2e-3, scaling=sc)
+    t_fit = time.perf_counter() - w0
+    mse1 = E.mse(net, sc, held)
+    del rows, held
+    cfg, f = case(8)
+    cmx = C.Config.from_buffer_copy(cfg)
+    cmx.precision = C.PREC_MIXED
+    ka, em, mx = Sim(cfg), Sim(cfg), Sim(cmx)
+    for s in (ka, em, mx):
+        s.load(f)
+    del f
+    em.set_mlp(net)
+    res = {}
+    for name, s in (("kessler", ka), ("mlp", em), ("mixed", mx)):
+        s.step(0.0, dt, a.sc_steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s.step(a.sc_steps * dt, dt, a.steps)
+        res[name] = (time.perf_counter() - t1) / a.steps
+    n_mlp, n_fb = em.mlp_counts()
+    ng = ka.L.ng
+    sl = (slice(ng, ng + nz), slice(ng, ng + ny), slice(ng, ng + nx))
+    slw = (slice(ng, ng + nz + 1), slice(ng, ng + ny), slice(ng, ng + nx))
+    ck = ka.download(C.F_CONS)
+    wk = ka.download(C.F_ZVEL)[0][slw]
+    rho_k = ck[C.RHO][sl]
+
+    def rel_errors(sim):
+        ce = sim.download(C.F_CONS)
+        out = {}
+        for name, comp in (("rho", C.RHO), ("rhotheta", C.RHOTHETA), ("qv", C.RHOQ1), ("qc", C.RHOQ2),
+                           ("qr", C.RHOQ3), ("w", None)):
+            if comp is None:
+                x, y = sim.download(C.F_ZVEL)[0][slw], wk
+            else:
+                x, y = ce[comp][sl], ck[comp][sl]
+                if comp not in (C.RHO, C.RHOTHETA):
+                    x, y = x / ce[C.RHO][sl], y / rho_k
+            r = y - y.mean(axis=(1, 2), keepdims=True) if name in ("rho", "rhotheta") else y
+            nrm = float(np.sqrt((r ** 2).mean()))
+            out[name] = float(np.sqrt(((x - y) ** 2).mean())) / nrm if nrm > 0 else 0.0
+        return out
+
+    errs = rel_errors(em)
+    floor = rel_errors(mx)
+    qc_max = float((ck[C.RHOQ2][sl] / rho_k).max())
+    for s in (ka, em, mx):
+        s.close()
+    cells = nx * ny * nz
+    out = {
+        "metric": "cell-updates/s per RK3 step (FP64 dycore, learned microphysics)",
+        "value": cells / res["mlp"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.sc_steps, "ms_per_step": 1e3 * res["mlp"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic (seeded)",
+        "config": {"workload": "supercell-type deep convection %dx%dx%d, dx=%g m, dz=%g m, Weisman-Klemp sounding "
+                               "(qv <= 14 g/kg), quarter-circle hodograph (5 m/s), +3 K warm bubble, Kessler with "
+                               "rain, dt=2 s, MRI ratio 6" % (nx, ny, nz, cfg.dx, cfg.dz),
+                   "cells": cells, "train_seeds": list(range(a.sc_seeds)), "eval_seed": 8,
+                   "steps_per_run": a.sc_steps},
+        "kessler": {"value": cells / res["kessler"], "ms_per_step": 1e3 * res["kessler"]},
+        "mlp_cells": n_mlp, "fallback_cells": n_fb,
+        "heldout_mse_scaled": {"untrained": mse0, "trained": mse1},
+        "fit_s": t_fit, "adam_steps": a.sc_fit, "harvest_s": t_harvest,
+        "relative_l2_vs_kessler_after_run": errs,
+        "relative_l2_mixed_precision_kessler_vs_fp64_kessler": floor,
+        "qc_max_kessler": qc_max,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -244,6 +352,9 @@ def main():
         return
     if a.workload == "ensemble":
         run_ensemble(a)
+        return
+    if a.workload == "supercell":
+        run_supercell(a)
         return
     import numpy as np
     import torch

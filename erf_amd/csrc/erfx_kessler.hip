@@ -248,9 +248,11 @@ void kessler_samples(const Params& P, const double* cons, const double* w, doubl
 //   E1  cells: the gate above; active cells inside the training range get a
 //       row of scaled inputs for the network, the others fall back to Kessler;
 //   E2  the network on the rows (erfx_mlp.hip);
-//   E3  cells: network tendencies projected in FP64 so that total water is
-//       conserved, no species goes negative and theta follows the latent heat
-//       of the vapour change; fallback cells run Kessler's own update.
+//   E3  cells: network tendencies projected in FP64 so that the vapour change
+//       has the sign and bound of Kessler's (toward saturation, never past
+//       it), total water is conserved, no species goes negative and theta
+//       follows the latent heat of the vapour change; fallback cells run
+//       Kessler's own update.
 // slot: per-cell int (-1 inactive, -2 fallback, else the row); x, y: rows of
 // network inputs and outputs; rows_cell: the cell of each row.
 void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, double* cons, const double* w,
@@ -315,9 +317,26 @@ void kessler_mlp_advance(const Params& P, const Net& N, const MlpScaling& S, dou
             double d[3] = {0.0, 0.0, 0.0};
             if (sl >= 0) {
                 for (int c = 0; c < 3; ++c) d[c] = (double)y[(long)sl * NO + c] * S.y_scale[c];
-                // water conservation: remove the mean
-                const double mean = (d[0] + d[1] + d[2]) / 3.0;
-                for (int c = 0; c < 3; ++c) d[c] -= mean;
+                // Kessler's local processes fix the direction of the vapour
+                // exchange (ERF_Kessler.cpp:126-150): in subsaturated air vapour
+                // only gains, up to saturation, and cloud water only loses; in
+                // supersaturated air vapour only condenses, down to saturation
+                // at most, and rain only grows.  The network's vapour change is
+                // held to that range; cloud and rain take the opposite change
+                // (so total water is conserved), split as the network splits it.
+                double qsat, dtqsat;
+                qsat_of(m.tabs, m.pres, qsat, dtqsat, &flag);
+                if (qsat <= 0.0) qsat = 0.0;
+                double dv = d[0];
+                if (qv < qsat) dv = smin(smax(dv, 0.0), qsat - qv);
+                else dv = smax(smin(dv, 0.0), qsat - qv);
+                const double cond = -dv;                     // change of cloud + rain
+                double dr = (qv < qsat) ? d[2] : smax(d[2], 0.0);
+                double dc = cond - dr;
+                if (qv < qsat && dc > 0.0) { dc = 0.0; dr = cond; }
+                d[0] = dv;
+                d[1] = dc;
+                d[2] = dr;
                 // non-negativity: raise negative results to 0 and take the
                 // deficit from the others in proportion to what they hold
                 const double qs[3] = {qv, qc, qp};

@@ -22,23 +22,32 @@ def _torch():
     return torch
 
 
-def harvest(seeds=(0, 1), nsteps=20, shape=(32, 32, 24), dt=1.0, max_rows=2_000_000):
+def harvest(seeds=(0, 1), nsteps=20, shape=(32, 32, 24), dt=1.0, max_rows=2_000_000, case=None, every=1):
     """Rows (n, 12) of (T, p, rho, qv, qc, qr, w, dt, dqv, dqc, dqr, dtheta) from
-    moist-bubble runs with Kessler, one run per seed."""
+    runs with Kessler, one run per seed: case(seed) -> (cfg, fields), the moist
+    bubble of `shape` by default; rows from every `every`-th step, at most
+    max_rows per run."""
     torch = _torch()
     out = []
     for seed in seeds:
-        cfg, f = P.moist_bubble(*shape, seed=seed)
+        cfg, f = case(seed) if case else P.moist_bubble(*shape, seed=seed)
         sim = Sim(cfg)
         sim.load(f)
+        del f
         buf = torch.zeros((max_rows, 12), dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
-        sim.set_sampling(buf.data_ptr(), max_rows)
-        sim.step(0.0, dt, nsteps)
-        n = min(sim.sample_count(), max_rows)
-        out.append(buf[:n].cpu().numpy().astype(np.float64))
-        sim.set_sampling(None, 0)
+        off = 0
+        for n in range(nsteps):
+            take = (n + 1) % every == 0 and off < max_rows
+            if take:
+                sim.set_sampling(buf.data_ptr() + off * 12 * 4, max_rows - off)
+            sim.step(n * dt, dt, 1)
+            if take:
+                off += min(sim.sample_count(), max_rows - off)
+                sim.set_sampling(None, 0)
+        out.append(buf[:off].cpu().numpy().astype(np.float64))
         sim.close()
+        del buf
     return np.concatenate(out, axis=0)
 
 

@@ -211,6 +211,109 @@ def moist_bubble(nx=24, ny=20, nz=16, seed=0, buoyancy_type=None, norain=False, 
     return cfg, f
 
 
+def wk_sounding(z, theta_0=300.0, theta_tr=343.0, z_tr=12000.0, T_tr=213.0, cp=C.Cp_d, g=C.CONST_GRAV):
+    """Weisman-Klemp (1982) analytic profiles at heights z: potential
+    temperature theta_0 + (theta_tr - theta_0) (z/z_tr)^(5/4) below the
+    tropopause, theta_tr exp(g (z - z_tr) / (c_p T_tr)) above; relative
+    humidity 1 - 0.75 (z/z_tr)^(5/4) below, 0.25 above."""
+    z = np.asarray(z, dtype=np.float64)
+    s = np.power(np.clip(z / z_tr, 0.0, None), 1.25)
+    theta = np.where(z <= z_tr, theta_0 + (theta_tr - theta_0) * s,
+                     theta_tr * np.exp(g / (cp * T_tr) * (z - z_tr)))
+    rh = np.where(z <= z_tr, 1.0 - 0.75 * s, 0.25)
+    return theta, rh
+
+
+def moist_hse(zc, dz, theta, rh, qv_max=0.014, rdOcp=C.R_d / C.Cp_d, g=C.CONST_GRAV, iters=40):
+    """Discrete hydrostatic column of a moist sounding on flat levels: the total
+    density r = rho_d (1 + qv) at cell centres with p_0 at z = 0,
+    p[0] = p_0 - dz/2 r[0] g, p[k] = p[k-1] - dz (r[k] + r[k-1])/2 g (the
+    recurrence of erf_enforce_hse), where rho_d theta (1 + R_v/R_d qv) gives
+    p through the moist EOS and qv = min(rh qsat(T, p), qv_max).  Each level is
+    solved by fixed-point iteration.  Returns (r, p, qv, rho_d)."""
+    nz = len(zc)
+    r = np.zeros(nz)
+    p = np.zeros(nz)
+    qv = np.zeros(nz)
+    rd = np.zeros(nz)
+
+    def state(pk, th, h):
+        T = th * (pk / C.p_0) ** rdOcp
+        es = float(_esat_bolton(T))
+        q = min(h * (C.R_d / C.R_v) * es / (pk / 100.0 - es), qv_max)
+        rho_d = float(_rt_of_p(pk)) / (1.0 + C.R_v / C.R_d * q) / th
+        return rho_d * (1.0 + q), q, rho_d
+
+    for k in range(nz):
+        pk = C.p_0 if k == 0 else p[k - 1]
+        for _ in range(iters):
+            rk, q, rho_d = state(pk, theta[k], rh[k])
+            pk = C.p_0 - 0.5 * dz * rk * g if k == 0 else p[k - 1] - dz * 0.5 * (rk + r[k - 1]) * g
+        r[k], qv[k], rd[k] = state(pk, theta[k], rh[k])
+        p[k] = pk
+    return r, p, qv, rd
+
+
+def supercell(nx=256, ny=256, nz=80, seed=0, lx=256000.0, ly=256000.0, lz=20000.0, radius=5.0, u_top=15.0,
+              bubble=3.0, adv=C.ADV_UPWIND3):
+    """Supercell-type deep convection: Weisman-Klemp sounding (wk_sounding,
+    qv capped at 14 g/kg) in discrete moist hydrostatic balance (moist_hse),
+    a quarter-circle hodograph of radius `radius` m/s over the lowest 2 km
+    (u = R (1 - cos a), v = R sin a, a = pi/2 z/2 km), then u rising linearly
+    to u_top at 6 km and constant above, v = R; a +bubble K cos^2 warm
+    ellipsoid of radii 10 km x 10 km x 1.5 km centred 1.5 km above the domain
+    centre (at the environment's pressure and vapour).  Default domain
+    256 x 256 x 20 km with dx = dy = 1 km, dz = 250 m at 256x256x80; periodic
+    laterally, slip walls.  Kessler with rain (buoyancy type 2), dycore U3,
+    moist scalars WENO3, dt = 2 s, MRI ratio 6.  The seed draws +-0.1 K theta
+    noise below 3 km and +-0.01 m/s velocity noise."""
+    cfg = C.default_config(nx, ny, nz, lx / nx, ly / ny, lz / nz)
+    cfg.dycore_horiz_adv = cfg.dycore_vert_adv = adv
+    cfg.dryscal_horiz_adv = cfg.dryscal_vert_adv = adv
+    cfg.mri_ratio = 6
+    C.set_moisture(cfg, C.MOIST_KESSLER)
+    L, f = empty_fields(cfg)
+    ng = L.ng
+    g = C.CONST_GRAV
+    rdOcp = C.R_d / cfg.c_p
+    zc = (np.arange(nz) + 0.5) * cfg.dz
+    th_col, rh_col = wk_sounding(zc)
+    r, _, qv_col, _ = moist_hse(zc, cfg.dz, th_col, rh_col, rdOcp=rdOcp, g=g)
+    rr = np.concatenate([[r[0]], r, [r[-1]]])          # k = -1 .. nz
+    p, pi = enforce_hse(rr, cfg.dz, rdOcp, g)
+    for fid, col in ((C.F_R0, rr), (C.F_P0, p), (C.F_PI0, pi)):
+        a = f[fid][0]
+        a[:] = 0.0
+        a[ng - 1:ng + nz + 1, :, :] = col[:, None, None]
+    ix, jy, kz = _cell_centres(cfg, L)
+    beta = np.sqrt(((ix[None, None, :] - 0.5 * lx) / 10000.0) ** 2 +
+                   ((jy[None, :, None] - 0.5 * ly) / 10000.0) ** 2 +
+                   ((kz[:, None, None] - 1500.0) / 1500.0) ** 2)
+    dth = np.where(beta <= 1.0, bubble * np.cos(0.5 * math.pi * beta) ** 2, 0.0)
+    rng = np.random.default_rng(seed)
+    shp = L.shape3()
+    noise = rng.uniform(-0.1, 0.1, shp) * (kz < 3000.0)[:, None, None]
+    sl = (slice(ng, ng + nz),)
+    theta = np.zeros(shp)
+    theta[sl] = th_col[:, None, None] + dth[sl] + noise[sl]
+    qv = np.zeros(shp)
+    qv[sl] = qv_col[:, None, None]
+    p3 = f[C.F_P0][0]
+    cons = f[C.F_CONS]
+    rt = _rt_of_p(p3) / (1.0 + C.R_v / C.R_d * qv)
+    cons[C.RHOTHETA][sl] = rt[sl]
+    cons[C.RHO][sl] = rt[sl] / theta[sl]
+    cons[C.RHOQ1][sl] = (cons[C.RHO] * qv)[sl]
+    _noise_vel(cfg, L, f, rng, 0.01, (0.0, 0.0, 0.0))
+    a = 0.5 * math.pi * np.clip(zc / 2000.0, 0.0, 1.0)
+    u = np.where(zc <= 2000.0, radius * (1.0 - np.cos(a)),
+                 radius + (u_top - radius) * np.clip((zc - 2000.0) / 4000.0, 0.0, 1.0))
+    v = np.where(zc <= 2000.0, radius * np.sin(a), radius)
+    f[C.F_XVEL][0][ng:ng + nz] += u[:, None, None]
+    f[C.F_YVEL][0][ng:ng + nz] += v[:, None, None]
+    return cfg, f
+
+
 def z_levels_uniform(nz, dz, zsurf=0.0):
     """init_zlevels without stretching (ERF_TerrainMetrics.cpp:27-33): k dz."""
     return zsurf + np.arange(nz + 1) * dz

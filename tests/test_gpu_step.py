@@ -234,3 +234,10 @@ def test_moist_terrain_case_exact():
     moist scalars, U3 momentum, Kessler, Rayleigh layer): bit-identical steps."""
     cfg, f = P.moist_terrain(24, 20, 16, seed=41)
     _compare(cfg, f, 1.0, 2, 1e-12, exact=True, rayleigh=P.rayleigh_profiles_uniform(cfg, f=f))
+
+
+def test_supercell_case_exact():
+    """The supercell set-up (Weisman-Klemp sounding, hodograph, warm bubble,
+    Kessler with rain, buoyancy type 2) at a reduced size: bit-identical."""
+    cfg, f = P.supercell(24, 20, 20, seed=4)
+    _compare(cfg, f, 2.0, 3, 1e-12, exact=True)
