@@ -265,6 +265,9 @@ def run_supercell(a):
     t_harvest = time.perf_counter() - w0
     print("rows: %d training, %d held out (%.0f s)" % (rows.shape[0], held.shape[0], t_harvest),
           file=sys.stderr, flush=True)
+    if rows.shape[0] < 1000 or held.shape[0] < 100:
+        sys.exit("supercell: too few cells with active microphysics in %d steps (%d training, %d held-out rows); "
+                 "the storm forms after about 150 steps: raise --sc-steps" % (a.sc_steps, rows.shape[0], held.shape[0]))
     sc = E.Scaling(rows)
     untrained = MLP(64)
     untrained.set_params(E.init_params(0))
