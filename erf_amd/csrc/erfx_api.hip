@@ -1288,7 +1288,8 @@ struct BatchSlot {
     BatchLaunch fn;
     unsigned width, total;
     int n;
-    size_t off;
+    size_t off;        // the items
+    size_t soff;       // the members' first work-group ids
 };
 struct BatchStep {
     double dt = 0, sdt[3] = {};
@@ -1390,8 +1391,9 @@ static int ens_record(const std::vector<erfx_sim*>& mem, double time, double dt,
     // item arrays, one per slot, each 256-byte aligned; t0 made cumulative
     size_t bytes = 0;
     for (size_t i = 0; i < ns; ++i) {
-        BatchSlot sl{seq[0][i].launch, seq[0][i].width, 0u, n, (bytes + 255) / 256 * 256};
-        bytes = sl.off + (size_t)n * seq[0][i].size;
+        BatchSlot sl{seq[0][i].launch, seq[0][i].width, 0u, n, (bytes + 255) / 256 * 256, 0};
+        sl.soff = (sl.off + (size_t)n * seq[0][i].size + 255) / 256 * 256;
+        bytes = sl.soff + (size_t)n * sizeof(unsigned);
         b->slots.push_back(sl);
     }
     std::vector<unsigned char> host(bytes);
@@ -1403,6 +1405,8 @@ static int ens_record(const std::vector<erfx_sim*>& mem, double time, double dt,
             BHdr h;
             std::memcpy(&h, it.bytes.data(), sizeof(BHdr));
             h.t0 = (unsigned)t0;
+            const unsigned start = (unsigned)t0;
+            std::memcpy(host.data() + sl.soff + (size_t)k * sizeof(unsigned), &start, sizeof(unsigned));
             t0 += h.tiles;
             unsigned char* dst = host.data() + sl.off + (size_t)k * it.size;
             std::memcpy(dst, it.bytes.data(), it.size);
@@ -1423,9 +1427,14 @@ static int ens_record(const std::vector<erfx_sim*>& mem, double time, double dt,
     return ERFX_OK;
 }
 
-static void ens_launch(const BatchStep* b, hipStream_t st) {
-    for (const BatchSlot& sl : b->slots)
-        if (sl.total) sl.fn(static_cast<const unsigned char*>(b->items) + sl.off, sl.n, sl.total, sl.width, st);
+static int ens_launch(const BatchStep* b, hipStream_t st) {
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+        const BatchSlot& sl = b->slots[i];
+        const unsigned char* base = static_cast<const unsigned char*>(b->items);
+        if (sl.total)
+            sl.fn(base + sl.off, reinterpret_cast<const unsigned*>(base + sl.soff), sl.n, sl.total, sl.width, st);
+    }
+    return ERFX_OK;
 }
 
 // one group's step: the recorded launch list for (dt, stage dts, parities,
@@ -1457,7 +1466,8 @@ static int ens_group_step(EnsGroup& G, bool graph_on, double time, double dt, in
     if (graph_on) {
         if (!b->gexec) {
             ERFX_HIP(hipStreamBeginCapture(G.stream, hipStreamCaptureModeRelaxed));
-            ens_launch(b, G.stream);
+            int rc = ens_launch(b, G.stream);
+            if (rc) return rc;
             hipGraph_t g = nullptr;
             hipError_t err = hipStreamEndCapture(G.stream, &g);
             if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
@@ -1467,7 +1477,8 @@ static int ens_group_step(EnsGroup& G, bool graph_on, double time, double dt, in
         }
         ERFX_HIP(hipGraphLaunch(b->gexec, G.stream));
     } else {
-        ens_launch(b, G.stream);
+        int rc = ens_launch(b, G.stream);
+        if (rc) return rc;
     }
     return launch_status();
 }

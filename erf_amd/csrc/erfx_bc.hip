@@ -543,8 +543,9 @@ struct CopyItem {
     BHdr h;
     CopyList c;
 };
-__global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* __restrict__ it, int n) {
-    const CopyItem& m = batch_member(it, n, blockIdx.x);
+__global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* __restrict__ it,
+                                                         const unsigned* __restrict__ starts, int n) {
+    const CopyItem& m = batch_member(it, starts, n, blockIdx.x);
     const BHdr h = m.h;
     const CopyList& c = m.c;
     const long stride = (long)h.tiles * 256;
@@ -556,9 +557,11 @@ __global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* __restr
         for (long t = first; t < n2; t += stride) dst[t] = src[t];
     }
 }
-static void copy_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+static void copy_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                       hipStream_t s) {
     (void)width;
-    hipLaunchKernelGGL(copy_kernel_batch, dim3(total), dim3(256), 0, s, static_cast<const CopyItem*>(items), n);
+    hipLaunchKernelGGL(copy_kernel_batch, dim3(total), dim3(256), 0, s, static_cast<const CopyItem*>(items), starts,
+                       n);
 }
 
 void copy_fields(const CopyList& c, hipStream_t s) {

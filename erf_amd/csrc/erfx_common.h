@@ -100,8 +100,10 @@ struct BHdr {
     unsigned gx, gy;      // tiles along x and y (column launches: along x only)
     unsigned t0, tiles;   // the member's first work-group id in the batch, and its count
 };
-// launches `total` work-groups of `width` lanes over n items stored at `items`
-using BatchLaunch = void (*)(const void* items, int n, unsigned total, unsigned width, hipStream_t s);
+// launches `total` work-groups of `width` lanes over n items stored at
+// `items`, whose first work-group ids are starts[0..n-1]
+using BatchLaunch = void (*)(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                             hipStream_t s);
 struct RecItem {
     BatchLaunch launch;   // one instance per item type, so it also identifies the kernel
     unsigned width;       // work-group size
@@ -134,7 +136,9 @@ inline void rec_push(BatchLaunch fn, unsigned width, const It& item) {
     g_rec->items.push_back(std::move(r));
 }
 // the item of work-group t: the last one whose range starts at or before t
-// (members with no work-groups share their start with the next and are skipped).
+// (members with no work-groups share their start with the next and are
+// skipped).  The starts are a compact array of their own, so the search reads
+// one or two cache lines that every work-group of the launch shares.
 // The items are read through a __restrict__ pointer to const: no store of the
 // kernel can reach them, so the captures are loaded as scalar (uniform) loads
 // where the functor uses them, as from a kernel argument.  (A local copy of
@@ -142,11 +146,12 @@ inline void rec_push(BatchLaunch fn, unsigned width, const It& item) {
 // Every functor call is forced inline: with two kernels per functor the
 // inliner would otherwise leave large bodies as calls, with their spills.
 template <class It>
-__device__ __forceinline__ const It& batch_member(const It* it, int n, unsigned t) {
+__device__ __forceinline__ const It& batch_member(const It* it, const unsigned* __restrict__ starts, int n,
+                                                  unsigned t) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (it[mid].h.t0 <= t) lo = mid;
+        if (starts[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
     return it[lo];
@@ -170,9 +175,9 @@ struct BoxItem {
     F f;
 };
 template <class F>
-__global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* __restrict__ it, int n) {
+__global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* __restrict__ it, const unsigned* __restrict__ starts, int n) {
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
-    const BoxItem<F>& m = batch_member(it, n, t);
+    const BoxItem<F>& m = batch_member(it, starts, n, t);
     const BHdr h = m.h;
     const F& f = m.f;
     const unsigned u = t - h.t0;
@@ -184,10 +189,11 @@ __global__ void __launch_bounds__(256) box_kernel_batch(const BoxItem<F>* __rest
     [[clang::always_inline]] f(i, j, k);
 }
 template <class F>
-void launch_box_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+void launch_box_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                      hipStream_t s) {
     (void)width;
     hipLaunchKernelGGL(box_kernel_batch<F>, dim3(total), dim3(64, 4, 1), 0, s, static_cast<const BoxItem<F>*>(items),
-                       n);
+                       starts, n);
 }
 
 template <class F>
@@ -230,16 +236,18 @@ struct GridItem {
     F f;
 };
 template <class F>
-__global__ void __launch_bounds__(256) grid_kernel_batch(const GridItem<F>* __restrict__ it, int n) {
-    const GridItem<F>& m = batch_member(it, n, blockIdx.x);
+__global__ void __launch_bounds__(256) grid_kernel_batch(const GridItem<F>* __restrict__ it, const unsigned* __restrict__ starts, int n) {
+    const GridItem<F>& m = batch_member(it, starts, n, blockIdx.x);
     const unsigned t0 = m.h.t0;
     const F& f = m.f;
     [[clang::always_inline]] f(blockIdx.x - t0);
 }
 template <class F>
-void launch_grid_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+void launch_grid_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                       hipStream_t s) {
     (void)width;
-    hipLaunchKernelGGL(grid_kernel_batch<F>, dim3(total), dim3(256), 0, s, static_cast<const GridItem<F>*>(items), n);
+    hipLaunchKernelGGL(grid_kernel_batch<F>, dim3(total), dim3(256), 0, s, static_cast<const GridItem<F>*>(items),
+                       starts, n);
 }
 template <class F>
 inline void launch_grid(unsigned blocks, hipStream_t s, F f) {
@@ -260,9 +268,9 @@ struct ColItem {
 // Batched columns run over the member's (i, j) range flattened, i fastest, so
 // a box narrower than the work-group fills its lanes with the next rows.
 template <class F>
-__global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* __restrict__ it, int n) {
+__global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* __restrict__ it, const unsigned* __restrict__ starts, int n) {
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
-    const ColItem<F>& m = batch_member(it, n, t);
+    const ColItem<F>& m = batch_member(it, starts, n, t);
     const BHdr h = m.h;
     const F& f = m.f;
     const unsigned nxr = (unsigned)(h.hi[0] - h.lo[0] + 1);
@@ -273,9 +281,10 @@ __global__ void __launch_bounds__(256) col_kernel_batch(const ColItem<F>* __rest
     [[clang::always_inline]] f(i, j);
 }
 template <class F>
-void launch_cols_batch(const void* items, int n, unsigned total, unsigned width, hipStream_t s) {
+void launch_cols_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                       hipStream_t s) {
     hipLaunchKernelGGL(col_kernel_batch<F>, dim3(total), dim3(width, 1, 1), 0, s, static_cast<const ColItem<F>*>(items),
-                       n);
+                       starts, n);
 }
 
 template <class F>

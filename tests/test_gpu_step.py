@@ -241,3 +241,26 @@ def test_supercell_case_exact():
     Kessler with rain, buoyancy type 2) at a reduced size: bit-identical."""
     cfg, f = P.supercell(24, 20, 20, seed=4)
     _compare(cfg, f, 2.0, 3, 1e-12, exact=True)
+
+
+@pytest.mark.parametrize("kind", ["thermal", "cbl", "moist", "hill", "supercell"])
+def test_hundred_steps(kind):
+    """100 steps from the same seeded state: within 1e-9 relative per field
+    (the bar for long runs); the step is bit-identical, so in practice the
+    fields agree exactly."""
+    if kind == "thermal":
+        cfg, f = P.rising_thermal(24, 20, 16, seed=41)
+        dt = 1.0
+    elif kind == "cbl":
+        cfg, f = P.cbl(16, 16, 16, seed=42)
+        dt = 0.5
+    elif kind == "moist":
+        cfg, f = P.moist_bubble(24, 20, 16, seed=43)
+        dt = 1.0
+    elif kind == "hill":
+        cfg, f = P.hill_flow(24, 20, 16, seed=44, moist=True)
+        dt = 1.0
+    else:
+        cfg, f = P.supercell(20, 20, 20, seed=45)
+        dt = 2.0
+    _compare(cfg, f, dt, 100, 1e-9)
