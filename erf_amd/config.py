@@ -8,7 +8,7 @@ import ctypes
 import math
 
 # ---- enums (include/erfx.h) -------------------------------------------------
-BC_PERIODIC, BC_SLIP_WALL, BC_NO_SLIP_WALL, BC_SYMMETRY, BC_OUTFLOW = range(5)
+BC_PERIODIC, BC_SLIP_WALL, BC_NO_SLIP_WALL, BC_SYMMETRY, BC_OUTFLOW, BC_INFLOW = range(6)
 ADV_CENTERED2, ADV_UPWIND3, ADV_CENTERED4, ADV_UPWIND5, ADV_CENTERED6 = range(5)
 # WENO family (scalars only): ERF_Interpolation_WENO.H, ERF_Interpolation_WENO_Z.H
 ADV_WENO3, ADV_WENOZ3, ADV_WENOMZQ3, ADV_WENO5, ADV_WENOZ5 = range(5, 10)
@@ -80,6 +80,10 @@ class Config(ctypes.Structure):
         ("moistscal_horiz_upw_frac", ctypes.c_double), ("moistscal_vert_upw_frac", ctypes.c_double),
         ("use_terrain", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("inflow_velocity", (ctypes.c_double * 3) * 6),
+        ("inflow_density", ctypes.c_double * 6), ("inflow_theta", ctypes.c_double * 6),
+        ("inflow_scalar", ctypes.c_double * 6), ("inflow_qv", ctypes.c_double * 6),
+        ("inflow_qc", ctypes.c_double * 6),
     ]
 
 

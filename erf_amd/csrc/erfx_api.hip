@@ -45,10 +45,23 @@ void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
                 case ERFX_BC_OUTFLOW: k = (v == dir) ? NEUMANN_INT : FOEXTRAP; break;
                 case ERFX_BC_NO_SLIP_WALL: k = EXT_DIR; break;
                 case ERFX_BC_SLIP_WALL: k = (v == dir) ? EXT_DIR : FOEXTRAP; break;
+                case ERFX_BC_INFLOW: k = EXT_DIR; P.vel_extdir[v][ori] = c.inflow_velocity[ori][v]; break;
             }
             P.bck[v][ori] = k;
         }
         P.bck[3][ori] = (t == ERFX_BC_PERIODIC) ? INT_DIR : (t == ERFX_BC_SYMMETRY ? REFLECT_EVEN : FOEXTRAP);
+        for (int n = 0; n < ERFX_NCONS_MOIST; ++n) P.cons_extdir[n][ori] = 0.0;
+        if (t == ERFX_BC_INFLOW && c.inflow_density[ori] != 0.0) {
+            const double r = c.inflow_density[ori];
+            P.bck[3][ori] = EXT_DIR;
+            P.cons_extdir[ERFX_RHO][ori] = r;
+            P.cons_extdir[ERFX_RHOTHETA][ori] = r * c.inflow_theta[ori];
+            P.cons_extdir[ERFX_RHOSCALAR][ori] = r * c.inflow_scalar[ori];
+            if (c.moisture_type != ERFX_MOIST_NONE) {
+                P.cons_extdir[ERFX_RHOQ1][ori] = r * c.inflow_qv[ori];
+                P.cons_extdir[ERFX_RHOQ2][ori] = r * c.inflow_qc[ori];
+            }
+        }
     }
     P.consA = c.molec_diff_type == ERFX_MOLEC_CONSTANT_ALPHA;
     P.use_turb = c.les_type == ERFX_LES_SMAGORINSKY;
@@ -82,7 +95,17 @@ static int validate(const erfx_config& c) {
         set_error("Kessler_NoRain requires buoyancy_type 1"); return ERFX_ERR_ARG;
     }
     for (int o = 0; o < 6; ++o) {
-        if (c.bc[o] < ERFX_BC_PERIODIC || c.bc[o] > ERFX_BC_OUTFLOW) { set_error("bad bc type"); return ERFX_ERR_ARG; }
+        if (c.bc[o] < ERFX_BC_PERIODIC || c.bc[o] > ERFX_BC_INFLOW) { set_error("bad bc type"); return ERFX_ERR_ARG; }
+        if (c.bc[o] == ERFX_BC_INFLOW) {
+            const int d = o % 3;
+            if (d == 2) { set_error("inflow on a vertical face is not supported"); return ERFX_ERR_UNSUPPORTED; }
+            if (c.bc[1 - d] != ERFX_BC_PERIODIC) {
+                set_error("inflow needs the other lateral direction periodic"); return ERFX_ERR_UNSUPPORTED;
+            }
+            if (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE || c.use_terrain) {
+                set_error("inflow with diffusion or terrain is not supported"); return ERFX_ERR_UNSUPPORTED;
+            }
+        }
         if ((c.bc[o] == ERFX_BC_PERIODIC) != (c.bc[o % 3 == o ? o + 3 : o - 3] == ERFX_BC_PERIODIC)) {
             set_error("periodic faces must come in pairs"); return ERFX_ERR_ARG;
         }

@@ -51,6 +51,16 @@ void fill_boundary_periodic(const Params& P, double* a, int ncomp, int ng, int n
 // ERFPhysBCFunct_cons::operator() with do_fb: FillBoundary of all components,
 // then impose_lateral_cons_bcs (ERF_BoundaryConditions_cons.cpp:17-246) and
 // impose_vertical_cons_bcs (:261-494) on components [icomp, icomp+ncomp).
+// the ext_dir values of every cons component on face `ori`
+struct ExtVals {
+    double v[ERFX_NCONS_MOIST];
+};
+static ExtVals ext_vals(const Params& P, int ori) {
+    ExtVals e;
+    for (int n = 0; n < ERFX_NCONS_MOIST; ++n) e.v[n] = P.cons_extdir[n][ori];
+    return e;
+}
+
 void fill_cons(const Params& P, double* c, int icomp, int ncomp, int ng, hipStream_t s) {
     fill_boundary_periodic(P, c, P.cfg.ncons, P.ng_cons, -1, s);
     const Lay L = P.L;
@@ -59,11 +69,17 @@ void fill_cons(const Params& P, double* c, int icomp, int ncomp, int ng, hipStre
     if (!P.periodic[0]) {
         const int tl = P.bck[3][0], th = P.bck[3][3];
         Box b(0, -ng, 0, 2 * ng - 1, ny - 1 + ng, nz - 1);
+        ExtVals xl = ext_vals(P, 0), xh = ext_vals(P, 3);
         launch_box(b, s, [=] __device__(int ip, int j, int k) {
             bool lo = ip < ng;
             int i = lo ? ip - ng : nx + (ip - ng);
             int t = lo ? tl : th;
             int src;
+            if (t == EXT_DIR) {   // ERF_BoundaryConditions_cons.cpp:68-104
+                const long d = L.ix(i, j, k);
+                for (int n = icomp; n < icomp + ncomp; ++n) c[d + n * ne] = lo ? xl.v[n] : xh.v[n];
+                return;
+            }
             if (t == FOEXTRAP) src = lo ? 0 : nx - 1;
             else if (t == REFLECT_EVEN) src = lo ? -1 - i : 2 * (nx - 1) + 1 - i;
             else return;
@@ -74,11 +90,17 @@ void fill_cons(const Params& P, double* c, int icomp, int ncomp, int ng, hipStre
     if (!P.periodic[1]) {
         const int tl = P.bck[3][1], th = P.bck[3][4];
         Box b(-ng, 0, 0, nx - 1 + ng, 2 * ng - 1, nz - 1);
+        ExtVals yl = ext_vals(P, 1), yh = ext_vals(P, 4);
         launch_box(b, s, [=] __device__(int i, int jp, int k) {
             bool lo = jp < ng;
             int j = lo ? jp - ng : ny + (jp - ng);
             int t = lo ? tl : th;
             int src;
+            if (t == EXT_DIR) {
+                const long d = L.ix(i, j, k);
+                for (int n = icomp; n < icomp + ncomp; ++n) c[d + n * ne] = lo ? yl.v[n] : yh.v[n];
+                return;
+            }
             if (t == FOEXTRAP) src = lo ? 0 : ny - 1;
             else if (t == REFLECT_EVEN) src = lo ? -1 - j : 2 * (ny - 1) + 1 - j;
             else return;

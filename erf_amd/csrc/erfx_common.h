@@ -353,6 +353,7 @@ struct Params {
     double dxInv[3];
     int bck[4][6];            // [x/y/z velocity, cons][ori]
     double vel_extdir[3][6];
+    double cons_extdir[ERFX_NCONS_MOIST][6];   // ext_dir values of the cons components (inflow faces)
     bool periodic[3];
     bool use_diff, use_turb, consA;
     KTimer* timer = nullptr;  // optional launch timing (host pointer, never read on the device)

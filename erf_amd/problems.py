@@ -594,6 +594,30 @@ def cbl(nx=256, ny=256, nz=128, seed=0, ug0=10.0):
 ENSEMBLE_SIZES = ((48, 48, 48), (64, 64, 96), (96, 96, 64), (128, 128, 80), (160, 160, 80))
 
 
+def inflow_outflow(nx=24, ny=16, nz=16, seed=0, u0=10.0, moist=False, adv=C.ADV_UPWIND3):
+    """Flow through the rising-thermal domain (20 x 20 x 10 km): ERF's Inflow
+    on the x-lo face (velocity (u0, 0, 0), density 1.1 kg/m^3, theta 300 K,
+    scalar 0.5 and, with moisture, qv 5 g/kg, qc 0), Outflow on x-hi, periodic
+    in y, slip walls; the thermal's +2 K bubble (or the moist bubble) carried
+    downstream by u0 with +-0.01 m/s noise; dt = 1 s, MRI ratio 6."""
+    if moist:
+        cfg, f = moist_bubble(nx, ny, nz, seed=seed, adv=adv)
+    else:
+        cfg, f = rising_thermal(nx, ny, nz, seed=seed, adv=adv)
+    cfg.bc[0] = C.BC_INFLOW
+    cfg.bc[3] = C.BC_OUTFLOW
+    cfg.inflow_velocity[0][0] = u0
+    cfg.inflow_density[0] = 1.1
+    cfg.inflow_theta[0] = 300.0
+    cfg.inflow_scalar[0] = 0.5
+    if moist:
+        cfg.inflow_qv[0] = 5.0e-3
+    L = C.layout_for(cfg)
+    ng = L.ng
+    f[C.F_XVEL][0][ng:ng + nz, ng:ng + ny, ng:ng + nx + 1] += u0
+    return cfg, f
+
+
 def cbl_ensemble(n=48, seed=0, sizes=ENSEMBLE_SIZES):
     """Members of the batched ensemble: independent dry CBL boxes (cbl), each
     with its own noise seed, a geostrophic speed drawn uniformly from

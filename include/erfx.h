@@ -56,7 +56,13 @@ enum {
     ERFX_BC_SLIP_WALL    = 1,
     ERFX_BC_NO_SLIP_WALL = 2,
     ERFX_BC_SYMMETRY     = 3,
-    ERFX_BC_OUTFLOW      = 4
+    ERFX_BC_OUTFLOW      = 4,
+    /* ERF's "Inflow" (ERF_init_bcs.cpp:100-178, 345-361, 544-580): velocity
+     * ext_dir with the face's inflow_velocity; cons ext_dir with density,
+     * density*theta, density*scalar, density*qv, density*qc (0 for the other
+     * components), or foextrap for every cons component when density is 0.
+     * Lateral faces only, with the other lateral direction periodic. */
+    ERFX_BC_INFLOW       = 5
 };
 
 /* ---- advection schemes (AdvType, Source/DataStructs/ERF_AdvStruct.H) ----
@@ -162,6 +168,16 @@ typedef struct erfx_config {
      * else (acoustic solve, averaged mass fluxes, base state, sums) in FP64.
      * No counterpart in the reference, whose results the FP64 form reproduces. */
     int    precision;
+
+    /* values of ERFX_BC_INFLOW faces, per orientation (xlo, ylo, zlo, xhi,
+     * yhi, zhi): the inputs xlo.velocity, xlo.density, xlo.theta, xlo.scalar,
+     * xlo.qv, xlo.qc of ERF_init_bcs.cpp:100-178 */
+    double inflow_velocity[6][3];
+    double inflow_density[6];
+    double inflow_theta[6];
+    double inflow_scalar[6];
+    double inflow_qv[6];
+    double inflow_qc[6];
 } erfx_config;
 
 enum { ERFX_PREC_FP64 = 0, ERFX_PREC_MIXED = 1 };

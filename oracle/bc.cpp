@@ -34,6 +34,9 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
                 case ERFX_BC_OUTFLOW:      k = (v == dir) ? NEUMANN_INT : FOEXTRAP; break;
                 case ERFX_BC_NO_SLIP_WALL: k = EXT_DIR; break;
                 case ERFX_BC_SLIP_WALL:    k = (v == dir) ? EXT_DIR : FOEXTRAP; break;
+                // inflow: every velocity component ext_dir with the face's
+                // values (ERF_init_bcs.cpp:117-120, 345-361)
+                case ERFX_BC_INFLOW:       k = EXT_DIR; vel_extdir[v][ori] = c.inflow_velocity[ori][v]; break;
                 default: throw std::runtime_error("bad bc");
             }
             bck[v][ori] = k;
@@ -43,6 +46,21 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
             case ERFX_BC_PERIODIC: kc = INT_DIR; break;
             case ERFX_BC_SYMMETRY: kc = REFLECT_EVEN; break;
             default:               kc = FOEXTRAP; break;   // outflow, slip and no-slip walls
+        }
+        for (int n = 0; n < ERFX_NCONS_MOIST; ++n) cons_extdir[n][ori] = 0.0;
+        // inflow: every cons component ext_dir with density times the face's
+        // value (m_bc_extdir_vals, ERF_init_bcs.cpp:125-163; 0 where not
+        // given), or foextrap for all when the density is 0 (:544-580)
+        if (t == ERFX_BC_INFLOW && c.inflow_density[ori] != 0.0) {
+            const double rho_in = c.inflow_density[ori];
+            kc = EXT_DIR;
+            cons_extdir[ERFX_RHO][ori] = rho_in;
+            cons_extdir[ERFX_RHOTHETA][ori] = rho_in * c.inflow_theta[ori];
+            cons_extdir[ERFX_RHOSCALAR][ori] = rho_in * c.inflow_scalar[ori];
+            if (c.moisture_type != ERFX_MOIST_NONE) {
+                cons_extdir[ERFX_RHOQ1][ori] = rho_in * c.inflow_qv[ori];
+                cons_extdir[ERFX_RHOQ2][ori] = rho_in * c.inflow_qc[ori];
+            }
         }
         bck[3][ori] = kc;
     }
@@ -107,12 +125,15 @@ void Level::fill_cons(Arr c, int icomp, int ncomp, int ng) {
         Box lo(-ng, -ng, 0, -1, ny - 1 + ng, nz - 1), hi(nx, -ng, 0, nx - 1 + ng, ny - 1 + ng, nz - 1);
         for (int n = icomp; n < icomp + ncomp; ++n) {
             int tl = bck[3][0], th = bck[3][3];
+            // ext_dir (ERF_BoundaryConditions_cons.cpp:68-104): the face's value
             loop(lo, [&](int i, int j, int k) {
-                if (tl == FOEXTRAP) c(i, j, k, n) = c(0, j, k, n);
+                if (tl == EXT_DIR) c(i, j, k, n) = cons_extdir[n][0];
+                else if (tl == FOEXTRAP) c(i, j, k, n) = c(0, j, k, n);
                 else if (tl == REFLECT_EVEN) c(i, j, k, n) = c(-1 - i, j, k, n);
             });
             loop(hi, [&](int i, int j, int k) {
-                if (th == FOEXTRAP) c(i, j, k, n) = c(nx - 1, j, k, n);
+                if (th == EXT_DIR) c(i, j, k, n) = cons_extdir[n][3];
+                else if (th == FOEXTRAP) c(i, j, k, n) = c(nx - 1, j, k, n);
                 else if (th == REFLECT_EVEN) c(i, j, k, n) = c(2 * (nx - 1) + 1 - i, j, k, n);
             });
         }
@@ -122,11 +143,13 @@ void Level::fill_cons(Arr c, int icomp, int ncomp, int ng) {
         for (int n = icomp; n < icomp + ncomp; ++n) {
             int tl = bck[3][1], th = bck[3][4];
             loop(lo, [&](int i, int j, int k) {
-                if (tl == FOEXTRAP) c(i, j, k, n) = c(i, 0, k, n);
+                if (tl == EXT_DIR) c(i, j, k, n) = cons_extdir[n][1];
+                else if (tl == FOEXTRAP) c(i, j, k, n) = c(i, 0, k, n);
                 else if (tl == REFLECT_EVEN) c(i, j, k, n) = c(i, -1 - j, k, n);
             });
             loop(hi, [&](int i, int j, int k) {
-                if (th == FOEXTRAP) c(i, j, k, n) = c(i, ny - 1, k, n);
+                if (th == EXT_DIR) c(i, j, k, n) = cons_extdir[n][4];
+                else if (th == FOEXTRAP) c(i, j, k, n) = c(i, ny - 1, k, n);
                 else if (th == REFLECT_EVEN) c(i, j, k, n) = c(i, 2 * (ny - 1) + 1 - j, k, n);
             });
         }

@@ -223,6 +223,7 @@ struct Level {
     // BC kinds: [var][ori], var 0..2 = x/y/z velocity, 3 = cons; ori 0..5 = xlo,ylo,zlo,xhi,yhi,zhi
     int bck[4][6];
     double vel_extdir[3][6];
+    double cons_extdir[ERFX_NCONS_MOIST][6];   // ext_dir values of the cons components (inflow faces)
     bool periodic[3];
 
     // persistent state (vars_new) and base state

@@ -35,6 +35,9 @@ def _setup(kind):
     elif kind == "norain":
         cfg, f = P.moist_bubble(24, 20, 16, seed=17, norain=True)
         dt = 1.0
+    elif kind.startswith("inflow"):   # ERF's Inflow on x-lo, Outflow on x-hi
+        cfg, f = P.inflow_outflow(24, 16, 16, seed=19, moist=(kind == "inflow_moist"))
+        dt = 1.0
     elif kind.startswith("hill"):   # terrain-following grid over a Gaussian hill
         adv = C.ADV_CENTERED2 if kind == "hill_c2" else C.ADV_UPWIND3
         cfg, f = P.hill_flow(24, 20, 16, seed=7, adv=adv, moist=(kind == "hill_moist"))
@@ -78,7 +81,7 @@ def _st(sim, s):
 
 TERRAIN_IDS = [C.F_Z_ND, C.D_DETJ, C.D_AX, C.D_AY, C.D_AZ, C.D_Z_CC]
 HILLS = ["hill", "hill_c2", "hill_moist"]
-KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain"] + HILLS
+KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain", "inflow", "inflow_moist"] + HILLS
 MOIST_BUOYANCY = ["moist1", "moist2", "moist3", "moist4"]
 
 

@@ -264,3 +264,35 @@ def test_hundred_steps(kind):
         cfg, f = P.supercell(20, 20, 20, seed=45)
         dt = 2.0
     _compare(cfg, f, dt, 100, 1e-9)
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_inflow_outflow_step_exact(moist):
+    """ERF's Inflow (velocity and cons ext_dir values) on x-lo, Outflow on x-hi."""
+    cfg, f = P.inflow_outflow(24, 16, 16, seed=23, moist=moist)
+    _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
+
+
+def test_inflow_outflow_multistep():
+    cfg, f = P.inflow_outflow(24, 16, 16, seed=24, adv=C.ADV_UPWIND5)
+    _compare(cfg, f, 1.0, 8, 1e-9)
+
+
+def test_inflow_refusals():
+    """Inflow on a vertical face, with the other lateral direction walled, or
+    with diffusion: refused at creation."""
+    from erf_amd.lib import Sim, ErfxError
+    cases = []
+    cfg, _ = P.inflow_outflow(16, 16, 16)
+    cfg.bc[1] = cfg.bc[4] = C.BC_SLIP_WALL
+    cases.append(cfg)
+    cfg, _ = P.inflow_outflow(16, 16, 16)
+    cfg.molec_diff_type = C.MOLEC_CONSTANT_ALPHA
+    cfg.dynamic_viscosity = 1.0
+    cases.append(cfg)
+    cfg, _ = P.inflow_outflow(16, 16, 16)
+    cfg.bc[5] = C.BC_INFLOW
+    cases.append(cfg)
+    for c in cases:
+        with pytest.raises(ErfxError):
+            Sim(c)
