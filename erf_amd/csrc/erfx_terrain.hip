@@ -226,20 +226,23 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         double J_km1 = J[q0], s0_km1 = sr0[q0], s1_km1 = sr1[q0];
         double p_km2 = prim[q0 - sz], p_km1 = prim[q0];
         double q1_km1 = moist ? pq1[q0] : 0.0, q2_km1 = moist ? pq2[q0] : 0.0;
+        // Each level reads everything it needs before its first store (the
+        // stores may alias the inputs as far as the compiler knows, so a load
+        // after a store waits for another round trip to memory).
         for (int k = 1; k <= nz - 1; ++k) {
             const long q = q0 + (long)k * sz;
-            const double om_kp1 = omega_old(k + 1);
-            double t0_k, t1_k;
-            trhs_at(k, t0_k, t1_k);
-            om[q] = om_k;
-            trhs0[q] = t0_k;
-            trhs1[q] = t1_k;
             const double Jk = J[q], s0_k = sr0[q], s1_k = sr1[q], p_k = prim[q], p_kp1 = prim[q + sz];
             const double drho_k = cc0[q] - sc0[q], drt_k = cc1[q] - sc1[q];
             const double q1_k = moist ? pq1[q] : 0.0, q2_k = moist ? pq2[q] : 0.0;
+            const double fP_k = fP[q], fQ_k = fQ[q], fA_k = fA[q], fB_k = fB[q], srz_k = srz[q];
+            const double dw_k = pzm[q] - szm[q];
+            const double om_kp1 = omega_old(k + 1);
+            double t0_k, t1_k;
+            trhs_at(k, t0_k, t1_k);
+            const double ofw = met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
             double detJ_on_kface = 0.5 * (Jk + J_km1);
-            double coeff_P = fP[q];
-            double coeff_Q = fQ[q];
+            double coeff_P = fP_k;
+            double coeff_Q = fQ_k;
             if (moist) {
                 double qf = 0.5 * (q1_k + q1_km1 + q2_k + q2_km1);
                 coeff_P /= (1.0 + qf);
@@ -255,10 +258,12 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             R1_tmp += (halfg) * (beta_1 * dzi * (om_kp1 - om_km1) + t0_k + t0_km1);
             R1_tmp += -(coeff_P * (beta_1 * dzi * (om_kp1 * theta_t_hi - om_k * theta_t_mid) + t1_k) +
                         coeff_Q * (beta_1 * dzi * (om_k * theta_t_mid - om_km1 * theta_t_lo) + t1_km1));
-            const double dw_k = pzm[q] - szm[q];
-            double rhs = detJ_on_kface * dw_k + dtau * (detJ_on_kface * srz[q] + R0_tmp + dtau * beta_2 * R1_tmp);
-            rhs += detJ_on_kface * met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
-            s_prev = (rhs - fA[q] * s_prev) * fB[q];
+            double rhs = detJ_on_kface * dw_k + dtau * (detJ_on_kface * srz_k + R0_tmp + dtau * beta_2 * R1_tmp);
+            rhs += detJ_on_kface * ofw;
+            s_prev = (rhs - fA_k * s_prev) * fB_k;
+            om[q] = om_k;
+            trhs0[q] = t0_k;
+            trhs1[q] = t1_k;
             soln[q] = s_prev;
             om_km1 = om_k; om_k = om_kp1;
             t0_km1 = t0_k; t1_km1 = t1_k;
@@ -276,24 +281,22 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         double om_hi = om_k, p_hi = prim[qt];
         for (int k = nz - 1; k >= 0; --k) {
             const long q = q0 + k * sz;
-            const double sol = soln[q] - (fC[q] * fB[q]) * sol_hi;
-            if (k >= 1) {
-                double wpp = met::w_from_omega(i, j, k, sol, fnu, fnv, z, dxi, dyi);
-                cz[q] = szm[q] + wpp;
-            } else {
-                cz[q] = szm[q];
-            }
-            const double om_lo = om[q];
+            // loads first, as in the forward sweep
+            const double soln_k = soln[q], fCB = fC[q] * fB[q], szm_k = szm[q], om_lo = om[q], azm_k = azm[q];
+            const double Jq = J[q], pq = prim[q], p_lo = prim[q - sz], tr0 = trhs0[q], tr1 = trhs1[q];
+            const double sr0_k = sr0[q], sr1_k = sr1[q], cc0_k = cc0[q], cc1_k = cc1[q];
+            const double sol = soln_k - fCB * sol_hi;
+            const double cz_k = (k >= 1) ? szm_k + met::w_from_omega(i, j, k, sol, fnu, fnv, z, dxi, dyi) : szm_k;
             double zflux_lo = beta_2 * sol + beta_1 * om_lo;
             double zflux_hi = beta_2 * sol_hi + beta_1 * om_hi;
-            azm[q] += facinv * zflux_lo / (1.0 * 1.0);
+            double fast_rhs_rho = -(tr0 + (zflux_hi - zflux_lo) * dzi) / Jq;
+            double fast_rhs_rhotheta = -(tr1 + 0.5 * (zflux_hi * (pq + p_hi) -
+                                                      zflux_lo * (pq + p_lo)) * dzi) / Jq;
+            cz[q] = cz_k;
+            azm[q] = azm_k + facinv * zflux_lo / (1.0 * 1.0);
             if (k == nz - 1) azm[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
-            const double Jq = J[q], pq = prim[q];
-            double fast_rhs_rho = -(trhs0[q] + (zflux_hi - zflux_lo) * dzi) / Jq;
-            cc0[q] += dtau * (sr0[q] + fast_rhs_rho);
-            double fast_rhs_rhotheta = -(trhs1[q] + 0.5 * (zflux_hi * (pq + p_hi) -
-                                                           zflux_lo * (pq + prim[q - sz])) * dzi) / Jq;
-            cc1[q] += dtau * (sr1[q] + fast_rhs_rhotheta);
+            cc0[q] = cc0_k + dtau * (sr0_k + fast_rhs_rho);
+            cc1[q] = cc1_k + dtau * (sr1_k + fast_rhs_rhotheta);
             sol_hi = sol;
             om_hi = om_lo;
             p_hi = pq;
