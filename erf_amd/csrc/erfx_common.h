@@ -224,6 +224,58 @@ __global__ void __launch_bounds__(256) col_kernel(int ilo, int ihi, int jlo, int
     [[clang::always_inline]] f(i, j);
 }
 
+// The same box launch with the tiles ordered k fastest: consecutive
+// work-groups take the same (x, y) tile at consecutive levels, so a kernel
+// whose stencils reach several levels up and down finds them in L2 (with x
+// fastest, the levels of a stencil are a whole plane of tiles apart).
+template <class F>
+__global__ void __launch_bounds__(256) box_kernel_z(Box b, unsigned gx, unsigned gz, F f) {
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned bz = t % gz, r = t / gz, bx = r % gx, by = r / gx;
+    int i = b.lo[0] + (int)(bx * 64 + threadIdx.x);
+    int j = b.lo[1] + (int)(by * 4 + threadIdx.y);
+    int k = b.lo[2] + (int)bz;
+    if (i > b.hi[0] || j > b.hi[1] || k > b.hi[2]) return;
+    [[clang::always_inline]] f(i, j, k);
+}
+template <class F>
+__global__ void __launch_bounds__(256) box_kernel_z_batch(const BoxItem<F>* __restrict__ it,
+                                                          const unsigned* __restrict__ starts, int n) {
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const BoxItem<F>& m = batch_member(it, starts, n, t);
+    const BHdr h = m.h;
+    const F& f = m.f;
+    const unsigned u = t - h.t0;
+    const unsigned gz = (unsigned)(h.hi[2] - h.lo[2] + 1);
+    const unsigned bz = u % gz, r = u / gz, bx = r % h.gx, by = r / h.gx;
+    int i = h.lo[0] + (int)(bx * 64 + threadIdx.x);
+    int j = h.lo[1] + (int)(by * 4 + threadIdx.y);
+    int k = h.lo[2] + (int)bz;
+    if (i > h.hi[0] || j > h.hi[1] || k > h.hi[2]) return;
+    [[clang::always_inline]] f(i, j, k);
+}
+template <class F>
+void launch_box_z_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                        hipStream_t s) {
+    (void)width;
+    hipLaunchKernelGGL(box_kernel_z_batch<F>, dim3(total), dim3(64, 4, 1), 0, s,
+                       static_cast<const BoxItem<F>*>(items), starts, n);
+}
+template <class F>
+inline void launch_box_z(const Box& b, hipStream_t s, F f) {
+    const bool ok = b.ok();
+    const unsigned gx = ok ? (unsigned)((b.hi[0] - b.lo[0] + 1 + 63) / 64) : 1u;
+    const unsigned gy = ok ? (unsigned)((b.hi[1] - b.lo[1] + 1 + 3) / 4) : 1u;
+    const unsigned gz = ok ? (unsigned)(b.hi[2] - b.lo[2] + 1) : 0u;
+    if (g_rec) {
+        BoxItem<F> it{{{b.lo[0], b.lo[1], b.lo[2]}, {b.hi[0], b.hi[1], b.hi[2]}, gx, gy, 0u, gx * gy * gz}, f};
+        rec_push(&launch_box_z_batch<F>, 256u, it);
+        return;
+    }
+    if (!ok) return;
+    hipLaunchKernelGGL(box_kernel_z<F>, dim3(gx * gy * gz), dim3(64, 4, 1), 0, s, b, gx, gz, f);
+}
+
 // Work-group launch for block-level reductions: f(u) runs on all 256 lanes of
 // work-group u of a 1-D grid.
 template <class F>

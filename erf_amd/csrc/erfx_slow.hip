@@ -897,8 +897,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const int nqs = P.n_qstate;
     const double* pavg = W.pavg;
     // momentum face values stay FP64 in both forms: this kernel streams at
-    // near the HBM peak, and FP32 face values only add registers to it
-    launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+    // near the HBM peak, and FP32 face values only add registers to it.
+    // Its stencils reach three levels up and down: tiles ordered k fastest.
+    launch_box_z(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         constexpr bool mixed = false;
         const TAdv<mixed> TA{TD};
         const long q = L.ix(i, j, k);
