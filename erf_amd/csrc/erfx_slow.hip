@@ -288,8 +288,10 @@ __device__ __forceinline__ double interp_z_all(const double* f, long q, long sz,
 // so each form is its own kernel (the FP64 kernel carries no FP32 code).
 template <class F>
 inline void launch_box_prec(bool mixed, const Box& b, hipStream_t s, F f) {
-    if (mixed) launch_box(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::true_type{}); });
-    else launch_box(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::false_type{}); });
+    // the scalar and theta stencils reach several levels up and down: tiles
+    // ordered k fastest (launch_box_z)
+    if (mixed) launch_box_z(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::true_type{}); });
+    else launch_box_z(b, s, [=] __device__(int i, int j, int k) { f(i, j, k, std::false_type{}); });
 }
 
 // ---- momentum advection on a terrain-following grid ---------------------
