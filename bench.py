@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X dycore step (one GPU).
 
-Workload: the dry convective-boundary-layer LES, 256 x 256 x 128 cells
-(dx = dy = 20 m, dz = 10 m), 5th-order upwind advection, Smagorinsky Cs = 0.1,
-Coriolis + geostrophic forcing, no-slip bottom with surface heat flux, FP64,
-dt = 0.5 s, MRI ratio 16 (25 acoustic substeps per RK3 step).  Synthetic
+Default workload (the headline): moist flow over terrain with Kessler warm
+rain on the terrain-following path, 512 x 512 x 96 cells (dx = dy = 200 m,
+levels stretched from 20 m to 400 m up to a 15 km top, 500 m Gaussian hill),
+WENO5 scalars, 3rd-order upwind momentum, Rayleigh damping above 12 km,
+dt = 1 s, MRI ratio 6 (1 + 3 + 6 acoustic substeps per step).  Synthetic
 seeded initial state.  A step is one full RK3 step (slow RHS, acoustic
-substeps, boundary fills) with the state resident in HBM.
+substeps, boundary fills, microphysics) with the state resident in HBM.
 
-Prints one JSON line: cell-updates/s per RK3 step, the roofline of the
-dominant kernel (HBM bytes per launch / average launch time, HIP events on the
-library's stream) and the CPU oracle timed on a bounded sample.
+The FP64 path and the mixed-precision path are timed on the same state, and
+the mixed path's error against FP64 is measured after --check-steps steps
+(100 by default) from the same initial state: relative L2 and Linf of rho,
+rho*theta, u, v, w, qv, qc and qr.  One JSON line carries cell-updates/s of
+both paths, the fraction of the HBM peak each reaches on the algorithmic bytes
+of a step, the roofline of the dominant kernel (HIP events on the library's
+stream around each launch) and the CPU oracle timed on a bounded sample.
+
+Other workloads (--workload): cbl, thermal, ensemble, supercell.
 """
 import argparse
 import ctypes
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -23,19 +31,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fast_rhs_N_column_cbl256x256x128_traffic.json")
+METRIC = "cell-updates/s per RK3 step, mixed vs FP64 GPU at stated L2 error; % HBM BW"
 
 
-def measured_traffic(kernel, bytes_per_launch):
-    """HBM bytes per launch of the timed kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE measurement, if it was taken on this kernel and
-    workload (same name and algorithmic byte count); otherwise None."""
+def measured_traffic(kernel, workload, bytes_per_launch):
+    """HBM bytes per launch of the timed kernel from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (profiles/<kernel>_<workload>_traffic.json),
+    if it was taken on this kernel and workload (same algorithmic byte count);
+    otherwise None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel.split(" ")[0], workload))
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or abs(t.get("algorithmic_bytes_per_launch", -1) - bytes_per_launch) > 0.5:
+    if abs(t.get("algorithmic_bytes_per_launch", -1) - bytes_per_launch) > 0.5:
         return None
     return t["traffic_bytes_per_launch"]
 
@@ -45,17 +55,23 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ny", type=int, default=256)
-    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--workload", choices=["cbl", "moist_terrain", "thermal", "ensemble", "supercell"], default="cbl",
-                    help="cbl (default): the dry CBL LES; moist_terrain: moist flow over a hill on the "
-                         "terrain-following path (stretched levels, WENO5, Kessler, Rayleigh), dt = 1 s; "
-                         "thermal: the 64^3 dry rising thermal, C2, dt = 1 s, 6 substeps per stage; "
-                         "ensemble: dry CBL members of uneven box sizes advanced as one batch; "
-                         "supercell: Weisman-Klemp deep convection, the learned Kessler stand-in trained on "
-                         "the plain method's runs and run against plain Kessler")
+    ap.add_argument("--workload", choices=["moist_terrain", "cbl", "thermal", "ensemble", "supercell"],
+                    default="moist_terrain",
+                    help="moist_terrain (default): the headline, moist flow over a hill on the terrain-following "
+                         "path, FP64 and mixed precision; cbl: the dry CBL LES 256x256x128; thermal: the 64^3 dry "
+                         "rising thermal; ensemble: dry CBL members of uneven box sizes advanced as one batch; "
+                         "supercell: Weisman-Klemp deep convection, the learned Kessler stand-in trained on the "
+                         "plain method's runs and run against plain Kessler")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default=None,
+                    help="paths to time (default: both for moist_terrain, fp64 otherwise).  With mixed or both the "
+                         "mixed path's error against FP64 is measured after --check-steps steps")
+    ap.add_argument("--check-steps", type=int, default=100,
+                    help="steps from the initial state after which the mixed path is compared with FP64 "
+                         "(0: no comparison)")
     ap.add_argument("--sc-seeds", type=int, default=2, help="supercell: training runs (seeds 0..n-1)")
     ap.add_argument("--sc-steps", type=int, default=600, help="supercell: steps of each training and evaluation run")
     ap.add_argument("--sc-every", type=int, default=2, help="supercell: rows harvested every n-th step")
@@ -68,16 +84,17 @@ def parse():
     ap.add_argument("--unbatched-steps", type=int, default=2,
                     help="ensemble: steps timed with every member launched on its own, for comparison")
     ap.add_argument("--graph", action="store_true", help="replay the steps from a captured hipGraph")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="mixed: FP32 face values of the advected quantities on perturbations; the same steps "
-                         "are then run in FP64 too and the relative L2 / Linf differences are reported")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default acoustic column kernel instead of timing its variants in the warm-up")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+                    help="keep the default acoustic column kernel instead of timing its variants in the warm-up "
+                         "(flat grids; the terrain column kernel has one form)")
+    ap.add_argument("--cpu-seconds", type=float, default=6.0, help="CPU oracle: seconds per timed run (3 runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (3 launches), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counter readings")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.precision is None:
+        a.precision = "both" if a.workload == "moist_terrain" else "fp64"
+    return a
 
 
 def calibrate_copy():
@@ -92,30 +109,270 @@ def calibrate_copy():
                       "launches": 3}))
 
 
-def cpu_baseline(nx, ny, nz, budget_s):
-    """Time the CPU oracle (test infrastructure, OpenMP) on a bounded sample of
-    the same workload: a 64 x 64 x nz column block of the CBL case, as many
-    steps as fit in ~budget_s."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+# ---- workloads -----------------------------------------------------------------
+def make_case(workload, nx=None, ny=None, nz=None, seed=0):
+    """(cfg, fields, rayleigh profiles or None, dt, description) of a single-box workload."""
     from erf_amd import problems as P
+    if workload == "moist_terrain":
+        n = (nx or 512, ny or 512, nz or 96)
+        cfg, f = P.moist_terrain(*n, seed=seed)
+        ray = P.rayleigh_profiles_uniform(cfg, f=f)
+        desc = ("moist flow over a 500 m Gaussian hill %dx%dx%d, dx=200 m, levels stretched 20-400 m to a 15 km top, "
+                "WENO5 scalars, U3 momentum, Kessler, Rayleigh above 12 km, dt=1 s, MRI ratio %d (%d substeps/step)")
+        return cfg, f, ray, 1.0, desc
+    if workload == "thermal":
+        n = (nx or 64, ny or 64, nz or 64)
+        cfg, f = P.rising_thermal(*n, seed=seed)
+        desc = "dry rising thermal %dx%dx%d, C2 advection, dt=1 s, MRI ratio %d (%d substeps/step)"
+        return cfg, f, None, 1.0, desc
+    n = (nx or 256, ny or 256, nz or 128)
+    cfg, f = P.cbl(*n, seed=seed)
+    desc = ("dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, "
+            "MRI ratio %d (%d substeps/step)")
+    return cfg, f, None, 0.5, desc
+
+
+def n_substeps(cfg):
+    return 1 + cfg.mri_ratio // 2 + cfg.mri_ratio
+
+
+def alg_bytes_per_cell(cfg):
+    """Algorithmic HBM bytes per cell and RK3 step: each distinct 3-D FP64 field
+    read or written once per sweep (slow RHS per stage, acoustic substep, slow
+    post per stage, once per step), intermediates, velocities and 1-D data not
+    counted: 8 B x (3 F_pre + N_sub F_sub + 3 F_post + F_once)."""
+    from erf_amd import config as C
+    moist = cfg.moisture_type != C.MOIST_NONE
+    smag = cfg.les_type == C.LES_SMAGORINSKY
+    terr = cfg.use_terrain != 0
+    f_pre = 10 + 2 * moist + smag + 4 * terr
+    f_sub = 28 + 2 * moist + 4 * terr
+    n_s = 4 if moist else 1
+    f_post = 3 * n_s + 4 + smag + terr
+    f_once = 5 * smag + 9 * moist
+    return 8 * (3 * f_pre + n_substeps(cfg) * f_sub + 3 * f_post + f_once)
+
+
+def time_leg(sim, dt, steps, warmup, use_timer, autotune, graph):
+    """Warm-up, then `steps` steps timed with HIP events on the sim's stream.
+    Returns (ms per step, timer reading or None)."""
+    import torch
+    from erf_amd.lib import lib, check
+    stream = torch.cuda.ExternalStream(sim.stream)
+    if graph:
+        sim.set_graph(True)
+    if autotune and warmup > 0:
+        sim.autotune(True)       # the warm-up steps time the column kernel's variants
+    sim.step(0.0, dt, warmup)
+    t = warmup * dt
+    variant, _ = sim.column_variant()
+    extra = 0
+    while variant < 0 and extra < 5:     # short runs: finish timing the variants before the timed steps
+        sim.step(t, dt, 1)
+        t += dt
+        extra += 1
+        variant, _ = sim.column_variant()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    if use_timer:
+        check(lib().erfx_sim_timer_reset(sim.h))
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    sim.step_async(t, dt, steps)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    sim.sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    timer = None
+    if use_timer:
+        kname = ctypes.create_string_buffer(128)
+        kms, kcnt, kbytes = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        check(lib().erfx_sim_timer_read(sim.h, kname, 128, ctypes.byref(kms), ctypes.byref(kcnt),
+                                        ctypes.byref(kbytes)))
+        timer = {"kernel": kname.value.decode(), "total_ms": kms.value, "launches": kcnt.value,
+                 "bytes_per_launch": kbytes.value}
+        check(lib().erfx_sim_timer_stop(sim.h))
+    return ms, timer, variant
+
+
+def check_state(sim, cfg):
+    from erf_amd import config as C
+    import numpy as np
+    ng = sim.L.ng
+    rho = sim.download(C.F_CONS)[0][ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
+    assert np.all(np.isfinite(rho)) and rho.min() > 0.05, "state blew up"
+
+
+def errors_vs(sim, ref, cfg):
+    """Relative L2 and Linf of the difference sim - ref over the valid cells:
+    rho and rho*theta relative to their deviation from the mean, velocities and
+    mixing ratios relative to the field itself."""
+    import numpy as np
+    from erf_amd import config as C
+    ng = sim.L.ng
+    fields = [(C.F_CONS, C.RHO, "rho"), (C.F_CONS, C.RHOTHETA, "rhotheta"), (C.F_XVEL, 0, "u"),
+              (C.F_YVEL, 0, "v"), (C.F_ZVEL, 0, "w")]
+    if cfg.moisture_type:
+        fields += [(C.F_CONS, C.RHOQ1, "qv"), (C.F_CONS, C.RHOQ2, "qc"), (C.F_CONS, C.RHOQ3, "qr")]
+    got = {}
+    out = {}
+    sl = (slice(ng, ng + cfg.nz), slice(ng, ng + cfg.ny), slice(ng, ng + cfg.nx))
+    for fid, comp, name in fields:
+        if fid not in got:
+            got[fid] = (sim.download(fid), ref.download(fid))
+        x = got[fid][0][comp][sl]
+        y = got[fid][1][comp][sl]
+        if fid == C.F_CONS and comp >= C.RHOQ1:      # mixing ratios q = rho q / rho
+            x = x / got[fid][0][C.RHO][sl]
+            y = y / got[fid][1][C.RHO][sl]
+        r = y - y.mean() if name in ("rho", "rhotheta") else y
+        nrm = float(np.sqrt((r ** 2).mean()))
+        amax = float(np.abs(r).max())
+        d = x - y
+        out[name] = {"rel_l2": float(np.sqrt((d ** 2).mean())) / nrm if nrm > 0 else float(np.abs(d).max()),
+                     "rel_linf": float(np.abs(d).max()) / amax if amax > 0 else float(np.abs(d).max())}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload, nz, budget_s):
+    """The CPU oracle (test infrastructure: C++ restatement of the reference,
+    OpenMP over k-planes, g++ -O3 -march=x86-64-v3 -ffp-contract=off) on a bounded
+    sample of the same workload: a 64 x 64 x nz block of the same case, as many
+    steps as fit in ~budget_s, three timed runs, the median reported."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
-    sx = min(nx, 64)
-    sy = min(ny, 64)
-    cfg, f = P.cbl(sx, sy, nz, seed=0)
+    cfg, f, ray, dt, _ = make_case(workload, 64, 64, nz, seed=0)
     orc = Oracle(cfg)
     orc.load(f)
-    orc.step(0.0, 0.5, 1)                     # warm-up
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        orc.step(0.5 * (n + 1), 0.5, 1)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    if ray is not None:
+        orc.set_rayleigh(*ray)
+    orc.step(0.0, dt, 1)                      # warm-up
+    t = dt
+    rates = []
+    steps_per_run = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            orc.step(t, dt, 1)
+            t += dt
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        el = time.perf_counter() - t0
+        rates.append(cfg.nx * cfg.ny * cfg.nz * n / el)
+        steps_per_run.append(n)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": sx * sy * nz * n / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d steps of the CBL case on a %dx%dx%d block (OpenMP C++ oracle, -O2)" % (n, sx, sy, nz)}
+    return {"value": statistics.median(rates), "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "runs": rates,
+            "sample": "median of 3 runs of %s steps of the same case on a 64x64x%d block (the oracle: C++ "
+                      "restatement of the reference, OpenMP, -O3 -march=x86-64-v3)"
+                      % ("/".join(str(s) for s in steps_per_run), nz)}
+
+
+def run_single(a):
+    """A single-box workload: the FP64 and / or mixed-precision path timed on the
+    same state, the mixed path's error against FP64 after --check-steps steps."""
+    import torch
+    from erf_amd import config as C
+    from erf_amd.lib import Sim
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    print("building the %s case" % a.workload, file=sys.stderr, flush=True)
+    cfg, f, ray, dt, desc = make_case(a.workload, a.nx, a.ny, a.nz, seed=0)
+    legs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    cells = cfg.nx * cfg.ny * cfg.nz
+    b_cell = alg_bytes_per_cell(cfg)
+
+    def make_sim(prec):
+        c = C.Config.from_buffer_copy(cfg)
+        c.precision = C.PREC_MIXED if prec == "mixed" else C.PREC_FP64
+        s_ = Sim(c)
+        s_.load(f)
+        if ray is not None:
+            s_.set_rayleigh(*ray)
+        return s_
+
+    sims = {}
+    res = {}
+    timer = None
+    variant = None
+    for leg in legs:
+        sims[leg] = make_sim(leg)
+        print("%s state uploaded" % leg, file=sys.stderr, flush=True)
+        ms, tm, var = time_leg(sims[leg], dt, a.steps, a.warmup, use_timer=(leg == legs[0] and not a.graph),
+                               autotune=not a.no_autotune, graph=a.graph)
+        check_state(sims[leg], cfg)
+        if tm is not None:
+            timer = tm
+        if leg == legs[0]:
+            variant = var
+        step_gbs = b_cell * cells / (ms * 1e-3) / 1e9
+        res[leg] = {"value": cells / (ms * 1e-3), "ms_per_step": ms,
+                    "step_hbm": {"bytes_per_step": b_cell * cells, "bytes_per_cell": b_cell,
+                                 "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS}}
+        print("%s: %.2f ms/step" % (leg, ms), file=sys.stderr, flush=True)
+
+    errors = None
+    if "mixed" in legs and a.check_steps > 0:
+        # both paths from the same initial state, check_steps steps each
+        ref = sims.get("fp64") or make_sim("fp64")
+        sims["fp64"] = ref
+        for leg in ("fp64", "mixed"):
+            sims[leg].load(f)
+            sims[leg].step(0.0, dt, a.check_steps)
+            print("%s: %d checking steps done" % (leg, a.check_steps), file=sys.stderr, flush=True)
+        errors = {"steps": a.check_steps, "fields": errors_vs(sims["mixed"], ref, cfg)}
+    for s_ in sims.values():
+        s_.close()
+
+    head = res[legs[-1]]
+    precision = "mixed vs fp64" if len(legs) == 2 else legs[0]
+    out = {
+        "metric": METRIC if a.workload == "moist_terrain" else
+        "cell-updates/s per RK3 step (%s GPU path)" % ("mixed" if legs[-1] == "mixed" else "FP64"),
+        "value": head["value"], "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if legs[-1] == "mixed" else "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": desc % (cfg.nx, cfg.ny, cfg.nz, cfg.mri_ratio, n_substeps(cfg)),
+                   "cells": cells, "precision": precision, "graph": bool(a.graph),
+                   "value_is": "the %s path" % legs[-1]},
+    }
+    for leg in legs:
+        out[leg] = res[leg]
+    if len(legs) == 2:
+        out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    if errors is not None:
+        out["error_vs_fp64"] = errors
+    if timer is not None and timer["launches"] > 0:
+        avg_ms = timer["total_ms"] / timer["launches"]
+        achieved = timer["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": timer["kernel"], "path": legs[0], "achieved": achieved,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(timer["kernel"], a.workload, timer["bytes_per_launch"]),
+                           "algorithmic_bytes": timer["bytes_per_launch"], "avg_launch_ms": avg_ms,
+                           "launches": timer["launches"]}
+    if variant is not None and not cfg.use_terrain:
+        out["column_variant"] = variant
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.workload, cfg.nz, a.cpu_seconds)
+    print(json.dumps(out))
 
 
 def run_ensemble(a):
@@ -254,7 +511,7 @@ def run_supercell(a):
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (256, 256, 80)
+    nx, ny, nz = a.nx or 256, a.ny or 256, a.nz or 80
     dt = 2.0
     case = lambda seed: P.supercell(nx, ny, nz, seed=seed)
     w0 = time.perf_counter()
@@ -359,163 +616,7 @@ def main():
     if a.workload == "supercell":
         run_supercell(a)
         return
-    import numpy as np
-    import torch
-    from erf_amd import config as C, problems as P
-    from erf_amd.lib import Sim, lib, check
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
-    ray = None
-    if a.workload == "thermal":
-        nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (64, 64, 64)
-        cfg, f = P.rising_thermal(nx, ny, nz, seed=0)
-        dt = 1.0
-    elif a.workload == "moist_terrain":
-        nx, ny, nz = (a.nx, a.ny, a.nz) if a.nx != 256 or a.nz != 128 else (512, 512, 96)
-        print("building the moist terrain case %dx%dx%d" % (nx, ny, nz), file=sys.stderr, flush=True)
-        cfg, f = P.moist_terrain(nx, ny, nz, seed=0)
-        ray = P.rayleigh_profiles_uniform(cfg, f=f)
-        dt = 1.0
-    else:
-        cfg, f = P.cbl(a.nx, a.ny, a.nz, seed=0)
-        dt = 0.5
-    if a.precision == "mixed":
-        cfg.precision = C.PREC_MIXED
-
-    def make_sim(c):
-        s_ = Sim(c)
-        s_.load(f)
-        if ray is not None:
-            s_.set_rayleigh(*ray)
-        return s_
-
-    sim = make_sim(cfg)
-    print("state uploaded", file=sys.stderr, flush=True)
-    stream = torch.cuda.ExternalStream(sim.stream)
-
-    if a.graph:
-        sim.set_graph(True)
-    if not a.no_autotune and a.warmup > 0:
-        sim.autotune(True)       # the warm-up steps time the column kernel's variants
-    sim.step(0.0, dt, a.warmup)
-    t = a.warmup * dt
-    col_variant, col_ms = sim.column_variant()
-    extra = 0
-    while col_variant < 0 and extra < 5:     # short runs: finish timing the variants before the timed steps
-        sim.step(t, dt, 1)
-        t += dt
-        extra += 1
-        col_variant, col_ms = sim.column_variant()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    if a.workload == "cbl" and not a.graph:
-        check(lib().erfx_sim_timer_reset(sim.h))
-    with torch.cuda.stream(stream):
-        e0.record(stream)
-    wall0 = time.perf_counter()
-    sim.step_async(t, dt, a.steps)
-    with torch.cuda.stream(stream):
-        e1.record(stream)
-    sim.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = e0.elapsed_time(e1)
-
-    cells = cfg.nx * cfg.ny * cfg.nz
-    nsub = 1 + cfg.mri_ratio // 2 + cfg.mri_ratio
-    value = cells * a.steps / (ms * 1e-3)
-
-    # dominant kernel: per-launch timing collected by the library with HIP events
-    kname = ctypes.create_string_buffer(128)
-    kms = ctypes.c_double()
-    kcnt = ctypes.c_long()
-    kbytes = ctypes.c_double()
-    check(lib().erfx_sim_timer_read(sim.h, kname, 128, ctypes.byref(kms), ctypes.byref(kcnt), ctypes.byref(kbytes)))
-    avg_ms = kms.value / max(kcnt.value, 1)
-    achieved = kbytes.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-
-    # sanity: the state stayed physical
-    rho = sim.download(C.F_CONS)[0]
-    ng = sim.L.ng
-    rv = rho[ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
-    assert np.all(np.isfinite(rv)) and rv.min() > 0.05, "state blew up"
-    errors = None
-    if a.precision == "mixed":
-        # the same steps in FP64 from the same state: relative L2 / Linf of the difference
-        c64 = C.Config.from_buffer_copy(cfg)
-        c64.precision = C.PREC_FP64
-        ref = make_sim(c64)
-        ref.step(0.0, dt, a.warmup + extra + a.steps)
-        errors = {}
-        fields = [(C.F_CONS, C.RHO, "rho"), (C.F_CONS, C.RHOTHETA, "rhotheta"), (C.F_XVEL, 0, "u"),
-                  (C.F_YVEL, 0, "v"), (C.F_ZVEL, 0, "w")]
-        if cfg.moisture_type:
-            fields += [(C.F_CONS, C.RHOQ1, "qv"), (C.F_CONS, C.RHOQ2, "qc"), (C.F_CONS, C.RHOQ3, "qr")]
-        got = {}
-        for fid, comp, name in fields:
-            if fid not in got:
-                got[fid] = (sim.download(fid), ref.download(fid))
-            x = got[fid][0][comp, ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
-            y = got[fid][1][comp, ng:ng + cfg.nz, ng:ng + cfg.ny, ng:ng + cfg.nx]
-            r = y - y.mean() if name in ("rho", "rhotheta") else y
-            nrm = float(np.sqrt((r ** 2).mean()))
-            errors[name] = {"rel_l2": float(np.sqrt(((x - y) ** 2).mean())) / nrm if nrm > 0 else 0.0,
-                            "rel_linf": float(np.abs(x - y).max() / np.abs(r).max()) if np.abs(r).max() > 0 else 0.0}
-        ref.close()
-    sim.close()
-
-    if a.workload != "cbl" or a.graph or a.precision == "mixed":
-        wl = {"thermal": "dry rising thermal %dx%dx%d, C2 advection, dt=1 s, MRI ratio %d (%d substeps/step)",
-              "moist_terrain": "moist flow over a 500 m Gaussian hill %dx%dx%d, dx=200 m, levels stretched "
-                               "20-400 m to a 15 km top, WENO5 scalars, U3 momentum, Kessler, Rayleigh above "
-                               "12 km, dt=1 s, MRI ratio %d (%d substeps/step)",
-              "cbl": "dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, dt=0.5 s, "
-                     "MRI ratio %d (%d substeps/step)"}[a.workload]
-        out = {
-            "metric": "cell-updates/s per RK3 step (%s GPU path)" % ("mixed" if a.precision == "mixed" else "FP64"),
-            "value": value, "unit": "cell-updates/s",
-            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms / a.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64+f32" if a.precision == "mixed" else "f64",
-            "data": "synthetic (seeded)",
-            "config": {"workload": wl % (cfg.nx, cfg.ny, cfg.nz, cfg.mri_ratio, nsub),
-                       "cells": cells, "precision": a.precision, "graph": bool(a.graph)},
-            "wall_s": wall,
-        }
-        if errors is not None:
-            out["error_vs_fp64"] = errors
-        print(json.dumps(out))
-        return
-
-    out = {
-        "metric": "cell-updates/s per RK3 step (FP64 GPU path)",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded)",
-        "config": {"workload": "dry CBL LES %dx%dx%d, U5 advection, Smagorinsky, Coriolis+geostrophic, "
-                               "dt=0.5 s, MRI ratio %d (%d substeps/step)" % (cfg.nx, cfg.ny, cfg.nz,
-                                                                             cfg.mri_ratio, nsub),
-                   "cells": cells, "precision": "fp64"},
-        "column_variant": {"variant": col_variant, "best_ms": col_ms},
-        "roofline": {"bound": "hbm", "kernel": kname.value.decode(), "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kname.value.decode(), kbytes.value),
-                     "algorithmic_bytes": kbytes.value, "avg_launch_ms": avg_ms, "launches": kcnt.value},
-        "wall_s": wall,
-    }
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg.nx, cfg.ny, cfg.nz, a.cpu_seconds)
-    print(json.dumps(out))
+    run_single(a)
 
 
 if __name__ == "__main__":

@@ -84,6 +84,10 @@ class Config(ctypes.Structure):
         ("inflow_density", ctypes.c_double * 6), ("inflow_theta", ctypes.c_double * 6),
         ("inflow_scalar", ctypes.c_double * 6), ("inflow_qv", ctypes.c_double * 6),
         ("inflow_qc", ctypes.c_double * 6),
+        ("prob_hi", ctypes.c_double * 3),
+        ("no_substepping", ctypes.c_int),
+        ("wall_velocity", (ctypes.c_double * 3) * 6),
+        ("abl_pressure_grad", ctypes.c_double * 3),
     ]
 
 

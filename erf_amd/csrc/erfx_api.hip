@@ -43,7 +43,10 @@ void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
                 case ERFX_BC_PERIODIC: k = INT_DIR; break;
                 case ERFX_BC_SYMMETRY: k = (v == dir) ? REFLECT_ODD : REFLECT_EVEN; break;
                 case ERFX_BC_OUTFLOW: k = (v == dir) ? NEUMANN_INT : FOEXTRAP; break;
-                case ERFX_BC_NO_SLIP_WALL: k = EXT_DIR; break;
+                case ERFX_BC_NO_SLIP_WALL:   // <face>.velocity, normal component 0 (ERF_init_bcs.cpp:184-200)
+                    k = EXT_DIR;
+                    P.vel_extdir[v][ori] = (v == dir) ? 0.0 : c.wall_velocity[ori][v];
+                    break;
                 case ERFX_BC_SLIP_WALL: k = (v == dir) ? EXT_DIR : FOEXTRAP; break;
                 case ERFX_BC_INFLOW: k = EXT_DIR; P.vel_extdir[v][ori] = c.inflow_velocity[ori][v]; break;
             }
@@ -102,8 +105,8 @@ static int validate(const erfx_config& c) {
             if (c.bc[1 - d] != ERFX_BC_PERIODIC) {
                 set_error("inflow needs the other lateral direction periodic"); return ERFX_ERR_UNSUPPORTED;
             }
-            if (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE || c.use_terrain) {
-                set_error("inflow with diffusion or terrain is not supported"); return ERFX_ERR_UNSUPPORTED;
+            if (c.les_type != ERFX_LES_NONE || c.use_terrain) {
+                set_error("inflow with an LES model or terrain is not supported"); return ERFX_ERR_UNSUPPORTED;
             }
         }
         if ((c.bc[o] == ERFX_BC_PERIODIC) != (c.bc[o % 3 == o ? o + 3 : o - 3] == ERFX_BC_PERIODIC)) {
@@ -129,6 +132,10 @@ static int validate(const erfx_config& c) {
     if ((c.rayleigh_damp_U || c.rayleigh_damp_V || c.rayleigh_damp_W || c.rayleigh_damp_T) && !(c.rayleigh_zdamp > 0)) {
         set_error("Rayleigh damping needs rayleigh_zdamp > 0"); return ERFX_ERR_ARG;
     }
+    if (c.no_substepping && c.use_terrain) {
+        set_error("substepping_type None on a terrain grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
+    }
+    if (c.no_substepping) return ERFX_OK;   // the ratio is not used (ERF_MRI.H:205-207)
     if (!(c.mri_ratio > 1 && c.mri_ratio % 2 == 0)) { set_error("mri_ratio must be even and > 1 (ERF_MRI.H:205-207)"); return ERFX_ERR_ARG; }
     if (!c.force_stage1_single_substep && c.mri_ratio % 3 != 0) {
         set_error("without the single first-stage substep the ratio must be a multiple of 6"); return ERFX_ERR_ARG;
@@ -478,7 +485,8 @@ int erfx_set_rayleigh(erfx_ctx* c, const double* ub, const double* vb, const dou
         h[RAY_THETABAR * n1 + k] = tb[k];
     }
     const double zlo = cf.prob_lo[2], dz = cf.dz;
-    const double ztop = zlo + cf.nz * dz;
+    // ERF's rayleigh_ztop = geom.ProbHi(2) (ERF_init_rayleigh.cpp:17)
+    const double ztop = cf.prob_hi[2] > cf.prob_lo[2] ? cf.prob_hi[2] : zlo + cf.nz * dz;
     const double zdamp = cf.rayleigh_zdamp, dampcoef = cf.rayleigh_dampcoef;
     for (int k = 0; k < n1; ++k) {
         double zcc = zlo + (k + 0.5) * dz;
@@ -985,9 +993,10 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
         cl.add(Sn.ymom, So.ymom, (long)ne); cl.add(Sn.zmom, So.zmom, (long)ne);
         copy_fields(cl, st);
     }
-    const double sub_timestep = dt / ratio;
+    const bool nosub = P.cfg.no_substepping != 0;
+    const double sub_timestep = nosub ? 0.0 : dt / ratio;
     // the substep chain folds apply_bcs into erf_fast_rhs_N's kernels (flat grids only)
-    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !P.terrain;
+    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !P.terrain && !nosub;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
     double sdt[3];
     stage_dts(time, dt, sdt);
@@ -995,18 +1004,27 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
         int nsub = 1;
         double dtau = 0;
         if (nrk == 0) {
-            if (P.cfg.force_stage1_single_substep) { nsub = 1; dtau = dt / 3.0; }
+            if (P.cfg.force_stage1_single_substep || nosub) { nsub = 1; dtau = dt / 3.0; }
             else { nsub = ratio / 3; dtau = sub_timestep; }
         } else if (nrk == 1) {
-            nsub = ratio / 2; dtau = sub_timestep;
+            if (nosub) { nsub = 1; dtau = 0.5 * dt; }
+            else { nsub = ratio / 2; dtau = sub_timestep; }
         } else {
-            nsub = ratio; dtau = sub_timestep;
+            if (nosub) { nsub = 1; dtau = dt; }
+            else { nsub = ratio; dtau = sub_timestep; }
         }
         if (nrk > 0) cons_to_prim(P, Sn.cons, s->prim, s->pi_stage, P.ng_cons, st);
         const double slow_dt = sdt[nrk];
         slow_rhs_pre(P, c->sw, nrk, slow_dt, Fs, Sn, s->prim, Sc, un, vn, wn, s->eddy, s->r0, s->p0, st);
         const double inv_fac = 1.0 / static_cast<double>(nsub);
-        for (int ks = 0; ks < nsub; ++ks) {
+        if (nosub) {
+            // no_substep_fun (ERF_TI_no_substep_fun.H:1-146): the fast
+            // variables from the slow RHS alone, then apply_bcs on S_sum with
+            // its full ghost widths (fast only)
+            no_substep(P, Ss, So, Fs, nsub * dtau, st);
+            apply_bcs(P, Ss, un, vn, wn, P.ng_cons, P.ng_vel, true, false, st);
+        }
+        for (int ks = 0; ks < (nosub ? 0 : nsub); ++ks) {
             // On a laterally periodic box the apply_bcs between two substeps is
             // folded into the next substep (round trip of the momenta on read,
             // ghost layer 1 of rho and rho*theta written by the column kernel);
@@ -1139,7 +1157,8 @@ int erfx_sim_autotune(erfx_sim* s, int on) {
     if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
     ERFX_HIP(hipStreamSynchronize(s->stream));
     tuner_free(s);
-    if (!on) return ERFX_OK;
+    // the terrain column kernel (erf_fast_rhs_T) has one form: nothing to time
+    if (!on || s->ctx->P.terrain) return ERFX_OK;
     s->tuner = new ColTuner();
     for (hipEvent_t& e : s->tuner->ev) {
         hipError_t r = hipEventCreate(&e);
@@ -1201,6 +1220,7 @@ int erfx_sim_est_dt(erfx_sim* s, const erfx_dt_control* d, double* dt, int* mri_
         if (ratio % 6 != 0) ratio = static_cast<int>(std::ceil(ratio / 6.0) * 6);
     }
     *mri_ratio = d->fixed_mri_ratio > 0 ? d->fixed_mri_ratio : (int)ratio;
+    if (P.cfg.no_substepping) *mri_ratio = 0;   // no ratio without substeps (:200-223)
     *dt = d->fixed_dt > 0.0 ? d->fixed_dt : estdt_comp;
     return launch_status();
 }
@@ -1220,7 +1240,9 @@ int erfx_sim_run(erfx_sim* s, double t0, int max_steps, const erfx_dt_control* d
         int ratio;
         int rc = erfx_sim_est_dt(s, d, &dt_tmp, &ratio);
         if (rc) return rc;
-        if (ratio < 2 || ratio % 2 != 0) { set_error("MRI ratio must be even and >= 2"); return ERFX_ERR_ARG; }
+        if (!s->ctx->P.cfg.no_substepping && (ratio < 2 || ratio % 2 != 0)) {
+            set_error("MRI ratio must be even and >= 2"); return ERFX_ERR_ARG;
+        }
         double dt_0 = dt_tmp;
         dt_tmp = std::min(dt_tmp, d->change_max * dt_prev);
         dt_0 = std::min(dt_0, 1 * dt_tmp);
@@ -1273,6 +1295,13 @@ int erfx_sim_timer_reset(erfx_sim* s) {
     }
     t.kernel = KT_FAST_COLUMN;
     s->ctx->P.timer = &t;
+    return ERFX_OK;
+}
+
+int erfx_sim_timer_stop(erfx_sim* s) {
+    if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    s->ctx->P.timer = nullptr;
     return ERFX_OK;
 }
 
@@ -1621,12 +1650,29 @@ int erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps) {
         }
         if (s->mlp || s->samples) { set_error("not batchable: learned microphysics or row sampling on a member"); return ERFX_ERR_UNSUPPORTED; }
         ERFX_HIP(hipStreamSynchronize(s->stream));   // members' own uploads and steps done
+        // a flag latched by the member's own earlier steps belongs to those steps
+        int rc = kessler_status(s->ctx);
+        if (rc) return rc;
     }
     for (int n = 0; n < nsteps; ++n) {
         int rc = ens_one_step(e, t0 + n * dt, dt);
         if (rc) { (void)ens_sync(e); return rc; }
     }
-    return ens_sync(e);
+    int rc = ens_sync(e);
+    if (rc) return rc;
+    // a member whose temperature left the saturation fit's range (ERF aborts
+    // there): report it as this call's error, naming the member, and clear
+    // every member's flag so no later call of a member reports it again
+    int bad = -1;
+    for (size_t m = 0; m < e->m.size(); ++m) {
+        if (kessler_status(e->m[m]->ctx) != ERFX_OK && bad < 0) bad = (int)m;
+    }
+    if (bad >= 0) {
+        set_error("ensemble member " + std::to_string(bad) +
+                  ": Kessler temperature below -85 C, outside the saturation fit (ERF_Microphysics_Utils.H:62)");
+        return ERFX_ERR_STATE;
+    }
+    return ERFX_OK;
 }
 
 int erfx_ensemble_set_graph(erfx_ensemble* e, int on) {
@@ -1637,7 +1683,9 @@ int erfx_ensemble_set_graph(erfx_ensemble* e, int on) {
 
 int erfx_ensemble_autotune(erfx_ensemble* e, int on) {
     if (!e) { set_error("null argument"); return ERFX_ERR_ARG; }
-    e->tune_next = on ? 0 : -1;
+    bool terrain = false;
+    for (erfx_sim* s : e->m) terrain = terrain || s->ctx->P.terrain;
+    e->tune_next = (on && !terrain) ? 0 : -1;
     for (float& v : e->tune_ms) v = 0.f;
     return ERFX_OK;
 }

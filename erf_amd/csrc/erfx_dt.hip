@@ -80,7 +80,7 @@ void est_dt_maxima(const Params& P, const double* cons, const double* u, const d
                    double* partial, double* host, hipStream_t s) {
     if (rec_refuse("time-step estimate")) return;
     hipLaunchKernelGGL(est_dt_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, u, v, w, P.dxInv[0], P.dxInv[1],
-                       1.0 / P.dz_min, P.cfg.implicit_substepping != 0, partial);   // ERF_ComputeTimestep.cpp:66
+                       1.0 / P.dz_min, P.cfg.implicit_substepping != 0 && !P.cfg.no_substepping, partial);   // ERF_ComputeTimestep.cpp:66
     hipLaunchKernelGGL(est_dt_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
                        partial + 2 * EST_BLOCKS);
     (void)hipMemcpyAsync(host, partial + 2 * EST_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, s);

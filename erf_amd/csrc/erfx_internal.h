@@ -61,6 +61,10 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 const double* fc, const erfx_state& data, const erfx_state& scratch, double dtau,
                 double beta_s, double facinv, hipStream_t s, const FastChain& ch = FastChain());
 bool fast_bcs_fusable(const Params& P);
+// no_substep_fun (substepping_type None): S_sum = S_old + slow_dt * F_slow for
+// rho, rho*theta and the momenta
+void no_substep(const Params& P, const erfx_state& ssum, const erfx_state& sold, const erfx_state& fslow,
+                double slow_dt, hipStream_t s);
 // erf_fast_rhs_T on a static terrain grid (erfx_terrain.hip): increments of the
 // previous substep and the updated horizontal momenta on faces grown by one
 struct TerrainWork {

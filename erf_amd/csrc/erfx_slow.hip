@@ -468,7 +468,9 @@ struct TAdv : TAdvData {
 
 // ---- strain (ComputeStrain_N, Source/Diffusion/ERF_ComputeStrain_N.cpp:28-229) ----
 struct StrainBC {
-    bool zl_u, zh_u, zl_v, zh_v;   // only the z walls can be Dirichlet for tangential velocity here
+    bool zl_u, zh_u, zl_v, zh_v;   // Dirichlet tangential velocity on the z walls
+    // Dirichlet tangential velocity on lateral faces (Inflow)
+    bool xl_v, xh_v, xl_w, xh_w, yl_u, yh_u, yl_w, yh_w;
 };
 
 __device__ __forceinline__ double strain13(const double* u, const double* w, long q, long sz, double dxi, double dzi,
@@ -497,7 +499,57 @@ static StrainBC strain_bc(const Params& P) {
     b.zh_u = P.bck[0][5] == EXT_DIR;
     b.zl_v = P.bck[1][2] == EXT_DIR;
     b.zh_v = P.bck[1][5] == EXT_DIR;
+    b.xl_v = P.bck[1][0] == EXT_DIR;
+    b.xh_v = P.bck[1][3] == EXT_DIR;
+    b.xl_w = P.bck[2][0] == EXT_DIR;
+    b.xh_w = P.bck[2][3] == EXT_DIR;
+    b.yl_u = P.bck[0][1] == EXT_DIR;
+    b.yh_u = P.bck[0][4] == EXT_DIR;
+    b.yl_w = P.bck[2][1] == EXT_DIR;
+    b.yh_w = P.bck[2][4] == EXT_DIR;
     return b;
+}
+
+// The edge strains of the valid nodal boxes with ComputeStrain_N's one-sided
+// stencils (ERF_ComputeStrain_N.cpp:67-205): on a face where the velocity
+// across the edge is Dirichlet the ghost holds the face value.  The x faces
+// are treated first, then y, then z, so an edge on two such faces takes the
+// stencil of the first (the reference shrinks the box after each face).
+__device__ __forceinline__ double strain12_bc(const StrainBC& b, const double* u, const double* v, long q, long sy,
+                                              double dxi, double dyi, int i, int j, int nx, int ny) {
+    if (b.xl_v && i == 0)
+        return 0.5 * ((u[q] / 1.0 - u[q - sy] / 1.0) * dyi +
+                      (-(8. / 3.) * v[q - 1] / 1.0 + 3. * v[q] / 1.0 - (1. / 3.) * v[q + 1] / 1.0) * dxi) * 1.0 * 1.0;
+    if (b.xh_v && i == nx)
+        return 0.5 * ((u[q] / 1.0 - u[q - sy] / 1.0) * dyi +
+                      -(-(8. / 3.) * v[q] / 1.0 + 3. * v[q - 1] / 1.0 - (1. / 3.) * v[q - 2] / 1.0) * dxi) * 1.0 * 1.0;
+    if (b.yl_u && j == 0)
+        return 0.5 * ((-(8. / 3.) * u[q - sy] / 1.0 + 3. * u[q] / 1.0 - (1. / 3.) * u[q + sy] / 1.0) * dyi +
+                      (v[q] / 1.0 - v[q - 1] / 1.0) * dxi) * 1.0 * 1.0;
+    if (b.yh_u && j == ny)
+        return 0.5 * (-(-(8. / 3.) * u[q] / 1.0 + 3. * u[q - sy] / 1.0 - (1. / 3.) * u[q - 2 * sy] / 1.0) * dyi +
+                      (v[q] / 1.0 - v[q - 1] / 1.0) * dxi) * 1.0 * 1.0;
+    return strain12(u, v, q, sy, dxi, dyi);
+}
+__device__ __forceinline__ double strain13_bc(const StrainBC& b, const double* u, const double* w, long q, long sz,
+                                              double dxi, double dzi, int i, int k, int nx, int nz) {
+    if (b.xl_w && i == 0)
+        return 0.5 * ((u[q] - u[q - sz]) * dzi +
+                      (-(8. / 3.) * w[q - 1] + 3. * w[q] - (1. / 3.) * w[q + 1]) * dxi * 1.0);
+    if (b.xh_w && i == nx)
+        return 0.5 * ((u[q] - u[q - sz]) * dzi +
+                      -(-(8. / 3.) * w[q] + 3. * w[q - 1] - (1. / 3.) * w[q - 2]) * dxi * 1.0);
+    return strain13(u, w, q, sz, dxi, dzi, b.zl_u && k == 0, b.zh_u && k == nz);
+}
+__device__ __forceinline__ double strain23_bc(const StrainBC& b, const double* v, const double* w, long q, long sy,
+                                              long sz, double dyi, double dzi, int j, int k, int ny, int nz) {
+    if (b.yl_w && j == 0)
+        return 0.5 * ((v[q] - v[q - sz]) * dzi +
+                      (-(8. / 3.) * w[q - sy] + 3. * w[q] - (1. / 3.) * w[q + sy]) * dyi * 1.0 * 1.0);
+    if (b.yh_w && j == ny)
+        return 0.5 * ((v[q] - v[q - sz]) * dzi +
+                      -(-(8. / 3.) * w[q] + 3. * w[q - sy] - (1. / 3.) * w[q - 2 * sy]) * dyi * 1.0 * 1.0);
+    return strain23(v, w, q, sy, sz, dyi, dzi, b.zl_v && k == 0, b.zh_v && k == nz);
 }
 
 static void check_lateral_strain(const Params& P) {
@@ -626,7 +678,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
         if (i < 0 || j < 0) return;
         // edges on nodal valid boxes
         if (k <= nz - 1) {                                   // xy edges i in [0,nx], j in [0,ny]
-            double s12 = strain12(u, v, q, sy, dxi, dyi);
+            double s12 = strain12_bc(sb, u, v, q, sy, dxi, dyi, i, j, nx, ny);
             double mu;
             double rho_bar = consA ? 0.25 * (cons[q - 1] + cons[q] + cons[q - 1 - sy] + cons[q - sy]) : 0.0;
             if (turb) {
@@ -638,7 +690,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
             }
         }
         if (j <= ny - 1) {                                   // xz edges i in [0,nx], k in [0,nz]
-            double s13 = strain13(u, w, q, sz, dxi, dzi, sb.zl_u && k == 0, sb.zh_u && k == nz);
+            double s13 = strain13_bc(sb, u, w, q, sz, dxi, dzi, i, k, nx, nz);
             double rho_bar = consA ? 0.25 * (cons[q - 1] + cons[q] + cons[q - 1 - sz] + cons[q - sz]) : 0.0;
             if (turb) {
                 double mu_bar = 0.25 * (mv[q - 1] + mv[q] + mv[q - 1 - sz] + mv[q - sz]);
@@ -649,7 +701,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
             }
         }
         if (i <= nx - 1) {                                   // yz edges j in [0,ny], k in [0,nz]
-            double s23 = strain23(v, w, q, sy, sz, dyi, dzi, sb.zl_v && k == 0, sb.zh_v && k == nz);
+            double s23 = strain23_bc(sb, v, w, q, sy, sz, dyi, dzi, j, k, ny, nz);
             double rho_bar = consA ? 0.25 * (cons[q - sy] + cons[q] + cons[q - sy - sz] + cons[q - sz]) : 0.0;
             if (turb) {
                 double mu_bar = 0.25 * (mv[q - sy] + mv[q] + mv[q - sy - sz] + mv[q - sz]);
@@ -675,6 +727,25 @@ __device__ __forceinline__ double diff_flux(const double* cons, const double* cp
     }
     if (turb) rhoAlpha += 0.5 * (mu[q] + mu[q - st]);
     return -rhoAlpha * (cp[q] - cp[q - st]) * dinv;
+}
+
+// The same flux on a face where the scalar is Dirichlet (an Inflow face): the
+// one-sided stencils through the face value held in the ghost cell
+// (ERF_DiffusionSrcForState_N.cpp:208-227), side -1 on the domain's lo face,
+// +1 on its hi face, 0 elsewhere.
+__device__ __forceinline__ double diff_flux_bc(const double* cons, const double* cp, const double* mu, long q,
+                                               long st, double dinv, double ae, bool consA, bool turb, int side) {
+    if (side == 0) return diff_flux(cons, cp, mu, q, st, dinv, ae, consA, turb);
+    double rhoAlpha;
+    if (consA) {
+        double rhoFace = 0.5 * (cons[q] + cons[q - st]);
+        rhoAlpha = rhoFace * ae;
+    } else {
+        rhoAlpha = ae;
+    }
+    if (turb) rhoAlpha += 0.5 * (mu[q] + mu[q - st]);
+    if (side < 0) return -rhoAlpha * (-(8. / 3.) * cp[q - st] + 3. * cp[q] - (1. / 3.) * cp[q + st]) * dinv;
+    return -rhoAlpha * ((8. / 3.) * cp[q] - 3. * cp[q - st] + (1. / 3.) * cp[q - 2 * st]) * dinv;
 }
 
 // Departure of cell q (level k) from the level's plane averages in the moist
@@ -742,6 +813,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const bool c2c2 = (ht == ERFX_ADV_CENTERED2 && vt == ERFX_ADV_CENTERED2);
     const bool mixed_rt = P.cfg.precision == ERFX_PREC_MIXED;
     double ae_theta = consA ? P.cfg.alpha_T : P.cfg.rho0_trans * P.cfg.alpha_T;
+    // Dirichlet theta faces (Inflow): one-sided diffusive fluxes there
+    const bool dxlo = P.bck[3][0] == EXT_DIR, dxhi = P.bck[3][3] == EXT_DIR;
+    const bool dylo = P.bck[3][1] == EXT_DIR, dyhi = P.bck[3][4] == EXT_DIR;
     const double* mu_th_h = eddy ? eddy + 1 * ne : nullptr;
     const double* mu_th_v = eddy ? eddy + 7 * ne : nullptr;
     const bool sflux = P.cfg.use_surface_heat_flux != 0;
@@ -842,10 +916,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             double r1 = -1. * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
             if (use_diff) {
                 const double* c = data.cons;
-                double dfx0 = diff_flux(c, prim, mu_th_h, q, 1, dxi, ae_theta, consA, turb) * 1.0;
-                double dfx1 = diff_flux(c, prim, mu_th_h, q + 1, 1, dxi, ae_theta, consA, turb) * 1.0;
-                double dfy0 = diff_flux(c, prim, mu_th_h, q, sy, dyi, ae_theta, consA, turb) * 1.0;
-                double dfy1 = diff_flux(c, prim, mu_th_h, q + sy, sy, dyi, ae_theta, consA, turb) * 1.0;
+                const int dsx0 = (dxlo && i == 0) ? -1 : 0, dsx1 = (dxhi && i == nx - 1) ? 1 : 0;
+                const int dsy0 = (dylo && j == 0) ? -1 : 0, dsy1 = (dyhi && j == ny - 1) ? 1 : 0;
+                double dfx0 = diff_flux_bc(c, prim, mu_th_h, q, 1, dxi, ae_theta, consA, turb, dsx0) * 1.0;
+                double dfx1 = diff_flux_bc(c, prim, mu_th_h, q + 1, 1, dxi, ae_theta, consA, turb, dsx1) * 1.0;
+                double dfy0 = diff_flux_bc(c, prim, mu_th_h, q, sy, dyi, ae_theta, consA, turb, dsy0) * 1.0;
+                double dfy1 = diff_flux_bc(c, prim, mu_th_h, q + sy, sy, dyi, ae_theta, consA, turb, dsy1) * 1.0;
                 double dfz0 = (sflux && k == 0) ? sflux_val : diff_flux(c, prim, mu_th_v, q, sz, dzi, ae_theta, consA, turb);
                 double dfz1 = diff_flux(c, prim, mu_th_v, q + sz, sz, dzi, ae_theta, consA, turb);
                 r1 -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
@@ -898,6 +974,9 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const int btype = moist ? P.cfg.buoyancy_type : 1;
     const int nqs = P.n_qstate;
     const double* pavg = W.pavg;
+    // the PressureGradient driver (ERF_slow_rhs_pre.cpp:592, 630, 679, 739)
+    const double apg0 = P.cfg.abl_pressure_grad[0], apg1 = P.cfg.abl_pressure_grad[1];
+    const double apg2 = P.cfg.abl_pressure_grad[2];
     // momentum face values stay FP64 in both forms: this kernel streams at
     // near the HBM peak, and FP32 face values only add registers to it.
     // Its stencils reach three levels up and down: tiles ordered k fastest.
@@ -965,7 +1044,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             }
             gpx *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - 1) : 0.0;
-            r += (-gpx - 0.0) / (1.0 + qf) + xs;
+            r += (-gpx - apg0) / (1.0 + qf) + xs;
             if ((zero_xlo && i == 0) || (zero_xhi && i == nx)) r = 0.;
             rur[q] = r;
         }
@@ -1028,7 +1107,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             }
             gpy *= 1.0;
             double qf = moist ? face_q(pq1, pq2, q, q - sy) : 0.0;
-            r += (-gpy - 0.0) / (1.0 + qf) + ys;
+            r += (-gpy - apg1) / (1.0 + qf) + ys;
             if ((zero_ylo && j == 0) || (zero_yhi && j == ny)) r = 0.;
             rvr[q] = r;
         }
@@ -1133,7 +1212,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             double met_h_zeta = terr ? met::h_zeta_Kface(zt, i, j, k, dzi) : 1;
             double gpz = dzi * (pp[q] - pp[q - sz]) / met_h_zeta;
             double qf = moist ? face_q(pq1, pq2, q, q - sz) : 0.0;
-            r += (zs - gpz - 0.0) / (1.0 + qf);
+            r += (zs - gpz - apg2) / (1.0 + qf);
             rwr[q] = r;
         }
     });
@@ -1166,6 +1245,9 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     const bool mixed_rt = P.cfg.precision == ERFX_PREC_MIXED;
     const bool use_diff = P.use_diff, consA = P.consA, turb = P.use_turb;
     const double ae = consA ? P.cfg.alpha_C : P.cfg.rho0_trans * P.cfg.alpha_C;
+    // Dirichlet scalar faces (Inflow): one-sided diffusive fluxes there
+    const bool dxlo = P.bck[3][0] == EXT_DIR, dxhi = P.bck[3][3] == EXT_DIR;
+    const bool dylo = P.bck[3][1] == EXT_DIR, dyhi = P.bck[3][4] == EXT_DIR;
     const double* mu_h = eddy ? eddy + B.mu_h * ne : nullptr;
     const double* mu_v = eddy ? eddy + B.mu_v * ne : nullptr;
     const int n0 = B.n0, nc_b = B.ncomp;
@@ -1205,10 +1287,12 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
             double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
             double r = -invdetJ * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
             if (use_diff) {
-                double dfx0 = diff_flux(nc, ps, mu_h, q, 1, dxi, ae, consA, turb) * 1.0;
-                double dfx1 = diff_flux(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb) * 1.0;
-                double dfy0 = diff_flux(nc, ps, mu_h, q, sy, dyi, ae, consA, turb) * 1.0;
-                double dfy1 = diff_flux(nc, ps, mu_h, q + sy, sy, dyi, ae, consA, turb) * 1.0;
+                const int dsx0 = (dxlo && i == 0) ? -1 : 0, dsx1 = (dxhi && i == nx - 1) ? 1 : 0;
+                const int dsy0 = (dylo && j == 0) ? -1 : 0, dsy1 = (dyhi && j == ny - 1) ? 1 : 0;
+                double dfx0 = diff_flux_bc(nc, ps, mu_h, q, 1, dxi, ae, consA, turb, dsx0) * 1.0;
+                double dfx1 = diff_flux_bc(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb, dsx1) * 1.0;
+                double dfy0 = diff_flux_bc(nc, ps, mu_h, q, sy, dyi, ae, consA, turb, dsy0) * 1.0;
+                double dfy1 = diff_flux_bc(nc, ps, mu_h, q + sy, sy, dyi, ae, consA, turb, dsy1) * 1.0;
                 double dfz0 = diff_flux(nc, ps, mu_v, q, sz, dzi, ae, consA, turb);
                 double dfz1 = diff_flux(nc, ps, mu_v, q + sz, sz, dzi, ae, consA, turb);
                 r -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;

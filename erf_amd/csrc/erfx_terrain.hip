@@ -186,6 +186,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     const double* fQ = fc + 4 * ne;
     double* cz = data.zmom;
     double* azm = scratch.zmom;
+    kt_start(P.timer, KT_FAST_COLUMN, s);
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // Omega of the old increments on face kf (0 at the bottom, the rho*w
@@ -302,6 +303,12 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             p_hi = pq;
         }
     });
+    // algorithmic bytes per cell: the fields of the substep this kernel reads
+    // (stage rho, rho*theta, rho*u, rho*v, rho*w; the previous rho*u, rho*v,
+    // rho*w; the new rho*u, rho*v; the current rho, rho*theta; the slow RHS of
+    // rho, rho*theta, rho*w; the averaged rho*w; z_nd, detJ; qv, qc when moist)
+    // and writes (rho*w, averaged rho*w, rho, rho*theta), 8 B each
+    kt_stop(P.timer, KT_FAST_COLUMN, s, (moist ? 24.0 : 22.0) * 8.0 * nx * ny * nz, "fast_rhs_T column");
 }
 
 }  // namespace erfx

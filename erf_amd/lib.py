@@ -84,6 +84,7 @@ SIGNATURES = {
     "erfx_sim_run": (_I, [_P, _D, _I, ctypes.POINTER(C.DtControl), ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_integrals": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "erfx_sim_timer_reset": (_I, [_P]),
+    "erfx_sim_timer_stop": (_I, [_P]),
     "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(_D)]),
     "erfx_ensemble_create": (_I, [_P, _I, ctypes.POINTER(_P)]),

@@ -178,6 +178,28 @@ typedef struct erfx_config {
     double inflow_scalar[6];
     double inflow_qv[6];
     double inflow_qc[6];
+
+    /* upper corner of the domain [m] (geometry.prob_hi).  The Rayleigh layer's
+     * ztop is prob_hi[2] (ERF_init_rayleigh.cpp:17); when prob_hi[2] <=
+     * prob_lo[2] (e.g. all zero) the top is taken as prob_lo[2] + nz * dz. */
+    double prob_hi[3];
+
+    /* substepping_type = None: each RK stage advances the fast variables with
+     * the slow right-hand side alone, S_sum = S_old + dt_stage * F_slow, and
+     * no acoustic substeps (ERF_MRI.H:255-320, ERF_TI_no_substep_fun.H:1-146);
+     * mri_ratio is then not used. */
+    int    no_substepping;
+
+    /* velocity of a NoSlipWall face (the <face>.velocity input,
+     * ERF_init_bcs.cpp:184-200): the ext_dir values of the three velocity
+     * components on that face; the normal component is forced to 0 as ERF
+     * does.  Per orientation (xlo, ylo, zlo, xhi, yhi, zhi). */
+    double wall_velocity[6][3];
+
+    /* constant driving pressure gradient of the "PressureGradient" ABL driver
+     * (erf.abl_pressure_grad, ERF_DataStruct.H:268-270), subtracted from the
+     * momentum right-hand sides (ERF_slow_rhs_pre.cpp:630, 679, 739) */
+    double abl_pressure_grad[3];
 } erfx_config;
 
 enum { ERFX_PREC_FP64 = 0, ERFX_PREC_MIXED = 1 };
@@ -418,7 +440,9 @@ int  erfx_sim_set_graph(erfx_sim* sim, int on);
  * with `on`, the next 27 launches of the erf_fast_rhs_N column kernel cycle
  * through its 9 variants (levels of loads in flight 2/3/4 x work-group width
  * 64/128/256), each timed with HIP events, and the fastest is kept for the
- * rest of the run.  Every variant computes the same values. */
+ * rest of the run.  Every variant computes the same values.  On a terrain
+ * grid (use_terrain) the erf_fast_rhs_T column kernel has one form and the
+ * call does nothing (graph replay stays available). */
 int  erfx_sim_autotune(erfx_sim* sim, int on);
 /* The column variant in use (0..8; -1 if still timing) and, when ms is not
  * NULL, the best time of each variant in ms (9 values, 0 if not timed). */
@@ -475,6 +499,8 @@ int  erfx_sim_sample_count(erfx_sim* sim, long* count);
 int  erfx_sim_timer_reset(erfx_sim* sim);
 int  erfx_sim_timer_read(erfx_sim* sim, char* name, int name_len, double* total_ms, long* launches,
                          double* bytes_per_launch);
+/* Stop the per-launch timing; the readings stay available to timer_read. */
+int  erfx_sim_timer_stop(erfx_sim* sim);
 
 /* ---- batched ensembles (no counterpart in the reference) --------------------
  * ERF advances each box of a level with its own kernel launches (one
@@ -504,7 +530,8 @@ int  erfx_ensemble_set_streams(erfx_ensemble* e, int nstreams);
 int  erfx_ensemble_set_graph(erfx_ensemble* e, int on);
 /* With `on`, the next 9 steps each use one variant of the acoustic column
  * kernel for all members, timed with HIP events; the fastest is kept.
- * Every variant computes the same values. */
+ * Every variant computes the same values.  Does nothing when a member has a
+ * terrain grid (the erf_fast_rhs_T column kernel has one form). */
 int  erfx_ensemble_autotune(erfx_ensemble* e, int on);
 int  erfx_ensemble_set_column_variant(erfx_ensemble* e, int variant);
 /* the variant in use (-1 while timing) and, when ms is not NULL, the step

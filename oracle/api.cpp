@@ -110,6 +110,17 @@ int orc_init_taylor_green(void* h, double rho_0, double V_0, double M_0) {
     return guard([&] { L.init_taylor_green(rho_0, V_0, M_0); });
 }
 
+int orc_init_couette_poiseuille(void* h, double rho_0, double T_0, double u_0, double v_0, double w_0, int prob_type,
+                                int uniform) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.init_couette_poiseuille(rho_0, T_0, u_0, v_0, w_0, prob_type, uniform != 0); });
+}
+
+int orc_init_ekman(void* h, double rho_0, double T_0, double u_g, double rot_period, double mu) {
+    Level& L = *static_cast<Level*>(h);
+    return guard([&] { L.init_ekman(rho_0, T_0, u_g, rot_period, mu); });
+}
+
 // ERF::Evolve's time loop (Source/ERF.cpp:346-399) with ComputeDt
 // (ERF_ComputeTimestep.cpp:9-44) before every step: dt from estTimeStep,
 // limited to change_max times the previous dt, init_shrink on the first step,

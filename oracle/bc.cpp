@@ -32,7 +32,12 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
                 case ERFX_BC_PERIODIC:     k = INT_DIR; break;
                 case ERFX_BC_SYMMETRY:     k = (v == dir) ? REFLECT_ODD : REFLECT_EVEN; break;
                 case ERFX_BC_OUTFLOW:      k = (v == dir) ? NEUMANN_INT : FOEXTRAP; break;
-                case ERFX_BC_NO_SLIP_WALL: k = EXT_DIR; break;
+                // no-slip wall: ext_dir with the face's <face>.velocity, its
+                // normal component forced to 0 (ERF_init_bcs.cpp:184-200)
+                case ERFX_BC_NO_SLIP_WALL:
+                    k = EXT_DIR;
+                    vel_extdir[v][ori] = (v == dir) ? 0.0 : c.wall_velocity[ori][v];
+                    break;
                 case ERFX_BC_SLIP_WALL:    k = (v == dir) ? EXT_DIR : FOEXTRAP; break;
                 // inflow: every velocity component ext_dir with the face's
                 // values (ERF_init_bcs.cpp:117-120, 345-361)

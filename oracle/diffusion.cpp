@@ -346,6 +346,11 @@ void Level::diffusion_src_state(const Box& bx, int start_comp, int num_comp, Arr
     const int idx_h[8] = {Theta_h, KE_h, QKE_h, Scalar_h, Q_h, Q_h, Q_h, Q_h};
     const int idx_v[8] = {Theta_v, KE_v, QKE_v, Scalar_v, Q_v, Q_v, Q_v, Q_v};
     const double dx_inv = dxInv[0], dy_inv = dxInv[1], dz_inv = dxInv[2];
+    // Dirichlet (ext_dir) cons faces of an Inflow boundary: one-sided
+    // second-order stencils through the face value held in the ghost cell
+    // (:208-227 and the same test in the other three flux branches)
+    const bool xlo_dir = bck[3][0] == EXT_DIR, xhi_dir = bck[3][3] == EXT_DIR;
+    const bool ylo_dir = bck[3][1] == EXT_DIR, yhi_dir = bck[3][4] == EXT_DIR;
     Field fx, fy, fz;
     fx.alloc(L, 1); fy.alloc(L, 1); fz.alloc(L, 1);
     Arr xflux = fx.a(L), yflux = fy.a(L), zflux = fz.a(L);
@@ -360,7 +365,14 @@ void Level::diffusion_src_state(const Box& bx, int start_comp, int num_comp, Arr
                 rhoAlpha = rhoFace * ae;
             } else rhoAlpha = ae;
             if (turb) rhoAlpha += 0.5 * (mt(i, j, k, ih) + mt(i - 1, j, k, ih));
-            xflux(i, j, k) = -rhoAlpha * (cp(i, j, k, prim_index) - cp(i - 1, j, k, prim_index)) * dx_inv * 1.0;
+            if (xlo_dir && i == 0)
+                xflux(i, j, k) = -rhoAlpha * (-(8. / 3.) * cp(i - 1, j, k, prim_index) + 3. * cp(i, j, k, prim_index)
+                                              - (1. / 3.) * cp(i + 1, j, k, prim_index)) * dx_inv;
+            else if (xhi_dir && i == nx + 1)
+                xflux(i, j, k) = -rhoAlpha * ((8. / 3.) * cp(i, j, k, prim_index) - 3. * cp(i - 1, j, k, prim_index)
+                                              + (1. / 3.) * cp(i - 2, j, k, prim_index)) * dx_inv;
+            else
+                xflux(i, j, k) = -rhoAlpha * (cp(i, j, k, prim_index) - cp(i - 1, j, k, prim_index)) * dx_inv * 1.0;
         });
         loop(Box(0, 0, 0, nx, ny + 1, nz), [&](int i, int j, int k) {
             double rhoAlpha;
@@ -369,7 +381,14 @@ void Level::diffusion_src_state(const Box& bx, int start_comp, int num_comp, Arr
                 rhoAlpha = rhoFace * ae;
             } else rhoAlpha = ae;
             if (turb) rhoAlpha += 0.5 * (mt(i, j, k, ih) + mt(i, j - 1, k, ih));
-            yflux(i, j, k) = -rhoAlpha * (cp(i, j, k, prim_index) - cp(i, j - 1, k, prim_index)) * dy_inv * 1.0;
+            if (ylo_dir && j == 0)
+                yflux(i, j, k) = -rhoAlpha * (-(8. / 3.) * cp(i, j - 1, k, prim_index) + 3. * cp(i, j, k, prim_index)
+                                              - (1. / 3.) * cp(i, j + 1, k, prim_index)) * dy_inv;
+            else if (yhi_dir && j == ny + 1)
+                yflux(i, j, k) = -rhoAlpha * ((8. / 3.) * cp(i, j, k, prim_index) - 3. * cp(i, j - 1, k, prim_index)
+                                              + (1. / 3.) * cp(i, j - 2, k, prim_index)) * dy_inv;
+            else
+                yflux(i, j, k) = -rhoAlpha * (cp(i, j, k, prim_index) - cp(i, j - 1, k, prim_index)) * dy_inv * 1.0;
         });
         const bool most_lo = cfg.use_surface_heat_flux && qty == 1;
         loop(Box(0, 0, 0, nx, ny, nz + 1), [&](int i, int j, int k) {

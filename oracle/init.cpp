@@ -179,7 +179,12 @@ void Level::init_scalar_advdiff(const double* pr) {
                 if (prob_type == 10) sc = A_0 * std::exp(-10. * r3d * r3d) + B_0 * std::sin(x);
                 else if (prob_type == 11) sc = A_0 * 0.25 * (1.0 + std::cos(PI * std::min(r2d_xy, r0) / r0));
                 else if (prob_type == 12) sc = A_0 * 0.25 * (1.0 + std::cos(PI * std::min(r2d_xz, r0) / r0));
-                else throw std::runtime_error("unsupported prob_type");
+                else if (prob_type == 13) {
+                    const double r0_z = rad_0 * (hi[2] - lo[2]);
+                    const double r2d_xz_ell = std::sqrt((x - xc) * (x - xc) / (r0 * r0) + (z - zc) * (z - zc) / (r0_z * r0_z));
+                    sc = A_0 * 0.25 * (1.0 + std::cos(PI * std::min(r2d_xz_ell, r0_z) / r0_z));
+                } else if (prob_type == 14) sc = std::cos(PI * x);
+                else sc = (r3d < r0) ? A_0 : 0.0;   // a ball of radius r0 (any other prob_type)
                 sc *= rho_0;
                 c(i, j, k, ERFX_RHOSCALAR) += sc;
             }
@@ -202,6 +207,81 @@ void Level::init_scalar_advdiff(const double* pr) {
     for (int k = 0; k <= nz; ++k)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) w(i, j, k) = w_0;
+}
+
+// The background state of ERF::init_only (Source/ERF.cpp:1316-1338) on a
+// constant-density base state without gravity: init_type uniform sets rho_0
+// and rho_0 T_0 (init_uniform, ERF_prob_common.H:450-463), the default type
+// the base state itself (init_from_hse); cells grown by one; every other
+// field zero.
+static void constant_background(Level& Lv, double rho_0, double T_0, bool uniform) {
+    const erfx_layout& L = Lv.L;
+    constant_base_state(Lv, rho_0);
+    std::fill(Lv.cons.v.begin(), Lv.cons.v.end(), 0.0);
+    std::fill(Lv.xvel.v.begin(), Lv.xvel.v.end(), 0.0);
+    std::fill(Lv.yvel.v.begin(), Lv.yvel.v.end(), 0.0);
+    std::fill(Lv.zvel.v.begin(), Lv.zvel.v.end(), 0.0);
+    Arr c = Lv.A(Lv.cons), p0a = Lv.A(Lv.p0), r0a = Lv.A(Lv.r0);
+    for (int k = -1; k <= L.nz; ++k)
+        for (int j = -1; j <= L.ny; ++j)
+            for (int i = -1; i <= L.nx; ++i) {
+                c(i, j, k, 0) = uniform ? rho_0 : r0a(i, j, k);
+                c(i, j, k, 1) = uniform ? rho_0 * T_0 : getRhoThetagivenP(p0a(i, j, k));
+            }
+}
+
+// Exec/RegTests/Couette_Poiseuille (ERF_prob.cpp:29-140): prob_type 1 (Couette)
+// u = u_0 z / prob_hi_z, v = v_0 z / prob_hi_z with z = (k + 1/2) dz, w = w_0;
+// prob_type 10 / 11 (Poiseuille) u (resp. v) = 1 - z_h^2 with z_h the cell
+// height from prob_lo; on the valid face boxes (init_custom, ERF_init_custom.cpp:56-117).
+void Level::init_couette_poiseuille(double rho_0, double T_0, double u_0, double v_0, double w_0, int prob_type,
+                                    bool uniform) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    constant_background(*this, rho_0, T_0, uniform);
+    Arr u = A(xvel), v = A(yvel), w = A(zvel);
+    const double dz = cfg.dz, zlo = cfg.prob_lo[2], zhi = flat_ztop(cfg);
+    if (prob_type == 1) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i <= nx; ++i) u(i, j, k) = u_0 * ((k + 0.5) * dz) / zhi;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j <= ny; ++j)
+                for (int i = 0; i < nx; ++i) v(i, j, k) = v_0 * ((k + 0.5) * dz) / zhi;
+        for (int k = 0; k <= nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) w(i, j, k) = w_0;
+    } else if (prob_type == 10 || prob_type == 11) {
+        for (int k = 0; k < nz; ++k) {
+            const double z_h = zlo + (k + 0.5) * dz;
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i <= nx; ++i) u(i, j, k) = (prob_type == 10) ? 1.0 - z_h * z_h : 0.0;
+            for (int j = 0; j <= ny; ++j)
+                for (int i = 0; i < nx; ++i) v(i, j, k) = (prob_type == 11) ? 1.0 - z_h * z_h : 0.0;
+        }
+    } else {
+        throw std::runtime_error("Couette/Poiseuille prob_type must be 1, 10 or 11");
+    }
+}
+
+// Exec/RegTests/EkmanSpiral (ERF_prob.cpp:28-100), init_type uniform: the
+// Ekman-layer solution u = u_g (1 - e^{-az} cos(-az)), v = -u_g e^{-az}
+// sin(-az), a = 1 / sqrt(2 nu / f), nu = mu / rho_0, f = 4 pi / period,
+// z = (k + 1/2) dz.
+void Level::init_ekman(double rho_0, double T_0, double u_g, double rot_period, double mu) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    constant_background(*this, rho_0, T_0, true);
+    Arr u = A(xvel), v = A(yvel);
+    const double coriolis_factor = 4.0 * PI / rot_period;
+    const double Az = mu / rho_0;
+    const double DE = std::sqrt(2.0 * Az / coriolis_factor);
+    const double a = 1.0 / DE;
+    for (int k = 0; k < nz; ++k) {
+        const double z = (k + 0.5) * cfg.dz;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i <= nx; ++i) u(i, j, k) = u_g * (1.0 - std::exp(-a * z) * std::cos(-a * z));
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i < nx; ++i) v(i, j, k) = -u_g * std::exp(-a * z) * std::sin(-a * z);
+    }
 }
 
 // Exec/RegTests/TaylorGreenVortex (ERF_prob.cpp:22-100) with the default init

@@ -29,8 +29,9 @@ void Level::advance_dycore(double old_time, double dt) {
     cons_to_prim(So.cons, ng_cons);
 
     // MRISplitIntegrator::advance
-    const int ratio = cfg.mri_ratio;
-    if (!(ratio > 1 && ratio % 2 == 0)) throw std::runtime_error("mri ratio must be even and > 1");
+    const bool nosub = cfg.no_substepping != 0;
+    const int ratio = nosub ? 2 : cfg.mri_ratio;   // not used without substepping
+    if (!nosub && !(ratio > 1 && ratio % 2 == 0)) throw std::runtime_error("mri ratio must be even and > 1");
     copy_field(S_new.cons, S_old.cons); copy_field(S_new.xmom, S_old.xmom);
     copy_field(S_new.ymom, S_old.ymom); copy_field(S_new.zmom, S_old.zmom);
     const double timestep = dt;
@@ -41,12 +42,20 @@ void Level::advance_dycore(double old_time, double dt) {
         old_time_stage = time_stage;
         int nsubsteps = 1; double dtau = 0;
         if (nrk == 0) {
-            if (cfg.force_stage1_single_substep) { nsubsteps = 1; dtau = timestep / 3.0; }
+            if (cfg.force_stage1_single_substep || nosub) { nsubsteps = 1; dtau = timestep / 3.0; }
             else { nsubsteps = ratio / 3; dtau = sub_timestep; }
             time_stage = old_time + timestep / 3.0;
         }
-        if (nrk == 1) { nsubsteps = ratio / 2; dtau = sub_timestep; time_stage = old_time + timestep / 2.0; }
-        if (nrk == 2) { nsubsteps = ratio; dtau = sub_timestep; time_stage = old_time + timestep; }
+        if (nrk == 1) {
+            if (nosub) { nsubsteps = 1; dtau = 0.5 * timestep; }
+            else { nsubsteps = ratio / 2; dtau = sub_timestep; }
+            time_stage = old_time + timestep / 2.0;
+        }
+        if (nrk == 2) {
+            if (nosub) { nsubsteps = 1; dtau = timestep; }
+            else { nsubsteps = ratio; dtau = sub_timestep; }
+            time_stage = old_time + timestep;
+        }
         if (nrk > 0) cons_to_prim(Sn.cons, ng_cons);   // pre_update
         // slow_rhs_fun_pre: make_sources, make_mom_sources, erf_slow_rhs_pre
         double slow_dt = time_stage - old_time;
@@ -54,7 +63,14 @@ void Level::advance_dycore(double old_time, double dt) {
         make_mom_sources(Sn, A(prim));
         slow_rhs_pre(nrk, slow_dt, Fs, Sn, A(prim), Sc, xv, yv, zv);
         double inv_fac = 1.0 / static_cast<double>(nsubsteps);
-        for (int ks = 0; ks < nsubsteps; ++ks) {
+        if (nosub) {
+            // no_substep_fun (ERF_TI_no_substep_fun.H:1-146), fixed or no terrain:
+            // S_sum = S_old + dt_stage * F_slow for rho, rho*theta and the
+            // momenta, then apply_bcs with the full ghost widths (fast only)
+            no_substep(Ss, So, Fs, nsubsteps * dtau);
+            apply_bcs(Ss, xv, yv, zv, ng_cons, ng_vel, true, false);
+        }
+        for (int ks = 0; ks < (nosub ? 0 : nsubsteps); ++ks) {
             // ERF_TI_fast_rhs_fun.H:122-175: erf_fast_rhs_T on a static terrain grid
             if (ks == 0) make_fast_coeffs(A(fast_coeffs), Sn, A(prim), A(pi_stage), dtau, beta_s);
             StateA prev = (ks == 0) ? So : Ss;
@@ -69,6 +85,27 @@ void Level::advance_dycore(double old_time, double dt) {
         (void)old_time_stage;
     }
     (void)use_diff;
+}
+
+// no_substep_fun (Source/TimeIntegration/ERF_TI_no_substep_fun.H:88-126), fixed
+// or no terrain: the fast variables advanced with the slow right-hand side
+// alone, cons comps 0 and 1 on the valid cells and the momenta on the nodal
+// tile boxes.
+void Level::no_substep(StateA ssum, StateA sold, StateA fslow, double slow_dt) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
+        for (int n = 0; n < 2; ++n)
+            ssum.cons(i, j, k, n) = sold.cons(i, j, k, n) + slow_dt * (fslow.cons(i, j, k, n));
+    });
+    loop(Box(0, 0, 0, nx, ny - 1, nz - 1), [&](int i, int j, int k) {
+        ssum.xmom(i, j, k) = sold.xmom(i, j, k) + slow_dt * fslow.xmom(i, j, k);
+    });
+    loop(Box(0, 0, 0, nx - 1, ny, nz - 1), [&](int i, int j, int k) {
+        ssum.ymom(i, j, k) = sold.ymom(i, j, k) + slow_dt * fslow.ymom(i, j, k);
+    });
+    loop(Box(0, 0, 0, nx - 1, ny - 1, nz), [&](int i, int j, int k) {
+        ssum.zmom(i, j, k) = sold.zmom(i, j, k) + slow_dt * fslow.zmom(i, j, k);
+    });
 }
 
 // ERF::Advance, level 0, dycore and microphysics (Source/TimeIntegration/ERF_Advance.cpp:20-233):
@@ -100,7 +137,8 @@ namespace orc {
 double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, long& ratio) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     Arr c = A(cons), u = A(xvel), v = A(yvel), w = A(zvel);
-    const bool implicit = cfg.implicit_substepping != 0;
+    // l_implicit_substepping (:78): false for explicit substeps and for none
+    const bool implicit = cfg.implicit_substepping != 0 && !cfg.no_substepping;
     const double dzinv = 1.0 / dz_min;   // ERF_ComputeTimestep.cpp:66
     double comp_inv = -1.e100, lowM_inv = -1.e100;
     for (int k = 0; k < nz; ++k)
@@ -133,6 +171,7 @@ double Level::est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, l
     double estdt_comp = cfl / comp_inv;
     double estdt_lowM = 1.e20;
     if (lowM_inv > 0.0) estdt_lowM = cfl / lowM_inv;
+    if (cfg.no_substepping) return fixed_dt > 0.0 ? fixed_dt : estdt_comp;   // no ratio (:200-223)
     if (fixed_dt > 0. && fixed_fast_dt > 0.) ratio = static_cast<long>(fixed_dt / fixed_fast_dt);
     else if (fixed_dt > 0.) ratio = static_cast<long>(std::ceil((fixed_dt / estdt_comp)));
     else ratio = (lowM_inv > 0.0) ? static_cast<long>(std::ceil((estdt_lowM / estdt_comp))) : 1;

@@ -47,6 +47,13 @@ constexpr double alphaelq = 1.e-3;      // ERF_Constants.H:48
 constexpr double Rd_on_Rv = R_d / R_v;  // ERF_Constants.H:87
 
 // ---- EOS (Source/Utils/ERF_EOS.H) -------------------------------------------
+// height of the domain top on a flat grid: geom.ProbHi(2), the Rayleigh
+// layer's rayleigh_ztop (ERF_init_rayleigh.cpp:17); prob_lo + nz dz when the
+// configuration gives no upper corner
+inline double flat_ztop(const erfx_config& c) {
+    return c.prob_hi[2] > c.prob_lo[2] ? c.prob_hi[2] : c.prob_lo[2] + c.nz * c.dz;
+}
+
 inline double getPgivenRTh(double rhotheta, double qv = 0.) {            // ERF_EOS.H:89-93
     return p_0 * erfx_pow(R_d * rhotheta * (1.0 + (R_v / R_d) * qv) * ip_0, Gamma);
 }
@@ -259,6 +266,7 @@ struct Level {
     void fill_patch(Arr c, Arr u, Arr v, Arr w);
 
     void make_fast_coeffs(Arr fc, StateA stage, Arr prim_, Arr pi_st, double dtau, double beta_s);
+    void no_substep(StateA ssum, StateA sold, StateA fslow, double slow_dt);
     void fast_rhs_N(int step, int nrk, StateA slow_rhs, StateA prev, StateA stage, Arr prim_,
                     Arr pi_st, Arr fc, StateA data, StateA scratch, double dtau, double beta_s,
                     double facinv);
@@ -288,6 +296,9 @@ struct Level {
     void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
     void init_scalar_advdiff(const double* params);
     void init_taylor_green(double rho_0, double V_0, double M_0);
+    void init_couette_poiseuille(double rho_0, double T_0, double u_0, double v_0, double w_0, int prob_type,
+                                 bool uniform);
+    void init_ekman(double rho_0, double T_0, double u_g, double rot_period, double mu);
     // ERF::estTimeStep (ERF_ComputeTimestep.cpp:49-233): returns dt, sets the MRI ratio
     double est_time_step(double cfl, double fixed_dt, double fixed_fast_dt, long& ratio);
     // Rayleigh damping reference profiles (nz+1 values each), set by orc_set_rayleigh

@@ -91,7 +91,7 @@ void Level::make_mom_sources(StateA data, Arr prim_) {
         throw std::runtime_error("Rayleigh damping profiles not set");
     {
         const double zlo = cfg.prob_lo[2], dz = cfg.dz;
-        const double ztop = terrain ? A(z_nd)(0, 0, nz) : zlo + nz * dz;   // ERF_init_rayleigh.cpp:17 (flat top)
+        const double ztop = terrain ? A(z_nd)(0, 0, nz) : flat_ztop(cfg);   // ERF_init_rayleigh.cpp:17 (ProbHi(2))
         const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
         if (cfg.rayleigh_damp_U) {
             loop(tbx, [&](int i, int j, int k) {
@@ -154,7 +154,7 @@ void Level::make_sources(StateA data, Arr prim_) {
     if ((int)ray_thetabar.size() < nz + 1) throw std::runtime_error("Rayleigh damping profiles not set");
     Arr cs = A(cc_src), c = data.cons;
     const double zlo = cfg.prob_lo[2], dz = cfg.dz;
-    const double ztop = terrain ? A(z_nd)(0, 0, nz) : zlo + nz * dz;
+    const double ztop = terrain ? A(z_nd)(0, 0, nz) : flat_ztop(cfg);
     const double zdamp = cfg.rayleigh_zdamp, dampcoef = cfg.rayleigh_dampcoef;
     loop(Box(0, 0, 0, nx - 1, ny - 1, nz - 1), [&](int i, int j, int k) {
         double zcc = terrain ? A(z_cc)(i, j, k) : zlo + (k + 0.5) * dz;
@@ -280,7 +280,7 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
         });
     }
     Arr xs = A(xmom_src), ys = A(ymom_src), zs = A(zmom_src);
-    const double apg0 = 0.0, apg1 = 0.0, apg2 = 0.0;
+    const double apg0 = cfg.abl_pressure_grad[0], apg1 = cfg.abl_pressure_grad[1], apg2 = cfg.abl_pressure_grad[2];
     // q = qv + qc averaged to the face (:622-626, :671-675, :732-736)
     const int q1 = ERFX_RHOQ1 - 1, q2 = ERFX_RHOQ2 - 1;
     Arr cp = prim_;

@@ -51,7 +51,10 @@ def read_plotfile(path):
 
 
 SMALL = ["ScalarDiffusionSine", "ScalarDiffusionGaussian", "ScalarAdvectionUniformU", "ScalarAdvectionShearedU",
-         "TaylorGreenAdvecting", "TaylorGreenAdvectingDiffusing"]
+         "TaylorGreenAdvecting", "TaylorGreenAdvectingDiffusing"] + \
+    ["ScalarAdvDiff_" + v for v in ("order2", "order3", "order4", "order5", "order6", "weno3", "weno3z", "weno5",
+                                    "weno5z", "wenomzq3")] + \
+    ["CouetteFlow_x", "CouetteFlow_y", "PoiseuilleFlow_x", "PoiseuilleFlow_y", "EkmanSpiral", "RayleighDamping"]
 
 
 def save_small(ref, case):
@@ -65,8 +68,12 @@ def save_small(ref, case):
 
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    only = sys.argv[2:]
     for case in SMALL:
-        save_small(ref, case)
+        if not only or case in only:
+            save_small(ref, case)
+    if only:
+        return
     pf = read_plotfile(os.path.join(ref, "Tests", "ERFGoldFiles", "DensityCurrent"))
     d = pf["data"]
     assert np.all(np.isfinite(d))

@@ -10,7 +10,26 @@ initial state (oracle/init.cpp) and ERF's time loop with its dt control
   ScalarAdvectionShearedU                      U3 advection in a log-profile wind, CFL dt
   TaylorGreenAdvecting(Diffusing)              momentum advection (and Constant diffusion), fixed dt, CFL ratio
 
+  ScalarAdvDiff_order2..6, _weno3, _weno3z, _weno5, _weno5z, _wenomzq3
+                                               every scalar and momentum stencil (C2, U3, C4, U5, C6 and
+                                               the WENO family) with an Inflow / Outflow pair and
+                                               Constant scalar diffusion, CFL dt
+  CouetteFlow_y                                no-slip walls with a moving top (one-sided 8/3 stress
+                                               stencils), Constant momentum diffusion, CFL dt
+  PoiseuilleFlow_x, PoiseuilleFlow_y           the no-substep MRI mode with a driving pressure gradient
+  EkmanSpiral                                  no-substep mode, Coriolis and geostrophic forcing,
+                                               no-slip bottom, fixed dt
+  RayleighDamping                              no-substep mode, Rayleigh damping of u, v, w and theta
+
+  CouetteFlow_x                                the same with the top moving along x, at the
+                                               fixed dt 0.1 its gold time implies
+
 The fixtures are the full gold arrays (tests/golden/make_gold_fixtures.py).
+CouetteFlow_x's gold file ends at t = 5.0 after 50 steps (dt = 0.1), which the
+CFL estimate of its deck (cfl 0.9, sound speed about 350 m/s on 1 m cells, dt
+near 2.6e-3 as in CouetteFlow_y) cannot give, so the case runs with a fixed dt
+of 0.1; its linear profile is the steady solution, so what it pins is that the
+no-slip stress stencils with a moving wall leave that profile unchanged.
 """
 import ctypes
 import math
@@ -82,30 +101,156 @@ def _case(name):
         cfg.alpha_T = cfg.alpha_C = 0.0
         # the deck's "erf.mri_fixed_dt_ratio" is not a key ERF reads: the ratio comes from estTimeStep
         return cfg, ("tg", (1.0, 1.0, 1.0)), (0.8, 0.16, -1.0, 0), 10
+    if name.startswith("ScalarAdvDiff_"):
+        # Tests/test_files/ScalarAdvDiff_<v>: 16^3 on the unit cube, Inflow (u = 100,
+        # rho = 1, theta = 1, scalar 0) on xlo, Outflow on xhi, periodic y, slip
+        # walls in z; Constant diffusion of the scalar (alpha_C = 1); cfl 0.5
+        dyc, scal = SAD_SCHEMES[name[len("ScalarAdvDiff_"):]]
+        cfg = _cfg((16, 16, 16), (1.0, 1.0, 1.0), adv=dyc, scal=scal)
+        cfg.bc[0], cfg.bc[3] = C.BC_INFLOW, C.BC_OUTFLOW
+        cfg.inflow_velocity[0][0] = 100.0
+        cfg.inflow_density[0], cfg.inflow_theta[0], cfg.inflow_scalar[0] = 1.0, 1.0, 0.0
+        cfg.molec_diff_type = C.MOLEC_CONSTANT
+        cfg.alpha_T, cfg.alpha_C, cfg.rho0_trans, cfg.dynamic_viscosity = 0.0, 1.0, 1.0, 0.0
+        init = ("sad", _sad_params(rho_0=1.0, A_0=1.0, u_0=100.0, v_0=0.0, uRef=0.0, prob_type=10))
+        return cfg, init, (0.5, -1.0, -1.0, 0), 20
+    if name in ("CouetteFlow_x", "CouetteFlow_y"):
+        # 32 x 4 x 16 (x) or 4 x 32 x 16 (y) unit cells, periodic x/y, no-slip
+        # walls with the top moving at 2 m/s along the flow; Constant momentum
+        # diffusion (mu = 0.1); cfl 0.9; the y deck sets init_type uniform
+        xdir = name.endswith("_x")
+        n = (32, 4, 16) if xdir else (4, 32, 16)
+        cfg = _cfg(n, tuple(float(v) for v in n))
+        cfg.bc[2] = cfg.bc[5] = C.BC_NO_SLIP_WALL
+        cfg.wall_velocity[5][0 if xdir else 1] = 2.0
+        cfg.molec_diff_type = C.MOLEC_CONSTANT
+        cfg.alpha_T = cfg.alpha_C = 0.0
+        cfg.dynamic_viscosity = 0.1
+        if xdir:
+            return cfg, ("cp", (1.0, 300.0, 2.0, 0.0, 0.0, 1, 0)), (0.9, 0.1, -1.0, 0), 50
+        return cfg, ("cp", (1.0, 300.0, 0.0, 2.0, 0.0, 1, 1)), (0.9, -1.0, -1.0, 0), 50
+    if name in ("PoiseuilleFlow_x", "PoiseuilleFlow_y"):
+        # 32 x 4 x 16 (x) or 4 x 32 x 16 (y) cells on [0, 4] x [0, 1] x [-1, 1] (or
+        # x <-> y), no-slip walls, substepping None, driving pressure gradient
+        # -0.2 along the flow, Constant momentum diffusion (mu = 0.1); cfl 0.5
+        xdir = name.endswith("_x")
+        n = (32, 4, 16) if xdir else (4, 32, 16)
+        ext = (4.0, 1.0, 2.0) if xdir else (1.0, 4.0, 2.0)
+        cfg = _cfg(n, ext)
+        cfg.prob_lo[2] = -1.0
+        cfg.prob_hi[0], cfg.prob_hi[1], cfg.prob_hi[2] = ext[0], ext[1], 1.0
+        cfg.bc[2] = cfg.bc[5] = C.BC_NO_SLIP_WALL
+        cfg.no_substepping = 1
+        cfg.molec_diff_type = C.MOLEC_CONSTANT
+        cfg.alpha_T = cfg.alpha_C = 0.0
+        cfg.rho0_trans, cfg.dynamic_viscosity = 1.0, 0.1
+        cfg.abl_pressure_grad[0 if xdir else 1] = -0.2
+        # the x deck sets change_max 1.05, the y deck keeps 1.1.  Both gold files
+        # are reproduced with init_type uniform (theta = T_0 = 300 K), which only
+        # the y deck states; the x deck's default init (theta from p_0, 348 K)
+        # gives a sound speed 11% higher and a CFL dt 11% shorter than its gold
+        init = ("cp", (1.0, 300.0, 0.0, 0.0, 0.0, 10 if xdir else 11, 1))
+        return cfg, init, (0.5, -1.0, -1.0, 0, 1.05 if xdir else 1.1, 1.0), 10
+    if name == "EkmanSpiral":
+        # 4 x 4 x 400 cells of 12.5 m, no-slip bottom, slip top, substepping None,
+        # fixed dt 0.5; Constant momentum diffusion (mu = 5); Coriolis at latitude
+        # 90 with a sidereal day and the geostrophic wind (15, 0, 0)
+        cfg = _cfg((4, 4, 400), (50.0, 50.0, 5000.0))
+        cfg.bc[2] = C.BC_NO_SLIP_WALL
+        cfg.no_substepping = 1
+        cfg.molec_diff_type = C.MOLEC_CONSTANT
+        cfg.alpha_T = cfg.alpha_C = 0.0
+        cfg.dynamic_viscosity = 5.0
+        period = 86164.0900027328
+        cfg.use_coriolis = 1
+        cfg.coriolis_factor = 2.0 * 2.0 * math.pi / period       # ERF_DataStruct.H:475-516
+        lat = 90.0 * (math.pi / 180.)
+        cfg.sinphi, cfg.cosphi = math.sin(lat), math.cos(lat)
+        ug = (15.0, 0.0, 0.0)
+        cfg.abl_geo_forcing[0] = -cfg.coriolis_factor * (ug[1] * cfg.sinphi - ug[2] * cfg.cosphi)
+        cfg.abl_geo_forcing[1] = cfg.coriolis_factor * ug[0] * cfg.sinphi
+        cfg.abl_geo_forcing[2] = -cfg.coriolis_factor * ug[0] * cfg.cosphi
+        return cfg, ("ekman", (1.0, 300.0, 15.0, period, 5.0)), (0.8, 0.5, -1.0, 0), 10
+    if name == "RayleighDamping":
+        # 64 x 4 x 64 on an 8 m cube, periodic x/y, slip walls, substepping None,
+        # cfl 0.9 with change_max 1.05; Rayleigh damping of u, v, w, theta towards
+        # (2, 1, 0, T_0) with a damping layer as deep as the domain (zdamp 1e34,
+        # dampcoef 1); the ScalarAdvDiff problem's log-profile wind and scalar ball
+        cfg = _cfg((64, 4, 64), (8.0, 8.0, 8.0))
+        cfg.no_substepping = 1
+        cfg.rayleigh_damp_T = cfg.rayleigh_damp_U = cfg.rayleigh_damp_V = cfg.rayleigh_damp_W = 1
+        cfg.rayleigh_zdamp, cfg.rayleigh_dampcoef = 1.0e34, 1.0
+        cfg.prob_hi[0] = cfg.prob_hi[1] = cfg.prob_hi[2] = 8.0
+        init = ("sad", _sad_params(rho_0=1.0, T_0=1.0, A_0=1.0, u_0=0.0, v_0=0.0, rad_0=0.25, z0=0.1, zRef=80.0,
+                                   uRef=8.0))
+        return cfg, init, (0.9, -1.0, -1.0, 0, 1.05, 1.0), 100
     raise KeyError(name)
 
 
+# ScalarAdvDiff_<v>: (dycore, dry scalar) advection of each deck; weno3z and
+# weno5z keep the default Upwind_3rd dycore
+SAD_SCHEMES = {
+    "order2": (C.ADV_CENTERED2, C.ADV_CENTERED2), "order3": (C.ADV_UPWIND3, C.ADV_UPWIND3),
+    "order4": (C.ADV_CENTERED4, C.ADV_CENTERED4), "order5": (C.ADV_UPWIND5, C.ADV_UPWIND5),
+    "order6": (C.ADV_CENTERED6, C.ADV_CENTERED6), "weno3": (C.ADV_CENTERED2, C.ADV_WENO3),
+    "weno3z": (C.ADV_UPWIND3, C.ADV_WENOZ3), "weno5": (C.ADV_CENTERED2, C.ADV_WENO5),
+    "weno5z": (C.ADV_UPWIND3, C.ADV_WENOZ5), "wenomzq3": (C.ADV_CENTERED2, C.ADV_WENOMZQ3),
+}
+
 CASES = ["ScalarDiffusionSine", "ScalarDiffusionGaussian", "ScalarAdvectionUniformU", "ScalarAdvectionShearedU",
-         "TaylorGreenAdvecting", "TaylorGreenAdvectingDiffusing"]
+         "TaylorGreenAdvecting", "TaylorGreenAdvectingDiffusing"] + \
+    ["ScalarAdvDiff_" + v for v in SAD_SCHEMES] + \
+    ["CouetteFlow_x", "CouetteFlow_y", "PoiseuilleFlow_x", "PoiseuilleFlow_y", "EkmanSpiral", "RayleighDamping"]
+
+
+def rayleigh_profiles(name, cfg):
+    """Rayleigh reference profiles of a case (ubar, vbar, wbar, thetabar; nz + 1
+    values), or None.  RayleighDamping: the ScalarAdvDiff problem's
+    erf_init_rayleigh (Exec/RegTests/ScalarAdvDiff/ERF_prob.cpp:39-55)."""
+    if name != "RayleighDamping":
+        return None
+    n = cfg.nz + 1
+    return np.full(n, 2.0), np.full(n, 1.0), np.zeros(n), np.full(n, 1.0)
+
+
+def init_oracle(orc, init):
+    """The case's initial state, restated in the oracle."""
+    l = olib()
+    if init[0] == "sad":
+        l.orc_init_scalar_advdiff.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        params = np.ascontiguousarray(init[1])
+        orc._chk(l.orc_init_scalar_advdiff(orc.h, params.ctypes.data))
+    elif init[0] == "cp":
+        l.orc_init_couette_poiseuille.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2
+        orc._chk(l.orc_init_couette_poiseuille(orc.h, *init[1]))
+    elif init[0] == "ekman":
+        l.orc_init_ekman.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5
+        orc._chk(l.orc_init_ekman(orc.h, *init[1]))
+    else:
+        l.orc_init_taylor_green.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
+        orc._chk(l.orc_init_taylor_green(orc.h, *init[1]))
+
+
+def run_args(run):
+    """(cfl, fixed_dt, fixed_fast_dt, fixed_mri_ratio, change_max, init_shrink)"""
+    run = tuple(run)
+    return run + (1.1, 1.0)[len(run) - 4:]
 
 
 def run_case(name):
     cfg, init, run, nsteps = _case(name)
     orc = Oracle(cfg)
     l = olib()
-    if init[0] == "sad":
-        l.orc_init_scalar_advdiff.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        params = np.ascontiguousarray(init[1])
-        orc._chk(l.orc_init_scalar_advdiff(orc.h, params.ctypes.data))
-    else:
-        l.orc_init_taylor_green.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
-        orc._chk(l.orc_init_taylor_green(orc.h, *init[1]))
+    init_oracle(orc, init)
+    ray = rayleigh_profiles(name, cfg)
+    if ray is not None:
+        orc.set_rayleigh(*ray)
     l.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                           ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                           ctypes.POINTER(ctypes.c_double)]
     t_end = ctypes.c_double()
-    cfl, fdt, ffdt, fratio = run
-    orc._chk(l.orc_run(orc.h, 0.0, nsteps, cfl, fdt, ffdt, fratio, 1.1, 1.0, 1.e100, ctypes.byref(t_end)))
+    cfl, fdt, ffdt, fratio, cmax, shrink = run_args(run)
+    orc._chk(l.orc_run(orc.h, 0.0, nsteps, cfl, fdt, ffdt, fratio, cmax, shrink, 1.e100, ctypes.byref(t_end)))
     return cfg, orc, t_end.value
 
 

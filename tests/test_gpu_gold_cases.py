@@ -2,14 +2,12 @@
 oracle's gold tests, ERF's time loop with its dt control (erfx_sim_run:
 ComputeDt / estTimeStep before every step), bit-identical to the oracle, which
 in turn matches the reference's gold files (test_gold_*.py)."""
-import ctypes
-
 import numpy as np
 import pytest
 
 from erf_amd import config as C
 from helpers import valid
-from oracle_lib import Oracle, lib as olib
+from oracle_lib import Oracle
 import test_gold_regression_cases as G
 
 pytestmark = pytest.mark.gpu
@@ -24,20 +22,18 @@ def test_gold_case_on_gpu(name):
     from erf_amd.lib import Sim
     cfg, init, run, nsteps = G._case(name)
     # initial state from the oracle's restatement of the case
+    print("gold case %s on the GPU" % name, flush=True)
     o0 = Oracle(cfg)
-    l = olib()
-    if init[0] == "sad":
-        l.orc_init_scalar_advdiff.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        params = np.ascontiguousarray(init[1])
-        o0._chk(l.orc_init_scalar_advdiff(o0.h, params.ctypes.data))
-    else:
-        l.orc_init_taylor_green.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
-        o0._chk(l.orc_init_taylor_green(o0.h, *init[1]))
+    G.init_oracle(o0, init)
     state = {fid: o0.get(fid) for fid in (C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0, C.F_PI0)}
-    cfl, fdt, ffdt, fratio = run
-    dtc = C.dt_control(cfl=cfl, fixed_dt=fdt, fixed_fast_dt=ffdt, fixed_mri_ratio=fratio)
+    cfl, fdt, ffdt, fratio, cmax, shrink = G.run_args(run)
+    dtc = C.dt_control(cfl=cfl, fixed_dt=fdt, fixed_fast_dt=ffdt, fixed_mri_ratio=fratio, change_max=cmax,
+                       init_shrink=shrink)
     sim = Sim(cfg)
     sim.load(state)
+    ray = G.rayleigh_profiles(name, cfg)
+    if ray is not None:
+        sim.set_rayleigh(*ray)
     t_gpu, n_gpu = sim.run(0.0, nsteps, dtc)
     _, orc, t_orc = G.run_case(name)
     assert n_gpu == nsteps
