@@ -75,7 +75,7 @@ void make_params(const erfx_config& c, const erfx_layout& lay, Params& P) {
     P.dz_min = c.dz;
 }
 
-static int validate(const erfx_config& c) {
+int validate_config(const erfx_config& c) {
     if (c.nx < 4 || c.ny < 4 || c.nz < 4) { set_error("box must be at least 4 cells in each direction"); return ERFX_ERR_ARG; }
     if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) { set_error("cell sizes must be positive"); return ERFX_ERR_ARG; }
     if (c.moisture_type < ERFX_MOIST_NONE || c.moisture_type > ERFX_MOIST_KESSLER_NORAIN) {
@@ -337,7 +337,7 @@ static int dalloc(std::vector<void*>& list, double** p, size_t n) {
 
 int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
     if (!cfg || !out) { set_error("null argument"); return ERFX_ERR_ARG; }
-    int rc = validate(*cfg);
+    int rc = validate_config(*cfg);
     if (rc) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return ERFX_ERR_NODEVICE; }

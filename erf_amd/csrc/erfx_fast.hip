@@ -12,6 +12,8 @@
 // The arithmetic of every point keeps the reference's operation order.
 #include "erfx_common.h"
 #include "erfx_internal.h"
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 namespace erfx {
@@ -456,3 +458,93 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
 }
 
 }  // namespace erfx
+
+// ---- erf_fast_rhs_N with no device allocation ---------------------------------
+// For callers that cannot allocate device memory (code under stream / graph
+// capture, tight serving loops, harnesses that forbid hipMalloc): the work
+// fields of the substep (theta extrapolated, the rho*w increment, the two
+// horizontal divergences and the tridiagonal solution: 5 fields of the
+// layout) live in one static device array instead of a context's workspace.
+// Each GPU holds its own copy of the array until the process ends; calls on
+// one GPU share it, so two calls that overlap on one GPU race.  No element of
+// it needs to start at zero: every launch of the substep writes each element
+// it reads (K1 the grown box of theta, the column kernel its own column)
+// before reading it, so whatever a previous call left there is never seen.
+namespace erfx {
+namespace noalloc {
+constexpr size_t SCRATCH_DOUBLES = size_t(8) << 20;      // 64 MiB
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+// The array's device address per device, looked up once on the device's first
+// call (the host-side name of a __device__ array is only a stand-in: passing it
+// to a kernel would fault).
+static double* g_ptr[MAX_DEVICES];
+static std::mutex g_mu;
+
+static int device_scratch(double** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("erfx_fast_rhs_N_noalloc: device index beyond the table of scratch addresses");
+        return ERFX_ERR_NODEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_ptr[dev];
+    return ERFX_OK;
+}
+}  // namespace noalloc
+
+}  // namespace erfx
+
+using namespace erfx;
+
+extern "C" int erfx_fast_rhs_N_noalloc(const erfx_config* cfg, void* stream, int step, int nrk,
+                                       const erfx_state* slow_rhs, const erfx_state* prev,
+                                       const erfx_state* stage, const double* prim, const double* pi_stage,
+                                       const double* fast_coeffs, erfx_state* data, erfx_state* scratch,
+                                       double dtau, double beta_s, double facinv) {
+    auto bad = [](const erfx_state* s) { return !s || !s->cons || !s->xmom || !s->ymom || !s->zmom; };
+    if (!cfg || bad(slow_rhs) || bad(prev) || bad(stage) || bad(data) || bad(scratch) || !prim || !pi_stage ||
+        !fast_coeffs) {
+        set_error("null argument");
+        return ERFX_ERR_ARG;
+    }
+    int rc = validate_config(*cfg);
+    if (rc) return rc;
+    if (cfg->use_terrain) { set_error("a terrain-following grid takes erf_fast_rhs_T"); return ERFX_ERR_STATE; }
+    if (!(nrk > 0 || step == 0)) {
+        set_error("the first RK stage takes a single substep (ERF_fast_rhs_N.cpp:106)");
+        return ERFX_ERR_ARG;
+    }
+    erfx_layout lay;
+    erfx_layout_for(cfg, &lay);
+    const size_t ne = (size_t)lay.nelem;
+    if (5 * ne > noalloc::SCRATCH_DOUBLES) {
+        set_error("erfx_fast_rhs_N_noalloc: the box needs " + std::to_string(5 * ne) +
+                  " doubles of work space, the static buffer holds " + std::to_string(noalloc::SCRATCH_DOUBLES) +
+                  " (5 fields of the layout: boxes up to about 96 x 96 x 96 cells)");
+        return ERFX_ERR_ARG;
+    }
+    double* base = nullptr;
+    rc = noalloc::device_scratch(&base);
+    if (rc) return rc;
+    Params P;
+    make_params(*cfg, lay, P);
+    FastWork W;
+    W.extrap = base;
+    W.drw = base + ne;
+    W.trhs = base + 2 * ne;      // two components
+    W.soln = base + 4 * ne;
+    fast_rhs_N(P, W, step, nrk, *slow_rhs, *prev, *stage, prim, pi_stage, fast_coeffs, *data, *scratch, dtau,
+               beta_s, facinv, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return ERFX_OK;
+}

@@ -37,6 +37,10 @@ struct CopyList {
 };
 void copy_fields(const CopyList& c, hipStream_t s);
 
+// erfx_api.hip: the checks of erfx_ctx_create on a configuration (status code,
+// erfx_last_error text)
+int validate_config(const erfx_config& c);
+
 // erfx_fast.hip
 struct FastWork {
     double* extrap;   // theta extrapolated forward, cells grown by 1

@@ -48,6 +48,8 @@ int main(void) {
     F(erfx_config, dycore_horiz_upw_frac) F(erfx_config, molec_diff_type) F(erfx_config, les_type)
     F(erfx_config, Cs) F(erfx_config, surface_heat_flux) F(erfx_config, c_p)
     F(erfx_config, abl_geo_forcing) F(erfx_config, mri_ratio) F(erfx_config, implicit_substepping)
+    F(erfx_config, inflow_qc) F(erfx_config, prob_hi) F(erfx_config, no_substepping)
+    F(erfx_config, wall_velocity) F(erfx_config, abl_pressure_grad)
     F(erfx_layout, sy) F(erfx_layout, nelem)
     return 0;
 }

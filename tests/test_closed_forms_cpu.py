@@ -1,0 +1,201 @@
+"""Closed-form checks of oracle functions that the reference's gold files do not
+pin, each against the published formula it restates (CPU only).
+
+  Smagorinsky        nu_t = (C_s Delta)^2 sqrt(2 S_ij S_ij) in a uniform shear
+                     u = gamma z: mu_t = rho C_s^2 Delta^2 |gamma|, Delta =
+                     (dx dy dz)^(1/3) (Smagorinsky 1963; the form of
+                     ERF_ComputeTurbulentViscosity.cpp:54-128); the other
+                     diffusivities scaled by 1/Pr_t and 1/Sc_t.
+  Kessler rates      Klemp & Wilhelmson (1978, J. Atmos. Sci. 35, eqs. 2.13-2.16):
+                     autoconversion k1 (q_c - a), k1 = 1e-3 s^-1, a = 1e-3;
+                     accretion k2 q_c q_r^0.875, k2 = 2.2 s^-1; evaporation of
+                     rain (1/rho)(1 - q_v/q_vs) C (rho q_r)^0.525 /
+                     (5.4e5 + 2.55e6 / (p q_vs)), C = 1.6 + 124.9 (rho q_r)^0.2046,
+                     rho in g cm^-3 and p in mb; terminal velocity 36.34 m/s
+                     (rho q_r)^0.1346 (rho / rho_0)^-1/2, rho_0 = 1.16 kg m^-3.
+                     Each case is set up so that only the process under test
+                     changes the quantity compared.
+  Upwind stencils    face values of U3 and U5 for either wind direction and of
+                     C4 and C6 against the textbook weights, e.g. U5 for u > 0:
+                     (2, -13, 47, 27, -3) / 60 over cells i-3 .. i+1.  A sign
+                     error in the upwind (dissipative) part fails them.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from erf_amd import config as C
+from helpers import valid
+from oracle_lib import Oracle, lib as olib, saturation
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+# ---- Smagorinsky eddy viscosity in a uniform shear ---------------------------------
+
+@pytest.mark.parametrize("gamma", [0.02, -0.035])
+def test_smagorinsky_uniform_shear(gamma):
+    nx, ny, nz = 8, 6, 10
+    dx, dy, dz = 20.0, 25.0, 8.0
+    cfg = C.default_config(nx, ny, nz, dx, dy, dz)
+    cfg.les_type = C.LES_SMAGORINSKY
+    cfg.Cs = 0.17
+    cfg.Pr_t = 0.4
+    cfg.Sc_t = 0.7
+    L = C.layout_for(cfg)
+    orc = Oracle(cfg)
+    ng = L.ng
+    shp = L.shape3()
+    rho = 1.13
+    cons = np.zeros((cfg.ncons,) + shp)
+    cons[C.RHO] = rho
+    cons[C.RHOTHETA] = rho * 300.0
+    # u = gamma z on every x-face, ghosts included, so every edge sees the same shear
+    z = (np.arange(shp[0]) - ng + 0.5) * dz
+    u = np.broadcast_to((gamma * z)[:, None, None], shp)[None].copy()
+    zero = np.zeros((1,) + shp)
+    orc.set(C.state_id(C.S_OLD, 0), cons)
+    orc.set(C.D_XVEL_OLD, u)
+    orc.set(C.D_YVEL_OLD, zero)
+    orc.set(C.D_ZVEL_OLD, zero)
+    orc.call("orc_turbulent_viscosity")
+    mu = orc.get(C.D_EDDY)[:, ng:ng + nz, ng:ng + ny, ng:ng + nx]
+    delta = (dx * dy * dz) ** (1.0 / 3.0)
+    mu_t = rho * cfg.Cs ** 2 * delta ** 2 * abs(gamma)
+    # EddyDiff order (ERF_IndexDefines.H:152-169): Mom, Theta, KE, QKE, Scalar, Q (h), then (v)
+    expect = {0: mu_t, 1: mu_t / cfg.Pr_t, 4: mu_t / cfg.Sc_t, 5: mu_t / cfg.Sc_t}
+    for comp, val in expect.items():
+        for off in (0, 6):
+            got = mu[comp + off]
+            assert np.abs(got - val).max() <= 1e-13 * val, (comp + off, got.min(), got.max(), val)
+
+
+# ---- Kessler process rates on one uniform state ------------------------------------
+
+def _kessler_case(qv_frac, qc, qr, T=288.0, rho=1.1, dt=2.0):
+    """A uniform 4 x 4 x 4 moist state at temperature ~T: returns the oracle after
+    one Kessler call, with the state's rho, T, p [mb], q_vs, theta and dt."""
+    cfg = C.default_config(4, 4, 4, 100.0, 100.0, 100.0)
+    C.set_moisture(cfg, C.MOIST_KESSLER)
+    L = C.layout_for(cfg)
+    # theta for the wanted T at this density (dry EOS is enough to place T; the
+    # moist EOS below gives the T the scheme sees)
+    p_d = rho * C.R_d * T
+    theta = T * (C.p_0 / p_d) ** (C.R_d / C.Cp_d)
+    # q_vs at the state's own T and p (iterate: q_v enters T and p)
+    qv = 0.0
+    for _ in range(6):
+        rt = rho * theta
+        p = C.p_0 * (C.R_d * rt * (1.0 + C.R_v / C.R_d * qv) / C.p_0) ** C.Gamma
+        Tm = p / (C.R_d * rho * (1.0 + C.R_v / C.R_d * qv))
+        qvs = float(saturation(np.array([Tm]), p / 100.0)[2][0])
+        qv = qv_frac * qvs
+    cons = np.zeros((cfg.ncons,) + L.shape3())
+    cons[C.RHO] = rho
+    cons[C.RHOTHETA] = rho * theta
+    cons[C.RHOQ1] = rho * qv
+    cons[C.RHOQ2] = rho * qc
+    cons[C.RHOQ3] = rho * qr
+    orc = Oracle(cfg)
+    orc.set(C.F_CONS, cons)
+    before = valid(orc.get(C.F_CONS), cfg, L, C.F_CONS)[:, 1, 1, 1].copy()
+    orc.kessler(dt)
+    after = valid(orc.get(C.F_CONS), cfg, L, C.F_CONS)[:, 1, 1, 1]
+    ra = orc.get(C.D_RAIN_ACCUM)[L.ng + 1, L.ng + 1]
+    # the T, p and q_vs the scheme evaluates (ERF_Init_Kessler.cpp:64-99)
+    rt = before[C.RHOTHETA]
+    qv0 = before[C.RHOQ1] / rho
+    p = C.p_0 * (C.R_d * rt * (1.0 + C.R_v / C.R_d * qv0) / C.p_0) ** C.Gamma
+    Tm = p / (C.R_d * rho * (1.0 + C.R_v / C.R_d * qv0))
+    qvs = float(saturation(np.array([Tm]), p / 100.0)[2][0])
+    return before, after, ra, dict(rho=rho, T=Tm, p_mb=p / 100.0, qvs=qvs, theta=rt / rho, dt=dt, qv=qv0)
+
+
+def test_kessler_autoconversion():
+    """Subsaturated cloud above the threshold, no rain: the rain gained is
+    Kessler's autoconversion k1 (q_c - a) dt alone (cloud evaporation changes
+    only vapour and cloud)."""
+    qc = 3.0e-3
+    b, a, ra, s = _kessler_case(qv_frac=0.6, qc=qc, qr=0.0)
+    k1, thresh = 1.0e-3, 1.0e-3
+    want = k1 * (qc - thresh) * s["dt"]
+    got = a[C.RHOQ3] / s["rho"]
+    assert abs(got - want) <= 1e-12 * want, (got, want)
+
+
+def test_kessler_accretion():
+    """Supersaturated air (no rain evaporation), cloud below the autoconversion
+    threshold, uniform rain (no sedimentation divergence): the rain gained is
+    the accretion k2 q_c q_r^0.875 dt alone."""
+    qc, qr = 5.0e-4, 2.0e-3
+    b, a, ra, s = _kessler_case(qv_frac=1.1, qc=qc, qr=qr)
+    k2 = 2.2
+    want = k2 * qc * qr ** 0.875 * s["dt"]
+    got = a[C.RHOQ3] / s["rho"] - qr
+    assert abs(got - want) <= 1e-10 * want, (got, want)
+
+
+def test_kessler_rain_evaporation():
+    """Subsaturated air, no cloud, uniform rain: rain lost to vapour at Klemp &
+    Wilhelmson's evaporation rate (CGS density, mb pressure), vapour gains it,
+    theta cools by (theta / T)(L_v / c_p) per unit of vapour gained."""
+    qr = 1.0e-3
+    b, a, ra, s = _kessler_case(qv_frac=0.7, qc=0.0, qr=qr)
+    rho_cgs = 1.0e-3 * s["rho"]
+    vent = 1.6 + 124.9 * (rho_cgs * qr) ** 0.2046
+    rate = (1.0 / rho_cgs) * (1.0 - s["qv"] / s["qvs"]) * vent * (rho_cgs * qr) ** 0.525 / \
+        (5.4e5 + 2.55e6 / (s["p_mb"] * s["qvs"]))
+    want = rate * s["dt"]
+    lost = qr - a[C.RHOQ3] / s["rho"]
+    gained = a[C.RHOQ1] / s["rho"] - s["qv"]
+    assert abs(lost - want) <= 1e-10 * want, (lost, want)
+    assert abs(gained - want) <= 1e-10 * want, (gained, want)
+    dtheta = a[C.RHOTHETA] / s["rho"] - s["theta"]
+    want_th = -s["theta"] / s["T"] * (2.501e6 / C.Cp_d) * want      # L_v = 2.501e6 J/kg (ERF_Constants.H:66)
+    assert abs(dtheta - want_th) <= 1e-9 * abs(want_th), (dtheta, want_th)
+
+
+@pytest.mark.parametrize("qr,rho", [(1.0e-3, 1.1), (4.0e-3, 0.7)])
+def test_kessler_terminal_velocity(qr, rho):
+    """Surface rain of one step from a uniform rain field: rho q_r V_t dt [mm],
+    V_t = 36.34 (rho q_r)^0.1346 (rho / 1.16)^-1/2 m/s with rho q_r in g cm^-3."""
+    b, a, ra, s = _kessler_case(qv_frac=1.05, qc=0.0, qr=qr, rho=rho)
+    vt = 36.34 * (1.0e-3 * rho * qr) ** 0.1346 * (rho / 1.16) ** -0.5
+    want = rho * qr * vt * s["dt"]          # kg m^-2 of water = mm
+    assert abs(ra - want) <= 1e-12 * want, (ra, want)
+    assert 1.0 < vt < 12.0                  # a raindrop's speed in m/s
+
+
+# ---- upwind and centred face interpolation against textbook weights ----------------
+
+def _face(type_, s, upw):
+    f = olib().orc_face_interp
+    f.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double]
+    f.restype = ctypes.c_double
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return f(type_, s.ctypes.data_as(_dp), upw, 1.0)
+
+
+# weights on cells i-3 .. i+2 for the face between i-1 and i (wind > 0; the
+# wind < 0 weights are the mirror image about the face)
+TEXTBOOK = {
+    C.ADV_UPWIND3: np.array([0, -1, 5, 2, 0, 0]) / 6.0,
+    C.ADV_UPWIND5: np.array([2, -13, 47, 27, -3, 0]) / 60.0,
+    C.ADV_CENTERED4: np.array([0, -1, 7, 7, -1, 0]) / 12.0,
+    C.ADV_CENTERED6: np.array([1, -8, 37, 37, -8, 1]) / 60.0,
+}
+
+
+@pytest.mark.parametrize("type_", list(TEXTBOOK))
+def test_face_interp_textbook_weights(type_):
+    rng = np.random.default_rng(7)
+    w = TEXTBOOK[type_]
+    for _ in range(20):
+        s = rng.normal(size=6)
+        for upw in (3.0, -3.0):
+            ww = w if upw > 0 else w[::-1]
+            want = float(np.dot(ww, s))
+            got = _face(type_, s, upw)
+            assert abs(got - want) <= 1e-14 * max(1.0, np.abs(s).max()), (type_, upw, got, want)

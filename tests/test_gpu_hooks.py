@@ -37,6 +37,16 @@ def _setup(kind):
         dt = 1.0
     elif kind.startswith("inflow"):   # ERF's Inflow on x-lo, Outflow on x-hi
         cfg, f = P.inflow_outflow(24, 16, 16, seed=19, moist=(kind == "inflow_moist"))
+        if kind == "inflow_diff":     # with Constant molecular diffusion
+            cfg.molec_diff_type = C.MOLEC_CONSTANT
+            cfg.dynamic_viscosity, cfg.alpha_T, cfg.alpha_C = 60.0, 40.0, 50.0
+        dt = 1.0
+    elif kind == "noslip":   # no-slip walls, the top moving; Constant diffusion
+        cfg, f = P.rising_thermal(24, 16, 16, seed=29, adv=C.ADV_UPWIND3)
+        cfg.bc[2] = cfg.bc[5] = C.BC_NO_SLIP_WALL
+        cfg.wall_velocity[5][0], cfg.wall_velocity[5][1] = 3.0, -1.5
+        cfg.molec_diff_type = C.MOLEC_CONSTANT
+        cfg.dynamic_viscosity, cfg.alpha_T = 40.0, 20.0
         dt = 1.0
     elif kind.startswith("hill"):   # terrain-following grid over a Gaussian hill
         adv = C.ADV_CENTERED2 if kind == "hill_c2" else C.ADV_UPWIND3
@@ -81,7 +91,8 @@ def _st(sim, s):
 
 TERRAIN_IDS = [C.F_Z_ND, C.D_DETJ, C.D_AX, C.D_AY, C.D_AZ, C.D_Z_CC]
 HILLS = ["hill", "hill_c2", "hill_moist"]
-KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain", "inflow", "inflow_moist"] + HILLS
+KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain", "inflow", "inflow_moist", "inflow_diff",
+         "noslip"] + HILLS
 MOIST_BUOYANCY = ["moist1", "moist2", "moist3", "moist4"]
 
 

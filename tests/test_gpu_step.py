@@ -273,22 +273,57 @@ def test_inflow_outflow_step_exact(moist):
     _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
 
 
+def test_no_slip_moving_wall_step():
+    """No-slip walls with a moving top (Dirichlet tangential velocity, one-sided
+    8/3 strain stencils at both walls) and Constant momentum diffusion."""
+    cfg, f = P.rising_thermal(24, 16, 16, seed=29, adv=C.ADV_UPWIND3)
+    cfg.bc[2] = cfg.bc[5] = C.BC_NO_SLIP_WALL
+    cfg.wall_velocity[5][0], cfg.wall_velocity[5][1] = 3.0, -1.5
+    cfg.wall_velocity[5][2] = 7.0           # normal component: forced to 0
+    cfg.molec_diff_type = C.MOLEC_CONSTANT
+    cfg.dynamic_viscosity, cfg.alpha_T = 40.0, 20.0
+    _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
+    _compare(cfg, f, 1.0, 5, 1e-9)
+
+
+def test_no_substepping_step():
+    """substepping_type None (ERF_TI_no_substep_fun.H) on the rising thermal
+    with a driving pressure gradient."""
+    cfg, f = P.rising_thermal(20, 16, 16, seed=31, adv=C.ADV_UPWIND3)
+    cfg.no_substepping = 1
+    cfg.abl_pressure_grad[0], cfg.abl_pressure_grad[1] = -2.0e-4, 1.0e-4
+    _compare(cfg, f, 0.5, 1, 1e-12, exact=True)
+    _compare(cfg, f, 0.5, 6, 1e-9)
+
+
 def test_inflow_outflow_multistep():
     cfg, f = P.inflow_outflow(24, 16, 16, seed=24, adv=C.ADV_UPWIND5)
     _compare(cfg, f, 1.0, 8, 1e-9)
 
 
+@pytest.mark.parametrize("mtype", [C.MOLEC_CONSTANT, C.MOLEC_CONSTANT_ALPHA])
+def test_inflow_outflow_diffusion(mtype):
+    """Inflow with molecular diffusion: one-sided fluxes of theta and the scalar
+    through the Dirichlet face, one-sided strain of v and w at the inflow face
+    (ERF_DiffusionSrcForState_N.cpp:208-227, ERF_ComputeStrain_N.cpp:67-120)."""
+    cfg, f = P.inflow_outflow(24, 16, 16, seed=25, adv=C.ADV_UPWIND5)
+    cfg.molec_diff_type = mtype
+    cfg.dynamic_viscosity, cfg.alpha_T, cfg.alpha_C = 60.0, 40.0, 50.0
+    _compare(cfg, f, 1.0, 1, 1e-12, exact=True)
+    _compare(cfg, f, 1.0, 6, 1e-9)
+
+
 def test_inflow_refusals():
     """Inflow on a vertical face, with the other lateral direction walled, or
-    with diffusion: refused at creation."""
+    with an LES model: refused at creation."""
     from erf_amd.lib import Sim, ErfxError
     cases = []
     cfg, _ = P.inflow_outflow(16, 16, 16)
     cfg.bc[1] = cfg.bc[4] = C.BC_SLIP_WALL
     cases.append(cfg)
     cfg, _ = P.inflow_outflow(16, 16, 16)
-    cfg.molec_diff_type = C.MOLEC_CONSTANT_ALPHA
-    cfg.dynamic_viscosity = 1.0
+    cfg.les_type = C.LES_SMAGORINSKY
+    cfg.Cs = 0.1
     cases.append(cfg)
     cfg, _ = P.inflow_outflow(16, 16, 16)
     cfg.bc[5] = C.BC_INFLOW
