@@ -5,7 +5,9 @@
 #include "erfx_mlp.h"
 
 #include <algorithm>
+#include <cerrno>
 #include <chrono>
+#include <sys/stat.h>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -157,6 +159,7 @@ struct erfx_ctx {
     FastWork fw{};
     SlowWork sw{};
     double* ray = nullptr;    // Rayleigh profiles and per-level factors (RayleighTable layout)
+    std::vector<double> ray_prof;   // host copy of the four profiles as given (4 x (nz+1))
     double* kfz = nullptr;    // Kessler sedimentation flux on z-faces (moist only)
     int* kbad = nullptr;      // device word: a Kessler temperature left the saturation fit's range
     // terrain-following grid: node heights and metrics (the Params pointers
@@ -444,6 +447,19 @@ int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
         for (int j = 0; j < L.ny; ++j)
             for (int i = 0; i < L.nx; ++i) jmin = std::min(jmin, J[(size_t)L.ix(i, j, k)]);
     c->P.dz_min = c->P.cfg.dz * jmin;
+    // a plane top: every node of level nz (ghosts included) at one height
+    {
+        std::vector<double> zt((size_t)L.sz);
+        ERFX_HIP(hipMemcpyAsync(zt.data(), c->z_nd + (size_t)L.nz * L.sz + (size_t)L.ng * L.sz,
+                                (size_t)L.sz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        ERFX_HIP(hipStreamSynchronize(c->stream));
+        bool flat = true;
+        const double z0 = zt[(size_t)(L.ng + L.ng * L.sy)];
+        for (int j = -L.ng; j <= L.ny + L.ng; ++j)
+            for (int i = -L.ng; i <= L.nx + L.ng; ++i)
+                flat = flat && zt[(size_t)((i + L.ng) + (j + L.ng) * L.sy)] == z0;
+        c->P.flat_top = flat;
+    }
     return build_ray_points(c);
 }
 
@@ -474,6 +490,14 @@ int erfx_set_base_state(erfx_ctx* c, const double* r0, const double* p0, const d
 // here on the host with the C library's sin, exactly as the reference
 // evaluates them per cell (ERF_make_sources.cpp:213-222, ERF_make_mom_sources.cpp:255-296).
 int erfx_set_rayleigh(erfx_ctx* c, const double* ub, const double* vb, const double* wb, const double* tb) {
+    if (c && ub && vb && wb && tb) {
+        const size_t n1 = (size_t)c->P.L.nz + 1;
+        c->ray_prof.assign(4 * n1, 0.0);
+        for (size_t k = 0; k < n1; ++k) {
+            c->ray_prof[k] = ub[k]; c->ray_prof[n1 + k] = vb[k];
+            c->ray_prof[2 * n1 + k] = wb[k]; c->ray_prof[3 * n1 + k] = tb[k];
+        }
+    }
     if (!c || !ub || !vb || !wb || !tb) { set_error("null argument"); return ERFX_ERR_ARG; }
     const erfx_config& cf = c->P.cfg;
     const int n1 = cf.nz + 1;
@@ -995,8 +1019,10 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     }
     const bool nosub = P.cfg.no_substepping != 0;
     const double sub_timestep = nosub ? 0.0 : dt / ratio;
-    // the substep chain folds apply_bcs into erf_fast_rhs_N's kernels (flat grids only)
-    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !P.terrain && !nosub;
+    // the substep chain folds apply_bcs into the substep's kernels: on a flat
+    // grid, and on a terrain grid whose top is a plane (there the wall w of the
+    // top face is the Dirichlet value, as the substep itself leaves it)
+    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !nosub && (!P.terrain || P.flat_top);
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
     double sdt[3];
     stage_dts(time, dt, sdt);
@@ -1031,13 +1057,13 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
             // the full fill runs after the stage's last substep.
             FastChain ch;
             ch.rt_prev = chain && ks > 0;
-            ch.ghost_writes = chain && ks < nsub - 1;
+            ch.ghost_writes = chain && ks < nsub - 1 && !P.terrain;   // the terrain substep reads images
             ch.w_lo = P.vel_extdir[2][2];
             ch.w_hi = P.vel_extdir[2][5];
             if (ks == 0) make_fast_coeffs(P, s->fast_coeffs, Sn, s->prim, s->pi_stage, s->r0, s->pi0, dtau, beta_s, st);
             if (P.terrain)
                 fast_rhs_T(P, c->fw, c->tw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs,
-                           Ss, Sc, dtau, beta_s, inv_fac, st);
+                           Ss, Sc, dtau, beta_s, inv_fac, st, ch);
             else
                 fast_rhs_N(P, c->fw, ks, nrk, Fs, ks == 0 ? So : Ss, Sn, s->prim, s->pi_stage, s->fast_coeffs, Ss, Sc,
                            dtau, beta_s, inv_fac, st, ch);
@@ -1272,6 +1298,118 @@ int erfx_sim_integrals(erfx_sim* s, double* mass, double* rhotheta, double* scal
     *rhotheta = s->red_host[2] + s->red_host[3];
     *scalar = s->red_host[4] + s->red_host[5];
     return launch_status();
+}
+
+// ---- checkpoint / restart -----------------------------------------------------
+// The state that a step reads and that no step recomputes: cons, the three
+// velocities, the base state, the node heights of a terrain grid, the
+// accumulated surface rain and the Rayleigh profiles; every other buffer of
+// the sim is made again from these at the start of each step.  The role of
+// ERF::WriteCheckpointFile / ReadCheckpointFile (IO/ERF_Checkpoint.cpp:25-590),
+// which write vars_new, the base state and z_phys_nd per level with AMReX's
+// VisMF; here one text Header and one raw little-endian FP64 file.
+static const char* CHK_MAGIC = "erfx-checkpoint-1";
+
+static std::vector<std::pair<int, long>> chk_fields(erfx_sim* s) {
+    std::vector<std::pair<int, long>> f;
+    for (int id : {ERFX_F_CONS, ERFX_F_XVEL, ERFX_F_YVEL, ERFX_F_ZVEL, ERFX_F_R0, ERFX_F_P0, ERFX_F_PI0}) {
+        long n = 0;
+        field_ptr(s, id, &n);
+        f.push_back({id, n});
+    }
+    if (s->ctx->P.terrain) f.push_back({ERFX_F_Z_ND, s->ctx->P.L.ne});
+    if (s->rain_accum) f.push_back({ERFX_D_RAIN_ACCUM, s->ctx->P.L.sz});
+    return f;
+}
+
+int erfx_sim_write_checkpoint(erfx_sim* s, const char* dir, double time, int step) {
+    if (!s || !dir) { set_error("null argument"); return ERFX_ERR_ARG; }
+    if (mkdir(dir, 0755) != 0 && errno != EEXIST) {
+        set_error(std::string("cannot create checkpoint directory ") + dir); return ERFX_ERR_ARG;
+    }
+    ERFX_HIP(hipStreamSynchronize(s->stream));
+    const Params& P = s->ctx->P;
+    const auto fields = chk_fields(s);
+    const std::string d(dir);
+    FILE* h = std::fopen((d + "/Header").c_str(), "w");
+    if (!h) { set_error("cannot write " + d + "/Header"); return ERFX_ERR_ARG; }
+    std::fprintf(h, "%s\n%d %d %d %d %ld %ld %ld\n%d %d %d\n%a\n%d\n%zu\n", CHK_MAGIC, P.L.nx, P.L.ny, P.L.nz,
+                 P.L.ng, P.L.sy, P.L.sz, P.L.ne, P.cfg.ncons, P.cfg.moisture_type, P.cfg.use_terrain, time, step,
+                 s->ctx->ray_prof.size());
+    for (const auto& f : fields) std::fprintf(h, "%d %ld\n", f.first, f.second);
+    std::fprintf(h, "# time %.17g, step %d\n", time, step);
+    std::fclose(h);
+    FILE* b = std::fopen((d + "/state.bin").c_str(), "wb");
+    if (!b) { set_error("cannot write " + d + "/state.bin"); return ERFX_ERR_ARG; }
+    std::vector<double> host;
+    for (const auto& f : fields) {
+        host.resize((size_t)f.second);
+        double* p = field_ptr(s, f.first, nullptr);
+        hipError_t e = hipMemcpy(host.data(), p, host.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { std::fclose(b); return hip_fail(e, "checkpoint download"); }
+        if (std::fwrite(host.data(), sizeof(double), host.size(), b) != host.size()) {
+            std::fclose(b); set_error("short write of " + d + "/state.bin"); return ERFX_ERR_ARG;
+        }
+    }
+    const auto& rp = s->ctx->ray_prof;
+    if (!rp.empty() && std::fwrite(rp.data(), sizeof(double), rp.size(), b) != rp.size()) {
+        std::fclose(b); set_error("short write of " + d + "/state.bin"); return ERFX_ERR_ARG;
+    }
+    if (std::fclose(b) != 0) { set_error("cannot close " + d + "/state.bin"); return ERFX_ERR_ARG; }
+    return ERFX_OK;
+}
+
+int erfx_sim_read_checkpoint(erfx_sim* s, const char* dir, double* time, int* step) {
+    if (!s || !dir || !time || !step) { set_error("null argument"); return ERFX_ERR_ARG; }
+    const Params& P = s->ctx->P;
+    const std::string d(dir);
+    FILE* h = std::fopen((d + "/Header").c_str(), "r");
+    if (!h) { set_error("cannot read " + d + "/Header"); return ERFX_ERR_ARG; }
+    char magic[64] = {0};
+    int nx, ny, nz, ng, nc, mt, ut, st;
+    long sy, sz, ne;
+    double t;
+    size_t nray = 0;
+    int got = std::fscanf(h, "%63s %d %d %d %d %ld %ld %ld %d %d %d %la %d %zu", magic, &nx, &ny, &nz, &ng, &sy, &sz,
+                          &ne, &nc, &mt, &ut, &t, &st, &nray);
+    const auto fields = chk_fields(s);
+    bool ok = got == 14 && std::strcmp(magic, CHK_MAGIC) == 0;
+    for (const auto& f : fields) {
+        int id = -1;
+        long n = -1;
+        ok = ok && std::fscanf(h, "%d %ld", &id, &n) == 2 && id == f.first && n == f.second;
+    }
+    std::fclose(h);
+    if (!ok || nx != P.L.nx || ny != P.L.ny || nz != P.L.nz || ng != P.L.ng || sy != P.L.sy || sz != P.L.sz ||
+        ne != P.L.ne || nc != P.cfg.ncons || mt != P.cfg.moisture_type || ut != P.cfg.use_terrain ||
+        (nray != 0 && nray != 4 * (size_t)(nz + 1))) {
+        set_error("checkpoint " + d + " does not match this sim's grid, components or fields");
+        return ERFX_ERR_ARG;
+    }
+    FILE* b = std::fopen((d + "/state.bin").c_str(), "rb");
+    if (!b) { set_error("cannot read " + d + "/state.bin"); return ERFX_ERR_ARG; }
+    std::vector<double> host;
+    for (const auto& f : fields) {
+        host.resize((size_t)f.second);
+        if (std::fread(host.data(), sizeof(double), host.size(), b) != host.size()) {
+            std::fclose(b); set_error("short read of " + d + "/state.bin"); return ERFX_ERR_ARG;
+        }
+        int rc = erfx_sim_upload(s, f.first, host.data());   // node heights remake the metrics
+        if (rc) { std::fclose(b); return rc; }
+    }
+    std::vector<double> rp(nray);
+    if (nray && std::fread(rp.data(), sizeof(double), nray, b) != nray) {
+        std::fclose(b); set_error("short read of " + d + "/state.bin"); return ERFX_ERR_ARG;
+    }
+    std::fclose(b);
+    if (nray) {
+        const size_t n1 = (size_t)nz + 1;
+        int rc = erfx_set_rayleigh(s->ctx, rp.data(), rp.data() + n1, rp.data() + 2 * n1, rp.data() + 3 * n1);
+        if (rc) return rc;
+    }
+    *time = t;
+    *step = st;
+    return ERFX_OK;
 }
 
 int erfx_sim_sync(erfx_sim* s) {

@@ -79,7 +79,7 @@ struct TerrainWork {
 void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int step, int nrk,
                 const erfx_state& slow_rhs, const erfx_state& prev, const erfx_state& stage, const double* prim,
                 const double* pi_stage, const double* fc, const erfx_state& data, const erfx_state& scratch,
-                double dtau, double beta_s, double facinv, hipStream_t s);
+                double dtau, double beta_s, double facinv, hipStream_t s, const FastChain& ch = FastChain());
 // make_J, make_areas, make_zcc (ERF_TerrainMetrics.cpp:583-715) on the whole
 // layout: detJ, ax, ay, az = 1 and z_cc = 0 outside the boxes they are made on
 // (and mrx, mry: the ratios h_xi / h_zeta, h_eta / h_zeta of the pressure

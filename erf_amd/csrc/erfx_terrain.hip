@@ -64,7 +64,7 @@ void terrain_metrics(const Params& P, const double* z_nd, double* detJ, double* 
 void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int step, int nrk,
                 const erfx_state& slow_rhs, const erfx_state& prev, const erfx_state& stage, const double* prim,
                 const double* pi_stage, const double* fc, const erfx_state& data, const erfx_state& scratch,
-                double dtau, double beta_s, double facinv, hipStream_t s) {
+                double dtau, double beta_s, double facinv, hipStream_t s, const FastChain& ch) {
     (void)nrk;
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -92,6 +92,35 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     const double* pxm = prev.xmom; const double* pym = prev.ymom; const double* pzm = prev.zmom;
     const double* sxm = stage.xmom; const double* sym = stage.ymom; const double* szm = stage.zmom;
     const bool first = step == 0;
+    // The substep chain (laterally periodic box, plane top): prev is S_sum as the
+    // previous substep left it, without the apply_bcs between the two.  What
+    // that apply_bcs would give, the kernels form as they read: ghost cells of
+    // rho*theta are the periodic images (foextrap rows above and below), and
+    // each momentum is put through the momentum -> velocity -> momentum round
+    // trip with the face density of its periodic image
+    // (ERF_MomentumToVelocity.cpp, ERF_VelocityToMomentum.cpp), so the rounding
+    // is the reference's.
+    const bool rt = ch.rt_prev && step > 0;
+    const double* prho = prev.cons;
+    auto wrap_c = [=] __device__(int a, int n) { return a < 0 ? a + n : (a > n - 1 ? a - n : a); };
+    auto wrap_f = [=] __device__(int a, int n) { return a < 0 ? a + n : (a > n ? a - n : a); };
+    // the cell holding the value of (possibly ghost) cell (i, j, k)
+    auto cimg = [=] __device__(int i, int j, int k) {
+        return L.ix(wrap_c(i, nx), wrap_c(j, ny), k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k));
+    };
+    // round trip of x-momentum on (possibly ghost) face (i, j, k), k valid
+    auto rt_x = [=] __device__(int i, int j, int k) {
+        const int fi = wrap_f(i, nx), cj = wrap_c(j, ny);
+        const double d = prho[L.ix(wrap_c(fi, nx), cj, k)] + prho[L.ix(wrap_c(fi - 1, nx), cj, k)];
+        const double u = pxm[L.ix(fi, cj, k)] * 2.0 / d;
+        return u * 0.5 * d;
+    };
+    auto rt_y = [=] __device__(int i, int j, int k) {
+        const int ci = wrap_c(i, nx), fj = wrap_f(j, ny);
+        const double d = prho[L.ix(ci, wrap_c(fj, ny), k)] + prho[L.ix(ci, wrap_c(fj - 1, ny), k)];
+        const double v = pym[L.ix(ci, fj, k)] * 2.0 / d;
+        return v * 0.5 * d;
+    };
 
     // :139-204: rho, rho*theta copied from prev on the grown box (step 0);
     // theta extrapolated forward and the lagged increment; the horizontal
@@ -102,18 +131,18 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         const long q = L.ix(i, j, k);
         if (i <= nx) {
             if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
-            double dt_ = cc1[q] - sc1[q];
+            double dt_ = (rt ? cc1[cimg(i, j, k)] : cc1[q]) - sc1[q];
             extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
             lagged1[q] = dt_;
         }
-        if (k >= 0) du[q] = pxm[q] - sxm[q];
-        if (k >= 0 && i <= nx) dv[q] = pym[q] - sym[q];
+        if (k >= 0) du[q] = (rt ? rt_x(i, j, k) : pxm[q]) - sxm[q];
+        if (k >= 0 && i <= nx) dv[q] = (rt ? rt_y(i, j, k) : pym[q]) - sym[q];
     });
     // the top layer of cells (k = nz)
     launch_box(Box(-1, -1, nz, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
-        double dt_ = cc1[q] - sc1[q];
+        double dt_ = (rt ? cc1[cimg(i, j, k)] : cc1[q]) - sc1[q];
         extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
         lagged1[q] = dt_;
     });
@@ -191,10 +220,19 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         const long q0 = L.ix(i, j, 0);
         // Omega of the old increments on face kf (0 at the bottom, the rho*w
         // increment at the top, OmegaFromW between)
+        // prev rho*w on interior face kf (round trip with the two cells' density
+        // in the substep chain); the top face keeps what the last substep wrote
+        auto pw = [&](int kf) -> double {
+            const long q = q0 + (long)kf * sz;
+            if (!rt || kf >= nz) return pzm[q];
+            const double d = prho[q] + prho[q - sz];
+            const double w = pzm[q] * 2.0 / d;
+            return w * 0.5 * d;
+        };
         auto omega_old = [&](int kf) -> double {
             const long q = q0 + (long)kf * sz;
             if (kf == 0) return 0.;
-            const double dwk = pzm[q] - szm[q];
+            const double dwk = pw(kf) - szm[q];
             if (kf == nz) return dwk;
             return met::omega_from_w(i, j, kf, dwk, fdu, fdv, z, dxi, dyi);
         };
@@ -236,7 +274,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             const double drho_k = cc0[q] - sc0[q], drt_k = cc1[q] - sc1[q];
             const double q1_k = moist ? pq1[q] : 0.0, q2_k = moist ? pq2[q] : 0.0;
             const double fP_k = fP[q], fQ_k = fQ[q], fA_k = fA[q], fB_k = fB[q], srz_k = srz[q];
-            const double dw_k = pzm[q] - szm[q];
+            const double dw_k = pw(k) - szm[q];
             const double om_kp1 = omega_old(k + 1);
             double t0_k, t1_k;
             trhs_at(k, t0_k, t1_k);

@@ -85,6 +85,8 @@ SIGNATURES = {
     "erfx_sim_integrals": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "erfx_sim_timer_reset": (_I, [_P]),
     "erfx_sim_timer_stop": (_I, [_P]),
+    "erfx_sim_write_checkpoint": (_I, [_P, ctypes.c_char_p, _D, _I]),
+    "erfx_sim_read_checkpoint": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "erfx_sim_timer_read": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(_D)]),
     "erfx_ensemble_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
@@ -193,6 +195,16 @@ class Sim:
         r = _I()
         check(lib().erfx_sim_est_dt(self.h, ctypes.byref(dtc), ctypes.byref(dt), ctypes.byref(r)))
         return dt.value, r.value
+
+    def write_checkpoint(self, path, time, step):
+        """The persistent state into directory `path` (erfx_sim_write_checkpoint)."""
+        check(lib().erfx_sim_write_checkpoint(self.h, path.encode(), time, step))
+
+    def read_checkpoint(self, path):
+        """Restore the state written by write_checkpoint; returns (time, step)."""
+        t, n = _D(), _I()
+        check(lib().erfx_sim_read_checkpoint(self.h, path.encode(), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
 
     def step_async(self, t0, dt, n):
         check(lib().erfx_sim_step_async(self.h, t0, dt, n))

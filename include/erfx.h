@@ -481,6 +481,18 @@ int  erfx_sim_run(erfx_sim* sim, double t0, int max_steps, const erfx_dt_control
  * exactly rounded sum of the per-cell terms, independent of reduction order. */
 int  erfx_sim_integrals(erfx_sim* sim, double* mass, double* rhotheta, double* scalar);
 
+/* Checkpoint and restart (the role of ERF::WriteCheckpointFile /
+ * ReadCheckpointFile, Source/IO/ERF_Checkpoint.cpp:25-590): the sim's
+ * persistent state -- cons, the three velocities, the base state, the node
+ * heights of a terrain grid, the accumulated surface rain and the Rayleigh
+ * profiles; every other buffer is made again from these at the start of a
+ * step -- written to directory `dir` (created if absent) as a text Header and
+ * one raw FP64 file, with the caller's time and step.  Reading it into a sim
+ * made from the same configuration and stepping on gives the results of the
+ * run that wrote it, bit for bit; a checkpoint of another grid is refused. */
+int  erfx_sim_write_checkpoint(erfx_sim* sim, const char* dir, double time, int step);
+int  erfx_sim_read_checkpoint(erfx_sim* sim, const char* dir, double* time, int* step);
+
 /* Microphysics of the sim's steps: the network (erfx_kessler_mlp_advance) when
  * m is non-NULL, Kessler otherwise; counts of network and fallback cells
  * summed over the steps since it was set. */
