@@ -10,44 +10,17 @@ reference is imported or executed.
 """
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
 
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+from erf_amd import plotfile  # noqa: E402
+
 
 def read_plotfile(path):
-    with open(os.path.join(path, "Header")) as f:
-        lines = f.read().split("\n")
-    assert lines[0].startswith("HyperCLaw")
-    nvar = int(lines[1])
-    names = lines[2:2 + nvar]
-    pos = 2 + nvar
-    dim = int(lines[pos]); time = float(lines[pos + 1])
-    lo = [float(x) for x in lines[pos + 3].split()]
-    hi = [float(x) for x in lines[pos + 4].split()]
-    dom = re.findall(r"-?\d+", lines[pos + 6])
-    n = [int(dom[3]) - int(dom[0]) + 1, int(dom[4]) - int(dom[1]) + 1, int(dom[5]) - int(dom[2]) + 1]
-    nstep = int(lines[pos + 7])
-    data = np.full((nvar, n[2], n[1], n[0]), np.nan)
-    with open(os.path.join(path, "Level_0", "Cell_H")) as f:
-        ch = f.read()
-    boxes = re.findall(r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\)", ch)
-    fabs = re.findall(r"FabOnDisk: (\S+) (\d+)", ch)
-    for (b, (fname, off)) in zip(boxes, fabs):
-        b = [int(x) for x in b]
-        with open(os.path.join(path, "Level_0", fname), "rb") as f:
-            f.seek(int(off))
-            head = f.readline().decode()
-            m = re.match(r"FAB \(\(8, \(([\d ]+)\)\),\(8, \(([\d ]+)\)\)\)\(\((-?\d+),(-?\d+),(-?\d+)\) "
-                         r"\((-?\d+),(-?\d+),(-?\d+)\) \((\d+),(\d+),(\d+)\)\) (\d+)", head)
-            assert m, head
-            order = m.group(2).split()
-            dt = np.dtype("<f8") if order == list("87654321") else np.dtype(">f8")
-            nc = int(m.group(12))
-            sh = (b[5] - b[2] + 1, b[4] - b[1] + 1, b[3] - b[0] + 1)
-            arr = np.frombuffer(f.read(8 * nc * sh[0] * sh[1] * sh[2]), dtype=dt).reshape((nc,) + sh)
-        data[:, b[2]:b[5] + 1, b[1]:b[4] + 1, b[0]:b[3] + 1] = arr
-    return dict(names=names, time=time, lo=lo, hi=hi, n=n, nstep=nstep, data=data, cell_h=ch)
+    return plotfile.read(path)
 
 
 SMALL = ["ScalarDiffusionSine", "ScalarDiffusionGaussian", "ScalarAdvectionUniformU", "ScalarAdvectionShearedU",
@@ -74,6 +47,9 @@ def main():
             save_small(ref, case)
     if only:
         return
+    # the text Header of one gold plotfile, to check the writer's format against
+    shutil.copyfile(os.path.join(ref, "Tests", "ERFGoldFiles", "ScalarAdvDiff_order2", "Header"),
+                    os.path.join(os.path.dirname(os.path.abspath(__file__)), "ScalarAdvDiff_order2.Header"))
     pf = read_plotfile(os.path.join(ref, "Tests", "ERFGoldFiles", "DensityCurrent"))
     d = pf["data"]
     assert np.all(np.isfinite(d))
