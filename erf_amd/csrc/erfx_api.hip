@@ -179,6 +179,11 @@ struct erfx_ctx {
     std::vector<void*> allocs;
 };
 
+namespace erfx {
+const Params& ctx_params(const erfx_ctx* c) { return c->P; }
+hipStream_t ctx_stream(const erfx_ctx* c) { return c->stream; }
+}  // namespace erfx
+
 // ---- learned microphysics ------------------------------------------------------
 struct erfx_mlp {
     Net net;
@@ -465,6 +470,8 @@ int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
 
 int erfx_ctx_destroy(erfx_ctx* c) {
     if (!c) return ERFX_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    views_release(c);
     if (c->ecount_host) (void)hipHostFree(c->ecount_host);
     for (void* p : c->allocs) (void)hipFree(p);
     delete c;

@@ -38,8 +38,12 @@ struct CopyList {
 void copy_fields(const CopyList& c, hipStream_t s);
 
 // erfx_api.hip: the checks of erfx_ctx_create on a configuration (status code,
-// erfx_last_error text)
+// erfx_last_error text); a context's run constants and stream
 int validate_config(const erfx_config& c);
+const Params& ctx_params(const erfx_ctx* c);
+hipStream_t ctx_stream(const erfx_ctx* c);
+// erfx_views.hip: free the staging fields of a context's view hooks
+void views_release(const erfx_ctx* c);
 
 // erfx_fast.hip
 struct FastWork {

@@ -24,8 +24,32 @@ class State(ctypes.Structure):
     _fields_ = [("cons", _P), ("xmom", _P), ("ymom", _P), ("zmom", _P)]
 
 
+class View(ctypes.Structure):
+    """erfx_view: one field of one box as ERF allocates it (an Array4)."""
+    _fields_ = [("p", _P), ("lo", _I * 3), ("hi", _I * 3), ("js", ctypes.c_long), ("ks", ctypes.c_long),
+                ("ns", ctypes.c_long), ("ncomp", _I), ("ixtype", _I * 3)]
+
+
+class StateView(ctypes.Structure):
+    """erfx_state_view: views of {cons, xmom, ymom, zmom}."""
+    _fields_ = [("cons", View), ("xmom", View), ("ymom", View), ("zmom", View)]
+
+
+_V = ctypes.POINTER(View)
+_SV = ctypes.POINTER(StateView)
+
 # every symbol include/erfx.h declares, with its ctypes signature
 SIGNATURES = {
+    "erfx_check_view": (_I, [ctypes.POINTER(C.Config), _I, _V]),
+    "erfx_cons_to_prim_v": (_I, [_P, _V, _V, _V, _I]),
+    "erfx_apply_bcs_v": (_I, [_P, _SV, _V, _V, _V, _I, _I, _I, _I]),
+    "erfx_make_fast_coeffs_v": (_I, [_P, _V, _SV, _V, _V, _D, _D]),
+    "erfx_fast_rhs_N_v": (_I, [_P, _I, _I, _SV, _SV, _SV, _V, _V, _V, _SV, _SV, _D, _D, _D]),
+    "erfx_fast_rhs_T_v": (_I, [_P, _I, _I, _SV, _SV, _SV, _V, _V, _V, _SV, _SV, _D, _D, _D]),
+    "erfx_turbulent_viscosity_v": (_I, [_P, _V, _V, _V, _V, _V]),
+    "erfx_slow_rhs_pre_v": (_I, [_P, _I, _D, _SV, _SV, _V, _SV, _V, _V, _V, _V]),
+    "erfx_slow_rhs_post_v": (_I, [_P, _I, _D, _SV, _SV, _SV, _SV, _V, _SV, _V]),
+    "erfx_kessler_advance_v": (_I, [_P, _V, _D]),
     "erfx_last_error": (ctypes.c_char_p, []),
     "erfx_config_defaults": (None, [ctypes.POINTER(C.Config)]),
     "erfx_ghost_cells": (_I, [ctypes.POINTER(C.Config)]),

@@ -323,6 +323,74 @@ int erfx_slow_rhs_post(erfx_ctx* ctx, int nrk, double slow_dt,
  * saturation fit's range (below -85 C), reported here as ERFX_ERR_STATE. */
 int erfx_kessler_advance(erfx_ctx* ctx, double* cons, double dt, double* rain_accum);
 
+/* ---- hooks on Array4-shaped views ---------------------------------------- *
+ * ERF holds each field of a box in its own FArrayBox: cons with
+ * ComputeGhostCells + 1 ghost cells, the face-centred momenta and velocities
+ * with ComputeGhostCells, each with its own strides
+ * (Source/ERF_make_new_arrays.cpp:112-130), reached through amrex::Array4.  A
+ * view describes one such field of one box: element (i, j, k, n) is
+ *     p[(i - lo[0]) + (j - lo[1]) * js + (k - lo[2]) * ks + n * ns]
+ * for lo <= (i, j, k) <= hi, with the indices of the valid box (cells 0 ..
+ * n-1; faces 0 .. n along a nodal direction), so lo is minus the ghost width.
+ * ixtype[d] = 1 where the field is nodal along d.  The _v hooks take views in
+ * place of the common-layout pointers: a view already in the common layout
+ * is used in place, any other is gathered into the context's staging field
+ * of that layout and scattered back after the hook when the hook writes it
+ * (views of the same memory share one staging field, as ERF's S_prev ==
+ * S_data).  Each view must cover the valid box grown by the ghost width its
+ * role needs (erfx_check_view), else the hook returns ERFX_ERR_ARG before any
+ * device work.  Results are those of the pointer hooks, bit for bit. */
+typedef struct erfx_view {
+    double* p;                    /* element (lo[0], lo[1], lo[2], 0)               */
+    int  lo[3], hi[3];            /* index box of the allocation                    */
+    long js, ks, ns;              /* strides in elements (i stride 1)               */
+    int  ncomp;
+    int  ixtype[3];               /* 1 = nodal along that direction                  */
+} erfx_view;
+typedef struct erfx_state_view { erfx_view cons, xmom, ymom, zmom; } erfx_state_view;
+
+/* what a view stands for: components, index type, ghost width needed
+ * (ng = erfx_ghost_cells(cfg)):
+ *   CONS ncons comps, cells, ng;  XMOM/YMOM/ZMOM and XVEL/YVEL/ZVEL 1 comp, nodal
+ *   in x / y / z, ng - 1;  PRIM ncons - 1 comps, cells, ng;  PI 1 comp, cells, 1;
+ *   FAST_COEFFS 5 comps, nodal in z, 0;  EDDY 13 comps, cells, 1 */
+enum {
+    ERFX_ROLE_CONS = 0, ERFX_ROLE_XMOM = 1, ERFX_ROLE_YMOM = 2, ERFX_ROLE_ZMOM = 3,
+    ERFX_ROLE_XVEL = 4, ERFX_ROLE_YVEL = 5, ERFX_ROLE_ZVEL = 6,
+    ERFX_ROLE_PRIM = 7, ERFX_ROLE_PI = 8, ERFX_ROLE_FAST_COEFFS = 9, ERFX_ROLE_EDDY = 10
+};
+
+/* Check a view against a role for boxes of this configuration (no device
+ * needed): ERFX_OK, or ERFX_ERR_ARG with the reason in erfx_last_error(). */
+int erfx_check_view(const erfx_config* cfg, int role, const erfx_view* v);
+
+/* The hooks above on views (same meaning, same argument order) */
+int erfx_cons_to_prim_v(erfx_ctx* ctx, const erfx_view* cons, erfx_view* prim, erfx_view* pi_stage, int ng);
+int erfx_apply_bcs_v(erfx_ctx* ctx, erfx_state_view* S, erfx_view* xvel, erfx_view* yvel, erfx_view* zvel,
+                     int ng_cons, int ng_vel, int fast_only, int vel_and_mom_synced);
+int erfx_make_fast_coeffs_v(erfx_ctx* ctx, erfx_view* fast_coeffs, const erfx_state_view* stage,
+                            const erfx_view* prim, const erfx_view* pi_stage, double dtau, double beta_s);
+int erfx_fast_rhs_N_v(erfx_ctx* ctx, int step, int nrk, const erfx_state_view* slow_rhs,
+                      const erfx_state_view* prev, const erfx_state_view* stage, const erfx_view* prim,
+                      const erfx_view* pi_stage, const erfx_view* fast_coeffs, erfx_state_view* data,
+                      erfx_state_view* scratch, double dtau, double beta_s, double facinv);
+int erfx_fast_rhs_T_v(erfx_ctx* ctx, int step, int nrk, const erfx_state_view* slow_rhs,
+                      const erfx_state_view* prev, const erfx_state_view* stage, const erfx_view* prim,
+                      const erfx_view* pi_stage, const erfx_view* fast_coeffs, erfx_state_view* data,
+                      erfx_state_view* scratch, double dtau, double beta_s, double facinv);
+int erfx_turbulent_viscosity_v(erfx_ctx* ctx, const erfx_view* cons, const erfx_view* xvel,
+                               const erfx_view* yvel, const erfx_view* zvel, erfx_view* eddy);
+/* eddy may be NULL when no LES model is configured */
+int erfx_slow_rhs_pre_v(erfx_ctx* ctx, int nrk, double slow_dt, erfx_state_view* rhs,
+                        const erfx_state_view* data, const erfx_view* prim, erfx_state_view* scratch,
+                        const erfx_view* xvel, const erfx_view* yvel, const erfx_view* zvel,
+                        const erfx_view* eddy);
+int erfx_slow_rhs_post_v(erfx_ctx* ctx, int nrk, double slow_dt, erfx_state_view* rhs,
+                         const erfx_state_view* old, erfx_state_view* snew, erfx_state_view* data,
+                         const erfx_view* prim, const erfx_state_view* scratch, const erfx_view* eddy);
+/* without the surface-rain accumulator */
+int erfx_kessler_advance_v(erfx_ctx* ctx, erfx_view* cons, double dt);
+
 /* ---- learned stand-in for the Kessler cell tendencies ------------------- *
  * No counterpart in ERF: a multilayer perceptron 8 -> 128 -> 128 -> 4 with GELU
  * activations, BF16 weights and activations, FP32 accumulation, trained on the

@@ -51,6 +51,8 @@ int main(void) {
     F(erfx_config, inflow_qc) F(erfx_config, prob_hi) F(erfx_config, no_substepping)
     F(erfx_config, wall_velocity) F(erfx_config, abl_pressure_grad)
     F(erfx_layout, sy) F(erfx_layout, nelem)
+    printf("erfx_view %zu\n", sizeof(erfx_view));
+    F(erfx_view, lo) F(erfx_view, hi) F(erfx_view, js) F(erfx_view, ns) F(erfx_view, ncomp) F(erfx_view, ixtype)
     return 0;
 }
 '''
@@ -67,11 +69,13 @@ def test_ctypes_structs_match_c_header():
     lay = _c_layout()
     assert lay["erfx_config"] == ctypes.sizeof(C.Config)
     assert lay["erfx_layout"] == ctypes.sizeof(C.Layout)
+    from erf_amd.lib import View
+    assert lay["erfx_view"] == ctypes.sizeof(View)
     for key, off in lay.items():
         if "." not in key:
             continue
         T, m = key.split(".")
-        cls = C.Config if T == "erfx_config" else C.Layout
+        cls = {"erfx_config": C.Config, "erfx_layout": C.Layout, "erfx_view": View}[T]
         assert getattr(cls, m).offset == off, key
 
 
@@ -105,3 +109,49 @@ def test_null_arguments_are_rejected_without_touching_the_device():
     assert b"null" in lib().erfx_last_error()
     assert lib().erfx_sim_create(None, None) == -1
     assert lib().erfx_ghost_cells(None) == -1
+
+
+def _view(lo, hi, ncomp, ixtype, p=1):
+    from erf_amd.lib import View
+    v = View()
+    v.p = p
+    for d in range(3):
+        v.lo[d], v.hi[d], v.ixtype[d] = lo[d], hi[d], ixtype[d]
+    nx, ny, nz = (hi[d] - lo[d] + 1 for d in range(3))
+    v.js, v.ks, v.ns = nx, nx * ny, nx * ny * nz
+    v.ncomp = ncomp
+    return v
+
+
+def test_view_checks_without_gpu():
+    """erfx_check_view: ERF's own allocation of each field passes (cons with
+    ng ghosts, momenta with ng - 1 and one more face along their direction);
+    too few ghosts, a wrong index type, too few components, a missing pointer
+    or overlapping strides give ERFX_ERR_ARG with the reason."""
+    from erf_amd.lib import lib
+    cfg = C.default_config(16, 12, 10, 100.0, 100.0, 100.0)
+    ng = lib().erfx_ghost_cells(ctypes.byref(cfg))
+    n = (16, 12, 10)
+    cons = _view([-ng] * 3, [n[d] - 1 + ng for d in range(3)], cfg.ncons, [0, 0, 0])
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(cons)) == 0
+    g = ng - 1
+    for role, d in ((1, 0), (2, 1), (3, 2), (4, 0), (5, 1), (6, 2)):
+        ix = [1 if e == d else 0 for e in range(3)]
+        mom = _view([-g] * 3, [n[e] - 1 + ix[e] + g for e in range(3)], 1, ix)
+        assert lib().erfx_check_view(ctypes.byref(cfg), role, ctypes.byref(mom)) == 0, role
+        short = _view([-g] * 3, [n[e] - 1 + g for e in range(3)], 1, ix)       # the extra face missing
+        assert lib().erfx_check_view(ctypes.byref(cfg), role, ctypes.byref(short)) == -1
+        assert b"does not cover" in lib().erfx_last_error()
+    thin = _view([-(ng - 1)] * 3, [n[d] - 1 + ng - 1 for d in range(3)], cfg.ncons, [0, 0, 0])
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(thin)) == -1
+    wrong = _view([-ng] * 3, [n[d] + ng for d in range(3)], cfg.ncons, [1, 0, 0])
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(wrong)) == -1
+    assert b"index type" in lib().erfx_last_error()
+    few = _view([-ng] * 3, [n[d] - 1 + ng for d in range(3)], cfg.ncons - 1, [0, 0, 0])
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(few)) == -1
+    assert b"components" in lib().erfx_last_error()
+    cons.js = 3
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(cons)) == -1
+    assert b"strides" in lib().erfx_last_error()
+    cons.p = None
+    assert lib().erfx_check_view(ctypes.byref(cfg), 0, ctypes.byref(cons)) == -1
