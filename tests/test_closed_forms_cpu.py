@@ -199,3 +199,42 @@ def test_face_interp_textbook_weights(type_):
             want = float(np.dot(ww, s))
             got = _face(type_, s, upw)
             assert abs(got - want) <= 1e-14 * max(1.0, np.abs(s).max()), (type_, upw, got, want)
+
+
+# ---- prescribed surface flux of rho*theta ------------------------------------------
+
+def test_surface_heat_flux_enters_the_lowest_cell_only():
+    """A uniform atmosphere at rest with molecular diffusion of theta and a
+    prescribed surface flux F of rho*theta (the explicit-MOST branch of
+    DiffusionSrcForState_N, ERF_DiffusionSrcForState_N.cpp:300-301): every
+    diffusive flux vanishes but the one through the bottom face, so the slow
+    tendency of rho*theta is F / dz in the lowest cells and 0 above."""
+    nx, ny, nz, dz = 6, 5, 8, 12.5
+    cfg = C.default_config(nx, ny, nz, 40.0, 40.0, dz)
+    cfg.use_gravity = 0
+    cfg.molec_diff_type = C.MOLEC_CONSTANT
+    cfg.alpha_T, cfg.rho0_trans = 3.0, 1.0
+    cfg.use_surface_heat_flux = 1
+    F = 0.11
+    cfg.surface_heat_flux = F
+    L = C.layout_for(cfg)
+    orc = Oracle(cfg)
+    rho, theta = 1.1, 300.0
+    cons = np.zeros((cfg.ncons,) + L.shape3())
+    cons[C.RHO], cons[C.RHOTHETA] = rho, rho * theta
+    p = C.p_0 * (C.R_d * rho * theta / C.p_0) ** C.Gamma
+    one = np.ones((1,) + L.shape3())
+    orc.set(C.F_R0, rho * one)
+    orc.set(C.F_P0, p * one)
+    orc.set(C.F_PI0, (p / C.p_0) ** (C.R_d / C.Cp_d) * one)
+    orc.set(C.state_id(C.S_NEW, 0), cons)
+    zero = np.zeros((1,) + L.shape3())
+    for fid in (C.F_XVEL, C.F_YVEL, C.F_ZVEL):
+        orc.set(fid, zero)
+    for c in (1, 2, 3):
+        orc.set(C.state_id(C.S_NEW, c), zero)
+    orc.call("orc_cons_to_prim", C.state_id(C.S_NEW, 0), L.ng)
+    orc.call("orc_slow_rhs_pre", 0, 1.0)
+    r = valid(orc.get(C.state_id(C.F_SLOW, 0)), cfg, L, C.F_CONS)[C.RHOTHETA]
+    assert np.all(r[0] == F / dz), (r[0].min(), r[0].max(), F / dz)
+    assert np.all(r[1:] == 0.0), np.abs(r[1:]).max()
