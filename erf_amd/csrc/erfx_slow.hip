@@ -163,22 +163,26 @@ __device__ __forceinline__ double interp_scalar_at(int t, const double* f, long 
 // FP32 error scales with the local variation, not with the value itself.
 namespace wenof {
 constexpr float eps = 1.0e-6f;
+// the hardware reciprocal (v_rcp_f32, 1 ulp): the FP32 weights need no
+// correctly rounded division, and this form is a few instructions instead of
+// the dozen of an IEEE division
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float w3(float sm1, float s, float sp1) {
     const float g1 = (1.0f / 3.0f), g2 = (2.0f / 3.0f);
     float b1 = (s - sm1) * (s - sm1);
     float b2 = (sp1 - s) * (sp1 - s);
-    float w1 = g1 / ((eps + b1) * (eps + b1));
-    float w2 = g2 / ((eps + b2) * (eps + b2));
-    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) / (2.0f * (w1 + w2)));
+    float w1 = g1 * rcp((eps + b1) * (eps + b1));
+    float w2 = g2 * rcp((eps + b2) * (eps + b2));
+    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) * rcp(2.0f * (w1 + w2)));
 }
 __device__ __forceinline__ float z3(float sm1, float s, float sp1) {
     const float g1 = (1.0f / 3.0f), g2 = (2.0f / 3.0f);
     float b1 = (s - sm1) * (s - sm1);
     float b2 = (sp1 - s) * (sp1 - s);
     float t5 = fabsf(b2 - b1);
-    float w1 = g1 * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
-    float w2 = g2 * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
-    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) / (2.0f * (w1 + w2)));
+    float w1 = g1 * (1.0f + (t5 * t5) * rcp((eps + b1) * (eps + b1)));
+    float w2 = g2 * (1.0f + (t5 * t5) * rcp((eps + b2) * (eps + b2)));
+    return ((w1 * (-sm1 + 3.0f * s) + w2 * (s + sp1)) * rcp(2.0f * (w1 + w2)));
 }
 __device__ __forceinline__ float mzq3(float sm1, float s, float sp1) {
     const float g = (1.0f / 3.0f);
@@ -186,9 +190,9 @@ __device__ __forceinline__ float mzq3(float sm1, float s, float sp1) {
     float b2 = (sp1 - s) * (sp1 - s);
     float b3 = ((13.0f / 12.0f) * ((sm1 - 2.0f * s + sp1) * (sm1 - 2.0f * s + sp1))) + (((sp1 - sm1) * (sp1 - sm1)) / 4.0f);
     float t5 = (fabsf(b3 - b1) + fabsf(b3 - b2)) / 32.0f;
-    float a1 = g * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
-    float a2 = g * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
-    float a3 = g * (1.0f + (t5 * t5) / ((eps + b3) * (eps + b3)));
+    float a1 = g * (1.0f + (t5 * t5) * rcp((eps + b1) * (eps + b1)));
+    float a2 = g * (1.0f + (t5 * t5) * rcp((eps + b2) * (eps + b2)));
+    float a3 = g * (1.0f + (t5 * t5) * rcp((eps + b3) * (eps + b3)));
     float asum = a1 + a2 + a3;
     float v1 = (-sm1 + 3.0f * s) / 2.0f, v2 = (s + sp1) / 2.0f, v3 = (-sm1 + 5.0f * s + 2.0f * sp1) / 6.0f;
     return ((a3 / asum / g) * (v3 - g * v1 - g * v2) + (a1 / asum) * v1 + (a2 / asum) * v2);
@@ -204,18 +208,18 @@ __device__ __forceinline__ float w5(float sm2, float sm1, float s, float sp1, fl
     float w1, w2, w3_;
     if (z) {
         float t5 = fabsf(b3 - b1);
-        w1 = g1 * (1.0f + (t5 * t5) / ((eps + b1) * (eps + b1)));
-        w2 = g2 * (1.0f + (t5 * t5) / ((eps + b2) * (eps + b2)));
-        w3_ = g3 * (1.0f + (t5 * t5) / ((eps + b3) * (eps + b3)));
+        w1 = g1 * (1.0f + (t5 * t5) * rcp((eps + b1) * (eps + b1)));
+        w2 = g2 * (1.0f + (t5 * t5) * rcp((eps + b2) * (eps + b2)));
+        w3_ = g3 * (1.0f + (t5 * t5) * rcp((eps + b3) * (eps + b3)));
     } else {
-        w1 = g1 / ((eps + b1) * (eps + b1));
-        w2 = g2 / ((eps + b2) * (eps + b2));
-        w3_ = g3 / ((eps + b3) * (eps + b3));
+        w1 = g1 * rcp((eps + b1) * (eps + b1));
+        w2 = g2 * rcp((eps + b2) * (eps + b2));
+        w3_ = g3 * rcp((eps + b3) * (eps + b3));
     }
     float v1 = 2.0f * sm2 - 7.0f * sm1 + 11.0f * s;
     float v2 = -sm1 + 5.0f * s + 2.0f * sp1;
     float v3 = 2.0f * s + 5.0f * sp1 - sp2;
-    return ((w1 * v1 + w2 * v2 + w3_ * v3) / (6.0f * (w1 + w2 + w3_)));
+    return ((w1 * v1 + w2 * v2 + w3_ * v3) * rcp(6.0f * (w1 + w2 + w3_)));
 }
 __device__ __forceinline__ float face(int t, float sp2, float sp1, float s, float sm1, float sm2, float sm3, double upw) {
     if (!(upw > weno::tol) && !(upw < -weno::tol)) return 0.5f * (s + sm1);
@@ -1218,6 +1222,150 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     });
 }
 
+// ---- k-marching launch that evaluates every face once --------------------------
+// A flux-form update needs, per cell, the fluxes through its six faces; a
+// kernel with one lane per cell evaluates every interior face twice (as the
+// hi face of one cell and the lo face of the next), and with WENO5 the face
+// evaluation is most of the work.  Here a work-group covers a tile of 64
+// cells in x, MR rows in y and up to MK levels in z: lane (lx, ly < MR) owns
+// cell (i0 + lx, j0 + ly) and marches up through the levels.  At each level
+// it evaluates its cell's lo x-face and lo y-face into LDS and its hi z-face
+// into registers (the lo z-face is the previous level's hi face); an extra
+// wave (ly = MR) evaluates the lo y-faces of row j0 + MR and, in MR of its
+// lanes, the x-face i0 + 64 of each row.  After one barrier each cell reads
+// its hi x- and y-faces from LDS.  LDS is double-buffered by level, so one
+// barrier per level suffices.  A face's value is computed by the same code
+// from the same inputs as in the per-cell kernel, so results are identical.
+constexpr int MARCH_MR = 4;    // rows of cells per work-group
+constexpr int MARCH_MK = 16;   // levels per work-group
+
+template <class Op>
+__global__ void __launch_bounds__(64 * (MARCH_MR + 1))
+march_kernel(const Op op, int nx, int ny, int nz, unsigned gx, unsigned gy) {
+    constexpr int NF = Op::NF;
+    constexpr int MR = MARCH_MR;
+    __shared__ double xs[2][NF][MR][65];
+    __shared__ double ys[2][NF][MR + 1][64];
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int i = (int)bx * 64 + lx;
+    const int j0 = (int)by * MR;
+    const int k0 = (int)bz * MARCH_MK;
+    const int k1 = min(k0 + MARCH_MK, nz);
+    const bool row = ly < MR;
+    const int j = j0 + ly;                    // cell row (row lanes) or y-face row j0 + MR
+    const bool cellok = row && i < nx && j < ny;
+    double gzl[NF], gzh[NF];
+    if (cellok) op.face_z(i, j, k0, gzl);
+    int b = 0;
+    for (int k = k0; k < k1; ++k) {
+        double v[NF];
+        if (row) {
+            if (i <= nx && j < ny) op.face_x(i, j, k, v);
+            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
+            for (int m = 0; m < NF; ++m) xs[b][m][ly][lx] = v[m];
+            if (i < nx && j <= ny) op.face_y(i, j, k, v);
+            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
+            for (int m = 0; m < NF; ++m) ys[b][m][ly][lx] = v[m];
+            if (cellok) op.face_z(i, j, k + 1, gzh);
+        } else {
+            if (i < nx && j <= ny) op.face_y(i, j, k, v);
+            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
+            for (int m = 0; m < NF; ++m) ys[b][m][MR][lx] = v[m];
+            if (lx < MR) {
+                const int ix = (int)bx * 64 + 64, jr = j0 + lx;
+                if (ix <= nx && jr < ny) op.face_x(ix, jr, k, v);
+                else for (int m = 0; m < NF; ++m) v[m] = 0.0;
+                for (int m = 0; m < NF; ++m) xs[b][m][lx][64] = v[m];
+            }
+        }
+        __syncthreads();
+        if (cellok) {
+            double gx0[NF], gx1[NF], gy0[NF], gy1[NF];
+            for (int m = 0; m < NF; ++m) {
+                gx0[m] = xs[b][m][ly][lx];
+                gx1[m] = xs[b][m][ly][lx + 1];
+                gy0[m] = ys[b][m][ly][lx];
+                gy1[m] = ys[b][m][ly + 1][lx];
+            }
+            op.cell(i, j, k, gx0, gx1, gy0, gy1, gzl, gzh);
+            for (int m = 0; m < NF; ++m) gzl[m] = gzh[m];
+        }
+        b ^= 1;
+    }
+}
+
+template <class Op>
+inline void launch_march(const Op& op, int nx, int ny, int nz, hipStream_t s) {
+    const unsigned gx = (unsigned)((nx + 63) / 64), gy = (unsigned)((ny + MARCH_MR - 1) / MARCH_MR);
+    const unsigned gz = (unsigned)((nz + MARCH_MK - 1) / MARCH_MK);
+    hipLaunchKernelGGL(march_kernel<Op>, dim3(gx * gy * gz), dim3(64, MARCH_MR + 1, 1), 0, s, op, nx, ny, nz, gx, gy);
+}
+
+// The scalar update of erf_slow_rhs_post as a marching operator: NF scalars
+// of one block, face values as in post_block below.
+template <bool mixed, int NF_>
+struct PostOp {
+    static constexpr int NF = NF_;
+    Lay L;
+    int nx, ny;
+    double dxi, dyi, dzi;
+    const double* Jc;
+    int ht, vt;
+    double hf, vf;
+    bool c2c2, use_diff, consA, turb, clip, copy_slow;
+    double ae, slow_dt;
+    bool dxlo, dxhi, dylo, dyhi;
+    const double* mu_h;
+    const double* mu_v;
+    int n0, ncons;
+    const double *ax, *ay, *az;
+    double* cur;
+    double* nc;
+    const double* oc;
+    double* crb;
+    const double* prim;
+    __device__ __forceinline__ void face(const double* a, int t, double fr, long q, long st, double* g) const {
+        const double f = a[q];
+        for (int m = 0; m < NF; ++m) {
+            const double* ps = prim + (n0 + m - 1) * L.ne;
+            g[m] = c2c2 ? f * (0.5 * (ps[q] + ps[q - st])) : f * interp_scalar_pick(mixed, t, ps, q, st, f, fr);
+        }
+    }
+    __device__ __forceinline__ void face_x(int i, int j, int k, double* g) const { face(ax, ht, hf, L.ix(i, j, k), 1, g); }
+    __device__ __forceinline__ void face_y(int i, int j, int k, double* g) const { face(ay, ht, hf, L.ix(i, j, k), L.sy, g); }
+    __device__ __forceinline__ void face_z(int i, int j, int k, double* g) const { face(az, vt, vf, L.ix(i, j, k), L.sz, g); }
+    __device__ __forceinline__ void cell(int i, int j, int k, const double* gx0, const double* gx1, const double* gy0,
+                                         const double* gy1, const double* gz0, const double* gz1) const {
+        const long q = L.ix(i, j, k);
+        const long ne = L.ne, sy = L.sy, sz = L.sz;
+        if (copy_slow)
+            for (int c = 2; c < ncons; ++c) cur[q + c * ne] = nc[q + c * ne];
+        for (int m = 0; m < NF; ++m) {
+            const int n = n0 + m;
+            const double* ps = prim + (n - 1) * ne;
+            double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
+            double r = -invdetJ * (1.0 * 1.0) * ((gx1[m] - gx0[m]) * dxi + (gy1[m] - gy0[m]) * dyi + (gz1[m] - gz0[m]) * dzi);
+            if (use_diff) {
+                const int dsx0 = (dxlo && i == 0) ? -1 : 0, dsx1 = (dxhi && i == nx - 1) ? 1 : 0;
+                const int dsy0 = (dylo && j == 0) ? -1 : 0, dsy1 = (dyhi && j == ny - 1) ? 1 : 0;
+                double dfx0 = diff_flux_bc(nc, ps, mu_h, q, 1, dxi, ae, consA, turb, dsx0) * 1.0;
+                double dfx1 = diff_flux_bc(nc, ps, mu_h, q + 1, 1, dxi, ae, consA, turb, dsx1) * 1.0;
+                double dfy0 = diff_flux_bc(nc, ps, mu_h, q, sy, dyi, ae, consA, turb, dsy0) * 1.0;
+                double dfy1 = diff_flux_bc(nc, ps, mu_h, q + sy, sy, dyi, ae, consA, turb, dsy1) * 1.0;
+                double dfz0 = diff_flux(nc, ps, mu_v, q, sz, dzi, ae, consA, turb);
+                double dfz1 = diff_flux(nc, ps, mu_v, q + sz, sz, dzi, ae, consA, turb);
+                r -= (dfx1 - dfx0) * dxi * 1.0 + (dfy1 - dfy0) * dyi * 1.0 + (dfz1 - dfz0) * dzi;
+            }
+            crb[q + n * ne] = r;
+            double v = oc[q + n * ne] + slow_dt * r;
+            if (clip) v = (v < 0.0) ? 0.0 : v;   // amrex::max(v, 0.0)
+            cur[q + n * ne] = v;
+        }
+    }
+};
+
 // One block of slow scalars of erf_slow_rhs_post (ERF_slow_rhs_post.cpp:364-497):
 // advection with the time-averaged momenta (AdvectionSrcForScalars), diffusion
 // (DiffusionSrcForState_N), then cur = old + dt * rhs, clipped at 0 for the
@@ -1259,6 +1407,30 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     double* nc = snew.cons;
     const double* oc = old.cons;
     double* crb = rhs.cons;
+    if (!g_rec && (nc_b == 1 || nc_b == 3)) {
+        // one evaluation per face (the batched ensembles record the per-cell form below)
+        auto run = [&](auto mc, auto nf) {
+            constexpr bool mixed = decltype(mc)::value;
+            PostOp<mixed, decltype(nf)::value> op;
+            op.L = L; op.nx = nx; op.ny = ny;
+            op.dxi = dxi; op.dyi = dyi; op.dzi = dzi; op.Jc = Jc;
+            op.ht = ht; op.vt = vt; op.hf = hf; op.vf = vf;
+            op.c2c2 = c2c2; op.use_diff = use_diff; op.consA = consA; op.turb = turb; op.clip = clip;
+            op.copy_slow = copy_slow; op.ae = ae; op.slow_dt = slow_dt;
+            op.dxlo = dxlo; op.dxhi = dxhi; op.dylo = dylo; op.dyhi = dyhi;
+            op.mu_h = mu_h; op.mu_v = mu_v; op.n0 = n0; op.ncons = ncons;
+            op.ax = ax; op.ay = ay; op.az = az; op.cur = cur; op.nc = nc; op.oc = oc; op.crb = crb; op.prim = prim;
+            launch_march(op, nx, ny, nz, s);
+        };
+        if (mixed_rt) {
+            if (nc_b == 1) run(std::true_type{}, std::integral_constant<int, 1>{});
+            else run(std::true_type{}, std::integral_constant<int, 3>{});
+        } else {
+            if (nc_b == 1) run(std::false_type{}, std::integral_constant<int, 1>{});
+            else run(std::false_type{}, std::integral_constant<int, 3>{});
+        }
+        return;
+    }
     launch_box_prec(mixed_rt, Box(0, 0, 0, nx - 1, ny - 1, nz - 1), s, [=] __device__(int i, int j, int k, auto mc) {
         constexpr bool mixed = decltype(mc)::value;
         const long q = L.ix(i, j, k);
