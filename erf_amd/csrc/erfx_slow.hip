@@ -1362,6 +1362,7 @@ struct PostOp {
             double v = oc[q + n * ne] + slow_dt * r;
             if (clip) v = (v < 0.0) ? 0.0 : v;   // amrex::max(v, 0.0)
             cur[q + n * ne] = v;
+            nc[q + n * ne] = v;   // S_new <- S_data for this component (nothing reads S_new's scalars here)
         }
     }
 };
@@ -1473,6 +1474,7 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
             double v = oc[q + n * ne] + slow_dt * r;
             if (clip) v = (v < 0.0) ? 0.0 : v;   // amrex::max(v, 0.0)
             cur[q + n * ne] = v;
+            nc[q + n * ne] = v;   // S_new <- S_data for this component (nothing reads S_new's scalars here)
         }
     });
 }
@@ -1500,11 +1502,14 @@ void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& r
     }
     double* cur = data.cons;
     double* nc = snew.cons;
-    // S_new <- S_data after every cell is updated (neighbours read S_new's rho above)
+    // S_new <- S_data after every cell is updated (neighbours read S_new's rho
+    // above).  The scalar blocks wrote their components to both, and with
+    // copy_slow the other slow components went from S_new to S_data, so rho and
+    // rho*theta are what is left of cons.
     launch_box(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (i <= nx - 1 && j <= ny - 1 && k <= nz - 1)
-            for (int c = 0; c < ncons; ++c) nc[q + c * ne] = cur[q + c * ne];
+            for (int c = 0; c < 2; ++c) nc[q + c * ne] = cur[q + c * ne];
         if (j <= ny - 1 && k <= nz - 1) snew.xmom[q] = data.xmom[q];
         if (i <= nx - 1 && k <= nz - 1) snew.ymom[q] = data.ymom[q];
         if (i <= nx - 1 && j <= ny - 1) snew.zmom[q] = data.zmom[q];
