@@ -472,9 +472,7 @@ struct TAdv : TAdvData {
 
 // ---- strain (ComputeStrain_N, Source/Diffusion/ERF_ComputeStrain_N.cpp:28-229) ----
 struct StrainBC {
-    bool zl_u, zh_u, zl_v, zh_v;   // Dirichlet tangential velocity on the z walls
-    // Dirichlet tangential velocity on lateral faces (Inflow)
-    bool xl_v, xh_v, xl_w, xh_w, yl_u, yh_u, yl_w, yh_w;
+    bool zl_u, zh_u, zl_v, zh_v;   // only the z walls can be Dirichlet for tangential velocity here
 };
 
 __device__ __forceinline__ double strain13(const double* u, const double* w, long q, long sz, double dxi, double dzi,
@@ -503,63 +501,17 @@ static StrainBC strain_bc(const Params& P) {
     b.zh_u = P.bck[0][5] == EXT_DIR;
     b.zl_v = P.bck[1][2] == EXT_DIR;
     b.zh_v = P.bck[1][5] == EXT_DIR;
-    b.xl_v = P.bck[1][0] == EXT_DIR;
-    b.xh_v = P.bck[1][3] == EXT_DIR;
-    b.xl_w = P.bck[2][0] == EXT_DIR;
-    b.xh_w = P.bck[2][3] == EXT_DIR;
-    b.yl_u = P.bck[0][1] == EXT_DIR;
-    b.yh_u = P.bck[0][4] == EXT_DIR;
-    b.yl_w = P.bck[2][1] == EXT_DIR;
-    b.yh_w = P.bck[2][4] == EXT_DIR;
     return b;
 }
 
-// The edge strains of the valid nodal boxes with ComputeStrain_N's one-sided
-// stencils (ERF_ComputeStrain_N.cpp:67-205): on a face where the velocity
-// across the edge is Dirichlet the ghost holds the face value.  The x faces
-// are treated first, then y, then z, so an edge on two such faces takes the
-// stencil of the first (the reference shrinks the box after each face).
-__device__ __forceinline__ double strain12_bc(const StrainBC& b, const double* u, const double* v, long q, long sy,
-                                              double dxi, double dyi, int i, int j, int nx, int ny) {
-    if (b.xl_v && i == 0)
-        return 0.5 * ((u[q] / 1.0 - u[q - sy] / 1.0) * dyi +
-                      (-(8. / 3.) * v[q - 1] / 1.0 + 3. * v[q] / 1.0 - (1. / 3.) * v[q + 1] / 1.0) * dxi) * 1.0 * 1.0;
-    if (b.xh_v && i == nx)
-        return 0.5 * ((u[q] / 1.0 - u[q - sy] / 1.0) * dyi +
-                      -(-(8. / 3.) * v[q] / 1.0 + 3. * v[q - 1] / 1.0 - (1. / 3.) * v[q - 2] / 1.0) * dxi) * 1.0 * 1.0;
-    if (b.yl_u && j == 0)
-        return 0.5 * ((-(8. / 3.) * u[q - sy] / 1.0 + 3. * u[q] / 1.0 - (1. / 3.) * u[q + sy] / 1.0) * dyi +
-                      (v[q] / 1.0 - v[q - 1] / 1.0) * dxi) * 1.0 * 1.0;
-    if (b.yh_u && j == ny)
-        return 0.5 * (-(-(8. / 3.) * u[q] / 1.0 + 3. * u[q - sy] / 1.0 - (1. / 3.) * u[q - 2 * sy] / 1.0) * dyi +
-                      (v[q] / 1.0 - v[q - 1] / 1.0) * dxi) * 1.0 * 1.0;
-    return strain12(u, v, q, sy, dxi, dyi);
-}
-__device__ __forceinline__ double strain13_bc(const StrainBC& b, const double* u, const double* w, long q, long sz,
-                                              double dxi, double dzi, int i, int k, int nx, int nz) {
-    if (b.xl_w && i == 0)
-        return 0.5 * ((u[q] - u[q - sz]) * dzi +
-                      (-(8. / 3.) * w[q - 1] + 3. * w[q] - (1. / 3.) * w[q + 1]) * dxi * 1.0);
-    if (b.xh_w && i == nx)
-        return 0.5 * ((u[q] - u[q - sz]) * dzi +
-                      -(-(8. / 3.) * w[q] + 3. * w[q - 1] - (1. / 3.) * w[q - 2]) * dxi * 1.0);
-    return strain13(u, w, q, sz, dxi, dzi, b.zl_u && k == 0, b.zh_u && k == nz);
-}
-__device__ __forceinline__ double strain23_bc(const StrainBC& b, const double* v, const double* w, long q, long sy,
-                                              long sz, double dyi, double dzi, int j, int k, int ny, int nz) {
-    if (b.yl_w && j == 0)
-        return 0.5 * ((v[q] - v[q - sz]) * dzi +
-                      (-(8. / 3.) * w[q - sy] + 3. * w[q] - (1. / 3.) * w[q + sy]) * dyi * 1.0 * 1.0);
-    if (b.yh_w && j == ny)
-        return 0.5 * ((v[q] - v[q - sz]) * dzi +
-                      -(-(8. / 3.) * w[q] + 3. * w[q - sy] - (1. / 3.) * w[q - 2 * sy]) * dyi * 1.0 * 1.0);
-    return strain23(v, w, q, sy, sz, dyi, dzi, b.zl_v && k == 0, b.zh_v && k == nz);
-}
-
 static void check_lateral_strain(const Params& P) {
-    // Lateral Dirichlet velocity faces would need the x/y one-sided stencils of
-    // ComputeStrain_N on the shrunk boxes; the supported face types make the
-    // strain boxes grown laterally, where the reference never applies them.
+    // ComputeStrain_N's one-sided stencils for a Dirichlet velocity on a
+    // lateral face (ERF_ComputeStrain_N.cpp:67-120) apply only where the edge
+    // box starts on the domain face; make_tau_terms grows the strain boxes by
+    // one laterally (ERF_make_tau_terms.cpp:135-143), so on a lateral face
+    // (Inflow, no-slip) they are never taken, and the centred stencil reads
+    // the ghost value there.  The z walls keep theirs (the boxes are not grown
+    // in z at a wall).
     (void)P;
 }
 
@@ -682,7 +634,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
         if (i < 0 || j < 0) return;
         // edges on nodal valid boxes
         if (k <= nz - 1) {                                   // xy edges i in [0,nx], j in [0,ny]
-            double s12 = strain12_bc(sb, u, v, q, sy, dxi, dyi, i, j, nx, ny);
+            double s12 = strain12(u, v, q, sy, dxi, dyi);
             double mu;
             double rho_bar = consA ? 0.25 * (cons[q - 1] + cons[q] + cons[q - 1 - sy] + cons[q - sy]) : 0.0;
             if (turb) {
@@ -694,7 +646,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
             }
         }
         if (j <= ny - 1) {                                   // xz edges i in [0,nx], k in [0,nz]
-            double s13 = strain13_bc(sb, u, w, q, sz, dxi, dzi, i, k, nx, nz);
+            double s13 = strain13(u, w, q, sz, dxi, dzi, sb.zl_u && k == 0, sb.zh_u && k == nz);
             double rho_bar = consA ? 0.25 * (cons[q - 1] + cons[q] + cons[q - 1 - sz] + cons[q - sz]) : 0.0;
             if (turb) {
                 double mu_bar = 0.25 * (mv[q - 1] + mv[q] + mv[q - 1 - sz] + mv[q - sz]);
@@ -705,7 +657,7 @@ static void make_tau_terms(const Params& P, const SlowWork& W, const double* con
             }
         }
         if (i <= nx - 1) {                                   // yz edges j in [0,ny], k in [0,nz]
-            double s23 = strain23_bc(sb, v, w, q, sy, sz, dyi, dzi, j, k, ny, nz);
+            double s23 = strain23(v, w, q, sy, sz, dyi, dzi, sb.zl_v && k == 0, sb.zh_v && k == nz);
             double rho_bar = consA ? 0.25 * (cons[q - sy] + cons[q] + cons[q - sy - sz] + cons[q - sz]) : 0.0;
             if (turb) {
                 double mu_bar = 0.25 * (mv[q - sy] + mv[q] + mv[q - sy - sz] + mv[q - sz]);

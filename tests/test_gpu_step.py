@@ -304,8 +304,9 @@ def test_inflow_outflow_multistep():
 @pytest.mark.parametrize("mtype", [C.MOLEC_CONSTANT, C.MOLEC_CONSTANT_ALPHA])
 def test_inflow_outflow_diffusion(mtype):
     """Inflow with molecular diffusion: one-sided fluxes of theta and the scalar
-    through the Dirichlet face, one-sided strain of v and w at the inflow face
-    (ERF_DiffusionSrcForState_N.cpp:208-227, ERF_ComputeStrain_N.cpp:67-120)."""
+    through the Dirichlet face (ERF_DiffusionSrcForState_N.cpp:208-227); the
+    strain takes its centred stencil through the Dirichlet ghost there (the
+    reference's strain boxes are grown laterally, ERF_make_tau_terms.cpp:135-143)."""
     cfg, f = P.inflow_outflow(24, 16, 16, seed=25, adv=C.ADV_UPWIND5)
     cfg.molec_diff_type = mtype
     cfg.dynamic_viscosity, cfg.alpha_T, cfg.alpha_C = 60.0, 40.0, 50.0

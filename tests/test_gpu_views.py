@@ -52,10 +52,19 @@ class ErfField:
                  self.lo[0] + g:self.hi[0] + g + 1]
 
     def differs(self, sim_ref):
-        """points of this view that differ from the same field of sim_ref"""
+        """points of this view that differ from the same field of sim_ref:
+        (on the valid region, on the ghosts, largest difference, first point)"""
         got = self.t[:, :, :, :self.ext[0]].cpu().numpy()
         want = self._region(sim_ref.download(self.fid)[:self.ncomp])
-        return int((got != want).sum())
+        d = got != want
+        if not d.any():
+            return None
+        g = [-self.lo[e] for e in range(3)]
+        nv = [self.ext[e] - 2 * g[e] for e in range(3)]
+        val = d[:, g[2]:g[2] + nv[2], g[1]:g[1] + nv[1], g[0]:g[0] + nv[0]]
+        first = np.argwhere(d)[0]
+        return (int(val.sum()), int(d.sum() - val.sum()), float(np.abs(got - want)[d].max()),
+                [int(first[0]), int(first[1]) - g[2], int(first[2]) - g[1], int(first[3]) - g[0]])
 
 
 def _fields(sim, cfg):
@@ -90,7 +99,8 @@ def _sv(f, s):
 def _check(f, ref, keys):
     bad = {k: f[k].differs(ref) for k in keys}
     bad = {k: v for k, v in bad.items() if v}
-    assert not bad, "view hook differs from the pointer hook (points): %s" % bad
+    assert not bad, ("view hook differs from the pointer hook (points on the valid region, on the ghosts, "
+                     "largest difference, first point (n, k, j, i)): %s" % bad)
 
 
 def _state_keys(*states):
