@@ -96,11 +96,16 @@ def _sv(f, s):
     return StateView(f[(s, 0)].v, f[(s, 1)].v, f[(s, 2)].v, f[(s, 3)].v)
 
 
-def _check(f, ref, keys):
+PROBLEMS = []
+
+
+def _check(f, ref, keys, what=""):
+    """record the views that differ from the pointer hook's fields (points on
+    the valid region, on the ghosts, largest difference, first point (n, k, j, i))"""
     bad = {k: f[k].differs(ref) for k in keys}
     bad = {k: v for k, v in bad.items() if v}
-    assert not bad, ("view hook differs from the pointer hook (points on the valid region, on the ghosts, "
-                     "largest difference, first point (n, k, j, i)): %s" % bad)
+    if bad:
+        PROBLEMS.append((what, bad))
 
 
 def _state_keys(*states):
@@ -119,6 +124,8 @@ def test_view_hooks_match_pointer_hooks(kind):
     f = _fields(sim, cfg)
     st = lambda s: ctypes.byref(ref.state(s))
     B = ctypes.byref
+    del PROBLEMS[:]
+    _check(f, ref, list(f), "initial state")
 
     # pre_update: cons_to_prim on S_new
     ng = ref.L.ng
@@ -126,7 +133,7 @@ def test_view_hooks_match_pointer_hooks(kind):
                                   ref.ptr(C.D_PI_STAGE), ng))
     check(lib().erfx_cons_to_prim_v(sim.ctx, B(f[(C.S_NEW, 0)].v), B(f["prim"].v), B(f["pi"].v), ng))
     ref.sync(); sim.sync()
-    _check(f, ref, ["prim", "pi"])
+    _check(f, ref, ["prim", "pi"], "cons_to_prim")
 
     # turbulent viscosity (LES)
     if cfg.les_type != C.LES_NONE:
@@ -135,7 +142,7 @@ def test_view_hooks_match_pointer_hooks(kind):
         check(lib().erfx_turbulent_viscosity_v(sim.ctx, B(f[(C.S_OLD, 0)].v), B(f["u_old"].v), B(f["v_old"].v),
                                                B(f["w_old"].v), B(f["eddy"].v)))
         ref.sync(); sim.sync()
-        _check(f, ref, ["eddy"])
+        _check(f, ref, ["eddy"], "turbulent_viscosity")
     eddy_p = ref.ptr(C.D_EDDY) if cfg.les_type != C.LES_NONE else None
     eddy_v = B(f["eddy"].v) if cfg.les_type != C.LES_NONE else None
 
@@ -147,7 +154,7 @@ def test_view_hooks_match_pointer_hooks(kind):
     check(lib().erfx_slow_rhs_pre_v(sim.ctx, 0, dt / 3, B(fs), B(sn), B(f["prim"].v), B(sc), B(f["u"].v),
                                     B(f["v"].v), B(f["w"].v), eddy_v))
     ref.sync(); sim.sync()
-    _check(f, ref, _state_keys(C.F_SLOW, C.S_SCRATCH))
+    _check(f, ref, _state_keys(C.F_SLOW, C.S_SCRATCH), "slow_rhs_pre")
 
     # the fast coefficients and two substeps (the second with S_prev == S_data)
     dtau = dt / cfg.mri_ratio
@@ -156,7 +163,7 @@ def test_view_hooks_match_pointer_hooks(kind):
                                       ref.ptr(C.D_PI_STAGE), dtau, beta_s))
     check(lib().erfx_make_fast_coeffs_v(sim.ctx, B(f["fc"].v), B(sn), B(f["prim"].v), B(f["pi"].v), dtau, beta_s))
     ref.sync(); sim.sync()
-    _check(f, ref, ["fc"])
+    _check(f, ref, ["fc"], "make_fast_coeffs")
     T = "T" if cfg.use_terrain else "N"
     so, ss = _sv(f, C.S_OLD), _sv(f, C.S_SUM)
     for step in (0, 1):
@@ -170,36 +177,37 @@ def test_view_hooks_match_pointer_hooks(kind):
                                                      B(f["pi"].v), B(f["fc"].v), B(ss), B(sc), dtau, beta_s,
                                                      1.0 / cfg.mri_ratio))
         ref.sync(); sim.sync()
-        _check(f, ref, _state_keys(C.S_SUM, C.S_SCRATCH))
+        _check(f, ref, _state_keys(C.S_SUM, C.S_SCRATCH), "fast_rhs step %d" % step)
         # the apply_bcs between substeps
         check(lib().erfx_apply_bcs(ref.ctx, st(C.S_SUM), ref.ptr(C.F_XVEL), ref.ptr(C.F_YVEL), ref.ptr(C.F_ZVEL),
                                    1, 1, 1, 0))
         check(lib().erfx_apply_bcs_v(sim.ctx, B(ss), B(f["u"].v), B(f["v"].v), B(f["w"].v), 1, 1, 1, 0))
         ref.sync(); sim.sync()
-        _check(f, ref, _state_keys(C.S_SUM) + ["u", "v", "w"])
+        _check(f, ref, _state_keys(C.S_SUM) + ["u", "v", "w"], "apply_bcs after step %d" % step)
 
     # slow_rhs_post: the slow scalars, S_new <- S_sum
     check(lib().erfx_slow_rhs_post(ref.ctx, 2, dt, st(C.F_SLOW), st(C.S_OLD), st(C.S_NEW), st(C.S_SUM),
                                    ref.ptr(C.D_PRIM), st(C.S_SCRATCH), None, None, eddy_p))
     check(lib().erfx_slow_rhs_post_v(sim.ctx, 2, dt, B(fs), B(so), B(sn), B(ss), B(f["prim"].v), B(sc), eddy_v))
     ref.sync(); sim.sync()
-    _check(f, ref, _state_keys(C.F_SLOW, C.S_NEW, C.S_SUM))
+    _check(f, ref, _state_keys(C.F_SLOW, C.S_NEW, C.S_SUM), "slow_rhs_post")
 
     # post_update on S_new with the full ghost widths
     check(lib().erfx_apply_bcs(ref.ctx, st(C.S_NEW), ref.ptr(C.F_XVEL), ref.ptr(C.F_YVEL), ref.ptr(C.F_ZVEL),
                                ng, ng - 1, 0, 0))
     check(lib().erfx_apply_bcs_v(sim.ctx, B(sn), B(f["u"].v), B(f["v"].v), B(f["w"].v), ng, ng - 1, 0, 0))
     ref.sync(); sim.sync()
-    _check(f, ref, _state_keys(C.S_NEW) + ["u", "v", "w"])
+    _check(f, ref, _state_keys(C.S_NEW) + ["u", "v", "w"], "apply_bcs post_update")
 
     # microphysics on the new cons
     if cfg.moisture_type != C.MOIST_NONE:
         check(lib().erfx_kessler_advance(ref.ctx, ref.ptr(C.state_id(C.S_NEW, 0)), dt, None))
         check(lib().erfx_kessler_advance_v(sim.ctx, B(f[(C.S_NEW, 0)].v), dt))
         ref.sync(); sim.sync()
-        _check(f, ref, [(C.S_NEW, 0)])
+        _check(f, ref, [(C.S_NEW, 0)], "kessler")
     sim.close()
     ref.close()
+    assert not PROBLEMS, "\n".join("%s: %s" % p for p in PROBLEMS)
 
 
 def test_view_hook_rejects_a_short_view_before_device_work():
