@@ -17,6 +17,7 @@ LES_NONE, LES_SMAGORINSKY = range(2)
 RHO, RHOTHETA, RHOKE, RHOQKE, RHOSCALAR = range(5)
 RHOQ1, RHOQ2, RHOQ3 = 5, 6, 7          # vapour, cloud water, rain (Kessler)
 NCONS_DRY, NCONS_MOIST = 5, 8
+COL_VARIANTS = 10   # ERFX_COL_VARIANTS: acoustic column kernel variants (9 = the tile form)
 MOIST_NONE, MOIST_KESSLER, MOIST_KESSLER_NORAIN = range(3)
 
 F_CONS, F_XVEL, F_YVEL, F_ZVEL, F_R0, F_P0, F_PI0 = range(7)

@@ -1215,7 +1215,7 @@ int erfx_sim_column_variant(erfx_sim* s, int* variant, float* ms) {
 
 int erfx_sim_set_column_variant(erfx_sim* s, int variant) {
     if (!s) { set_error("null argument"); return ERFX_ERR_ARG; }
-    if (variant < 0 || variant >= COL_NV) { set_error("column variant must be 0..8"); return ERFX_ERR_ARG; }
+    if (variant < 0 || variant >= COL_NV) { set_error("column variant must be 0.." + std::to_string(COL_NV - 1)); return ERFX_ERR_ARG; }
     ERFX_HIP(hipStreamSynchronize(s->stream));
     tuner_free(s);
     s->ctx->P.col_variant = variant;

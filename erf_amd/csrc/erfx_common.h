@@ -354,14 +354,158 @@ inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f, 
     hipLaunchKernelGGL(col_kernel<F>, dim3(gx * gy), dim3((unsigned)width, 1, 1), 0, s, ilo, ihi, jlo, jhi, gx, f);
 }
 
+// ---- column tiles: one work-group of NW waves per 64 columns -------------
+// For column solves whose per-level work is independent except for a
+// tridiagonal recurrence: the waves split the levels into chunks and form the
+// per-level terms of their chunk in parallel, one wave runs the recurrence
+// over the tile's 64 columns from LDS, and the waves finish their chunks in
+// parallel again.  Lane l of every wave holds column tile * 64 + l of the
+// (i, j) range flattened i fastest (a box narrower than 64 fills the lanes
+// with the next rows), so every level's access of a wave is one contiguous
+// row segment.  The functor is called on all lanes, `valid` false past the
+// range's end, so that every lane reaches the work-group barriers inside it;
+// `lds` holds three arrays of tile_rows<CM, NW>() rows of 64
+// doubles, one value per level and column.
+template <int CM, int NW>
+constexpr int tile_rows() { return NW * CM + 1; }
+template <int CM, int NW>
+constexpr int tile_lds_doubles() { return 3 * tile_rows<CM, NW>() * 64; }
+// levels per wave chunk for nz levels, and the smallest instantiated CM >= it
+// at 8 waves (0: nz above 96, no tile form: three arrays of 97 rows are
+// 149 KB of the CU's 160 KB of LDS)
+__host__ __device__ inline int tile_chunk(int nz, int nw) { return (nz + nw - 1) / nw; }
+inline int tile_cm8(int nz) {
+    const int c = tile_chunk(nz, 8);
+    return c <= 4 ? 4 : c <= 8 ? 8 : c <= 12 ? 12 : 0;
+}
+// Ends one level of a wave's chunk: keeps the compiler from hoisting the next
+// levels' loads above it (one level of loads per wave covers the latency with
+// the tile's waves in flight, while hoisting several levels exceeds a lane's
+// registers and spills).
+__device__ __forceinline__ void tile_level_fence() { asm volatile("" ::: "memory"); }
+
+// The tridiagonal sweeps of a tile over its LDS rows (row k of column `lane`
+// at k * 64 + lane).  Forward: X holds the right-hand sides of faces 0..nz, Y
+// the sub-diagonal a_k and Z the inverted pivots b_k; X becomes s_0 = X_0 b_0,
+// s_k = (X_k - a_k s_{k-1}) b_k.  Backward: Y holds c_k b_k; X becomes
+// x_nz = s_nz, x_k = s_k - (c_k b_k) x_{k+1}.  Both keep the operation order
+// of a per-column Thomas sweep, so the values are the same; the reads of the
+// next U levels are issued before their dependent arithmetic.
+__device__ __forceinline__ void tile_forward(double* X, const double* Y, const double* Z, int nz, int lane) {
+    constexpr int U = 8;
+    double s = X[lane] * Z[lane];
+    X[lane] = s;
+    int k = 1;
+    for (; k + U <= nz + 1; k += U) {
+        double a[U], b[U], r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int o = (k + u) * 64 + lane;
+            a[u] = Y[o]; b[u] = Z[o]; r[u] = X[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s = (r[u] - a[u] * s) * b[u];
+            X[(k + u) * 64 + lane] = s;
+        }
+    }
+    for (; k <= nz; ++k) {
+        const int o = k * 64 + lane;
+        s = (X[o] - Y[o] * s) * Z[o];
+        X[o] = s;
+    }
+}
+__device__ __forceinline__ void tile_backward(double* X, const double* Y, int nz, int lane) {
+    constexpr int U = 8;
+    double s = X[nz * 64 + lane];
+    int k = nz - 1;
+    for (; k - U + 1 >= 0; k -= U) {
+        double cb[U], r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int o = (k - u) * 64 + lane;
+            cb[u] = Y[o]; r[u] = X[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s = r[u] - cb[u] * s;
+            X[(k - u) * 64 + lane] = s;
+        }
+    }
+    for (; k >= 0; --k) {
+        const int o = k * 64 + lane;
+        s = X[o] - Y[o] * s;
+        X[o] = s;
+    }
+}
+
+template <int CM, int NW, class F>
+__global__ void __launch_bounds__(64 * NW) tile_kernel(int ilo, int jlo, unsigned nxr, unsigned ncols, F f) {
+    __shared__ double lds[tile_lds_doubles<CM, NW>()];
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned c = t * 64u + (unsigned)lane;
+    const bool valid = c < ncols;
+    const int i = ilo + (int)(c % nxr), j = jlo + (int)(c / nxr);
+    [[clang::always_inline]] f(i, j, valid, lane, wave, lds);
+}
+template <class F>
+struct TileItem {
+    BHdr h;
+    F f;
+};
+template <int CM, int NW, class F>
+__global__ void __launch_bounds__(64 * NW) tile_kernel_batch(const TileItem<F>* __restrict__ it,
+                                                             const unsigned* __restrict__ starts, int n) {
+    __shared__ double lds[tile_lds_doubles<CM, NW>()];
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const TileItem<F>& m = batch_member(it, starts, n, t);
+    const BHdr h = m.h;
+    const F& f = m.f;
+    const unsigned nxr = (unsigned)(h.hi[0] - h.lo[0] + 1);
+    const unsigned ncols = nxr * (unsigned)(h.hi[1] - h.lo[1] + 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned c = (t - h.t0) * 64u + (unsigned)lane;
+    const bool valid = c < ncols;
+    const int i = h.lo[0] + (int)(c % nxr), j = h.lo[1] + (int)(c / nxr);
+    [[clang::always_inline]] f(i, j, valid, lane, wave, lds);
+}
+template <int CM, int NW, class F>
+void launch_tiles_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
+                        hipStream_t s) {
+    (void)width;
+    hipLaunchKernelGGL((tile_kernel_batch<CM, NW, F>), dim3(total), dim3(64 * NW), 0, s,
+                       static_cast<const TileItem<F>*>(items), starts, n);
+}
+// Tiles over the columns (ilo..ihi) x (jlo..jhi); CM >= tile_chunk(nz, NW) of
+// the functor's box.
+template <int CM, int NW, class F>
+inline void launch_tiles(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
+    const bool ok = ilo <= ihi && jlo <= jhi;
+    const unsigned nxr = ok ? (unsigned)(ihi - ilo + 1) : 1u;
+    const unsigned ncols = ok ? nxr * (unsigned)(jhi - jlo + 1) : 0u;
+    const unsigned tiles = (ncols + 63u) / 64u;
+    if (g_rec) {
+        TileItem<F> it{{{ilo, jlo, 0}, {ihi, jhi, 0}, 1u, 1u, 0u, tiles}, f};
+        rec_push(&launch_tiles_batch<CM, NW, F>, (unsigned)(64 * NW), it);
+        return;
+    }
+    if (!ok) return;
+    hipLaunchKernelGGL((tile_kernel<CM, NW, F>), dim3(tiles), dim3(64 * NW), 0, s, ilo, jlo, nxr, ncols, f);
+}
+
 // Variants of the acoustic column kernel (erf_fast_rhs_N): levels of loads
 // kept in flight per lane and work-group width.  Every variant computes the
 // same values; the sim's autotuner times them on the first launches of a run
 // and keeps the fastest.
-constexpr int COL_NV = 9;
-constexpr int COL_DEPTH[COL_NV] = {2, 3, 4, 2, 3, 4, 2, 3, 4};
-constexpr int COL_WIDTH[COL_NV] = {64, 64, 64, 128, 128, 128, 256, 256, 256};
-constexpr int COL_DEFAULT = 7;   // depth 3, 256 lanes
+// Variant COL_TILE is the tile form (launch_tiles, boxes up to 128 levels;
+// COL_FALLBACK above that).
+constexpr int COL_NV = ERFX_COL_VARIANTS;
+constexpr int COL_DEPTH[COL_NV] = {2, 3, 4, 2, 3, 4, 2, 3, 4, 0};
+constexpr int COL_WIDTH[COL_NV] = {64, 64, 64, 128, 128, 128, 256, 256, 256, 0};
+constexpr int COL_TILE = 9;
+constexpr int COL_FALLBACK = 7;  // depth 3, 256 lanes
+constexpr int COL_DEFAULT = COL_TILE;
 struct ColTuner {
     static constexpr int SAMPLES = 3;
     hipEvent_t ev[2 * COL_NV * SAMPLES] = {};

@@ -423,6 +423,224 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
             }
         }
     };
+    // The same column work as a tile of 64 columns per work-group of 8 waves
+    // (launch_tiles), for boxes of up to 96 levels:
+    //  1. each wave takes a chunk of faces k = ka..kb of its 64 columns and forms,
+    //     level by level as the column kernel does, the horizontal divergences of
+    //     cell k and Omega of face k (kept in the lane's registers) and the
+    //     tridiagonal right-hand side of face k, staged in LDS with the face's
+    //     a_k and 1/b_k;
+    //  2. wave 0 runs the elimination up the 64 columns in LDS; the waves then
+    //     stage c_k / b_k and Omega in LDS in place of a_k and 1/b_k, and wave 0
+    //     runs the substitution down the columns;
+    //  3. each wave updates rho*w, the averaged rho*w, rho and rho*theta of its
+    //     chunk's cells from the solution in LDS.
+    // Every value is computed by the same expression as in the column kernel, so
+    // the results are bit-identical; the sweep's intermediates never reach HBM,
+    // the per-level loads of a column are spread over 8 waves, and the serial
+    // sweeps read only LDS.
+    const bool implicit_ = implicit;
+    auto tile = [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds, auto cmc) {
+        constexpr int CM = decltype(cmc)::value;
+        constexpr int NW = 8;
+        const int ch = tile_chunk(nz, NW);
+        const int ka = wave * ch;
+        const int kb = (ka + ch < nz ? ka + ch : nz) - 1;
+        const bool work = valid && ka <= kb;
+        const long q0 = valid ? L.ix(i, j, 0) : 0;
+        constexpr int R = tile_rows<CM, NW>() * 64;
+        double* X = lds;            // right-hand side, then the solution, per face
+        double* Y = lds + R;        // a_k, then c_k / b_k
+        double* Z = lds + 2 * R;    // 1 / b_k, then Omega
+        struct Fw {
+            double P, Q, ws, pkp1, pr, st, prt_, srt_, s0, s1;
+            double dxl, dxh, sxl, sxh, dyl, dyh, syl, syh, pc, pxm, pxp, pym, pyp;
+            double q1, q2;
+        };
+        auto fload = [&](long q, Fw& v) {
+            v.P = Pc[q]; v.Q = Qc[q]; v.ws = slow_rhs.zmom[q];
+            v.pkp1 = prim[q + sz];
+            v.pr = pr0[q];
+            v.st = st0[q]; v.prt_ = prt[q]; v.srt_ = srt[q]; v.s0 = sr0[q]; v.s1 = sr1[q];
+            v.dxl = data.xmom[q]; v.dxh = data.xmom[q + 1]; v.sxl = stage.xmom[q]; v.sxh = stage.xmom[q + 1];
+            v.dyl = data.ymom[q]; v.dyh = data.ymom[q + sy]; v.syl = stage.ymom[q]; v.syh = stage.ymom[q + sy];
+            v.pc = prim[q]; v.pxm = prim[q - 1]; v.pxp = prim[q + 1]; v.pym = prim[q - sy]; v.pyp = prim[q + sy];
+            if (moist) { v.q1 = pq1[q]; v.q2 = pq2[q]; } else { v.q1 = 0.0; v.q2 = 0.0; }
+        };
+        auto hdiv = [&](const Fw& v, double& t0, double& t1) {
+            double xflux_lo = v.dxl - v.sxl;
+            double xflux_hi = v.dxh - v.sxh;
+            double yflux_lo = v.dyl - v.syl;
+            double yflux_hi = v.dyh - v.syh;
+            double pc = v.pc;
+            t0 = (xflux_hi - xflux_lo) * dxi + (yflux_hi - yflux_lo) * dyi;
+            t1 = ((xflux_hi * (pc + v.pxp) - xflux_lo * (pc + v.pxm)) * dxi +
+                  (yflux_hi * (pc + v.pyp) - yflux_lo * (pc + v.pym)) * dyi) * 0.5;
+        };
+        // Omega of the old increments on face kf: prev rho*w (after the round
+        // trip in the substep chain) minus the stage's
+        auto om_face = [&](int kf) -> double {
+            const long qf = q0 + (long)kf * sz;
+            double pzv;
+            if (kf == 0) {
+                pzv = rt ? ch_w_lo * 0.5 * (pr0[q0] + pr0[q0]) : prev.zmom[q0];
+            } else if (!rt) {
+                pzv = prev.zmom[qf];
+            } else if (kf >= nz) {
+                pzv = ch_w_hi * 0.5 * (pr0[qf - sz] + pr0[qf - sz]);
+            } else {
+                double d = pr0[qf] + pr0[qf - sz];
+                double w = prev.zmom[qf] * 2.0 / d;
+                pzv = w * 0.5 * d;
+            }
+            return pzv - stage.zmom[qf];
+        };
+        double t0r[CM], t1r[CM], omr[CM + 1];
+        // ---- phase 1: right-hand sides of the chunk's faces ----
+        if (work) {
+            double t0_km1, t1_km1, drho_km1, drt_km1, s0_km1, s1_km1, p_km2, p_km1, p_k, q1_km1, q2_km1;
+            double Om_km1, Om_k;
+            {
+                const int kc = ka == 0 ? 0 : ka - 1;      // the cell below the chunk's first face
+                Fw c;
+                fload(q0 + (long)kc * sz, c);
+                hdiv(c, t0_km1, t1_km1);
+                drho_km1 = c.pr - c.st;
+                drt_km1 = c.prt_ - c.srt_;
+                s0_km1 = c.s0; s1_km1 = c.s1;
+                p_km2 = prim[q0 + (long)(kc - 1) * sz]; p_km1 = c.pc; p_k = c.pkp1;
+                q1_km1 = c.q1; q2_km1 = c.q2;
+                Om_km1 = om_face(kc);
+                Om_k = om_face(kc + 1);
+                if (ka == 0) {      // face 0: the bottom row (:423-432)
+                    X[lane] = Om_km1 + dtau * c.ws;
+                    Z[lane] = iB[q0];
+                    t0r[0] = t0_km1; t1r[0] = t1_km1; omr[0] = Om_km1;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k == 0) continue;
+                if (k > kb) break;
+                const long q = q0 + (long)k * sz;
+                Fw c;
+                fload(q, c);
+                const double a_k = A[q], b_k = iB[q];
+                double t0_k, t1_k;
+                hdiv(c, t0_k, t1_k);
+                double Om_kp1 = om_face(k + 1);
+                double drho_k = c.pr - c.st;
+                double drt_k = c.prt_ - c.srt_;
+                double coeff_P = c.P, coeff_Q = c.Q;
+                if (moist) {   // :379-384
+                    double qf = 0.5 * (c.q1 + q1_km1 + c.q2 + q2_km1);
+                    coeff_P /= (1.0 + qf);
+                    coeff_Q /= (1.0 + qf);
+                }
+                double theta_t_lo = 0.5 * (p_km2 + p_km1);
+                double theta_t_mid = 0.5 * (p_km1 + p_k);
+                double theta_t_hi = 0.5 * (p_k + c.pkp1);
+                double R0_tmp = coeff_P * drt_k + coeff_Q * drt_km1 - halfg * (drho_k + drho_km1);
+                double R1_tmp = halfg * (-c.s0 - s0_km1 + t0_k + t0_km1) +
+                                (coeff_P * (c.s1 - t1_k) + coeff_Q * (s1_km1 - t1_km1));
+                R1_tmp += beta_1 * dzi * ((Om_kp1 - Om_km1) * halfg
+                                          - (Om_kp1 * theta_t_hi - Om_k * theta_t_mid) * coeff_P
+                                          - (Om_k * theta_t_mid - Om_km1 * theta_t_lo) * coeff_Q);
+                X[k * 64 + lane] = Om_k + dtau * (c.ws + R0_tmp + dtau * beta_2 * R1_tmp);
+                Y[k * 64 + lane] = a_k;
+                Z[k * 64 + lane] = b_k;
+                t0r[m] = t0_k; t1r[m] = t1_k; omr[m] = Om_k;
+                tile_level_fence();
+                p_km2 = p_km1; p_km1 = p_k; p_k = c.pkp1;
+                q1_km1 = c.q1; q2_km1 = c.q2;
+                Om_km1 = Om_k; Om_k = Om_kp1;
+                drho_km1 = drho_k; drt_km1 = drt_k;
+                s0_km1 = c.s0; s1_km1 = c.s1;
+                t0_km1 = t0_k; t1_km1 = t1_k;
+            }
+#pragma unroll
+            for (int m = 0; m <= CM; ++m)      // Omega of the face above the chunk
+                if (m == kb - ka + 1) omr[m] = Om_k;
+            if (kb == nz - 1) {                // the top face's row (:423-432)
+                const long qt = q0 + (long)nz * sz;
+                X[nz * 64 + lane] = Om_k + dtau * slow_rhs.zmom[qt];
+                Y[nz * 64 + lane] = A[qt];
+                Z[nz * 64 + lane] = iB[qt];
+            }
+        }
+        __syncthreads();
+        // ---- phase 2: elimination (wave 0), c_k / b_k and Omega staged, substitution (wave 0) ----
+        if (wave == 0 && valid && implicit_) tile_forward(X, Y, Z, nz, lane);
+        __syncthreads();
+        if (wave == 0 && valid) {
+            const long qt = q0 + (long)nz * sz;
+            data.zmom[qt] = stage.zmom[qt] + X[nz * 64 + lane];
+        }
+        if (work) {
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k > kb) break;
+                if (implicit_) Y[k * 64 + lane] = Cc[q0 + (long)k * sz] * Z[k * 64 + lane];
+                Z[k * 64 + lane] = omr[m];
+            }
+            if (kb == nz - 1) {
+#pragma unroll
+                for (int m = 0; m <= CM; ++m)
+                    if (m == kb - ka + 1) Z[nz * 64 + lane] = omr[m];
+            }
+        }
+        __syncthreads();
+        if (wave == 0 && valid && implicit_) tile_backward(X, Y, nz, lane);
+        __syncthreads();
+        // ---- phase 3: the chunk's cells ----
+        if (work) {
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k > kb) break;
+                const long q = q0 + (long)k * sz;
+                const double sol = X[k * 64 + lane], sol_next = X[(k + 1) * 64 + lane];
+                const double szz = stage.zmom[q], azm = scratch.zmom[q];
+                const double pcq = prim[q], p_next = prim[q + sz], pm = prim[q - sz];
+                const double s0 = sr0[q], s1 = sr1[q];
+                const double c0 = step == 0 ? prev.cons[q] : cc0[q];
+                const double c1 = step == 0 ? prev.cons[q + ne] : cc1[q];
+                double drw_k = Z[k * 64 + lane], drw_next = Z[(k + 1) * 64 + lane];
+                data.zmom[q] = szz + sol;
+                double zflux_lo = beta_2 * sol + beta_1 * drw_k;
+                double zflux_hi = beta_2 * sol_next + beta_1 * drw_next;
+                scratch.zmom[q] = azm + facinv * zflux_lo / (1.0 * 1.0);
+                if (k == nz - 1) scratch.zmom[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
+                double t0 = t0r[m] + dzi * (zflux_hi - zflux_lo);
+                double t1 = t1r[m] + 0.5 * dzi * (zflux_hi * (pcq + p_next) - zflux_lo * (pcq + pm));
+                double r_new = c0 + dtau * (s0 - t0);
+                double rt_new = c1 + dtau * (s1 - t1);
+                cc0[q] = r_new;
+                cc1[q] = rt_new;
+                if (gw) {
+                    const int ia[2] = {i, i == 0 ? nx : (i == nx - 1 ? -1 : i)};
+                    const int ja[2] = {j, j == 0 ? ny : (j == ny - 1 ? -1 : j)};
+                    const int nk = (k == 0 ? 1 : 0) + (k == nz - 1 ? 1 : 0);
+                    const int ka3[3] = {k, k == 0 ? -1 : nz, nz};
+                    for (int a = 0; a < 2; ++a) {
+                        if (a == 1 && ia[1] == i) continue;
+                        for (int b = 0; b < 2; ++b) {
+                            if (b == 1 && ja[1] == j) continue;
+                            for (int c3 = 0; c3 <= nk; ++c3) {
+                                if (a == 0 && b == 0 && c3 == 0) continue;
+                                long g = L.ix(ia[a], ja[b], ka3[c3]);
+                                cc0[g] = r_new;
+                                cc1[g] = rt_new;
+                            }
+                        }
+                    }
+                }
+                tile_level_fence();
+            }
+        }
+    };
     // the variant: the sim's choice, or the next one to time while tuning
     int var = P.col_variant;
     int tag = -1;
@@ -431,9 +649,26 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
         var = tag % COL_NV;
         (void)hipEventRecord(P.tuner->ev[2 * tag], s);
     }
-    const int width = COL_WIDTH[var];
+    // the tile form needs nz <= 96; a recorded (batched) launch takes the
+    // largest chunk, so every member of a batch records the same kernel
+    const int cm = tile_cm8(nz) == 0 ? 0 : (g_rec ? 12 : tile_cm8(nz));
+    if (var == COL_TILE && cm == 0) var = COL_FALLBACK;
     kt_start(P.timer, KT_FAST_COLUMN, s);
-    switch (COL_DEPTH[var]) {
+    if (var == COL_TILE) {
+        auto launch = [&](auto cmc) {
+            launch_tiles<decltype(cmc)::value, 8>(0, nx - 1, 0, ny - 1, s,
+                [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds) {
+                    tile(i, j, valid, lane, wave, lds, cmc);
+                });
+        };
+        switch (cm) {
+            case 4: launch(std::integral_constant<int, 4>{}); break;
+            case 8: launch(std::integral_constant<int, 8>{}); break;
+            default: launch(std::integral_constant<int, 12>{}); break;
+        }
+    } else {
+        const int width = COL_WIDTH[var];
+        switch (COL_DEPTH[var]) {
         case 2:
             launch_cols(0, nx - 1, 0, ny - 1, s,
                         [=] __device__(int i, int j) { column(i, j, std::integral_constant<int, 2>{}); }, width);
@@ -446,6 +681,7 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
             launch_cols(0, nx - 1, 0, ny - 1, s,
                         [=] __device__(int i, int j) { column(i, j, std::integral_constant<int, 3>{}); }, width);
             break;
+        }
     }
     if (tag >= 0) (void)hipEventRecord(P.tuner->ev[2 * tag + 1], s);
     // algorithmic bytes per cell: the fields of the substep this kernel reads

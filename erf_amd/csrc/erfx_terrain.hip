@@ -216,6 +216,199 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     double* cz = data.zmom;
     double* azm = scratch.zmom;
     kt_start(P.timer, KT_FAST_COLUMN, s);
+    // The tile form (launch_tiles; the default for boxes of up to 96 levels):
+    // the 8 waves of a 64-column tile form the right-hand sides of their
+    // chunks of faces in parallel and stage them in LDS with a_k and 1/b_k
+    // (Omega and the horizontal divergences stay in registers), wave 0 runs the
+    // elimination in LDS, the waves stage c_k / b_k and Omega, wave 0 runs the
+    // substitution and writes the top face, and the waves update their chunks'
+    // cells.  Every value is the column kernel's expression, so the results are
+    // bit-identical; the serial sweeps read only LDS.
+    auto tile = [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds, auto cmc) {
+        constexpr int CM = decltype(cmc)::value;
+        constexpr int NW = 8;
+        const int ch = tile_chunk(nz, NW);
+        const int ka = wave * ch;
+        const int kb = (ka + ch < nz ? ka + ch : nz) - 1;
+        const bool work = valid && ka <= kb;
+        const long q0 = valid ? L.ix(i, j, 0) : 0;
+        constexpr int R = tile_rows<CM, NW>() * 64;
+        double* X = lds;            // right-hand side, then the solution, per face
+        double* Y = lds + R;        // a_k, then c_k / b_k
+        double* Z = lds + 2 * R;    // 1 / b_k, then Omega
+        auto pw = [&](int kf) -> double {
+            const long q = q0 + (long)kf * sz;
+            if (!rt || kf >= nz) return pzm[q];
+            const double d = prho[q] + prho[q - sz];
+            const double w = pzm[q] * 2.0 / d;
+            return w * 0.5 * d;
+        };
+        auto omega_old = [&](int kf) -> double {
+            const long q = q0 + (long)kf * sz;
+            if (kf == 0) return 0.;
+            const double dwk = pw(kf) - szm[q];
+            if (kf == nz) return dwk;
+            return met::omega_from_w(i, j, kf, dwk, fdu, fdv, z, dxi, dyi);
+        };
+        auto trhs_at = [&](int k, double& t0, double& t1) {
+            const long q = q0 + (long)k * sz;
+            double h_zeta_cc_xface_hi = 0.5 * dzi * (z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1) - z(i + 1, j, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_xface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i, j + 1, k + 1) - z(i, j, k) - z(i, j + 1, k));
+            double h_zeta_cc_yface_hi = 0.5 * dzi * (z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1) - z(i, j + 1, k) - z(i + 1, j + 1, k));
+            double h_zeta_cc_yface_lo = 0.5 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k) - z(i + 1, j, k));
+            double xflux_lo = nu[q] * h_zeta_cc_xface_lo / 1.0;
+            double xflux_hi = nu[q + 1] * h_zeta_cc_xface_hi / 1.0;
+            double yflux_lo = nv[q] * h_zeta_cc_yface_lo / 1.0;
+            double yflux_hi = nv[q + sy] * h_zeta_cc_yface_hi / 1.0;
+            double mfsq = 1.0 * 1.0;
+            t0 = (xflux_hi - xflux_lo) * dxi * mfsq + (yflux_hi - yflux_lo) * dyi * mfsq;
+            t1 = ((xflux_hi * (prim[q] + prim[q + 1]) - xflux_lo * (prim[q] + prim[q - 1])) * dxi * mfsq +
+                  (yflux_hi * (prim[q] + prim[q + sy]) - yflux_lo * (prim[q] + prim[q - sy])) * dyi * mfsq) * 0.5;
+        };
+        double t0r[CM], t1r[CM], omr[CM + 1];
+        // ---- phase 1: right-hand sides of the chunk's faces ----
+        if (work) {
+            const int kc = ka == 0 ? 0 : ka - 1;      // the cell below the chunk's first face
+            const long qc = q0 + (long)kc * sz;
+            double t0_km1, t1_km1;
+            trhs_at(kc, t0_km1, t1_km1);
+            double drho_km1 = cc0[qc] - sc0[qc], drt_km1 = cc1[qc] - sc1[qc];
+            double J_km1 = J[qc], s0_km1 = sr0[qc], s1_km1 = sr1[qc];
+            double p_km2 = prim[qc - sz], p_km1 = prim[qc];
+            double q1_km1 = moist ? pq1[qc] : 0.0, q2_km1 = moist ? pq2[qc] : 0.0;
+            double om_km1 = omega_old(kc), om_k = omega_old(kc + 1);
+            if (ka == 0) {        // face 0 and cell 0
+                X[lane] = dtau * srz[q0];
+                Z[lane] = fB[q0];
+                omr[0] = om_km1;
+                t0r[0] = t0_km1; t1r[0] = t1_km1;
+            }
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k == 0) continue;
+                if (k > kb) break;
+                const long q = q0 + (long)k * sz;
+                const double Jk = J[q], s0_k = sr0[q], s1_k = sr1[q], p_k = prim[q], p_kp1 = prim[q + sz];
+                const double drho_k = cc0[q] - sc0[q], drt_k = cc1[q] - sc1[q];
+                const double q1_k = moist ? pq1[q] : 0.0, q2_k = moist ? pq2[q] : 0.0;
+                const double fP_k = fP[q], fQ_k = fQ[q], srz_k = srz[q], fA_k = fA[q], fB_k = fB[q];
+                const double dw_k = pw(k) - szm[q];
+                const double om_kp1 = omega_old(k + 1);
+                double t0_k, t1_k;
+                trhs_at(k, t0_k, t1_k);
+                const double ofw = met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
+                double detJ_on_kface = 0.5 * (Jk + J_km1);
+                double coeff_P = fP_k;
+                double coeff_Q = fQ_k;
+                if (moist) {
+                    double qf = 0.5 * (q1_k + q1_km1 + q2_k + q2_km1);
+                    coeff_P /= (1.0 + qf);
+                    coeff_Q /= (1.0 + qf);
+                }
+                double theta_t_lo = 0.5 * (p_km2 + p_km1);
+                double theta_t_mid = 0.5 * (p_km1 + p_k);
+                double theta_t_hi = 0.5 * (p_k + p_kp1);
+                double R0_tmp = (-halfg * drho_k + coeff_P * drt_k) * Jk
+                              + (-halfg * drho_km1 + coeff_Q * drt_km1) * J_km1;
+                double R1_tmp = -halfg * (s0_k * Jk + s0_km1 * J_km1)
+                              + (coeff_P * s1_k * Jk + coeff_Q * s1_km1 * J_km1);
+                R1_tmp += (halfg) * (beta_1 * dzi * (om_kp1 - om_km1) + t0_k + t0_km1);
+                R1_tmp += -(coeff_P * (beta_1 * dzi * (om_kp1 * theta_t_hi - om_k * theta_t_mid) + t1_k) +
+                            coeff_Q * (beta_1 * dzi * (om_k * theta_t_mid - om_km1 * theta_t_lo) + t1_km1));
+                double rhs = detJ_on_kface * dw_k + dtau * (detJ_on_kface * srz_k + R0_tmp + dtau * beta_2 * R1_tmp);
+                rhs += detJ_on_kface * ofw;
+                X[k * 64 + lane] = rhs;
+                Y[k * 64 + lane] = fA_k;
+                Z[k * 64 + lane] = fB_k;
+                omr[m] = om_k;
+                t0r[m] = t0_k; t1r[m] = t1_k;
+                tile_level_fence();
+                om_km1 = om_k; om_k = om_kp1;
+                t0_km1 = t0_k; t1_km1 = t1_k;
+                drho_km1 = drho_k; drt_km1 = drt_k;
+                J_km1 = Jk; s0_km1 = s0_k; s1_km1 = s1_k;
+                p_km2 = p_km1; p_km1 = p_k;
+                q1_km1 = q1_k; q2_km1 = q2_k;
+            }
+#pragma unroll
+            for (int m = 0; m <= CM; ++m)      // Omega of the face above the chunk
+                if (m == kb - ka + 1) omr[m] = om_k;
+            if (kb == nz - 1) {   // the top face: the right-hand side of its elimination step
+                const long qt = q0 + (long)nz * sz;
+                X[nz * 64 + lane] = dtau * srz[qt];
+                Y[nz * 64 + lane] = fA[qt];
+                Z[nz * 64 + lane] = fB[qt];
+            }
+        }
+        __syncthreads();
+        // ---- phase 2: elimination (wave 0), c_k / b_k and Omega staged, substitution (wave 0) ----
+        if (wave == 0 && valid) {
+            tile_forward(X, Y, Z, nz, lane);
+            const long qt = q0 + (long)nz * sz;
+            cz[qt] = szm[qt];
+        }
+        __syncthreads();
+        if (work) {
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k > kb) break;
+                Y[k * 64 + lane] = fC[q0 + (long)k * sz] * Z[k * 64 + lane];
+                Z[k * 64 + lane] = omr[m];
+            }
+            if (kb == nz - 1) {
+#pragma unroll
+                for (int m = 0; m <= CM; ++m)
+                    if (m == kb - ka + 1) Z[nz * 64 + lane] = omr[m];
+            }
+        }
+        __syncthreads();
+        if (wave == 0 && valid) tile_backward(X, Y, nz, lane);
+        __syncthreads();
+        // ---- phase 3: rho*w, the averaged Omega, rho and rho*theta of the chunk ----
+        if (work) {
+#pragma unroll
+            for (int m = 0; m < CM; ++m) {
+                const int k = ka + m;
+                if (k > kb) break;
+                const long q = q0 + (long)k * sz;
+                const double sol = X[k * 64 + lane], sol_hi = X[(k + 1) * 64 + lane];
+                const double om_lo = Z[k * 64 + lane], om_hi = Z[(k + 1) * 64 + lane];
+                const double szm_k = szm[q], azm_k = azm[q];
+                const double Jq = J[q], pq = prim[q], p_lo = prim[q - sz], p_hi = prim[q + sz];
+                const double sr0_k = sr0[q], sr1_k = sr1[q], cc0_k = cc0[q], cc1_k = cc1[q];
+                const double cz_k = (k >= 1) ? szm_k + met::w_from_omega(i, j, k, sol, fnu, fnv, z, dxi, dyi) : szm_k;
+                double zflux_lo = beta_2 * sol + beta_1 * om_lo;
+                double zflux_hi = beta_2 * sol_hi + beta_1 * om_hi;
+                double fast_rhs_rho = -(t0r[m] + (zflux_hi - zflux_lo) * dzi) / Jq;
+                double fast_rhs_rhotheta = -(t1r[m] + 0.5 * (zflux_hi * (pq + p_hi) -
+                                                             zflux_lo * (pq + p_lo)) * dzi) / Jq;
+                cz[q] = cz_k;
+                azm[q] = azm_k + facinv * zflux_lo / (1.0 * 1.0);
+                if (k == nz - 1) azm[q + sz] += facinv * zflux_hi / (1.0 * 1.0);
+                cc0[q] = cc0_k + dtau * (sr0_k + fast_rhs_rho);
+                cc1[q] = cc1_k + dtau * (sr1_k + fast_rhs_rhotheta);
+                tile_level_fence();
+            }
+        }
+    };
+    // a recorded (batched) launch takes the largest chunk, so every member of a
+    // batch records the same kernel
+    const int cm = tile_cm8(nz) == 0 ? 0 : (g_rec ? 12 : tile_cm8(nz));
+    if (cm != 0 && P.col_variant == COL_TILE) {
+        auto launch = [&](auto cmc) {
+            launch_tiles<decltype(cmc)::value, 8>(0, nx - 1, 0, ny - 1, s,
+                [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds) {
+                    tile(i, j, valid, lane, wave, lds, cmc);
+                });
+        };
+        switch (cm) {
+            case 4: launch(std::integral_constant<int, 4>{}); break;
+            case 8: launch(std::integral_constant<int, 8>{}); break;
+            default: launch(std::integral_constant<int, 12>{}); break;
+        }
+    } else
     launch_cols(0, nx - 1, 0, ny - 1, s, [=] __device__(int i, int j) {
         const long q0 = L.ix(i, j, 0);
         // Omega of the old increments on face kf (0 at the bottom, the rho*w

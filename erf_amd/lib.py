@@ -243,7 +243,7 @@ class Sim:
     def column_variant(self):
         """(variant in use or -1 while timing, best ms per variant)."""
         v = _I()
-        ms = (ctypes.c_float * 9)()
+        ms = (ctypes.c_float * C.COL_VARIANTS)()
         check(lib().erfx_sim_column_variant(self.h, ctypes.byref(v), ms))
         return v.value, list(ms)
 
@@ -332,7 +332,7 @@ class Ensemble:
 
     def column_variant(self):
         v = _I()
-        ms = (ctypes.c_float * 9)()
+        ms = (ctypes.c_float * C.COL_VARIANTS)()
         check(lib().erfx_ensemble_column_variant(self.h, ctypes.byref(v), ms))
         return v.value, list(ms)
 

@@ -504,18 +504,23 @@ int  erfx_sim_sync(erfx_sim* sim);
  * The results are those of the direct path.  Not used while the learned
  * microphysics, row sampling or the kernel timer is on. */
 int  erfx_sim_set_graph(erfx_sim* sim, int on);
+/* Variants of the acoustic column kernel of erf_fast_rhs_N: 0..8 one lane per
+ * column (levels of loads in flight 2/3/4 x work-group width 64/128/256), 9
+ * the tile form (64 columns per work-group of 16 waves, the sweep's
+ * intermediates in LDS; boxes of up to 128 levels, variant 7 above that), the
+ * default.  Every variant computes the same values. */
+#define ERFX_COL_VARIANTS 10
 /* Autotuner of the acoustic column kernel (no counterpart in the reference):
- * with `on`, the next 27 launches of the erf_fast_rhs_N column kernel cycle
- * through its 9 variants (levels of loads in flight 2/3/4 x work-group width
- * 64/128/256), each timed with HIP events, and the fastest is kept for the
- * rest of the run.  Every variant computes the same values.  On a terrain
- * grid (use_terrain) the erf_fast_rhs_T column kernel has one form and the
- * call does nothing (graph replay stays available). */
+ * with `on`, the next 3 x ERFX_COL_VARIANTS launches of the erf_fast_rhs_N
+ * column kernel cycle through the variants, each timed with HIP events, and
+ * the fastest is kept for the rest of the run.  On a terrain grid
+ * (use_terrain) the call does nothing (graph replay stays available). */
 int  erfx_sim_autotune(erfx_sim* sim, int on);
-/* The column variant in use (0..8; -1 if still timing) and, when ms is not
- * NULL, the best time of each variant in ms (9 values, 0 if not timed). */
+/* The column variant in use (0..ERFX_COL_VARIANTS-1; -1 if still timing) and,
+ * when ms is not NULL, the best time of each variant in ms
+ * (ERFX_COL_VARIANTS values, 0 if not timed). */
 int  erfx_sim_column_variant(erfx_sim* sim, int* variant, float* ms);
-/* Fix the column variant (0..8) and stop any tuning. */
+/* Fix the column variant (0..ERFX_COL_VARIANTS-1) and stop any tuning. */
 int  erfx_sim_set_column_variant(erfx_sim* sim, int variant);
 /* The sim's stream (a hipStream_t) for event timing. */
 void* erfx_sim_stream(erfx_sim* sim);
@@ -608,14 +613,15 @@ int  erfx_ensemble_step(erfx_ensemble* e, double t0, double dt, int nsteps);
 int  erfx_ensemble_set_streams(erfx_ensemble* e, int nstreams);
 /* replay each recorded step as a hipGraph (one graph launch per group and step) */
 int  erfx_ensemble_set_graph(erfx_ensemble* e, int on);
-/* With `on`, the next 9 steps each use one variant of the acoustic column
- * kernel for all members, timed with HIP events; the fastest is kept.
+/* With `on`, the next ERFX_COL_VARIANTS steps each use one variant of the
+ * acoustic column kernel for all members, timed with HIP events; the fastest
+ * is kept.
  * Every variant computes the same values.  Does nothing when a member has a
  * terrain grid (the erf_fast_rhs_T column kernel has one form). */
 int  erfx_ensemble_autotune(erfx_ensemble* e, int on);
 int  erfx_ensemble_set_column_variant(erfx_ensemble* e, int variant);
 /* the variant in use (-1 while timing) and, when ms is not NULL, the step
- * time in ms of each variant timed (9 values) */
+ * time in ms of each variant timed (ERFX_COL_VARIANTS values) */
 int  erfx_ensemble_column_variant(erfx_ensemble* e, int* variant, float* ms);
 /* kernel launches per step (batched, all groups) and the sum of the members' cells */
 int  erfx_ensemble_info(erfx_ensemble* e, long* launches_per_step, long* cells);

@@ -114,12 +114,12 @@ def test_ensemble_autotune_keeps_results():
     a, b = _pairs(cases)
     ens = Ensemble(a)
     ens.autotune(True)
-    ens.step(0.0, 0.5, 9)            # one step per column variant
+    ens.step(0.0, 0.5, C.COL_VARIANTS)     # one step per column variant
     v, ms = ens.column_variant()
-    assert 0 <= v < 9 and all(m > 0 for m in ms) and ms[v] == min(ms)
-    ens.step(4.5, 0.5, 2)
+    assert 0 <= v < C.COL_VARIANTS and all(m > 0 for m in ms) and ms[v] == min(ms)
+    ens.step(0.5 * C.COL_VARIANTS, 0.5, 2)
     for s in b:
-        s.step(0.0, 0.5, 11)
+        s.step(0.0, 0.5, C.COL_VARIANTS + 2)
     assert _diff(a, b) == []
     ens.close()
     for s in a + b:

@@ -63,7 +63,7 @@ def test_graph_replay_matches_direct(case):
     b.close()
 
 
-@pytest.mark.parametrize("variant", list(range(9)))
+@pytest.mark.parametrize("variant", list(range(C.COL_VARIANTS)))
 def test_column_variants_identical(variant):
     cfg, f = P.rising_thermal(40, 24, 16, seed=11, adv=C.ADV_UPWIND3)
     a, b = _pair(cfg, f)
@@ -81,12 +81,37 @@ def test_autotune_picks_a_variant_and_keeps_results():
     a, b = _pair(cfg, f)
     a.set_graph(False)
     a.autotune(True)
-    a.step(0.0, 1.0, 3)          # 3 steps x 10 substeps: every variant timed 3 times
+    a.step(0.0, 1.0, 3)          # 3 steps x 10 substeps: each of the 10 variants timed 3 times
     b.step(0.0, 1.0, 3)
     v, ms = a.column_variant()
-    assert 0 <= v < 9 and all(m > 0 for m in ms) and ms[v] == min(ms)
+    assert 0 <= v < C.COL_VARIANTS and all(m > 0 for m in ms) and ms[v] == min(ms)
     a.step(3.0, 1.0, 2)          # with the chosen variant
     b.step(3.0, 1.0, 2)
+    assert _same(a, b) == []
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("case", ["flat37", "flat_moist_50", "hill21", "hill_moist_30"])
+def test_tile_form_matches_column_form(case):
+    """The tile form of the acoustic column kernel (the default) against the
+    one-lane-per-column form on level counts that leave the last wave's chunk
+    partial and trailing waves empty (37 levels: chunks of 3 over 16 waves;
+    21 levels on terrain: chunks of 3 over 8 waves), flat and terrain, dry
+    and moist: bit-identical after several steps."""
+    if case == "flat37":
+        cfg, f = P.rising_thermal(40, 24, 37, seed=21, adv=C.ADV_UPWIND3)
+    elif case == "flat_moist_50":
+        cfg, f = P.moist_bubble(24, 20, 50, seed=22)
+    elif case == "hill21":
+        cfg, f = P.hill_flow(24, 20, 21, seed=23)
+    else:
+        cfg, f = P.hill_flow(24, 20, 30, seed=24, moist=True)
+    a, b = _pair(cfg, f)
+    a.set_graph(False)
+    a.set_column_variant(7)          # one lane per column
+    a.step(0.0, 1.0, 3)
+    b.step(0.0, 1.0, 3)
     assert _same(a, b) == []
     a.close()
     b.close()
