@@ -13,8 +13,8 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     moist = c.moisture_type != ERFX_MOIST_NONE;
     n_qstate = moist ? 3 : 0;
     if (ncons != (moist ? ERFX_NCONS_MOIST : ERFX_NCONS_DRY)) throw std::runtime_error("ncons does not match moisture_type");
-    if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE))
-        throw std::runtime_error("terrain with diffusion is not restated");
+    if (c.use_terrain && c.les_type != ERFX_LES_NONE)
+        throw std::runtime_error("an LES model on a terrain grid is not restated");
     ng_cons = lay.ng;
     ng_vel = lay.ng - 1;
     rdOcp = R_d / c.c_p;
@@ -79,6 +79,7 @@ void Level::init(const erfx_config& c, const erfx_layout& lay) {
     omega.alloc(L, 1); eddy.alloc(L, EddyDiff::NumDiffs); hfx3.alloc(L, 1);
     tau11.alloc(L, 1); tau22.alloc(L, 1); tau33.alloc(L, 1);
     tau12.alloc(L, 1); tau13.alloc(L, 1); tau23.alloc(L, 1);
+    tau21.alloc(L, 1); tau31.alloc(L, 1); tau32.alloc(L, 1);
     xmom_src.alloc(L, 1); ymom_src.alloc(L, 1); zmom_src.alloc(L, 1); cc_src.alloc(L, ncons);
     rain_accum.assign((size_t)L.sz, 0.0);
     if (c.use_terrain) {

@@ -49,7 +49,125 @@ static void isentropic_hse(double r_sfc, double theta, std::vector<double>& r, s
 // (ERF_init_from_hse.cpp:30-50) and the problem's perturbation
 // (Exec/RegTests/DensityCurrent/ERF_prob.cpp:34-160): a cold cos^2 bubble
 // entering rho only; velocities zero (U_0 = 0).
+// HSEutils::init_isentropic_hse_terrain (Source/Utils/ERF_HSE_utils.H:199-298),
+// klo = 0: the same Newton iteration on one column of cell-centre heights
+// z_cc (the bottom cell's half height is z_cc(0), the spacing between levels
+// z_cc(k) - z_cc(k-1)).
+static void isentropic_hse_terrain(double r_sfc, double theta, std::vector<double>& r, std::vector<double>& p,
+                                   const std::vector<double>& zc, int khi) {
+    const int MAX_ITER = 10;
+    const double TOL = 1.e-8;
+    const double half_dz = zc[0];
+    r[0] = r_sfc;
+    p[0] = p_0 - half_dz * r[0] * CONST_GRAV;
+    for (int iter = 0; iter < MAX_ITER; iter++) {
+        double p_hse = p_0 - half_dz * r[0] * CONST_GRAV;
+        double p_eos = getPgivenRTh(r[0] * theta);
+        double A = p_hse - p_eos;
+        double dpdr = getdPdRgivenConstantTheta(r[0], theta);
+        double drho = A / (dpdr + half_dz * CONST_GRAV);
+        r[0] = r[0] + drho;
+        p[0] = getPgivenRTh(r[0] * theta);
+        if (std::abs(drho) < TOL) break;
+    }
+    for (int k = 1; k <= khi; k++) {
+        double dz_loc = (zc[k] - zc[k - 1]);
+        r[k] = r[k - 1];
+        for (int iter = 0; iter < MAX_ITER; iter++) {
+            double p_hse = p[k - 1] - dz_loc * 0.5 * (r[k - 1] + r[k]) * CONST_GRAV;
+            double p_eos = getPgivenRTh(r[k] * theta);
+            double A = p_hse - p_eos;
+            double dpdr = getdPdRgivenConstantTheta(r[k], theta);
+            double drho = A / (dpdr + dz_loc * CONST_GRAV);
+            r[k] = r[k] + drho;
+            p[k] = getPgivenRTh(r[k] * theta);
+            if (std::abs(drho) < TOL * r[k - 1]) break;
+        }
+    }
+}
+
+// The same case on a terrain-following grid (DensityCurrent_detJ2): the
+// terrain branches of erf_init_dens_hse (ERF_init_density_hse_dry.H:29-74) and
+// erf_enforce_hse (ERF_init1d.cpp:190-303) per column of the grid's z_cc, the
+// lateral ghost columns of p, pi copied from the edge (:248-300), and the
+// bubble placed at z_cc (Exec/RegTests/DensityCurrent/ERF_prob.cpp:60-95).
+void Level::init_density_current_T(double x_c, double z_c, double x_r, double z_r, double T_pert) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
+    const double dx = cfg.dx;
+    const double T_sfc = 300.;
+    const double rho_sfc = p_0 / (R_d * T_sfc);
+    const double Thetabar = T_sfc;
+    const double g = gravity;
+    Arr r0a = A(r0), p0a = A(p0), pi0a = A(pi0), c = A(cons), zcc = A(z_cc);
+    std::vector<double> r(nz + 1), p(nz + 1), zc(nz + 1);
+    for (int j = -1; j <= ny; ++j)
+        for (int i = -1; i <= nx; ++i) {
+            for (int k = 0; k <= nz; ++k) zc[k] = zcc(i, j, k);
+            isentropic_hse_terrain(rho_sfc, Thetabar, r, p, zc, nz - 1);
+            for (int k = 0; k <= nz - 1; ++k) r0a(i, j, k) = r[k];
+            r0a(i, j, -1) = r0a(i, j, 0);
+            r0a(i, j, nz) = r0a(i, j, nz - 1);
+            // erf_enforce_hse, klo = 0, khi = nz
+            double hz = zcc(i, j, 0);
+            p0a(i, j, 0) = p_0 - hz * r0a(i, j, 0) * g;
+            pi0a(i, j, 0) = getExnergivenP(p0a(i, j, 0), rdOcp);
+            p0a(i, j, -1) = p_0 + hz * r0a(i, j, 0) * g;
+            pi0a(i, j, -1) = getExnergivenP(p0a(i, j, -1), rdOcp);
+            for (int k = 1; k <= nz; ++k) {
+                double dz_loc = (zcc(i, j, k) - zcc(i, j, k - 1));
+                double dens_interp = 0.5 * (r0a(i, j, k) + r0a(i, j, k - 1));
+                p0a(i, j, k) = p0a(i, j, k - 1) - dz_loc * dens_interp * g;
+                pi0a(i, j, k) = getExnergivenP(p0a(i, j, k), rdOcp);
+            }
+        }
+    for (int k = 0; k <= nz; ++k) {
+        for (int j = -1; j <= ny; ++j) {
+            p0a(-1, j, k) = p0a(0, j, k); pi0a(-1, j, k) = pi0a(0, j, k);
+            p0a(nx, j, k) = p0a(nx - 1, j, k); pi0a(nx, j, k) = pi0a(nx - 1, j, k);
+        }
+        for (int i = -1; i <= nx; ++i) {
+            p0a(i, -1, k) = p0a(i, 0, k); pi0a(i, -1, k) = pi0a(i, 0, k);
+            p0a(i, ny, k) = p0a(i, ny - 1, k); pi0a(i, ny, k) = pi0a(i, ny - 1, k);
+        }
+    }
+    // the remaining ghost columns as their nearest filled column (periodic
+    // images in ERF; the case is laterally uniform)
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -ng; j <= ny + ng; ++j)
+            for (int i = -ng; i <= nx + ng; ++i) {
+                const int ii = std::min(std::max(i, -1), nx), jj = std::min(std::max(j, -1), ny);
+                if (ii == i && jj == j) continue;
+                r0a(i, j, k) = r0a(ii, jj, k); p0a(i, j, k) = p0a(ii, jj, k); pi0a(i, j, k) = pi0a(ii, jj, k);
+            }
+    std::fill(cons.v.begin(), cons.v.end(), 0.0);
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -1; j <= ny; ++j)
+            for (int i = -1; i <= nx; ++i) {
+                c(i, j, k, 0) = r0a(i, j, k);
+                c(i, j, k, 1) = getRhoThetagivenP(p0a(i, j, k));
+            }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = cfg.prob_lo[0] + (i + 0.5) * dx;
+                const double z = zcc(i, j, k);
+                double Lr = std::sqrt(std::pow((x - x_c) / x_r, 2) + std::pow((z - z_c) / z_r, 2));
+                double pert = 0.0;
+                if (Lr <= 1.0) {
+                    double dT = T_pert * (std::cos(PI * Lr) + 1.0) / 2.0;
+                    double Tbar_hse = p0a(i, j, k) / (R_d * r0a(i, j, k));
+                    double theta_perturbed = (Tbar_hse + dT) * std::pow(p_0 / p0a(i, j, k), rdOcp);
+                    pert = getRhoThetagivenP(p0a(i, j, k)) / theta_perturbed - r0a(i, j, k);
+                }
+                c(i, j, k, 0) += pert;
+            }
+    std::fill(xvel.v.begin(), xvel.v.end(), 0.0);
+    std::fill(yvel.v.begin(), yvel.v.end(), 0.0);
+    std::fill(zvel.v.begin(), zvel.v.end(), 0.0);
+}
+
 void Level::init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert) {
+    if (terrain) { init_density_current_T(x_c, z_c, x_r, z_r, T_pert); return; }
     const int nx = L.nx, ny = L.ny, nz = L.nz, ng = L.ng;
     const double dz = cfg.dz, dx = cfg.dx;
     // base density

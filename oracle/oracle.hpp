@@ -173,6 +173,40 @@ inline double h_zeta_EdgeJ(Arr z, int i, int j, int k, double dzi) {     // :268
 inline double h_zeta_EdgeI(Arr z, int i, int j, int k, double dzi) {     // :307-316
     return 0.25 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k - 1) - z(i + 1, j, k - 1));
 }
+// cell-centre, k-face and edge-centre slopes for the stresses and the
+// diffusive fluxes (:54-83, :192-219, :236-262, :279-303, :318-344)
+inline double h_xi_Cell(Arr z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j, k) + z(i + 1, j + 1, k) + z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1)
+                         - z(i, j, k) - z(i, j + 1, k) - z(i, j, k + 1) - z(i, j + 1, k + 1));
+}
+inline double h_eta_Cell(Arr z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i, j + 1, k) + z(i + 1, j + 1, k) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1)
+                         - z(i, j, k) - z(i + 1, j, k) - z(i, j, k + 1) - z(i + 1, j, k + 1));
+}
+inline double h_xi_Kface(Arr z, int i, int j, int k, double dxi) {
+    return 0.5 * dxi * (z(i + 1, j, k) + z(i + 1, j + 1, k) - z(i, j, k) - z(i, j + 1, k));
+}
+inline double h_eta_Kface(Arr z, int i, int j, int k, double dyi) {
+    return 0.5 * dyi * (z(i, j + 1, k) + z(i + 1, j + 1, k) - z(i, j, k) - z(i + 1, j, k));
+}
+inline double h_xi_EdgeK(Arr z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j, k) + z(i + 1, j, k + 1) - z(i - 1, j, k) - z(i - 1, j, k + 1));
+}
+inline double h_eta_EdgeK(Arr z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i, j + 1, k) + z(i, j + 1, k + 1) - z(i, j - 1, k) - z(i, j - 1, k + 1));
+}
+inline double h_xi_EdgeJ(Arr z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j + 1, k) + z(i + 1, j, k) - z(i - 1, j + 1, k) - z(i - 1, j, k));
+}
+inline double h_eta_EdgeJ(Arr z, int i, int j, int k, double dyi) {
+    return dyi * (z(i, j + 1, k) - z(i, j, k));
+}
+inline double h_xi_EdgeI(Arr z, int i, int j, int k, double dxi) {
+    return dxi * (z(i + 1, j, k) - z(i, j, k));
+}
+inline double h_eta_EdgeI(Arr z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i + 1, j + 1, k) + z(i, j + 1, k) - z(i + 1, j - 1, k) - z(i, j - 1, k));
+}
 // slopes of the z-face (i, j, k) used by OmegaFromW / WFromOmega (:368-381)
 inline double zlo_xi(Arr z, int i, int j, int k, double dxi) {
     return 0.5 * dxi * (z(i + 1, j + 1, k) + z(i + 1, j, k) - z(i, j + 1, k) - z(i, j, k));
@@ -243,6 +277,7 @@ struct Level {
     Field prim, pi_stage, fast_coeffs;
     Field omega, eddy, hfx3;
     Field tau11, tau22, tau33, tau12, tau13, tau23;
+    Field tau21, tau31, tau32;   // terrain grids: the non-symmetric stresses
     Field xmom_src, ymom_src, zmom_src, cc_src;
 
     void init(const erfx_config& c, const erfx_layout& lay);
@@ -275,6 +310,14 @@ struct Level {
                           Arr t11, Arr t22, Arr t33, Arr t12, Arr t13, Arr t23);
     void turbulent_viscosity(Arr cons_in, Arr u, Arr v, Arr w);
     void make_tau_terms(StateA data, Arr u, Arr v, Arr w);
+    // terrain-following grid (molecular diffusion)
+    void compute_strain_T(Box bxcc, Box tbxxy, Box tbxxz, Box tbxyz, Arr u, Arr v, Arr w, Arr t11, Arr t22,
+                          Arr t33, Arr t12, Arr t13, Arr t21, Arr t23, Arr t31, Arr t32);
+    void compute_stress_cons_T(Box bxcc, Box tbxxy, Box tbxxz, Box tbxyz, double mu_eff, bool use_rho, Arr cd,
+                               Arr t11, Arr t22, Arr t33, Arr t12, Arr t13, Arr t21, Arr t23, Arr t31, Arr t32,
+                               Arr er);
+    void make_tau_terms_T(StateA data, Arr u, Arr v, Arr w);
+    void diffusion_src_state_T(const Box& bx, int start_comp, int num_comp, Arr cell_data, Arr cell_prim, Arr rhs);
     void make_mom_sources(StateA data, Arr prim_);
     // PlaneAverage of ERF_make_buoyancy.cpp:216-255: per level k of the valid
     // box, the sums of value / (nx ny); rows rho, theta, qv, qc, qp
@@ -294,6 +337,7 @@ struct Level {
                     Arr fc, StateA data, StateA scratch, double dtau, double beta_s, double facinv);
     void make_sources(StateA data, Arr prim_);
     void init_density_current(double x_c, double z_c, double x_r, double z_r, double T_pert);
+    void init_density_current_T(double x_c, double z_c, double x_r, double z_r, double T_pert);
     void init_scalar_advdiff(const double* params);
     void init_taylor_green(double rho_0, double V_0, double M_0);
     void init_couette_poiseuille(double rho_0, double T_0, double u_0, double v_0, double w_0, int prob_type,

@@ -176,8 +176,12 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     (void)nrk;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const bool use_diff = cfg.molec_diff_type != ERFX_MOLEC_NONE || cfg.les_type != ERFX_LES_NONE;
-    if (use_diff && terrain) throw std::runtime_error("diffusion on a terrain-following grid is not restated");
-    if (use_diff) make_tau_terms(data, u, v, w);
+    if (use_diff && terrain && cfg.les_type != ERFX_LES_NONE)
+        throw std::runtime_error("an LES model on a terrain-following grid is not restated");
+    if (use_diff) {
+        if (terrain) make_tau_terms_T(data, u, v, w);
+        else make_tau_terms(data, u, v, w);
+    }
     Arr rwr = rhs.zmom;
     // S_rhs[zmom].setVal(0.0) (:232)
     {
@@ -247,7 +251,10 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     });
     advection_src_scalars(bx, 1, 1, ax, ay, az, prim_, cr, cfg.dycore_horiz_adv, cfg.dycore_vert_adv,
                           cfg.dycore_horiz_upw_frac, cfg.dycore_vert_upw_frac);
-    if (use_diff) diffusion_src_state(bx, 1, 1, c, prim_, cr);
+    if (use_diff) {
+        if (terrain) diffusion_src_state_T(bx, 1, 1, c, prim_, cr);
+        else diffusion_src_state(bx, 1, 1, c, prim_, cr);
+    }
     Arr cs = A(cc_src);
     loop(bx, [&](int i, int j, int k) {
         cr(i, j, k, 0) += cs(i, j, k, 0);
@@ -258,7 +265,34 @@ void Level::slow_rhs_pre(int nrk, double dt, StateA rhs, StateA data, Arr prim_,
     Arr rur = rhs.xmom, rvr = rhs.ymom;
     if (terrain) advection_src_mom_T(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
     else advection_src_mom(tbx, tby, tbz, rur, rvr, rwr, u, v, w, data.xmom, data.ymom, om);
-    if (use_diff) {
+    if (use_diff && terrain) {   // DiffusionSrcForMom_T (ERF_DiffusionSrcForMom_T.cpp:30-79)
+        Arr t11 = A(tau11), t22 = A(tau22), t33 = A(tau33), t12 = A(tau12), t13 = A(tau13), t23 = A(tau23);
+        Arr t21 = A(tau21), t31 = A(tau31), t32 = A(tau32), Jc = A(detJ);
+        loop(tbx, [&](int i, int j, int k) {
+            double mf = 1.0;
+            double diffContrib = ((t11(i, j, k) - t11(i - 1, j, k)) * dxi * mf
+                                + (t12(i, j + 1, k) - t12(i, j, k)) * dyi * mf
+                                + (t13(i, j, k + 1) - t13(i, j, k)) * dzi);
+            diffContrib /= 0.5 * (Jc(i, j, k) + Jc(i - 1, j, k));
+            rur(i, j, k) -= diffContrib;
+        });
+        loop(tby, [&](int i, int j, int k) {
+            double mf = 1.0;
+            double diffContrib = ((t21(i + 1, j, k) - t21(i, j, k)) * dxi * mf
+                                + (t22(i, j, k) - t22(i, j - 1, k)) * dyi * mf
+                                + (t23(i, j, k + 1) - t23(i, j, k)) * dzi);
+            diffContrib /= 0.5 * (Jc(i, j, k) + Jc(i, j - 1, k));
+            rvr(i, j, k) -= diffContrib;
+        });
+        loop(tbz, [&](int i, int j, int k) {
+            double mf = 1.0;
+            double diffContrib = ((t31(i + 1, j, k) - t31(i, j, k)) * dxi * mf
+                                + (t32(i, j + 1, k) - t32(i, j, k)) * dyi * mf
+                                + (t33(i, j, k) - t33(i, j, k - 1)) * dzi);
+            diffContrib /= 0.5 * (Jc(i, j, k) + Jc(i, j, k - 1));
+            rwr(i, j, k) -= diffContrib;
+        });
+    } else if (use_diff) {
         Arr t11 = A(tau11), t22 = A(tau22), t33 = A(tau33), t12 = A(tau12), t13 = A(tau13), t23 = A(tau23);
         loop(tbx, [&](int i, int j, int k) {
             double mf = 1.0;
@@ -359,12 +393,18 @@ void Level::slow_rhs_post(int nrk, double dt, StateA rhs, StateA old, StateA sne
     advection_src_scalars(tbx, ivar, 1, scratch.xmom, scratch.ymom, scratch.zmom, prim_, cr,
                           cfg.dryscal_horiz_adv, cfg.dryscal_vert_adv, cfg.dryscal_horiz_upw_frac,
                           cfg.dryscal_vert_upw_frac);
-    if (use_diff) diffusion_src_state(tbx, ivar, 1, ncons_, prim_, cr);
+    if (use_diff) {
+        if (terrain) diffusion_src_state_T(tbx, ivar, 1, ncons_, prim_, cr);
+        else diffusion_src_state(tbx, ivar, 1, ncons_, prim_, cr);
+    }
     if (moist) {
         advection_src_scalars(tbx, ERFX_RHOQ1, n_qstate, scratch.xmom, scratch.ymom, scratch.zmom, prim_, cr,
                               cfg.moistscal_horiz_adv, cfg.moistscal_vert_adv, cfg.moistscal_horiz_upw_frac,
                               cfg.moistscal_vert_upw_frac);
-        if (use_diff) diffusion_src_state(tbx, ERFX_RHOQ1, n_qstate, ncons_, prim_, cr);
+        if (use_diff) {
+            if (terrain) diffusion_src_state_T(tbx, ERFX_RHOQ1, n_qstate, ncons_, prim_, cr);
+            else diffusion_src_state(tbx, ERFX_RHOQ1, n_qstate, ncons_, prim_, cr);
+        }
     }
     Arr cs = A(cc_src);
     loop(tbx, [&](int i, int j, int k) {

@@ -63,6 +63,16 @@ def main():
                         lo=np.array(pf["lo"]), hi=np.array(pf["hi"]), n=np.array(pf["n"]),
                         plane=d[:, :, 0, :])
     print("wrote fixture:", pf["names"], d.shape, "time", pf["time"], "steps", pf["nstep"])
+    # the same case on a terrain-following grid of flat 100 m levels over a
+    # 50 m computational dz (detJ = 2): its y-invariant j = 0 plane
+    pf = read_plotfile(os.path.join(ref, "Tests", "ERFGoldFiles", "DensityCurrent_detJ2"))
+    d = pf["data"]
+    assert np.all(np.isfinite(d)) and np.all(d == d[:, :, :1, :])
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "density_current_detJ2_step10.npz"),
+                        names=np.array(pf["names"]), time=pf["time"], nstep=pf["nstep"],
+                        lo=np.array(pf["lo"]), hi=np.array(pf["hi"]), n=np.array(pf["n"]),
+                        plane=d[:, :, 0, :])
+    print("wrote fixture:", pf["names"], d.shape, "time", pf["time"], "steps", pf["nstep"])
 
 
 if __name__ == "__main__":

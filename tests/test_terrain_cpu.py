@@ -220,11 +220,48 @@ def test_wall_kinematic_condition():
     assert np.abs(w0 - us * dhdx).max() < 0.05 * np.abs(w0).max()
 
 
-def test_terrain_with_diffusion_rejected():
+def test_terrain_with_les_rejected():
     cfg, _ = P.hill_flow(8, 8, 8)
-    cfg.molec_diff_type = C.MOLEC_CONSTANT
+    cfg.les_type = C.LES_SMAGORINSKY
     with pytest.raises(RuntimeError):
         Oracle(cfg)
+
+
+@pytest.mark.parametrize("molec", [C.MOLEC_CONSTANT, C.MOLEC_CONSTANT_ALPHA])
+def test_terrain_diffusion_on_flat_levels_is_the_flat_diffusion(molec):
+    """Over flat ground with uniform levels every slope of the terrain grid
+    vanishes and detJ = 1, so DiffusionSrcForState_T, ComputeStrain_T /
+    ComputeStress_T and DiffusionSrcForMom_T reduce to their flat forms (which
+    the DensityCurrent, ScalarDiffusion and Couette gold files pin): the
+    terrain step must give the flat step's fields to rounding.  A dropped
+    metric factor or a slope term with the wrong sign would not vanish here only
+    if it multiplied detJ or h_zeta, which is what DensityCurrent_detJ2 pins."""
+    cfg, f = P.hill_flow(12, 10, 10, seed=5, h0=0.0, noise=0.5)
+    cfg.molec_diff_type = molec
+    cfg.dynamic_viscosity = 20.0
+    cfg.rho0_trans = 1.0
+    cfg.alpha_T = 30.0
+    cfg.alpha_C = 10.0
+    orc_t = Oracle(cfg)
+    orc_t.load(f)
+    flat = C.Config.from_buffer_copy(cfg)
+    flat.use_terrain = 0
+    orc_f = Oracle(flat)
+    orc_f.load({k: v for k, v in f.items() if k != C.F_Z_ND})
+    orc_t.step(0.0, 1.0, 2)
+    orc_f.step(0.0, 1.0, 2)
+    L = C.layout_for(cfg)
+    assert L.nelem == C.layout_for(flat).nelem
+    ext = {C.F_CONS: (0, 0, 0), C.F_XVEL: (1, 0, 0), C.F_YVEL: (0, 1, 0), C.F_ZVEL: (0, 0, 1)}
+    for fid, (ex, ey, ez) in ext.items():
+        a = _valid(cfg, L, orc_t.get(fid), ex, ey, ez)
+        b = _valid(cfg, L, orc_f.get(fid), ex, ey, ez)
+        init = _valid(cfg, L, f[fid], ex, ey, ez)
+        for c in range(a.shape[0]):
+            if fid == C.F_CONS and c in (C.RHOKE, C.RHOQKE):
+                continue
+            change = np.abs(b[c] - init[c]).max()     # 0 for the scalar, which starts at 0
+            assert np.abs(a[c] - b[c]).max() <= 1e-10 * change, (fid, c)
 
 
 def test_stretched_levels_and_btf_grid():
