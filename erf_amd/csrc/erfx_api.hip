@@ -92,8 +92,8 @@ int validate_config(const erfx_config& c) {
     }
     if (c.buoyancy_type < 1 || c.buoyancy_type > 4) { set_error("buoyancy_type must be 1, 2, 3 or 4"); return ERFX_ERR_ARG; }
     if (c.precision != ERFX_PREC_FP64 && c.precision != ERFX_PREC_MIXED) { set_error("bad precision"); return ERFX_ERR_ARG; }
-    if (c.use_terrain && (c.molec_diff_type != ERFX_MOLEC_NONE || c.les_type != ERFX_LES_NONE)) {
-        set_error("diffusion on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
+    if (c.use_terrain && c.les_type != ERFX_LES_NONE) {
+        set_error("an LES model on a terrain-following grid is not implemented"); return ERFX_ERR_UNSUPPORTED;
     }
     if (c.moisture_type == ERFX_MOIST_KESSLER_NORAIN && c.buoyancy_type != 1) {
         // asserted by ERF (ERF_make_buoyancy.cpp:171-175)
@@ -378,6 +378,12 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
                          &c->tw.du, &c->tw.dv, &c->tw.nu, &c->tw.nv, &c->tw.om};
         for (double** b : tb)
             if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
+        if (c->P.use_diff) {
+            double** db[] = {&c->sw.tau_t[0], &c->sw.tau_t[1], &c->sw.tau_t[2], &c->sw.er, &c->sw.dfx, &c->sw.dfy};
+            for (double** b : db)
+                if ((rc = dalloc(c->allocs, b, ne))) { erfx_ctx_destroy(c); return rc; }
+            if ((rc = dalloc(c->allocs, &c->sw.dstate, (size_t)cfg->ncons * ne))) { erfx_ctx_destroy(c); return rc; }
+        }
         c->P.z_nd = c->z_nd; c->P.detJ = c->detJ; c->P.ax = c->ax; c->P.ay = c->ay; c->P.az = c->az;
         c->P.z_cc = c->z_cc;
         c->P.mrx = c->mrx;
@@ -829,7 +835,7 @@ int erfx_slow_rhs_post(erfx_ctx* c, int nrk, double slow_dt, erfx_state* rhs, co
         set_error("null argument"); return ERFX_ERR_ARG;
     }
     if (c->P.use_turb && !eddy) { set_error("LES model needs the eddy viscosity"); return ERFX_ERR_ARG; }
-    slow_rhs_post(c->P, nrk, slow_dt, *rhs, *old, *snew, *data, prim, *scratch, eddy, c->stream);
+    slow_rhs_post(c->P, c->sw, nrk, slow_dt, *rhs, *old, *snew, *data, prim, *scratch, eddy, c->stream);
     return launch_status();
 }
 
@@ -1076,7 +1082,7 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
                            dtau, beta_s, inv_fac, st, ch);
             if (!chain || ks == nsub - 1) apply_bcs(P, Ss, un, vn, wn, 1, 1, true, false, st);
         }
-        slow_rhs_post(P, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);
+        slow_rhs_post(P, c->sw, nrk, slow_dt, Fs, So, Sn, Ss, s->prim, Sc, s->eddy, st);
         apply_bcs(P, Sn, un, vn, wn, P.ng_cons, P.ng_vel, false, false, st);
     }
     // advance_microphysics on the new state (ERF_Advance.cpp:172-174)

@@ -121,14 +121,30 @@ struct SlowWork {
     double* pavg = nullptr;       // moist buoyancy plane averages (PLANE_NOUT x nz)
     double* pavg_part = nullptr;  // their partial sums
     double* omega = nullptr;      // Omega on z-faces grown by one (terrain grids)
+    // terrain grids with diffusion: the non-symmetric stresses tau21, tau31,
+    // tau32, the expansion rate, the two lateral diffusive fluxes of one
+    // scalar and the diffusion term of each diffused cons component
+    double* tau_t[3] = {nullptr, nullptr, nullptr};
+    double* er = nullptr;
+    double* dfx = nullptr;
+    double* dfy = nullptr;
+    double* dstate = nullptr;     // ncons components (cons indexing)
 };
+// erf_make_tau_terms and DiffusionSrcForState_T on a terrain grid (molecular
+// diffusion; erfx_slow.hip): stresses into W.tau / W.tau_t, and the diffusion
+// term of cons components n0..n0+nc-1 into W.dstate
+void make_tau_terms_T(const Params& P, const SlowWork& W, const double* cons, const double* u, const double* v,
+                      const double* w, hipStream_t s);
+void diffusion_state_T(const Params& P, const SlowWork& W, int n0, int nc, const double* cons, const double* prim,
+                       hipStream_t s);
 void turbulent_viscosity(const Params& P, const SlowWork& W, const double* cons, const double* u,
                          const double* v, const double* w, double* eddy, hipStream_t s);
 void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, const erfx_state& rhs,
                   const erfx_state& data, const double* prim, const erfx_state& scratch, const double* u,
                   const double* v, const double* w, const double* eddy, const double* r0, const double* p0,
                   hipStream_t s);
-void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& rhs, const erfx_state& old,
+void slow_rhs_post(const Params& P, const SlowWork& W, int nrk, double slow_dt, const erfx_state& rhs,
+                   const erfx_state& old,
                    const erfx_state& snew, const erfx_state& data, const double* prim,
                    const erfx_state& scratch, const double* eddy, hipStream_t s);
 

@@ -739,7 +739,14 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const long ne = L.ne, sy = L.sy, sz = L.sz;
     const double dxi = P.dxInv[0], dyi = P.dxInv[1], dzi = P.dxInv[2];
     const bool use_diff = P.use_diff, consA = P.consA, turb = P.use_turb;
-    if (use_diff) make_tau_terms(P, W, data.cons, u, v, w, eddy, s);
+    if (use_diff) {
+        if (P.terrain) {   // make_tau_terms' terrain branch; DiffusionSrcForState_T of theta
+            make_tau_terms_T(P, W, data.cons, u, v, w, s);
+            diffusion_state_T(P, W, 1, 1, data.cons, prim, s);
+        } else {
+            make_tau_terms(P, W, data.cons, u, v, w, eddy, s);
+        }
+    }
 
     // p' on cells grown by one, k >= 0 (ERF_slow_rhs_pre.cpp:337-358)
     double* pp = W.pp;
@@ -796,6 +803,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
         const Zn z{P.z_nd, L};
         const Fld fru{ru, L}, frv{rv, L};
         const double* Ax = P.ax; const double* Ay = P.ay; const double* Az = P.az; const double* Jc = P.detJ;
+        const bool tdiff = use_diff;
+        const double* dst_theta = use_diff ? W.dstate + ne : nullptr;
         double* omw = W.omega;
         double* sxm = scratch.xmom; double* sym = scratch.ymom; double* szm = scratch.zmom;
         launch_box(Box(-1, -1, -1, nx, ny, nz + 1), s, [=] __device__(int i, int j, int k) {
@@ -839,6 +848,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             }
             double invdetJ = (dJ > 0.) ? 1. / dJ : 1.;
             double r1 = -invdetJ * mfsq * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
+            if (tdiff) r1 -= dst_theta[q];   // DiffusionSrcForState_T (made above)
             double src = 0.0;
             if (rayT && RA.act_cc(q, k)) {   // ERF_make_sources.cpp:203-225 with z_cc
                 double theta = prim[q];
@@ -908,6 +918,8 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     const double* cons = data.cons;
     const double* t11 = W.tau[0]; const double* t22 = W.tau[1]; const double* t33 = W.tau[2];
     const double* t12 = W.tau[3]; const double* t13 = W.tau[4]; const double* t23 = W.tau[5];
+    const double* t21 = W.tau_t[0]; const double* t31 = W.tau_t[1]; const double* t32 = W.tau_t[2];
+    const double* Jd = P.detJ;
     const bool cor = P.cfg.use_coriolis != 0;
     const double cf = P.cfg.coriolis_factor, sinphi = P.cfg.sinphi, cosphi = P.cfg.cosphi;
     const double g0 = P.cfg.abl_geo_forcing[0], g1 = P.cfg.abl_geo_forcing[1], g2 = P.cfg.abl_geo_forcing[2];
@@ -972,7 +984,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             }
             double r = -src;
-            if (use_diff)
+            if (use_diff && terr) {   // DiffusionSrcForMom_T (ERF_DiffusionSrcForMom_T.cpp:50-58)
+                double diffContrib = ((t11[q] - t11[q - 1]) * dxi * 1.0 + (t12[q + sy] - t12[q]) * dyi * 1.0 +
+                                      (t13[q + sz] - t13[q]) * dzi);
+                diffContrib /= 0.5 * (Jd[q] + Jd[q - 1]);
+                r -= diffContrib;
+            } else if (use_diff)
                 r -= ((t11[q] - t11[q - 1]) * dxi * 1.0 + (t12[q + sy] - t12[q]) * dyi * 1.0 +
                       (t13[q + sz] - t13[q]) * dzi);
             double xs = 0.0;
@@ -1036,7 +1053,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             }
             double r = -src;
-            if (use_diff)
+            if (use_diff && terr) {   // :60-68, tau21 in place of tau12
+                double diffContrib = ((t21[q + 1] - t21[q]) * dxi * 1.0 + (t22[q] - t22[q - sy]) * dyi * 1.0 +
+                                      (t23[q + sz] - t23[q]) * dzi);
+                diffContrib /= 0.5 * (Jd[q] + Jd[q - sy]);
+                r -= diffContrib;
+            } else if (use_diff)
                 r -= ((t12[q + 1] - t12[q]) * dxi * 1.0 + (t22[q] - t22[q - sy]) * dyi * 1.0 +
                       (t23[q + sz] - t23[q]) * dzi);
             double ys = 0.0;
@@ -1130,7 +1152,12 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
                       (zflux_hi - zflux_lo) * dzi;
             }
             double r = -src;
-            if (use_diff)
+            if (use_diff && terr) {   // :70-78, tau31 and tau32 in place of tau13 and tau23
+                double diffContrib = ((t31[q + 1] - t31[q]) * dxi * 1.0 + (t32[q + sy] - t32[q]) * dyi * 1.0 +
+                                      (t33[q] - t33[q - sz]) * dzi);
+                diffContrib /= 0.5 * (Jd[q] + Jd[q - sz]);
+                r -= diffContrib;
+            } else if (use_diff)
                 r -= ((t13[q + 1] - t13[q]) * dxi * 1.0 + (t23[q + sy] - t23[q]) * dyi * 1.0 +
                       (t33[q] - t33[q - sz]) * dzi);
             double zs;
@@ -1278,6 +1305,7 @@ struct PostOp {
     const double* oc;
     double* crb;
     const double* prim;
+    const double* dstate;     // terrain grid: the diffusion terms (DiffusionSrcForState_T), else null
     __device__ __forceinline__ void face(const double* a, int t, double fr, long q, long st, double* g) const {
         const double f = a[q];
         for (int m = 0; m < NF; ++m) {
@@ -1299,7 +1327,9 @@ struct PostOp {
             const double* ps = prim + (n - 1) * ne;
             double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
             double r = -invdetJ * (1.0 * 1.0) * ((gx1[m] - gx0[m]) * dxi + (gy1[m] - gy0[m]) * dyi + (gz1[m] - gz0[m]) * dzi);
-            if (use_diff) {
+            if (use_diff && dstate) {
+                r -= dstate[q + n * ne];
+            } else if (use_diff) {
                 const int dsx0 = (dxlo && i == 0) ? -1 : 0, dsx1 = (dxhi && i == nx - 1) ? 1 : 0;
                 const int dsy0 = (dylo && j == 0) ? -1 : 0, dsy1 = (dyhi && j == ny - 1) ? 1 : 0;
                 double dfx0 = diff_flux_bc(nc, ps, mu_h, q, 1, dxi, ae, consA, turb, dsx0) * 1.0;
@@ -1330,9 +1360,9 @@ struct ScalarBlock {
     int mu_h, mu_v;           // EddyDiff comps
     bool clip;
 };
-static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, double slow_dt, const erfx_state& rhs,
-                       const erfx_state& old, const erfx_state& snew, const erfx_state& data, const double* prim,
-                       const erfx_state& scratch, const double* eddy, hipStream_t s) {
+static void post_block(const Params& P, const SlowWork& W, const ScalarBlock& B, bool copy_slow, double slow_dt,
+                       const erfx_state& rhs, const erfx_state& old, const erfx_state& snew, const erfx_state& data,
+                       const double* prim, const erfx_state& scratch, const double* eddy, hipStream_t s) {
     const Lay L = P.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long ne = L.ne, sy = L.sy, sz = L.sz;
@@ -1360,6 +1390,12 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
     double* nc = snew.cons;
     const double* oc = old.cons;
     double* crb = rhs.cons;
+    // on a terrain grid the diffusion terms are made beforehand (DiffusionSrcForState_T)
+    const double* dst = nullptr;
+    if (use_diff && P.terrain) {
+        diffusion_state_T(P, W, n0, nc_b, nc, prim, s);
+        dst = W.dstate;
+    }
     if (!g_rec && (nc_b == 1 || nc_b == 3)) {
         // one evaluation per face (the batched ensembles record the per-cell form below)
         auto run = [&](auto mc, auto nf) {
@@ -1373,6 +1409,7 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
             op.dxlo = dxlo; op.dxhi = dxhi; op.dylo = dylo; op.dyhi = dyhi;
             op.mu_h = mu_h; op.mu_v = mu_v; op.n0 = n0; op.ncons = ncons;
             op.ax = ax; op.ay = ay; op.az = az; op.cur = cur; op.nc = nc; op.oc = oc; op.crb = crb; op.prim = prim;
+            op.dstate = dst;
             launch_march(op, nx, ny, nz, s);
         };
         if (mixed_rt) {
@@ -1411,7 +1448,9 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
             }
             double invdetJ = Jc ? ((Jc[q] > 0.) ? 1. / Jc[q] : 1.) : 1.;
             double r = -invdetJ * (1.0 * 1.0) * ((gx1 - gx0) * dxi + (gy1 - gy0) * dyi + (gz1 - gz0) * dzi);
-            if (use_diff) {
+            if (use_diff && dst) {
+                r -= dst[q + n * ne];
+            } else if (use_diff) {
                 const int dsx0 = (dxlo && i == 0) ? -1 : 0, dsx1 = (dxhi && i == nx - 1) ? 1 : 0;
                 const int dsy0 = (dylo && j == 0) ? -1 : 0, dsy1 = (dyhi && j == ny - 1) ? 1 : 0;
                 double dfx0 = diff_flux_bc(nc, ps, mu_h, q, 1, dxi, ae, consA, turb, dsx0) * 1.0;
@@ -1435,8 +1474,8 @@ static void post_block(const Params& P, const ScalarBlock& B, bool copy_slow, do
 // slow variables are RhoScalar (:168-172) and, with moisture, the n_qstate
 // species as one block with the moist advection types (:173-175, :364-434);
 // then S_new <- S_data.
-void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& rhs, const erfx_state& old,
-                   const erfx_state& snew, const erfx_state& data, const double* prim,
+void slow_rhs_post(const Params& P, const SlowWork& W, int nrk, double slow_dt, const erfx_state& rhs,
+                   const erfx_state& old, const erfx_state& snew, const erfx_state& data, const double* prim,
                    const erfx_state& scratch, const double* eddy, hipStream_t s) {
     (void)nrk;
     const Lay L = P.L;
@@ -1446,11 +1485,11 @@ void slow_rhs_post(const Params& P, int nrk, double slow_dt, const erfx_state& r
     const erfx_config& c = P.cfg;
     ScalarBlock scal{ERFX_RHOSCALAR, 1, c.dryscal_horiz_adv, c.dryscal_vert_adv, c.dryscal_horiz_upw_frac,
                      c.dryscal_vert_upw_frac, 4 /* Scalar_h */, 10 /* Scalar_v */, false};
-    post_block(P, scal, true, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
+    post_block(P, W, scal, true, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
     if (P.moist) {
         ScalarBlock moist{ERFX_RHOQ1, P.n_qstate, c.moistscal_horiz_adv, c.moistscal_vert_adv,
                           c.moistscal_horiz_upw_frac, c.moistscal_vert_upw_frac, 5 /* Q_h */, 11 /* Q_v */, true};
-        post_block(P, moist, false, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
+        post_block(P, W, moist, false, slow_dt, rhs, old, snew, data, prim, scratch, eddy, s);
     }
     double* cur = data.cons;
     double* nc = snew.cons;

@@ -50,6 +50,40 @@ __device__ __forceinline__ double h_zeta_EdgeJ(const Zn& z, int i, int j, int k,
 __device__ __forceinline__ double h_zeta_EdgeI(const Zn& z, int i, int j, int k, double dzi) {     // :307-316
     return 0.25 * dzi * (z(i, j, k + 1) + z(i + 1, j, k + 1) - z(i, j, k - 1) - z(i + 1, j, k - 1));
 }
+// cell-centre, k-face and edge-centre slopes for the stresses and the
+// diffusive fluxes (:54-83, :192-219, :236-262, :279-303, :318-344)
+__device__ __forceinline__ double h_xi_Cell(const Zn& z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j, k) + z(i + 1, j + 1, k) + z(i + 1, j, k + 1) + z(i + 1, j + 1, k + 1)
+                         - z(i, j, k) - z(i, j + 1, k) - z(i, j, k + 1) - z(i, j + 1, k + 1));
+}
+__device__ __forceinline__ double h_eta_Cell(const Zn& z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i, j + 1, k) + z(i + 1, j + 1, k) + z(i, j + 1, k + 1) + z(i + 1, j + 1, k + 1)
+                         - z(i, j, k) - z(i + 1, j, k) - z(i, j, k + 1) - z(i + 1, j, k + 1));
+}
+__device__ __forceinline__ double h_xi_Kface(const Zn& z, int i, int j, int k, double dxi) {
+    return 0.5 * dxi * (z(i + 1, j, k) + z(i + 1, j + 1, k) - z(i, j, k) - z(i, j + 1, k));
+}
+__device__ __forceinline__ double h_eta_Kface(const Zn& z, int i, int j, int k, double dyi) {
+    return 0.5 * dyi * (z(i, j + 1, k) + z(i + 1, j + 1, k) - z(i, j, k) - z(i + 1, j, k));
+}
+__device__ __forceinline__ double h_xi_EdgeK(const Zn& z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j, k) + z(i + 1, j, k + 1) - z(i - 1, j, k) - z(i - 1, j, k + 1));
+}
+__device__ __forceinline__ double h_eta_EdgeK(const Zn& z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i, j + 1, k) + z(i, j + 1, k + 1) - z(i, j - 1, k) - z(i, j - 1, k + 1));
+}
+__device__ __forceinline__ double h_xi_EdgeJ(const Zn& z, int i, int j, int k, double dxi) {
+    return 0.25 * dxi * (z(i + 1, j + 1, k) + z(i + 1, j, k) - z(i - 1, j + 1, k) - z(i - 1, j, k));
+}
+__device__ __forceinline__ double h_eta_EdgeJ(const Zn& z, int i, int j, int k, double dyi) {
+    return dyi * (z(i, j + 1, k) - z(i, j, k));
+}
+__device__ __forceinline__ double h_xi_EdgeI(const Zn& z, int i, int j, int k, double dxi) {
+    return dxi * (z(i + 1, j, k) - z(i, j, k));
+}
+__device__ __forceinline__ double h_eta_EdgeI(const Zn& z, int i, int j, int k, double dyi) {
+    return 0.25 * dyi * (z(i + 1, j + 1, k) + z(i, j + 1, k) - z(i + 1, j - 1, k) - z(i, j - 1, k));
+}
 // slopes of the z-face (i, j, k) of OmegaFromW / WFromOmega (:368-381)
 __device__ __forceinline__ double zlo_xi(const Zn& z, int i, int j, int k, double dxi) {
     return 0.5 * dxi * (z(i + 1, j + 1, k) + z(i + 1, j, k) - z(i, j + 1, k) - z(i, j, k));

@@ -51,6 +51,14 @@ def _setup(kind):
     elif kind.startswith("hill"):   # terrain-following grid over a Gaussian hill
         adv = C.ADV_CENTERED2 if kind == "hill_c2" else C.ADV_UPWIND3
         cfg, f = P.hill_flow(24, 20, 16, seed=7, adv=adv, moist=(kind == "hill_moist"))
+        if kind == "hill_diff":       # ConstantAlpha molecular diffusion on the terrain grid
+            cfg.molec_diff_type = C.MOLEC_CONSTANT_ALPHA
+            cfg.rho0_trans, cfg.dynamic_viscosity, cfg.alpha_T, cfg.alpha_C = 1.0, 60.0, 45.0, 30.0
+        elif kind == "hill_noslip":   # Constant diffusion, no-slip walls, the top moving
+            cfg.bc[2] = cfg.bc[5] = C.BC_NO_SLIP_WALL
+            cfg.wall_velocity[5][0], cfg.wall_velocity[5][1] = 3.0, -1.5
+            cfg.molec_diff_type = C.MOLEC_CONSTANT
+            cfg.dynamic_viscosity, cfg.alpha_T, cfg.alpha_C = 40.0, 20.0, 10.0
         dt = 1.0
     else:  # walls + molecular diffusion
         cfg, f = P.rising_thermal(24, 8, 16, seed=11, adv=C.ADV_UPWIND3)
@@ -90,7 +98,7 @@ def _st(sim, s):
 
 
 TERRAIN_IDS = [C.F_Z_ND, C.D_DETJ, C.D_AX, C.D_AY, C.D_AZ, C.D_Z_CC]
-HILLS = ["hill", "hill_c2", "hill_moist"]
+HILLS = ["hill", "hill_c2", "hill_moist", "hill_diff", "hill_noslip"]
 KINDS = ["thermal", "cbl", "walls", "moist1", "moist2", "norain", "inflow", "inflow_moist", "inflow_diff",
          "noslip"] + HILLS
 MOIST_BUOYANCY = ["moist1", "moist2", "moist3", "moist4"]

@@ -137,6 +137,26 @@ def test_density_current_gold_case():
     _compare(cfg, fields, 1.0, 10, 1e-11, exact=True)
 
 
+def test_density_current_detJ2_gold_case():
+    """The DensityCurrent_detJ2 regression case (the same deck on a
+    terrain-following grid with detJ = 2, ConstantAlpha diffusion on the
+    terrain grid; its gold file the oracle reproduces,
+    tests/test_gold_density_current.py), 10 steps on the GPU from the oracle's
+    initial state: bit-identical to the oracle."""
+    import ctypes
+    from oracle_lib import lib as olib
+    from test_gold_density_current import density_current_detJ2_config, detJ2_nodes
+    cfg = density_current_detJ2_config()
+    orc0 = Oracle(cfg)
+    orc0.set(C.F_Z_ND, detJ2_nodes(cfg, orc0.L))
+    f = olib().orc_init_density_current
+    f.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5
+    orc0._chk(f(orc0.h, 0.0, 3000.0, 4000.0, 2000.0, -15.0))
+    fields = {fid: orc0.get(fid) for fid in (C.F_Z_ND, C.F_CONS, C.F_XVEL, C.F_YVEL, C.F_ZVEL, C.F_R0, C.F_P0,
+                                              C.F_PI0)}
+    _compare(cfg, fields, 1.0, 10, 1e-11, exact=True)
+
+
 def test_volume_integrals_exactly_rounded():
     """Mass, rho*theta and scalar sums (volWgtSumMF) equal the exactly rounded
     sum of the per-cell terms (math.fsum), before and after steps; mass is
@@ -234,6 +254,17 @@ def test_moist_terrain_case_exact():
     moist scalars, U3 momentum, Kessler, Rayleigh layer): bit-identical steps."""
     cfg, f = P.moist_terrain(24, 20, 16, seed=41)
     _compare(cfg, f, 1.0, 2, 1e-12, exact=True, rayleigh=P.rayleigh_profiles_uniform(cfg, f=f))
+
+
+def test_moist_terrain_case_hundred_steps():
+    """The headline recipe itself on a 24 x 20 column block: its 96 stretched
+    levels (20 m at the surface, 15 km top), the 500 m hill, WENO5 dry and
+    moist scalars, U3 momentum, Kessler, the Rayleigh layer above 12 km and
+    its time step, for 100 steps: within 1e-9 relative per field (the long-run
+    bar; in practice bit-identical)."""
+    cfg, f = P.moist_terrain(24, 20, 96, seed=43)
+    worst = _compare(cfg, f, P.MOIST_TERRAIN_DT, 100, 1e-9, rayleigh=P.rayleigh_profiles_uniform(cfg, f=f))
+    print("moist terrain, 100 steps, worst relative difference per field:", worst)
 
 
 def test_supercell_case_exact():
