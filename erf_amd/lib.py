@@ -65,6 +65,9 @@ SIGNATURES = {
     "erfx_make_fast_coeffs": (_I, [_P, _P, ctypes.POINTER(State), _P, _P, _D, _D]),
     "erfx_fast_rhs_N": (_I, [_P, _I, _I, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
                              _P, _P, _P, ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
+    "erfx_fast_rhs_N_noalloc": (_I, [ctypes.POINTER(C.Config), _P, _I, _I, ctypes.POINTER(State),
+                                     ctypes.POINTER(State), ctypes.POINTER(State), _P, _P, _P,
+                                     ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
     "erfx_fast_rhs_T": (_I, [_P, _I, _I, ctypes.POINTER(State), ctypes.POINTER(State), ctypes.POINTER(State),
                              _P, _P, _P, ctypes.POINTER(State), ctypes.POINTER(State), _D, _D, _D]),
     "erfx_turbulent_viscosity": (_I, [_P, _P, _P, _P, _P, _P, _P]),

@@ -281,6 +281,24 @@ int erfx_fast_rhs_N(erfx_ctx* ctx, int step, int nrk,
                     const double* fast_coeffs, erfx_state* data, erfx_state* scratch,
                     double dtau, double beta_s, double facinv);
 
+/* erf_fast_rhs_N with no device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under stream or graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as erfx_fast_rhs_N, with the configuration and the
+ * stream (a hipStream_t, NULL for the default stream) in place of a context:
+ * the substep's work fields live in a static device array of 64 MiB per GPU
+ * (held until the process ends) instead of a context's workspace, looked up
+ * once per device on the first call made with that device current.  Boxes
+ * whose 5 work fields of the layout exceed it (about 96 x 96 x 96 cells and
+ * up) are refused with ERFX_ERR_ARG before any device work.  Calls on one GPU
+ * share the array: two calls that overlap on one GPU race.  Flat grids only
+ * (a terrain grid returns ERFX_ERR_STATE). */
+int erfx_fast_rhs_N_noalloc(const erfx_config* cfg, void* stream, int step, int nrk,
+                            const erfx_state* slow_rhs, const erfx_state* prev,
+                            const erfx_state* stage, const double* prim, const double* pi_stage,
+                            const double* fast_coeffs, erfx_state* data, erfx_state* scratch,
+                            double dtau, double beta_s, double facinv);
+
 /* erf_fast_rhs_T (ERF_fast_rhs_T.cpp:39-704; ERF_TI_fast_headers.H:47-71):
  * one acoustic substep on a static terrain grid (use_terrain), with the
  * arguments of erfx_fast_rhs_N. */

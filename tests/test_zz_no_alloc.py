@@ -16,14 +16,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _noalloc():
-    from erf_amd.lib import lib, State
-    f = lib().erfx_fast_rhs_N_noalloc
-    f.restype = ctypes.c_int
-    st = ctypes.POINTER(State)
-    f.argtypes = [ctypes.POINTER(C.Config), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, st, st, st,
-                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, st, st,
-                  ctypes.c_double, ctypes.c_double, ctypes.c_double]
-    return f
+    from erf_amd.lib import lib
+    return lib().erfx_fast_rhs_N_noalloc      # argument types from erf_amd.lib.SIGNATURES
 
 
 def _pair(nx, ny, nz, seed):
@@ -73,9 +67,11 @@ def test_noalloc_fast_rhs_N_matches_context_hook():
         diff = {}
         for fid in ALL_IDS:
             x, y = a.download(fid), b.download(fid)
-            nd = int((x != y).sum())
+            # cells neither call writes keep the oracle's NaN fill on both sides
+            d = (x != y) & ~(np.isnan(x) & np.isnan(y))
+            nd = int(d.sum())
             if nd:
-                diff[fid] = (nd, float(np.abs(x - y).max()))
+                diff[fid] = (nd, float(np.abs(x - y)[d].max()))
         a.close()
         b.close()
         assert not diff, "not bit-identical: %s" % diff
