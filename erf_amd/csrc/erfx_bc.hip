@@ -550,14 +550,31 @@ void cons_to_prim(const Params& P, const double* c, double* prim, double* pi_sta
 
 namespace erfx {
 
+// Copy n doubles, 16 bytes per lane where both ends share their offset within
+// 16 bytes (a field allocated with its rows shifted onto cache lines starts 8
+// bytes into a 16-byte unit: its first double is then copied on its own), 8
+// bytes per lane otherwise.  `first`/`stride` are the lane's index and the
+// grid's width in lanes.
+__device__ __forceinline__ void copy_span(double* dst, const double* src, long n, long first, long stride) {
+    const unsigned long long as = reinterpret_cast<unsigned long long>(src);
+    const unsigned long long ad = reinterpret_cast<unsigned long long>(dst);
+    if (((as ^ ad) & 15ull) != 0) {
+        for (long t = first; t < n; t += stride) dst[t] = src[t];
+        return;
+    }
+    const long lead = (as & 15ull) ? 1 : 0;
+    if (lead && first == 0 && n > 0) dst[0] = src[0];
+    const long n2 = (n - lead) / 2;
+    const double2* s2 = reinterpret_cast<const double2*>(src + lead);
+    double2* d2 = reinterpret_cast<double2*>(dst + lead);
+    for (long t = first; t < n2; t += stride) d2[t] = s2[t];
+    const long tail = lead + 2 * n2;
+    if (tail < n && first == 0) dst[tail] = src[tail];
+}
+
 __global__ void __launch_bounds__(256) copy_kernel(CopyList c) {
     const long stride = (long)gridDim.x * 256;
-    for (int a = 0; a < c.count; ++a) {
-        const long n2 = c.n[a] / 2;
-        const double2* src = reinterpret_cast<const double2*>(c.src[a]);
-        double2* dst = reinterpret_cast<double2*>(c.dst[a]);
-        for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n2; t += stride) dst[t] = src[t];
-    }
+    for (int a = 0; a < c.count; ++a) copy_span(c.dst[a], c.src[a], c.n[a], (long)blockIdx.x * 256 + threadIdx.x, stride);
 }
 
 // batched form: each member's lists copied by its own range of work-groups
@@ -572,12 +589,7 @@ __global__ void __launch_bounds__(256) copy_kernel_batch(const CopyItem* __restr
     const CopyList& c = m.c;
     const long stride = (long)h.tiles * 256;
     const long first = (long)(blockIdx.x - h.t0) * 256 + threadIdx.x;
-    for (int a = 0; a < c.count; ++a) {
-        const long n2 = c.n[a] / 2;
-        const double2* src = reinterpret_cast<const double2*>(c.src[a]);
-        double2* dst = reinterpret_cast<double2*>(c.dst[a]);
-        for (long t = first; t < n2; t += stride) dst[t] = src[t];
-    }
+    for (int a = 0; a < c.count; ++a) copy_span(c.dst[a], c.src[a], c.n[a], first, stride);
 }
 static void copy_batch(const void* items, const unsigned* starts, int n, unsigned total, unsigned width,
                        hipStream_t s) {

@@ -92,12 +92,16 @@ __device__ __forceinline__ double w5(double sm2, double sm1, double s, double sp
 __device__ __forceinline__ double face(int t, double sp2, double sp1, double s, double sm1, double sm2, double sm3,
                                        double upw) {
     if (!(upw > tol) && !(upw < -tol)) return 0.5 * (s + sm1);
+    // The stencil seen from the upwind side, far upwind point first: the
+    // operands are selected per lane and the weights evaluated once, so lanes
+    // of one wave with opposite flow directions do not run both evaluations
+    // (the result is the same call on the same operands as the two-branch form).
     const bool pos = upw > tol;
-    if (t == ERFX_ADV_WENO3) return pos ? w3(sm2, sm1, s) : w3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENOZ3) return pos ? z3(sm2, sm1, s) : z3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENOMZQ3) return pos ? mzq3(sm2, sm1, s) : mzq3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENO5) return pos ? w5(sm3, sm2, sm1, s, sp1, false) : w5(sp2, sp1, s, sm1, sm2, false);
-    return pos ? w5(sm3, sm2, sm1, s, sp1, true) : w5(sp2, sp1, s, sm1, sm2, true);
+    const double a = pos ? sm3 : sp2, b = pos ? sm2 : sp1, c = pos ? sm1 : s, d = pos ? s : sm1, e = pos ? sp1 : sm2;
+    if (t == ERFX_ADV_WENO3) return w3(b, c, d);
+    if (t == ERFX_ADV_WENOZ3) return z3(b, c, d);
+    if (t == ERFX_ADV_WENOMZQ3) return mzq3(b, c, d);
+    return w5(a, b, c, d, e, t == ERFX_ADV_WENOZ5);
 }
 }  // namespace weno
 
@@ -223,12 +227,13 @@ __device__ __forceinline__ float w5(float sm2, float sm1, float s, float sp1, fl
 }
 __device__ __forceinline__ float face(int t, float sp2, float sp1, float s, float sm1, float sm2, float sm3, double upw) {
     if (!(upw > weno::tol) && !(upw < -weno::tol)) return 0.5f * (s + sm1);
+    // operands selected per lane, weights evaluated once (as in weno::face)
     const bool pos = upw > weno::tol;
-    if (t == ERFX_ADV_WENO3) return pos ? w3(sm2, sm1, s) : w3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENOZ3) return pos ? z3(sm2, sm1, s) : z3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENOMZQ3) return pos ? mzq3(sm2, sm1, s) : mzq3(sp1, s, sm1);
-    if (t == ERFX_ADV_WENO5) return pos ? w5(sm3, sm2, sm1, s, sp1, false) : w5(sp2, sp1, s, sm1, sm2, false);
-    return pos ? w5(sm3, sm2, sm1, s, sp1, true) : w5(sp2, sp1, s, sm1, sm2, true);
+    const float a = pos ? sm3 : sp2, b = pos ? sm2 : sp1, c = pos ? sm1 : s, d = pos ? s : sm1, e = pos ? sp1 : sm2;
+    if (t == ERFX_ADV_WENO3) return w3(b, c, d);
+    if (t == ERFX_ADV_WENOZ3) return z3(b, c, d);
+    if (t == ERFX_ADV_WENOMZQ3) return mzq3(b, c, d);
+    return w5(a, b, c, d, e, t == ERFX_ADV_WENOZ5);
 }
 }  // namespace wenof
 
