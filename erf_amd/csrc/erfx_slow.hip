@@ -272,6 +272,58 @@ __device__ __forceinline__ double interp_scalar_pick(bool mixed, int t, const do
                                                      double fr) {
     return mixed ? interp_mixed(t, f, q, st, upw, fr) : interp_scalar_at(t, f, q, st, upw, fr);
 }
+// The same face value from the six stencil values at offsets -3..+2 (sm3..sp2)
+// already at hand (staged in LDS or kept in registers): the arithmetic of
+// interp_scalar_pick, term for term.  Stencils narrower than six points do not
+// read the outer values.
+template <bool mixed>
+__device__ __forceinline__ double interp_scalar_vals(int t, double sp2, double sp1, double s, double sm1, double sm2,
+                                                     double sm3, double upw, double fr) {
+    if (!mixed) {
+        if (is_weno(t)) return weno::face(t, sp2, sp1, s, sm1, sm2, sm3, upw);
+        return interp_line(t, sp2, sp1, s, sm1, sm2, sm3, upw, fr);
+    }
+    const double c = 0.5 * (s + sm1);
+    if (t == ERFX_ADV_CENTERED2) return c;
+    const bool wide = is_weno(t) || wide_stencil(t);
+    const float fp1 = (float)(sp1 - c), f0 = (float)(s - c), fm1 = (float)(sm1 - c), fm2 = (float)(sm2 - c);
+    const float fp2 = wide ? (float)(sp2 - c) : 0.f, fm3 = wide ? (float)(sm3 - c) : 0.f;
+    const float v = is_weno(t) ? wenof::face(t, fp2, fp1, f0, fm1, fm2, fm3, upw)
+                               : interp_line_f(t, fp2, fp1, f0, fm1, fm2, fm3, upw, fr);
+    return c + (double)v;
+}
+// The same with the type fixed at compile time: TT < 0 takes t at run time;
+// TT = WENO5 or WENO-Z5 is straight-line code (the upwind operands selected
+// per lane, the weights always evaluated, the centred value selected where the
+// face velocity is within the tolerance of zero), so the x-, y- and z-faces of
+// a level can be evaluated interleaved.  The value is the same expression on
+// the same operands as the run-time form.
+template <bool mixed, int TT>
+__device__ __forceinline__ double interp_scalar_vals_t(int t, double sp2, double sp1, double s, double sm1,
+                                                       double sm2, double sm3, double upw, double fr) {
+    if constexpr (TT < 0) {
+        return interp_scalar_vals<mixed>(t, sp2, sp1, s, sm1, sm2, sm3, upw, fr);
+    } else {
+        static_assert(TT == ERFX_ADV_WENO5 || TT == ERFX_ADV_WENOZ5, "fixed types: WENO5, WENO-Z5");
+        const bool pos = upw > weno::tol;
+        const bool upwinded = upw > weno::tol || upw < -weno::tol;
+        if constexpr (!mixed) {
+            const double a = pos ? sm3 : sp2, b = pos ? sm2 : sp1, c = pos ? sm1 : s, d = pos ? s : sm1;
+            const double e = pos ? sp1 : sm2;
+            const double r = weno::w5(a, b, c, d, e, TT == ERFX_ADV_WENOZ5);
+            return upwinded ? r : 0.5 * (s + sm1);
+        } else {
+            const double c0 = 0.5 * (s + sm1);
+            const float fp1 = (float)(sp1 - c0), f0 = (float)(s - c0), fm1 = (float)(sm1 - c0);
+            const float fm2 = (float)(sm2 - c0), fp2 = (float)(sp2 - c0), fm3 = (float)(sm3 - c0);
+            const float a = pos ? fm3 : fp2, b = pos ? fm2 : fp1, c = pos ? fm1 : f0, d = pos ? f0 : fm1;
+            const float e = pos ? fp1 : fm2;
+            const float r = wenof::w5(a, b, c, d, e, TT == ERFX_ADV_WENOZ5);
+            const float v = upwinded ? r : 0.5f * (f0 + fm1);
+            return c0 + (double)v;
+        }
+    }
+}
 
 // UPWINDALL::InterpolateInZ: wall-reduced vertical interpolation for w.
 __device__ __forceinline__ double interp_z_all(const double* f, long q, long sz, double upw_lo, double upw_frac, int t) {
@@ -1211,87 +1263,206 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
 // kernel with one lane per cell evaluates every interior face twice (as the
 // hi face of one cell and the lo face of the next), and with WENO5 the face
 // evaluation is most of the work.  Here a work-group covers a tile of 64
-// cells in x, MR rows in y and up to MK levels in z: lane (lx, ly < MR) owns
-// cell (i0 + lx, j0 + ly) and marches up through the levels.  At each level
-// it evaluates its cell's lo x-face and lo y-face into LDS and its hi z-face
-// into registers (the lo z-face is the previous level's hi face); an extra
-// wave (ly = MR) evaluates the lo y-faces of row j0 + MR and, in MR of its
-// lanes, the x-face i0 + 64 of each row.  After one barrier each cell reads
-// its hi x- and y-faces from LDS.  LDS is double-buffered by level, so one
-// barrier per level suffices.  A face's value is computed by the same code
-// from the same inputs as in the per-cell kernel, so results are identical.
-constexpr int MARCH_MR = 4;    // rows of cells per work-group
+// cells in x, MR rows in y and up to mk levels in z: lane (lx, ly < MR) owns
+// cell (i0 + lx, j0 + ly) and marches up through the levels; an extra wave
+// (ly = MR) evaluates the lo y-faces of row j0 + MR and, in MR of its lanes,
+// the x-face i0 + 64 of each row.  Per level:
+//   1. the advected fields' values on the level, rows j0-3 .. j0+MR+2 and
+//      columns i0-3 .. i0+66 (the tile and the reach of a six-point stencil),
+//      are in LDS (`pl`): each x- and y-face takes its six stencil values from
+//      there, each row of the level is read from memory once;
+//   2. the z-stencils come from a ring of six levels per field in the owning
+//      lane's registers, one new level loaded per level;
+//   3. the lo x- and y-faces go to LDS (`xs`, `ys`), the hi z-face stays in
+//      registers (the lo z-face is the previous level's hi face); after a
+//      barrier each cell reads its hi x- and y-faces from LDS and is updated.
+// The next level's rows are loaded into registers before the cell updates
+// and stored to LDS after them, so their latency overlaps the updates.  A
+// face's value is computed by the same expression from the same operands as
+// in the per-cell kernel, so results are identical.
+// Rows of cells per work-group: 8 for the FP64 update, 4 for the mixed one
+// (measured on the 512 x 512 x 96 moist terrain case, one WENO5 scalar per
+// launch: FP64 1154 us at 8 rows against 1279 us at 4; mixed 855 us at 4
+// against 977 us at 8).
+template <bool mixed>
+constexpr int march_rows() { return mixed ? 4 : 8; }
 constexpr int MARCH_MK = 16;   // levels per work-group
+constexpr int MARCH_PW = 72;   // LDS row of the level tile: columns i0-3 .. i0+66, padded
 
-template <class Op>
-__global__ void __launch_bounds__(64 * (MARCH_MR + 1))
-march_kernel(const Op op, int nx, int ny, int nz, unsigned gx, unsigned gy) {
+template <class Op, int MR>
+__global__ void __launch_bounds__(64 * (MR + 1))
+march_kernel(const Op op, int nx, int ny, int nz, unsigned gx, unsigned gy, int mk) {
     constexpr int NF = Op::NF;
-    constexpr int MR = MARCH_MR;
-    __shared__ double xs[2][NF][MR][65];
-    __shared__ double ys[2][NF][MR + 1][64];
+    constexpr int PR = MR + 6;                        // rows of the level tile
+    constexpr int RPT = (PR + MR) / (MR + 1);         // tile rows loaded per lane (rows ly, ly + MR + 1, ...)
+    __shared__ double pl[NF][PR][MARCH_PW];
+    __shared__ double xs[NF][MR][65];
+    __shared__ double ys[NF][MR + 1][64];
+    const Lay L = op.L;
     const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
     const unsigned bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int lx = threadIdx.x, ly = threadIdx.y;
-    const int i = (int)bx * 64 + lx;
-    const int j0 = (int)by * MR;
-    const int k0 = (int)bz * MARCH_MK;
-    const int k1 = min(k0 + MARCH_MK, nz);
+    const int i0 = (int)bx * 64, j0 = (int)by * MR;
+    const int i = i0 + lx;
+    const int k0 = (int)bz * mk;
+    const int k1 = min(k0 + mk, nz);
     const bool row = ly < MR;
     const int j = j0 + ly;                    // cell row (row lanes) or y-face row j0 + MR
     const bool cellok = row && i < nx && j < ny;
-    double gzl[NF], gzh[NF];
-    if (cellok) op.face_z(i, j, k0, gzl);
-    int b = 0;
-    for (int k = k0; k < k1; ++k) {
-        double v[NF];
-        if (row) {
-            if (i <= nx && j < ny) op.face_x(i, j, k, v);
-            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
-            for (int m = 0; m < NF; ++m) xs[b][m][ly][lx] = v[m];
-            if (i < nx && j <= ny) op.face_y(i, j, k, v);
-            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
-            for (int m = 0; m < NF; ++m) ys[b][m][ly][lx] = v[m];
-            if (cellok) op.face_z(i, j, k + 1, gzh);
-        } else {
-            if (i < nx && j <= ny) op.face_y(i, j, k, v);
-            else for (int m = 0; m < NF; ++m) v[m] = 0.0;
-            for (int m = 0; m < NF; ++m) ys[b][m][MR][lx] = v[m];
-            if (lx < MR) {
-                const int ix = (int)bx * 64 + 64, jr = j0 + lx;
-                if (ix <= nx && jr < ny) op.face_x(ix, jr, k, v);
-                else for (int m = 0; m < NF; ++m) v[m] = 0.0;
-                for (int m = 0; m < NF; ++m) xs[b][m][lx][64] = v[m];
+    const int ng = L.ng;
+    // the level tile's loads of this lane: main column i0-3+lx of its rows and,
+    // in lanes 0..5, the halo column i0+61+lx; points outside the layout read 0
+    // (no face of the box uses them)
+    const int cm = i0 - 3 + lx, ch = i0 + 61 + lx;
+    const bool cm_ok = cm >= -ng && cm <= nx + ng, ch_ok = lx < 6 && ch <= nx + ng;
+    double rm[NF][RPT], rh[NF][RPT];
+    auto load_level = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int rr = ly + r * (MR + 1);
+            const int jj = j0 - 3 + rr;
+            const bool rok = rr < PR && jj >= -ng && jj <= ny + ng;
+#pragma unroll
+            for (int m = 0; m < NF; ++m) {
+                const double* ps = op.field(m);
+                rm[m][r] = (rok && cm_ok) ? ps[L.ix(cm, jj, k)] : 0.0;
+                rh[m][r] = (rok && ch_ok) ? ps[L.ix(ch, jj, k)] : 0.0;
             }
         }
-        __syncthreads();
+    };
+    auto store_level = [&]() {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int rr = ly + r * (MR + 1);
+            if (rr >= PR) break;
+#pragma unroll
+            for (int m = 0; m < NF; ++m) {
+                pl[m][rr][lx] = rm[m][r];
+                if (lx < 6) pl[m][rr][64 + lx] = rh[m][r];
+            }
+        }
+    };
+    // z ring of the owning lane: zr[m][0..5] = levels k-2 .. k+3 at level k
+    // (levels -3 .. nz+2 lie in the layout: at least 3 ghost layers)
+    double zr[NF][6], zn[NF], gzl[NF], gzh[NF];
+#pragma unroll
+    for (int m = 0; m < NF; ++m) {
+        zn[m] = 0.0; gzl[m] = 0.0; gzh[m] = 0.0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) zr[m][d] = 0.0;
+    }
+    const long qc = cellok ? L.ix(i, j, 0) : 0;
+    const long sz = L.sz;
+    if (cellok) {
+#pragma unroll
+        for (int m = 0; m < NF; ++m) {
+            const double* ps = op.field(m);
+            const double lo3 = ps[qc + (long)(k0 - 3) * sz];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) zr[m][d] = ps[qc + (long)(k0 - 2 + d) * sz];
+            // lo face of level k0: stencil levels k0-3 .. k0+2
+            gzl[m] = op.face_val(op.az, L.ix(i, j, k0), op.vt, op.vf, zr[m][4], zr[m][3], zr[m][2], zr[m][1],
+                                 zr[m][0], lo3);
+        }
+    }
+    load_level(k0);
+    store_level();
+    for (int k = k0; k < k1; ++k) {
+        __syncthreads();   // the level's tile is in LDS; last level's cell updates are done
+        if (row) {
+            // the lo x- and y-faces of this cell and the hi z-face, evaluated
+            // for every lane (a face outside the box gets a zero flux and is
+            // discarded) so that the three evaluations form one block of code
+            const int c = lx + 3, rr = ly + 3;   // this cell's place in the tile
+            const bool okx = i <= nx && j < ny, oky = i < nx && j <= ny;
+            const long q = L.ix(i, j, k);
+            const double fx = okx ? op.ax[q] : 0.0;
+            const double fy = oky ? op.ay[q] : 0.0;
+            const double fz = cellok ? op.az[q + L.sz] : 0.0;
+#pragma unroll
+            for (int m = 0; m < NF; ++m) {
+                const double vx = op.face_of(fx, op.ht, op.hf, pl[m][rr][c + 2], pl[m][rr][c + 1], pl[m][rr][c],
+                                             pl[m][rr][c - 1], pl[m][rr][c - 2], pl[m][rr][c - 3]);
+                const double vy = op.face_of(fy, op.ht, op.hf, pl[m][rr + 2][c], pl[m][rr + 1][c], pl[m][rr][c],
+                                             pl[m][rr - 1][c], pl[m][rr - 2][c], pl[m][rr - 3][c]);
+                const double vz = op.face_of(fz, op.vt, op.vf, zr[m][5], zr[m][4], zr[m][3], zr[m][2], zr[m][1],
+                                             zr[m][0]);
+                xs[m][ly][lx] = okx ? vx : 0.0;
+                ys[m][ly][lx] = oky ? vy : 0.0;
+                gzh[m] = vz;
+            }
+        } else {
+            // the y-faces of row j0 + MR and, in lanes 0 .. MR-1, x-face i0 + 64 of row j0 + lx
+            const int c = lx + 3, rr = MR + 3;
+            const bool oky = i < nx && j <= ny;
+            const int ix = i0 + 64, jr = j0 + (lx < MR ? lx : 0), r2 = (lx < MR ? lx : 0) + 3, c2 = 64 + 3;
+            const bool okx = lx < MR && ix <= nx && jr < ny;
+            const double fy = oky ? op.ay[L.ix(i, j, k)] : 0.0;
+            const double fx = okx ? op.ax[L.ix(ix, jr, k)] : 0.0;
+#pragma unroll
+            for (int m = 0; m < NF; ++m) {
+                const double vy = op.face_of(fy, op.ht, op.hf, pl[m][rr + 2][c], pl[m][rr + 1][c], pl[m][rr][c],
+                                             pl[m][rr - 1][c], pl[m][rr - 2][c], pl[m][rr - 3][c]);
+                const double vx = op.face_of(fx, op.ht, op.hf, pl[m][r2][c2 + 2], pl[m][r2][c2 + 1], pl[m][r2][c2],
+                                             pl[m][r2][c2 - 1], pl[m][r2][c2 - 2], pl[m][r2][c2 - 3]);
+                ys[m][MR][lx] = oky ? vy : 0.0;
+                if (lx < MR) xs[m][lx][64] = okx ? vx : 0.0;
+            }
+        }
+        __syncthreads();   // faces in LDS; every read of the level tile is done
+        const bool more = k + 1 < k1;
+        if (more) {
+            load_level(k + 1);
+            if (cellok) {
+#pragma unroll
+                for (int m = 0; m < NF; ++m) zn[m] = op.field(m)[qc + (long)(k + 4) * sz];
+            }
+        }
         if (cellok) {
             double gx0[NF], gx1[NF], gy0[NF], gy1[NF];
+#pragma unroll
             for (int m = 0; m < NF; ++m) {
-                gx0[m] = xs[b][m][ly][lx];
-                gx1[m] = xs[b][m][ly][lx + 1];
-                gy0[m] = ys[b][m][ly][lx];
-                gy1[m] = ys[b][m][ly + 1][lx];
+                gx0[m] = xs[m][ly][lx];
+                gx1[m] = xs[m][ly][lx + 1];
+                gy0[m] = ys[m][ly][lx];
+                gy1[m] = ys[m][ly + 1][lx];
             }
             op.cell(i, j, k, gx0, gx1, gy0, gy1, gzl, gzh);
+#pragma unroll
             for (int m = 0; m < NF; ++m) gzl[m] = gzh[m];
         }
-        b ^= 1;
+        if (more) {
+            // every read of this level's tile was before the barrier above, and
+            // the next level's faces overwrite xs / ys only after the barrier at
+            // the loop's top, which every lane reaches after its cell update
+            store_level();
+            if (cellok) {
+#pragma unroll
+                for (int m = 0; m < NF; ++m) {
+#pragma unroll
+                    for (int d = 0; d < 5; ++d) zr[m][d] = zr[m][d + 1];
+                    zr[m][5] = zn[m];
+                }
+            }
+        }
     }
 }
 
 template <class Op>
 inline void launch_march(const Op& op, int nx, int ny, int nz, hipStream_t s) {
-    const unsigned gx = (unsigned)((nx + 63) / 64), gy = (unsigned)((ny + MARCH_MR - 1) / MARCH_MR);
+    constexpr int MR = march_rows<Op::MIXED>();
+    const unsigned gx = (unsigned)((nx + 63) / 64), gy = (unsigned)((ny + MR - 1) / MR);
     const unsigned gz = (unsigned)((nz + MARCH_MK - 1) / MARCH_MK);
-    hipLaunchKernelGGL(march_kernel<Op>, dim3(gx * gy * gz), dim3(64, MARCH_MR + 1, 1), 0, s, op, nx, ny, nz, gx, gy);
+    hipLaunchKernelGGL((march_kernel<Op, MR>), dim3(gx * gy * gz), dim3(64, MR + 1, 1), 0, s, op, nx, ny, nz, gx, gy,
+                       MARCH_MK);
 }
 
 // The scalar update of erf_slow_rhs_post as a marching operator: NF scalars
 // of one block, face values as in post_block below.
-template <bool mixed, int NF_>
+template <bool mixed, int NF_, int TT = -1>
 struct PostOp {
     static constexpr int NF = NF_;
+    static constexpr bool MIXED = mixed;
     Lay L;
     int nx, ny;
     double dxi, dyi, dzi;
@@ -1311,16 +1482,19 @@ struct PostOp {
     double* crb;
     const double* prim;
     const double* dstate;     // terrain grid: the diffusion terms (DiffusionSrcForState_T), else null
-    __device__ __forceinline__ void face(const double* a, int t, double fr, long q, long st, double* g) const {
-        const double f = a[q];
-        for (int m = 0; m < NF; ++m) {
-            const double* ps = prim + (n0 + m - 1) * L.ne;
-            g[m] = c2c2 ? f * (0.5 * (ps[q] + ps[q - st])) : f * interp_scalar_pick(mixed, t, ps, q, st, f, fr);
-        }
+    // the advected field m: prim component n0 + m - 1
+    __device__ __forceinline__ const double* field(int m) const { return prim + (long)(n0 + m - 1) * L.ne; }
+    // flux through face q with mass flux a[q] and stencil values sm3..sp2
+    __device__ __forceinline__ double face_val(const double* a, long q, int t, double fr, double sp2, double sp1,
+                                               double s, double sm1, double sm2, double sm3) const {
+        return face_of(a[q], t, fr, sp2, sp1, s, sm1, sm2, sm3);
     }
-    __device__ __forceinline__ void face_x(int i, int j, int k, double* g) const { face(ax, ht, hf, L.ix(i, j, k), 1, g); }
-    __device__ __forceinline__ void face_y(int i, int j, int k, double* g) const { face(ay, ht, hf, L.ix(i, j, k), L.sy, g); }
-    __device__ __forceinline__ void face_z(int i, int j, int k, double* g) const { face(az, vt, vf, L.ix(i, j, k), L.sz, g); }
+    // the same with the mass flux f at hand
+    __device__ __forceinline__ double face_of(double f, int t, double fr, double sp2, double sp1, double s, double sm1,
+                                              double sm2, double sm3) const {
+        if constexpr (TT >= 0) return f * interp_scalar_vals_t<mixed, TT>(t, sp2, sp1, s, sm1, sm2, sm3, f, fr);
+        return c2c2 ? f * (0.5 * (s + sm1)) : f * interp_scalar_vals<mixed>(t, sp2, sp1, s, sm1, sm2, sm3, f, fr);
+    }
     __device__ __forceinline__ void cell(int i, int j, int k, const double* gx0, const double* gx1, const double* gy0,
                                          const double* gy1, const double* gz0, const double* gz1) const {
         const long q = L.ix(i, j, k);
@@ -1401,28 +1575,42 @@ static void post_block(const Params& P, const SlowWork& W, const ScalarBlock& B,
         diffusion_state_T(P, W, n0, nc_b, nc, prim, s);
         dst = W.dstate;
     }
-    if (!g_rec && (nc_b == 1 || nc_b == 3)) {
-        // one evaluation per face (the batched ensembles record the per-cell form below)
-        auto run = [&](auto mc, auto nf) {
+    if (!g_rec) {
+        // one evaluation per face (the batched ensembles record the per-cell
+        // form below), one launch per scalar of the block: the six-level z
+        // rings and the staged rows of several fields at once would exceed the
+        // registers of four waves per SIMD.  The first launch makes the copy of
+        // the slow components; the later ones read only prim, the face mass
+        // fluxes and the old state, which no launch of the block writes.
+        auto run = [&](auto mc, auto tc, int m) {
             constexpr bool mixed = decltype(mc)::value;
-            PostOp<mixed, decltype(nf)::value> op;
+            PostOp<mixed, 1, decltype(tc)::value> op;
             op.L = L; op.nx = nx; op.ny = ny;
             op.dxi = dxi; op.dyi = dyi; op.dzi = dzi; op.Jc = Jc;
             op.ht = ht; op.vt = vt; op.hf = hf; op.vf = vf;
             op.c2c2 = c2c2; op.use_diff = use_diff; op.consA = consA; op.turb = turb; op.clip = clip;
-            op.copy_slow = copy_slow; op.ae = ae; op.slow_dt = slow_dt;
+            op.copy_slow = copy_slow && m == 0; op.ae = ae; op.slow_dt = slow_dt;
             op.dxlo = dxlo; op.dxhi = dxhi; op.dylo = dylo; op.dyhi = dyhi;
-            op.mu_h = mu_h; op.mu_v = mu_v; op.n0 = n0; op.ncons = ncons;
+            op.mu_h = mu_h; op.mu_v = mu_v; op.n0 = n0 + m; op.ncons = ncons;
             op.ax = ax; op.ay = ay; op.az = az; op.cur = cur; op.nc = nc; op.oc = oc; op.crb = crb; op.prim = prim;
             op.dstate = dst;
             launch_march(op, nx, ny, nz, s);
         };
-        if (mixed_rt) {
-            if (nc_b == 1) run(std::true_type{}, std::integral_constant<int, 1>{});
-            else run(std::true_type{}, std::integral_constant<int, 3>{});
-        } else {
-            if (nc_b == 1) run(std::false_type{}, std::integral_constant<int, 1>{});
-            else run(std::false_type{}, std::integral_constant<int, 3>{});
+        // WENO5 / WENO-Z5 in all three directions: the straight-line form
+        const bool w5 = !c2c2 && ht == vt && (ht == ERFX_ADV_WENO5 || ht == ERFX_ADV_WENOZ5);
+        for (int m = 0; m < nc_b; ++m) {
+            using W5 = std::integral_constant<int, ERFX_ADV_WENO5>;
+            using Z5 = std::integral_constant<int, ERFX_ADV_WENOZ5>;
+            using RT = std::integral_constant<int, -1>;
+            if (mixed_rt) {
+                if (w5 && ht == ERFX_ADV_WENO5) run(std::true_type{}, W5{}, m);
+                else if (w5) run(std::true_type{}, Z5{}, m);
+                else run(std::true_type{}, RT{}, m);
+            } else {
+                if (w5 && ht == ERFX_ADV_WENO5) run(std::false_type{}, W5{}, m);
+                else if (w5) run(std::false_type{}, Z5{}, m);
+                else run(std::false_type{}, RT{}, m);
+            }
         }
         return;
     }

@@ -102,6 +102,18 @@ __device__ __forceinline__ double v_at_zface(const V& v, int i, int j, int k) {
     return (k == 0) ? 1.5 * (0.5 * (v(i, j, k) + v(i, j + 1, k))) - 0.5 * (0.5 * (v(i, j, k + 1) + v(i, j + 1, k + 1)))
                     : 0.25 * (v(i, j, k - 1) + v(i, j + 1, k - 1) + v(i, j, k) + v(i, j + 1, k));
 }
+// The two slope terms of OmegaFromW / WFromOmega on z-face (i, j, k):
+// px = zlo_xi * u and py = zlo_eta * v at the face, so that OmegaFromW is
+// (w - px) - py and WFromOmega (omega + px) + py, the rounding of both
+// functions below.
+template <class U, class V>
+__device__ __forceinline__ void zface_slope_terms(int i, int j, int k, const U& u, const V& v, const Zn& z, double dxi,
+                                                  double dyi, double& px, double& py) {
+    const double mx = zlo_xi(z, i, j, k, dxi);
+    const double my = zlo_eta(z, i, j, k, dyi);
+    px = mx * u_at_zface(u, i, j, k);
+    py = my * v_at_zface(v, i, j, k);
+}
 // OmegaFromW (:359-392) and WFromOmega (:397-445); u, v: anything read as u(i, j, k)
 template <class U, class V>
 __device__ __forceinline__ double omega_from_w(int i, int j, int k, double w, const U& u, const V& v,

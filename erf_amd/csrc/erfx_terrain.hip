@@ -265,7 +265,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
             t1 = ((xflux_hi * (prim[q] + prim[q + 1]) - xflux_lo * (prim[q] + prim[q - 1])) * dxi * mfsq +
                   (yflux_hi * (prim[q] + prim[q + sy]) - yflux_lo * (prim[q] + prim[q - sy])) * dyi * mfsq) * 0.5;
         };
-        double t0r[CM], t1r[CM], omr[CM + 1];
+        double t0r[CM], t1r[CM], omr[CM + 1], pxr[CM] = {}, pyr[CM] = {};
         // ---- phase 1: right-hand sides of the chunk's faces ----
         if (work) {
             const int kc = ka == 0 ? 0 : ka - 1;      // the cell below the chunk's first face
@@ -297,7 +297,13 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
                 const double om_kp1 = omega_old(k + 1);
                 double t0_k, t1_k;
                 trhs_at(k, t0_k, t1_k);
-                const double ofw = met::omega_from_w(i, j, k, 0., fnu, fnv, z, dxi, dyi);
+                // the slope terms of face k with the new increments: Omega of
+                // them here, rho*w from the solution in phase 3 (kept, so the
+                // increments and node heights are read once)
+                double px_k, py_k;
+                met::zface_slope_terms(i, j, k, fnu, fnv, z, dxi, dyi, px_k, py_k);
+                const double ofw = (0. - px_k) - py_k;
+                pxr[m] = px_k; pyr[m] = py_k;
                 double detJ_on_kface = 0.5 * (Jk + J_km1);
                 double coeff_P = fP_k;
                 double coeff_Q = fQ_k;
@@ -378,7 +384,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
                 const double szm_k = szm[q], azm_k = azm[q];
                 const double Jq = J[q], pq = prim[q], p_lo = prim[q - sz], p_hi = prim[q + sz];
                 const double sr0_k = sr0[q], sr1_k = sr1[q], cc0_k = cc0[q], cc1_k = cc1[q];
-                const double cz_k = (k >= 1) ? szm_k + met::w_from_omega(i, j, k, sol, fnu, fnv, z, dxi, dyi) : szm_k;
+                const double cz_k = (k >= 1) ? szm_k + ((sol + pxr[m]) + pyr[m]) : szm_k;
                 double zflux_lo = beta_2 * sol + beta_1 * om_lo;
                 double zflux_hi = beta_2 * sol_hi + beta_1 * om_hi;
                 double fast_rhs_rho = -(t0r[m] + (zflux_hi - zflux_lo) * dzi) / Jq;
