@@ -35,11 +35,12 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s per RK3 step, mixed vs FP64 GPU at stated L2 error; % HBM BW"
 
 
-def measured_traffic(kernel, workload, bytes_per_launch):
+def measured_traffic(kernel, workload, bytes_per_launch, variant=None):
     """HBM bytes per launch of the timed kernel from a committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement (profiles/<kernel>_<workload>_traffic.json),
-    if it was taken on this kernel and workload (same algorithmic byte count);
-    otherwise None."""
+    if it was taken on this kernel and workload (same algorithmic byte count)
+    and, where the file names one, on the same column-kernel variant as the
+    timed run; otherwise None."""
     path = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel.split(" ")[0], workload))
     try:
         with open(path) as f:
@@ -47,6 +48,8 @@ def measured_traffic(kernel, workload, bytes_per_launch):
     except (OSError, ValueError):
         return None
     if abs(t.get("algorithmic_bytes_per_launch", -1) - bytes_per_launch) > 0.5:
+        return None
+    if t.get("column_variant") is not None and variant is not None and t["column_variant"] != variant:
         return None
     return t["traffic_bytes_per_launch"]
 
@@ -367,7 +370,8 @@ def run_single(a):
         achieved = timer["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": timer["kernel"], "path": legs[0], "achieved": achieved,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": measured_traffic(timer["kernel"], a.workload, timer["bytes_per_launch"]),
+                           "traffic": measured_traffic(timer["kernel"], a.workload, timer["bytes_per_launch"],
+                                                       None if cfg.use_terrain else variant),
                            "algorithmic_bytes": timer["bytes_per_launch"], "avg_launch_ms": avg_ms,
                            "launches": timer["launches"]}
     if variant is not None and not cfg.use_terrain:
