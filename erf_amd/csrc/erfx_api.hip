@@ -388,6 +388,7 @@ int erfx_ctx_create(const erfx_config* cfg, void* stream, erfx_ctx** out) {
                          &c->tw.du, &c->tw.dv, &c->tw.nu, &c->tw.nv, &c->tw.om};
         for (double** b : tb)
             if ((rc = dalloc(c->allocs, b, ne, sh))) { erfx_ctx_destroy(c); return rc; }
+        if ((rc = dalloc(c->allocs, &c->tw.wtop, (size_t)lay.sz, sh))) { erfx_ctx_destroy(c); return rc; }
         if (c->P.use_diff) {
             double** db[] = {&c->sw.tau_t[0], &c->sw.tau_t[1], &c->sw.tau_t[2], &c->sw.er, &c->sw.dfx, &c->sw.dfy};
             for (double** b : db)
@@ -468,19 +469,6 @@ int erfx_set_terrain(erfx_ctx* c, const double* z_nd) {
         for (int j = 0; j < L.ny; ++j)
             for (int i = 0; i < L.nx; ++i) jmin = std::min(jmin, J[(size_t)L.ix(i, j, k)]);
     c->P.dz_min = c->P.cfg.dz * jmin;
-    // a plane top: every node of level nz (ghosts included) at one height
-    {
-        std::vector<double> zt((size_t)L.sz);
-        ERFX_HIP(hipMemcpyAsync(zt.data(), c->z_nd + (size_t)L.nz * L.sz + (size_t)L.ng * L.sz,
-                                (size_t)L.sz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        ERFX_HIP(hipStreamSynchronize(c->stream));
-        bool flat = true;
-        const double z0 = zt[(size_t)(L.ng + L.ng * L.sy)];
-        for (int j = -L.ng; j <= L.ny + L.ng; ++j)
-            for (int i = -L.ng; i <= L.nx + L.ng; ++i)
-                flat = flat && zt[(size_t)((i + L.ng) + (j + L.ng) * L.sy)] == z0;
-        c->P.flat_top = flat;
-    }
     return build_ray_points(c);
 }
 
@@ -1043,10 +1031,10 @@ static int sim_one_step(erfx_sim* s, double time, double dt, int ratio) {
     }
     const bool nosub = P.cfg.no_substepping != 0;
     const double sub_timestep = nosub ? 0.0 : dt / ratio;
-    // the substep chain folds apply_bcs into the substep's kernels: on a flat
-    // grid, and on a terrain grid whose top is a plane (there the wall w of the
-    // top face is the Dirichlet value, as the substep itself leaves it)
-    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !nosub && (!P.terrain || P.flat_top);
+    // the substep chain folds apply_bcs into the substep's kernels (on a
+    // terrain grid the column kernel forms the top face's wall value as the
+    // fill would, from the previous substep's momenta)
+    const bool chain = fast_bcs_fusable(P) && P.bck[2][5] == EXT_DIR && !nosub;
     const double beta_s = P.cfg.implicit_substepping ? 0.1 : -1.0;
     double sdt[3];
     stage_dts(time, dt, sdt);

@@ -593,9 +593,6 @@ struct Params {
     const double* mrx = nullptr;
     const double* mry = nullptr;
     double dz_min = 0.0;      // dz, times the smallest detJ of the valid cells on a terrain grid
-    // every node of the top level at one height (a terrain grid whose top is a
-    // plane): the wall value of w there is exactly the face's Dirichlet value
-    bool flat_top = false;
     // terrain grid with Rayleigh damping: factors and flags per point (the
     // heights vary along a level), RPT_* blocks of ne values
     const double* ray_pt = nullptr;

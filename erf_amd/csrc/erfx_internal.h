@@ -79,6 +79,7 @@ struct TerrainWork {
     double *du, *dv;          // horizontal momentum increments prev - stage
     double *nu, *nv;          // new horizontal momentum increments
     double *om;               // Omega of the old increments on z-faces
+    double *wtop;             // one plane (i, j) at L.ix(i, j, -ng): rho*w of the top face, substep chain
 };
 void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int step, int nrk,
                 const erfx_state& slow_rhs, const erfx_state& prev, const erfx_state& stage, const double* prim,

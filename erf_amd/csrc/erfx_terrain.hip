@@ -85,6 +85,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     double* lagged1 = scratch.cons + ne;
     double* du = T.du; double* dv = T.dv;
     double* nu = T.nu; double* nv = T.nv; double* om = T.om;
+    double* wtop = T.wtop;
     double* extrap = W.extrap;
     double* trhs0 = W.trhs;
     double* trhs1 = W.trhs + ne;
@@ -92,7 +93,7 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     const double* pxm = prev.xmom; const double* pym = prev.ymom; const double* pzm = prev.zmom;
     const double* sxm = stage.xmom; const double* sym = stage.ymom; const double* szm = stage.zmom;
     const bool first = step == 0;
-    // The substep chain (laterally periodic box, plane top): prev is S_sum as the
+    // The substep chain (laterally periodic box, Dirichlet top): prev is S_sum as the
     // previous substep left it, without the apply_bcs between the two.  What
     // that apply_bcs would give, the kernels form as they read: ghost cells of
     // rho*theta are the periodic images (foextrap rows above and below), and
@@ -121,6 +122,38 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         const double v = pym[L.ix(ci, fj, k)] * 2.0 / d;
         return v * 0.5 * d;
     };
+    // rho*w on the top face of column (i, j) as the skipped apply_bcs leaves it
+    // (Dirichlet top on a terrain grid, ERF_BoundaryConditions_zvel.cpp:220-257):
+    // w = WFromOmega of the top value with u and v as the fill makes them --
+    // prev's momenta over the face density of the images, the level above the
+    // top from the xvel / yvel top rules -- then rho*w with the top cell's
+    // density (the cell above it is a copy).  On a plane top the slopes vanish
+    // and w is the Dirichlet value.
+    const int xth = P.bck[0][5], yth = P.bck[1][5];
+    const double xvh = P.vel_extdir[0][5], yvh = P.vel_extdir[1][5], zvh = P.vel_extdir[2][5];
+    auto u_fill = [=] __device__(int ii, int jj, int kk) -> double {
+        if (kk > nz - 1) {
+            if (xth == EXT_DIR) return xvh;
+            kk = nz - 1;
+        }
+        const int fi = wrap_f(ii, nx), cj = wrap_c(jj, ny);
+        const double d = prho[L.ix(wrap_c(fi, nx), cj, kk)] + prho[L.ix(wrap_c(fi - 1, nx), cj, kk)];
+        return pxm[L.ix(fi, cj, kk)] * 2.0 / d;
+    };
+    auto v_fill = [=] __device__(int ii, int jj, int kk) -> double {
+        if (kk > nz - 1) {
+            if (yth == EXT_DIR) return yvh;
+            kk = nz - 1;
+        }
+        const int ci = wrap_c(ii, nx), fj = wrap_f(jj, ny);
+        const double d = prho[L.ix(ci, wrap_c(fj, ny), kk)] + prho[L.ix(ci, wrap_c(fj - 1, ny), kk)];
+        return pym[L.ix(ci, fj, kk)] * 2.0 / d;
+    };
+    auto pw_top = [=] __device__(int i, int j) -> double {
+        const double w = met::w_from_omega(i, j, nz, zvh, u_fill, v_fill, z, dxi, dyi);
+        const double r = prho[L.ix(i, j, nz - 1)];
+        return w * 0.5 * (r + r);
+    };
 
     // :139-204: rho, rho*theta copied from prev on the grown box (step 0);
     // theta extrapolated forward and the lagged increment; the horizontal
@@ -138,13 +171,16 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         if (k >= 0) du[q] = (rt ? rt_x(i, j, k) : pxm[q]) - sxm[q];
         if (k >= 0 && i <= nx) dv[q] = (rt ? rt_y(i, j, k) : pym[q]) - sym[q];
     });
-    // the top layer of cells (k = nz)
+    // the top layer of cells (k = nz); in the chain also rho*w of the top face
+    // of every column (read by the column kernel, after the horizontal
+    // momenta and the densities of prev are overwritten)
     launch_box(Box(-1, -1, nz, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
         const long q = L.ix(i, j, k);
         if (first) { cc0[q] = pc0[q]; cc1[q] = pc1[q]; }
         double dt_ = (rt ? cc1[cimg(i, j, k)] : cc1[q]) - sc1[q];
         extrap[q] = first ? dt_ : dt_ + beta_d * (dt_ - lagged1[q]);
         lagged1[q] = dt_;
+        if (rt && i >= 0 && i <= nx - 1 && j >= 0 && j <= ny - 1) wtop[L.ix(i, j, -L.ng)] = pw_top(i, j);
     });
 
 
@@ -238,7 +274,8 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         double* Z = lds + 2 * R;    // 1 / b_k, then Omega
         auto pw = [&](int kf) -> double {
             const long q = q0 + (long)kf * sz;
-            if (!rt || kf >= nz) return pzm[q];
+            if (!rt) return pzm[q];
+            if (kf >= nz) return wtop[L.ix(i, j, -L.ng)];
             const double d = prho[q] + prho[q - sz];
             const double w = pzm[q] * 2.0 / d;
             return w * 0.5 * d;
@@ -423,7 +460,8 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
         // in the substep chain); the top face keeps what the last substep wrote
         auto pw = [&](int kf) -> double {
             const long q = q0 + (long)kf * sz;
-            if (!rt || kf >= nz) return pzm[q];
+            if (!rt) return pzm[q];
+            if (kf >= nz) return wtop[L.ix(i, j, -L.ng)];
             const double d = prho[q] + prho[q - sz];
             const double w = pzm[q] * 2.0 / d;
             return w * 0.5 * d;
