@@ -15,9 +15,10 @@
 //   A (32 x 16): row l % 32, columns 8 (l / 32) .. 8 (l / 32) + 7
 //   B (16 x 32): column l % 32, rows 8 (l / 32) .. 8 (l / 32) + 7
 //   D (32 x 32): column l % 32, rows 8 (i / 4) + 4 (l / 32) + i % 4, i = 0..15
-// The hidden layer (128 x 128, 89% of the flops) and its two backward
-// products use the matrix cores; the 8-wide input and 4-wide output layers
-// run on the vector ALUs.
+// The forward pass runs all three layers on the matrix cores (the input and
+// output layers zero-padded to the instruction's K = 16 and N = 32); the
+// backward pass runs the hidden layer's two products there and the 8-wide and
+// 4-wide layers on the vector ALUs.
 #include "erfx_common.h"
 #include "erfx_internal.h"
 #include "erfx_mlp.h"
@@ -61,45 +62,74 @@ __device__ __forceinline__ void wave_gemm_32x128x128(const __bf16* a_lds, const 
 // row of accumulator element i of lane l in a 32 x 32 tile
 __device__ __forceinline__ int drow(int l, int i) { return 8 * (i >> 2) + 4 * (l >> 5) + (i & 3); }
 
-// Forward pass of up to 32 samples per wave (4 waves per block).  x: normalised
+// Forward pass of 32 samples per wave, WPB waves per block.  x: normalised
 // inputs [n][NI]; y: scaled outputs [n][NO].  When `train` the activations the
 // backward pass needs are stored: h1 (bf16), z1 and z2 pre-activations (fp32),
-// h2 (bf16).
-template <bool train>
-__global__ void __launch_bounds__(256) fwd_kernel(const float* __restrict__ x, int n, const Weights W,
-                                                  float* __restrict__ y, TrainBuf T) {
-    __shared__ __bf16 tile[4][32 * TP];
+// h2 (bf16).  All three layers run on the matrix cores: the input layer as one
+// 32x32x16 product per 32 units (the 8 inputs in the first half of K, zeros in
+// the second), the output layer as eight K-steps into one 32 x 32 tile of
+// which the first 4 columns are the outputs; the small weights and the biases
+// are staged in LDS once per block.  The GELUs stay on the vector ALUs (two
+// per unit and sample: they, not the products, bound the kernel).
+template <bool train, int WPB>
+__global__ void __launch_bounds__(64 * WPB) fwd_kernel(const float* __restrict__ x, int n, const Weights W,
+                                                        float* __restrict__ y, TrainBuf T) {
+    __shared__ __bf16 tile[WPB][32 * TP];
+    __shared__ __bf16 sw1[NI * NH];   // [k][u]
+    __shared__ __bf16 sw3[NH * NO];   // [u][o]
+    __shared__ float sb1[NH], sb2[NH], sb3[NO];
+    for (int e = threadIdx.x; e < NI * NH; e += 64 * WPB) sw1[e] = W.w1[e];
+    for (int e = threadIdx.x; e < NH * NO; e += 64 * WPB) sw3[e] = W.w3[e];
+    for (int e = threadIdx.x; e < NH; e += 64 * WPB) { sb1[e] = W.b1[e]; sb2[e] = W.b2[e]; }
+    if (threadIdx.x < NO) sb3[threadIdx.x] = W.b3[threadIdx.x];
+    __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int s0 = (blockIdx.x * 4 + wv) * 32;   // waves past n run on dead lanes (no early exit: block barriers)
+    const int s0 = (blockIdx.x * WPB + wv) * 32;   // waves past n run on dead lanes (no early exit: block barriers)
     __bf16* h = tile[wv];
     const int cell = l & 31, half = l >> 5;
     const int s = s0 + cell;
     const bool live = s < n;
-    // layer 1 on the vector ALUs: lane (cell, half) computes 64 of the 128 units
-    float xin[NI];
+    const __bf16 zero = f2bf(0.0f);
+    // ---- input layer: A = x (sample `cell`, inputs 0..7 in lanes 0..31), B = W1
+    bf16x8 xa;
 #pragma unroll
-    for (int k = 0; k < NI; ++k) xin[k] = live ? bf2f(f2bf(x[(long)s * NI + k])) : 0.0f;
-    for (int j = 0; j < 64; ++j) {
-        const int u = half * 64 + j;
-        float z = W.b1[u];
+    for (int q = 0; q < 8; ++q) xa[q] = (half == 0 && live) ? f2bf(x[(long)s * NI + q]) : zero;
+    floatx16 acc[4];
 #pragma unroll
-        for (int k = 0; k < NI; ++k) z += xin[k] * bf2f(W.w1[k * NH + u]);
-        const __bf16 hv = f2bf(gelu(z));
-        h[cell * TP + u] = hv;
-        if (train && live) {
-            T.z1[(long)s * NH + u] = z;
-            T.h1[(long)s * NH + u] = hv;
+    for (int t = 0; t < 4; ++t) {
+        bf16x8 wb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wb[q] = half == 0 ? sw1[q * NH + 32 * t + cell] : zero;
+        floatx16 z0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) z0[i] = 0.0f;
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa, wb, z0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int u = 32 * t + cell;
+        const float b = sb1[u];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = drow(l, i);
+            const float z = acc[t][i] + b;
+            const __bf16 hv = f2bf(gelu(z));
+            h[row * TP + u] = hv;
+            const int sr = s0 + row;
+            if (train && sr < n) {
+                T.z1[(long)sr * NH + u] = z;
+                T.h1[(long)sr * NH + u] = hv;
+            }
         }
     }
     __syncthreads();
-    // hidden layer on the matrix cores
-    floatx16 acc[4];
+    // ---- hidden layer
     wave_gemm_32x128x128(h, W.w2t, acc);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int u = 32 * t + cell;
-        const float b = W.b2[u];
+        const float b = sb2[u];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int row = drow(l, i);
@@ -114,16 +144,30 @@ __global__ void __launch_bounds__(256) fwd_kernel(const float* __restrict__ x, i
         }
     }
     __syncthreads();
-    // output layer on the vector ALUs: lane (cell, half) computes outputs 2 half, 2 half + 1
-    float o0 = W.b3[2 * half], o1 = W.b3[2 * half + 1];
-    for (int u = 0; u < NH; ++u) {
-        const float hv = bf2f(h[cell * TP + u]);
-        o0 += hv * bf2f(W.w3[u * NO + 2 * half]);
-        o1 += hv * bf2f(W.w3[u * NO + 2 * half + 1]);
+    // ---- output layer: column o = cell of the tile is output o for cell < NO
+    floatx16 ao;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ao[i] = 0.0f;
+    const int kh = half * 8;
+    const int oc = cell < NO ? cell : NO - 1;   // lanes of the padding columns read a valid entry, then use 0
+#pragma unroll
+    for (int c = 0; c < NH / 16; ++c) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(h + cell * TP + 16 * c + kh);
+        bf16x8 wb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const __bf16 w = sw3[(16 * c + kh + q) * NO + oc];
+            wb[q] = cell < NO ? w : zero;
+        }
+        ao = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wb, ao, 0, 0, 0);
     }
-    if (live) {
-        y[(long)s * NO + 2 * half] = o0;
-        y[(long)s * NO + 2 * half + 1] = o1;
+    if (cell < NO) {
+        const float b = sb3[cell];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int sr = s0 + drow(l, i);
+            if (sr < n) y[(long)sr * NO + cell] = ao[i] + b;
+        }
     }
 }
 
@@ -296,10 +340,21 @@ __global__ void loss_kernel(const float* __restrict__ loss, int n, float* __rest
 // ---- host side ------------------------------------------------------------------
 using namespace mlp;
 
+// Forward launches: 4 waves per block (the weights staged once for 128
+// samples) when that still gives every CU several blocks, else one wave per
+// block so that a small batch spreads over the 256 CUs.
+template <bool train>
+static void launch_fwd(const Net& N, const float* x, float* y, int n, const TrainBuf& T, hipStream_t s) {
+    if ((n + 127) / 128 >= 1024) {
+        hipLaunchKernelGGL((fwd_kernel<train, 4>), dim3((n + 127) / 128), dim3(256), 0, s, x, n, N.W, y, T);
+    } else {
+        hipLaunchKernelGGL((fwd_kernel<train, 1>), dim3((n + 31) / 32), dim3(64), 0, s, x, n, N.W, y, T);
+    }
+}
+
 void mlp_forward(const Net& N, const float* x, float* y, int n, hipStream_t s) {
     if (n <= 0) return;
-    const int blocks = (n + 127) / 128;
-    hipLaunchKernelGGL(fwd_kernel<false>, dim3(blocks), dim3(256), 0, s, x, n, N.W, y, TrainBuf{});
+    launch_fwd<false>(N, x, y, n, TrainBuf{}, s);
 }
 
 void mlp_refresh(const Net& N, hipStream_t s) {
@@ -312,7 +367,7 @@ void mlp_refresh(const Net& N, hipStream_t s) {
 void mlp_train_pass(Net& N, const float* x, const float* tgt, int n, bool update, float lr, float* grad_out,
                     hipStream_t s) {
     const int blocks = (n + 127) / 128;
-    hipLaunchKernelGGL(fwd_kernel<true>, dim3(blocks), dim3(256), 0, s, x, n, N.W, N.ybuf, N.T);
+    launch_fwd<true>(N, x, N.ybuf, n, N.T, s);
     const float scale = 1.0f / (float)(n * NO);
     hipLaunchKernelGGL(bwd_kernel, dim3(blocks), dim3(256), 0, s, (const float*)N.ybuf, tgt, n, N.W, N.T, scale);
     const int nblk = (n + WG_S - 1) / WG_S;

@@ -11,7 +11,9 @@
   cuts the held-out error (seed 3) by at least 5x against the untrained
   network, and a 10-step run with the trained network stays within stated
   bounds of plain Kessler (relative L2: rho 1e-4, rho theta 1e-3, qv 1e-2,
-  qc and qr 0.1); the numbers are printed.
+  qc and qr 0.1); the numbers are printed;
+* the cost per microphysics call of both, and the gain of the network over
+  Kessler, are printed (recorded, not required: there is none on MI355X).
 """
 import ctypes
 import time
@@ -138,16 +140,24 @@ def test_training_gain_and_emulated_run():
     # rho differs only through the dynamics' response to the heating
     assert errs["rho"] < 1e-4 and errs["rhotheta"] < 1e-3 and errs["qv"] < 1e-2
     assert errs["qc"] < 0.1 and errs["qr"] < 0.1
-    # microphysics cost per call on the last state, both ways (reported only)
+    # microphysics cost per call on the last state, both ways, and the gain of
+    # the network over Kessler it gives: recorded, not required (on this GPU
+    # plain Kessler is a memory-bound sweep and the network adds work to it,
+    # so the ratio is below 1)
     from erf_amd.lib import lib, check
+    cost = {}
     for name, call in (("kessler", lambda s: lib().erfx_kessler_advance(s.ctx, s.ptr(C.F_CONS), 1.0, None)),
                        ("network", lambda s: lib().erfx_kessler_mlp_advance(s.ctx, net.h, s.ptr(C.F_CONS),
                                                                             s.ptr(C.F_ZVEL), 1.0, None, None, None))):
         check(call(ka))
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(5):
+        for _ in range(20):
             check(call(ka))
-        print("%s microphysics on %dx%dx%d: %.3f ms per call" % (name, cfg.nx, cfg.ny, cfg.nz,
-                                                                   (time.perf_counter() - t0) / 5 * 1e3))
+        torch.cuda.synchronize()
+        cost[name] = (time.perf_counter() - t0) / 20 * 1e3
+        print("%s microphysics on %dx%dx%d: %.3f ms per call" % (name, cfg.nx, cfg.ny, cfg.nz, cost[name]))
+    print("gain of the network over Kessler (cost per call): %.2fx (%s)"
+          % (cost["kessler"] / cost["network"], "none" if cost["network"] >= cost["kessler"] else "faster"))
     ka.close()
     em.close()

@@ -102,3 +102,43 @@ def test_adam_training_reduces_loss():
     p = net.get_params()
     assert np.all(np.isfinite(p))
     net.close()
+
+
+def test_forward_throughput_against_matrix_core_peak():
+    """Forward pass rate, printed as useful FLOP/s (2 x (8x128 + 128x128 + 128x4)
+    per sample) and as a fraction of the 2.5 PFLOP/s dense BF16 peak, for a
+    large batch (4 waves per block) and a small one (one wave per block, so it
+    still spreads over the CUs).  The outputs are checked against the numpy
+    reference on a slice; the rate is only required to be positive: the two
+    GELUs per unit and sample run on the vector ALUs and bound the kernel, so
+    the matrix cores stay a few percent busy."""
+    torch = _torch()
+    from erf_amd.lib import MLP
+    p = _params(5)
+    flops = 2.0 * (M.NI * M.NH + M.NH * M.NH + M.NH * M.NO)
+    for n in (1 << 20, 8192):
+        rng = np.random.default_rng(n)
+        x = rng.normal(0, 1, (n, M.NI)).astype(np.float32)
+        net = MLP(n)
+        net.set_params(p)
+        xd = torch.from_numpy(x).cuda()
+        yd = torch.zeros((n, M.NO), dtype=torch.float32, device="cuda")
+        net.forward(xd.data_ptr(), yd.data_ptr(), n)
+        torch.cuda.synchronize()
+        reps = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            net.forward(xd.data_ptr(), yd.data_ptr(), n)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = flops * n / (ms * 1e-3) / 1e12
+        print("MLP forward n=%d: %.4f ms, %.1f TFLOP/s useful, %.2f%% of the 2.5 PFLOP/s BF16 peak"
+              % (n, ms, tf, 100.0 * tf / 2500.0))
+        m = 2000
+        y = yd[:m].cpu().numpy().astype(np.float64)
+        yr, _ = M.forward(p, x[:m].astype(np.float64), bf=True)
+        assert np.linalg.norm(y - yr) / np.linalg.norm(yr) <= 5e-3
+        assert tf > 0.0
+        net.close()
