@@ -1005,9 +1005,19 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
     // momentum face values stay FP64 in both forms: this kernel streams at
     // near the HBM peak, and FP32 face values only add registers to it.
     // Its stencils reach three levels up and down: tiles ordered k fastest.
-    launch_box_z(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) {
+    // The advection type is a compile-time constant when both directions use
+    // U3 or U5 (tc; -1: the types at run time), so the face stencils compile
+    // to straight-line code.
+    const int ht_rt = ht, vt_rt = vt;
+    auto body = [=] __device__(int i, int j, int k, auto tc) {
+        constexpr int TT = decltype(tc)::value;
+        const int ht = TT >= 0 ? TT : ht_rt;
+        const int vt = TT >= 0 ? TT : vt_rt;
         constexpr bool mixed = false;
-        const TAdv<mixed> TA{TD};
+        TAdvData td = TD;
+        td.ht = ht;
+        td.vt = vt;
+        const TAdv<mixed> TA{td};
         const long q = L.ix(i, j, k);
         // ---- x-momentum on x-faces ----
         if (j <= ny - 1 && k <= nz - 1) {
@@ -1255,7 +1265,13 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
             r += (zs - gpz - apg2) / (1.0 + qf);
             rwr[q] = r;
         }
-    });
+    };
+    auto go = [&](auto tc) {
+        launch_box_z(Box(0, 0, 0, nx, ny, nz), s, [=] __device__(int i, int j, int k) { body(i, j, k, tc); });
+    };
+    if (!c2c2 && ht == vt && ht == ERFX_ADV_UPWIND3) go(std::integral_constant<int, ERFX_ADV_UPWIND3>{});
+    else if (!c2c2 && ht == vt && ht == ERFX_ADV_UPWIND5) go(std::integral_constant<int, ERFX_ADV_UPWIND5>{});
+    else go(std::integral_constant<int, -1>{});
 }
 
 // ---- k-marching launch that evaluates every face once --------------------------
