@@ -195,39 +195,66 @@ void fast_rhs_T(const Params& P, const FastWork& W, const TerrainWork& T, int st
     double* dym = data.ymom;
     const double* mrx = P.mrx;
     const double* mry = P.mry;
-    launch_box(Box(0, 0, 0, nx, ny, nz - 1), s, [=] __device__(int i, int j, int k) {
-        const long q = L.ix(i, j, k);
-        if (j <= ny - 1) {
-            double gp_xi = (extrap[q] - extrap[q - 1]) * dxi;
-            double gp_zeta_on_iface = (k == 0)
-                ? 0.5 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1] - extrap[q])
-                : 0.25 * dzi * (extrap[q - 1 + sz] + extrap[q + sz] - extrap[q - 1 - sz] - extrap[q - sz]);
-            double gpx = gp_xi - mrx[q] * gp_zeta_on_iface;   // (h_xi / h_zeta) made once
-            gpx *= 1.0;
-            if (moist) gpx /= (1.0 + face_q(pq1, pq2, q, q - 1));
-            double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
-            double fast_rhs_rho_u = -Gamma * R_d * pi_c * gpx;
-            double n_ = du[q] + dtau * fast_rhs_rho_u + dtau * srx[q];
-            nu[q] = n_;
-            if (k == nz - 1) nu[q + sz] = n_;
-            axm[q] += facinv * n_;
-            dxm[q] = sxm[q] + n_;
+    // Each lane marches up a chunk of KC levels of its (i, j) and keeps extrap
+    // of its own column and of the columns at i-1 and j-1 for levels k-1, k
+    // and k+1 in registers, so every extrap value is read once per chunk
+    // instead of at three levels (the box order would put the levels of the
+    // gradient's stencil a whole plane of tiles apart).
+    constexpr int KC = 24;
+    const int nch = (nz + KC - 1) / KC;
+    launch_box(Box(0, 0, 0, nx, ny, nch - 1), s, [=] __device__(int i, int j, int c) {
+        const int k0 = c * KC, k1 = k0 + KC < nz ? k0 + KC : nz;
+        const bool xf = j <= ny - 1, yf = i <= nx - 1;
+        long q = L.ix(i, j, k0);
+        // c: column (i, j); x: (i-1, j); y: (i, j-1); m / 0 / p: levels k-1 / k / k+1
+        double ecm = 0.0, exm = 0.0, eym = 0.0;
+        if (k0 > 0) {
+            ecm = extrap[q - sz];
+            if (xf) exm = extrap[q - 1 - sz];
+            if (yf) eym = extrap[q - sy - sz];
         }
-        if (i <= nx - 1) {
-            double gp_eta = (extrap[q] - extrap[q - sy]) * dyi;
-            double gp_zeta_on_jface = (k == 0)
-                ? 0.5 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q] - extrap[q - sy])
-                : 0.25 * dzi * (extrap[q + sz] + extrap[q - sy + sz] - extrap[q - sz] - extrap[q - sy - sz]);
-            double gpy = gp_eta - mry[q] * gp_zeta_on_jface;   // (h_eta / h_zeta) made once
-            gpy *= 1.0;
-            if (moist) gpy /= (1.0 + face_q(pq1, pq2, q, q - sy));
-            double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
-            double fast_rhs_rho_v = -Gamma * R_d * pi_c * gpy;
-            double n_ = dv[q] + dtau * fast_rhs_rho_v + dtau * sry[q];
-            nv[q] = n_;
-            if (k == nz - 1) nv[q + sz] = n_;
-            aym[q] += facinv * n_;
-            dym[q] = sym[q] + n_;
+        double ec0 = extrap[q];
+        double ex0 = xf ? extrap[q - 1] : 0.0;
+        double ey0 = yf ? extrap[q - sy] : 0.0;
+        for (int k = k0; k < k1; ++k, q += sz) {
+            const double ecp = extrap[q + sz];
+            const double exp_ = xf ? extrap[q - 1 + sz] : 0.0;
+            const double eyp = yf ? extrap[q - sy + sz] : 0.0;
+            if (xf) {
+                double gp_xi = (ec0 - ex0) * dxi;
+                double gp_zeta_on_iface = (k == 0)
+                    ? 0.5 * dzi * (exp_ + ecp - ex0 - ec0)
+                    : 0.25 * dzi * (exp_ + ecp - exm - ecm);
+                double gpx = gp_xi - mrx[q] * gp_zeta_on_iface;   // (h_xi / h_zeta) made once
+                gpx *= 1.0;
+                if (moist) gpx /= (1.0 + face_q(pq1, pq2, q, q - 1));
+                double pi_c = 0.5 * (pi_stage[q - 1] + pi_stage[q]);
+                double fast_rhs_rho_u = -Gamma * R_d * pi_c * gpx;
+                double n_ = du[q] + dtau * fast_rhs_rho_u + dtau * srx[q];
+                nu[q] = n_;
+                if (k == nz - 1) nu[q + sz] = n_;
+                axm[q] += facinv * n_;
+                dxm[q] = sxm[q] + n_;
+            }
+            if (yf) {
+                double gp_eta = (ec0 - ey0) * dyi;
+                double gp_zeta_on_jface = (k == 0)
+                    ? 0.5 * dzi * (ecp + eyp - ec0 - ey0)
+                    : 0.25 * dzi * (ecp + eyp - ecm - eym);
+                double gpy = gp_eta - mry[q] * gp_zeta_on_jface;   // (h_eta / h_zeta) made once
+                gpy *= 1.0;
+                if (moist) gpy /= (1.0 + face_q(pq1, pq2, q, q - sy));
+                double pi_c = 0.5 * (pi_stage[q - sy] + pi_stage[q]);
+                double fast_rhs_rho_v = -Gamma * R_d * pi_c * gpy;
+                double n_ = dv[q] + dtau * fast_rhs_rho_v + dtau * sry[q];
+                nv[q] = n_;
+                if (k == nz - 1) nv[q + sz] = n_;
+                aym[q] += facinv * n_;
+                dym[q] = sym[q] + n_;
+            }
+            ecm = ec0; ec0 = ecp;
+            exm = ex0; ex0 = exp_;
+            eym = ey0; ey0 = eyp;
         }
     });
 
