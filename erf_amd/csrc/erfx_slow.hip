@@ -293,7 +293,8 @@ __device__ __forceinline__ double interp_scalar_vals(int t, double sp2, double s
     return c + (double)v;
 }
 // The same with the type fixed at compile time: TT < 0 takes t at run time;
-// TT = WENO5 or WENO-Z5 is straight-line code (the upwind operands selected
+// TT = U3 or U5 folds the type's branches; TT = WENO5 or WENO-Z5 is
+// straight-line code (the upwind operands selected
 // per lane, the weights always evaluated, the centred value selected where the
 // face velocity is within the tolerance of zero), so the x-, y- and z-faces of
 // a level can be evaluated interleaved.  The value is the same expression on
@@ -303,8 +304,11 @@ __device__ __forceinline__ double interp_scalar_vals_t(int t, double sp2, double
                                                        double sm2, double sm3, double upw, double fr) {
     if constexpr (TT < 0) {
         return interp_scalar_vals<mixed>(t, sp2, sp1, s, sm1, sm2, sm3, upw, fr);
+    } else if constexpr (TT == ERFX_ADV_UPWIND3 || TT == ERFX_ADV_UPWIND5) {
+        // linear stencils: the run-time form with the type a constant
+        return interp_scalar_vals<mixed>(TT, sp2, sp1, s, sm1, sm2, sm3, upw, fr);
     } else {
-        static_assert(TT == ERFX_ADV_WENO5 || TT == ERFX_ADV_WENOZ5, "fixed types: WENO5, WENO-Z5");
+        static_assert(TT == ERFX_ADV_WENO5 || TT == ERFX_ADV_WENOZ5, "fixed types: U3, U5, WENO5, WENO-Z5");
         const bool pos = upw > weno::tol;
         const bool upwinded = upw > weno::tol || upw < -weno::tol;
         if constexpr (!mixed) {
@@ -1613,21 +1617,22 @@ static void post_block(const Params& P, const SlowWork& W, const ScalarBlock& B,
             op.dstate = dst;
             launch_march(op, nx, ny, nz, s);
         };
-        // WENO5 / WENO-Z5 in all three directions: the straight-line form
-        const bool w5 = !c2c2 && ht == vt && (ht == ERFX_ADV_WENO5 || ht == ERFX_ADV_WENOZ5);
-        for (int m = 0; m < nc_b; ++m) {
-            using W5 = std::integral_constant<int, ERFX_ADV_WENO5>;
-            using Z5 = std::integral_constant<int, ERFX_ADV_WENOZ5>;
-            using RT = std::integral_constant<int, -1>;
-            if (mixed_rt) {
-                if (w5 && ht == ERFX_ADV_WENO5) run(std::true_type{}, W5{}, m);
-                else if (w5) run(std::true_type{}, Z5{}, m);
-                else run(std::true_type{}, RT{}, m);
-            } else {
-                if (w5 && ht == ERFX_ADV_WENO5) run(std::false_type{}, W5{}, m);
-                else if (w5) run(std::false_type{}, Z5{}, m);
-                else run(std::false_type{}, RT{}, m);
+        // one type in all three directions (WENO5, WENO-Z5, U3, U5): that type
+        // fixed at compile time
+        const int fixed = (!c2c2 && ht == vt && (ht == ERFX_ADV_WENO5 || ht == ERFX_ADV_WENOZ5 ||
+                                                 ht == ERFX_ADV_UPWIND3 || ht == ERFX_ADV_UPWIND5)) ? ht : -1;
+        auto typed = [&](auto mc, int m) {
+            switch (fixed) {
+                case ERFX_ADV_WENO5: run(mc, std::integral_constant<int, ERFX_ADV_WENO5>{}, m); break;
+                case ERFX_ADV_WENOZ5: run(mc, std::integral_constant<int, ERFX_ADV_WENOZ5>{}, m); break;
+                case ERFX_ADV_UPWIND3: run(mc, std::integral_constant<int, ERFX_ADV_UPWIND3>{}, m); break;
+                case ERFX_ADV_UPWIND5: run(mc, std::integral_constant<int, ERFX_ADV_UPWIND5>{}, m); break;
+                default: run(mc, std::integral_constant<int, -1>{}, m); break;
             }
+        };
+        for (int m = 0; m < nc_b; ++m) {
+            if (mixed_rt) typed(std::true_type{}, m);
+            else typed(std::false_type{}, m);
         }
         return;
     }
