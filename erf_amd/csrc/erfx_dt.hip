@@ -118,8 +118,11 @@ __device__ __forceinline__ void block_sum(DD* acc, double* out) {
 }
 
 // sums of (cons comp / mf^2) * cell volume over the valid cells for rho,
-// rho*theta and RhoScalar (the quantities of ERF::sum_integrated_quantities)
-__global__ void __launch_bounds__(256) sums_pass1(Lay L, const double* cons, double cell_vol, double* partial) {
+// rho*theta and RhoScalar (the quantities of ERF::sum_integrated_quantities);
+// on a terrain grid the volume is cell_vol * detJ, made first as the
+// reference's volume MultiFab is (ERF_WriteScalarProfiles.cpp:405-408)
+__global__ void __launch_bounds__(256) sums_pass1(Lay L, const double* cons, const double* J, double cell_vol,
+                                                  double* partial) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const long n = (long)nx * ny * nz;
     const int comps[3] = {ERFX_RHO, ERFX_RHOTHETA, ERFX_RHOSCALAR};
@@ -129,9 +132,10 @@ __global__ void __launch_bounds__(256) sums_pass1(Lay L, const double* cons, dou
         const int j = (int)((t / nx) % ny);
         const int k = (int)(t / ((long)nx * ny));
         const long q = L.ix(i, j, k);
+        const double vol = J ? cell_vol * J[q] : cell_vol;
         for (int c = 0; c < 3; ++c) {
             double phi = cons[q + comps[c] * L.ne] / (1.0 * 1.0);
-            acc[c] = dd_add(acc[c], DD{phi * cell_vol, 0.0});
+            acc[c] = dd_add(acc[c], DD{phi * vol, 0.0});
         }
     }
     block_sum<3>(acc, partial + 6 * blockIdx.x);
@@ -147,7 +151,8 @@ __global__ void __launch_bounds__(256) sums_pass2(const double* partial, int nbl
 void volume_sums(const Params& P, const double* cons, double* partial, double* host, hipStream_t s) {
     if (rec_refuse("volume sums")) return;
     const double cell_vol = P.cfg.dx * P.cfg.dy * P.cfg.dz;
-    hipLaunchKernelGGL(sums_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons, cell_vol, partial);
+    hipLaunchKernelGGL(sums_pass1, dim3(EST_BLOCKS), dim3(256), 0, s, P.L, cons,
+                       (const double*)(P.terrain ? P.detJ : nullptr), cell_vol, partial);
     hipLaunchKernelGGL(sums_pass2, dim3(1), dim3(256), 0, s, (const double*)partial, EST_BLOCKS,
                        partial + 6 * EST_BLOCKS);
     (void)hipMemcpyAsync(host, partial + 6 * EST_BLOCKS, 6 * sizeof(double), hipMemcpyDeviceToHost, s);
