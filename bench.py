@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--sc-every", type=int, default=2, help="supercell: rows harvested every n-th step")
     ap.add_argument("--sc-rows", type=int, default=4_000_000, help="supercell: row cap per training run")
     ap.add_argument("--sc-fit", type=int, default=3000, help="supercell: Adam steps")
+    ap.add_argument("--sc-per-step", type=int, default=0,
+                    help="supercell: rows drawn at random from each harvested step (0: all, up to --sc-rows)")
     ap.add_argument("--members", type=int, default=48, help="ensemble members (--workload ensemble)")
     ap.add_argument("--ens-streams", type=int, default=2, help="ensemble: member groups, one stream each")
     ap.add_argument("--ens-box", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
@@ -521,10 +523,12 @@ def run_supercell(a):
     dt = 2.0
     case = lambda seed: P.supercell(nx, ny, nz, seed=seed)
     w0 = time.perf_counter()
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    per = a.sc_per_step or None
     rows = E.harvest(seeds=tuple(range(a.sc_seeds)), nsteps=a.sc_steps, dt=dt, case=case, every=a.sc_every,
-                     max_rows=a.sc_rows)
+                     max_rows=a.sc_rows, per_step=per, log=log)
     held = E.harvest(seeds=(8,), nsteps=a.sc_steps, dt=dt, case=case, every=4 * a.sc_every,
-                     max_rows=min(a.sc_rows, 1_000_000))
+                     max_rows=min(a.sc_rows, 1_000_000), per_step=per, log=log)
     t_harvest = time.perf_counter() - w0
     print("rows: %d training, %d held out (%.0f s)" % (rows.shape[0], held.shape[0], t_harvest),
           file=sys.stderr, flush=True)
@@ -539,7 +543,9 @@ def run_supercell(a):
     w0 = time.perf_counter()
     net, sc, losses = E.fit(rows, steps=a.sc_fit, batch=8192, lr=2e-3, scaling=sc)
     t_fit = time.perf_counter() - w0
+    log("fit: %d Adam steps on %d rows (%.1f s)" % (a.sc_fit, rows.shape[0], t_fit))
     mse1 = E.mse(net, sc, held)
+    n_rows, n_held = rows.shape[0], held.shape[0]
     del rows, held
     cfg, f = case(8)
     cmx = C.Config.from_buffer_copy(cfg)
@@ -551,8 +557,10 @@ def run_supercell(a):
     em.set_mlp(net)
     res = {}
     for name, s in (("kessler", ka), ("mlp", em), ("mixed", mx)):
+        w0 = time.perf_counter()
         s.step(0.0, dt, a.sc_steps)
         torch.cuda.synchronize()
+        log("held-out run (seed 8) with %s: %d steps (%.0f s)" % (name, a.sc_steps, time.perf_counter() - w0))
         t1 = time.perf_counter()
         s.step(a.sc_steps * dt, dt, a.steps)
         res[name] = (time.perf_counter() - t1) / a.steps
@@ -600,6 +608,7 @@ def run_supercell(a):
         "mlp_cells": n_mlp, "fallback_cells": n_fb,
         "heldout_mse_scaled": {"untrained": mse0, "trained": mse1},
         "fit_s": t_fit, "adam_steps": a.sc_fit, "harvest_s": t_harvest,
+        "train_rows": n_rows, "heldout_rows": n_held, "rows_per_harvested_step": a.sc_per_step or None,
         "relative_l2_vs_kessler_after_run": errs,
         "relative_l2_mixed_precision_kessler_vs_fp64_kessler": floor,
         "qc_max_kessler": qc_max,

@@ -7,6 +7,8 @@ Feature scaling: inputs are centred and divided by their standard deviation;
 each output is divided by its standard deviation.  The raw training range of
 every input is kept: outside it the step falls back to Kessler.
 """
+import time
+
 import numpy as np
 
 from . import config as C, problems as P
@@ -22,32 +24,60 @@ def _torch():
     return torch
 
 
-def harvest(seeds=(0, 1), nsteps=20, shape=(32, 32, 24), dt=1.0, max_rows=2_000_000, case=None, every=1):
+def harvest(seeds=(0, 1), nsteps=20, shape=(32, 32, 24), dt=1.0, max_rows=2_000_000, case=None, every=1,
+            per_step=None, log=None):
     """Rows (n, 12) of (T, p, rho, qv, qc, qr, w, dt, dqv, dqc, dqr, dtheta) from
     runs with Kessler, one run per seed: case(seed) -> (cfg, fields), the moist
     bubble of `shape` by default; rows from every `every`-th step, at most
-    max_rows per run."""
+    max_rows per run.  With per_step, each harvested step contributes at most
+    that many of its rows, drawn uniformly at random on the GPU (seeded by the
+    run's seed), so a long run's rows cover its whole time span.  log(msg), if
+    given, is called after each run."""
     torch = _torch()
     out = []
     for seed in seeds:
+        t0 = time.perf_counter()
         cfg, f = case(seed) if case else P.moist_bubble(*shape, seed=seed)
         sim = Sim(cfg)
         sim.load(f)
         del f
         buf = torch.zeros((max_rows, 12), dtype=torch.float32, device="cuda")
+        if per_step is not None:
+            cells = cfg.nx * cfg.ny * cfg.nz
+            tmp = torch.zeros((cells, 12), dtype=torch.float32, device="cuda")
+            g = torch.Generator(device="cuda")
+            g.manual_seed(1000 + seed)
         torch.cuda.synchronize()
         off = 0
         for n in range(nsteps):
             take = (n + 1) % every == 0 and off < max_rows
             if take:
-                sim.set_sampling(buf.data_ptr() + off * 12 * 4, max_rows - off)
+                if per_step is None:
+                    sim.set_sampling(buf.data_ptr() + off * 12 * 4, max_rows - off)
+                else:
+                    sim.set_sampling(tmp.data_ptr(), cells)
             sim.step(n * dt, dt, 1)
             if take:
-                off += min(sim.sample_count(), max_rows - off)
+                cnt = sim.sample_count()
                 sim.set_sampling(None, 0)
+                if per_step is None:
+                    off += min(cnt, max_rows - off)
+                else:
+                    k = min(cnt, per_step, max_rows - off)
+                    if k < cnt:
+                        idx = torch.randperm(cnt, device="cuda", generator=g)[:k]
+                        buf[off:off + k] = tmp[idx]
+                    else:
+                        buf[off:off + k] = tmp[:k]
+                    off += k
+                    torch.cuda.synchronize()   # the gather reads tmp before the sim's stream writes it again
         out.append(buf[:off].cpu().numpy().astype(np.float64))
         sim.close()
         del buf
+        if per_step is not None:
+            del tmp
+        if log:
+            log("seed %d: %d rows from %d steps (%.0f s)" % (seed, off, nsteps, time.perf_counter() - t0))
     return np.concatenate(out, axis=0)
 
 
