@@ -368,8 +368,8 @@ inline void launch_cols(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f, 
 // doubles, one value per level and column.
 template <int CM, int NW>
 constexpr int tile_rows() { return NW * CM + 1; }
-template <int CM, int NW>
-constexpr int tile_lds_doubles() { return 3 * tile_rows<CM, NW>() * 64; }
+template <int CM, int NW, int TC = 64>
+constexpr int tile_lds_doubles() { return 3 * tile_rows<CM, NW>() * TC; }
 // levels per wave chunk for nz levels, and the smallest instantiated CM >= it
 // at 8 waves (0: nz above 96, no tile form: three arrays of 97 rows are
 // 149 KB of the CU's 160 KB of LDS)
@@ -385,12 +385,13 @@ inline int tile_cm8(int nz) {
 __device__ __forceinline__ void tile_level_fence() { asm volatile("" ::: "memory"); }
 
 // The tridiagonal sweeps of a tile over its LDS rows (row k of column `lane`
-// at k * 64 + lane).  Forward: X holds the right-hand sides of faces 0..nz, Y
+// at k * TC + lane, TC columns per tile).  Forward: X holds the right-hand sides of faces 0..nz, Y
 // the sub-diagonal a_k and Z the inverted pivots b_k; X becomes s_0 = X_0 b_0,
 // s_k = (X_k - a_k s_{k-1}) b_k.  Backward: Y holds c_k b_k; X becomes
 // x_nz = s_nz, x_k = s_k - (c_k b_k) x_{k+1}.  Both keep the operation order
 // of a per-column Thomas sweep, so the values are the same; the reads of the
 // next U levels are issued before their dependent arithmetic.
+template <int TC = 64>
 __device__ __forceinline__ void tile_forward(double* X, const double* Y, const double* Z, int nz, int lane) {
     constexpr int U = 8;
     double s = X[lane] * Z[lane];
@@ -400,40 +401,41 @@ __device__ __forceinline__ void tile_forward(double* X, const double* Y, const d
         double a[U], b[U], r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int o = (k + u) * 64 + lane;
+            const int o = (k + u) * TC + lane;
             a[u] = Y[o]; b[u] = Z[o]; r[u] = X[o];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             s = (r[u] - a[u] * s) * b[u];
-            X[(k + u) * 64 + lane] = s;
+            X[(k + u) * TC + lane] = s;
         }
     }
     for (; k <= nz; ++k) {
-        const int o = k * 64 + lane;
+        const int o = k * TC + lane;
         s = (X[o] - Y[o] * s) * Z[o];
         X[o] = s;
     }
 }
+template <int TC = 64>
 __device__ __forceinline__ void tile_backward(double* X, const double* Y, int nz, int lane) {
     constexpr int U = 8;
-    double s = X[nz * 64 + lane];
+    double s = X[nz * TC + lane];
     int k = nz - 1;
     for (; k - U + 1 >= 0; k -= U) {
         double cb[U], r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int o = (k - u) * 64 + lane;
+            const int o = (k - u) * TC + lane;
             cb[u] = Y[o]; r[u] = X[o];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             s = r[u] - cb[u] * s;
-            X[(k - u) * 64 + lane] = s;
+            X[(k - u) * TC + lane] = s;
         }
     }
     for (; k >= 0; --k) {
-        const int o = k * 64 + lane;
+        const int o = k * TC + lane;
         s = X[o] - Y[o] * s;
         X[o] = s;
     }
@@ -516,12 +518,47 @@ inline void launch_tiles(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f)
                        ncols, gx, f);
 }
 
+// Narrow tiles for boxes of few columns: a work-group of 8 waves takes TC
+// (16 or 32) consecutive columns of the flattened range and splits their
+// levels into 512 / TC chunks; thread t works on column t % TC in chunk
+// t / TC, so the functor sees `lane` = the column within the tile and `wave`
+// = the chunk (chunk 0, the first TC threads, runs the serial sweeps).  The
+// 4,096 columns of a 64 x 64 box make 256 tiles of 16 instead of 64 of 64.
+template <int CM, int TC, class F>
+__global__ void __launch_bounds__(512) tile_kernel_n(int ilo, int jlo, unsigned nxr, unsigned ncols, F f) {
+    __shared__ double lds[tile_lds_doubles<CM, 512 / TC, TC>()];
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = (int)threadIdx.x % TC, chunk = (int)threadIdx.x / TC;
+    const unsigned c = t * (unsigned)TC + (unsigned)lane;
+    const bool valid = c < ncols;
+    const int i = ilo + (int)(c % nxr), j = jlo + (int)(c / nxr);
+    [[clang::always_inline]] f(i, j, valid, lane, chunk, lds);
+}
+template <int CM, int TC, class F>
+inline void launch_tiles_n(int ilo, int ihi, int jlo, int jhi, hipStream_t s, F f) {
+    static_assert(TC == 16 || TC == 32, "narrow tile width");
+    if (ilo > ihi || jlo > jhi) return;
+    const unsigned nxr = (unsigned)(ihi - ilo + 1);
+    const unsigned ncols = nxr * (unsigned)(jhi - jlo + 1);
+    hipLaunchKernelGGL((tile_kernel_n<CM, TC, F>), dim3((ncols + TC - 1) / TC), dim3(512), 0, s, ilo, jlo, nxr, ncols,
+                       f);
+}
+// columns per tile for a box of ncols columns: 64 while that gives every CU a
+// tile, else 32 or 16
+inline int tile_columns(long ncols) {
+    constexpr long CUS = 256;
+    if ((ncols + 63) / 64 >= CUS) return 64;
+    if ((ncols + 31) / 32 >= CUS) return 32;
+    return 16;
+}
+
 // Variants of the acoustic column kernel (erf_fast_rhs_N): levels of loads
 // kept in flight per lane and work-group width.  Every variant computes the
 // same values; the sim's autotuner times them on the first launches of a run
 // and keeps the fastest.
-// Variant COL_TILE is the tile form (launch_tiles, boxes up to 128 levels;
-// COL_FALLBACK above that).
+// Variant COL_TILE is the tile form (launch_tiles; launch_tiles_n for boxes
+// of few columns, which may have up to 128 levels); COL_FALLBACK for boxes
+// the tile form does not take.
 constexpr int COL_NV = ERFX_COL_VARIANTS;
 constexpr int COL_DEPTH[COL_NV] = {2, 3, 4, 2, 3, 4, 2, 3, 4, 0};
 constexpr int COL_WIDTH[COL_NV] = {64, 64, 64, 128, 128, 128, 256, 256, 256, 0};

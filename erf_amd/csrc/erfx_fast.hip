@@ -440,15 +440,16 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     // the per-level loads of a column are spread over 8 waves, and the serial
     // sweeps read only LDS.
     const bool implicit_ = implicit;
-    auto tile = [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds, auto cmc) {
+    auto tile = [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds, auto cmc, auto tcc) {
         constexpr int CM = decltype(cmc)::value;
-        constexpr int NW = 8;
+        constexpr int TC = decltype(tcc)::value;   // columns per tile
+        constexpr int NW = 512 / TC;               // chunks of levels
         const int ch = tile_chunk(nz, NW);
         const int ka = wave * ch;
         const int kb = (ka + ch < nz ? ka + ch : nz) - 1;
         const bool work = valid && ka <= kb;
         const long q0 = valid ? L.ix(i, j, 0) : 0;
-        constexpr int R = tile_rows<CM, NW>() * 64;
+        constexpr int R = tile_rows<CM, NW>() * TC;
         double* X = lds;            // right-hand side, then the solution, per face
         double* Y = lds + R;        // a_k, then c_k / b_k
         double* Z = lds + 2 * R;    // 1 / b_k, then Omega
@@ -547,9 +548,9 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 R1_tmp += beta_1 * dzi * ((Om_kp1 - Om_km1) * halfg
                                           - (Om_kp1 * theta_t_hi - Om_k * theta_t_mid) * coeff_P
                                           - (Om_k * theta_t_mid - Om_km1 * theta_t_lo) * coeff_Q);
-                X[k * 64 + lane] = Om_k + dtau * (c.ws + R0_tmp + dtau * beta_2 * R1_tmp);
-                Y[k * 64 + lane] = a_k;
-                Z[k * 64 + lane] = b_k;
+                X[k * TC + lane] = Om_k + dtau * (c.ws + R0_tmp + dtau * beta_2 * R1_tmp);
+                Y[k * TC + lane] = a_k;
+                Z[k * TC + lane] = b_k;
                 t0r[m] = t0_k; t1r[m] = t1_k; omr[m] = Om_k;
                 tile_level_fence();
                 p_km2 = p_km1; p_km1 = p_k; p_k = c.pkp1;
@@ -564,35 +565,35 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 if (m == kb - ka + 1) omr[m] = Om_k;
             if (kb == nz - 1) {                // the top face's row (:423-432)
                 const long qt = q0 + (long)nz * sz;
-                X[nz * 64 + lane] = Om_k + dtau * slow_rhs.zmom[qt];
-                Y[nz * 64 + lane] = A[qt];
-                Z[nz * 64 + lane] = iB[qt];
+                X[nz * TC + lane] = Om_k + dtau * slow_rhs.zmom[qt];
+                Y[nz * TC + lane] = A[qt];
+                Z[nz * TC + lane] = iB[qt];
             }
         }
         __syncthreads();
         // ---- phase 2: elimination (wave 0), c_k / b_k and Omega staged, substitution (wave 0) ----
-        if (wave == 0 && valid && implicit_) tile_forward(X, Y, Z, nz, lane);
+        if (wave == 0 && valid && implicit_) tile_forward<TC>(X, Y, Z, nz, lane);
         __syncthreads();
         if (wave == 0 && valid) {
             const long qt = q0 + (long)nz * sz;
-            data.zmom[qt] = stage.zmom[qt] + X[nz * 64 + lane];
+            data.zmom[qt] = stage.zmom[qt] + X[nz * TC + lane];
         }
         if (work) {
 #pragma unroll
             for (int m = 0; m < CM; ++m) {
                 const int k = ka + m;
                 if (k > kb) break;
-                if (implicit_) Y[k * 64 + lane] = Cc[q0 + (long)k * sz] * Z[k * 64 + lane];
-                Z[k * 64 + lane] = omr[m];
+                if (implicit_) Y[k * TC + lane] = Cc[q0 + (long)k * sz] * Z[k * TC + lane];
+                Z[k * TC + lane] = omr[m];
             }
             if (kb == nz - 1) {
 #pragma unroll
                 for (int m = 0; m <= CM; ++m)
-                    if (m == kb - ka + 1) Z[nz * 64 + lane] = omr[m];
+                    if (m == kb - ka + 1) Z[nz * TC + lane] = omr[m];
             }
         }
         __syncthreads();
-        if (wave == 0 && valid && implicit_) tile_backward(X, Y, nz, lane);
+        if (wave == 0 && valid && implicit_) tile_backward<TC>(X, Y, nz, lane);
         __syncthreads();
         // ---- phase 3: the chunk's cells ----
         if (work) {
@@ -601,13 +602,13 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
                 const int k = ka + m;
                 if (k > kb) break;
                 const long q = q0 + (long)k * sz;
-                const double sol = X[k * 64 + lane], sol_next = X[(k + 1) * 64 + lane];
+                const double sol = X[k * TC + lane], sol_next = X[(k + 1) * TC + lane];
                 const double szz = stage.zmom[q], azm = scratch.zmom[q];
                 const double pcq = prim[q], p_next = prim[q + sz], pm = prim[q - sz];
                 const double s0 = sr0[q], s1 = sr1[q];
                 const double c0 = step == 0 ? prev.cons[q] : cc0[q];
                 const double c1 = step == 0 ? prev.cons[q + ne] : cc1[q];
-                double drw_k = Z[k * 64 + lane], drw_next = Z[(k + 1) * 64 + lane];
+                double drw_k = Z[k * TC + lane], drw_next = Z[(k + 1) * TC + lane];
                 data.zmom[q] = szz + sol;
                 double zflux_lo = beta_2 * sol + beta_1 * drw_k;
                 double zflux_hi = beta_2 * sol_next + beta_1 * drw_next;
@@ -652,19 +653,44 @@ void fast_rhs_N(const Params& P, const FastWork& W, int step, int nrk, const erf
     // the tile form needs nz <= 96; a recorded (batched) launch takes the
     // largest chunk, so every member of a batch records the same kernel
     const int cm = tile_cm8(nz) == 0 ? 0 : (g_rec ? 12 : tile_cm8(nz));
-    if (var == COL_TILE && cm == 0) var = COL_FALLBACK;
+    // 97..128 levels on a box of few columns: the tile form in 32- or
+    // 16-column tiles (three arrays of 129 rows x 32 columns are 99 KB of LDS;
+    // 64 columns would not fit).  A box of many columns keeps the per-lane
+    // form: on the 256 x 256 x 128 CBL the autotuner prefers it to 32-column
+    // tiles.
+    const bool deep = cm == 0 && !g_rec && nz <= 128 && tile_columns((long)nx * ny) < 64;
+    if (var == COL_TILE && cm == 0 && !deep) var = COL_FALLBACK;
     kt_start(P.timer, KT_FAST_COLUMN, s);
     if (var == COL_TILE) {
         auto launch = [&](auto cmc) {
             launch_tiles<decltype(cmc)::value, 8>(0, nx - 1, 0, ny - 1, s,
                 [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds) {
-                    tile(i, j, valid, lane, wave, lds, cmc);
+                    tile(i, j, valid, lane, wave, lds, cmc, std::integral_constant<int, 64>{});
                 });
         };
-        switch (cm) {
-            case 4: launch(std::integral_constant<int, 4>{}); break;
-            case 8: launch(std::integral_constant<int, 8>{}); break;
-            default: launch(std::integral_constant<int, 12>{}); break;
+        auto launch_n = [&](auto cmc, auto tcc) {
+            launch_tiles_n<decltype(cmc)::value, decltype(tcc)::value>(0, nx - 1, 0, ny - 1, s,
+                [=] __device__(int i, int j, bool valid, int lane, int wave, double* lds) {
+                    tile(i, j, valid, lane, wave, lds, cmc, tcc);
+                });
+        };
+        using IC4 = std::integral_constant<int, 4>;
+        using IC8 = std::integral_constant<int, 8>;
+        // a box of few columns or of more than 96 levels (a recorded launch
+        // excepted) takes narrow tiles
+        int tc = g_rec ? 64 : tile_columns((long)nx * ny);
+        if (deep && tc > 32) tc = 32;
+        if (tc == 16) {   // 32 chunks: at most 4 levels each up to 128 levels
+            launch_n(IC4{}, std::integral_constant<int, 16>{});
+        } else if (tc == 32) {   // 16 chunks: at most 8 levels each up to 128 levels
+            if (tile_chunk(nz, 16) <= 4) launch_n(IC4{}, std::integral_constant<int, 32>{});
+            else launch_n(IC8{}, std::integral_constant<int, 32>{});
+        } else {
+            switch (cm) {
+                case 4: launch(IC4{}); break;
+                case 8: launch(IC8{}); break;
+                default: launch(std::integral_constant<int, 12>{}); break;
+            }
         }
     } else {
         const int width = COL_WIDTH[var];

@@ -1469,13 +1469,29 @@ march_kernel(const Op op, int nx, int ny, int nz, unsigned gx, unsigned gy, int 
     }
 }
 
+template <class Op, int MR>
+inline void launch_march_rows(const Op& op, int nx, int ny, int nz, int mk, hipStream_t s) {
+    const unsigned gx = (unsigned)((nx + 63) / 64), gy = (unsigned)((ny + MR - 1) / MR);
+    const unsigned gz = (unsigned)((nz + mk - 1) / mk);
+    hipLaunchKernelGGL((march_kernel<Op, MR>), dim3(gx * gy * gz), dim3(64, MR + 1, 1), 0, s, op, nx, ny, nz, gx, gy,
+                       mk);
+}
+// A box whose default tiling gives fewer work-groups than the GPU has CUs
+// (the 64^3 rising thermal: 24) takes shorter stacks of levels and, if that
+// is not enough, 4-row tiles: a small box's march is bound by the latency of
+// its levels in sequence, not by the extra halo reads.  The tiling does not
+// change any value.
+constexpr int MARCH_MIN_GROUPS = 256;
 template <class Op>
 inline void launch_march(const Op& op, int nx, int ny, int nz, hipStream_t s) {
     constexpr int MR = march_rows<Op::MIXED>();
-    const unsigned gx = (unsigned)((nx + 63) / 64), gy = (unsigned)((ny + MR - 1) / MR);
-    const unsigned gz = (unsigned)((nz + MARCH_MK - 1) / MARCH_MK);
-    hipLaunchKernelGGL((march_kernel<Op, MR>), dim3(gx * gy * gz), dim3(64, MR + 1, 1), 0, s, op, nx, ny, nz, gx, gy,
-                       MARCH_MK);
+    auto groups = [&](int mr, int mk) {
+        return ((nx + 63) / 64) * ((ny + mr - 1) / mr) * ((nz + mk - 1) / mk);
+    };
+    int mk = MARCH_MK;
+    while (mk > 4 && groups(MR, mk) < MARCH_MIN_GROUPS) mk /= 2;
+    if (MR != 4 && groups(MR, mk) < MARCH_MIN_GROUPS) launch_march_rows<Op, 4>(op, nx, ny, nz, mk, s);
+    else launch_march_rows<Op, MR>(op, nx, ny, nz, mk, s);
 }
 
 // The scalar update of erf_slow_rhs_post as a marching operator: NF scalars

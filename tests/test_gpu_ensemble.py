@@ -72,6 +72,29 @@ def test_ensemble_matches_members_alone(kind, graph):
         s.close()
 
 
+# A batched member's scalar update is the per-cell form; a sim stepped alone
+# uses the k-marching form, whose tiling depends on the box: at these sizes
+# (at least 256 work-groups) its default tiles of 12 rows x 16 levels, while
+# the small members above take the tiling for small boxes.
+@pytest.mark.parametrize("kind", ["cbl_u5", "moist_u3_weno3", "hill_weno5"])
+def test_ensemble_matches_member_alone_default_march_tiling(kind):
+    from erf_amd.lib import Ensemble
+    if kind == "cbl_u5":
+        case, dt = P.cbl(256, 256, 64, seed=2), 0.5
+    elif kind == "moist_u3_weno3":   # dry scalars U3, moist scalars WENO3
+        case, dt = P.moist_bubble(256, 256, 64, seed=3), 1.0
+    else:
+        case, dt = P.hill_flow(256, 256, 64, seed=4, adv=C.ADV_UPWIND3, scal_adv=C.ADV_WENO5), 1.0
+    a, b = _pairs([case])
+    ens = Ensemble(a)
+    ens.step(0.0, dt, 2)
+    b[0].step(0.0, dt, 2)
+    assert _diff(a, b) == [], kind
+    ens.close()
+    for s in a + b:
+        s.close()
+
+
 @pytest.mark.parametrize("streams", [1, 3])
 def test_ensemble_stream_groups(streams):
     """members split over 1 or 3 groups (streams): the same results"""

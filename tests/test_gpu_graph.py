@@ -92,15 +92,25 @@ def test_autotune_picks_a_variant_and_keeps_results():
     b.close()
 
 
-@pytest.mark.parametrize("case", ["flat37", "flat_moist_50", "hill21", "hill_moist_30"])
+@pytest.mark.parametrize("case", ["flat37", "flat_moist_50", "hill21", "hill_moist_30", "flat_deep128",
+                                  "flat_tc32_40", "flat_tc32_80", "flat_tc32_deep100"])
 def test_tile_form_matches_column_form(case):
     """The tile form of the acoustic column kernel (the default) against the
     one-lane-per-column form on level counts that leave the last wave's chunk
     partial and trailing waves empty (37 levels: chunks of 3 over 16 waves;
     21 levels on terrain: chunks of 3 over 8 waves), flat and terrain, dry
-    and moist: bit-identical after several steps."""
+    and moist: bit-identical after several steps.  Boxes of fewer than
+    16,384 columns take 16-column tiles (32 chunks of levels) and those of
+    8,192 or more 32-column tiles (16 chunks), also with 97..128 levels: 40 x
+    24 x 128 (16 columns, chunks of 4), 96 x 96 x 40 and x 80 (32 columns,
+    chunks of 3 and 5), 96 x 96 x 100 (32 columns, chunks of 7)."""
     if case == "flat37":
         cfg, f = P.rising_thermal(40, 24, 37, seed=21, adv=C.ADV_UPWIND3)
+    elif case == "flat_deep128":
+        cfg, f = P.rising_thermal(40, 24, 128, seed=25, adv=C.ADV_UPWIND3)
+    elif case.startswith("flat_tc32"):
+        nz = {"flat_tc32_40": 40, "flat_tc32_80": 80, "flat_tc32_deep100": 100}[case]
+        cfg, f = P.rising_thermal(96, 96, nz, seed=26, adv=C.ADV_UPWIND3)
     elif case == "flat_moist_50":
         cfg, f = P.moist_bubble(24, 20, 50, seed=22)
     elif case == "hill21":
