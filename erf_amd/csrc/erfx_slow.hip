@@ -1300,14 +1300,14 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
 // and stored to LDS after them, so their latency overlaps the updates.  A
 // face's value is computed by the same expression from the same operands as
 // in the per-cell kernel, so results are identical.
-// Rows of cells per work-group: 15 for the FP64 update (16 waves, the
-// largest work-group), 4 for the mixed one (measured on the 512 x 512 x 96
-// moist terrain case, one WENO5 scalar per launch: FP64 874 us at 15 rows,
-// 1015 us at 12, 1154 us at 8, 1279 us at 4; mixed 855 us at 4 against 977 us
-// at 8).  The FP64 form's registers (106 VGPRs) allow one work-group per CU,
+// Rows of cells per work-group: 15 in both forms (16 waves, the largest
+// work-group; measured on the 512 x 512 x 96 moist terrain case, one WENO5
+// scalar per launch: FP64 874 us at 15 rows, 1015 us at 12, 1154 us at 8,
+// 1279 us at 4; mixed 717 us at 15 rows, 957 us at 9, 854 us at 4).  The
+// registers (106 VGPRs FP64, 90 mixed) allow one 16-wave work-group per CU,
 // so taller tiles mean more waves per CU as well as less y-halo.
 template <bool mixed>
-constexpr int march_rows() { return mixed ? 4 : 15; }
+constexpr int march_rows() { return 15; }
 constexpr int MARCH_MK = 16;   // levels per work-group
 constexpr int MARCH_PW = 72;   // LDS row of the level tile: columns i0-3 .. i0+66, padded
 

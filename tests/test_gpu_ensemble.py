@@ -76,10 +76,16 @@ def test_ensemble_matches_members_alone(kind, graph):
 # uses the k-marching form, whose tiling depends on the box: at these sizes
 # (at least 256 work-groups) its default tiles of 15 rows x 16 levels, while
 # the small members above take the tiling for small boxes.
-@pytest.mark.parametrize("kind", ["cbl_u5", "moist_u3_weno3", "hill_weno5"])
+@pytest.mark.parametrize("kind", ["cbl_u5", "moist_u3_weno3", "hill_weno5", "cbl_u5_mixed", "moist_mixed"])
 def test_ensemble_matches_member_alone_default_march_tiling(kind):
+    """also for the mixed-precision form (FP32 face values), whose march and
+    per-cell forms evaluate each face by the same expressions"""
     from erf_amd.lib import Ensemble
-    if kind == "cbl_u5":
+    if kind.endswith("_mixed"):
+        cfg, f = P.cbl(256, 256, 64, seed=2) if kind.startswith("cbl") else P.moist_bubble(256, 256, 64, seed=3)
+        cfg.precision = C.PREC_MIXED
+        case, dt = (cfg, f), 0.5 if kind.startswith("cbl") else 1.0
+    elif kind == "cbl_u5":
         case, dt = P.cbl(256, 256, 64, seed=2), 0.5
     elif kind == "moist_u3_weno3":   # dry scalars U3, moist scalars WENO3
         case, dt = P.moist_bubble(256, 256, 64, seed=3), 1.0
