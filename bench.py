@@ -619,7 +619,7 @@ def run_supercell(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: erfx has no multi-GPU path "
               "(the x-y domain decomposition is a design note, not code)", file=sys.stderr)
         sys.exit(2)
     if a.calibrate_copy:

@@ -3,7 +3,7 @@
 // weights and activations and FP32 accumulation, trained on the GPU by the
 // kernels below (forward, backward, Adam).  It has no counterpart in ERF; the
 // plain Kessler scheme (Source/Microphysics/Kessler/ERF_Kessler.cpp:11-250)
-// produces its training data and is its fallback and the judge of its error.
+// produces its training data and is its fallback and the reference for its error.
 //
 // Inputs (T, p [mbar], rho, qv, qc, qr, w, dt), normalised per feature;
 // outputs the local process tendencies of one step (dqv, dqc, dqr, dtheta,
