@@ -1308,7 +1308,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
 // so taller tiles mean more waves per CU as well as less y-halo.
 template <bool mixed>
 constexpr int march_rows() { return 15; }
-constexpr int MARCH_MK = 16;   // levels per work-group (one WENO5 scalar at 15 rows: 859 / 879 / 925 / 971 us at 12 / 16 / 24 / 32)
+constexpr int MARCH_MK = 12;   // levels per work-group (one WENO5 scalar at 15 rows: 859 / 879 / 925 / 971 us at 12 / 16 / 24 / 32)
 constexpr int MARCH_PW = 72;   // LDS row of the level tile: columns i0-3 .. i0+66, padded
 
 template <class Op, int MR>
@@ -1490,7 +1490,7 @@ inline void launch_march(const Op& op, int nx, int ny, int nz, hipStream_t s) {
         return ((nx + 63) / 64) * ((ny + mr - 1) / mr) * ((nz + mk - 1) / mk);
     };
     int mk = MARCH_MK;
-    while (mk > 4 && groups(MR, mk) < MARCH_MIN_GROUPS) mk /= 2;
+    while (mk > 4 && groups(MR, mk) < MARCH_MIN_GROUPS) mk = mk / 2 > 4 ? mk / 2 : 4;
     if (MR != 4 && groups(MR, mk) < MARCH_MIN_GROUPS) launch_march_rows<Op, 4>(op, nx, ny, nz, mk, s);
     else launch_march_rows<Op, MR>(op, nx, ny, nz, mk, s);
 }

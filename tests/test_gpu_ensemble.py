@@ -74,7 +74,7 @@ def test_ensemble_matches_members_alone(kind, graph):
 
 # A batched member's scalar update is the per-cell form; a sim stepped alone
 # uses the k-marching form, whose tiling depends on the box: at these sizes
-# (at least 256 work-groups) its default tiles of 15 rows x 16 levels, while
+# (at least 256 work-groups) its default tiles of 15 rows x 12 levels, while
 # the small members above take the tiling for small boxes.
 @pytest.mark.parametrize("kind", ["cbl_u5", "moist_u3_weno3", "hill_weno5", "cbl_u5_mixed", "moist_mixed"])
 def test_ensemble_matches_member_alone_default_march_tiling(kind):
