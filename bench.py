@@ -420,7 +420,7 @@ def run_ensemble(a):
     warm = a.warmup
     if not a.no_autotune:
         ens.autotune(True)           # one warm-up step per column variant
-        warm = max(warm, 9)
+        warm = max(warm, C.COL_VARIANTS)
     ens.step(0.0, dt, warm)
     t = warm * dt
     variant, var_ms = ens.column_variant()
@@ -435,7 +435,8 @@ def run_ensemble(a):
     un = None
     if a.unbatched_steps > 0:
         for s in sims:
-            s.set_column_variant(variant)
+            if variant >= 0:          # the tuned variant (with --no-autotune: each sim's default)
+                s.set_column_variant(variant)
         torch.cuda.synchronize()
         w0 = time.perf_counter()
         for n in range(a.unbatched_steps):
