@@ -1308,7 +1308,7 @@ void slow_rhs_pre(const Params& P, const SlowWork& W, int nrk, double slow_dt, c
 // so taller tiles mean more waves per CU as well as less y-halo.
 template <bool mixed>
 constexpr int march_rows() { return 15; }
-constexpr int MARCH_MK = 12;   // levels per work-group (one WENO5 scalar at 15 rows: 859 / 879 / 925 / 971 us at 12 / 16 / 24 / 32)
+constexpr int MARCH_MK = 12;   // levels per work-group (one WENO5 scalar at 15 rows: 882 / 879 / 859 / 879 / 925 / 971 us at 8 / 10 / 12 / 16 / 24 / 32)
 constexpr int MARCH_PW = 72;   // LDS row of the level tile: columns i0-3 .. i0+66, padded
 
 template <class Op, int MR>
